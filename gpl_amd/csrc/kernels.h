@@ -1,0 +1,125 @@
+// Device kernels of the Nesterov engine (kernels.hip) and their launchers, called by the host
+// controller (nesterov.cpp).  Layout in HBM, per region r of the problem:
+//   cells   struct-of-arrays, each region's cells contiguous [cell_begin, cell_end):
+//           dbox int4 (dlx, dly, dux, duy: the density box), density_scale f32, flags u8,
+//           the coordinate slots float2 (cur_slp, next_slp, prev_slp, cur, next, snapshots),
+//           the gradient slots float2 (wl, den, sum per slp slot), pin CSR (start int32,
+//           per cell-pin: net int32, offset int2), preconditioner base (pins, area) float2.
+//   pins    in net order (net CSR): cell int32, offset int2, initial position int2.
+//   nets    sorted by degree (the host keeps the permutation): start int32, weight f32,
+//           box int4, sums 2 x float4 (min_x, x_min_x, max_x, x_max_x | the same in y).
+//   bins    y-major (k = y*cnt_x + x): inst/filler int64 accumulators, non_place (+unscaled)
+//           int64, area f32*target (scaled), density f32, field_x/field_y f32, scratch for the
+//           transforms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gplk {
+
+struct RegionDev {
+    int32_t lx, ly, ux, uy;
+    int32_t cnt_x, cnt_y;
+    double size_x, size_y;
+    float target_density;
+    int32_t cell_begin, cell_end;
+    // bins
+    const int32_t* bin_x;  // cnt_x + 1
+    const int32_t* bin_y;  // cnt_y + 1
+    unsigned long long* inst;    // int64 accumulators (two's complement in u64)
+    unsigned long long* filler;
+    const long long* non_place;
+    const long long* non_place_unscaled;
+    float* density;
+    float* field_x;
+    float* field_y;
+    float* phi;
+    float* work0;  // transform scratch, cnt_x*cnt_y each
+    float* work1;
+    float* work2;
+    float* work3;
+    const float* wx;
+    const float* wy;
+    // transform tables: for a length L, twiddles e^{-2 pi i k / L} (k < L/2) and the DCT
+    // post/pre-twiddles e^{-i pi k / 2L} (k < L), as float2
+    const float2* tw_x;
+    const float2* dw_x;
+    const float2* tw_y;
+    const float2* dw_y;
+};
+
+struct CellsDev {
+    int4* dbox;
+    const float* dscale;
+    const uint8_t* flags;       // GPL_CELL_* (locked is updated on convergence)
+    uint8_t* centered;          // the density box has been centred once (pins follow it)
+    const int32_t* pin_start;   // num_cells + 1
+    const int32_t* cp_net;      // per cell-pin: the net (sorted net index)
+    const int2* cp_off;         // per cell-pin: the offset
+    const float2* pre;          // (pin count, dx*dy) per cell, as float
+};
+
+struct NetsDev {
+    int32_t num_nets;
+    const int32_t* start;   // num_nets + 1, into the net-order pin arrays
+    const int32_t* pcell;   // per net-pin: cell or -1
+    const int2* poff;       // per net-pin: offset
+    const int2* pxy0;       // per net-pin: position as init left it
+    const float* weight;
+    int4* box;
+    float4* sx;             // min_x, x_min_x, max_x, x_max_x
+    float4* sy;             // min_y, y_min_y, max_y, y_max_y
+    int32_t num_small;      // nets [0, num_small) take one thread each; the rest a block each
+};
+
+// Per-trial FP64 reductions of one region, finished into this struct by k_finish.
+struct Sums {
+    double overflow, overflow_unscaled;   // sum of the per-bin overflow terms
+    double wl_abs, den_abs;               // sum |wl| and |den| over x and y
+    double coord_sq, grad_sq;             // squared distances (step length)
+    unsigned long long hpwl;              // pin HPWL (integer, exact)
+    int32_t pad[2];
+};
+
+constexpr int kPartials = 2048;  // per-block partial slots per quantity
+
+// ---- launchers (stream-ordered, asynchronous) -----------------------------------------------
+void launch_zero_bins(hipStream_t s, const RegionDev& r);
+// init_density1's layout-inside + centring: cur_slp = prev_slp = cur = density centre; scatter.
+void launch_center_init(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* cur_slp, float2* prev_slp, float2* cur);
+// centre the region's cells at `xy` and scatter them into the bins.
+void launch_center_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy);
+// nesterovUpdateCoordinates + centring + scatter.
+void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
+                           const float2* cur, float2* next, float2* next_slp, float step, float coeff);
+// per bin: density, overflow partials, accumulators cleared.
+void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials);
+// the Poisson solve of r.density into r.field_x / r.field_y (and r.phi when want_phi).
+void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi);
+// WA forward over every net; HPWL partials.
+void launch_wa_forward(hipStream_t s, const NetsDev& n, const CellsDev& c, float coef_x, float coef_y, unsigned long long* hpwl);
+// per cell of a region: WA gradient, density gradient, preconditioned sum; partial sums of |wl|,
+// |den| and (when cmp_slp != nullptr) the squared distances to cmp_slp/cmp_sum.
+void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, float coef_x, float coef_y,
+                     float penalty, float2* wl, float2* den, float2* sum, const float2* a_slp, const float2* b_slp,
+                     const float2* a_sum, double* partials, int nan_inject);
+// finish: sums of the block partials into *out (fixed order).
+void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int nparts_cells, Sums* out, const unsigned long long* hpwl);
+// squared distance between two coordinate sets (prev/cur at init).
+void launch_distance(hipStream_t s, int n, const float2* a, const float2* b, const float2* c, const float2* d, double* partials);
+// initial prev_slp: cur_slp - c * cur_sum, kept inside.
+void launch_initial_prev(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
+                         float coef, float2* prev_slp);
+// lock the instance cells of a converged region; rotate locked cells' slots.
+void launch_lock_fix(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* const* next_vec, const float2* const* cur_vec, int nvec);
+void launch_hpwl(hipStream_t s, const NetsDev& n, const CellsDev& c, unsigned long long* hpwl);
+// the objective (exact exponential, continuous coordinates): WA value and gradient per cell.
+void launch_objective(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, const float2* xy, float coef,
+                      double* net_val, float2* grad_wl);
+
+int block_partials_bins(const RegionDev& r);
+int block_partials_cells(const RegionDev& r);
+
+}  // namespace gplk

@@ -1,0 +1,897 @@
+// HIP kernels of the ePlace Nesterov loop for CDNA4 (gfx950), 64-wide wavefronts.
+//
+// Arithmetic follows the reference function it replaces (cited per kernel) with the same float
+// types and evaluation order; the build passes -ffp-contract=off so no multiply-add is fused,
+// as the reference's is not.  Global sums (overflow, |gradient| sums, the step-length norms)
+// reduce in FP64 in a fixed tree (per-block partials, then one block in index order), so every
+// run gives the same bits.  Integer work (bin areas, HPWL) is exact: int64 atomics are
+// associative.
+#include "kernels.h"
+
+#include <climits>
+
+namespace gplk {
+
+constexpr int kBlock = 256;
+constexpr int kMaxBlocks = kPartials;
+
+// ---------------------------------------------------------------------------------------------
+// Shared device helpers
+// ---------------------------------------------------------------------------------------------
+
+// nesterov.rs:308 fastExp: (1 + x/1024)^1024, ten squarings.
+__device__ __forceinline__ float fast_exp(float x) {
+    float e = 1.0f + x / 1024.0f;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) e *= e;
+    return e;
+}
+
+// nesterov.rs:275 density_idx_x / 280 density_idx_y: f64 quotient, truncated / ceiled, clamped.
+__device__ __forceinline__ void bin_span(int lo, int hi, int org, double size, int cnt, int& a, int& b) {
+    int l = (int)((double)(lo - org) / size);
+    int u = (int)ceil((double)(hi - org) / size);
+    a = l > 0 ? l : 0;
+    b = u < cnt ? u : cnt;
+}
+
+// nesterov.rs:298 getOverlapDensityArea.
+__device__ __forceinline__ float overlap_area(int blx, int bly, int bux, int buy, int4 d) {
+    int lx = max(blx, d.x), ly = max(bly, d.y), ux = min(bux, d.z), uy = min(buy, d.w);
+    if (lx >= ux || ly >= uy) return 0.0f;
+    return (float)(ux - lx) * (float)(uy - ly);
+}
+
+// Rust's f32::max(a, b) for b not NaN: NaN a gives b.
+__device__ __forceinline__ float rmax(float a, float b) { return a >= b ? a : b; }
+
+// nesterov.rs:2188 getDensityCoordiLayoutInsideX for a box of half width `half`.
+__device__ __forceinline__ float inside(float c, float half, float lo, float hi) {
+    float adj = c;
+    if (c - half < lo) adj = lo + half;
+    if (c + half > hi) adj = hi - half;
+    return adj;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Block sum of a double (blockDim.x == kBlock), in a fixed tree; thread 0 gets the result.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlock / 64; ++i) r += sh[i];
+    return r;
+}
+
+// A cell's density box centred at the truncated point (nesterov.rs:2139 setDensityCenterLocation):
+// the halves of the current size, so an odd size loses one.
+__device__ __forceinline__ int4 centre_box(int4 d, float x, float y) {
+    int cx = (int)x, cy = (int)y;
+    int hx = (d.z - d.x) / 2, hy = (d.w - d.y) / 2;
+    return make_int4(cx - hx, cy - hy, cx + hx, cy + hy);
+}
+
+// nesterov.rs:2225-2250: one cell's overlaps, truncated to int64, added into the bins.
+__device__ __forceinline__ void scatter_cell(const RegionDev& r, int4 d, float scale, uint8_t flags) {
+    int x0, x1, y0, y1;
+    bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
+    bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
+    const bool inst = flags & 1u, macro = inst && (flags & 2u);
+    unsigned long long* acc = inst ? r.inst : r.filler;
+    for (int y = y0; y < y1; ++y) {
+        int by0 = r.bin_y[y], by1 = r.bin_y[y + 1];
+        for (int x = x0; x < x1; ++x) {
+            float a = overlap_area(r.bin_x[x], by0, r.bin_x[x + 1], by1, d) * scale;
+            if (macro) a *= r.target_density;
+            long long v = (long long)a;
+            if (v != 0) atomicAdd(acc + (size_t)y * r.cnt_x + x, (unsigned long long)v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Scatter family
+// ---------------------------------------------------------------------------------------------
+
+__global__ void k_zero_bins(unsigned long long* a, unsigned long long* b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        a[i] = 0ull;
+        b[i] = 0ull;
+    }
+}
+
+// initDensity1 (nesterov.rs:2546-2570) per cell: the density box moved inside the grid
+// (updateDensityCoordiLayoutInside, 2173), its centre as cur_slp / prev_slp / cur, the box
+// centred there, and the scatter.
+__global__ void k_center_init(RegionDev r, CellsDev c, float2* cur_slp, float2* prev_slp, float2* cur) {
+    int n = r.cell_end - r.cell_begin;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int g = r.cell_begin + k;
+        int4 d = c.dbox[g];
+        int ddx = d.z - d.x, ddy = d.w - d.y;
+        float tlx = rmax((float)d.x, (float)r.lx), tly = rmax((float)d.y, (float)r.ly);
+        if (tlx + (float)ddx > (float)r.ux) tlx = (float)(r.ux - ddx);
+        if (tly + (float)ddy > (float)r.uy) tly = (float)(r.uy - ddy);
+        int lx = (int)tlx, ly = (int)tly;
+        d = make_int4(lx, ly, lx + ddx, ly + ddy);
+        float2 p = make_float2((float)((d.z + d.x) / 2), (float)((d.w + d.y) / 2));
+        cur_slp[k] = p;
+        prev_slp[k] = p;
+        cur[k] = p;
+        d = centre_box(d, p.x, p.y);
+        c.dbox[g] = d;
+        c.centered[g] = 1;
+        scatter_cell(r, d, c.dscale[g], c.flags[g]);
+    }
+}
+
+// updateGCellDensityCenterLocation (nesterov.rs:2268) + the scatter, at coordinates xy.
+__global__ void k_center_scatter(RegionDev r, CellsDev c, const float2* xy) {
+    int n = r.cell_end - r.cell_begin;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int g = r.cell_begin + k;
+        float2 p = xy[k];
+        int4 d = centre_box(c.dbox[g], p.x, p.y);
+        c.dbox[g] = d;
+        c.centered[g] = 1;
+        scatter_cell(r, d, c.dscale[g], c.flags[g]);
+    }
+}
+
+// nesterovUpdateCoordinates (nesterov.rs:2718-2745) fused with the centring and the scatter.
+__global__ void k_update_scatter(RegionDev r, CellsDev c, const float2* cur_slp, const float2* cur_sum, const float2* cur,
+                                 float2* next, float2* next_slp, float s, float coeff) {
+    int n = r.cell_end - r.cell_begin;
+    const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int g = r.cell_begin + k;
+        int4 d = c.dbox[g];
+        uint8_t f = c.flags[g];
+        float2 ns;
+        if (f & 4u) {
+            next[k] = cur[k];
+            ns = cur_slp[k];
+        } else {
+            float2 a = cur_slp[k], sm = cur_sum[k], cu = cur[k];
+            float nx = a.x + s * sm.x;
+            float ny = a.y + s * sm.y;
+            float sx = nx + coeff * (nx - cu.x);
+            float sy = ny + coeff * (ny - cu.y);
+            float hx = (float)(d.z - d.x) / 2.0f, hy = (float)(d.w - d.y) / 2.0f;
+            next[k] = make_float2(inside(nx, hx, lx, ux), inside(ny, hy, ly, uy));
+            ns = make_float2(inside(sx, hx, lx, ux), inside(sy, hy, ly, uy));
+        }
+        next_slp[k] = ns;
+        d = centre_box(d, ns.x, ns.y);
+        c.dbox[g] = d;
+        scatter_cell(r, d, c.dscale[g], f);
+    }
+}
+
+// updateBinsGCellDensityArea's per-bin part (nesterov.rs:2252-2262): density, the overflow
+// terms (summed here in FP64), and the accumulators cleared for the next scatter.
+__global__ void k_bin_density(RegionDev r, double* partials) {
+    __shared__ double sh[kBlock / 64];
+    const size_t M = (size_t)r.cnt_x * r.cnt_y;
+    double o = 0.0, ou = 0.0;
+    const float td = r.target_density;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < M; k += (size_t)gridDim.x * blockDim.x) {
+        int x = (int)(k % r.cnt_x), y = (int)(k / r.cnt_x);
+        long long inst = (long long)r.inst[k], fill = (long long)r.filler[k];
+        long long np = r.non_place[k], npu = r.non_place_unscaled[k];
+        long long area = (long long)(r.bin_x[x + 1] - r.bin_x[x]) * (long long)(r.bin_y[y + 1] - r.bin_y[y]);
+        float scaled = (float)area * td;
+        r.density[k] = ((float)inst + (float)fill + (float)np) / scaled;
+        float t = rmax((float)inst + (float)np - scaled, 0.0f);
+        float tu = rmax((float)inst + (float)npu - scaled, 0.0f);
+        if (!(t >= 0.0f)) t = 0.0f;
+        if (!(tu >= 0.0f)) tu = 0.0f;
+        o += (double)t;
+        ou += (double)tu;
+        r.inst[k] = 0ull;
+        r.filler[k] = 0ull;
+    }
+    double so = block_sum(o, sh);
+    double sou = block_sum(ou, sh);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = so;
+        partials[kPartials + blockIdx.x] = sou;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// The Poisson solve: 1-D cosine / sine transforms along rows, one 64-lane block per row, the
+// row in LDS; complex radix-2 FFT of the row's length L (a power of two), Makhoul's mapping
+// between the real transform and the FFT.  Conventions are Ooura's (ooura.rs:158-255):
+//   DCT-II   C[k] = sum_j a[j] cos(pi (j+1/2) k / L)
+//   DCT-III  C[k] = sum_j a[j] cos(pi j (k+1/2) / L)
+//   DST-III  S[k] = sum_{j=1..L} A[j] sin(pi j (k+1/2) / L), A[L] in a[0]
+// ---------------------------------------------------------------------------------------------
+
+enum RowMode { ROW_DCT2 = 0, ROW_DCT2_SCALE = 1, ROW_POISSON = 2, ROW_INV_FIELDS = 3, ROW_INV_PHI = 4 };
+
+__device__ __forceinline__ int bitrev(int m, int logL) { return (int)(__brev((unsigned)m) >> (32 - logL)); }
+
+// In-place FFT on (re, im) in LDS, input already in bit-reversed order.  dir -1: forward
+// (e^{-2 pi i jk/L}), +1: inverse, unnormalised.  tw[k] = e^{-2 pi i k / L}, k < L/2.
+__device__ void fft_lds(float* re, float* im, int L, const float2* tw, int dir) {
+    for (int len = 2; len <= L; len <<= 1) {
+        int half = len >> 1, stride = L / len;
+        __syncthreads();
+        for (int t = threadIdx.x; t < (L >> 1); t += blockDim.x) {
+            int grp = t / half, pos = t - grp * half;
+            int i0 = grp * len + pos, i1 = i0 + half;
+            float2 w = tw[pos * stride];
+            float wr = w.x, wi = dir < 0 ? w.y : -w.y;
+            float br = re[i1] * wr - im[i1] * wi;
+            float bi = re[i1] * wi + im[i1] * wr;
+            float ar = re[i0], ai = im[i0];
+            re[i0] = ar + br;
+            im[i0] = ai + bi;
+            re[i1] = ar - br;
+            im[i1] = ai - bi;
+        }
+    }
+    __syncthreads();
+}
+
+// DCT-II of the real row held in `src` (global, contiguous), result in re[0..L).
+__device__ void dct2_row(const float* src, float* re, float* im, int L, int logL, const float2* tw, const float2* dw) {
+    int h = L >> 1;
+    for (int m = threadIdx.x; m < L; m += blockDim.x) {
+        int s = m < h ? 2 * m : 2 * (L - 1 - m) + 1;
+        int q = bitrev(m, logL);
+        re[q] = src[s];
+        im[q] = 0.0f;
+    }
+    fft_lds(re, im, L, tw, -1);
+    // C[k] = Re(e^{-i pi k / 2L} V[k]); dw[k] = (cos, -sin)(pi k / 2L); each k is its own thread's
+    for (int k = threadIdx.x; k < L; k += blockDim.x) re[k] = dw[k].x * re[k] - dw[k].y * im[k];
+    __syncthreads();
+}
+
+// DCT-III (or DST-III when `sine`) of the row given by coef(k), result in im[0..L) -- the
+// caller's functor supplies a[k].  V[k] = e^{i pi k/2L} (c[k] - i c[L-k]), c[0] = 2 a[0],
+// c[L] = 0 (for the sine: c[j] = b[j], b[0] = a[0], b[j] = a[L-j]); y = IDFT(V) / 2, unshuffled.
+template <typename F>
+__device__ void dct3_row(F coef, bool sine, float* re, float* im, int L, int logL, const float2* tw, const float2* dw) {
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        float ck, cl;
+        if (!sine) {
+            ck = k == 0 ? 2.0f * coef(0) : coef(k);
+            cl = k == 0 ? 0.0f : coef(L - k);
+        } else {
+            // b[j] = a[L-j] (j >= 1), b[0] = a[0]
+            ck = k == 0 ? 2.0f * coef(0) : coef(L - k);
+            cl = k == 0 ? 0.0f : coef(k);
+        }
+        float cs = dw[k].x, sn = -dw[k].y;  // cos, sin (pi k / 2L)
+        float vr = cs * ck + sn * cl;
+        float vi = sn * ck - cs * cl;
+        int q = bitrev(k, logL);
+        re[q] = vr;
+        im[q] = vi;
+    }
+    fft_lds(re, im, L, tw, +1);
+    // unshuffle y[2m] = v[m], y[2m+1] = v[L-1-m] into im (the imaginary parts are not needed)
+    for (int j = threadIdx.x; j < L; j += blockDim.x) {
+        int m = (j & 1) ? (L - 1 - (j >> 1)) : (j >> 1);
+        float v = re[m] * 0.5f;
+        if (sine && (j & 1)) v = -v;
+        im[j] = v;
+    }
+    __syncthreads();
+}
+
+struct RowArgs {
+    const float* in0;
+    const float* in1;
+    float* out0;
+    float* out1;
+    float* out2;
+    int L, logL, nrows;
+    const float2* tw;
+    const float2* dw;
+    const float* wrow;   // wavenumber of the row index
+    const float* wcol;   // wavenumber of the element index
+    double scale;
+    int want_phi;
+};
+
+// One block (64 lanes) per row.  Modes:
+//  ROW_DCT2        out0[row] = DCT-II(in0[row])
+//  ROW_DCT2_SCALE  the same, then the halving of row/column 0 and x 4/(nx ny) (nesterov.rs:415-424)
+//  ROW_POISSON     in0 = scaled spectrum rows A[u][v] (u = row): Ex = p wx[u], Ey = p wy[v],
+//                  p = A / (wx^2 + wy^2) (nesterov.rs:425-440); out0 = DCT-III(Ex), out1 =
+//                  DST-III(Ey), out2 = DCT-III(p) when want_phi
+//  ROW_INV_FIELDS  out0 = DST-III(in0), out1 = DCT-III(in1)
+//  ROW_INV_PHI     out0 = DCT-III(in0)
+template <int MODE>
+__global__ void __launch_bounds__(64) k_rows(RowArgs a) {
+    extern __shared__ float lds[];
+    float* re = lds;
+    float* im = lds + a.L;
+    const int row = blockIdx.x;
+    const int L = a.L;
+    const size_t off = (size_t)row * L;
+    if (MODE == ROW_DCT2 || MODE == ROW_DCT2_SCALE) {
+        dct2_row(a.in0 + off, re, im, L, a.logL, a.tw, a.dw);
+        for (int k = threadIdx.x; k < L; k += blockDim.x) {
+            float v = re[k];
+            if (MODE == ROW_DCT2_SCALE) {
+                if ((row >= 1 && k == 0) || (row == 0 && k >= 1)) v = (float)((double)v * 0.5);
+                v = (float)((double)v * a.scale);
+            }
+            a.out0[off + k] = v;
+        }
+    } else if (MODE == ROW_POISSON) {
+        const float wu = a.wrow[row], wu2 = wu * wu;
+        const float* A = a.in0 + off;
+        auto pot = [&](int v) -> float {
+            if (row == 0 && v == 0) return 0.0f;
+            float wv = a.wcol[v];
+            return A[v] / (wu2 + wv * wv);
+        };
+        dct3_row([&](int v) { return row == 0 && v == 0 ? 0.0f : pot(v) * wu; }, false, re, im, L, a.logL, a.tw, a.dw);
+        for (int k = threadIdx.x; k < L; k += blockDim.x) a.out0[off + k] = im[k];
+        __syncthreads();
+        dct3_row([&](int v) { return row == 0 && v == 0 ? 0.0f : pot(v) * a.wcol[v]; }, true, re, im, L, a.logL, a.tw, a.dw);
+        for (int k = threadIdx.x; k < L; k += blockDim.x) a.out1[off + k] = im[k];
+        if (a.want_phi) {
+            __syncthreads();
+            dct3_row(pot, false, re, im, L, a.logL, a.tw, a.dw);
+            for (int k = threadIdx.x; k < L; k += blockDim.x) a.out2[off + k] = im[k];
+        }
+    } else if (MODE == ROW_INV_FIELDS) {
+        const float* X = a.in0 + off;
+        dct3_row([&](int v) { return X[v]; }, true, re, im, L, a.logL, a.tw, a.dw);
+        for (int k = threadIdx.x; k < L; k += blockDim.x) a.out0[off + k] = im[k];
+        __syncthreads();
+        const float* Y = a.in1 + off;
+        dct3_row([&](int v) { return Y[v]; }, false, re, im, L, a.logL, a.tw, a.dw);
+        for (int k = threadIdx.x; k < L; k += blockDim.x) a.out1[off + k] = im[k];
+    } else if (MODE == ROW_INV_PHI) {
+        const float* X = a.in0 + off;
+        dct3_row([&](int v) { return X[v]; }, false, re, im, L, a.logL, a.tw, a.dw);
+        for (int k = threadIdx.x; k < L; k += blockDim.x) a.out0[off + k] = im[k];
+    }
+}
+
+// Tiled transpose: in [R][C] -> out [C][R].
+__global__ void k_transpose(const float* in, float* out, int R, int C) {
+    __shared__ float tile[32][33];
+    int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int j = threadIdx.y; j < 32; j += blockDim.y) {
+        int r = by + j, c = bx + threadIdx.x;
+        if (r < R && c < C) tile[j][threadIdx.x] = in[(size_t)r * C + c];
+    }
+    __syncthreads();
+    for (int j = threadIdx.y; j < 32; j += blockDim.y) {
+        int c = bx + j, r = by + threadIdx.x;
+        if (r < R && c < C) out[(size_t)c * R + r] = tile[threadIdx.x][j];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// WA wirelength (nesterov.rs:2325-2430)
+// ---------------------------------------------------------------------------------------------
+
+// A net pin's position: its cell's density centre plus its offset once the cell has been
+// centred, else where init left it (nesterov.rs:2150).
+__device__ __forceinline__ int2 pin_pos(const NetsDev& n, const CellsDev& c, int k) {
+    int cell = n.pcell[k];
+    if (cell >= 0 && c.centered[cell]) {
+        int4 d = c.dbox[cell];
+        int2 o = n.poff[k];
+        return make_int2((d.x + d.z) / 2 + o.x, (d.y + d.w) / 2 + o.y);
+    }
+    return n.pxy0[k];
+}
+
+// One thread per net, the net's pins in its order (the reference's float sums, bit for bit).
+__global__ void k_wa_small(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl) {
+    __shared__ unsigned long long sh[kBlock / 64];
+    unsigned long long h = 0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n.num_small; e += gridDim.x * blockDim.x) {
+        int s = n.start[e], t = n.start[e + 1];
+        int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+        for (int k = s; k < t; ++k) {
+            int2 p = pin_pos(n, c, k);
+            lx = min(lx, p.x);
+            ly = min(ly, p.y);
+            ux = max(ux, p.x);
+            uy = max(uy, p.y);
+        }
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k = s; k < t; ++k) {
+            int2 p = pin_pos(n, c, k);
+            float e0 = (float)(lx - p.x) * cx, e1 = (float)(p.x - ux) * cx;
+            float e2 = (float)(ly - p.y) * cy, e3 = (float)(p.y - uy) * cy;
+            if (e0 > -300.0f) {
+                float v = fast_exp(e0);
+                a.x += v;
+                a.y += (float)p.x * v;
+            }
+            if (e1 > -300.0f) {
+                float v = fast_exp(e1);
+                a.z += v;
+                a.w += (float)p.x * v;
+            }
+            if (e2 > -300.0f) {
+                float v = fast_exp(e2);
+                b.x += v;
+                b.y += (float)p.y * v;
+            }
+            if (e3 > -300.0f) {
+                float v = fast_exp(e3);
+                b.z += v;
+                b.w += (float)p.y * v;
+            }
+        }
+        n.box[e] = make_int4(lx, ly, ux, uy);
+        n.sx[e] = a;
+        n.sy[e] = b;
+        if (ux >= lx) h += (unsigned long long)((long long)ux - lx + (long long)uy - ly);
+    }
+    h = wave_sum(h);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
+        if (t) atomicAdd(hpwl, t);
+    }
+}
+
+// One block per high-fanout net: the box and the sums reduced across the block (FP64 partial
+// sums, rounded once to float).
+__global__ void k_wa_big(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl) {
+    __shared__ int shi[4][kBlock / 64];
+    __shared__ double shd[kBlock / 64];
+    const int e = n.num_small + blockIdx.x;
+    const int s = n.start[e], t = n.start[e + 1];
+    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+    for (int k = s + threadIdx.x; k < t; k += blockDim.x) {
+        int2 p = pin_pos(n, c, k);
+        lx = min(lx, p.x);
+        ly = min(ly, p.y);
+        ux = max(ux, p.x);
+        uy = max(uy, p.y);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lx = min(lx, __shfl_down(lx, o, 64));
+        ly = min(ly, __shfl_down(ly, o, 64));
+        ux = max(ux, __shfl_down(ux, o, 64));
+        uy = max(uy, __shfl_down(uy, o, 64));
+    }
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        shi[0][w] = lx;
+        shi[1][w] = ly;
+        shi[2][w] = ux;
+        shi[3][w] = uy;
+    }
+    __syncthreads();
+    lx = shi[0][0];
+    ly = shi[1][0];
+    ux = shi[2][0];
+    uy = shi[3][0];
+    for (int i = 1; i < kBlock / 64; ++i) {
+        lx = min(lx, shi[0][i]);
+        ly = min(ly, shi[1][i]);
+        ux = max(ux, shi[2][i]);
+        uy = max(uy, shi[3][i]);
+    }
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = s + threadIdx.x; k < t; k += blockDim.x) {
+        int2 p = pin_pos(n, c, k);
+        float e0 = (float)(lx - p.x) * cx, e1 = (float)(p.x - ux) * cx;
+        float e2 = (float)(ly - p.y) * cy, e3 = (float)(p.y - uy) * cy;
+        if (e0 > -300.0f) {
+            float v = fast_exp(e0);
+            acc[0] += v;
+            acc[1] += (double)((float)p.x * v);
+        }
+        if (e1 > -300.0f) {
+            float v = fast_exp(e1);
+            acc[2] += v;
+            acc[3] += (double)((float)p.x * v);
+        }
+        if (e2 > -300.0f) {
+            float v = fast_exp(e2);
+            acc[4] += v;
+            acc[5] += (double)((float)p.y * v);
+        }
+        if (e3 > -300.0f) {
+            float v = fast_exp(e3);
+            acc[6] += v;
+            acc[7] += (double)((float)p.y * v);
+        }
+    }
+    float r[8];
+    for (int q = 0; q < 8; ++q) r[q] = (float)block_sum(acc[q], shd);
+    if (threadIdx.x == 0) {
+        n.box[e] = make_int4(lx, ly, ux, uy);
+        n.sx[e] = make_float4(r[0], r[1], r[2], r[3]);
+        n.sy[e] = make_float4(r[4], r[5], r[6], r[7]);
+        if (ux >= lx) atomicAdd(hpwl, (unsigned long long)((long long)ux - lx + (long long)uy - ly));
+    }
+}
+
+// HPWL only (nesterov.rs:2297): boxes of every net.
+__global__ void k_hpwl(NetsDev n, CellsDev c, unsigned long long* hpwl) {
+    __shared__ unsigned long long sh[kBlock / 64];
+    unsigned long long h = 0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n.num_nets; e += gridDim.x * blockDim.x) {
+        int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+        for (int k = n.start[e]; k < n.start[e + 1]; ++k) {
+            int2 p = pin_pos(n, c, k);
+            lx = min(lx, p.x);
+            ly = min(ly, p.y);
+            ux = max(ux, p.x);
+            uy = max(uy, p.y);
+        }
+        if (ux >= lx) h += (unsigned long long)((long long)ux - lx + (long long)uy - ly);
+    }
+    h = wave_sum(h);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
+        if (t) atomicAdd(hpwl, t);
+    }
+}
+
+// getWireLengthGradientPinWA (nesterov.rs:2392): one exponential's term; the numerator and the
+// quotient in double, the result a float.
+__device__ __forceinline__ float wa_term_min(float S0, float S1, float v, float coef, float pc) {
+    double num = (double)S0 * ((double)v * (1.0 - (double)(coef * pc))) + (double)(coef * v * S1);
+    return (float)(num / (double)(S0 * S0));
+}
+__device__ __forceinline__ float wa_term_max(float S0, float S1, float v, float coef, float pc) {
+    double num = (double)S0 * ((double)v * (1.0 + (double)(coef * pc))) - (double)(coef * v * S1);
+    return (float)(num / (double)(S0 * S0));
+}
+
+// updateGradients (nesterov.rs:2455-2489) per cell: the WA gradient over the cell's pins in
+// their order, each times its net's weight (2418); the density gradient over the bins its box
+// covers, x outer, y inner (2434); the preconditioned sum; FP64 partial sums of |wl|, |den|,
+// and, for a trial, the squared coordinate and gradient distances (nesterovUpdateStepLength,
+// 2755-2756) to a_slp/a_sum.
+__global__ void k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float cy, float penalty, float2* wl_out, float2* den_out,
+                           float2* sum_out, const float2* a_slp, const float2* b_slp, const float2* a_sum, double* partials,
+                           int nan_inject) {
+    __shared__ double sh[kBlock / 64];
+    const int n = r.cell_end - r.cell_begin;
+    double s_wl = 0.0, s_den = 0.0, s_c = 0.0, s_g = 0.0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int g = r.cell_begin + k;
+        const int4 d = c.dbox[g];
+        const int dcx = (d.x + d.z) / 2, dcy = (d.y + d.w) / 2;
+        float wx = 0.0f, wy = 0.0f;
+        for (int q = c.pin_start[g]; q < c.pin_start[g + 1]; ++q) {
+            const int e = c.cp_net[q];
+            const int2 o = c.cp_off[q];
+            const int px = dcx + o.x, py = dcy + o.y;
+            const int4 b = nt.box[e];
+            const float4 SX = nt.sx[e], SY = nt.sy[e];
+            const float fpx = (float)px, fpy = (float)py;
+            float gmin_x = 0.0f, gmax_x = 0.0f, gmin_y = 0.0f, gmax_y = 0.0f;
+            float e0 = (float)(b.x - px) * cx, e1 = (float)(px - b.z) * cx;
+            float e2 = (float)(b.y - py) * cy, e3 = (float)(py - b.w) * cy;
+            if (e0 > -300.0f) gmin_x = wa_term_min(SX.x, SX.y, fast_exp(e0), cx, fpx);
+            if (e1 > -300.0f) gmax_x = wa_term_max(SX.z, SX.w, fast_exp(e1), cx, fpx);
+            if (e2 > -300.0f) gmin_y = wa_term_min(SY.x, SY.y, fast_exp(e2), cy, fpy);
+            if (e3 > -300.0f) gmax_y = wa_term_max(SY.z, SY.w, fast_exp(e3), cy, fpy);
+            const float w = nt.weight[e];
+            float tx = gmin_x - gmax_x, ty = gmin_y - gmax_y;
+            tx *= w;
+            ty *= w;
+            wx += tx;
+            wy += ty;
+        }
+        // density gradient
+        int x0, x1, y0, y1;
+        bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
+        bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
+        const float scale = c.dscale[g];
+        float fx = 0.0f, fy = 0.0f;
+        for (int x = x0; x < x1; ++x) {
+            int bx0 = r.bin_x[x], bx1 = r.bin_x[x + 1];
+            for (int y = y0; y < y1; ++y) {
+                float a = overlap_area(bx0, r.bin_y[y], bx1, r.bin_y[y + 1], d) * scale;
+                size_t kb = (size_t)y * r.cnt_x + x;
+                fx += a * r.field_x[kb];
+                fy += a * r.field_y[kb];
+            }
+        }
+        float2 pr = c.pre[g];
+        float sx = wx + penalty * fx;
+        float sy = wy + penalty * fy;
+        float pre = rmax(pr.x + penalty * pr.y, 1.0f);
+        sx /= pre;
+        sy /= pre;
+        wl_out[k] = make_float2(wx, wy);
+        den_out[k] = make_float2(fx, fy);
+        sum_out[k] = make_float2(sx, sy);
+        s_wl += (double)fabsf(wx) + (double)fabsf(wy);
+        s_den += (double)fabsf(fx) + (double)fabsf(fy);
+        if (a_slp) {
+            float2 pa = a_slp[k], pb = b_slp[k], ga = a_sum[k];
+            float dx = pa.x - pb.x, dy = pa.y - pb.y;
+            float gx = ga.x - sx, gy = ga.y - sy;
+            s_c += (double)(dx * dx) + (double)(dy * dy);
+            s_g += (double)(gx * gx) + (double)(gy * gy);
+        }
+    }
+    double t0 = block_sum(s_wl, sh);
+    double t1 = block_sum(s_den, sh);
+    double t2 = block_sum(s_c, sh);
+    double t3 = block_sum(s_g, sh);
+    if (threadIdx.x == 0) {
+        if (nan_inject) t0 = __longlong_as_double(0x7ff8000000000000ll);
+        partials[2 * kPartials + blockIdx.x] = t0;
+        partials[3 * kPartials + blockIdx.x] = t1;
+        partials[4 * kPartials + blockIdx.x] = t2;
+        partials[5 * kPartials + blockIdx.x] = t3;
+    }
+}
+
+// Squared distances between coordinate sets (getDistance's sums, nesterov.rs:319) as partials 4
+// (a - b) and 5 (c - d).
+__global__ void k_distance(int n, const float2* a, const float2* b, const float2* c, const float2* d, double* partials) {
+    __shared__ double sh[kBlock / 64];
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        float2 p = a[k], q = b[k], u = c[k], v = d[k];
+        float dx = p.x - q.x, dy = p.y - q.y, gx = u.x - v.x, gy = u.y - v.y;
+        s0 += (double)(dx * dx) + (double)(dy * dy);
+        s1 += (double)(gx * gx) + (double)(gy * gy);
+    }
+    double t0 = block_sum(s0, sh), t1 = block_sum(s1, sh);
+    if (threadIdx.x == 0) {
+        partials[4 * kPartials + blockIdx.x] = t0;
+        partials[5 * kPartials + blockIdx.x] = t1;
+    }
+}
+
+// updateInitialPrevSLPCoordi (nesterov.rs:2583).
+__global__ void k_initial_prev(RegionDev r, CellsDev c, const float2* cur_slp, const float2* cur_sum, float coef, float2* prev_slp) {
+    int n = r.cell_end - r.cell_begin;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int g = r.cell_begin + k;
+        float2 a = cur_slp[k];
+        if (c.flags[g] & 4u) {
+            prev_slp[k] = a;
+            continue;
+        }
+        int4 d = c.dbox[g];
+        float px = a.x - coef * cur_sum[k].x, py = a.y - coef * cur_sum[k].y;
+        float hx = (float)(d.z - d.x) / 2.0f, hy = (float)(d.w - d.y) / 2.0f;
+        prev_slp[k] = make_float2(inside(px, hx, (float)r.lx, (float)r.ux), inside(py, hy, (float)r.ly, (float)r.uy));
+    }
+}
+
+// The finish: each quantity's block partials summed in index order by one block (a fixed
+// per-thread stride then a fixed tree).
+__global__ void k_finish(const double* partials, int nb_bins, int nb_cells, Sums* out, const unsigned long long* hpwl) {
+    __shared__ double sh[kBlock / 64];
+    double v[6];
+    for (int q = 0; q < 6; ++q) {
+        int cnt = q < 2 ? nb_bins : nb_cells;
+        double s = 0.0;
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) s += partials[q * kPartials + i];
+        v[q] = block_sum(s, sh);
+    }
+    if (threadIdx.x == 0) {
+        if (nb_bins > 0) {
+            out->overflow = v[0];
+            out->overflow_unscaled = v[1];
+        }
+        if (nb_cells > 0) {
+            out->wl_abs = v[2];
+            out->den_abs = v[3];
+            out->coord_sq = v[4];
+            out->grad_sq = v[5];
+        }
+        if (hpwl) out->hpwl = *hpwl;
+    }
+}
+
+// nesterov.rs:2850-2859: a locked instance cell's next slots take its current values.
+__global__ void k_lock_fix(RegionDev r, CellsDev c, float2* n0, float2* n1, float2* n2, float2* n3, float2* n4, const float2* c0,
+                           const float2* c1, const float2* c2, const float2* c3, const float2* c4) {
+    int n = r.cell_end - r.cell_begin;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        uint8_t f = c.flags[r.cell_begin + k];
+        if ((f & 1u) && (f & 4u)) {
+            n0[k] = c0[k];
+            n1[k] = c1[k];
+            n2[k] = c2[k];
+            n3[k] = c3[k];
+            n4[k] = c4[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------------------------
+
+static int grid_for(size_t n) {
+    size_t b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    if (b > (size_t)kMaxBlocks) b = kMaxBlocks;
+    return (int)b;
+}
+
+int block_partials_bins(const RegionDev& r) { return grid_for((size_t)r.cnt_x * r.cnt_y); }
+int block_partials_cells(const RegionDev& r) { return grid_for((size_t)(r.cell_end - r.cell_begin)); }
+
+void launch_zero_bins(hipStream_t s, const RegionDev& r) {
+    size_t M = (size_t)r.cnt_x * r.cnt_y;
+    hipLaunchKernelGGL(k_zero_bins, dim3(grid_for(M)), dim3(kBlock), 0, s, r.inst, r.filler, M);
+}
+
+void launch_center_init(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* cur_slp, float2* prev_slp, float2* cur) {
+    int n = r.cell_end - r.cell_begin;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_center_init, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, cur_slp, prev_slp, cur);
+}
+
+void launch_center_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy) {
+    int n = r.cell_end - r.cell_begin;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_center_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, xy);
+}
+
+void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
+                           const float2* cur, float2* next, float2* next_slp, float step, float coeff) {
+    int n = r.cell_end - r.cell_begin;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_update_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, cur_slp, cur_sum, cur, next, next_slp, step, coeff);
+}
+
+void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials) {
+    hipLaunchKernelGGL(k_bin_density, dim3(block_partials_bins(r)), dim3(kBlock), 0, s, r, partials);
+}
+
+static int ilog2(int L) {
+    int k = 0;
+    while ((1 << k) < L) ++k;
+    return k;
+}
+
+static void transpose(hipStream_t s, const float* in, float* out, int R, int C) {
+    dim3 g((C + 31) / 32, (R + 31) / 32), b(32, 8);
+    hipLaunchKernelGGL(k_transpose, g, b, 0, s, in, out, R, C);
+}
+
+// Bins are y-major D[y][x].  P1: rows along x, DCT-II -> B[y][u]; transpose -> [u][y];
+// P2: rows along y, DCT-II + halving + scale -> A[u][v]; P3: rows along v, the Poisson divide and
+// the inverse transforms along y -> Ex'[u][y] (cos), Ey'[u][y] (sin), phi'[u][y]; transpose ->
+// [y][u]; P4: rows along u -> field_x = sin along x, field_y = cos along x, phi = cos along x.
+void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi) {
+    const int nx = r.cnt_x, ny = r.cnt_y;
+    const int lnx = ilog2(nx), lny = ilog2(ny);
+    RowArgs a{};
+    // P1
+    a.in0 = r.density;
+    a.out0 = r.work0;
+    a.L = nx;
+    a.logL = lnx;
+    a.nrows = ny;
+    a.tw = r.tw_x;
+    a.dw = r.dw_x;
+    hipLaunchKernelGGL(k_rows<ROW_DCT2>, dim3(ny), dim3(64), 2 * nx * sizeof(float), s, a);
+    transpose(s, r.work0, r.work1, ny, nx);  // [u][y]
+    // P2
+    a = RowArgs{};
+    a.in0 = r.work1;
+    a.out0 = r.work0;
+    a.L = ny;
+    a.logL = lny;
+    a.nrows = nx;
+    a.tw = r.tw_y;
+    a.dw = r.dw_y;
+    a.scale = 4.0 / (double)nx / (double)ny;
+    hipLaunchKernelGGL(k_rows<ROW_DCT2_SCALE>, dim3(nx), dim3(64), 2 * ny * sizeof(float), s, a);  // A[u][v]
+    // P3
+    a = RowArgs{};
+    a.in0 = r.work0;
+    a.out0 = r.work1;  // Ex' [u][y]
+    a.out1 = r.work2;  // Ey' [u][y]
+    a.out2 = r.work3;  // phi' [u][y]
+    a.L = ny;
+    a.logL = lny;
+    a.nrows = nx;
+    a.tw = r.tw_y;
+    a.dw = r.dw_y;
+    a.wrow = r.wx;
+    a.wcol = r.wy;
+    a.want_phi = want_phi ? 1 : 0;
+    hipLaunchKernelGGL(k_rows<ROW_POISSON>, dim3(nx), dim3(64), 2 * ny * sizeof(float), s, a);
+    transpose(s, r.work1, r.work0, nx, ny);  // Ex [y][u]
+    transpose(s, r.work2, r.work1, nx, ny);  // Ey [y][u]
+    if (want_phi) transpose(s, r.work3, r.work2, nx, ny);
+    // P4
+    a = RowArgs{};
+    a.in0 = r.work0;
+    a.in1 = r.work1;
+    a.out0 = r.field_x;
+    a.out1 = r.field_y;
+    a.L = nx;
+    a.logL = lnx;
+    a.nrows = ny;
+    a.tw = r.tw_x;
+    a.dw = r.dw_x;
+    hipLaunchKernelGGL(k_rows<ROW_INV_FIELDS>, dim3(ny), dim3(64), 2 * nx * sizeof(float), s, a);
+    if (want_phi) {
+        RowArgs p{};
+        p.in0 = r.work2;
+        p.out0 = r.phi;
+        p.L = nx;
+        p.logL = lnx;
+        p.nrows = ny;
+        p.tw = r.tw_x;
+        p.dw = r.dw_x;
+        hipLaunchKernelGGL(k_rows<ROW_INV_PHI>, dim3(ny), dim3(64), 2 * nx * sizeof(float), s, p);
+    }
+}
+
+void launch_wa_forward(hipStream_t s, const NetsDev& n, const CellsDev& c, float coef_x, float coef_y, unsigned long long* hpwl) {
+    if (n.num_small > 0) hipLaunchKernelGGL(k_wa_small, dim3(grid_for(n.num_small)), dim3(kBlock), 0, s, n, c, coef_x, coef_y, hpwl);
+    int big = n.num_nets - n.num_small;
+    if (big > 0) hipLaunchKernelGGL(k_wa_big, dim3(big), dim3(kBlock), 0, s, n, c, coef_x, coef_y, hpwl);
+}
+
+void launch_hpwl(hipStream_t s, const NetsDev& n, const CellsDev& c, unsigned long long* hpwl) {
+    hipLaunchKernelGGL(k_hpwl, dim3(grid_for(n.num_nets)), dim3(kBlock), 0, s, n, c, hpwl);
+}
+
+void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, float coef_x, float coef_y,
+                     float penalty, float2* wl, float2* den, float2* sum, const float2* a_slp, const float2* b_slp,
+                     const float2* a_sum, double* partials, int nan_inject) {
+    hipLaunchKernelGGL(k_gradient, dim3(block_partials_cells(r)), dim3(kBlock), 0, s, r, c, n, coef_x, coef_y, penalty, wl, den, sum,
+                       a_slp, b_slp, a_sum, partials, nan_inject);
+}
+
+void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int nparts_cells, Sums* out, const unsigned long long* hpwl) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, s, partials, nparts_bins, nparts_cells, out, hpwl);
+}
+
+void launch_distance(hipStream_t s, int n, const float2* a, const float2* b, const float2* c, const float2* d, double* partials) {
+    hipLaunchKernelGGL(k_distance, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, b, c, d, partials);
+}
+
+void launch_initial_prev(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
+                         float coef, float2* prev_slp) {
+    int n = r.cell_end - r.cell_begin;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_initial_prev, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, cur_slp, cur_sum, coef, prev_slp);
+}
+
+void launch_lock_fix(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* const* nv, const float2* const* cv, int nvec) {
+    (void)nvec;
+    int n = r.cell_end - r.cell_begin;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_lock_fix, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, nv[0], nv[1], nv[2], nv[3], nv[4], cv[0], cv[1], cv[2],
+                       cv[3], cv[4]);
+}
+
+}  // namespace gplk

@@ -1,0 +1,1285 @@
+// The Nesterov placement loop on the GPU: the host side of NesterovPlace / NesterovBase
+// (nesterov.rs:2512-3168, 4115-4610) driving the HIP kernels of kernels.hip.
+//
+// The per-cell, per-pin, per-net and per-bin work runs on the device; the scalar control (step
+// length acceptance, the penalty and WA-coefficient schedule, divergence, convergence) runs
+// here on values the device reduces in FP64 and hands back once per back-tracking trial (one
+// 128-byte copy per region).  Slot rotation (prev <- cur <- next) is a swap of device pointers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "host.h"
+#include "kernels.h"
+
+namespace {
+
+using gpl::strfmt;
+using Clock = std::chrono::steady_clock;
+
+enum { S_CUR_SLP, S_CUR_SLP_WL, S_CUR_SLP_DEN, S_CUR_SLP_SUM, S_NEXT_SLP, S_NEXT_SLP_WL, S_NEXT_SLP_DEN, S_NEXT_SLP_SUM,
+       S_PREV_SLP, S_PREV_SLP_WL, S_PREV_SLP_DEN, S_PREV_SLP_SUM, S_CUR, S_NEXT, S_SNAP, S_SNAP_SLP, S_SNAP_SLP_SUM,
+       S_SNAP_PREV_SLP_SUM, S_COUNT };
+enum Slot { SLOT_PREV, SLOT_CUR, SLOT_NEXT };
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError{e, what};
+}
+
+struct Region {
+    gpl_region R{};
+    int32_t n = 0;
+    gplk::RegionDev dev{};
+    float2* v[S_COUNT] = {};
+    double* partials = nullptr;
+    bool has_locked = false;
+    int64_t overflow = 0, overflow_unscaled = 0;
+    float snapshot_density_penalty = 0, snapshot_step_length = 0, step_length = 0, density_penalty = 0;
+    float wl_grad_sum = 0, density_grad_sum = 0, sum_overflow = 0, sum_overflow_unscaled = 0, base_wl_coef_nb = 0;
+    int64_t prev_hpwl = 0, prev_reported_hpwl = 0;
+    float prev_reported_overflow_unscaled = 0, min_sum_overflow = 1e30f, hpwl_with_min_sum_overflow = 1e30f;
+    float coordi_distance = 0, peak_coordi_distance = 0, max_phi_coef = 0;
+    bool max_phi_coef_changed = false, is_converged = false, is_diverged = false, reprint_header = false;
+    int64_t nia = 0;  // nesterov_insts_area
+};
+
+// Device time per kernel family (profiling mode).
+enum Fam { F_SCATTER, F_BINS, F_DCT, F_WA, F_GRAD, F_REDUCE, F_COUNT };
+const char* kFamName[F_COUNT] = {"update_scatter", "bin_density", "dct_solve", "wa_forward", "gradient", "reduce"};
+
+}  // namespace
+
+struct gpl_ctx {
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    int32_t ncells = 0, npins = 0, nnets = 0;
+    gplk::CellsDev cells{};
+    gplk::NetsDev nets{};
+    std::vector<Region> rg;
+    gplk::Sums* d_sums = nullptr;  // one per region
+    gplk::Sums* h_sums = nullptr;  // pinned host copy
+    unsigned long long* d_hpwl = nullptr;
+    uint8_t* d_flags = nullptr;
+    std::vector<uint8_t> h_flags;
+    std::vector<int32_t> net_sorted_of;  // original net -> sorted index
+    std::vector<int32_t> net_orig_of;    // sorted index -> original net
+    std::vector<int32_t> cell_inst, inst_box;
+    int32_t site_x = 0, dbu = 1000;
+    gpl_opts o{};
+    // NesterovPlace
+    float wl_coef_x = 0, wl_coef_y = 0, base_wl_coef = 0;
+    int32_t recursion_wl = 0, recursion_init_slp = 0;
+    float initial_prev_coordi_update_coef = 100.0f, reference_hpwl = 0;
+    int32_t max_iter = 0;
+    float average_overflow = 0, average_overflow_unscaled = 0;
+    int64_t min_hpwl = INT64_MAX;
+    bool is_min_hpwl = false;
+    int32_t diverge_snapshot_iter = 0;
+    float diverge_snapshot_overflow = 0;
+    int32_t num_region_diverged = 0, diverge_code = 0;
+    std::string diverge_msg;
+    // LoopSnapshots (nesterov.rs:4428), routability off
+    bool diverge_saved = false;
+    float diverge_coef_x = 0, diverge_coef_y = 0;
+    int64_t diverge_hpwl = 0;
+    int32_t diverge_reverts = 0;
+    // bookkeeping
+    int64_t cur_hpwl = 0;
+    int32_t trials = 0, total_trials = 0, last_iter = -1;
+    bool converged = false, initialized = false;
+    double t_init = 0, t_loop = 0, t_converge = 0;
+    std::vector<gpl_iter_row> hist;
+    gpl::Log log;
+    std::string err;
+    int32_t inject_kind = 0, inject_after = -1, iters_run = 0;
+    // profiling
+    bool profiling = false;
+    hipEvent_t ev[64];
+    int nev = 0;
+    int ev_fam[32];
+    double fam_ms[F_COUNT] = {};
+    int fam_n[F_COUNT] = {};
+
+    template <typename T>
+    T* dalloc(size_t n) {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+        allocs.push_back(p);
+        return (T*)p;
+    }
+    template <typename T>
+    T* upload(const T* h, size_t n) {
+        T* d = dalloc<T>(n);
+        if (n) check(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, stream), "upload");
+        return d;
+    }
+    ~gpl_ctx() {
+        if (stream) hipStreamSynchronize(stream);
+        for (void* p : allocs) hipFree(p);
+        if (h_sums) hipHostFree(h_sums);
+        if (profiling)
+            for (int i = 0; i < 64; ++i) hipEventDestroy(ev[i]);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    // ---- profiling: events bracket each kernel family; times summed at each sync ----
+    void mark(int fam) {
+        if (!profiling || nev >= 62) return;
+        check(hipEventRecord(ev[nev], stream), "event");
+        ev_fam[nev / 2] = fam;
+        ++nev;
+    }
+    void mark_end() {
+        if (!profiling || nev % 2 == 0) return;
+        check(hipEventRecord(ev[nev], stream), "event");
+        ++nev;
+    }
+    void collect() {
+        if (!profiling) return;
+        for (int i = 0; i + 1 < nev; i += 2) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+            fam_ms[ev_fam[i / 2]] += ms;
+            fam_n[ev_fam[i / 2]] += 1;
+        }
+        nev = 0;
+    }
+
+    void sync_stats() {
+        check(hipMemcpyAsync(h_sums, d_sums, sizeof(gplk::Sums) * (rg.size() + 1), hipMemcpyDeviceToHost, stream), "stats copy");
+        check(hipStreamSynchronize(stream), "sync");
+        collect();
+    }
+
+    // ---- the reference's steps ------------------------------------------------------------
+    int64_t nesterov_insts_area(const Region& r) const {
+        return r.R.std_insts_area + (int64_t)std::round(float(r.R.macro_insts_area) * r.R.target_density);
+    }
+
+    void zero_hpwl() { check(hipMemsetAsync(d_hpwl, 0, sizeof(unsigned long long), stream), "memset"); }
+
+    // getHpwl (nesterov.rs:2297), read back.
+    int64_t hpwl_now() {
+        zero_hpwl();
+        gplk::launch_hpwl(stream, nets, cells, d_hpwl);
+        unsigned long long h = 0;
+        check(hipMemcpyAsync(&h, d_hpwl, sizeof h, hipMemcpyDeviceToHost, stream), "hpwl copy");
+        check(hipStreamSynchronize(stream), "sync");
+        return (int64_t)h;
+    }
+
+    void bins_and_finish(Region& r, int b) {
+        mark(F_BINS);
+        gplk::launch_bin_density(stream, r.dev, r.partials);
+        gplk::launch_finish(stream, r.partials, gplk::block_partials_bins(r.dev), 0, d_sums + b, nullptr);
+        mark_end();
+    }
+
+    void solve(Region& r, bool want_phi = false) {
+        mark(F_DCT);
+        gplk::launch_solve(stream, r.dev, want_phi);
+        mark_end();
+    }
+
+    void wa_forward() {
+        mark(F_WA);
+        zero_hpwl();
+        gplk::launch_wa_forward(stream, nets, cells, wl_coef_x, wl_coef_y, d_hpwl);
+        mark_end();
+    }
+
+    // The gradient of region b into `slot`; with `trial`, the distances of the step length too.
+    void launch_gradient(int b, int slot, bool trial, int nan_inject = 0) {
+        Region& r = rg[b];
+        int base = slot == SLOT_PREV ? S_PREV_SLP : slot == SLOT_CUR ? S_CUR_SLP : S_NEXT_SLP;
+        mark(F_GRAD);
+        gplk::launch_gradient(stream, r.dev, cells, nets, wl_coef_x, wl_coef_y, r.density_penalty, r.v[base + 1], r.v[base + 2],
+                              r.v[base + 3], trial ? r.v[S_CUR_SLP] : nullptr, trial ? r.v[S_NEXT_SLP] : nullptr,
+                              trial ? r.v[S_CUR_SLP_SUM] : nullptr, r.partials, nan_inject);
+        gplk::launch_finish(stream, r.partials, 0, gplk::block_partials_cells(r.dev), d_sums + b, d_hpwl);
+        mark_end();
+    }
+
+    bool inject_now() const { return inject_kind == 1 && inject_after >= 0 && iters_run >= inject_after; }
+
+    // updateGradients' sums read back; npUpdate*Gradient's zero-sum recursion and NaN check
+    // (nesterov.rs:2512-2528).  Regions from `first` are (re)evaluated in order.
+    void np_update_gradients(int first, int last, int slot, bool trial) {
+        for (int b = first; b < last; ++b) {
+            if (rg[b].is_converged) continue;
+            launch_gradient(b, slot, trial, slot == SLOT_NEXT && inject_now());
+        }
+        sync_stats();
+        for (int b = first; b < last; ++b) {
+            Region& r = rg[b];
+            if (!r.is_converged) {
+                r.wl_grad_sum = (float)h_sums[b].wl_abs;
+                r.density_grad_sum = (float)h_sums[b].den_abs;
+            }
+            if (r.wl_grad_sum == 0.0f && recursion_wl < 10) {
+                wl_coef_x = (float)((double)wl_coef_x * 0.5);
+                wl_coef_y = (float)((double)wl_coef_y * 0.5);
+                base_wl_coef = (float)((double)base_wl_coef * 0.5);
+                wa_forward();
+                recursion_wl += 1;
+                np_update_gradients(b, last, slot, trial);
+                return;
+            }
+            auto bad = [](float v) { return std::isnan(v) || std::isinf(v); };
+            if (bad(r.wl_grad_sum) || bad(r.density_grad_sum)) {
+                diverge_code = 306;
+                diverge_msg = "RePlAce diverged at wire/density gradient Sum. An internal value is NaN or Inf.";
+                num_region_diverged = 1;
+            }
+        }
+    }
+
+    void update_wirelength_coef(float overflow) {
+        float c;
+        if (overflow > 1.0f)
+            c = (float)0.1;
+        else if ((double)overflow < 0.1)
+            c = 10.0f;
+        else
+            c = (float)(1.0 / std::pow(10.0, ((double)overflow - 0.1) * 20.0 / 9.0 - 1.0));
+        wl_coef_x = c * base_wl_coef;
+        wl_coef_y = c * base_wl_coef;
+    }
+
+    // initDensity1 (nesterov.rs:2546).
+    void init_density1(int b) {
+        Region& r = rg[b];
+        mark(F_SCATTER);
+        gplk::launch_center_init(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_PREV_SLP], r.v[S_CUR]);
+        mark_end();
+        bins_and_finish(r, b);
+        r.prev_hpwl = hpwl_now();
+        solve(r);
+        sync_stats();
+        r.overflow = (int64_t)h_sums[b].overflow;
+        r.overflow_unscaled = (int64_t)h_sums[b].overflow_unscaled;
+        r.base_wl_coef_nb = (float)((double)o.init_wirelength_coef / ((double)(float)(r.R.size_x + r.R.size_y) * 0.5));
+        float nia = (float)r.nia;
+        r.sum_overflow = (float)r.overflow / nia;
+        r.sum_overflow_unscaled = (float)r.overflow_unscaled / nia;
+    }
+
+    // getDistance of two vector pairs (nesterov.rs:319), FP64 sums.
+    std::pair<float, float> distances(int b, const float2* a, const float2* bb, const float2* c, const float2* d) {
+        Region& r = rg[b];
+        mark(F_REDUCE);
+        gplk::launch_distance(stream, r.n, a, bb, c, d, r.partials);
+        gplk::launch_finish(stream, r.partials, 0, gplk::block_partials_cells(r.dev), d_sums + b, nullptr);
+        mark_end();
+        sync_stats();
+        if (r.n == 0) return {0.0f, 0.0f};
+        float x = (float)std::sqrt(h_sums[b].coord_sq / (2.0 * (double)r.n));
+        float y = (float)std::sqrt(h_sums[b].grad_sq / (2.0 * (double)r.n));
+        return {x, y};
+    }
+
+    // initDensity2 (nesterov.rs:2600).
+    float init_density2(int b) {
+        Region& r = rg[b];
+        if (r.wl_grad_sum == 0.0f) {
+            r.density_penalty = o.init_density_penalty;
+            launch_gradient(b, SLOT_PREV, false);
+            sync_stats();
+            r.wl_grad_sum = (float)h_sums[b].wl_abs;
+            r.density_grad_sum = (float)h_sums[b].den_abs;
+        }
+        if (r.wl_grad_sum != 0.0f) r.density_penalty = (r.wl_grad_sum / r.density_grad_sum) * o.init_density_penalty;
+        float nia = (float)r.nia;
+        r.sum_overflow = (float)r.overflow / nia;
+        r.sum_overflow_unscaled = (float)r.overflow_unscaled / nia;
+        auto dd = distances(b, r.v[S_PREV_SLP], r.v[S_CUR_SLP], r.v[S_PREV_SLP_SUM], r.v[S_CUR_SLP_SUM]);
+        r.coordi_distance = dd.first;
+        r.step_length = r.coordi_distance / dd.second;
+        return r.step_length;
+    }
+
+    // NesterovPlace::init (nesterov.rs:2674).
+    int32_t np_init() {
+        float total_sum_overflow = 0.0f, total_base = 0.0f;
+        for (size_t b = 0; b < rg.size(); ++b) {
+            init_density1(int(b));
+            total_sum_overflow += rg[b].sum_overflow;
+            total_base += rg[b].base_wl_coef_nb;
+        }
+        float n = (float)rg.size();
+        average_overflow = total_sum_overflow / n;
+        base_wl_coef = total_base / n;
+        update_wirelength_coef(average_overflow);
+        wa_forward();
+        for (size_t b = 0; b < rg.size(); ++b) {
+            Region& r = rg[b];
+            np_update_gradients(int(b), int(b) + 1, SLOT_CUR, false);
+            // only region b's gradient is wanted here; later regions are recomputed below
+            mark(F_SCATTER);
+            gplk::launch_initial_prev(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], initial_prev_coordi_update_coef, r.v[S_PREV_SLP]);
+            gplk::launch_center_scatter(stream, r.dev, cells, r.v[S_PREV_SLP]);
+            mark_end();
+            bins_and_finish(r, int(b));
+            solve(r);
+            sync_stats();
+            r.overflow = (int64_t)h_sums[b].overflow;
+            r.overflow_unscaled = (int64_t)h_sums[b].overflow_unscaled;
+        }
+        wa_forward();
+        np_update_gradients(0, int(rg.size()), SLOT_PREV, false);
+        for (size_t b = 0; b < rg.size(); ++b) {
+            float step = init_density2(int(b));
+            if ((std::isnan(step) || std::isinf(step)) && recursion_init_slp < 10) {
+                initial_prev_coordi_update_coef *= 10.0f;
+                recursion_init_slp += 1;
+                return np_init();
+            }
+            if (std::isnan(step) || std::isinf(step)) {
+                log.error(304, strfmt("RePlAce diverged at initial iteration with steplength being %g. Re-run with a smaller init_density_penalty value.", step));
+                return 304;
+            }
+        }
+        return GPL_OK;
+    }
+
+    // nesterovUpdateCoordinates for every region, then the field.
+    void update_coordinates(float coeff) {
+        for (size_t b = 0; b < rg.size(); ++b) {
+            Region& r = rg[b];
+            if (r.is_converged) continue;
+            mark(F_SCATTER);
+            gplk::launch_update_scatter(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
+                                        r.step_length, coeff);
+            mark_end();
+            bins_and_finish(r, int(b));
+            solve(r);
+        }
+    }
+
+    // nesterovUpdateStepLength (nesterov.rs:2750) from the trial's FP64 distance sums.
+    bool update_step_length(int b) {
+        Region& r = rg[b];
+        if (r.is_converged) return true;
+        double n2 = 2.0 * (double)r.n;
+        r.coordi_distance = r.n ? (float)std::sqrt(h_sums[b].coord_sq / n2) : 0.0f;
+        float grad = r.n ? (float)std::sqrt(h_sums[b].grad_sq / n2) : 0.0f;
+        float nw = r.coordi_distance / grad;
+        if (inject_kind == 2 && inject_after >= 0 && iters_run >= inject_after) nw = NAN;
+        if (std::isnan(nw) || std::isinf(nw)) {
+            r.is_diverged = true;
+            return false;
+        }
+        if ((double)nw > (double)r.step_length * 0.95) {
+            r.step_length = nw;
+            return false;
+        }
+        if ((double)nw < 0.01) {
+            r.step_length = (float)0.01;
+            return false;
+        }
+        r.step_length = nw;
+        return true;
+    }
+
+    // doBackTracking (nesterov.rs:2776).
+    void do_back_tracking(float coeff) {
+        trials = 0;
+        for (int nbt = 0; nbt < 10; ++nbt) {
+            ++trials;
+            update_coordinates(coeff);
+            wa_forward();
+            num_region_diverged = 0;
+            np_update_gradients(0, int(rg.size()), SLOT_NEXT, true);
+            for (size_t b = 0; b < rg.size(); ++b) {
+                Region& r = rg[b];
+                if (!r.is_converged) {
+                    r.overflow = (int64_t)h_sums[b].overflow;
+                    r.overflow_unscaled = (int64_t)h_sums[b].overflow_unscaled;
+                }
+                num_region_diverged += r.is_diverged ? 1 : 0;
+            }
+            cur_hpwl = (int64_t)h_sums[rg.size()].hpwl;
+            if (num_region_diverged > 0) {
+                diverge_code = 306;
+                diverge_msg = "RePlAce diverged at wire/density gradient Sum.";
+                break;
+            }
+            size_t ok = 0;
+            num_region_diverged = 0;
+            for (size_t b = 0; b < rg.size(); ++b) {
+                ok += update_step_length(int(b)) ? 1 : 0;
+                num_region_diverged += rg[b].is_diverged ? 1 : 0;
+            }
+            if (num_region_diverged > 0) {
+                diverge_code = 305;
+                diverge_msg = "RePlAce diverged during gradient descent calculation, resulting in an invalid step length (Inf or NaN). This is often caused by numerical instability or high placement density. Consider reducing placement density to potentially resolve the issue.";
+                break;
+            }
+            if (ok != rg.size()) break;
+        }
+    }
+
+    void adjust_phi(Region& r) {
+        if (r.is_converged) return;
+        if (!r.max_phi_coef_changed && r.sum_overflow_unscaled < 0.35f) {
+            r.max_phi_coef_changed = true;
+            r.max_phi_coef = (float)((double)r.max_phi_coef * 0.99);
+        }
+        if (r.max_phi_coef <= 1.0f) r.max_phi_coef = 1.01f;
+    }
+
+    float phi_coef(const Region& r, float scaled) const {
+        float m = r.max_phi_coef;
+        float v = scaled < 0.0f ? m : (float)((double)m * std::pow((double)m, -(double)scaled));
+        return std::max(o.min_phi_coef, v);
+    }
+
+    void report_header() {
+        log.report(strfmt("%9s | %8s | %13s | %8s | %9s | %5s", "Iteration", "Overflow", "HPWL (um)", "HPWL(%)", "Penalty", "Group"));
+        log.report("---------------------------------------------------------------");
+    }
+
+    // NesterovBase::updateNextIter (nesterov.rs:2837).
+    void nb_update_next_iter(int b, int iter) {
+        Region& r = rg[b];
+        if (r.is_converged) return;
+        for (int q = 0; q < 4; ++q) std::swap(r.v[S_PREV_SLP + q], r.v[S_CUR_SLP + q]);
+        if (r.has_locked) {
+            float2* nv[5] = {r.v[S_NEXT_SLP], r.v[S_NEXT_SLP_WL], r.v[S_NEXT_SLP_DEN], r.v[S_NEXT_SLP_SUM], r.v[S_NEXT]};
+            const float2* cv[5] = {r.v[S_CUR_SLP], r.v[S_CUR_SLP_WL], r.v[S_CUR_SLP_DEN], r.v[S_CUR_SLP_SUM], r.v[S_CUR]};
+            gplk::launch_lock_fix(stream, r.dev, cells, nv, cv, 5);
+        }
+        for (int q = 0; q < 4; ++q) std::swap(r.v[S_CUR_SLP + q], r.v[S_NEXT_SLP + q]);
+        std::swap(r.v[S_CUR], r.v[S_NEXT]);
+        float fraction = (float)iter / (float)max_iter;
+        float denom = std::max((float)r.nia, fraction * (float)r.R.non_place_insts_area * 0.05f);
+        r.sum_overflow = (float)r.overflow / denom;
+        r.sum_overflow_unscaled = (float)r.overflow_unscaled / denom;
+        int64_t hpwl = cur_hpwl;
+        if (iter == 0 || iter % 10 == 0) {
+            float pct = 0.0f;
+            if (r.prev_reported_hpwl != 0) pct = (float)((double)(hpwl - r.prev_reported_hpwl) / (double)r.prev_reported_hpwl * 100.0);
+            r.prev_reported_hpwl = hpwl;
+            r.prev_reported_overflow_unscaled = r.sum_overflow_unscaled;
+            std::string group = r.R.group >= 0 ? strfmt(" (group%d)", r.R.group) : std::string();
+            if ((iter == 0 || r.reprint_header) && r.R.group < 0) {
+                if (iter == 0) log.info(31, "HPWL: Half-Perimeter Wirelength");
+                report_header();
+                r.reprint_header = false;
+            }
+            log.report(strfmt("%9d | %8.4f | %13.6e | %+7.2f%% | %9.2e | %5s", iter, r.sum_overflow_unscaled, (double)hpwl / (double)dbu, pct,
+                              (double)r.density_penalty, group.c_str()));
+        }
+        float scaled = (float)(hpwl - r.prev_hpwl) / reference_hpwl;
+        float phi = phi_coef(r, scaled);
+        r.density_penalty *= phi;
+        r.prev_hpwl = hpwl;
+        r.peak_coordi_distance = std::max(r.peak_coordi_distance, r.coordi_distance);
+        if (iter > 50 && r.min_sum_overflow > r.sum_overflow_unscaled) {
+            r.min_sum_overflow = r.sum_overflow_unscaled;
+            r.hpwl_with_min_sum_overflow = (float)r.prev_hpwl;
+        }
+    }
+
+    // NesterovPlace::updateNextIter (nesterov.rs:2907).
+    void np_update_next_iter(int iter) {
+        float total = 0.0f, total_unscaled = 0.0f;
+        for (size_t b = 0; b < rg.size(); ++b) {
+            nb_update_next_iter(int(b), iter);
+            total += rg[b].sum_overflow;
+            total_unscaled += rg[b].sum_overflow_unscaled;
+        }
+        float n = (float)rg.size();
+        average_overflow = total / n;
+        average_overflow_unscaled = total_unscaled / n;
+        update_wirelength_coef(average_overflow);
+        if (!o.disable_revert_if_diverge) {
+            int64_t hpwl = cur_hpwl;
+            if (hpwl < min_hpwl && (double)average_overflow_unscaled <= 0.25) {
+                min_hpwl = hpwl;
+                diverge_snapshot_overflow = average_overflow_unscaled;
+                diverge_snapshot_iter = iter + 1;
+                is_min_hpwl = true;
+            } else {
+                is_min_hpwl = false;
+            }
+        }
+    }
+
+    void copy_vec(float2* dst, const float2* src, int n) {
+        if (n > 0) check(hipMemcpyAsync(dst, src, sizeof(float2) * n, hipMemcpyDeviceToDevice, stream), "snapshot copy");
+    }
+
+    void save_snapshot(Region& r) {
+        if (r.is_converged) return;
+        copy_vec(r.v[S_SNAP], r.v[S_CUR], r.n);
+        copy_vec(r.v[S_SNAP_SLP], r.v[S_CUR_SLP], r.n);
+        copy_vec(r.v[S_SNAP_SLP_SUM], r.v[S_CUR_SLP_SUM], r.n);
+        copy_vec(r.v[S_SNAP_PREV_SLP_SUM], r.v[S_PREV_SLP_SUM], r.n);
+        r.snapshot_density_penalty = r.density_penalty;
+        r.snapshot_step_length = r.step_length;
+    }
+
+    void revert_to_snapshot(int b) {
+        Region& r = rg[b];
+        if (r.is_converged) return;
+        copy_vec(r.v[S_CUR], r.v[S_SNAP], r.n);
+        copy_vec(r.v[S_CUR_SLP], r.v[S_SNAP_SLP], r.n);
+        copy_vec(r.v[S_CUR_SLP_SUM], r.v[S_SNAP_SLP_SUM], r.n);
+        copy_vec(r.v[S_PREV_SLP_SUM], r.v[S_SNAP_PREV_SLP_SUM], r.n);
+        r.density_penalty = r.snapshot_density_penalty;
+        r.step_length = r.snapshot_step_length;
+        mark(F_SCATTER);
+        gplk::launch_center_scatter(stream, r.dev, cells, r.v[S_CUR]);
+        mark_end();
+        bins_and_finish(r, b);
+        solve(r);
+        sync_stats();
+        r.overflow = (int64_t)h_sums[b].overflow;
+        r.overflow_unscaled = (int64_t)h_sums[b].overflow_unscaled;
+        r.is_diverged = false;
+        r.prev_reported_hpwl = 0;
+        r.prev_reported_overflow_unscaled = 0.0f;
+    }
+
+    bool check_divergence(Region& r) {
+        if (r.sum_overflow_unscaled < 0.2f && r.sum_overflow_unscaled - r.min_sum_overflow >= 0.02f &&
+            r.hpwl_with_min_sum_overflow * 1.2f < (float)r.prev_hpwl) {
+            r.is_diverged = true;
+            log.warn(323, "Divergence detected between consecutive iterations");
+        }
+        if (r.prev_reported_overflow_unscaled > 0.0f && r.prev_reported_hpwl > 0) {
+            float change = r.sum_overflow_unscaled - r.prev_reported_overflow_unscaled;
+            float increase = (float)(r.prev_hpwl - r.prev_reported_hpwl) / (float)r.prev_reported_hpwl;
+            if (change >= 0.05f && increase >= 0.25f) {
+                r.is_diverged = true;
+                log.warn(324, strfmt("Divergence detected between reported values. Overflow change: %g, HPWL increase: %g%%.", change, increase * 100.0f));
+            }
+        }
+        return r.is_diverged;
+    }
+
+    void revert_to_diverge_snapshot() {
+        log.warn(999, strfmt("Revert to iter: %4d overflow: %.3f HPWL: %lld", diverge_snapshot_iter, diverge_snapshot_overflow, (long long)diverge_hpwl));
+        wl_coef_x = diverge_coef_x;
+        wl_coef_y = diverge_coef_y;
+        wa_forward();
+        for (size_t b = 0; b < rg.size(); ++b) {
+            revert_to_snapshot(int(b));
+            rg[b].min_sum_overflow = 1e30f;
+            rg[b].hpwl_with_min_sum_overflow = 1e30f;
+        }
+        sync_stats();
+        cur_hpwl = hpwl_now();
+    }
+
+    // isDiverged (nesterov.rs:3009), routability off: 0 none, 1 resume, 2 stop.
+    int np_is_diverged(float* cur_a) {
+        num_region_diverged = 0;
+        for (auto& r : rg) num_region_diverged += check_divergence(r) ? 1 : 0;
+        if (num_region_diverged == 0 && is_min_hpwl && !o.disable_revert_if_diverge) {
+            diverge_coef_x = wl_coef_x;
+            diverge_coef_y = wl_coef_y;
+            diverge_hpwl = min_hpwl;
+            for (auto& r : rg) save_snapshot(r);
+            diverge_saved = true;
+        }
+        if (num_region_diverged == 0) return 0;
+        log.report(strfmt("Divergence occured in %d regions.", num_region_diverged));
+        bool disable = o.disable_revert_if_diverge != 0;
+        if (!disable && diverge_saved && diverge_reverts < 3) {
+            diverge_reverts += 1;
+            log.warn(998, strfmt("Divergence detected, reverting to snapshot with min hpwl and resuming (%d of 3 reverts allowed).", diverge_reverts));
+            revert_to_diverge_snapshot();
+            *cur_a = 1.0f;
+            diverge_saved = false;
+            num_region_diverged = 0;
+            return 1;
+        }
+        if (!disable && diverge_reverts > 0) {
+            log.warn(997, "Divergence detected again, reverting to snapshot with min hpwl and stopping.");
+            revert_to_diverge_snapshot();
+            num_region_diverged = 0;
+            return 2;
+        }
+        diverge_code = 307;
+        diverge_msg = "RePlAce divergence detected: Current overflow is low and increasing relative to minimum,and the HPWL has significantly worsened. Consider re-running with a smaller max_phi_cof value.";
+        return 2;
+    }
+
+    void lock_region(int b);
+
+    // checkConvergence (nesterov.rs:3107).
+    bool check_convergence(int b, int iter) {
+        Region& r = rg[b];
+        if (r.is_converged) return true;
+        if (r.sum_overflow_unscaled > o.overflow) return false;
+        int64_t hpwl = cur_hpwl;
+        std::string group = r.R.group >= 0 ? strfmt("group%d", r.R.group) : std::string();
+        log.report(strfmt("%9d | %8.4f | %13.6e | %8s | %9.2e | %5s", iter, r.sum_overflow_unscaled, (double)hpwl / (double)dbu, "",
+                          (double)r.density_penalty, group.c_str()));
+        log.report("---------------------------------------------------------------");
+        if (r.R.group >= 0)
+            log.info(1016, strfmt("Region '%s' placement finished at iteration %d", group.c_str(), iter));
+        else
+            log.info(1001, strfmt("Global placement finished at iteration %d", iter));
+        double dbu2 = double(dbu) * double(dbu);
+        double w = double(r.R.white_space_area);
+        log.info(1002, strfmt("Placed Cell Area            %10.4f", double(r.nia) / dbu2));
+        log.info(1003, strfmt("Available Free Area         %10.4f", w / dbu2));
+        log.info(1004, strfmt("Minimum Feasible Density        %.4f (cell_area / free_area)", r.R.uniform_target_density));
+        log.info(1006, "  Suggested Target Densities:");
+        log.info(1007, strfmt("    - For 90%% usage of free space: %.4f", double(r.nia) / (w * 0.90)));
+        log.info(1008, strfmt("    - For 80%% usage of free space: %.4f", double(r.nia) / (w * 0.80)));
+        if (double(r.nia) / (w * 0.50) <= 1.0) log.info(1009, strfmt("    - For 50%% usage of free space: %.4f", double(r.nia) / (w * 0.50)));
+        if (r.R.uniform_target_density > 0.95f)
+            log.warn(1015, strfmt("High uniform density (>%.2f) may cause congestion or legalization issues.", r.R.uniform_target_density));
+        lock_region(b);
+        r.is_converged = true;
+        return true;
+    }
+
+    void push_history(int iter) {
+        gpl_iter_row row{};
+        row.iter = iter;
+        row.trials = trials;
+        row.overflow = average_overflow_unscaled;
+        row.density_penalty = rg[0].density_penalty;
+        row.step_length = rg[0].step_length;
+        row.wl_coef = wl_coef_x;
+        row.hpwl = rg[0].prev_hpwl;
+        hist.push_back(row);
+    }
+
+    // resume_nesterov_place (nesterov.rs:4440), routability and timing off.
+    int32_t resume(int32_t start_iter, int32_t* iter_out) {
+        auto t0 = Clock::now();
+        log.info(84, "---- Execute Nesterov Global Placement.");
+        if (num_region_diverged > 0) {
+            log.error(diverge_code, diverge_msg);
+            return diverge_code;
+        }
+        int64_t original_area = 0;
+        for (auto& r : rg) {
+            r.max_phi_coef_changed = false;
+            r.min_sum_overflow = 1e30f;
+            r.hpwl_with_min_sum_overflow = 1e30f;
+            r.is_converged = false;
+            original_area += r.nia;
+        }
+        diverge_saved = false;
+        diverge_reverts = 0;
+        diverge_hpwl = 0;
+        float cur_a = 1.0f;
+        int32_t iter = start_iter;
+        converged = false;
+        while (iter < max_iter) {
+            float prev_a = cur_a;
+            cur_a = (float)((1.0 + std::sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
+            float coeff = (float)(((double)prev_a - 1.0) / (double)cur_a);
+            do_back_tracking(coeff);
+            total_trials += trials;
+            for (auto& r : rg) adjust_phi(r);
+            if (num_region_diverged > 0) break;
+            np_update_next_iter(iter);
+            push_history(iter);
+            last_iter = iter;
+            ++iters_run;
+            int act = np_is_diverged(&cur_a);
+            if (act == 2) break;
+            if (act == 1) {
+                ++iter;
+                continue;
+            }
+            size_t conv = 0;
+            for (size_t b = 0; b < rg.size(); ++b) conv += check_convergence(int(b), iter) ? 1 : 0;
+            if (conv == rg.size()) {
+                converged = true;
+                t_converge = t_init + std::chrono::duration<double>(Clock::now() - t0).count();
+                break;
+            }
+            ++iter;
+        }
+        if (iter >= max_iter && !o.disable_revert_if_diverge && (diverge_saved || diverge_reverts > 0) &&
+            diverge_snapshot_overflow <= average_overflow_unscaled && diverge_hpwl < cur_hpwl) {
+            log.warn(121, "Reached the maximum number of iterations, reverting to snapshot with min hpwl.");
+            revert_to_diverge_snapshot();
+        }
+        if (iter >= max_iter)
+            log.warn(1010, strfmt("GPL reached the maximum number of iterations for nesterov %d. Placement may have failed to converge.", max_iter));
+        int64_t new_area = 0;
+        for (auto& r : rg) new_area += r.nia;
+        double dbu2 = double(dbu) * double(dbu);
+        float diff = (float)(100.0 * double(new_area - original_area) / double(original_area));
+        log.info(1014, strfmt("Final placement area: %.2f (%+.2f%%)", double(new_area) / dbu2, diff));
+        check(hipStreamSynchronize(stream), "sync");
+        t_loop = std::chrono::duration<double>(Clock::now() - t0).count();
+        if (iter_out) *iter_out = iter;
+        if (num_region_diverged > 0) {
+            log.error(diverge_code, diverge_msg);
+            return diverge_code;
+        }
+        return GPL_OK;
+    }
+};
+
+extern "C" __global__ void gpl_k_lock_region(gplk::RegionDev r, uint8_t* flags) {
+    int n = r.cell_end - r.cell_begin;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        uint8_t f = flags[r.cell_begin + k];
+        if (f & 1u) flags[r.cell_begin + k] = f | 4u;
+    }
+}
+
+void gpl_ctx::lock_region(int b) {
+    Region& r = rg[b];
+    if (r.n > 0) hipLaunchKernelGGL(gpl_k_lock_region, dim3((r.n + 255) / 256), dim3(256), 0, stream, r.dev, d_flags);
+    r.has_locked = true;
+}
+
+namespace {
+
+void make_tables(int L, std::vector<float2>& tw, std::vector<float2>& dw) {
+    const double pi = 3.14159265358979323846;
+    tw.resize(std::max(1, L / 2));
+    dw.resize(L);
+    for (int k = 0; k < L / 2; ++k) tw[k] = make_float2((float)std::cos(-2.0 * pi * k / L), (float)std::sin(-2.0 * pi * k / L));
+    for (int k = 0; k < L; ++k) dw[k] = make_float2((float)std::cos(pi * k / (2.0 * L)), (float)-std::sin(pi * k / (2.0 * L)));
+}
+
+bool pow2(int v) { return v >= 2 && (v & (v - 1)) == 0; }
+
+int32_t fail(gpl_ctx* c, int32_t code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
+    if (!p || !out) return GPL_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        fprintf(stderr, "gpl_ctx_create: no HIP device\n");
+        return GPL_ERR_HIP;
+    }
+    for (int b = 0; b < p->num_regions; ++b) {
+        const gpl_region& R = p->regions[b];
+        if (!pow2(R.cnt_x) || !pow2(R.cnt_y) || R.cnt_x > 4096 || R.cnt_y > 4096) {
+            fprintf(stderr, "gpl_ctx_create: bin grid %d x %d is not a power of two in [2, 4096]\n", R.cnt_x, R.cnt_y);
+            return GPL_ERR_REFUSED;
+        }
+    }
+    gpl_ctx* c = new gpl_ctx();
+    try {
+        check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
+        c->o = p->opts;
+        c->max_iter = p->opts.max_iter;
+        c->dbu = p->dbu;
+        c->site_x = p->site_x;
+        c->ncells = p->num_cells;
+        c->npins = p->num_pins;
+        c->nnets = p->num_nets;
+        const int32_t N = p->num_cells, E = p->num_nets;
+        // nets sorted by degree (stable); high-fanout nets last
+        std::vector<int32_t> order(E);
+        std::iota(order.begin(), order.end(), 0);
+        auto deg = [&](int32_t e) { return p->net_pin_start[e + 1] - p->net_pin_start[e]; };
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return deg(a) < deg(b); });
+        c->net_orig_of = order;
+        c->net_sorted_of.assign(E, 0);
+        for (int32_t i = 0; i < E; ++i) c->net_sorted_of[order[i]] = i;
+        const int32_t kBig = 128;
+        int32_t num_small = 0;
+        while (num_small < E && deg(order[num_small]) <= kBig) ++num_small;
+        std::vector<int32_t> nstart(E + 1, 0), pcell, np_net;
+        std::vector<int2> poff, pxy0;
+        std::vector<float> nweight(E);
+        for (int32_t i = 0; i < E; ++i) {
+            int32_t e = order[i];
+            nweight[i] = p->net_weight[e];
+            for (int32_t k = p->net_pin_start[e]; k < p->net_pin_start[e + 1]; ++k) {
+                int32_t q = p->net_pins[k];
+                pcell.push_back(p->pin_cell[q]);
+                poff.push_back(make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]));
+                pxy0.push_back(make_int2(p->pin_xy[2 * q], p->pin_xy[2 * q + 1]));
+            }
+            nstart[i + 1] = int32_t(pcell.size());
+        }
+        c->nets.num_nets = E;
+        c->nets.num_small = num_small;
+        c->nets.start = c->upload(nstart.data(), nstart.size());
+        c->nets.pcell = c->upload(pcell.data(), pcell.size());
+        c->nets.poff = c->upload(poff.data(), poff.size());
+        c->nets.pxy0 = c->upload(pxy0.data(), pxy0.size());
+        c->nets.weight = c->upload(nweight.data(), nweight.size());
+        c->nets.box = c->dalloc<int4>(E);
+        c->nets.sx = c->dalloc<float4>(E);
+        c->nets.sy = c->dalloc<float4>(E);
+        // cells
+        std::vector<int4> dbox(N);
+        std::vector<float2> pre(N);
+        std::vector<int32_t> cp_net(p->cell_pin_start[N]);
+        std::vector<int2> cp_off(p->cell_pin_start[N]);
+        for (int32_t g = 0; g < N; ++g) {
+            const int32_t* d = &p->cell_dbox[4 * g];
+            dbox[g] = make_int4(d[0], d[1], d[2], d[3]);
+            const int32_t* bx = &p->cell_box[4 * g];
+            pre[g] = make_float2((float)(p->cell_pin_start[g + 1] - p->cell_pin_start[g]), (float)(bx[2] - bx[0]) * (float)(bx[3] - bx[1]));
+            for (int32_t k = p->cell_pin_start[g]; k < p->cell_pin_start[g + 1]; ++k) {
+                int32_t q = p->cell_pins[k];
+                cp_net[k] = c->net_sorted_of[p->pin_net[q]];
+                cp_off[k] = make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]);
+            }
+        }
+        c->h_flags.assign(p->cell_flags, p->cell_flags + N);
+        c->cells.dbox = c->upload(dbox.data(), N);
+        c->cells.dscale = c->upload(p->density_scale, N);
+        c->d_flags = c->upload(p->cell_flags, N);
+        c->cells.flags = c->d_flags;
+        c->cells.centered = c->dalloc<uint8_t>(N);
+        check(hipMemsetAsync(c->cells.centered, 0, N, c->stream), "memset");
+        c->cells.pin_start = c->upload(p->cell_pin_start, size_t(N) + 1);
+        c->cells.cp_net = c->upload(cp_net.data(), cp_net.size());
+        c->cells.cp_off = c->upload(cp_off.data(), cp_off.size());
+        c->cells.pre = c->upload(pre.data(), N);
+        c->cell_inst.assign(p->cell_inst, p->cell_inst + N);
+        c->inst_box.assign(p->inst_box, p->inst_box + 4 * size_t(p->num_insts));
+        // regions
+        c->rg.resize(p->num_regions);
+        for (int32_t b = 0; b < p->num_regions; ++b) {
+            Region& r = c->rg[b];
+            r.R = p->regions[b];
+            r.n = r.R.cell_end - r.R.cell_begin;
+            r.max_phi_coef = c->o.max_phi_coef;
+            r.nia = c->nesterov_insts_area(r);
+            for (int32_t g = r.R.cell_begin; g < r.R.cell_end; ++g)
+                if (p->cell_flags[g] & GPL_CELL_LOCKED) r.has_locked = true;
+            gplk::RegionDev& d = r.dev;
+            d.lx = r.R.lx;
+            d.ly = r.R.ly;
+            d.ux = r.R.ux;
+            d.uy = r.R.uy;
+            d.cnt_x = r.R.cnt_x;
+            d.cnt_y = r.R.cnt_y;
+            d.size_x = r.R.size_x;
+            d.size_y = r.R.size_y;
+            d.target_density = r.R.target_density;
+            d.cell_begin = r.R.cell_begin;
+            d.cell_end = r.R.cell_end;
+            size_t M = size_t(r.R.cnt_x) * r.R.cnt_y;
+            d.bin_x = c->upload(r.R.bin_x, size_t(r.R.cnt_x) + 1);
+            d.bin_y = c->upload(r.R.bin_y, size_t(r.R.cnt_y) + 1);
+            d.inst = c->dalloc<unsigned long long>(M);
+            d.filler = c->dalloc<unsigned long long>(M);
+            d.non_place = (const long long*)c->upload(r.R.non_place, M);
+            d.non_place_unscaled = (const long long*)c->upload(r.R.non_place_unscaled, M);
+            d.density = c->dalloc<float>(M);
+            d.field_x = c->dalloc<float>(M);
+            d.field_y = c->dalloc<float>(M);
+            d.phi = c->dalloc<float>(M);
+            d.work0 = c->dalloc<float>(M);
+            d.work1 = c->dalloc<float>(M);
+            d.work2 = c->dalloc<float>(M);
+            d.work3 = c->dalloc<float>(M);
+            d.wx = c->upload(r.R.wx, r.R.cnt_x);
+            d.wy = c->upload(r.R.wy, r.R.cnt_y);
+            std::vector<float2> tw, dw;
+            make_tables(r.R.cnt_x, tw, dw);
+            d.tw_x = c->upload(tw.data(), tw.size());
+            d.dw_x = c->upload(dw.data(), dw.size());
+            make_tables(r.R.cnt_y, tw, dw);
+            d.tw_y = c->upload(tw.data(), tw.size());
+            d.dw_y = c->upload(dw.data(), dw.size());
+            for (int s = 0; s < S_COUNT; ++s) {
+                r.v[s] = c->dalloc<float2>(r.n);
+                check(hipMemsetAsync(r.v[s], 0, sizeof(float2) * std::max(1, r.n), c->stream), "memset");
+            }
+            r.partials = c->dalloc<double>(6 * gplk::kPartials);
+            check(hipMemsetAsync(r.partials, 0, sizeof(double) * 6 * gplk::kPartials, c->stream), "memset");
+            gplk::launch_zero_bins(c->stream, d);
+        }
+        // one Sums per region, then one whose hpwl field is the HPWL counter the WA kernels add into
+        c->d_sums = c->dalloc<gplk::Sums>(size_t(p->num_regions) + 1);
+        check(hipMemsetAsync(c->d_sums, 0, sizeof(gplk::Sums) * (size_t(p->num_regions) + 1), c->stream), "memset");
+        check(hipHostMalloc((void**)&c->h_sums, sizeof(gplk::Sums) * (size_t(p->num_regions) + 1), hipHostMallocDefault), "pinned");
+        c->d_hpwl = &c->d_sums[p->num_regions].hpwl;
+        check(hipStreamSynchronize(c->stream), "sync");
+    } catch (const HipError& e) {
+        fprintf(stderr, "gpl_ctx_create: %s: %s\n", e.what, hipGetErrorString(e.e));
+        delete c;
+        return GPL_ERR_HIP;
+    }
+    *out = c;
+    return GPL_OK;
+}
+
+void gpl_ctx_destroy(gpl_ctx* c) { delete c; }
+
+const char* gpl_last_error(const gpl_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int32_t gpl_init_nesterov_loop(gpl_ctx* c) {
+    if (!c) return GPL_ERR_ARG;
+    try {
+        auto t0 = Clock::now();
+        int64_t design_hpwl = c->hpwl_now();
+        float user = c->o.reference_hpwl;
+        c->reference_hpwl = user > 0.0f ? user : std::max(446000000.0f, 0.01f * (float)design_hpwl);
+        int32_t rc = c->np_init();
+        c->initialized = true;
+        check(hipStreamSynchronize(c->stream), "sync");
+        c->t_init = std::chrono::duration<double>(Clock::now() - t0).count();
+        if (rc != GPL_OK) {
+            c->num_region_diverged = 1;
+            c->diverge_code = rc;
+        }
+        return rc;
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+}
+
+int32_t gpl_run_nesterov_loop(gpl_ctx* c, int32_t start_iter, int32_t* iter_out) {
+    if (!c) return GPL_ERR_ARG;
+    if (!c->initialized) return fail(c, GPL_ERR_STATE, "gpl_init_nesterov_loop first");
+    try {
+        return c->resume(start_iter, iter_out);
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+}
+
+int32_t gpl_do_nesterov_place(gpl_ctx* c, int32_t* iter_out) {
+    int32_t rc = gpl_init_nesterov_loop(c);
+    if (rc < 0) return rc;
+    return gpl_run_nesterov_loop(c, 0, iter_out);
+}
+
+int32_t gpl_continue_nesterov_place(gpl_ctx* c, float overflow, int32_t max_iter, int32_t start_iter, int32_t* iter_out) {
+    if (!c) return GPL_ERR_ARG;
+    c->o.overflow = overflow;
+    c->max_iter = max_iter;
+    return gpl_run_nesterov_loop(c, start_iter, iter_out);
+}
+
+int32_t gpl_get_stats(gpl_ctx* c, gpl_stats* s) {
+    if (!c || !s) return GPL_ERR_ARG;
+    std::memset(s, 0, sizeof *s);
+    s->iter = c->last_iter;
+    s->converged = c->converged ? 1 : 0;
+    s->status = c->num_region_diverged > 0 ? c->diverge_code : 0;
+    s->total_trials = c->total_trials;
+    s->hpwl = c->cur_hpwl;
+    s->overflow = c->average_overflow_unscaled;
+    s->overflow_scaled = c->average_overflow;
+    s->density_penalty = c->rg[0].density_penalty;
+    s->step_length = c->rg[0].step_length;
+    s->wl_coef = c->wl_coef_x;
+    s->base_wl_coef = c->base_wl_coef;
+    s->reference_hpwl = c->reference_hpwl;
+    s->wl_grad_sum = c->rg[0].wl_grad_sum;
+    s->density_grad_sum = c->rg[0].density_grad_sum;
+    s->num_diverge_reverts = c->diverge_reverts;
+    s->init_seconds = c->t_init;
+    s->loop_seconds = c->t_loop;
+    s->first_below_target_seconds = c->t_converge;
+    return GPL_OK;
+}
+
+int32_t gpl_get_history(gpl_ctx* c, gpl_iter_row* rows, int32_t cap) {
+    if (!c) return GPL_ERR_ARG;
+    int32_t n = std::min<int32_t>(cap, int32_t(c->hist.size()));
+    if (rows && n > 0) std::memcpy(rows, c->hist.data(), sizeof(gpl_iter_row) * n);
+    return int32_t(c->hist.size());
+}
+
+int32_t gpl_get_log(gpl_ctx* c, char* buf, int32_t cap) {
+    if (!c) return GPL_ERR_ARG;
+    std::string s = c->log.joined();
+    if (buf && cap > 0) {
+        size_t n = std::min<size_t>(size_t(cap) - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return int32_t(s.size());
+}
+
+int32_t gpl_placed_locations(gpl_ctx* c, int32_t* out, int32_t cap) {
+    if (!c) return GPL_ERR_ARG;
+    try {
+        std::vector<int4> dbox(c->ncells);
+        check(hipMemcpyAsync(dbox.data(), c->cells.dbox, sizeof(int4) * c->ncells, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+        int32_t n = 0;
+        for (int32_t g = 0; g < c->ncells; ++g) {
+            int32_t k = c->cell_inst[g];
+            if (k < 0) continue;
+            if (out && n < cap) {
+                const int32_t* b = &c->inst_box[4 * size_t(k)];
+                int32_t dcx = (dbox[g].x + dbox[g].z) / 2, dcy = (dbox[g].y + dbox[g].w) / 2;
+                out[3 * n] = k;
+                out[3 * n + 1] = dcx - (b[2] - b[0]) / 2 + c->site_x * c->o.pad_left;
+                out[3 * n + 2] = dcy - (b[3] - b[1]) / 2;
+            }
+            ++n;
+        }
+        return n;
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+}
+
+// ---- kernel-level entries ---------------------------------------------------------------------
+
+static float2* slot_ptr(gpl_ctx* c, int region, int slot) {
+    Region& r = c->rg[region];
+    switch (slot) {
+    case GPL_SLOT_CUR_SLP: return r.v[S_CUR_SLP];
+    case GPL_SLOT_NEXT_SLP: return r.v[S_NEXT_SLP];
+    case GPL_SLOT_PREV_SLP: return r.v[S_PREV_SLP];
+    case GPL_SLOT_CUR: return r.v[S_CUR];
+    case GPL_SLOT_NEXT: return r.v[S_NEXT];
+    default: return nullptr;
+    }
+}
+
+static bool bad_region(gpl_ctx* c, int32_t region) { return !c || region < 0 || region >= (int32_t)c->rg.size(); }
+
+int32_t gpl_set_coordinates(gpl_ctx* c, int32_t region, int32_t slot, const float* xy) {
+    if (bad_region(c, region) || !xy) return GPL_ERR_ARG;
+    float2* d = slot_ptr(c, region, slot);
+    if (!d) return GPL_ERR_ARG;
+    try {
+        int n = c->rg[region].n;
+        check(hipMemcpyAsync(d, xy, sizeof(float2) * n, hipMemcpyHostToDevice, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+int32_t gpl_get_coordinates(gpl_ctx* c, int32_t region, int32_t slot, float* xy) {
+    if (bad_region(c, region) || !xy) return GPL_ERR_ARG;
+    float2* d = slot_ptr(c, region, slot);
+    if (!d) return GPL_ERR_ARG;
+    try {
+        int n = c->rg[region].n;
+        check(hipMemcpyAsync(xy, d, sizeof(float2) * n, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+int32_t gpl_density_scatter(gpl_ctx* c, int32_t region, int32_t slot, int64_t* inst_area, int64_t* filler_area, float* density,
+                            double* overflow_f64) {
+    if (bad_region(c, region)) return GPL_ERR_ARG;
+    float2* xy = slot_ptr(c, region, slot);
+    if (!xy) return GPL_ERR_ARG;
+    try {
+        Region& r = c->rg[region];
+        size_t M = size_t(r.R.cnt_x) * r.R.cnt_y;
+        gplk::launch_center_scatter(c->stream, r.dev, c->cells, xy);
+        if (inst_area) check(hipMemcpyAsync(inst_area, r.dev.inst, 8 * M, hipMemcpyDeviceToHost, c->stream), "copy");
+        if (filler_area) check(hipMemcpyAsync(filler_area, r.dev.filler, 8 * M, hipMemcpyDeviceToHost, c->stream), "copy");
+        c->bins_and_finish(r, region);
+        if (density) check(hipMemcpyAsync(density, r.dev.density, 4 * M, hipMemcpyDeviceToHost, c->stream), "copy");
+        c->sync_stats();
+        r.overflow = (int64_t)c->h_sums[region].overflow;
+        r.overflow_unscaled = (int64_t)c->h_sums[region].overflow_unscaled;
+        if (overflow_f64) {
+            overflow_f64[0] = c->h_sums[region].overflow;
+            overflow_f64[1] = c->h_sums[region].overflow_unscaled;
+        }
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+int32_t gpl_set_density(gpl_ctx* c, int32_t region, const float* density) {
+    if (bad_region(c, region) || !density) return GPL_ERR_ARG;
+    try {
+        Region& r = c->rg[region];
+        size_t M = size_t(r.R.cnt_x) * r.R.cnt_y;
+        check(hipMemcpyAsync(r.dev.density, density, 4 * M, hipMemcpyHostToDevice, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+int32_t gpl_density_solve(gpl_ctx* c, int32_t region, float* phi, float* field_x, float* field_y) {
+    if (bad_region(c, region)) return GPL_ERR_ARG;
+    try {
+        Region& r = c->rg[region];
+        size_t M = size_t(r.R.cnt_x) * r.R.cnt_y;
+        c->solve(r, phi != nullptr);
+        if (phi) check(hipMemcpyAsync(phi, r.dev.phi, 4 * M, hipMemcpyDeviceToHost, c->stream), "copy");
+        if (field_x) check(hipMemcpyAsync(field_x, r.dev.field_x, 4 * M, hipMemcpyDeviceToHost, c->stream), "copy");
+        if (field_y) check(hipMemcpyAsync(field_y, r.dev.field_y, 4 * M, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+        c->collect();
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+int32_t gpl_wa_forward(gpl_ctx* c, float wl_coef_x, float wl_coef_y, float* net_sums, int32_t* net_box, int64_t* hpwl) {
+    if (!c) return GPL_ERR_ARG;
+    try {
+        c->wl_coef_x = wl_coef_x;
+        c->wl_coef_y = wl_coef_y;
+        c->wa_forward();
+        const int32_t E = c->nnets;
+        std::vector<float4> sx(E), sy(E);
+        std::vector<int4> box(E);
+        unsigned long long h = 0;
+        check(hipMemcpyAsync(sx.data(), c->nets.sx, sizeof(float4) * E, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipMemcpyAsync(sy.data(), c->nets.sy, sizeof(float4) * E, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipMemcpyAsync(box.data(), c->nets.box, sizeof(int4) * E, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipMemcpyAsync(&h, c->d_hpwl, sizeof h, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+        c->collect();
+        for (int32_t e = 0; e < E; ++e) {
+            int32_t i = c->net_sorted_of[e];
+            if (net_sums) {
+                float* s = &net_sums[8 * size_t(e)];
+                s[0] = sx[i].x;
+                s[1] = sx[i].y;
+                s[2] = sx[i].z;
+                s[3] = sx[i].w;
+                s[4] = sy[i].x;
+                s[5] = sy[i].y;
+                s[6] = sy[i].z;
+                s[7] = sy[i].w;
+            }
+            if (net_box) {
+                int32_t* b = &net_box[4 * size_t(e)];
+                b[0] = box[i].x;
+                b[1] = box[i].y;
+                b[2] = box[i].z;
+                b[3] = box[i].w;
+            }
+        }
+        if (hpwl) *hpwl = (int64_t)h;
+        c->cur_hpwl = (int64_t)h;
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+// The gradient kernel into scratch slots (the next slp slots), read back.
+static int32_t gradients_out(gpl_ctx* c, int32_t region, float penalty, float* wl, float* den, float* sum, double* abs_sums, int slot) {
+    if (bad_region(c, region)) return GPL_ERR_ARG;
+    try {
+        Region& r = c->rg[region];
+        float saved = r.density_penalty;
+        r.density_penalty = penalty;
+        c->launch_gradient(region, slot, false);
+        r.density_penalty = saved;
+        int base = slot == SLOT_PREV ? S_PREV_SLP : slot == SLOT_CUR ? S_CUR_SLP : S_NEXT_SLP;
+        if (wl) check(hipMemcpyAsync(wl, r.v[base + 1], sizeof(float2) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
+        if (den) check(hipMemcpyAsync(den, r.v[base + 2], sizeof(float2) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
+        if (sum) check(hipMemcpyAsync(sum, r.v[base + 3], sizeof(float2) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
+        c->sync_stats();
+        if (abs_sums) {
+            abs_sums[0] = c->h_sums[region].wl_abs;
+            abs_sums[1] = c->h_sums[region].den_abs;
+        }
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+int32_t gpl_wa_backward(gpl_ctx* c, int32_t region, float* wl_grad) { return gradients_out(c, region, 0.0f, wl_grad, nullptr, nullptr, nullptr, SLOT_NEXT); }
+
+int32_t gpl_density_gather(gpl_ctx* c, int32_t region, float* den_grad) {
+    return gradients_out(c, region, 0.0f, nullptr, den_grad, nullptr, nullptr, SLOT_NEXT);
+}
+
+int32_t gpl_update_gradients(gpl_ctx* c, int32_t region, int32_t slot, float density_penalty, float* wl, float* den, float* sum,
+                             double* abs_sums) {
+    int s = slot == GPL_SLOT_PREV_SLP ? SLOT_PREV : slot == GPL_SLOT_CUR_SLP ? SLOT_CUR : SLOT_NEXT;
+    return gradients_out(c, region, density_penalty, wl, den, sum, abs_sums, s);
+}
+
+int32_t gpl_nesterov_step(gpl_ctx* c, int32_t region, float step, float coeff, float* next, float* next_slp) {
+    if (bad_region(c, region)) return GPL_ERR_ARG;
+    try {
+        Region& r = c->rg[region];
+        gplk::launch_update_scatter(c->stream, r.dev, c->cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
+                                    step, coeff);
+        c->bins_and_finish(r, region);
+        if (next) check(hipMemcpyAsync(next, r.v[S_NEXT], sizeof(float2) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
+        if (next_slp) check(hipMemcpyAsync(next_slp, r.v[S_NEXT_SLP], sizeof(float2) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
+        c->sync_stats();
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter) {
+    if (!c || kind < 0 || kind > 2) return GPL_ERR_ARG;
+    c->inject_kind = kind;
+    c->inject_after = kind == 0 ? -1 : after_iter;
+    return GPL_OK;
+}
+
+int32_t gpl_set_profiling(gpl_ctx* c, int32_t on) {
+    if (!c) return GPL_ERR_ARG;
+    if (on && !c->profiling) {
+        for (int i = 0; i < 64; ++i) hipEventCreate(&c->ev[i]);
+    }
+    if (!on && c->profiling) {
+        hipStreamSynchronize(c->stream);
+        for (int i = 0; i < 64; ++i) hipEventDestroy(c->ev[i]);
+    }
+    c->profiling = on != 0;
+    c->nev = 0;
+    for (int f = 0; f < F_COUNT; ++f) {
+        c->fam_ms[f] = 0;
+        c->fam_n[f] = 0;
+    }
+    return GPL_OK;
+}
+
+int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap) {
+    if (!c) return GPL_ERR_ARG;
+    int32_t n = 0;
+    for (int f = 0; f < F_COUNT && n < cap; ++f, ++n) {
+        std::memset(&out[n], 0, sizeof out[n]);
+        std::snprintf(out[n].name, sizeof out[n].name, "%s", kFamName[f]);
+        out[n].launches = c->fam_n[f];
+        out[n].ms = c->fam_ms[f];
+    }
+    return n;
+}
+
+int32_t gpl_objective_forward(gpl_ctx*, const float*, float, float, double*, double*, double*) { return GPL_ERR_REFUSED; }
+int32_t gpl_objective_backward(gpl_ctx*, const float*, float, float, float*) { return GPL_ERR_REFUSED; }
+
+}  // extern "C"

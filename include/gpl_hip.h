@@ -1,0 +1,370 @@
+/*
+ * gpl_hip.h -- C ABI of the MI355X ePlace Nesterov engine (libgpl_hip.so).
+ *
+ * Plain C, no C++ or torch types across the boundary, so a Rust host binds it with
+ * `extern "C"` unchanged.  Every entry point returns an int32 status:
+ *   0                 GPL_OK
+ *   negative          API (-1 argument, -3 state, -4 refused) or HIP (-2) error
+ *   positive          the reference's GPL message id the run stopped with
+ *                     (304, 305, 306, 307: the divergence codes of nesterov.rs:2525, 2788,
+ *                      2798, 2707, 3048)
+ * The text of the last error of a handle is read with gpl_last_error().
+ *
+ * Ownership: device memory belongs to the context; host pointers passed in are borrowed for the
+ * duration of the call only.  One context per host thread; each context owns one HIP stream.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the reference checkout):
+ *   gpl_design_*            main.rs:44-214 (the ODB reader) -> a seeded synthetic generator,
+ *                           placer_base.rs:301-648, initial_place.rs:420-446
+ *   gpl_init_nesterov_place nesterov.rs:787  init_nesterov_place
+ *   gpl_init_nesterov_loop  nesterov.rs:4123 init_nesterov_loop (reference HPWL + np_init)
+ *   gpl_run_nesterov_loop   nesterov.rs:4131 run_nesterov_loop
+ *   gpl_do_nesterov_place   nesterov.rs:4115 do_nesterov_place
+ *   gpl_continue_nesterov_place nesterov.rs:4243 continue_nesterov_place
+ *   gpl_placed_locations    nesterov.rs:4099 placed_locations
+ *   kernel-level entries    the per-function loops of nesterov.rs:2139-2507, 2718-2772
+ */
+#ifndef GPL_HIP_H
+#define GPL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPL_OK 0
+#define GPL_ERR_ARG (-1)
+#define GPL_ERR_HIP (-2)
+#define GPL_ERR_STATE (-3)
+#define GPL_ERR_REFUSED (-4)
+
+/* Cell flags (gpl_problem.cell_flags). */
+#define GPL_CELL_INSTANCE 1u /* GCell::is_instance (nesterov.rs:161); a filler otherwise */
+#define GPL_CELL_MACRO 2u    /* GCell::macro_ (nesterov.rs:116) */
+#define GPL_CELL_LOCKED 4u   /* GCell::locked (nesterov.rs:127) */
+
+/* ------------------------------------------------------------------------------------------ */
+/* Options                                                                                    */
+/* ------------------------------------------------------------------------------------------ */
+
+/* `Opts` (nesterov.rs:35-82), `IpVars` (initial_place.rs:19-32) and the construction `Vars`
+ * (placer_base.rs:209-216), with the reference's defaults from gpl_opts_default(). */
+typedef struct gpl_opts {
+    float density;                 /* -density, 0.7 */
+    int32_t uniform_density;       /* -density uniform */
+    int32_t bin_cnt;               /* -bin_grid_count; 0: derived per region (nesterov.rs:1847) */
+    float init_density_penalty;    /* 8e-5 */
+    float init_wirelength_coef;    /* 0.25 */
+    float min_phi_coef;            /* 0.95 */
+    float max_phi_coef;            /* 1.05 */
+    float overflow;                /* target overflow, 0.1 */
+    float reference_hpwl;          /* 0: derived (nesterov.rs:96) */
+    int32_t max_iter;              /* 5000 */
+    int32_t seed;                  /* -random_seed, 1 */
+    float perturb_dist;            /* -perturb_dist, -1: min(0.5 um, row) */
+    int32_t disable_revert_if_diverge;
+    int32_t pad_left;
+    int32_t pad_right;
+    int32_t skip_io;
+    int32_t disable_pin_density_adjust;
+    int32_t initial_place_max_iter;   /* 20 */
+    int32_t initial_place_max_fanout; /* 200 */
+    int32_t initial_place_min_diff_length; /* 1500 */
+    int32_t initial_place_max_solver_iter; /* 100 */
+    float initial_place_net_weight_scale;  /* 800 */
+    int32_t skip_initial_place;
+} gpl_opts;
+
+void gpl_opts_default(gpl_opts* o);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Synthetic design (replaces the ODB reader; no database is available)                      */
+/* ------------------------------------------------------------------------------------------ */
+
+/* A seeded netlist generator: movable standard cells one row tall, fixed IO ports on the die
+ * perimeter, nets of geometric degree, optional high-fanout nets, fixed macros and region
+ * groups.  Every field is an input; 0 picks the documented default. */
+typedef struct gpl_gen_config {
+    int32_t num_cells;        /* movable standard cells (outside region groups) */
+    int32_t num_nets;         /* nets of degree 2 + Geom(degree_p), capped */
+    double degree_p;          /* geometric success probability */
+    int32_t degree_cap;
+    int32_t num_hf_nets;      /* extra high-fanout nets */
+    int32_t hf_min_pins;
+    int32_t hf_max_pins;
+    int32_t num_io;           /* fixed ports, evenly on the die perimeter */
+    double utilization;       /* movable area / free core area */
+    double macro_area_frac;   /* fraction of the core covered by fixed macros */
+    int32_t macro_rows_min;
+    int32_t macro_rows_max;
+    int32_t row_height;       /* DBU, 2000 */
+    int32_t site_width;       /* DBU, 200 */
+    int32_t width_min_sites;  /* 1 */
+    int32_t width_max_sites;  /* 8 */
+    int32_t core_rows;        /* 0: square core from the utilization */
+    int32_t core_sites;
+    int32_t dbu;              /* DBU per micron, 1000 */
+    int32_t locality_window;  /* a net's cells come from a window of this many cell indices */
+    uint32_t seed;
+    /* Region groups (all zero for none): group g has group_cells[g] movable cells in its own
+     * rectangular region; regions are laid out left to right along the bottom of the core. */
+    int32_t num_groups;
+    const int32_t* group_cells;
+} gpl_gen_config;
+
+typedef struct gpl_design gpl_design;
+
+/* Builds the design records, the placer (placer_base_common + one placer_base per region,
+ * placer_base.rs:301, 430) and the placer pins (initial_place.rs:100). */
+int32_t gpl_design_generate(const gpl_gen_config* cfg, const gpl_opts* o, gpl_design** out);
+void gpl_design_destroy(gpl_design* d);
+
+/* B2B + BiCGSTAB initial placement (initial_place.rs:420 do_bicgstab_place), host side. */
+int32_t gpl_design_initial_place(gpl_design* d);
+
+/* Design statistics for reports: counts and areas. */
+typedef struct gpl_design_info {
+    int32_t dbu;
+    int32_t die[4];
+    int32_t core[4];
+    int32_t num_insts;        /* placer instances (movable + fixed) */
+    int32_t num_movable;
+    int32_t num_fixed;
+    int32_t num_nets;
+    int32_t num_pins;
+    int32_t num_regions;
+    int64_t movable_area;
+    int64_t fixed_area;
+    int64_t initial_hpwl;     /* initial_place.rs:381 hpwl of the database boxes */
+    int32_t ip_rounds;        /* initial-placement rounds run */
+    float ip_error;           /* the last round's larger residual */
+} gpl_design_info;
+int32_t gpl_design_get_info(const gpl_design* d, gpl_design_info* out);
+
+/* Host helpers the construction uses, exported so their rules can be pinned by the reference's
+ * known-answer tests: mt19937 outputs (normal.rs:38-120), boost uniform_int draws
+ * (normal.rs:127; n draws from one engine, draw i in [lo[i], hi[i]]), the x87 wavenumbers (nesterov.rs:351, 359), the fixed-area union
+ * (nesterov.rs:1901) and roundDownToPowerOfTwo (nesterov.rs:771).  `rects` holds 4 int32 per
+ * rectangle (lx, ly, ux, uy). */
+void gpl_util_mt19937(uint32_t seed, int32_t n, uint32_t* out);
+void gpl_util_uniform_int(uint32_t seed, int32_t n, const int32_t* lo, const int32_t* hi, int32_t* out);
+float gpl_util_wavenumber_x(int32_t i, int32_t nx);
+float gpl_util_wavenumber_y(int32_t i, int32_t ny, float bin_x, float bin_y);
+int64_t gpl_util_union_area(const int32_t* rects, int32_t n);
+uint32_t gpl_util_round_down_pow2(uint32_t x);
+
+/* ------------------------------------------------------------------------------------------ */
+/* The Nesterov placer problem: the state init_nesterov_place (nesterov.rs:787) leaves        */
+/* ------------------------------------------------------------------------------------------ */
+
+/* One region (`Nb`, nesterov.rs:451): its bin grid, its cells (a contiguous range of the
+ * problem's cells: its instances in placer order, then its fillers) and its fixed area. */
+typedef struct gpl_region {
+    int32_t lx, ly, ux, uy;          /* bin grid box (BinGrid, nesterov.rs:260) */
+    int32_t cnt_x, cnt_y;
+    double size_x, size_y;
+    float target_density;
+    float uniform_target_density;
+    int32_t cell_begin, cell_end;    /* [begin, end) into the problem's cells */
+    int32_t num_fillers;             /* the last num_fillers cells of the range */
+    int32_t filler_dx, filler_dy;
+    int64_t std_insts_area;
+    int64_t macro_insts_area;
+    int64_t non_place_insts_area;
+    int64_t white_space_area;
+    int64_t movable_area;
+    int64_t total_filler_area;
+    const int64_t* non_place;          /* cnt_x*cnt_y, bin k = y*cnt_x + x */
+    const int64_t* non_place_unscaled; /* cnt_x*cnt_y */
+    const int32_t* bin_x;              /* cnt_x + 1 bin edges along x (Bin::lx/ux) */
+    const int32_t* bin_y;              /* cnt_y + 1 bin edges along y */
+    const float* wx;                   /* cnt_x wavenumbers (nesterov.rs:351, x87 extended) */
+    const float* wy;                   /* cnt_y wavenumbers (nesterov.rs:359) */
+    int32_t group;                     /* -1: top level */
+} gpl_region;
+
+typedef struct gpl_problem {
+    int32_t num_cells;
+    const int32_t* cell_box;      /* 4 per cell: lx, ly, ux, uy (GCell box) */
+    const int32_t* cell_dbox;     /* 4 per cell: dlx, dly, dux, duy (density box) */
+    const float* density_scale;   /* per cell */
+    const uint8_t* cell_flags;    /* GPL_CELL_* */
+    const int32_t* cell_inst;     /* placer instance of an instance cell, -1 for a filler */
+    const int32_t* cell_pin_start;/* num_cells + 1: the cell's pins in its own pin order */
+    const int32_t* cell_pins;
+    int32_t num_pins;
+    const int32_t* pin_cell;      /* -1: no cell (a port) */
+    const int32_t* pin_offset;    /* 2 per pin: off_x, off_y */
+    const int32_t* pin_xy;        /* 2 per pin: cx, cy as init leaves them */
+    const int32_t* pin_net;
+    int32_t num_nets;
+    const int32_t* net_pin_start; /* num_nets + 1: the net's pins in net order */
+    const int32_t* net_pins;
+    const float* net_weight;      /* timing x custom weight (nesterov.rs:222), 1 */
+    int32_t num_regions;
+    const gpl_region* regions;
+    gpl_opts opts;
+    int32_t dbu;
+    int32_t site_x;
+    int32_t site_y;
+    const int32_t* inst_box;      /* 4 per placer instance: the instance box (for placed_locations) */
+    int32_t num_insts;
+} gpl_problem;
+
+/* Builds the Nesterov placer on the host (nesterov.rs:787 init_nesterov_place: cells, pins,
+ * nets; per region fillers, perturbation, bin grid, fixed area, FFT wavenumbers, density
+ * sizes).  The design keeps the arrays; gpl_design_problem returns a view of them valid until
+ * the design is destroyed or re-initialised. */
+int32_t gpl_init_nesterov_place(gpl_design* d, const gpl_opts* o);
+int32_t gpl_design_problem(const gpl_design* d, gpl_problem* view);
+
+/* ------------------------------------------------------------------------------------------ */
+/* GPU context                                                                                */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct gpl_ctx gpl_ctx;
+
+/* Uploads a problem to device 0 (or the current device) and allocates the per-cell, per-pin,
+ * per-net and per-bin state.  The problem's arrays are copied; they may be freed afterwards. */
+int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out);
+void gpl_ctx_destroy(gpl_ctx* c);
+const char* gpl_last_error(const gpl_ctx* c);
+
+/* Loop statistics (NesterovPlace's scalars plus per-region ones of region 0). */
+typedef struct gpl_stats {
+    int32_t iter;                  /* last iteration run */
+    int32_t converged;
+    int32_t status;                /* GPL_OK or a positive GPL code */
+    int32_t total_trials;          /* back-tracking trials summed over iterations */
+    int64_t hpwl;                  /* pin HPWL (nesterov.rs:2297) */
+    float overflow;                /* average sum_overflow_unscaled */
+    float overflow_scaled;         /* average sum_overflow */
+    float density_penalty;         /* region 0 */
+    float step_length;             /* region 0 */
+    float wl_coef;                 /* wl_coef_x */
+    float base_wl_coef;
+    float reference_hpwl;
+    float wl_grad_sum;             /* region 0 */
+    float density_grad_sum;        /* region 0 */
+    int32_t num_diverge_reverts;
+    double init_seconds;           /* np_init wall time */
+    double loop_seconds;           /* the loop's wall time */
+    double first_below_target_seconds; /* np_init entry to the convergence iteration */
+} gpl_stats;
+
+/* nesterov.rs:4123: the reference HPWL from the design HPWL, then np_init (nesterov.rs:2674). */
+int32_t gpl_init_nesterov_loop(gpl_ctx* c);
+/* nesterov.rs:4440 resume_nesterov_place from iteration `start_iter`; returns the status and
+ * writes the final iteration to *iter_out. */
+int32_t gpl_run_nesterov_loop(gpl_ctx* c, int32_t start_iter, int32_t* iter_out);
+/* nesterov.rs:4115: init, then the loop from 0. */
+int32_t gpl_do_nesterov_place(gpl_ctx* c, int32_t* iter_out);
+/* nesterov.rs:4243: the loop again with a new target overflow and iteration limit. */
+int32_t gpl_continue_nesterov_place(gpl_ctx* c, float overflow, int32_t max_iter, int32_t start_iter,
+                                    int32_t* iter_out);
+int32_t gpl_get_stats(gpl_ctx* c, gpl_stats* out);
+
+/* Per-iteration history rows (iteration, overflow, HPWL, penalty, step, trials); returns the
+ * number of rows written (at most `cap`). */
+typedef struct gpl_iter_row {
+    int32_t iter;
+    int32_t trials;
+    float overflow;
+    float density_penalty;
+    float step_length;
+    float wl_coef;
+    int64_t hpwl;
+} gpl_iter_row;
+int32_t gpl_get_history(gpl_ctx* c, gpl_iter_row* rows, int32_t cap);
+
+/* The log lines written so far ("[INFO GPL-0001] ...", the progress table), joined with '\n'
+ * into buf (truncated to cap bytes, NUL-terminated); returns the full length. */
+int32_t gpl_get_log(gpl_ctx* c, char* buf, int32_t cap);
+
+/* nesterov.rs:4099: each placed instance's lower-left (3 int32 per instance: placer instance
+ * index, lx, ly); returns the number of instances written. */
+int32_t gpl_placed_locations(gpl_ctx* c, int32_t* out, int32_t cap);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Kernel-level entries (parity tests and the differentiable objective)                       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Which per-cell coordinate slot an entry reads or writes (Nb's vectors, nesterov.rs:480-493). */
+#define GPL_SLOT_CUR_SLP 0
+#define GPL_SLOT_NEXT_SLP 1
+#define GPL_SLOT_PREV_SLP 2
+#define GPL_SLOT_CUR 3
+#define GPL_SLOT_NEXT 4
+
+/* Host <-> device copies of per-cell float pairs (x, y interleaved) of one region. */
+int32_t gpl_set_coordinates(gpl_ctx* c, int32_t region, int32_t slot, const float* xy);
+int32_t gpl_get_coordinates(gpl_ctx* c, int32_t region, int32_t slot, float* xy);
+
+/* updateGCellDensityCenterLocation + updateBinsGCellDensityArea (nesterov.rs:2268, 2218): the
+ * region's cells' density centres from slot `slot`, the bins' integer areas, density and the
+ * overflows.  Outputs (any may be NULL): inst/filler area per bin (int64, cnt_x*cnt_y), the
+ * density per bin (float), and the two overflows as FP64 sums and as the integers the loop
+ * keeps. */
+int32_t gpl_density_scatter(gpl_ctx* c, int32_t region, int32_t slot, int64_t* inst_area,
+                            int64_t* filler_area, float* density, double* overflow_f64);
+
+/* updateDensityFieldBin (nesterov.rs:2279): the DCT Poisson solve of the region's current
+ * density.  Outputs per bin (y-major, any may be NULL): potential, field x, field y. */
+int32_t gpl_density_solve(gpl_ctx* c, int32_t region, float* phi, float* field_x, float* field_y);
+
+/* Sets the density of a region directly (for solver tests), y-major. */
+int32_t gpl_set_density(gpl_ctx* c, int32_t region, const float* density);
+
+/* updateWireLengthForceWA (nesterov.rs:2325) at the current pin positions with the given WA
+ * coefficients.  Outputs (any may be NULL): per net the 8 sums (min_x, x_min_x, max_x, x_max_x,
+ * min_y, y_min_y, max_y, y_max_y) and box (lx, ly, ux, uy), and the pin HPWL. */
+int32_t gpl_wa_forward(gpl_ctx* c, float wl_coef_x, float wl_coef_y, float* net_sums,
+                       int32_t* net_box, int64_t* hpwl);
+
+/* getWireLengthGradientWA for every cell of a region (nesterov.rs:2418) after gpl_wa_forward:
+ * per cell x, y. */
+int32_t gpl_wa_backward(gpl_ctx* c, int32_t region, float* wl_grad);
+
+/* getDensityGradient for every cell of a region (nesterov.rs:2434) after gpl_density_solve. */
+int32_t gpl_density_gather(gpl_ctx* c, int32_t region, float* den_grad);
+
+/* updateGradients (nesterov.rs:2455) into a slot, at the current state: per cell the WA and
+ * density gradients and the preconditioned sum; the FP64 sums |wl| and |den|. */
+int32_t gpl_update_gradients(gpl_ctx* c, int32_t region, int32_t slot, float density_penalty,
+                             float* wl, float* den, float* sum, double* abs_sums);
+
+/* nesterovUpdateCoordinates' arithmetic (nesterov.rs:2718-2737) from the device state: next and
+ * next look-ahead points for step `step` and momentum `coeff`. */
+int32_t gpl_nesterov_step(gpl_ctx* c, int32_t region, float step, float coeff, float* next,
+                          float* next_slp);
+
+/* The differentiable objective (no reference counterpart: the reference never evaluates it):
+ * f(x) = WA(x) + lambda * sum_bins phi*rho for the cell centres `xy` (region 0), with the
+ * exact exponential and continuous coordinates.  forward returns f (FP64 reduction);
+ * backward writes df/dxy per cell. */
+int32_t gpl_objective_forward(gpl_ctx* c, const float* xy, float wl_coef, float lambda,
+                              double* value, double* wa_part, double* density_part);
+int32_t gpl_objective_backward(gpl_ctx* c, const float* xy, float wl_coef, float lambda,
+                               float* grad);
+
+/* Fault injection for host-driven failure tests: after `after_iter` iterations the next
+ * gradient sum reads NaN (GPL-0306) -- kind 1 -- or the next step length is NaN (GPL-0305) --
+ * kind 2.  kind 0 clears it. */
+int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter);
+
+/* Device timing of the kernels of the last run (milliseconds summed per kernel family and
+ * launch counts), for the roofline report.  names: scatter, bin_density, dct, wa_forward,
+ * gradient, update, reduce.  Returns the number of families written. */
+typedef struct gpl_kernel_time {
+    char name[24];
+    int32_t launches;
+    double ms;
+} gpl_kernel_time;
+int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap);
+int32_t gpl_set_profiling(gpl_ctx* c, int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPL_HIP_H */

@@ -1,0 +1,248 @@
+"""TEST INFRASTRUCTURE -- ctypes access to the oracle (liboracle.so, built from oracle.c and
+dct.c by the Makefile here).  Imported only by tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline leg, as the checker; the product (gpl_amd/) never loads it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        sig = [
+            ("orc_fast_exp", C.c_float, [C.c_float]),
+            ("orc_distance", C.c_float, [f32p, f32p, C.c_int32]),
+            ("orc_derived_reference_hpwl", C.c_float, [C.c_float, C.c_int64]),
+            ("orc_wirelength_coef_factor", C.c_float, [C.c_float]),
+            ("orc_phi_coef", C.c_float, [C.c_float, C.c_float, C.c_float]),
+            ("orc_diverge_fallback", C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
+            ("orc_max_iter_takes_snapshot", C.c_int32,
+             [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_float, C.c_float, C.c_int64]),
+            ("orc_dct2_f32", None, [C.c_int, f32p]),
+            ("orc_dct3_f32", None, [C.c_int, f32p]),
+            ("orc_dst3_f32", None, [C.c_int, f32p]),
+            ("orc_dct2_f64", None, [C.c_int, f64p]),
+            ("orc_dct3_f64", None, [C.c_int, f64p]),
+            ("orc_dst3_f64", None, [C.c_int, f64p]),
+            ("orc_direct_dct2_f64", None, [C.c_int, f64p]),
+            ("orc_direct_dct3_f64", None, [C.c_int, f64p]),
+            ("orc_direct_dst3_f64", None, [C.c_int, f64p]),
+            ("orc_fft_solve_f32", None, [C.c_int32, C.c_int32, f32p, f32p, f32p, f32p, f32p, f32p]),
+            ("orc_fft_solve_f64", None, [C.c_int32, C.c_int32, f32p, f32p, f32p, f64p, f64p, f64p]),
+            ("orc_create", C.c_void_p, [C.c_void_p]),
+            ("orc_destroy", None, [C.c_void_p]),
+            ("orc_init", C.c_int32, [C.c_void_p]),
+            ("orc_run", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, i32p]),
+            ("orc_stats", None, [C.c_void_p, C.c_void_p]),
+            ("orc_history", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32]),
+            ("orc_get_slot", None, [C.c_void_p, C.c_int32, C.c_int32, f32p]),
+            ("orc_get_grad", None, [C.c_void_p, C.c_int32, C.c_int32, f32p]),
+            ("orc_get_bins", None, [C.c_void_p, C.c_int32, f32p, f32p, f32p, i64p]),
+            ("orc_get_scalars", None, [C.c_void_p, C.c_int32, f32p]),
+            ("orc_set_inject", None, [C.c_void_p, C.c_int32]),
+            ("orc_np_hpwl", C.c_int64, [C.c_void_p]),
+            ("orc_scatter", None, [C.c_void_p, C.c_int32, f32p, i64p, i64p, f32p, i64p, f64p]),
+            ("orc_place_pins", None, [C.c_void_p, C.c_int32, f32p]),
+            ("orc_wa_forward", None, [C.c_void_p, C.c_float, C.c_float, f32p, i32p, f64p, i64p]),
+            ("orc_wa_backward", None, [C.c_void_p, C.c_int32, f32p, f64p, f64p]),
+            ("orc_gather", None, [C.c_void_p, C.c_int32, f32p, f32p, f32p, f64p, f64p, f64p]),
+            ("orc_get_dbox", None, [C.c_void_p, i32p]),
+            ("orc_get_pins", None, [C.c_void_p, i32p]),
+        ]
+        for name, res, args in sig:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def f32(a):
+    return ptr(a, f32p)
+
+
+def f64(a):
+    return ptr(a, f64p)
+
+
+def i64(a):
+    return ptr(a, i64p)
+
+
+def i32(a):
+    return ptr(a, i32p)
+
+
+class Oracle:
+    """The faithful CPU placer on a problem view (a gpl_amd Problem ctypes struct)."""
+
+    def __init__(self, problem):
+        self.problem = problem
+        self._h = lib().orc_create(C.addressof(problem))
+        self.regions = [problem.regions[b] for b in range(problem.num_regions)]
+
+    def n(self, b=0):
+        R = self.regions[b]
+        return R.cell_end - R.cell_begin
+
+    def m(self, b=0):
+        R = self.regions[b]
+        return R.cnt_x * R.cnt_y
+
+    def init(self):
+        return lib().orc_init(self._h)
+
+    def run(self, start_iter=0, iter_budget=-1, time_budget_s=0.0):
+        it = C.c_int32(0)
+        rc = lib().orc_run(self._h, start_iter, iter_budget, time_budget_s, C.byref(it))
+        return rc, it.value
+
+    def stats(self):
+        from gpl_amd._ffi import Stats
+        s = Stats()
+        lib().orc_stats(self._h, C.byref(s))
+        return s
+
+    def history(self):
+        from gpl_amd._ffi import IterRow
+        n = lib().orc_history(self._h, None, 0)
+        rows = (IterRow * max(n, 1))()
+        lib().orc_history(self._h, rows, n)
+        return [dict(iter=r.iter, trials=r.trials, overflow=r.overflow, density_penalty=r.density_penalty,
+                     step_length=r.step_length, wl_coef=r.wl_coef, hpwl=r.hpwl) for r in rows[:n]]
+
+    def slot(self, b, slot):
+        a = np.zeros(2 * self.n(b), np.float32)
+        lib().orc_get_slot(self._h, b, slot, f32(a))
+        return a.reshape(-1, 2)
+
+    def grad(self, b, which):
+        a = np.zeros(2 * self.n(b), np.float32)
+        lib().orc_get_grad(self._h, b, which, f32(a))
+        return a.reshape(-1, 2)
+
+    def bins(self, b=0):
+        M = self.m(b)
+        d, fx, fy = (np.zeros(M, np.float32) for _ in range(3))
+        ov = np.zeros(2, np.int64)
+        lib().orc_get_bins(self._h, b, f32(d), f32(fx), f32(fy), i64(ov))
+        return d, fx, fy, ov
+
+    def scalars(self, b=0):
+        a = np.zeros(10, np.float32)
+        lib().orc_get_scalars(self._h, b, f32(a))
+        keys = ["step_length", "density_penalty", "wl_grad_sum", "density_grad_sum", "sum_overflow",
+                "sum_overflow_unscaled", "wl_coef", "base_wl_coef", "reference_hpwl", "coordi_distance"]
+        return dict(zip(keys, a.tolist()))
+
+    def hpwl(self):
+        return lib().orc_np_hpwl(self._h)
+
+    def scatter(self, b, xy):
+        M = self.m(b)
+        xy = np.ascontiguousarray(xy, np.float32).reshape(-1)
+        inst, fill = np.zeros(M, np.int64), np.zeros(M, np.int64)
+        dens = np.zeros(M, np.float32)
+        ovi, ovf = np.zeros(2, np.int64), np.zeros(2, np.float64)
+        lib().orc_scatter(self._h, b, f32(xy), i64(inst), i64(fill), f32(dens), i64(ovi), f64(ovf))
+        return inst, fill, dens, ovi, ovf
+
+    def place_pins(self, b, xy):
+        xy = np.ascontiguousarray(xy, np.float32).reshape(-1)
+        lib().orc_place_pins(self._h, b, f32(xy))
+
+    def wa_forward(self, cx, cy):
+        E = self.problem.num_nets
+        sums = np.zeros(8 * E, np.float32)
+        box = np.zeros(4 * E, np.int32)
+        s64 = np.zeros(8 * E, np.float64)
+        h = C.c_int64(0)
+        lib().orc_wa_forward(self._h, cx, cy, f32(sums), i32(box), f64(s64), C.byref(h))
+        return sums.reshape(-1, 8), box.reshape(-1, 4), s64.reshape(-1, 8), h.value
+
+    def wa_backward(self, b, sums_f64):
+        n = self.n(b)
+        wl = np.zeros(2 * n, np.float32)
+        w64 = np.zeros(2 * n, np.float64)
+        s = np.ascontiguousarray(sums_f64, np.float64).reshape(-1)
+        lib().orc_wa_backward(self._h, b, f32(wl), f64(s), f64(w64))
+        return wl.reshape(-1, 2), w64.reshape(-1, 2)
+
+    def gather(self, b, fx, fy, fx64=None, fy64=None):
+        n = self.n(b)
+        den = np.zeros(2 * n, np.float32)
+        d64 = np.zeros(2 * n, np.float64)
+        fx = np.ascontiguousarray(fx, np.float32)
+        fy = np.ascontiguousarray(fy, np.float32)
+        fx64 = None if fx64 is None else np.ascontiguousarray(fx64, np.float64)
+        fy64 = None if fy64 is None else np.ascontiguousarray(fy64, np.float64)
+        lib().orc_gather(self._h, b, f32(fx), f32(fy), f32(den), f64(fx64), f64(fy64), f64(d64) if fx64 is not None else None)
+        return den.reshape(-1, 2), d64.reshape(-1, 2)
+
+    def dbox(self):
+        a = np.zeros(4 * self.problem.num_cells, np.int32)
+        lib().orc_get_dbox(self._h, i32(a))
+        return a.reshape(-1, 4)
+
+    def pins(self):
+        a = np.zeros(2 * self.problem.num_pins, np.int32)
+        lib().orc_get_pins(self._h, i32(a))
+        return a.reshape(-1, 2)
+
+    def set_inject(self, after_iter):
+        lib().orc_set_inject(self._h, after_iter)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_f64(nx, ny, wx, wy, density_ym):
+    M = nx * ny
+    phi, fx, fy = (np.zeros(M, np.float64) for _ in range(3))
+    d = np.ascontiguousarray(density_ym, np.float32)
+    lib().orc_fft_solve_f64(nx, ny, f32(np.ascontiguousarray(wx, np.float32)), f32(np.ascontiguousarray(wy, np.float32)),
+                            f32(d), f64(phi), f64(fx), f64(fy))
+    return phi, fx, fy
+
+
+def solve_f32(nx, ny, wx, wy, density_ym):
+    """Fft::solve in float on y-major bins: returns phi, field_x, field_y (y-major)."""
+    d = np.ascontiguousarray(np.asarray(density_ym, np.float32).reshape(ny, nx).T.reshape(-1))  # x-major
+    phi, fx, fy = (np.zeros(nx * ny, np.float32) for _ in range(3))
+    lib().orc_fft_solve_f32(nx, ny, f32(np.ascontiguousarray(wx, np.float32)), f32(np.ascontiguousarray(wy, np.float32)),
+                            f32(d), f32(phi), f32(fx), f32(fy))
+    back = lambda a: a.reshape(nx, ny).T.reshape(-1).copy()
+    return back(phi), back(fx), back(fy)

@@ -1,0 +1,231 @@
+/*
+ * TEST INFRASTRUCTURE -- part of the oracle.  Only tests/, the smoke check and the bench's CPU
+ * baseline may call it; the product never does.
+ *
+ * 1-D cosine and sine transforms in the conventions of Ooura's fftsg package, which the
+ * reference transcribes (ooura.rs:1-39, 158-255):
+ *   dct2 (ddct, isgn = -1):  C[k] = sum_{j<n} a[j] cos(pi (j + 1/2) k / n)
+ *   dct3 (ddct, isgn = +1):  C[k] = sum_{j<n} a[j] cos(pi j (k + 1/2) / n)
+ *   dst3 (ddst, isgn = +1):  S[k] = sum_{j=1..n} A[j] sin(pi j (k + 1/2) / n), A[n] held in a[0]
+ * and the three 2-D transforms the density solve calls (ooura.rs:71-98): rows of length n2
+ * first, then columns of length n1, on a row-major n1 x n2 grid.
+ *
+ * The arithmetic is this file's own: Makhoul's DCT through a half-length complex FFT (radix-2,
+ * twiddles from the C library in double), not Ooura's split-radix butterflies, so results agree
+ * with the reference's to rounding, not bit for bit (the reference pins its own bits on macOS
+ * only, ooura.rs:3124-3159).  Powers of two use the FFT; any other length falls back to the
+ * direct O(n^2) sum with double accumulation.
+ *
+ * Pinned by tests/test_oracle.py: brute-force O(n^2) FP64 sums, scipy.fft, forward-inverse
+ * identity, and the single-cosine-mode eigenfunction of the Poisson solve.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PI 3.14159265358979323846
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* ---- generic body, instantiated for float (f32) and double (f64) ---------------------------- */
+#define DEFINE_DCT(T, SFX)                                                                          \
+    /* in-place radix-2 complex FFT of m points (re, im interleaved), sign -1 forward, +1 inverse  \
+     * (no scaling); twiddles exp(sign 2 pi i k / m) from double cos/sin */                         \
+    static void fft_##SFX(int m, T* z, int sign) {                                                  \
+        for (int i = 1, j = 0; i < m; ++i) {                                                        \
+            int bit = m >> 1;                                                                       \
+            for (; j & bit; bit >>= 1) j ^= bit;                                                    \
+            j ^= bit;                                                                               \
+            if (i < j) {                                                                            \
+                T tr = z[2 * i], ti = z[2 * i + 1];                                                 \
+                z[2 * i] = z[2 * j];                                                                \
+                z[2 * i + 1] = z[2 * j + 1];                                                        \
+                z[2 * j] = tr;                                                                      \
+                z[2 * j + 1] = ti;                                                                  \
+            }                                                                                       \
+        }                                                                                           \
+        for (int len = 2; len <= m; len <<= 1) {                                                    \
+            int h = len >> 1;                                                                       \
+            for (int k = 0; k < h; ++k) {                                                           \
+                double ang = sign * 2.0 * ORC_PI * k / len;                                         \
+                T wr = (T)cos(ang), wi = (T)sin(ang);                                               \
+                for (int s = 0; s < m; s += len) {                                                  \
+                    T* a = z + 2 * (s + k);                                                         \
+                    T* b = z + 2 * (s + k + h);                                                     \
+                    T br = b[0] * wr - b[1] * wi, bi = b[0] * wi + b[1] * wr;                       \
+                    b[0] = a[0] - br;                                                               \
+                    b[1] = a[1] - bi;                                                               \
+                    a[0] = a[0] + br;                                                               \
+                    a[1] = a[1] + bi;                                                               \
+                }                                                                                   \
+            }                                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+    /* DFT V[k] = sum_m v[m] e^{-2 pi i m k / n}, k = 0..n/2, of a real sequence v (n even),        \
+     * through an n/2-point complex FFT; out holds n/2 + 1 complex values */                        \
+    static void rfft_##SFX(int n, const T* v, T* out, T* z) {                                       \
+        int h = n / 2;                                                                              \
+        for (int m = 0; m < h; ++m) {                                                               \
+            z[2 * m] = v[2 * m];                                                                    \
+            z[2 * m + 1] = v[2 * m + 1];                                                            \
+        }                                                                                           \
+        fft_##SFX(h, z, -1);                                                                        \
+        for (int k = 0; k <= h; ++k) {                                                              \
+            int k1 = k % h, k2 = (h - k) % h;                                                       \
+            T ar = z[2 * k1], ai = z[2 * k1 + 1], br = z[2 * k2], bi = -z[2 * k2 + 1];              \
+            T er = (ar + br) * (T)0.5, ei = (ai + bi) * (T)0.5; /* even part */                     \
+            T orr = (ar - br) * (T)0.5, oi = (ai - bi) * (T)0.5; /* i * odd part */                 \
+            double ang = -2.0 * ORC_PI * k / n;                                                     \
+            T wr = (T)cos(ang), wi = (T)sin(ang);                                                   \
+            /* V = E + w * O, O = -i * (orr + i oi) = oi - i orr */                                 \
+            T xr = oi, xi = -orr;                                                                   \
+            out[2 * k] = er + (wr * xr - wi * xi);                                                  \
+            out[2 * k + 1] = ei + (wr * xi + wi * xr);                                              \
+        }                                                                                           \
+    }                                                                                               \
+    static void direct_dct2_##SFX(int n, T* a) {                                                    \
+        double* t = (double*)malloc(sizeof(double) * n);                                            \
+        for (int k = 0; k < n; ++k) {                                                               \
+            double s = 0.0;                                                                         \
+            for (int j = 0; j < n; ++j) s += (double)a[j] * cos(ORC_PI * (j + 0.5) * k / n);        \
+            t[k] = s;                                                                               \
+        }                                                                                           \
+        for (int k = 0; k < n; ++k) a[k] = (T)t[k];                                                 \
+        free(t);                                                                                    \
+    }                                                                                               \
+    static void direct_dct3_##SFX(int n, T* a) {                                                    \
+        double* t = (double*)malloc(sizeof(double) * n);                                            \
+        for (int k = 0; k < n; ++k) {                                                               \
+            double s = 0.0;                                                                         \
+            for (int j = 0; j < n; ++j) s += (double)a[j] * cos(ORC_PI * j * (k + 0.5) / n);        \
+            t[k] = s;                                                                               \
+        }                                                                                           \
+        for (int k = 0; k < n; ++k) a[k] = (T)t[k];                                                 \
+        free(t);                                                                                    \
+    }                                                                                               \
+    /* Ooura ddct isgn = -1 */                                                                      \
+    void orc_dct2_##SFX(int n, T* a) {                                                              \
+        if (n == 1) return;                                                                         \
+        if (!is_pow2(n)) {                                                                          \
+            direct_dct2_##SFX(n, a);                                                                \
+            return;                                                                                 \
+        }                                                                                           \
+        T* v = (T*)malloc(sizeof(T) * (5 * n + 4));                                                 \
+        T* V = v + n;                                                                               \
+        T* z = V + n + 2;                                                                           \
+        for (int k = 0; k < n / 2; ++k) {                                                           \
+            v[k] = a[2 * k];                                                                        \
+            v[n - 1 - k] = a[2 * k + 1];                                                            \
+        }                                                                                           \
+        rfft_##SFX(n, v, V, z);                                                                     \
+        for (int k = 0; k < n; ++k) {                                                               \
+            T vr, vi;                                                                               \
+            if (k <= n / 2) {                                                                       \
+                vr = V[2 * k];                                                                      \
+                vi = V[2 * k + 1];                                                                  \
+            } else {                                                                                \
+                vr = V[2 * (n - k)];                                                                \
+                vi = -V[2 * (n - k) + 1];                                                           \
+            }                                                                                       \
+            double ang = -ORC_PI * k / (2.0 * n);                                                   \
+            a[k] = (T)cos(ang) * vr - (T)sin(ang) * vi;                                             \
+        }                                                                                           \
+        free(v);                                                                                    \
+    }                                                                                               \
+    /* Ooura ddct isgn = +1: y[m] = sum_k a[k] cos(pi k (2m+1) / 2n).  With V[k] =                  \
+     * e^{i pi k / 2n} (c[k] - i c[n-k]), c = a except c[0] = 2 a[0], c[n] = 0, the real sequence   \
+     * v = IDFT(V) / 2 gives y[2m] = v[m], y[2m+1] = v[n-1-m]. */                                  \
+    void orc_dct3_##SFX(int n, T* a) {                                                              \
+        if (n == 1) return;                                                                         \
+        if (!is_pow2(n)) {                                                                          \
+            direct_dct3_##SFX(n, a);                                                                \
+            return;                                                                                 \
+        }                                                                                           \
+        int h = n / 2;                                                                              \
+        T* buf = (T*)malloc(sizeof(T) * (4 * n + 8));                                               \
+        T* V = buf;           /* n/2 + 1 complex */                                                 \
+        T* z = V + n + 2;     /* n/2 complex */                                                     \
+        for (int k = 0; k <= h; ++k) {                                                              \
+            T ck = (k == 0) ? (T)2 * a[0] : a[k % n];                                               \
+            T cnk = (k == 0) ? (T)0 : a[n - k];                                                     \
+            if (k == h) ck = a[h], cnk = a[h];                                                      \
+            double ang = ORC_PI * k / (2.0 * n);                                                    \
+            T wr = (T)cos(ang), wi = (T)sin(ang);                                                   \
+            V[2 * k] = wr * ck + wi * cnk;                                                          \
+            V[2 * k + 1] = wi * ck - wr * cnk;                                                      \
+        }                                                                                           \
+        /* inverse real DFT via an n/2-point complex IFFT:                                          \
+         * Z[k] = (V[k] + conj(V[h-k])) + i e^{2 pi i k/n} (V[k] - conj(V[h-k])) */                 \
+        for (int k = 0; k < h; ++k) {                                                               \
+            T ar = V[2 * k], ai = V[2 * k + 1], br = V[2 * (h - k)], bi = -V[2 * (h - k) + 1];      \
+            T er = ar + br, ei = ai + bi, dr = ar - br, di = ai - bi;                               \
+            double ang = 2.0 * ORC_PI * k / n;                                                      \
+            T wr = (T)cos(ang), wi = (T)sin(ang);                                                   \
+            T tr = wr * dr - wi * di, ti = wr * di + wi * dr; /* w * d */                           \
+            z[2 * k] = er - ti;                                                                     \
+            z[2 * k + 1] = ei + tr;                                                                 \
+        }                                                                                           \
+        fft_##SFX(h, z, +1);                                                                        \
+        /* v[2m] = Re z[m] / n, v[2m+1] = Im z[m] / n; y = v * n / 2 */                             \
+        T* y = (T*)malloc(sizeof(T) * n);                                                           \
+        for (int m = 0; m < h; ++m) {                                                               \
+            T v0 = z[2 * m] * (T)0.5, v1 = z[2 * m + 1] * (T)0.5;                                   \
+            y[2 * m] = v0;                                                                          \
+            y[2 * m + 1] = v1;                                                                      \
+        }                                                                                           \
+        for (int m = 0; m < h; ++m) {                                                               \
+            a[2 * m] = y[m];                                                                        \
+            a[2 * m + 1] = y[n - 1 - m];                                                            \
+        }                                                                                           \
+        free(y);                                                                                    \
+        free(buf);                                                                                  \
+    }                                                                                               \
+    /* Ooura ddst isgn = +1: S[k] = (-1)^k dct3(b)[k], b[0] = a[0] (= A[n]), b[j] = a[n-j] */       \
+    void orc_dst3_##SFX(int n, T* a) {                                                              \
+        if (n == 1) return;                                                                         \
+        for (int j = 1; j < n - j; ++j) {                                                           \
+            T t = a[j];                                                                             \
+            a[j] = a[n - j];                                                                        \
+            a[n - j] = t;                                                                           \
+        }                                                                                           \
+        orc_dct3_##SFX(n, a);                                                                       \
+        for (int k = 1; k < n; k += 2) a[k] = -a[k];                                                \
+    }                                                                                               \
+    /* 2-D: rows (length n2) with `row`, then columns (length n1) with `col` */                     \
+    static void xt2d_##SFX(int n1, int n2, T* a, void (*row)(int, T*), void (*col)(int, T*)) {      \
+        for (int i = 0; i < n1; ++i) row(n2, a + (size_t)i * n2);                                   \
+        T* t = (T*)malloc(sizeof(T) * n1);                                                          \
+        for (int j = 0; j < n2; ++j) {                                                              \
+            for (int i = 0; i < n1; ++i) t[i] = a[(size_t)i * n2 + j];                              \
+            col(n1, t);                                                                             \
+            for (int i = 0; i < n1; ++i) a[(size_t)i * n2 + j] = t[i];                              \
+        }                                                                                           \
+        free(t);                                                                                    \
+    }                                                                                               \
+    void orc_ddct2d_fwd_##SFX(int n1, int n2, T* a) { xt2d_##SFX(n1, n2, a, orc_dct2_##SFX, orc_dct2_##SFX); } \
+    void orc_ddct2d_inv_##SFX(int n1, int n2, T* a) { xt2d_##SFX(n1, n2, a, orc_dct3_##SFX, orc_dct3_##SFX); } \
+    /* ddsct2d(+1): cosine along the rows, sine along the columns */                                \
+    void orc_ddsct2d_inv_##SFX(int n1, int n2, T* a) { xt2d_##SFX(n1, n2, a, orc_dct3_##SFX, orc_dst3_##SFX); } \
+    /* ddcst2d(+1): sine along the rows, cosine along the columns */                                \
+    void orc_ddcst2d_inv_##SFX(int n1, int n2, T* a) { xt2d_##SFX(n1, n2, a, orc_dst3_##SFX, orc_dct3_##SFX); }
+
+DEFINE_DCT(float, f32)
+DEFINE_DCT(double, f64)
+
+/* Direct O(n^2) FP64 references, for pinning the fast transforms. */
+void orc_direct_dct2_f64(int n, double* a) { direct_dct2_f64(n, a); }
+void orc_direct_dct3_f64(int n, double* a) { direct_dct3_f64(n, a); }
+void orc_direct_dst3_f64(int n, double* a) {
+    double* t = (double*)malloc(sizeof(double) * n);
+    for (int k = 0; k < n; ++k) {
+        double s = 0.0;
+        for (int j = 1; j <= n; ++j) {
+            double A = (j == n) ? a[0] : a[j];
+            s += A * sin(ORC_PI * j * (k + 0.5) / n);
+        }
+        t[k] = s;
+    }
+    memcpy(a, t, sizeof(double) * n);
+    free(t);
+}

@@ -1,0 +1,1350 @@
+/*
+ * TEST INFRASTRUCTURE -- the oracle.  A single-thread CPU restatement of the reference's
+ * Nesterov placement loop (vyges-gpl 0.1.56, src/nesterov.rs), used as the parity checker of the
+ * HIP path and as the CPU baseline the bench times.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product (gpl_amd/) never does.
+ *
+ * Input: the placer state init_nesterov_place leaves (nesterov.rs:787-933), as the C ABI's
+ * gpl_problem holds it.  Every value keeps the reference's type and every expression its
+ * evaluation order (float state, the double expressions where the reference has them, sums in
+ * the reference's order), so a path that agrees with this agrees with the reference up to the
+ * DCT's rounding (dct.c: this file's own FFT, not Ooura's butterflies).
+ *
+ * Besides the faithful loop it holds FP64 versions of the per-kernel arithmetic (bin density,
+ * the Poisson solve, WA forward and backward, the density gather) for the tolerance checks.
+ *
+ * Pinning (tests/test_oracle.py): the reference's known-answer tests (fast_exp nesterov.rs:4857,
+ * distance 4893, wavenumbers 4697, reference HPWL 4813, divergence and snapshot rules
+ * 4627-4669), brute-force sums on tiny inputs, closed forms (2-pin WA, single-mode Poisson),
+ * invariants (translation invariance of WA, the scatter's area conservation) and scipy.fft.
+ * Functions without such a check say "parity unpinned" in their comment.
+ */
+#define _POSIX_C_SOURCE 199309L
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "../include/gpl_hip.h"
+
+void orc_dct2_f32(int n, float* a);
+void orc_dct3_f32(int n, float* a);
+void orc_dst3_f32(int n, float* a);
+void orc_ddct2d_fwd_f32(int n1, int n2, float* a);
+void orc_ddct2d_inv_f32(int n1, int n2, float* a);
+void orc_ddsct2d_inv_f32(int n1, int n2, float* a);
+void orc_ddcst2d_inv_f32(int n1, int n2, float* a);
+void orc_ddct2d_fwd_f64(int n1, int n2, double* a);
+void orc_ddct2d_inv_f64(int n1, int n2, double* a);
+void orc_ddsct2d_inv_f64(int n1, int n2, double* a);
+void orc_ddcst2d_inv_f64(int n1, int n2, double* a);
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ---- small known-answer functions ---------------------------------------------------------- */
+
+/* nesterov.rs:308 fastExp: (1 + x/1024)^1024 by ten squarings, in float. */
+float orc_fast_exp(float x) {
+    float e = 1.0f + x / 1024.0f;
+    for (int i = 0; i < 10; ++i) e *= e;
+    return e;
+}
+
+/* nesterov.rs:319 getDistance: a float sum of squared differences in point order, then
+ * sqrt(sum / (2 n)) in double.  a, b: n points, x and y interleaved. */
+float orc_distance(const float* a, const float* b, int32_t n) {
+    float sum = 0.0f;
+    for (int32_t i = 0; i < n; ++i) {
+        sum += (a[2 * i] - b[2 * i]) * (a[2 * i] - b[2 * i]);
+        sum += (a[2 * i + 1] - b[2 * i + 1]) * (a[2 * i + 1] - b[2 * i + 1]);
+    }
+    if (n == 0) return 0.0f;
+    return (float)sqrt((double)sum / (2.0 * (double)n));
+}
+
+/* nesterov.rs:96 derived_reference_hpwl. */
+float orc_derived_reference_hpwl(float user, int64_t design_hpwl) {
+    if (user > 0.0f) return user;
+    float f = 0.01f * (float)design_hpwl;
+    return f > 446000000.0f ? f : 446000000.0f;
+}
+
+/* nesterov.rs:2532 updateWireLengthCoef's factor (times the base coefficient by the caller). */
+float orc_wirelength_coef_factor(float overflow) {
+    if (overflow > 1.0f) return (float)0.1;
+    if ((double)overflow < 0.1) return 10.0f;
+    return (float)(1.0 / pow(10.0, ((double)overflow - 0.1) * 20.0 / 9.0 - 1.0));
+}
+
+/* nesterov.rs:2827 getPhiCoef. */
+float orc_phi_coef(float max_phi, float min_phi, float scaled_diff_hpwl) {
+    float r = scaled_diff_hpwl < 0.0f ? max_phi : (float)((double)max_phi * pow((double)max_phi, -(double)scaled_diff_hpwl));
+    return min_phi > r ? min_phi : r;
+}
+
+/* nesterov.rs:4406 diverge_fallback: 0 revert and resume, 1 revert and stop, 2 error. */
+int32_t orc_diverge_fallback(int32_t saved, int32_t reverts, int32_t disable_revert) {
+    if (!disable_revert && saved && reverts < 3) return 0;
+    if (!disable_revert && reverts > 0) return 1;
+    return 2;
+}
+
+/* nesterov.rs:4420 max_iter_takes_min_hpwl_snapshot. */
+int32_t orc_max_iter_takes_snapshot(int32_t iter, int32_t max_iter, int32_t disable_revert, int32_t saved, int32_t reverts,
+                                    int64_t snap_hpwl, float snapshot_overflow, float overflow, int64_t hpwl) {
+    return iter >= max_iter && !disable_revert && (saved || reverts > 0) && snapshot_overflow <= overflow && snap_hpwl < hpwl;
+}
+
+/* ---- the Poisson solve (nesterov.rs:412-444 Fft::solve) ------------------------------------- */
+
+/* Faithful: the density in x-major layout (a[x*ny + y]) in, potential and field out, all float;
+ * the halving and scale in double as the reference writes them. */
+void orc_fft_solve_f32(int32_t nx, int32_t ny, const float* wx, const float* wy, float* density, float* phi, float* field_x, float* field_y) {
+    orc_ddct2d_fwd_f32(nx, ny, density);
+    for (int i = 1; i < nx; ++i) density[(size_t)i * ny] = (float)((double)density[(size_t)i * ny] * 0.5);
+    for (int j = 1; j < ny; ++j) density[j] = (float)((double)density[j] * 0.5);
+    double scale = 4.0 / (double)nx / (double)ny;
+    for (size_t k = 0; k < (size_t)nx * ny; ++k) density[k] = (float)((double)density[k] * scale);
+    for (int i = 0; i < nx; ++i) {
+        float wxi = wx[i], wx2 = wx[i] * wx[i];
+        for (int j = 0; j < ny; ++j) {
+            float wyj = wy[j], wy2 = wy[j] * wy[j];
+            size_t k = (size_t)i * ny + j;
+            float p = 0.0f, ex = 0.0f, ey = 0.0f;
+            if (!(i == 0 && j == 0)) {
+                p = density[k] / (wx2 + wy2);
+                ex = p * wxi;
+                ey = p * wyj;
+            }
+            phi[k] = p;
+            field_x[k] = ex;
+            field_y[k] = ey;
+        }
+    }
+    orc_ddct2d_inv_f32(nx, ny, phi);
+    orc_ddsct2d_inv_f32(nx, ny, field_x);
+    orc_ddcst2d_inv_f32(nx, ny, field_y);
+}
+
+/* FP64: bins in y-major layout (k = y*nx + x) in and out, the same arithmetic in double with
+ * the float wavenumbers widened. */
+void orc_fft_solve_f64(int32_t nx, int32_t ny, const float* wx, const float* wy, const float* density_ym, double* phi_ym, double* fx_ym, double* fy_ym) {
+    size_t M = (size_t)nx * ny;
+    double* a = (double*)malloc(sizeof(double) * M * 4);
+    double *ph = a + M, *ex = a + 2 * M, *ey = a + 3 * M;
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) a[(size_t)x * ny + y] = (double)density_ym[(size_t)y * nx + x];
+    orc_ddct2d_fwd_f64(nx, ny, a);
+    for (int i = 1; i < nx; ++i) a[(size_t)i * ny] *= 0.5;
+    for (int j = 1; j < ny; ++j) a[j] *= 0.5;
+    double scale = 4.0 / (double)nx / (double)ny;
+    for (size_t k = 0; k < M; ++k) a[k] *= scale;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            size_t k = (size_t)i * ny + j;
+            double w2 = (double)wx[i] * wx[i] + (double)wy[j] * wy[j];
+            double p = (i == 0 && j == 0) ? 0.0 : a[k] / w2;
+            ph[k] = p;
+            ex[k] = p * wx[i];
+            ey[k] = p * wy[j];
+        }
+    orc_ddct2d_inv_f64(nx, ny, ph);
+    orc_ddsct2d_inv_f64(nx, ny, ex);
+    orc_ddcst2d_inv_f64(nx, ny, ey);
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            size_t s = (size_t)x * ny + y, d = (size_t)y * nx + x;
+            if (phi_ym) phi_ym[d] = ph[s];
+            if (fx_ym) fx_ym[d] = ex[s];
+            if (fy_ym) fy_ym[d] = ey[s];
+        }
+    free(a);
+}
+
+/* ---- the placer state ----------------------------------------------------------------------- */
+
+typedef struct {
+    float x, y;
+} fpt;
+
+enum { S_CUR_SLP, S_CUR_SLP_WL, S_CUR_SLP_DEN, S_CUR_SLP_SUM, S_NEXT_SLP, S_NEXT_SLP_WL, S_NEXT_SLP_DEN, S_NEXT_SLP_SUM,
+       S_PREV_SLP, S_PREV_SLP_WL, S_PREV_SLP_DEN, S_PREV_SLP_SUM, S_CUR, S_NEXT, S_SNAP, S_SNAP_SLP, S_SNAP_SLP_SUM,
+       S_SNAP_PREV_SLP_SUM, S_COUNT };
+
+typedef struct {
+    gpl_region R;
+    int32_t n, begin;
+    int64_t *non_place, *non_place_unscaled, *inst, *filler;
+    int32_t *bin_x, *bin_y;
+    float *wx, *wy;
+    float *density, *field_x, *field_y, *phi, *fft_a, *fft_b, *fft_c, *fft_d;
+    int64_t overflow, overflow_unscaled;
+    fpt* v[S_COUNT];
+    float snapshot_density_penalty, snapshot_step_length, step_length, density_penalty, wl_grad_sum, density_grad_sum;
+    float sum_overflow, sum_overflow_unscaled, base_wl_coef_nb;
+    int64_t prev_hpwl, prev_reported_hpwl;
+    float prev_reported_overflow_unscaled, min_sum_overflow, hpwl_with_min_sum_overflow, coordi_distance, peak_coordi_distance;
+    float max_phi_coef;
+    int32_t max_phi_coef_changed, is_converged, is_diverged;
+} orc_nb;
+
+typedef struct orc_np {
+    gpl_opts o;
+    int32_t ncells, npins, nnets, nregions;
+    /* cells */
+    int32_t *lx, *ly, *ux, *uy, *dlx, *dly, *dux, *duy, *pin_start, *pins;
+    float* dscale;
+    uint8_t* flags;
+    /* pins */
+    int32_t *pcx, *pcy, *poff_x, *poff_y, *pcell, *pnet;
+    uint8_t* has;  /* bit 0 min_x, 1 max_x, 2 min_y, 3 max_y */
+    float* pexp;   /* 4 per pin: min_x, max_x, min_y, max_y */
+    /* nets */
+    int32_t *net_start, *net_pins, *nbox;
+    float *nw, *nsum; /* 8 per net */
+    orc_nb* nbs;
+    /* NesterovPlace */
+    float wl_coef_x, wl_coef_y, base_wl_coef;
+    int32_t recursion_wl, recursion_init_slp;
+    float initial_prev_coordi_update_coef, reference_hpwl;
+    int32_t max_iter;
+    float average_overflow, average_overflow_unscaled;
+    int64_t min_hpwl;
+    int32_t is_min_hpwl, diverge_snapshot_iter;
+    float diverge_snapshot_overflow;
+    int32_t num_region_diverged, diverge_code;
+    /* loop snapshots (nesterov.rs:4428) */
+    int32_t diverge_saved, diverge_reverts;
+    float diverge_coef_x, diverge_coef_y;
+    int64_t diverge_hpwl;
+    /* bookkeeping */
+    int32_t trials, total_trials, iter_done, converged;
+    double t_init, t_loop, t_converge;
+    gpl_iter_row* hist;
+    int32_t nhist, cap_hist;
+    int32_t inject_nan_after;
+} orc_np;
+
+static void* xcalloc(size_t n, size_t s) {
+    void* p = calloc(n ? n : 1, s);
+    if (!p) {
+        fprintf(stderr, "oracle: out of memory\n");
+        abort();
+    }
+    return p;
+}
+
+orc_np* orc_create(const gpl_problem* p) {
+    orc_np* np = (orc_np*)xcalloc(1, sizeof(orc_np));
+    np->o = p->opts;
+    int32_t N = p->num_cells, P = p->num_pins, E = p->num_nets;
+    np->ncells = N;
+    np->npins = P;
+    np->nnets = E;
+    np->lx = (int32_t*)xcalloc(N, 4);
+    np->ly = (int32_t*)xcalloc(N, 4);
+    np->ux = (int32_t*)xcalloc(N, 4);
+    np->uy = (int32_t*)xcalloc(N, 4);
+    np->dlx = (int32_t*)xcalloc(N, 4);
+    np->dly = (int32_t*)xcalloc(N, 4);
+    np->dux = (int32_t*)xcalloc(N, 4);
+    np->duy = (int32_t*)xcalloc(N, 4);
+    np->dscale = (float*)xcalloc(N, 4);
+    np->flags = (uint8_t*)xcalloc(N, 1);
+    np->pin_start = (int32_t*)xcalloc(N + 1, 4);
+    np->pins = (int32_t*)xcalloc(p->cell_pin_start[N], 4);
+    for (int32_t i = 0; i < N; ++i) {
+        np->lx[i] = p->cell_box[4 * i];
+        np->ly[i] = p->cell_box[4 * i + 1];
+        np->ux[i] = p->cell_box[4 * i + 2];
+        np->uy[i] = p->cell_box[4 * i + 3];
+        np->dlx[i] = p->cell_dbox[4 * i];
+        np->dly[i] = p->cell_dbox[4 * i + 1];
+        np->dux[i] = p->cell_dbox[4 * i + 2];
+        np->duy[i] = p->cell_dbox[4 * i + 3];
+        np->dscale[i] = p->density_scale[i];
+        np->flags[i] = p->cell_flags[i];
+    }
+    memcpy(np->pin_start, p->cell_pin_start, sizeof(int32_t) * (N + 1));
+    memcpy(np->pins, p->cell_pins, sizeof(int32_t) * p->cell_pin_start[N]);
+    np->pcx = (int32_t*)xcalloc(P, 4);
+    np->pcy = (int32_t*)xcalloc(P, 4);
+    np->poff_x = (int32_t*)xcalloc(P, 4);
+    np->poff_y = (int32_t*)xcalloc(P, 4);
+    np->pcell = (int32_t*)xcalloc(P, 4);
+    np->pnet = (int32_t*)xcalloc(P, 4);
+    np->has = (uint8_t*)xcalloc(P, 1);
+    np->pexp = (float*)xcalloc((size_t)P * 4, 4);
+    for (int32_t q = 0; q < P; ++q) {
+        np->pcx[q] = p->pin_xy[2 * q];
+        np->pcy[q] = p->pin_xy[2 * q + 1];
+        np->poff_x[q] = p->pin_offset[2 * q];
+        np->poff_y[q] = p->pin_offset[2 * q + 1];
+        np->pcell[q] = p->pin_cell[q];
+        np->pnet[q] = p->pin_net[q];
+    }
+    np->net_start = (int32_t*)xcalloc(E + 1, 4);
+    memcpy(np->net_start, p->net_pin_start, sizeof(int32_t) * (E + 1));
+    np->net_pins = (int32_t*)xcalloc(p->net_pin_start[E], 4);
+    memcpy(np->net_pins, p->net_pins, sizeof(int32_t) * p->net_pin_start[E]);
+    np->nw = (float*)xcalloc(E, 4);
+    memcpy(np->nw, p->net_weight, sizeof(float) * E);
+    np->nbox = (int32_t*)xcalloc((size_t)E * 4, 4);
+    np->nsum = (float*)xcalloc((size_t)E * 8, 4);
+    np->nregions = p->num_regions;
+    np->nbs = (orc_nb*)xcalloc(p->num_regions, sizeof(orc_nb));
+    for (int32_t b = 0; b < p->num_regions; ++b) {
+        orc_nb* nb = &np->nbs[b];
+        nb->R = p->regions[b];
+        nb->begin = nb->R.cell_begin;
+        nb->n = nb->R.cell_end - nb->R.cell_begin;
+        size_t M = (size_t)nb->R.cnt_x * nb->R.cnt_y;
+        nb->non_place = (int64_t*)xcalloc(M, 8);
+        nb->non_place_unscaled = (int64_t*)xcalloc(M, 8);
+        memcpy(nb->non_place, nb->R.non_place, 8 * M);
+        memcpy(nb->non_place_unscaled, nb->R.non_place_unscaled, 8 * M);
+        nb->inst = (int64_t*)xcalloc(M, 8);
+        nb->filler = (int64_t*)xcalloc(M, 8);
+        nb->bin_x = (int32_t*)xcalloc(nb->R.cnt_x + 1, 4);
+        nb->bin_y = (int32_t*)xcalloc(nb->R.cnt_y + 1, 4);
+        memcpy(nb->bin_x, nb->R.bin_x, 4 * (nb->R.cnt_x + 1));
+        memcpy(nb->bin_y, nb->R.bin_y, 4 * (nb->R.cnt_y + 1));
+        nb->wx = (float*)xcalloc(nb->R.cnt_x, 4);
+        nb->wy = (float*)xcalloc(nb->R.cnt_y, 4);
+        memcpy(nb->wx, nb->R.wx, 4 * nb->R.cnt_x);
+        memcpy(nb->wy, nb->R.wy, 4 * nb->R.cnt_y);
+        nb->density = (float*)xcalloc(M, 4);
+        nb->field_x = (float*)xcalloc(M, 4);
+        nb->field_y = (float*)xcalloc(M, 4);
+        nb->phi = (float*)xcalloc(M, 4);
+        nb->fft_a = (float*)xcalloc(M * 4, 4);
+        nb->fft_b = nb->fft_a + M;
+        nb->fft_c = nb->fft_a + 2 * M;
+        nb->fft_d = nb->fft_a + 3 * M;
+        for (int s = 0; s < S_COUNT; ++s) nb->v[s] = (fpt*)xcalloc(nb->n, sizeof(fpt));
+        nb->max_phi_coef = np->o.max_phi_coef;
+        nb->min_sum_overflow = 1e30f;
+        nb->hpwl_with_min_sum_overflow = 1e30f;
+    }
+    np->initial_prev_coordi_update_coef = 100.0f;
+    np->max_iter = np->o.max_iter;
+    np->min_hpwl = INT64_MAX;
+    np->inject_nan_after = -1;
+    return np;
+}
+
+void orc_destroy(orc_np* np) {
+    if (!np) return;
+    for (int32_t b = 0; b < np->nregions; ++b) {
+        orc_nb* nb = &np->nbs[b];
+        free(nb->non_place);
+        free(nb->non_place_unscaled);
+        free(nb->inst);
+        free(nb->filler);
+        free(nb->bin_x);
+        free(nb->bin_y);
+        free(nb->wx);
+        free(nb->wy);
+        free(nb->density);
+        free(nb->field_x);
+        free(nb->field_y);
+        free(nb->phi);
+        free(nb->fft_a);
+        for (int s = 0; s < S_COUNT; ++s) free(nb->v[s]);
+    }
+    free(np->nbs);
+    int32_t* ia[] = {np->lx, np->ly, np->ux, np->uy, np->dlx, np->dly, np->dux, np->duy, np->pin_start, np->pins, np->pcx, np->pcy,
+                     np->poff_x, np->poff_y, np->pcell, np->pnet, np->net_start, np->net_pins, np->nbox};
+    for (size_t i = 0; i < sizeof ia / sizeof ia[0]; ++i) free(ia[i]);
+    free(np->dscale);
+    free(np->flags);
+    free(np->has);
+    free(np->pexp);
+    free(np->nw);
+    free(np->nsum);
+    free(np->hist);
+    free(np);
+}
+
+static int64_t nesterov_insts_area(const orc_nb* nb) {
+    return nb->R.std_insts_area + (int64_t)roundf((float)nb->R.macro_insts_area * nb->R.target_density);
+}
+
+static int d_cx(const orc_np* np, int g) { return (np->dux[g] + np->dlx[g]) / 2; }
+static int d_cy(const orc_np* np, int g) { return (np->duy[g] + np->dly[g]) / 2; }
+
+/* nesterov.rs:2139 set_density_center with the cell's pins. */
+static void set_density_center(orc_np* np, int g, int cx, int cy) {
+    int hx = (np->dux[g] - np->dlx[g]) / 2, hy = (np->duy[g] - np->dly[g]) / 2;
+    np->dlx[g] = cx - hx;
+    np->dly[g] = cy - hy;
+    np->dux[g] = cx + hx;
+    np->duy[g] = cy + hy;
+    int dcx = d_cx(np, g), dcy = d_cy(np, g);
+    for (int k = np->pin_start[g]; k < np->pin_start[g + 1]; ++k) {
+        int p = np->pins[k];
+        np->pcx[p] = dcx + np->poff_x[p];
+        np->pcy[p] = dcy + np->poff_y[p];
+    }
+}
+
+/* nesterov.rs:2156 set_density_location. */
+static void set_density_location(orc_np* np, int g, int lx, int ly) {
+    np->dux[g] = lx + (np->dux[g] - np->dlx[g]);
+    np->duy[g] = ly + (np->duy[g] - np->dly[g]);
+    np->dlx[g] = lx;
+    np->dly[g] = ly;
+    int dcx = d_cx(np, g), dcy = d_cy(np, g);
+    for (int k = np->pin_start[g]; k < np->pin_start[g + 1]; ++k) {
+        int p = np->pins[k];
+        np->pcx[p] = dcx + np->poff_x[p];
+        np->pcy[p] = dcy + np->poff_y[p];
+    }
+}
+
+/* nesterov.rs:2188 inside_x / 2202 inside_y. */
+static float inside_x(const orc_np* np, const orc_nb* nb, int g, float cx) {
+    float half = (float)(np->dux[g] - np->dlx[g]) / 2.0f, adj = cx;
+    if (cx - half < (float)nb->R.lx) adj = (float)nb->R.lx + half;
+    if (cx + half > (float)nb->R.ux) adj = (float)nb->R.ux - half;
+    return adj;
+}
+static float inside_y(const orc_np* np, const orc_nb* nb, int g, float cy) {
+    float half = (float)(np->duy[g] - np->dly[g]) / 2.0f, adj = cy;
+    if (cy - half < (float)nb->R.ly) adj = (float)nb->R.ly + half;
+    if (cy + half > (float)nb->R.uy) adj = (float)nb->R.uy - half;
+    return adj;
+}
+
+static void didx_x(const orc_nb* nb, int32_t lo, int32_t hi, int* x0, int* x1) {
+    int a = (int)((double)(lo - nb->R.lx) / nb->R.size_x);
+    int b = (int)ceil((double)(hi - nb->R.lx) / nb->R.size_x);
+    *x0 = a > 0 ? a : 0;
+    *x1 = b < nb->R.cnt_x ? b : nb->R.cnt_x;
+}
+static void didx_y(const orc_nb* nb, int32_t lo, int32_t hi, int* y0, int* y1) {
+    int a = (int)((double)(lo - nb->R.ly) / nb->R.size_y);
+    int b = (int)ceil((double)(hi - nb->R.ly) / nb->R.size_y);
+    *y0 = a > 0 ? a : 0;
+    *y1 = b < nb->R.cnt_y ? b : nb->R.cnt_y;
+}
+
+/* nesterov.rs:298 getOverlapDensityArea: a float product of the integer overlaps. */
+static float overlap_density_area(int blx, int bly, int bux, int buy, int dlx, int dly, int dux, int duy) {
+    int lx = blx > dlx ? blx : dlx, ly = bly > dly ? bly : dly;
+    int ux = bux < dux ? bux : dux, uy = buy < duy ? buy : duy;
+    if (lx >= ux || ly >= uy) return 0.0f;
+    return (float)(ux - lx) * (float)(uy - ly);
+}
+
+/* nesterov.rs:2218 updateBinsGCellDensityArea. */
+static void update_bins_density_area(orc_np* np, orc_nb* nb) {
+    size_t M = (size_t)nb->R.cnt_x * nb->R.cnt_y;
+    memset(nb->inst, 0, 8 * M);
+    memset(nb->filler, 0, 8 * M);
+    int cnt_x = nb->R.cnt_x;
+    float td = nb->R.target_density;
+    for (int i = 0; i < nb->n; ++i) {
+        int g = nb->begin + i, x0, x1, y0, y1;
+        didx_x(nb, np->dlx[g], np->dux[g], &x0, &x1);
+        didx_y(nb, np->dly[g], np->duy[g], &y0, &y1);
+        int inst = np->flags[g] & GPL_CELL_INSTANCE, macro = inst && (np->flags[g] & GPL_CELL_MACRO);
+        for (int y = y0; y < y1; ++y)
+            for (int x = x0; x < x1; ++x) {
+                float a = overlap_density_area(nb->bin_x[x], nb->bin_y[y], nb->bin_x[x + 1], nb->bin_y[y + 1], np->dlx[g], np->dly[g], np->dux[g], np->duy[g]) * np->dscale[g];
+                size_t k = (size_t)y * cnt_x + x;
+                if (inst) {
+                    if (macro) a *= td;
+                    nb->inst[k] += (int64_t)a;
+                } else {
+                    nb->filler[k] += (int64_t)a;
+                }
+            }
+    }
+    nb->overflow = 0;
+    nb->overflow_unscaled = 0;
+    for (int y = 0; y < nb->R.cnt_y; ++y)
+        for (int x = 0; x < cnt_x; ++x) {
+            size_t k = (size_t)y * cnt_x + x;
+            int64_t area = (int64_t)(nb->bin_x[x + 1] - nb->bin_x[x]) * (int64_t)(nb->bin_y[y + 1] - nb->bin_y[y]);
+            float scaled = (float)area * td;
+            nb->density[k] = ((float)nb->inst[k] + (float)nb->filler[k] + (float)nb->non_place[k]) / scaled;
+            float o = (float)nb->inst[k] + (float)nb->non_place[k] - scaled;
+            o = o > 0.0f ? o : 0.0f;
+            nb->overflow = (int64_t)((float)nb->overflow + o);
+            float ou = (float)nb->inst[k] + (float)nb->non_place_unscaled[k] - scaled;
+            ou = ou > 0.0f ? ou : 0.0f;
+            nb->overflow_unscaled = (int64_t)((float)nb->overflow_unscaled + ou);
+        }
+}
+
+/* nesterov.rs:2268 updateGCellDensityCenterLocation then the bins. */
+static void update_center_location(orc_np* np, orc_nb* nb, const fpt* c) {
+    for (int i = 0; i < nb->n; ++i) set_density_center(np, nb->begin + i, (int)c[i].x, (int)c[i].y);
+    update_bins_density_area(np, nb);
+}
+
+/* nesterov.rs:2279 updateDensityFieldBin. */
+static void update_density_field_bin(orc_nb* nb) {
+    int nx = nb->R.cnt_x, ny = nb->R.cnt_y;
+    float *a = nb->fft_a, *ph = nb->fft_b, *fx = nb->fft_c, *fy = nb->fft_d;
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) a[(size_t)x * ny + y] = nb->density[(size_t)y * nx + x];
+    orc_fft_solve_f32(nx, ny, nb->wx, nb->wy, a, ph, fx, fy);
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            size_t s = (size_t)x * ny + y, d = (size_t)y * nx + x;
+            nb->field_x[d] = fx[s];
+            nb->field_y[d] = fy[s];
+            nb->phi[d] = ph[s];
+        }
+}
+
+/* nesterov.rs:2307 GNet::updateBox. */
+static void update_box(orc_np* np, int n) {
+    int lx = INT32_MAX, ly = INT32_MAX, ux = INT32_MIN, uy = INT32_MIN;
+    for (int k = np->net_start[n]; k < np->net_start[n + 1]; ++k) {
+        int p = np->net_pins[k];
+        if (np->pcx[p] < lx) lx = np->pcx[p];
+        if (np->pcy[p] < ly) ly = np->pcy[p];
+        if (np->pcx[p] > ux) ux = np->pcx[p];
+        if (np->pcy[p] > uy) uy = np->pcy[p];
+    }
+    np->nbox[4 * n] = lx;
+    np->nbox[4 * n + 1] = ly;
+    np->nbox[4 * n + 2] = ux;
+    np->nbox[4 * n + 3] = uy;
+}
+
+/* nesterov.rs:2297 getHpwl. */
+int64_t orc_np_hpwl(orc_np* np) {
+    int64_t total = 0;
+    for (int n = 0; n < np->nnets; ++n) {
+        update_box(np, n);
+        int32_t* b = &np->nbox[4 * n];
+        if (b[2] < b[0]) continue;
+        total += ((int64_t)b[2] - b[0]) + ((int64_t)b[3] - b[1]);
+    }
+    return total;
+}
+
+/* nesterov.rs:2325 updateWireLengthForceWA. */
+static void update_wirelength_force_wa(orc_np* np) {
+    float cx = np->wl_coef_x, cy = np->wl_coef_y;
+    memset(np->has, 0, np->npins);
+    memset(np->pexp, 0, sizeof(float) * 4 * (size_t)np->npins);
+    for (int n = 0; n < np->nnets; ++n) {
+        float* s = &np->nsum[8 * n];
+        for (int q = 0; q < 8; ++q) s[q] = 0.0f;
+        update_box(np, n);
+        int nlx = np->nbox[4 * n], nly = np->nbox[4 * n + 1], nux = np->nbox[4 * n + 2], nuy = np->nbox[4 * n + 3];
+        for (int k = np->net_start[n]; k < np->net_start[n + 1]; ++k) {
+            int p = np->net_pins[k], px = np->pcx[p], py = np->pcy[p];
+            float emin_x = (float)(nlx - px) * cx, emax_x = (float)(px - nux) * cx;
+            float emin_y = (float)(nly - py) * cy, emax_y = (float)(py - nuy) * cy;
+            if (emin_x > -300.0f) {
+                float v = orc_fast_exp(emin_x);
+                np->has[p] |= 1;
+                np->pexp[4 * p] = v;
+                s[0] += v;
+                s[1] += (float)px * v;
+            }
+            if (emax_x > -300.0f) {
+                float v = orc_fast_exp(emax_x);
+                np->has[p] |= 2;
+                np->pexp[4 * p + 1] = v;
+                s[2] += v;
+                s[3] += (float)px * v;
+            }
+            if (emin_y > -300.0f) {
+                float v = orc_fast_exp(emin_y);
+                np->has[p] |= 4;
+                np->pexp[4 * p + 2] = v;
+                s[4] += v;
+                s[5] += (float)py * v;
+            }
+            if (emax_y > -300.0f) {
+                float v = orc_fast_exp(emax_y);
+                np->has[p] |= 8;
+                np->pexp[4 * p + 3] = v;
+                s[6] += v;
+                s[7] += (float)py * v;
+            }
+        }
+    }
+}
+
+/* nesterov.rs:2392 getWireLengthGradientPinWA. */
+static fpt wl_gradient_pin(const orc_np* np, int p, int n) {
+    const float* g = &np->nsum[8 * n];
+    const float* q = &np->pexp[4 * p];
+    float cx = np->wl_coef_x, cy = np->wl_coef_y;
+    float gmin_x = 0.0f, gmax_x = 0.0f, gmin_y = 0.0f, gmax_y = 0.0f;
+    float qcx = (float)np->pcx[p], qcy = (float)np->pcy[p];
+    if (np->has[p] & 1) {
+        double num = (double)g[0] * ((double)q[0] * (1.0 - (double)(cx * qcx))) + (double)(cx * q[0] * g[1]);
+        gmin_x = (float)(num / (double)(g[0] * g[0]));
+    }
+    if (np->has[p] & 2) {
+        double num = (double)g[2] * ((double)q[1] * (1.0 + (double)(cx * qcx))) - (double)(cx * q[1] * g[3]);
+        gmax_x = (float)(num / (double)(g[2] * g[2]));
+    }
+    if (np->has[p] & 4) {
+        double num = (double)g[4] * ((double)q[2] * (1.0 - (double)(cy * qcy))) + (double)(cy * q[2] * g[5]);
+        gmin_y = (float)(num / (double)(g[4] * g[4]));
+    }
+    if (np->has[p] & 8) {
+        double num = (double)g[6] * ((double)q[3] * (1.0 + (double)(cy * qcy))) - (double)(cy * q[3] * g[7]);
+        gmax_y = (float)(num / (double)(g[6] * g[6]));
+    }
+    fpt r = {gmin_x - gmax_x, gmin_y - gmax_y};
+    return r;
+}
+
+/* nesterov.rs:2418 getWireLengthGradientWA. */
+static fpt wl_gradient(const orc_np* np, int g) {
+    fpt s = {0.0f, 0.0f};
+    for (int k = np->pin_start[g]; k < np->pin_start[g + 1]; ++k) {
+        int p = np->pins[k], n = np->pnet[p];
+        fpt t = wl_gradient_pin(np, p, n);
+        float w = np->nw[n];
+        t.x *= w;
+        t.y *= w;
+        s.x += t.x;
+        s.y += t.y;
+    }
+    return s;
+}
+
+/* nesterov.rs:2434 getDensityGradient: x outer, y inner. */
+static fpt density_gradient(const orc_np* np, const orc_nb* nb, int g) {
+    int x0, x1, y0, y1;
+    didx_x(nb, np->dlx[g], np->dux[g], &x0, &x1);
+    didx_y(nb, np->dly[g], np->duy[g], &y0, &y1);
+    fpt f = {0.0f, 0.0f};
+    for (int x = x0; x < x1; ++x)
+        for (int y = y0; y < y1; ++y) {
+            size_t k = (size_t)y * nb->R.cnt_x + x;
+            float a = overlap_density_area(nb->bin_x[x], nb->bin_y[y], nb->bin_x[x + 1], nb->bin_y[y + 1], np->dlx[g], np->dly[g], np->dux[g], np->duy[g]) * np->dscale[g];
+            f.x += a * nb->field_x[k];
+            f.y += a * nb->field_y[k];
+        }
+    return f;
+}
+
+enum { SLOT_PREV, SLOT_CUR, SLOT_NEXT };
+
+/* nesterov.rs:2455 updateGradients into a slot. */
+static void update_gradients(orc_np* np, orc_nb* nb, int slot) {
+    if (nb->is_converged) return;
+    int base = slot == SLOT_PREV ? S_PREV_SLP : slot == SLOT_CUR ? S_CUR_SLP : S_NEXT_SLP;
+    fpt *wl = nb->v[base + 1], *den = nb->v[base + 2], *sum = nb->v[base + 3];
+    for (int i = 0; i < nb->n; ++i) wl[i] = wl_gradient(np, nb->begin + i);
+    for (int i = 0; i < nb->n; ++i) den[i] = density_gradient(np, nb, nb->begin + i);
+    float p = nb->density_penalty;
+    for (int i = 0; i < nb->n; ++i) {
+        int g = nb->begin + i;
+        sum[i].x = wl[i].x + p * den[i].x;
+        sum[i].y = wl[i].y + p * den[i].y;
+        float wl_pre = (float)(np->pin_start[g + 1] - np->pin_start[g]);
+        float area = (float)(np->ux[g] - np->lx[g]) * (float)(np->uy[g] - np->ly[g]);
+        float pre = wl_pre + p * area;
+        if (!(pre >= 1.0f)) pre = pre > 1.0f ? pre : 1.0f;
+        sum[i].x /= pre;
+        sum[i].y /= pre;
+    }
+    nb->wl_grad_sum = 0.0f;
+    nb->density_grad_sum = 0.0f;
+    for (int i = 0; i < nb->n; ++i) {
+        nb->wl_grad_sum += fabsf(wl[i].x);
+        nb->wl_grad_sum += fabsf(wl[i].y);
+        nb->density_grad_sum += fabsf(den[i].x);
+        nb->density_grad_sum += fabsf(den[i].y);
+    }
+}
+
+static int bad_f(float v) { return isnan(v) || isinf(v); }
+
+/* nesterov.rs:2512 npUpdate*Gradient. */
+static void np_update_gradient(orc_np* np, int b, int slot) {
+    orc_nb* nb = &np->nbs[b];
+    update_gradients(np, nb, slot);
+    if (np->inject_nan_after >= 0 && np->iter_done >= np->inject_nan_after && slot == SLOT_NEXT) nb->wl_grad_sum = NAN;
+    if (nb->wl_grad_sum == 0.0f && np->recursion_wl < 10) {
+        np->wl_coef_x = (float)((double)np->wl_coef_x * 0.5);
+        np->wl_coef_y = (float)((double)np->wl_coef_y * 0.5);
+        np->base_wl_coef = (float)((double)np->base_wl_coef * 0.5);
+        update_wirelength_force_wa(np);
+        np->recursion_wl += 1;
+        np_update_gradient(np, b, slot);
+        return;
+    }
+    if (bad_f(nb->wl_grad_sum) || bad_f(nb->density_grad_sum)) {
+        np->diverge_code = 306;
+        np->num_region_diverged = 1;
+    }
+}
+
+/* nesterov.rs:2532 updateWireLengthCoef. */
+static void update_wirelength_coef(orc_np* np, float overflow) {
+    float c = orc_wirelength_coef_factor(overflow);
+    np->wl_coef_x = c * np->base_wl_coef;
+    np->wl_coef_y = c * np->base_wl_coef;
+}
+
+/* nesterov.rs:2546 initDensity1. */
+static void init_density1(orc_np* np, int b) {
+    orc_nb* nb = &np->nbs[b];
+    for (int i = 0; i < nb->n; ++i) {
+        int g = nb->begin + i;
+        /* updateDensityCoordiLayoutInside (nesterov.rs:2173) */
+        float tlx = (float)np->dlx[g] > (float)nb->R.lx ? (float)np->dlx[g] : (float)nb->R.lx;
+        float tly = (float)np->dly[g] > (float)nb->R.ly ? (float)np->dly[g] : (float)nb->R.ly;
+        int ddx = np->dux[g] - np->dlx[g], ddy = np->duy[g] - np->dly[g];
+        if (tlx + (float)ddx > (float)nb->R.ux) tlx = (float)(nb->R.ux - ddx);
+        if (tly + (float)ddy > (float)nb->R.uy) tly = (float)(nb->R.uy - ddy);
+        set_density_location(np, g, (int)tlx, (int)tly);
+        fpt p = {(float)d_cx(np, g), (float)d_cy(np, g)};
+        nb->v[S_CUR_SLP][i] = p;
+        nb->v[S_PREV_SLP][i] = p;
+        nb->v[S_CUR][i] = p;
+    }
+    update_center_location(np, nb, nb->v[S_CUR_SLP]);
+    nb->prev_hpwl = orc_np_hpwl(np);
+    update_density_field_bin(nb);
+    nb->base_wl_coef_nb = (float)((double)np->o.init_wirelength_coef / ((double)(float)(nb->R.size_x + nb->R.size_y) * 0.5));
+    float nia = (float)nesterov_insts_area(nb);
+    nb->sum_overflow = (float)nb->overflow / nia;
+    nb->sum_overflow_unscaled = (float)nb->overflow_unscaled / nia;
+}
+
+/* nesterov.rs:2583 updateInitialPrevSLPCoordi. */
+static void update_initial_prev_slp(orc_np* np, int b) {
+    orc_nb* nb = &np->nbs[b];
+    float c = np->initial_prev_coordi_update_coef;
+    for (int i = 0; i < nb->n; ++i) {
+        int g = nb->begin + i;
+        if (np->flags[g] & GPL_CELL_LOCKED) {
+            nb->v[S_PREV_SLP][i] = nb->v[S_CUR_SLP][i];
+            continue;
+        }
+        float px = nb->v[S_CUR_SLP][i].x - c * nb->v[S_CUR_SLP_SUM][i].x;
+        float py = nb->v[S_CUR_SLP][i].y - c * nb->v[S_CUR_SLP_SUM][i].y;
+        nb->v[S_PREV_SLP][i].x = inside_x(np, nb, g, px);
+        nb->v[S_PREV_SLP][i].y = inside_y(np, nb, g, py);
+    }
+}
+
+/* nesterov.rs:2600 initDensity2. */
+static float init_density2(orc_np* np, int b) {
+    orc_nb* nb = &np->nbs[b];
+    if (nb->wl_grad_sum == 0.0f) {
+        nb->density_penalty = np->o.init_density_penalty;
+        update_gradients(np, nb, SLOT_PREV);
+    }
+    if (nb->wl_grad_sum != 0.0f) nb->density_penalty = (nb->wl_grad_sum / nb->density_grad_sum) * np->o.init_density_penalty;
+    float nia = (float)nesterov_insts_area(nb);
+    nb->sum_overflow = (float)nb->overflow / nia;
+    nb->sum_overflow_unscaled = (float)nb->overflow_unscaled / nia;
+    nb->coordi_distance = orc_distance((const float*)nb->v[S_PREV_SLP], (const float*)nb->v[S_CUR_SLP], nb->n);
+    float grad = orc_distance((const float*)nb->v[S_PREV_SLP_SUM], (const float*)nb->v[S_CUR_SLP_SUM], nb->n);
+    nb->step_length = nb->coordi_distance / grad;
+    return nb->step_length;
+}
+
+/* nesterov.rs:2674 NesterovPlace::init. */
+static int32_t np_init(orc_np* np) {
+    float total_sum_overflow = 0.0f, total_base = 0.0f;
+    for (int b = 0; b < np->nregions; ++b) {
+        init_density1(np, b);
+        total_sum_overflow += np->nbs[b].sum_overflow;
+        total_base += np->nbs[b].base_wl_coef_nb;
+    }
+    float n = (float)np->nregions;
+    np->average_overflow = total_sum_overflow / n;
+    np->base_wl_coef = total_base / n;
+    update_wirelength_coef(np, np->average_overflow);
+    update_wirelength_force_wa(np);
+    for (int b = 0; b < np->nregions; ++b) {
+        orc_nb* nb = &np->nbs[b];
+        np_update_gradient(np, b, SLOT_CUR);
+        update_initial_prev_slp(np, b);
+        update_center_location(np, nb, nb->v[S_PREV_SLP]);
+        update_density_field_bin(nb);
+    }
+    update_wirelength_force_wa(np);
+    for (int b = 0; b < np->nregions; ++b) np_update_gradient(np, b, SLOT_PREV);
+    for (int b = 0; b < np->nregions; ++b) {
+        float step = init_density2(np, b);
+        if (bad_f(step) && np->recursion_init_slp < 10) {
+            np->initial_prev_coordi_update_coef *= 10.0f;
+            np->recursion_init_slp += 1;
+            return np_init(np);
+        }
+        if (bad_f(step)) return 304;
+    }
+    return 0;
+}
+
+/* nesterov.rs:4123 init_nesterov_loop. */
+int32_t orc_init(orc_np* np) {
+    double t0 = now_s();
+    int64_t design_hpwl = orc_np_hpwl(np);
+    np->reference_hpwl = orc_derived_reference_hpwl(np->o.reference_hpwl, design_hpwl);
+    int32_t rc = np_init(np);
+    np->t_init = now_s() - t0;
+    if (rc) {
+        np->num_region_diverged = 1;
+        np->diverge_code = rc;
+    }
+    return rc;
+}
+
+/* nesterov.rs:2718 nesterovUpdateCoordinates. */
+static void nesterov_update_coordinates(orc_np* np, int b, float coeff) {
+    orc_nb* nb = &np->nbs[b];
+    if (nb->is_converged) return;
+    float s = nb->step_length;
+    fpt *cur_slp = nb->v[S_CUR_SLP], *sum = nb->v[S_CUR_SLP_SUM], *cur = nb->v[S_CUR];
+    fpt *next = nb->v[S_NEXT], *next_slp = nb->v[S_NEXT_SLP];
+    for (int k = 0; k < nb->n; ++k) {
+        int g = nb->begin + k;
+        if (np->flags[g] & GPL_CELL_LOCKED) {
+            next[k] = cur[k];
+            next_slp[k] = cur_slp[k];
+            continue;
+        }
+        float nx = cur_slp[k].x + s * sum[k].x;
+        float ny = cur_slp[k].y + s * sum[k].y;
+        float sx = nx + coeff * (nx - cur[k].x);
+        float sy = ny + coeff * (ny - cur[k].y);
+        next[k].x = inside_x(np, nb, g, nx);
+        next[k].y = inside_y(np, nb, g, ny);
+        next_slp[k].x = inside_x(np, nb, g, sx);
+        next_slp[k].y = inside_y(np, nb, g, sy);
+    }
+    update_center_location(np, nb, next_slp);
+    update_density_field_bin(nb);
+}
+
+/* nesterov.rs:2750 nesterovUpdateStepLength. */
+static int nesterov_update_step_length(orc_np* np, orc_nb* nb) {
+    if (nb->is_converged) return 1;
+    nb->coordi_distance = orc_distance((const float*)nb->v[S_CUR_SLP], (const float*)nb->v[S_NEXT_SLP], nb->n);
+    float grad = orc_distance((const float*)nb->v[S_CUR_SLP_SUM], (const float*)nb->v[S_NEXT_SLP_SUM], nb->n);
+    float nw = nb->coordi_distance / grad;
+    if (bad_f(nw)) {
+        nb->is_diverged = 1;
+        return 0;
+    }
+    if ((double)nw > (double)nb->step_length * 0.95) {
+        nb->step_length = nw;
+        return 0;
+    }
+    if ((double)nw < 0.01) {
+        nb->step_length = (float)0.01;
+        return 0;
+    }
+    nb->step_length = nw;
+    return 1;
+}
+
+/* nesterov.rs:2776 doBackTracking. */
+static void do_back_tracking(orc_np* np, float coeff) {
+    np->trials = 0;
+    for (int nbt = 0; nbt < 10; ++nbt) {
+        np->trials++;
+        for (int b = 0; b < np->nregions; ++b) nesterov_update_coordinates(np, b, coeff);
+        update_wirelength_force_wa(np);
+        np->num_region_diverged = 0;
+        for (int b = 0; b < np->nregions; ++b) {
+            np_update_gradient(np, b, SLOT_NEXT);
+            np->num_region_diverged += np->nbs[b].is_diverged;
+        }
+        if (np->num_region_diverged > 0) {
+            np->diverge_code = 306;
+            break;
+        }
+        int ok = 0;
+        np->num_region_diverged = 0;
+        for (int b = 0; b < np->nregions; ++b) {
+            ok += nesterov_update_step_length(np, &np->nbs[b]);
+            np->num_region_diverged += np->nbs[b].is_diverged;
+        }
+        if (np->num_region_diverged > 0) {
+            np->diverge_code = 305;
+            break;
+        }
+        if (ok != np->nregions) break;
+    }
+}
+
+/* nesterov.rs:2812 nesterovAdjustPhi. */
+static void nesterov_adjust_phi(orc_nb* nb) {
+    if (nb->is_converged) return;
+    if (!nb->max_phi_coef_changed && nb->sum_overflow_unscaled < 0.35f) {
+        nb->max_phi_coef_changed = 1;
+        nb->max_phi_coef = (float)((double)nb->max_phi_coef * 0.99);
+    }
+    if (nb->max_phi_coef <= 1.0f) nb->max_phi_coef = 1.01f;
+}
+
+static void swap_slots(orc_nb* nb, int a, int b) {
+    fpt* t = nb->v[a];
+    nb->v[a] = nb->v[b];
+    nb->v[b] = t;
+}
+
+/* nesterov.rs:2837 NesterovBase::updateNextIter. */
+static void nb_update_next_iter(orc_np* np, int b, int iter) {
+    orc_nb* nb = &np->nbs[b];
+    if (nb->is_converged) return;
+    for (int q = 0; q < 4; ++q) swap_slots(nb, S_PREV_SLP + q, S_CUR_SLP + q);
+    for (int k = 0; k < nb->n; ++k) {
+        int g = nb->begin + k;
+        if ((np->flags[g] & GPL_CELL_INSTANCE) && (np->flags[g] & GPL_CELL_LOCKED)) {
+            for (int q = 0; q < 4; ++q) nb->v[S_NEXT_SLP + q][k] = nb->v[S_CUR_SLP + q][k];
+            nb->v[S_NEXT][k] = nb->v[S_CUR][k];
+        }
+    }
+    for (int q = 0; q < 4; ++q) swap_slots(nb, S_CUR_SLP + q, S_NEXT_SLP + q);
+    swap_slots(nb, S_CUR, S_NEXT);
+    float fraction = (float)iter / (float)np->max_iter;
+    float a = (float)nesterov_insts_area(nb), c = fraction * (float)nb->R.non_place_insts_area * 0.05f;
+    float denom = a > c ? a : c;
+    nb->sum_overflow = (float)nb->overflow / denom;
+    nb->sum_overflow_unscaled = (float)nb->overflow_unscaled / denom;
+    int64_t hpwl = orc_np_hpwl(np);
+    if (iter == 0 || iter % 10 == 0) {
+        nb->prev_reported_hpwl = hpwl;
+        nb->prev_reported_overflow_unscaled = nb->sum_overflow_unscaled;
+    }
+    float scaled = (float)(hpwl - nb->prev_hpwl) / np->reference_hpwl;
+    float phi = orc_phi_coef(nb->max_phi_coef, np->o.min_phi_coef, scaled);
+    nb->density_penalty *= phi;
+    nb->prev_hpwl = hpwl;
+    if (nb->coordi_distance > nb->peak_coordi_distance) nb->peak_coordi_distance = nb->coordi_distance;
+    if (iter > 50 && nb->min_sum_overflow > nb->sum_overflow_unscaled) {
+        nb->min_sum_overflow = nb->sum_overflow_unscaled;
+        nb->hpwl_with_min_sum_overflow = (float)nb->prev_hpwl;
+    }
+}
+
+/* nesterov.rs:2907 NesterovPlace::updateNextIter. */
+static void np_update_next_iter(orc_np* np, int iter) {
+    float total = 0.0f, total_unscaled = 0.0f;
+    for (int b = 0; b < np->nregions; ++b) {
+        nb_update_next_iter(np, b, iter);
+        total += np->nbs[b].sum_overflow;
+        total_unscaled += np->nbs[b].sum_overflow_unscaled;
+    }
+    float n = (float)np->nregions;
+    np->average_overflow = total / n;
+    np->average_overflow_unscaled = total_unscaled / n;
+    update_wirelength_coef(np, np->average_overflow);
+    if (!np->o.disable_revert_if_diverge) {
+        int64_t hpwl = orc_np_hpwl(np);
+        if (hpwl < np->min_hpwl && (double)np->average_overflow_unscaled <= 0.25) {
+            np->min_hpwl = hpwl;
+            np->diverge_snapshot_overflow = np->average_overflow_unscaled;
+            np->diverge_snapshot_iter = iter + 1;
+            np->is_min_hpwl = 1;
+        } else {
+            np->is_min_hpwl = 0;
+        }
+    }
+}
+
+/* nesterov.rs:2933 saveSnapshot / 2950 revertToSnapshot. */
+static void save_snapshot(orc_nb* nb) {
+    if (nb->is_converged) return;
+    memcpy(nb->v[S_SNAP], nb->v[S_CUR], sizeof(fpt) * nb->n);
+    memcpy(nb->v[S_SNAP_SLP], nb->v[S_CUR_SLP], sizeof(fpt) * nb->n);
+    memcpy(nb->v[S_SNAP_SLP_SUM], nb->v[S_CUR_SLP_SUM], sizeof(fpt) * nb->n);
+    memcpy(nb->v[S_SNAP_PREV_SLP_SUM], nb->v[S_PREV_SLP_SUM], sizeof(fpt) * nb->n);
+    nb->snapshot_density_penalty = nb->density_penalty;
+    nb->snapshot_step_length = nb->step_length;
+}
+
+static void revert_to_snapshot(orc_np* np, orc_nb* nb) {
+    if (nb->is_converged) return;
+    memcpy(nb->v[S_CUR], nb->v[S_SNAP], sizeof(fpt) * nb->n);
+    memcpy(nb->v[S_CUR_SLP], nb->v[S_SNAP_SLP], sizeof(fpt) * nb->n);
+    memcpy(nb->v[S_CUR_SLP_SUM], nb->v[S_SNAP_SLP_SUM], sizeof(fpt) * nb->n);
+    memcpy(nb->v[S_PREV_SLP_SUM], nb->v[S_SNAP_PREV_SLP_SUM], sizeof(fpt) * nb->n);
+    nb->density_penalty = nb->snapshot_density_penalty;
+    nb->step_length = nb->snapshot_step_length;
+    update_center_location(np, nb, nb->v[S_CUR]);
+    update_density_field_bin(nb);
+    nb->is_diverged = 0;
+    nb->prev_reported_hpwl = 0;
+    nb->prev_reported_overflow_unscaled = 0.0f;
+}
+
+/* nesterov.rs:2976 checkDivergence. */
+static int check_divergence(orc_nb* nb) {
+    if (nb->sum_overflow_unscaled < 0.2f && nb->sum_overflow_unscaled - nb->min_sum_overflow >= 0.02f &&
+        nb->hpwl_with_min_sum_overflow * 1.2f < (float)nb->prev_hpwl)
+        nb->is_diverged = 1;
+    if (nb->prev_reported_overflow_unscaled > 0.0f && nb->prev_reported_hpwl > 0) {
+        float change = nb->sum_overflow_unscaled - nb->prev_reported_overflow_unscaled;
+        float increase = (float)(nb->prev_hpwl - nb->prev_reported_hpwl) / (float)nb->prev_reported_hpwl;
+        if (change >= 0.05f && increase >= 0.25f) nb->is_diverged = 1;
+    }
+    return nb->is_diverged;
+}
+
+static void revert_to_diverge_snapshot(orc_np* np) {
+    np->wl_coef_x = np->diverge_coef_x;
+    np->wl_coef_y = np->diverge_coef_y;
+    update_wirelength_force_wa(np);
+    for (int b = 0; b < np->nregions; ++b) {
+        revert_to_snapshot(np, &np->nbs[b]);
+        np->nbs[b].min_sum_overflow = 1e30f;
+        np->nbs[b].hpwl_with_min_sum_overflow = 1e30f;
+    }
+}
+
+/* nesterov.rs:3009 isDiverged (routability off): 0 none, 1 resume, 2 stop. */
+static int np_is_diverged(orc_np* np, float* cur_a) {
+    np->num_region_diverged = 0;
+    for (int b = 0; b < np->nregions; ++b) np->num_region_diverged += check_divergence(&np->nbs[b]);
+    if (np->num_region_diverged == 0 && np->is_min_hpwl && !np->o.disable_revert_if_diverge) {
+        np->diverge_coef_x = np->wl_coef_x;
+        np->diverge_coef_y = np->wl_coef_y;
+        np->diverge_hpwl = np->min_hpwl;
+        for (int b = 0; b < np->nregions; ++b) save_snapshot(&np->nbs[b]);
+        np->diverge_saved = 1;
+    }
+    if (np->num_region_diverged == 0) return 0;
+    switch (orc_diverge_fallback(np->diverge_saved, np->diverge_reverts, np->o.disable_revert_if_diverge)) {
+    case 0:
+        np->diverge_reverts += 1;
+        revert_to_diverge_snapshot(np);
+        *cur_a = 1.0f;
+        np->diverge_saved = 0;
+        np->num_region_diverged = 0;
+        return 1;
+    case 1:
+        revert_to_diverge_snapshot(np);
+        np->num_region_diverged = 0;
+        return 2;
+    default:
+        np->diverge_code = 307;
+        return 2;
+    }
+}
+
+/* nesterov.rs:3107 checkConvergence (report lines aside). */
+static int check_convergence(orc_np* np, int b) {
+    orc_nb* nb = &np->nbs[b];
+    if (nb->is_converged) return 1;
+    if (nb->sum_overflow_unscaled > np->o.overflow) return 0;
+    for (int k = 0; k < nb->n; ++k) {
+        int g = nb->begin + k;
+        if (np->flags[g] & GPL_CELL_INSTANCE) np->flags[g] |= GPL_CELL_LOCKED;
+    }
+    nb->is_converged = 1;
+    return 1;
+}
+
+static void push_history(orc_np* np, int iter) {
+    if (np->nhist == np->cap_hist) {
+        np->cap_hist = np->cap_hist ? 2 * np->cap_hist : 1024;
+        np->hist = (gpl_iter_row*)realloc(np->hist, sizeof(gpl_iter_row) * np->cap_hist);
+    }
+    gpl_iter_row* r = &np->hist[np->nhist++];
+    r->iter = iter;
+    r->trials = np->trials;
+    r->overflow = np->average_overflow_unscaled;
+    r->density_penalty = np->nbs[0].density_penalty;
+    r->step_length = np->nbs[0].step_length;
+    r->wl_coef = np->wl_coef_x;
+    r->hpwl = np->nbs[0].prev_hpwl;
+}
+
+/* nesterov.rs:4440 resume_nesterov_place (no routability, timing or penalty guard).  Runs at most
+ * `iter_budget` iterations (-1: no limit) and stops early once `time_budget_s` of wall time has
+ * passed (<= 0: no limit) -- the bench's bounded CPU sample.  Returns the status; *iter_out is
+ * the final iteration. */
+int32_t orc_run(orc_np* np, int32_t start_iter, int32_t iter_budget, double time_budget_s, int32_t* iter_out) {
+    double t0 = now_s();
+    if (np->num_region_diverged > 0) return np->diverge_code;
+    for (int b = 0; b < np->nregions; ++b) {
+        orc_nb* nb = &np->nbs[b];
+        nb->max_phi_coef_changed = 0;
+        nb->min_sum_overflow = 1e30f;
+        nb->hpwl_with_min_sum_overflow = 1e30f;
+        nb->is_converged = 0;
+    }
+    np->diverge_saved = 0;
+    np->diverge_reverts = 0;
+    np->diverge_hpwl = 0;
+    float cur_a = 1.0f;
+    int iter = start_iter, ran = 0;
+    np->converged = 0;
+    while (iter < np->max_iter) {
+        if (iter_budget >= 0 && ran >= iter_budget) break;
+        if (time_budget_s > 0.0 && now_s() - t0 > time_budget_s) break;
+        float prev_a = cur_a;
+        cur_a = (float)((1.0 + sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
+        float coeff = (float)(((double)prev_a - 1.0) / (double)cur_a);
+        do_back_tracking(np, coeff);
+        np->total_trials += np->trials;
+        for (int b = 0; b < np->nregions; ++b) nesterov_adjust_phi(&np->nbs[b]);
+        if (np->num_region_diverged > 0) break;
+        np_update_next_iter(np, iter);
+        push_history(np, iter);
+        ran++;
+        np->iter_done = ran;
+        int act = np_is_diverged(np, &cur_a);
+        if (act == 2) break;
+        if (act == 1) {
+            iter++;
+            continue;
+        }
+        int conv = 0;
+        for (int b = 0; b < np->nregions; ++b) conv += check_convergence(np, b);
+        if (conv == np->nregions) {
+            np->converged = 1;
+            np->t_converge = np->t_init + (now_s() - t0);
+            break;
+        }
+        iter++;
+    }
+    if (orc_max_iter_takes_snapshot(iter, np->max_iter, np->o.disable_revert_if_diverge, np->diverge_saved, np->diverge_reverts,
+                                    np->diverge_hpwl, np->diverge_snapshot_overflow, np->average_overflow_unscaled, orc_np_hpwl(np)))
+        revert_to_diverge_snapshot(np);
+    np->t_loop = now_s() - t0;
+    if (iter_out) *iter_out = iter;
+    if (np->num_region_diverged > 0) return np->diverge_code;
+    return 0;
+}
+
+/* Loop statistics, in the C ABI's layout. */
+void orc_stats(orc_np* np, gpl_stats* s) {
+    memset(s, 0, sizeof *s);
+    s->iter = np->nhist ? np->hist[np->nhist - 1].iter : -1;
+    s->converged = np->converged;
+    s->status = np->num_region_diverged > 0 ? np->diverge_code : 0;
+    s->total_trials = np->total_trials;
+    s->hpwl = orc_np_hpwl(np);
+    s->overflow = np->average_overflow_unscaled;
+    s->overflow_scaled = np->average_overflow;
+    s->density_penalty = np->nbs[0].density_penalty;
+    s->step_length = np->nbs[0].step_length;
+    s->wl_coef = np->wl_coef_x;
+    s->base_wl_coef = np->base_wl_coef;
+    s->reference_hpwl = np->reference_hpwl;
+    s->wl_grad_sum = np->nbs[0].wl_grad_sum;
+    s->density_grad_sum = np->nbs[0].density_grad_sum;
+    s->num_diverge_reverts = np->diverge_reverts;
+    s->init_seconds = np->t_init;
+    s->loop_seconds = np->t_loop;
+    s->first_below_target_seconds = np->t_converge;
+}
+
+int32_t orc_history(orc_np* np, gpl_iter_row* rows, int32_t cap) {
+    int32_t n = np->nhist < cap ? np->nhist : cap;
+    memcpy(rows, np->hist, sizeof(gpl_iter_row) * n);
+    return np->nhist;
+}
+
+/* Region 0's per-cell vectors by slot (the C ABI's GPL_SLOT_* numbering) and bins. */
+void orc_get_slot(orc_np* np, int32_t b, int32_t slot, float* xy) {
+    static const int map[] = {S_CUR_SLP, S_NEXT_SLP, S_PREV_SLP, S_CUR, S_NEXT};
+    memcpy(xy, np->nbs[b].v[map[slot]], sizeof(fpt) * np->nbs[b].n);
+}
+void orc_get_grad(orc_np* np, int32_t b, int32_t which, float* xy) {
+    /* 0 cur_slp_wl, 1 cur_slp_den, 2 cur_slp_sum, 3 prev_slp_sum, 4 next_slp_sum */
+    static const int map[] = {S_CUR_SLP_WL, S_CUR_SLP_DEN, S_CUR_SLP_SUM, S_PREV_SLP_SUM, S_NEXT_SLP_SUM};
+    memcpy(xy, np->nbs[b].v[map[which]], sizeof(fpt) * np->nbs[b].n);
+}
+void orc_get_bins(orc_np* np, int32_t b, float* density, float* field_x, float* field_y, int64_t* ovf2) {
+    orc_nb* nb = &np->nbs[b];
+    size_t M = (size_t)nb->R.cnt_x * nb->R.cnt_y;
+    if (density) memcpy(density, nb->density, 4 * M);
+    if (field_x) memcpy(field_x, nb->field_x, 4 * M);
+    if (field_y) memcpy(field_y, nb->field_y, 4 * M);
+    if (ovf2) {
+        ovf2[0] = nb->overflow;
+        ovf2[1] = nb->overflow_unscaled;
+    }
+}
+void orc_get_scalars(orc_np* np, int32_t b, float* out) {
+    orc_nb* nb = &np->nbs[b];
+    out[0] = nb->step_length;
+    out[1] = nb->density_penalty;
+    out[2] = nb->wl_grad_sum;
+    out[3] = nb->density_grad_sum;
+    out[4] = nb->sum_overflow;
+    out[5] = nb->sum_overflow_unscaled;
+    out[6] = np->wl_coef_x;
+    out[7] = np->base_wl_coef;
+    out[8] = np->reference_hpwl;
+    out[9] = nb->coordi_distance;
+}
+void orc_set_inject(orc_np* np, int32_t after_iter) { np->inject_nan_after = after_iter; }
+
+/* ---- per-kernel oracles on a caller-given state --------------------------------------------- */
+
+/* The scatter of a region's cells at coordinates xy (density centres truncated, as
+ * update_gcell_density_center_location does), with the integer bin areas, the float density and
+ * the two overflows: the reference's f32-through-i64 sums (ovf_i64[2]) and exact FP64 sums of
+ * the same per-bin terms (ovf_f64[2]).  The cells' density sizes come from the problem (after
+ * the first centring the boxes are 2*(d/2) wide). */
+void orc_scatter(orc_np* np, int32_t b, const float* xy, int64_t* inst, int64_t* filler, float* density, int64_t* ovf_i64, double* ovf_f64) {
+    orc_nb* nb = &np->nbs[b];
+    update_center_location(np, nb, (const fpt*)xy);
+    size_t M = (size_t)nb->R.cnt_x * nb->R.cnt_y;
+    if (inst) memcpy(inst, nb->inst, 8 * M);
+    if (filler) memcpy(filler, nb->filler, 8 * M);
+    if (density) memcpy(density, nb->density, 4 * M);
+    if (ovf_i64) {
+        ovf_i64[0] = nb->overflow;
+        ovf_i64[1] = nb->overflow_unscaled;
+    }
+    if (ovf_f64) {
+        double o = 0.0, ou = 0.0;
+        float td = nb->R.target_density;
+        for (int y = 0; y < nb->R.cnt_y; ++y)
+            for (int x = 0; x < nb->R.cnt_x; ++x) {
+                size_t k = (size_t)y * nb->R.cnt_x + x;
+                int64_t area = (int64_t)(nb->bin_x[x + 1] - nb->bin_x[x]) * (int64_t)(nb->bin_y[y + 1] - nb->bin_y[y]);
+                float scaled = (float)area * td;
+                float t = (float)nb->inst[k] + (float)nb->non_place[k] - scaled;
+                float tu = (float)nb->inst[k] + (float)nb->non_place_unscaled[k] - scaled;
+                o += t > 0.0f ? (double)t : 0.0;
+                ou += tu > 0.0f ? (double)tu : 0.0;
+            }
+        ovf_f64[0] = o;
+        ovf_f64[1] = ou;
+    }
+}
+
+/* Sets the pins of a region's cells from density centres (ints), without touching the bins. */
+void orc_place_pins(orc_np* np, int32_t b, const float* xy) {
+    orc_nb* nb = &np->nbs[b];
+    for (int i = 0; i < nb->n; ++i) set_density_center(np, nb->begin + i, (int)xy[2 * i], (int)xy[2 * i + 1]);
+}
+
+/* WA forward at the current pins: faithful float net sums and boxes, FP64 net sums, HPWL. */
+void orc_wa_forward(orc_np* np, float coef_x, float coef_y, float* sums, int32_t* box, double* sums_f64, int64_t* hpwl) {
+    np->wl_coef_x = coef_x;
+    np->wl_coef_y = coef_y;
+    update_wirelength_force_wa(np);
+    if (sums) memcpy(sums, np->nsum, sizeof(float) * 8 * (size_t)np->nnets);
+    if (box) memcpy(box, np->nbox, sizeof(int32_t) * 4 * (size_t)np->nnets);
+    if (sums_f64) {
+        for (int n = 0; n < np->nnets; ++n) {
+            double s[8] = {0};
+            int nlx = np->nbox[4 * n], nly = np->nbox[4 * n + 1], nux = np->nbox[4 * n + 2], nuy = np->nbox[4 * n + 3];
+            for (int k = np->net_start[n]; k < np->net_start[n + 1]; ++k) {
+                int p = np->net_pins[k], px = np->pcx[p], py = np->pcy[p];
+                float e[4] = {(float)(nlx - px) * coef_x, (float)(px - nux) * coef_x, (float)(nly - py) * coef_y, (float)(py - nuy) * coef_y};
+                for (int q = 0; q < 4; ++q)
+                    if (e[q] > -300.0f) {
+                        double v = (double)orc_fast_exp(e[q]);
+                        s[2 * q] += v;
+                        s[2 * q + 1] += (double)(q < 2 ? px : py) * v;
+                    }
+            }
+            for (int q = 0; q < 8; ++q) sums_f64[8 * n + q] = s[q];
+        }
+    }
+    if (hpwl) {
+        int64_t t = 0;
+        for (int n = 0; n < np->nnets; ++n) {
+            int32_t* bb = &np->nbox[4 * n];
+            if (bb[2] >= bb[0]) t += ((int64_t)bb[2] - bb[0]) + ((int64_t)bb[3] - bb[1]);
+        }
+        *hpwl = t;
+    }
+}
+
+/* WA backward of a region's cells after orc_wa_forward: faithful float, and FP64 from the FP64
+ * net sums (the same quotient and per-pin exponentials, accumulated in double). */
+void orc_wa_backward(orc_np* np, int32_t b, float* wl, const double* sums_f64, double* wl_f64) {
+    orc_nb* nb = &np->nbs[b];
+    for (int i = 0; i < nb->n; ++i) {
+        int g = nb->begin + i;
+        fpt v = wl_gradient(np, g);
+        if (wl) {
+            wl[2 * i] = v.x;
+            wl[2 * i + 1] = v.y;
+        }
+        if (wl_f64 && sums_f64) {
+            double sx = 0.0, sy = 0.0;
+            for (int k = np->pin_start[g]; k < np->pin_start[g + 1]; ++k) {
+                int p = np->pins[k], n = np->pnet[p];
+                const double* G = &sums_f64[8 * n];
+                const float* q = &np->pexp[4 * p];
+                double cx = np->wl_coef_x, cy = np->wl_coef_y, px = np->pcx[p], py = np->pcy[p];
+                double g0 = 0, g1 = 0, g2 = 0, g3 = 0;
+                if (np->has[p] & 1) g0 = (G[0] * q[0] * (1.0 - cx * px) + cx * q[0] * G[1]) / (G[0] * G[0]);
+                if (np->has[p] & 2) g1 = (G[2] * q[1] * (1.0 + cx * px) - cx * q[1] * G[3]) / (G[2] * G[2]);
+                if (np->has[p] & 4) g2 = (G[4] * q[2] * (1.0 - cy * py) + cy * q[2] * G[5]) / (G[4] * G[4]);
+                if (np->has[p] & 8) g3 = (G[6] * q[3] * (1.0 + cy * py) - cy * q[3] * G[7]) / (G[6] * G[6]);
+                sx += (g0 - g1) * np->nw[n];
+                sy += (g2 - g3) * np->nw[n];
+            }
+            wl_f64[2 * i] = sx;
+            wl_f64[2 * i + 1] = sy;
+        }
+    }
+}
+
+/* Density gather of a region's cells at their current density boxes over the given fields
+ * (y-major): faithful float, and FP64 over FP64 fields. */
+void orc_gather(orc_np* np, int32_t b, const float* field_x, const float* field_y, float* den, const double* fx64, const double* fy64, double* den_f64) {
+    orc_nb* nb = &np->nbs[b];
+    size_t M = (size_t)nb->R.cnt_x * nb->R.cnt_y;
+    if (field_x) memcpy(nb->field_x, field_x, 4 * M);
+    if (field_y) memcpy(nb->field_y, field_y, 4 * M);
+    for (int i = 0; i < nb->n; ++i) {
+        int g = nb->begin + i;
+        if (den) {
+            fpt f = density_gradient(np, nb, g);
+            den[2 * i] = f.x;
+            den[2 * i + 1] = f.y;
+        }
+        if (den_f64 && fx64 && fy64) {
+            int x0, x1, y0, y1;
+            didx_x(nb, np->dlx[g], np->dux[g], &x0, &x1);
+            didx_y(nb, np->dly[g], np->duy[g], &y0, &y1);
+            double sx = 0.0, sy = 0.0;
+            for (int x = x0; x < x1; ++x)
+                for (int y = y0; y < y1; ++y) {
+                    size_t k = (size_t)y * nb->R.cnt_x + x;
+                    double a = (double)overlap_density_area(nb->bin_x[x], nb->bin_y[y], nb->bin_x[x + 1], nb->bin_y[y + 1], np->dlx[g], np->dly[g], np->dux[g], np->duy[g]) * np->dscale[g];
+                    sx += a * fx64[k];
+                    sy += a * fy64[k];
+                }
+            den_f64[2 * i] = sx;
+            den_f64[2 * i + 1] = sy;
+        }
+    }
+}
+
+/* The current density boxes and pin positions (for comparing state). */
+void orc_get_dbox(orc_np* np, int32_t* dbox) {
+    for (int g = 0; g < np->ncells; ++g) {
+        dbox[4 * g] = np->dlx[g];
+        dbox[4 * g + 1] = np->dly[g];
+        dbox[4 * g + 2] = np->dux[g];
+        dbox[4 * g + 3] = np->duy[g];
+    }
+}
+void orc_get_pins(orc_np* np, int32_t* xy) {
+    for (int p = 0; p < np->npins; ++p) {
+        xy[2 * p] = np->pcx[p];
+        xy[2 * p + 1] = np->pcy[p];
+    }
+}
