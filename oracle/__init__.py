@@ -69,6 +69,7 @@ def lib():
             ("orc_gather", None, [C.c_void_p, C.c_int32, f32p, f32p, f32p, f64p, f64p, f64p]),
             ("orc_get_dbox", None, [C.c_void_p, i32p]),
             ("orc_get_pins", None, [C.c_void_p, i32p]),
+            ("orc_wa_net_f64", C.c_double, [C.c_int32, f64p, C.c_double, C.c_int32, f64p]),
         ]
         for name, res, args in sig:
             fn = getattr(L, name)

@@ -1348,3 +1348,38 @@ void orc_get_pins(orc_np* np, int32_t* xy) {
         xy[2 * p + 1] = np->pcy[p];
     }
 }
+
+/* ---- one net's WA model along one axis, for pinning the gradient expression ----------------- */
+
+/* The WA wirelength W = X_max - X_min of one net's pin coordinates xs[0..n) (X_max / X_min the
+ * exponentially weighted averages with shifts at the box, nesterov.rs:2354-2384) and per pin the
+ * reference's gradient expression gmin - gmax (nesterov.rs:2392-2413) in double, with the
+ * reference's fast exponential (use_fast) or the C library's exp.  With exp, gmin - gmax is
+ * exactly -dW/dx_i, which the tests check against central differences; the force bar is ignored
+ * here (every term kept). */
+double orc_wa_net_f64(int32_t n, const double* xs, double coef, int32_t use_fast, double* grad) {
+    double lx = xs[0], ux = xs[0];
+    for (int i = 1; i < n; ++i) {
+        if (xs[i] < lx) lx = xs[i];
+        if (xs[i] > ux) ux = xs[i];
+    }
+    double s0n = 0, s1n = 0, s0x = 0, s1x = 0;
+    double* en = (double*)malloc(sizeof(double) * 2 * (size_t)n);
+    double* ex = en + n;
+    for (int i = 0; i < n; ++i) {
+        double a = (lx - xs[i]) * coef, b = (xs[i] - ux) * coef;
+        en[i] = use_fast ? (double)orc_fast_exp((float)a) : exp(a);
+        ex[i] = use_fast ? (double)orc_fast_exp((float)b) : exp(b);
+        s0n += en[i];
+        s1n += xs[i] * en[i];
+        s0x += ex[i];
+        s1x += xs[i] * ex[i];
+    }
+    for (int i = 0; i < n && grad; ++i) {
+        double gmin = (s0n * (en[i] * (1.0 - coef * xs[i])) + coef * en[i] * s1n) / (s0n * s0n);
+        double gmax = (s0x * (ex[i] * (1.0 + coef * xs[i])) - coef * ex[i] * s1x) / (s0x * s0x);
+        grad[i] = gmin - gmax;
+    }
+    free(en);
+    return s1x / s0x - s1n / s0n;
+}
