@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Throughput of the ePlace Nesterov placement loop on one MI355X.
+
+One step is one outer Nesterov iteration (its back-tracking trials included: position update,
+density scatter, Poisson solve, WA wirelength forward and gradient, density gather, step length)
+of the 1M-cell synthetic placement (1,000,000 one-row cells, 1,050,000 nets, 50 nets of 1k-5k pins,
+5 % fixed macros, 1024 x 1024 bins).  The design is generated and initially placed on the host
+before anything is timed; np_init and W warmup iterations run untimed, then exactly K iterations
+are timed between two stream synchronisations, then the loop runs on to overflow 0.10 for the
+time-to-overflow figure.  One JSON line on stdout.
+
+Besides the throughput the line carries:
+  roofline      the dominant kernel family's algorithmic bytes per launch over its average device
+                duration (HIP events on the library's stream), against MI355X's 8 TB/s;
+  cpu_baseline  the oracle (the C restatement of the reference's loop, one host thread) on a bounded
+                sample of the same placement: np_init and the first iterations within --cpu-budget
+                seconds;
+  hpwl_delta    GPU vs CPU final HPWL on the small (2k-cell) placement, run to convergence on both.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Nesterov iters/s + time to overflow 0.10 at 1M cells vs CPU; HPWL delta"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c1": "2k-cell synthetic placement: 2,000 cells, 2,200 nets, 96 IO ports, 64x64 bins",
+    "c2": "200k-cell synthetic placement: 200,000 cells, 210,000 nets (+20 of 500-2,000 pins), 512x512 bins",
+    "c3": "1M-cell synthetic placement: 1,000,000 cells, 1,050,000 nets (+50 of 1k-5k pins), 5% fixed macros, 1024x1024 bins",
+}
+
+
+def algorithmic_bytes(fam, N, P, E, M):
+    """Compulsory bytes per launch of each kernel family (every array touched once):
+    update_scatter  77 N + 16 M   per cell: read cur_slp, cur_sum, cur (24), box (16), flags (1),
+                                  scale (4); write next, next_slp (16), box (16); the two int64 bin
+                                  accumulators written once (16 per bin)
+    bin_density     52 M          read inst, filler, non_place, non_place_unscaled (32); write the
+                                  density (4); clear the accumulators (16)
+    dct_solve       12 M          read the density (4); write field_x, field_y (8)
+    wa_forward      28 P + 52 E   per pin: cell (4), offset (8), its cell's box (16); per net:
+                                  start (4), box (16), sums (32)
+    gradient        80 N + 64 P + 8 M   per cell: box, pin start, preconditioner, scale (32),
+                                  3 slots read, 3 written (48); per cell pin: net (4), offset (8),
+                                  net box + sums + weight (52); the two fields once (8 per bin)
+    """
+    return {
+        "update_scatter": 77 * N + 16 * M,
+        "bin_density": 52 * M,
+        "dct_solve": 12 * M,
+        "wa_forward": 28 * P + 52 * E,
+        "gradient": 80 * N + 64 * P + 8 * M,
+    }.get(fam)
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400, help="timed iterations (K)")
+    ap.add_argument("--warmup", type=int, default=50, help="untimed iterations after np_init (W)")
+    ap.add_argument("--config", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline sampling")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the HPWL check")
+    ap.add_argument("--hpwl-config", default="c1", choices=sorted(WORKLOADS))
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: one placement runs on one MI355X; this build measures N=1 only "
+              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import gpl_amd as G
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    # ---- the workload, built on the host (untimed) ----
+    t = time.perf_counter()
+    d = G.Design.from_config(args.config)
+    d.initial_place()
+    d.init_nesterov_place()
+    setup_s = time.perf_counter() - t
+    p = d.problem()
+    N, P, E = p.num_cells, p.num_pins, p.num_nets
+    M = sum(p.regions[b].cnt_x * p.regions[b].cnt_y for b in range(p.num_regions))
+    log(f"[bench] {args.config}: cells {N} pins {P} nets {E} bins {M}; setup {setup_s:.1f}s")
+
+    # ---- the timed run ----
+    g = G.NesterovGPU(d)
+    g.init_nesterov_loop()
+    g.loop_begin(0)
+    it, done = g.loop_step(args.warmup)
+    it0 = it
+    t0 = time.perf_counter()
+    it, done = g.loop_step(args.steps)
+    t1 = time.perf_counter()
+    steps = it - it0 if not done else it - it0 + 1
+    log(f"[bench] timed {steps} iterations in {t1 - t0:.3f}s")
+    while not done:
+        it, done = g.loop_step(1 << 20)
+    rc, final_iter = g.loop_end(check=False)
+    s = g.stats()
+    hist = g.history()
+    g.close()
+    timed_s = t1 - t0
+    value = steps / timed_s if timed_s > 0 else 0.0
+    trials = sum(h["trials"] for h in hist) / max(len(hist), 1)
+    log(f"[bench] converged={s.converged} at iter {final_iter}; time to overflow {s.first_below_target_seconds:.3f}s; "
+        f"hpwl {s.hpwl}; overflow {s.overflow:.4f}")
+
+    # ---- a profiled run for the roofline: device times per kernel family (HIP events) ----
+    gp = G.NesterovGPU(d)
+    gp.set_profiling(True)
+    gp.init_nesterov_loop()
+    gp.loop_begin(0)
+    gp.loop_step(args.warmup + args.steps)
+    kt = gp.kernel_times()
+    gp.close()
+    fams = {k: v for k, v in kt.items() if algorithmic_bytes(k, N, P, E, M) is not None and v[0] > 0}
+    dom = max(fams, key=lambda k: fams[k][1])
+    launches, ms = fams[dom]
+    avg_s = ms / 1000.0 / launches
+    bytes_launch = algorithmic_bytes(dom, N, P, E, M)
+    achieved = bytes_launch / avg_s / 1e9
+    families = {k: {"launches": v[0], "avg_us": round(1000.0 * v[1] / max(v[0], 1), 2),
+                    "share": round(v[1] / max(sum(x[1] for x in kt.values()), 1e-12), 4),
+                    "algorithmic_bytes": algorithmic_bytes(k, N, P, E, M),
+                    "achieved_GBs": (round(algorithmic_bytes(k, N, P, E, M) / (v[1] / 1000.0 / v[0]) / 1e9, 1)
+                                     if algorithmic_bytes(k, N, P, E, M) and v[0] else None)}
+                for k, v in kt.items()}
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
+                "algorithmic_bytes_per_launch": bytes_launch, "families": families}
+
+    # ---- CPU baseline (the oracle, one host thread, bounded sample of the same placement) ----
+    cpu = None
+    hpwl_delta = None
+    if not args.no_cpu:
+        import oracle as O
+        o = O.Oracle(d.problem())
+        o.init()
+        orc, oit = o.run(time_budget_s=args.cpu_budget)
+        os_ = o.stats()
+        n_it = len(o.history())
+        model, ncpu = cpu_info()
+        cpu_val = n_it / os_.loop_seconds if os_.loop_seconds > 0 else 0.0
+        cpu = {"value": round(cpu_val, 4), "unit": "iters/s", "cores": 1, "kind": "port",
+               "sample": f"np_init ({os_.init_seconds:.2f}s) + the first {n_it} iterations of the same {args.config} "
+                         f"placement in {os_.loop_seconds:.1f}s on one host thread ({model}, {ncpu} CPUs)",
+               "init_seconds": round(os_.init_seconds, 3)}
+        o.close()
+        log(f"[bench] cpu baseline {cpu_val:.3f} it/s over {n_it} iterations")
+        # full-run HPWL parity on the small placement
+        dh = G.Design.from_config(args.hpwl_config)
+        dh.initial_place()
+        dh.init_nesterov_place()
+        gh = G.NesterovGPU(dh)
+        gh.do_nesterov_place()
+        sg = gh.stats()
+        gh.close()
+        oh = O.Oracle(dh.problem())
+        oh.init()
+        oh.run()
+        so = oh.stats()
+        oh.close()
+        hpwl_delta = {"config": args.hpwl_config, "gpu_hpwl": sg.hpwl, "cpu_hpwl": so.hpwl,
+                      "delta": round((sg.hpwl - so.hpwl) / so.hpwl, 6), "gpu_iter": sg.iter, "cpu_iter": so.iter,
+                      "gpu_overflow": round(sg.overflow, 4), "cpu_overflow": round(so.overflow, 4)}
+        dh.close()
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * timed_s / max(steps, 1), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS[args.config], "name": args.config, "cells": N, "pins": P, "nets": E, "bins": M,
+                   "seed": 1},
+        "time_to_overflow_s": round(s.first_below_target_seconds, 4),
+        "iterations_to_overflow": final_iter,
+        "converged": bool(s.converged),
+        "final_overflow": round(s.overflow, 4),
+        "final_hpwl": s.hpwl,
+        "trials_per_iteration": round(trials, 3),
+        "np_init_s": round(s.init_seconds, 4),
+        "setup_s": round(setup_s, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu and cpu["value"] > 0 else None,
+        "hpwl_delta": hpwl_delta,
+    }
+    print(json.dumps(out), flush=True)
+    d.close()
+
+
+if __name__ == "__main__":
+    main()

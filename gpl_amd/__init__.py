@@ -229,6 +229,22 @@ class NesterovGPU:
             "continue_nesterov_place", self._h)
         return it.value
 
+    def loop_begin(self, start_iter=0):
+        return _ok(_ffi.lib().gpl_loop_begin(self._h, start_iter), "gpl_loop_begin", self._h)
+
+    def loop_step(self, n):
+        """Runs at most n iterations; returns (iteration, done)."""
+        it, done = C.c_int32(0), C.c_int32(0)
+        _ok(_ffi.lib().gpl_loop_step(self._h, n, C.byref(it), C.byref(done)), "gpl_loop_step", self._h)
+        return it.value, bool(done.value)
+
+    def loop_end(self, check=True):
+        it = C.c_int32(0)
+        rc = _ffi.lib().gpl_loop_end(self._h, C.byref(it))
+        if check:
+            _ok(rc, "gpl_loop_end", self._h)
+        return it.value if check else (rc, it.value)
+
     def stats(self):
         s = Stats()
         _ok(_ffi.lib().gpl_get_stats(self._h, C.byref(s)), "gpl_get_stats")

@@ -119,6 +119,9 @@ SIGNATURES = [
     ("gpl_do_nesterov_place", i32, [C.c_void_p, P(i32)]),
     ("gpl_continue_nesterov_place", i32, [C.c_void_p, f32, i32, i32, P(i32)]),
     ("gpl_get_stats", i32, [C.c_void_p, P(Stats)]),
+    ("gpl_loop_begin", i32, [C.c_void_p, i32]),
+    ("gpl_loop_step", i32, [C.c_void_p, i32, P(i32), P(i32)]),
+    ("gpl_loop_end", i32, [C.c_void_p, P(i32)]),
     ("gpl_get_history", i32, [C.c_void_p, P(IterRow), i32]),
     ("gpl_get_log", i32, [C.c_void_p, C.c_char_p, i32]),
     ("gpl_placed_locations", i32, [C.c_void_p, P(i32), i32]),

@@ -150,32 +150,131 @@ __global__ void k_center_scatter(RegionDev r, CellsDev c, const float2* xy) {
 }
 
 // nesterovUpdateCoordinates (nesterov.rs:2718-2745) fused with the centring and the scatter.
-__global__ void k_update_scatter(RegionDev r, CellsDev c, const float2* cur_slp, const float2* cur_sum, const float2* cur,
-                                 float2* next, float2* next_slp, float s, float coeff) {
-    int n = r.cell_end - r.cell_begin;
+// A block takes kChunk consecutive cells (the device keeps each region's cells in Morton order of
+// their bins, so a chunk covers a compact window): the block's bin window is reduced first; when
+// it fits the LDS tile the chunk's overlaps are summed there with LDS atomics and each touched
+// bin gets one global atomic, else every overlap goes straight to global memory.  Integer sums:
+// the result does not depend on the path.
+constexpr int kCellsPerThread = 2;
+constexpr int kChunk = kBlock * kCellsPerThread;
+constexpr int kTile = 2048;
+
+__global__ void __launch_bounds__(kBlock) k_update_scatter(RegionDev r, CellsDev c, const float2* cur_slp, const float2* cur_sum,
+                                                           const float2* cur, float2* next, float2* next_slp, float s, float coeff) {
+    __shared__ unsigned long long t_inst[kTile];
+    __shared__ unsigned long long t_fill[kTile];
+    __shared__ int s_win[4][kBlock / 64];
+    const int n = r.cell_end - r.cell_begin;
     const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int g = r.cell_begin + k;
-        int4 d = c.dbox[g];
-        uint8_t f = c.flags[g];
-        float2 ns;
-        if (f & 4u) {
-            next[k] = cur[k];
-            ns = cur_slp[k];
-        } else {
-            float2 a = cur_slp[k], sm = cur_sum[k], cu = cur[k];
-            float nx = a.x + s * sm.x;
-            float ny = a.y + s * sm.y;
-            float sx = nx + coeff * (nx - cu.x);
-            float sy = ny + coeff * (ny - cu.y);
-            float hx = (float)(d.z - d.x) / 2.0f, hy = (float)(d.w - d.y) / 2.0f;
-            next[k] = make_float2(inside(nx, hx, lx, ux), inside(ny, hy, ly, uy));
-            ns = make_float2(inside(sx, hx, lx, ux), inside(sy, hy, ly, uy));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int base = blockIdx.x * kChunk; base < n; base += gridDim.x * kChunk) {
+        int4 box[kCellsPerThread];
+        int span[kCellsPerThread][4];
+        uint8_t fl[kCellsPerThread];
+        float sc[kCellsPerThread];
+        int wx0 = INT_MAX, wx1 = INT_MIN, wy0 = INT_MAX, wy1 = INT_MIN;
+#pragma unroll
+        for (int j = 0; j < kCellsPerThread; ++j) {
+            const int k = base + j * kBlock + threadIdx.x;
+            span[j][0] = span[j][1] = span[j][2] = span[j][3] = 0;
+            if (k >= n) continue;
+            const int g = r.cell_begin + k;
+            int4 d = c.dbox[g];
+            const uint8_t f = c.flags[g];
+            float2 ns;
+            if (f & 4u) {
+                next[k] = cur[k];
+                ns = cur_slp[k];
+            } else {
+                float2 a = cur_slp[k], sm = cur_sum[k], cu = cur[k];
+                float nx = a.x + s * sm.x;
+                float ny = a.y + s * sm.y;
+                float sx = nx + coeff * (nx - cu.x);
+                float sy = ny + coeff * (ny - cu.y);
+                float hx = (float)(d.z - d.x) / 2.0f, hy = (float)(d.w - d.y) / 2.0f;
+                next[k] = make_float2(inside(nx, hx, lx, ux), inside(ny, hy, ly, uy));
+                ns = make_float2(inside(sx, hx, lx, ux), inside(sy, hy, ly, uy));
+            }
+            next_slp[k] = ns;
+            d = centre_box(d, ns.x, ns.y);
+            c.dbox[g] = d;
+            box[j] = d;
+            fl[j] = f;
+            sc[j] = c.dscale[g];
+            bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, span[j][0], span[j][1]);
+            bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, span[j][2], span[j][3]);
+            if (span[j][0] < span[j][1] && span[j][2] < span[j][3]) {
+                wx0 = min(wx0, span[j][0]);
+                wx1 = max(wx1, span[j][1]);
+                wy0 = min(wy0, span[j][2]);
+                wy1 = max(wy1, span[j][3]);
+            }
         }
-        next_slp[k] = ns;
-        d = centre_box(d, ns.x, ns.y);
-        c.dbox[g] = d;
-        scatter_cell(r, d, c.dscale[g], f);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            wx0 = min(wx0, __shfl_down(wx0, o, 64));
+            wx1 = max(wx1, __shfl_down(wx1, o, 64));
+            wy0 = min(wy0, __shfl_down(wy0, o, 64));
+            wy1 = max(wy1, __shfl_down(wy1, o, 64));
+        }
+        if (lane == 0) {
+            s_win[0][wv] = wx0;
+            s_win[1][wv] = wx1;
+            s_win[2][wv] = wy0;
+            s_win[3][wv] = wy1;
+        }
+        __syncthreads();
+        wx0 = s_win[0][0];
+        wx1 = s_win[1][0];
+        wy0 = s_win[2][0];
+        wy1 = s_win[3][0];
+        for (int i = 1; i < kBlock / 64; ++i) {
+            wx0 = min(wx0, s_win[0][i]);
+            wx1 = max(wx1, s_win[1][i]);
+            wy0 = min(wy0, s_win[2][i]);
+            wy1 = max(wy1, s_win[3][i]);
+        }
+        // no cell of the chunk touches a bin: the window stays empty (W = H = 0)
+        const int W = wx1 > wx0 ? wx1 - wx0 : 0, H = wy1 > wy0 ? wy1 - wy0 : 0;
+        const bool tiled = W > 0 && H > 0 && (long long)W * H <= kTile;
+        if (tiled) {
+            for (int i = threadIdx.x; i < W * H; i += kBlock) {
+                t_inst[i] = 0ull;
+                t_fill[i] = 0ull;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < kCellsPerThread; ++j) {
+            if (span[j][0] >= span[j][1] || span[j][2] >= span[j][3]) continue;
+            const int4 d = box[j];
+            const bool inst = fl[j] & 1u, macro = inst && (fl[j] & 2u);
+            for (int y = span[j][2]; y < span[j][3]; ++y) {
+                const int by0 = r.bin_y[y], by1 = r.bin_y[y + 1];
+                for (int x = span[j][0]; x < span[j][1]; ++x) {
+                    float a = overlap_area(r.bin_x[x], by0, r.bin_x[x + 1], by1, d) * sc[j];
+                    if (macro) a *= r.target_density;
+                    const long long v = (long long)a;
+                    if (v == 0) continue;
+                    if (tiled) {
+                        const int t = (y - wy0) * W + (x - wx0);
+                        atomicAdd(inst ? &t_inst[t] : &t_fill[t], (unsigned long long)v);
+                    } else {
+                        atomicAdd((inst ? r.inst : r.filler) + (size_t)y * r.cnt_x + x, (unsigned long long)v);
+                    }
+                }
+            }
+        }
+        if (tiled) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < W * H; i += kBlock) {
+                const size_t kb = (size_t)(wy0 + i / W) * r.cnt_x + (wx0 + i % W);
+                const unsigned long long a = t_inst[i], b = t_fill[i];
+                if (a) atomicAdd(r.inst + kb, a);
+                if (b) atomicAdd(r.filler + kb, b);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -764,7 +863,9 @@ void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c,
                            const float2* cur, float2* next, float2* next_slp, float step, float coeff) {
     int n = r.cell_end - r.cell_begin;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_update_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, cur_slp, cur_sum, cur, next, next_slp, step, coeff);
+    int chunks = (n + kChunk - 1) / kChunk;
+    int grid = chunks < 4096 ? chunks : 4096;
+    hipLaunchKernelGGL(k_update_scatter, dim3(grid), dim3(kBlock), 0, s, r, c, cur_slp, cur_sum, cur, next, next_slp, step, coeff);
 }
 
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials) {

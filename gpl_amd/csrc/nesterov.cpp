@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -76,6 +77,7 @@ struct gpl_ctx {
     std::vector<int32_t> net_sorted_of;  // original net -> sorted index
     std::vector<int32_t> net_orig_of;    // sorted index -> original net
     std::vector<int32_t> cell_inst, inst_box;
+    std::vector<int32_t> cell_old_of, cell_new_of;  // device cell order <-> problem order
     int32_t site_x = 0, dbu = 1000;
     gpl_opts o{};
     // NesterovPlace
@@ -519,6 +521,29 @@ struct gpl_ctx {
         }
     }
 
+    // Per-cell float pairs of region b between the problem's order (host) and the device's.
+    void upload_region_vec(int b, float2* dev, const float* host) {
+        Region& r = rg[b];
+        std::vector<float2> tmp(r.n);
+        for (int k = 0; k < r.n; ++k) {
+            int old = cell_old_of[r.R.cell_begin + k] - r.R.cell_begin;
+            tmp[k] = make_float2(host[2 * old], host[2 * old + 1]);
+        }
+        check(hipMemcpyAsync(dev, tmp.data(), sizeof(float2) * r.n, hipMemcpyHostToDevice, stream), "copy");
+        check(hipStreamSynchronize(stream), "sync");
+    }
+    void download_region_vec(int b, const float2* dev, float* host) {
+        Region& r = rg[b];
+        std::vector<float2> tmp(r.n);
+        check(hipMemcpyAsync(tmp.data(), dev, sizeof(float2) * r.n, hipMemcpyDeviceToHost, stream), "copy");
+        check(hipStreamSynchronize(stream), "sync");
+        for (int k = 0; k < r.n; ++k) {
+            int old = cell_old_of[r.R.cell_begin + k] - r.R.cell_begin;
+            host[2 * old] = tmp[k].x;
+            host[2 * old + 1] = tmp[k].y;
+        }
+    }
+
     void copy_vec(float2* dst, const float2* src, int n) {
         if (n > 0) check(hipMemcpyAsync(dst, src, sizeof(float2) * n, hipMemcpyDeviceToDevice, stream), "snapshot copy");
     }
@@ -664,55 +689,91 @@ struct gpl_ctx {
         hist.push_back(row);
     }
 
-    // resume_nesterov_place (nesterov.rs:4440), routability and timing off.
-    int32_t resume(int32_t start_iter, int32_t* iter_out) {
-        auto t0 = Clock::now();
+    // resume_nesterov_place (nesterov.rs:4440), routability and timing off, in three pieces so
+    // a caller can run it in slices: begin (the per-call resets), step (iterations), end (the
+    // max-iteration snapshot rule and the closing report).
+    struct LoopState {
+        float cur_a = 1.0f;
+        int32_t iter = 0;
+        Clock::time_point t0;
+        int64_t original_area = 0;
+        bool active = false, done = false;
+    } ls;
+
+    int32_t loop_begin(int32_t start_iter) {
+        ls = LoopState{};
+        ls.t0 = Clock::now();
         log.info(84, "---- Execute Nesterov Global Placement.");
         if (num_region_diverged > 0) {
             log.error(diverge_code, diverge_msg);
+            ls.done = true;
             return diverge_code;
         }
-        int64_t original_area = 0;
         for (auto& r : rg) {
             r.max_phi_coef_changed = false;
             r.min_sum_overflow = 1e30f;
             r.hpwl_with_min_sum_overflow = 1e30f;
             r.is_converged = false;
-            original_area += r.nia;
+            ls.original_area += r.nia;
         }
         diverge_saved = false;
         diverge_reverts = 0;
         diverge_hpwl = 0;
-        float cur_a = 1.0f;
-        int32_t iter = start_iter;
         converged = false;
-        while (iter < max_iter) {
-            float prev_a = cur_a;
-            cur_a = (float)((1.0 + std::sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
-            float coeff = (float)(((double)prev_a - 1.0) / (double)cur_a);
+        ls.iter = start_iter;
+        ls.active = true;
+        return GPL_OK;
+    }
+
+    // At most n iterations of the loop body (nesterov.rs:4478-4575).
+    void loop_step(int32_t n) {
+        for (int32_t k = 0; k < n && ls.active && !ls.done; ++k) {
+            if (ls.iter >= max_iter) {
+                ls.done = true;
+                break;
+            }
+            float prev_a = ls.cur_a;
+            ls.cur_a = (float)((1.0 + std::sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
+            float coeff = (float)(((double)prev_a - 1.0) / (double)ls.cur_a);
             do_back_tracking(coeff);
             total_trials += trials;
             for (auto& r : rg) adjust_phi(r);
-            if (num_region_diverged > 0) break;
-            np_update_next_iter(iter);
-            push_history(iter);
-            last_iter = iter;
+            if (num_region_diverged > 0) {
+                ls.done = true;
+                break;
+            }
+            np_update_next_iter(ls.iter);
+            push_history(ls.iter);
+            last_iter = ls.iter;
             ++iters_run;
-            int act = np_is_diverged(&cur_a);
-            if (act == 2) break;
+            int act = np_is_diverged(&ls.cur_a);
+            if (act == 2) {
+                ls.done = true;
+                break;
+            }
             if (act == 1) {
-                ++iter;
+                ++ls.iter;
                 continue;
             }
             size_t conv = 0;
-            for (size_t b = 0; b < rg.size(); ++b) conv += check_convergence(int(b), iter) ? 1 : 0;
+            for (size_t b = 0; b < rg.size(); ++b) conv += check_convergence(int(b), ls.iter) ? 1 : 0;
             if (conv == rg.size()) {
                 converged = true;
-                t_converge = t_init + std::chrono::duration<double>(Clock::now() - t0).count();
+                t_converge = t_init + std::chrono::duration<double>(Clock::now() - ls.t0).count();
+                ls.done = true;
                 break;
             }
-            ++iter;
+            ++ls.iter;
         }
+    }
+
+    int32_t loop_end(int32_t* iter_out) {
+        if (!ls.active) {
+            if (iter_out) *iter_out = ls.iter;
+            return num_region_diverged > 0 ? diverge_code : GPL_OK;
+        }
+        ls.active = false;
+        int32_t iter = ls.iter;
         if (iter >= max_iter && !o.disable_revert_if_diverge && (diverge_saved || diverge_reverts > 0) &&
             diverge_snapshot_overflow <= average_overflow_unscaled && diverge_hpwl < cur_hpwl) {
             log.warn(121, "Reached the maximum number of iterations, reverting to snapshot with min hpwl.");
@@ -723,16 +784,27 @@ struct gpl_ctx {
         int64_t new_area = 0;
         for (auto& r : rg) new_area += r.nia;
         double dbu2 = double(dbu) * double(dbu);
-        float diff = (float)(100.0 * double(new_area - original_area) / double(original_area));
+        float diff = (float)(100.0 * double(new_area - ls.original_area) / double(ls.original_area));
         log.info(1014, strfmt("Final placement area: %.2f (%+.2f%%)", double(new_area) / dbu2, diff));
         check(hipStreamSynchronize(stream), "sync");
-        t_loop = std::chrono::duration<double>(Clock::now() - t0).count();
+        t_loop = std::chrono::duration<double>(Clock::now() - ls.t0).count();
         if (iter_out) *iter_out = iter;
         if (num_region_diverged > 0) {
             log.error(diverge_code, diverge_msg);
             return diverge_code;
         }
         return GPL_OK;
+    }
+
+    int32_t resume(int32_t start_iter, int32_t* iter_out) {
+        int32_t rc = loop_begin(start_iter);
+        if (rc != GPL_OK) {
+            if (iter_out) *iter_out = start_iter;
+            ls.active = false;
+            return rc;
+        }
+        loop_step(INT32_MAX);
+        return loop_end(iter_out);
     }
 };
 
@@ -797,11 +869,48 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         c->npins = p->num_pins;
         c->nnets = p->num_nets;
         const int32_t N = p->num_cells, E = p->num_nets;
-        // nets sorted by degree (stable); high-fanout nets last
+        // Cells in Morton order of their initial density-box centre's bin, within each region:
+        // a block of consecutive cells then covers a compact window of bins, which the scatter
+        // accumulates in LDS.  new_of[old] / old_of[new]; the C ABI's per-cell vectors stay in
+        // the problem's order (permuted at the boundary).
+        c->cell_old_of.resize(N);
+        c->cell_new_of.resize(N);
+        std::iota(c->cell_old_of.begin(), c->cell_old_of.end(), 0);
+        for (int32_t b = 0; b < p->num_regions; ++b) {
+            const gpl_region& R = p->regions[b];
+            std::vector<uint64_t> key(size_t(R.cell_end - R.cell_begin));
+            for (int32_t g = R.cell_begin; g < R.cell_end; ++g) {
+                const int32_t* d = &p->cell_dbox[4 * g];
+                double cx = 0.5 * (double(d[0]) + double(d[2])), cy = 0.5 * (double(d[1]) + double(d[3]));
+                uint32_t bx = uint32_t(std::min(std::max((cx - R.lx) / R.size_x, 0.0), double(R.cnt_x - 1)));
+                uint32_t by = uint32_t(std::min(std::max((cy - R.ly) / R.size_y, 0.0), double(R.cnt_y - 1)));
+                uint64_t m = 0;
+                for (int bit = 0; bit < 16; ++bit) m |= (uint64_t((bx >> bit) & 1u) << (2 * bit)) | (uint64_t((by >> bit) & 1u) << (2 * bit + 1));
+                key[g - R.cell_begin] = m;
+            }
+            std::stable_sort(c->cell_old_of.begin() + R.cell_begin, c->cell_old_of.begin() + R.cell_end,
+                             [&](int32_t a, int32_t bb) { return key[a - R.cell_begin] < key[bb - R.cell_begin]; });
+        }
+        for (int32_t i = 0; i < N; ++i) c->cell_new_of[c->cell_old_of[i]] = i;
+        auto newc = [&](int32_t old) { return old >= 0 ? c->cell_new_of[old] : -1; };
+        // nets sorted by degree, then by their first pin's cell (spatial order within a degree);
+        // nets above 128 pins last, one block each
         std::vector<int32_t> order(E);
         std::iota(order.begin(), order.end(), 0);
         auto deg = [&](int32_t e) { return p->net_pin_start[e + 1] - p->net_pin_start[e]; };
-        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return deg(a) < deg(b); });
+        auto first_cell = [&](int32_t e) {
+            int32_t best = INT32_MAX;
+            for (int32_t k = p->net_pin_start[e]; k < p->net_pin_start[e + 1]; ++k) {
+                int32_t q = p->net_pins[k];
+                if (p->pin_cell[q] >= 0) best = std::min(best, newc(p->pin_cell[q]));
+            }
+            return best;
+        };
+        std::vector<int32_t> fc(E);
+        for (int32_t e = 0; e < E; ++e) fc[e] = first_cell(e);
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            return deg(a) != deg(b) ? deg(a) < deg(b) : fc[a] < fc[b];
+        });
         c->net_orig_of = order;
         c->net_sorted_of.assign(E, 0);
         for (int32_t i = 0; i < E; ++i) c->net_sorted_of[order[i]] = i;
@@ -816,7 +925,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             nweight[i] = p->net_weight[e];
             for (int32_t k = p->net_pin_start[e]; k < p->net_pin_start[e + 1]; ++k) {
                 int32_t q = p->net_pins[k];
-                pcell.push_back(p->pin_cell[q]);
+                pcell.push_back(newc(p->pin_cell[q]));
                 poff.push_back(make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]));
                 pxy0.push_back(make_int2(p->pin_xy[2 * q], p->pin_xy[2 * q + 1]));
             }
@@ -832,34 +941,43 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         c->nets.box = c->dalloc<int4>(E);
         c->nets.sx = c->dalloc<float4>(E);
         c->nets.sy = c->dalloc<float4>(E);
-        // cells
+        // cells, in the new order
         std::vector<int4> dbox(N);
         std::vector<float2> pre(N);
+        std::vector<float> dscale(N);
+        std::vector<uint8_t> flags(N);
+        std::vector<int32_t> pin_start(size_t(N) + 1, 0), cell_inst(N);
         std::vector<int32_t> cp_net(p->cell_pin_start[N]);
         std::vector<int2> cp_off(p->cell_pin_start[N]);
-        for (int32_t g = 0; g < N; ++g) {
+        for (int32_t i = 0; i < N; ++i) {
+            int32_t g = c->cell_old_of[i];
             const int32_t* d = &p->cell_dbox[4 * g];
-            dbox[g] = make_int4(d[0], d[1], d[2], d[3]);
+            dbox[i] = make_int4(d[0], d[1], d[2], d[3]);
             const int32_t* bx = &p->cell_box[4 * g];
-            pre[g] = make_float2((float)(p->cell_pin_start[g + 1] - p->cell_pin_start[g]), (float)(bx[2] - bx[0]) * (float)(bx[3] - bx[1]));
-            for (int32_t k = p->cell_pin_start[g]; k < p->cell_pin_start[g + 1]; ++k) {
+            pre[i] = make_float2((float)(p->cell_pin_start[g + 1] - p->cell_pin_start[g]), (float)(bx[2] - bx[0]) * (float)(bx[3] - bx[1]));
+            dscale[i] = p->density_scale[g];
+            flags[i] = p->cell_flags[g];
+            cell_inst[i] = p->cell_inst[g];
+            int32_t at = pin_start[i];
+            for (int32_t k = p->cell_pin_start[g]; k < p->cell_pin_start[g + 1]; ++k, ++at) {
                 int32_t q = p->cell_pins[k];
-                cp_net[k] = c->net_sorted_of[p->pin_net[q]];
-                cp_off[k] = make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]);
+                cp_net[at] = c->net_sorted_of[p->pin_net[q]];
+                cp_off[at] = make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]);
             }
+            pin_start[i + 1] = at;
         }
-        c->h_flags.assign(p->cell_flags, p->cell_flags + N);
+        c->h_flags = flags;
         c->cells.dbox = c->upload(dbox.data(), N);
-        c->cells.dscale = c->upload(p->density_scale, N);
-        c->d_flags = c->upload(p->cell_flags, N);
+        c->cells.dscale = c->upload(dscale.data(), N);
+        c->d_flags = c->upload(flags.data(), N);
         c->cells.flags = c->d_flags;
         c->cells.centered = c->dalloc<uint8_t>(N);
         check(hipMemsetAsync(c->cells.centered, 0, N, c->stream), "memset");
-        c->cells.pin_start = c->upload(p->cell_pin_start, size_t(N) + 1);
+        c->cells.pin_start = c->upload(pin_start.data(), size_t(N) + 1);
         c->cells.cp_net = c->upload(cp_net.data(), cp_net.size());
         c->cells.cp_off = c->upload(cp_off.data(), cp_off.size());
         c->cells.pre = c->upload(pre.data(), N);
-        c->cell_inst.assign(p->cell_inst, p->cell_inst + N);
+        c->cell_inst = cell_inst;
         c->inst_box.assign(p->inst_box, p->inst_box + 4 * size_t(p->num_insts));
         // regions
         c->rg.resize(p->num_regions);
@@ -965,6 +1083,39 @@ int32_t gpl_run_nesterov_loop(gpl_ctx* c, int32_t start_iter, int32_t* iter_out)
     }
 }
 
+int32_t gpl_loop_begin(gpl_ctx* c, int32_t start_iter) {
+    if (!c) return GPL_ERR_ARG;
+    if (!c->initialized) return fail(c, GPL_ERR_STATE, "gpl_init_nesterov_loop first");
+    try {
+        return c->loop_begin(start_iter);
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+}
+
+int32_t gpl_loop_step(gpl_ctx* c, int32_t n, int32_t* iter_out, int32_t* done) {
+    if (!c) return GPL_ERR_ARG;
+    if (!c->ls.active) return fail(c, GPL_ERR_STATE, "gpl_loop_begin first");
+    try {
+        c->loop_step(n);
+        check(hipStreamSynchronize(c->stream), "sync");
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    if (iter_out) *iter_out = c->ls.iter;
+    if (done) *done = c->ls.done ? 1 : 0;
+    return GPL_OK;
+}
+
+int32_t gpl_loop_end(gpl_ctx* c, int32_t* iter_out) {
+    if (!c) return GPL_ERR_ARG;
+    try {
+        return c->loop_end(iter_out);
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+}
+
 int32_t gpl_do_nesterov_place(gpl_ctx* c, int32_t* iter_out) {
     int32_t rc = gpl_init_nesterov_loop(c);
     if (rc < 0) return rc;
@@ -1027,7 +1178,8 @@ int32_t gpl_placed_locations(gpl_ctx* c, int32_t* out, int32_t cap) {
         check(hipMemcpyAsync(dbox.data(), c->cells.dbox, sizeof(int4) * c->ncells, hipMemcpyDeviceToHost, c->stream), "copy");
         check(hipStreamSynchronize(c->stream), "sync");
         int32_t n = 0;
-        for (int32_t g = 0; g < c->ncells; ++g) {
+        for (int32_t g_old = 0; g_old < c->ncells; ++g_old) {
+            int32_t g = c->cell_new_of[g_old];
             int32_t k = c->cell_inst[g];
             if (k < 0) continue;
             if (out && n < cap) {
@@ -1066,9 +1218,7 @@ int32_t gpl_set_coordinates(gpl_ctx* c, int32_t region, int32_t slot, const floa
     float2* d = slot_ptr(c, region, slot);
     if (!d) return GPL_ERR_ARG;
     try {
-        int n = c->rg[region].n;
-        check(hipMemcpyAsync(d, xy, sizeof(float2) * n, hipMemcpyHostToDevice, c->stream), "copy");
-        check(hipStreamSynchronize(c->stream), "sync");
+        c->upload_region_vec(region, d, xy);
     } catch (const HipError& e) {
         return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
     }
@@ -1080,9 +1230,7 @@ int32_t gpl_get_coordinates(gpl_ctx* c, int32_t region, int32_t slot, float* xy)
     float2* d = slot_ptr(c, region, slot);
     if (!d) return GPL_ERR_ARG;
     try {
-        int n = c->rg[region].n;
-        check(hipMemcpyAsync(xy, d, sizeof(float2) * n, hipMemcpyDeviceToHost, c->stream), "copy");
-        check(hipStreamSynchronize(c->stream), "sync");
+        c->download_region_vec(region, d, xy);
     } catch (const HipError& e) {
         return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
     }
@@ -1200,10 +1348,10 @@ static int32_t gradients_out(gpl_ctx* c, int32_t region, float penalty, float* w
         c->launch_gradient(region, slot, false);
         r.density_penalty = saved;
         int base = slot == SLOT_PREV ? S_PREV_SLP : slot == SLOT_CUR ? S_CUR_SLP : S_NEXT_SLP;
-        if (wl) check(hipMemcpyAsync(wl, r.v[base + 1], sizeof(float2) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
-        if (den) check(hipMemcpyAsync(den, r.v[base + 2], sizeof(float2) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
-        if (sum) check(hipMemcpyAsync(sum, r.v[base + 3], sizeof(float2) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
         c->sync_stats();
+        if (wl) c->download_region_vec(region, r.v[base + 1], wl);
+        if (den) c->download_region_vec(region, r.v[base + 2], den);
+        if (sum) c->download_region_vec(region, r.v[base + 3], sum);
         if (abs_sums) {
             abs_sums[0] = c->h_sums[region].wl_abs;
             abs_sums[1] = c->h_sums[region].den_abs;
@@ -1233,9 +1381,9 @@ int32_t gpl_nesterov_step(gpl_ctx* c, int32_t region, float step, float coeff, f
         gplk::launch_update_scatter(c->stream, r.dev, c->cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
                                     step, coeff);
         c->bins_and_finish(r, region);
-        if (next) check(hipMemcpyAsync(next, r.v[S_NEXT], sizeof(float2) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
-        if (next_slp) check(hipMemcpyAsync(next_slp, r.v[S_NEXT_SLP], sizeof(float2) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
         c->sync_stats();
+        if (next) c->download_region_vec(region, r.v[S_NEXT], next);
+        if (next_slp) c->download_region_vec(region, r.v[S_NEXT_SLP], next_slp);
     } catch (const HipError& e) {
         return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
     }

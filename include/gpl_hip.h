@@ -265,6 +265,15 @@ int32_t gpl_continue_nesterov_place(gpl_ctx* c, float overflow, int32_t max_iter
                                     int32_t* iter_out);
 int32_t gpl_get_stats(gpl_ctx* c, gpl_stats* out);
 
+/* The loop of gpl_run_nesterov_loop in slices, for callers that time or interleave it (no
+ * reference counterpart; the three together are resume_nesterov_place, nesterov.rs:4440):
+ * gpl_loop_begin does the per-call resets, gpl_loop_step runs at most n iterations (the device
+ * work synchronised on return; *done is set once the loop ended: converged, diverged or out of
+ * iterations), gpl_loop_end applies the max-iteration snapshot rule and writes the report. */
+int32_t gpl_loop_begin(gpl_ctx* c, int32_t start_iter);
+int32_t gpl_loop_step(gpl_ctx* c, int32_t n, int32_t* iter_out, int32_t* done);
+int32_t gpl_loop_end(gpl_ctx* c, int32_t* iter_out);
+
 /* Per-iteration history rows (iteration, overflow, HPWL, penalty, step, trials); returns the
  * number of rows written (at most `cap`). */
 typedef struct gpl_iter_row {
