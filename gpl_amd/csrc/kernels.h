@@ -42,11 +42,14 @@ struct RegionDev {
     float* work3;
     const float* wx;
     const float* wy;
-    // transform tables: for a length L, twiddles e^{-2 pi i k / L} (k < L/2) and the DCT
-    // post/pre-twiddles e^{-i pi k / 2L} (k < L), as float2
+    // transform tables for a length L (h = L/2), as float2: the half-length FFT's twiddles
+    // e^{-2 pi i m / h} (m < h), the real-transform split e^{-2 pi i k / L} (k <= h) and the DCT
+    // twiddles (cos, -sin)(pi k / 2L) (k < L)
     const float2* tw_x;
+    const float2* th_x;
     const float2* dw_x;
     const float2* tw_y;
+    const float2* th_y;
     const float2* dw_y;
 };
 

@@ -320,76 +320,138 @@ __global__ void k_bin_density(RegionDev r, double* partials) {
 
 enum RowMode { ROW_DCT2 = 0, ROW_DCT2_SCALE = 1, ROW_POISSON = 2, ROW_INV_FIELDS = 3, ROW_INV_PHI = 4 };
 
-__device__ __forceinline__ int bitrev(int m, int logL) { return (int)(__brev((unsigned)m) >> (32 - logL)); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
-// In-place FFT on (re, im) in LDS, input already in bit-reversed order.  dir -1: forward
-// (e^{-2 pi i jk/L}), +1: inverse, unnormalised.  tw[k] = e^{-2 pi i k / L}, k < L/2.
-__device__ void fft_lds(float* re, float* im, int L, const float2* tw, int dir) {
-    for (int len = 2; len <= L; len <<= 1) {
-        int half = len >> 1, stride = L / len;
-        __syncthreads();
-        for (int t = threadIdx.x; t < (L >> 1); t += blockDim.x) {
-            int grp = t / half, pos = t - grp * half;
-            int i0 = grp * len + pos, i1 = i0 + half;
-            float2 w = tw[pos * stride];
-            float wr = w.x, wi = dir < 0 ? w.y : -w.y;
-            float br = re[i1] * wr - im[i1] * wi;
-            float bi = re[i1] * wi + im[i1] * wr;
-            float ar = re[i0], ai = im[i0];
-            re[i0] = ar + br;
-            im[i0] = ai + bi;
-            re[i1] = ar - br;
-            im[i1] = ai - bi;
+// Complex FFT of N points (a power of two) held in LDS buffer x, by T = max(N/4, 1) threads of a
+// row group (tid < T), Stockham autosort: radix-2 first when log2 N is odd, then radix-4 stages;
+// natural order in and out.  tw[m] = e^{-2 pi i m / N}, m < N; dir -1 forward, +1 inverse
+// (unnormalised).  Every thread of the block calls it (block barriers inside); returns the buffer
+// that holds the result.
+__device__ float2* fft_stockham(float2* x, float2* y, int N, int logN, int T, int tid, const float2* tw, int dir) {
+    int Ns = 1;
+    __syncthreads();
+    if (logN & 1) {
+        for (int j = tid; j < (N >> 1); j += T) {
+            float2 v0 = x[j], v1 = x[j + (N >> 1)];
+            // Ns = 1: no twiddle
+            int d = j * 2;
+            y[d] = make_float2(v0.x + v1.x, v0.y + v1.y);
+            y[d + 1] = make_float2(v0.x - v1.x, v0.y - v1.y);
         }
+        Ns = 2;
+        float2* t = x;
+        x = y;
+        y = t;
+        __syncthreads();
     }
-    __syncthreads();
+    const int q = N >> 2;
+    for (; Ns < N; Ns <<= 2) {
+        if (tid < q) {
+            const int j = tid;
+            const int k = j & (Ns - 1);
+            float2 v0 = x[j], v1 = x[j + q], v2 = x[j + 2 * q], v3 = x[j + 3 * q];
+            if (Ns > 1) {
+                const int step = N / (Ns << 2);  // twiddle index stride
+                float2 w1 = tw[k * step], w2 = tw[2 * k * step], w3 = tw[3 * k * step];
+                if (dir > 0) {
+                    w1 = cconj(w1);
+                    w2 = cconj(w2);
+                    w3 = cconj(w3);
+                }
+                v1 = cmul(v1, w1);
+                v2 = cmul(v2, w2);
+                v3 = cmul(v3, w3);
+            }
+            float2 t0 = make_float2(v0.x + v2.x, v0.y + v2.y), t1 = make_float2(v0.x - v2.x, v0.y - v2.y);
+            float2 t2 = make_float2(v1.x + v3.x, v1.y + v3.y), t3 = make_float2(v1.x - v3.x, v1.y - v3.y);
+            // forward: X1 = t1 - i t3, X3 = t1 + i t3; inverse the other way
+            float2 it3 = dir < 0 ? make_float2(t3.y, -t3.x) : make_float2(-t3.y, t3.x);  // -i t3 or +i t3
+            const int d = (j / Ns) * Ns * 4 + k;
+            y[d] = make_float2(t0.x + t2.x, t0.y + t2.y);
+            y[d + Ns] = make_float2(t1.x + it3.x, t1.y + it3.y);
+            y[d + 2 * Ns] = make_float2(t0.x - t2.x, t0.y - t2.y);
+            y[d + 3 * Ns] = make_float2(t1.x - it3.x, t1.y - it3.y);
+        }
+        float2* t = x;
+        x = y;
+        y = t;
+        __syncthreads();
+    }
+    return x;
 }
 
-// DCT-II of the real row held in `src` (global, contiguous), result in re[0..L).
-__device__ void dct2_row(const float* src, float* re, float* im, int L, int logL, const float2* tw, const float2* dw) {
-    int h = L >> 1;
-    for (int m = threadIdx.x; m < L; m += blockDim.x) {
-        int s = m < h ? 2 * m : 2 * (L - 1 - m) + 1;
-        int q = bitrev(m, logL);
-        re[q] = src[s];
-        im[q] = 0.0f;
-    }
-    fft_lds(re, im, L, tw, -1);
-    // C[k] = Re(e^{-i pi k / 2L} V[k]); dw[k] = (cos, -sin)(pi k / 2L); each k is its own thread's
-    for (int k = threadIdx.x; k < L; k += blockDim.x) re[k] = dw[k].x * re[k] - dw[k].y * im[k];
-    __syncthreads();
+struct RowCtx {
+    int L, h, logh, T, tid;
+    const float2* tw;  // e^{-2 pi i m / h}, m < h
+    const float2* th;  // e^{-2 pi i k / L}, k <= h
+    const float2* dw;  // (cos, -sin)(pi k / 2L), k < L
+    float2* A;
+    float2* B;
+};
+
+// DCT-II of the real row at src (Makhoul: v = (x[0], x[2], ..., x[3], x[1]), its DFT through an
+// L/2-point complex FFT of the pairs, C[k] = Re(e^{-i pi k / 2L} V[k])); calls out(k, C[k]).
+template <typename OUT>
+__device__ void dct2_row(const RowCtx& R, bool active, const float* src, OUT out) {
+    const int L = R.L, h = R.h;
+    if (active)
+        for (int m = R.tid; m < h; m += R.T) {
+            int q0 = 2 * m, q1 = 2 * m + 1;
+            float a = q0 < h ? src[2 * q0] : src[2 * (L - 1 - q0) + 1];
+            float b = q1 < h ? src[2 * q1] : src[2 * (L - 1 - q1) + 1];
+            R.A[m] = make_float2(a, b);
+        }
+    float2* Z = fft_stockham(R.A, R.B, h, R.logh, R.T, R.tid, R.tw, -1);
+    if (active)
+        for (int k = R.tid; k <= h; k += R.T) {
+            float2 zk = Z[k == h ? 0 : k], zc = cconj(Z[k == 0 ? 0 : h - k]);
+            float2 E = make_float2((zk.x + zc.x) * 0.5f, (zk.y + zc.y) * 0.5f);
+            float2 Op = make_float2((zk.x - zc.x) * 0.5f, (zk.y - zc.y) * 0.5f);
+            float2 O = make_float2(Op.y, -Op.x);  // -i Op
+            float2 wO = cmul(R.th[k], O);
+            float2 V = make_float2(E.x + wO.x, E.y + wO.y);
+            out(k, R.dw[k].x * V.x - R.dw[k].y * V.y);
+            if (k > 0 && k < h) out(L - k, R.dw[L - k].x * V.x + R.dw[L - k].y * V.y);
+        }
 }
 
-// DCT-III (or DST-III when `sine`) of the row given by coef(k), result in im[0..L) -- the
-// caller's functor supplies a[k].  V[k] = e^{i pi k/2L} (c[k] - i c[L-k]), c[0] = 2 a[0],
-// c[L] = 0 (for the sine: c[j] = b[j], b[0] = a[0], b[j] = a[L-j]); y = IDFT(V) / 2, unshuffled.
-template <typename F>
-__device__ void dct3_row(F coef, bool sine, float* re, float* im, int L, int logL, const float2* tw, const float2* dw) {
-    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+// DCT-III (sine: DST-III) of the row given by coef(j), j < L (Ooura's conventions); calls
+// out(j, y[j]).  V[q] = e^{i pi q / 2L} (c[q] - i c[L-q]) with c[0] = 2 a[0], c[L] = 0 (the sine:
+// c from b[0] = a[0], b[j] = a[L-j]); the real inverse DFT through an L/2-point complex IFFT of
+// Z[k] = (V[k] + conj V[h-k]) + i e^{2 pi i k / L} (V[k] - conj V[h-k]); y = v / 2 unshuffled,
+// odd outputs negated for the sine.
+template <typename F, typename OUT>
+__device__ void dct3_row(const RowCtx& R, bool active, F coef, bool sine, OUT out) {
+    const int L = R.L, h = R.h;
+    auto V = [&](int q) -> float2 {
         float ck, cl;
         if (!sine) {
-            ck = k == 0 ? 2.0f * coef(0) : coef(k);
-            cl = k == 0 ? 0.0f : coef(L - k);
+            ck = q == 0 ? 2.0f * coef(0) : coef(q);
+            cl = q == 0 ? 0.0f : coef(L - q);
         } else {
-            // b[j] = a[L-j] (j >= 1), b[0] = a[0]
-            ck = k == 0 ? 2.0f * coef(0) : coef(L - k);
-            cl = k == 0 ? 0.0f : coef(k);
+            ck = q == 0 ? 2.0f * coef(0) : coef(L - q);
+            cl = q == 0 ? 0.0f : coef(q);
         }
-        float cs = dw[k].x, sn = -dw[k].y;  // cos, sin (pi k / 2L)
-        float vr = cs * ck + sn * cl;
-        float vi = sn * ck - cs * cl;
-        int q = bitrev(k, logL);
-        re[q] = vr;
-        im[q] = vi;
-    }
-    fft_lds(re, im, L, tw, +1);
-    // unshuffle y[2m] = v[m], y[2m+1] = v[L-1-m] into im (the imaginary parts are not needed)
-    for (int j = threadIdx.x; j < L; j += blockDim.x) {
-        int m = (j & 1) ? (L - 1 - (j >> 1)) : (j >> 1);
-        float v = re[m] * 0.5f;
-        if (sine && (j & 1)) v = -v;
-        im[j] = v;
-    }
+        float cs = R.dw[q].x, sn = -R.dw[q].y;
+        return make_float2(cs * ck + sn * cl, sn * ck - cs * cl);
+    };
+    if (active)
+        for (int k = R.tid; k < h; k += R.T) {
+            float2 vk = V(k), vc = cconj(V(h - k));
+            float2 E = make_float2(vk.x + vc.x, vk.y + vc.y), D = make_float2(vk.x - vc.x, vk.y - vc.y);
+            float2 wD = cmul(cconj(R.th[k]), D);
+            R.A[k] = make_float2(E.x - wD.y, E.y + wD.x);
+        }
+    float2* z = fft_stockham(R.A, R.B, h, R.logh, R.T, R.tid, R.tw, +1);
+    if (active)
+        for (int j = R.tid; j < L; j += R.T) {
+            int m = (j & 1) ? (L - 1 - (j >> 1)) : (j >> 1);
+            float2 zz = z[m >> 1];
+            float v = ((m & 1) ? zz.y : zz.x) * 0.5f;
+            if (sine && (j & 1)) v = -v;
+            out(j, v);
+        }
     __syncthreads();
 }
 
@@ -399,8 +461,9 @@ struct RowArgs {
     float* out0;
     float* out1;
     float* out2;
-    int L, logL, nrows;
+    int L, logh, nrows, T;
     const float2* tw;
+    const float2* th;
     const float2* dw;
     const float* wrow;   // wavenumber of the row index
     const float* wcol;   // wavenumber of the element index
@@ -408,7 +471,7 @@ struct RowArgs {
     int want_phi;
 };
 
-// One block (64 lanes) per row.  Modes:
+// Row transforms, L/8 threads per row, 256 / (L/8) rows per block.  Modes:
 //  ROW_DCT2        out0[row] = DCT-II(in0[row])
 //  ROW_DCT2_SCALE  the same, then the halving of row/column 0 and x 4/(nx ny) (nesterov.rs:415-424)
 //  ROW_POISSON     in0 = scaled spectrum rows A[u][v] (u = row): Ex = p wx[u], Ey = p wy[v],
@@ -417,53 +480,62 @@ struct RowArgs {
 //  ROW_INV_FIELDS  out0 = DST-III(in0), out1 = DCT-III(in1)
 //  ROW_INV_PHI     out0 = DCT-III(in0)
 template <int MODE>
-__global__ void __launch_bounds__(64) k_rows(RowArgs a) {
-    extern __shared__ float lds[];
-    float* re = lds;
-    float* im = lds + a.L;
-    const int row = blockIdx.x;
+__global__ void __launch_bounds__(256) k_rows(RowArgs a) {
+    extern __shared__ float2 lds2[];
+    const int T = a.T;
+    const int sub = threadIdx.x / T;
+    RowCtx R;
+    R.L = a.L;
+    R.h = a.L >> 1;
+    R.logh = a.logh;
+    R.T = T;
+    R.tid = threadIdx.x - sub * T;
+    R.tw = a.tw;
+    R.th = a.th;
+    R.dw = a.dw;
+    R.A = lds2 + (size_t)sub * a.L;
+    R.B = R.A + R.h;
+    const int row = blockIdx.x * (blockDim.x / T) + sub;
+    const bool active = row < a.nrows;
     const int L = a.L;
-    const size_t off = (size_t)row * L;
+    const size_t off = (size_t)(active ? row : 0) * L;
     if (MODE == ROW_DCT2 || MODE == ROW_DCT2_SCALE) {
-        dct2_row(a.in0 + off, re, im, L, a.logL, a.tw, a.dw);
-        for (int k = threadIdx.x; k < L; k += blockDim.x) {
-            float v = re[k];
+        float* o = a.out0 + off;
+        const double scale = a.scale;
+        dct2_row(R, active, a.in0 + off, [&](int k, float v) {
             if (MODE == ROW_DCT2_SCALE) {
                 if ((row >= 1 && k == 0) || (row == 0 && k >= 1)) v = (float)((double)v * 0.5);
-                v = (float)((double)v * a.scale);
+                v = (float)((double)v * scale);
             }
-            a.out0[off + k] = v;
-        }
+            o[k] = v;
+        });
     } else if (MODE == ROW_POISSON) {
-        const float wu = a.wrow[row], wu2 = wu * wu;
+        const float wu = active ? a.wrow[row] : 0.0f, wu2 = wu * wu;
         const float* A = a.in0 + off;
         auto pot = [&](int v) -> float {
             if (row == 0 && v == 0) return 0.0f;
             float wv = a.wcol[v];
             return A[v] / (wu2 + wv * wv);
         };
-        dct3_row([&](int v) { return row == 0 && v == 0 ? 0.0f : pot(v) * wu; }, false, re, im, L, a.logL, a.tw, a.dw);
-        for (int k = threadIdx.x; k < L; k += blockDim.x) a.out0[off + k] = im[k];
-        __syncthreads();
-        dct3_row([&](int v) { return row == 0 && v == 0 ? 0.0f : pot(v) * a.wcol[v]; }, true, re, im, L, a.logL, a.tw, a.dw);
-        for (int k = threadIdx.x; k < L; k += blockDim.x) a.out1[off + k] = im[k];
+        float* o0 = a.out0 + off;
+        float* o1 = a.out1 + off;
+        dct3_row(R, active, [&](int v) { return pot(v) * wu; }, false, [&](int j, float y) { o0[j] = y; });
+        dct3_row(R, active, [&](int v) { return pot(v) * a.wcol[v]; }, true, [&](int j, float y) { o1[j] = y; });
         if (a.want_phi) {
-            __syncthreads();
-            dct3_row(pot, false, re, im, L, a.logL, a.tw, a.dw);
-            for (int k = threadIdx.x; k < L; k += blockDim.x) a.out2[off + k] = im[k];
+            float* o2 = a.out2 + off;
+            dct3_row(R, active, pot, false, [&](int j, float y) { o2[j] = y; });
         }
     } else if (MODE == ROW_INV_FIELDS) {
         const float* X = a.in0 + off;
-        dct3_row([&](int v) { return X[v]; }, true, re, im, L, a.logL, a.tw, a.dw);
-        for (int k = threadIdx.x; k < L; k += blockDim.x) a.out0[off + k] = im[k];
-        __syncthreads();
         const float* Y = a.in1 + off;
-        dct3_row([&](int v) { return Y[v]; }, false, re, im, L, a.logL, a.tw, a.dw);
-        for (int k = threadIdx.x; k < L; k += blockDim.x) a.out1[off + k] = im[k];
+        float* o0 = a.out0 + off;
+        float* o1 = a.out1 + off;
+        dct3_row(R, active, [&](int v) { return X[v]; }, true, [&](int j, float y) { o0[j] = y; });
+        dct3_row(R, active, [&](int v) { return Y[v]; }, false, [&](int j, float y) { o1[j] = y; });
     } else if (MODE == ROW_INV_PHI) {
         const float* X = a.in0 + off;
-        dct3_row([&](int v) { return X[v]; }, false, re, im, L, a.logL, a.tw, a.dw);
-        for (int k = threadIdx.x; k < L; k += blockDim.x) a.out0[off + k] = im[k];
+        float* o0 = a.out0 + off;
+        dct3_row(R, active, [&](int v) { return X[v]; }, false, [&](int j, float y) { o0[j] = y; });
     }
 }
 
@@ -883,35 +955,51 @@ static void transpose(hipStream_t s, const float* in, float* out, int R, int C) 
     hipLaunchKernelGGL(k_transpose, g, b, 0, s, in, out, R, C);
 }
 
+static void launch_rows(hipStream_t s, int mode, RowArgs a) {
+    const int h = a.L >> 1;
+    a.T = h >= 4 ? h >> 2 : 1;  // L/8 threads per row (one for the tiny lengths)
+    a.logh = ilog2(h);
+    const int rpb = 256 / a.T > 0 ? 256 / a.T : 1;
+    const int block = a.T * rpb;
+    const int grid = (a.nrows + rpb - 1) / rpb;
+    const size_t lds = sizeof(float2) * (size_t)a.L * rpb;
+    switch (mode) {
+    case ROW_DCT2: hipLaunchKernelGGL(k_rows<ROW_DCT2>, dim3(grid), dim3(block), lds, s, a); break;
+    case ROW_DCT2_SCALE: hipLaunchKernelGGL(k_rows<ROW_DCT2_SCALE>, dim3(grid), dim3(block), lds, s, a); break;
+    case ROW_POISSON: hipLaunchKernelGGL(k_rows<ROW_POISSON>, dim3(grid), dim3(block), lds, s, a); break;
+    case ROW_INV_FIELDS: hipLaunchKernelGGL(k_rows<ROW_INV_FIELDS>, dim3(grid), dim3(block), lds, s, a); break;
+    default: hipLaunchKernelGGL(k_rows<ROW_INV_PHI>, dim3(grid), dim3(block), lds, s, a); break;
+    }
+}
+
 // Bins are y-major D[y][x].  P1: rows along x, DCT-II -> B[y][u]; transpose -> [u][y];
 // P2: rows along y, DCT-II + halving + scale -> A[u][v]; P3: rows along v, the Poisson divide and
 // the inverse transforms along y -> Ex'[u][y] (cos), Ey'[u][y] (sin), phi'[u][y]; transpose ->
 // [y][u]; P4: rows along u -> field_x = sin along x, field_y = cos along x, phi = cos along x.
 void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi) {
     const int nx = r.cnt_x, ny = r.cnt_y;
-    const int lnx = ilog2(nx), lny = ilog2(ny);
     RowArgs a{};
     // P1
     a.in0 = r.density;
     a.out0 = r.work0;
     a.L = nx;
-    a.logL = lnx;
     a.nrows = ny;
     a.tw = r.tw_x;
+    a.th = r.th_x;
     a.dw = r.dw_x;
-    hipLaunchKernelGGL(k_rows<ROW_DCT2>, dim3(ny), dim3(64), 2 * nx * sizeof(float), s, a);
+    launch_rows(s, ROW_DCT2, a);
     transpose(s, r.work0, r.work1, ny, nx);  // [u][y]
     // P2
     a = RowArgs{};
     a.in0 = r.work1;
     a.out0 = r.work0;
     a.L = ny;
-    a.logL = lny;
     a.nrows = nx;
     a.tw = r.tw_y;
+    a.th = r.th_y;
     a.dw = r.dw_y;
     a.scale = 4.0 / (double)nx / (double)ny;
-    hipLaunchKernelGGL(k_rows<ROW_DCT2_SCALE>, dim3(nx), dim3(64), 2 * ny * sizeof(float), s, a);  // A[u][v]
+    launch_rows(s, ROW_DCT2_SCALE, a);  // A[u][v]
     // P3
     a = RowArgs{};
     a.in0 = r.work0;
@@ -919,14 +1007,14 @@ void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi) {
     a.out1 = r.work2;  // Ey' [u][y]
     a.out2 = r.work3;  // phi' [u][y]
     a.L = ny;
-    a.logL = lny;
     a.nrows = nx;
     a.tw = r.tw_y;
+    a.th = r.th_y;
     a.dw = r.dw_y;
     a.wrow = r.wx;
     a.wcol = r.wy;
     a.want_phi = want_phi ? 1 : 0;
-    hipLaunchKernelGGL(k_rows<ROW_POISSON>, dim3(nx), dim3(64), 2 * ny * sizeof(float), s, a);
+    launch_rows(s, ROW_POISSON, a);
     transpose(s, r.work1, r.work0, nx, ny);  // Ex [y][u]
     transpose(s, r.work2, r.work1, nx, ny);  // Ey [y][u]
     if (want_phi) transpose(s, r.work3, r.work2, nx, ny);
@@ -937,21 +1025,21 @@ void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi) {
     a.out0 = r.field_x;
     a.out1 = r.field_y;
     a.L = nx;
-    a.logL = lnx;
     a.nrows = ny;
     a.tw = r.tw_x;
+    a.th = r.th_x;
     a.dw = r.dw_x;
-    hipLaunchKernelGGL(k_rows<ROW_INV_FIELDS>, dim3(ny), dim3(64), 2 * nx * sizeof(float), s, a);
+    launch_rows(s, ROW_INV_FIELDS, a);
     if (want_phi) {
         RowArgs p{};
         p.in0 = r.work2;
         p.out0 = r.phi;
         p.L = nx;
-        p.logL = lnx;
         p.nrows = ny;
         p.tw = r.tw_x;
+        p.th = r.th_x;
         p.dw = r.dw_x;
-        hipLaunchKernelGGL(k_rows<ROW_INV_PHI>, dim3(ny), dim3(64), 2 * nx * sizeof(float), s, p);
+        launch_rows(s, ROW_INV_PHI, p);
     }
 }
 

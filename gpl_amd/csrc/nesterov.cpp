@@ -824,11 +824,15 @@ void gpl_ctx::lock_region(int b) {
 
 namespace {
 
-void make_tables(int L, std::vector<float2>& tw, std::vector<float2>& dw) {
+// Twiddle tables of the length-L transforms (h = L/2), computed in double and rounded once.
+void make_tables(int L, std::vector<float2>& tw, std::vector<float2>& th, std::vector<float2>& dw) {
     const double pi = 3.14159265358979323846;
-    tw.resize(std::max(1, L / 2));
+    const int h = L / 2;
+    tw.resize(h);
+    th.resize(size_t(h) + 1);
     dw.resize(L);
-    for (int k = 0; k < L / 2; ++k) tw[k] = make_float2((float)std::cos(-2.0 * pi * k / L), (float)std::sin(-2.0 * pi * k / L));
+    for (int m = 0; m < h; ++m) tw[m] = make_float2((float)std::cos(-2.0 * pi * m / h), (float)std::sin(-2.0 * pi * m / h));
+    for (int k = 0; k <= h; ++k) th[k] = make_float2((float)std::cos(-2.0 * pi * k / L), (float)std::sin(-2.0 * pi * k / L));
     for (int k = 0; k < L; ++k) dw[k] = make_float2((float)std::cos(pi * k / (2.0 * L)), (float)-std::sin(pi * k / (2.0 * L)));
 }
 
@@ -853,8 +857,8 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
     }
     for (int b = 0; b < p->num_regions; ++b) {
         const gpl_region& R = p->regions[b];
-        if (!pow2(R.cnt_x) || !pow2(R.cnt_y) || R.cnt_x > 4096 || R.cnt_y > 4096) {
-            fprintf(stderr, "gpl_ctx_create: bin grid %d x %d is not a power of two in [2, 4096]\n", R.cnt_x, R.cnt_y);
+        if (!pow2(R.cnt_x) || !pow2(R.cnt_y) || R.cnt_x > 2048 || R.cnt_y > 2048) {
+            fprintf(stderr, "gpl_ctx_create: bin grid %d x %d is not a power of two in [2, 2048]\n", R.cnt_x, R.cnt_y);
             return GPL_ERR_REFUSED;
         }
     }
@@ -1018,12 +1022,14 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             d.work3 = c->dalloc<float>(M);
             d.wx = c->upload(r.R.wx, r.R.cnt_x);
             d.wy = c->upload(r.R.wy, r.R.cnt_y);
-            std::vector<float2> tw, dw;
-            make_tables(r.R.cnt_x, tw, dw);
+            std::vector<float2> tw, th, dw;
+            make_tables(r.R.cnt_x, tw, th, dw);
             d.tw_x = c->upload(tw.data(), tw.size());
+            d.th_x = c->upload(th.data(), th.size());
             d.dw_x = c->upload(dw.data(), dw.size());
-            make_tables(r.R.cnt_y, tw, dw);
+            make_tables(r.R.cnt_y, tw, th, dw);
             d.tw_y = c->upload(tw.data(), tw.size());
+            d.th_y = c->upload(th.data(), th.size());
             d.dw_y = c->upload(dw.data(), dw.size());
             for (int s = 0; s < S_COUNT; ++s) {
                 r.v[s] = c->dalloc<float2>(r.n);
