@@ -138,6 +138,7 @@ def main():
     gp.loop_step(args.warmup + args.steps)
     kt = gp.kernel_times()
     gp.close()
+    resorts, global_adds = kt.pop("scatter_global_adds", (0, 0.0))
     fams = {k: v for k, v in kt.items() if algorithmic_bytes(k, N, P, E, M) is not None and v[0] > 0}
     dom = max(fams, key=lambda k: fams[k][1])
     launches, ms = fams[dom]

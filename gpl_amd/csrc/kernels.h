@@ -55,25 +55,33 @@ struct RegionDev {
 
 struct CellsDev {
     int4* dbox;
-    const float* dscale;
-    const uint8_t* flags;       // GPL_CELL_* (locked is updated on convergence)
-    uint8_t* centered;          // the density box has been centred once (pins follow it)
-    const int32_t* pin_start;   // num_cells + 1
-    const int32_t* cp_net;      // per cell-pin: the net (sorted net index)
-    const int2* cp_off;         // per cell-pin: the offset
-    const float2* pre;          // (pin count, dx*dy) per cell, as float
+    int2* ctr;                  // the density box's centre; (INT_MIN, INT_MIN) until first centred
+    float* dscale;
+    uint8_t* flags;             // GPL_CELL_* (locked is updated on convergence)
+    int32_t* pin_start;         // num_cells + 1
+    int32_t* cp_net;            // per cell-pin: the net (sorted net index)
+    int2* cp_off;               // per cell-pin: the offset
+    float2* pre;                // (pin count, dx*dy) per cell, as float
+    int32_t* old_of;            // the problem's index of each device cell
+};
+
+// One net's state in one 64-byte record (one cache line for the gradient's per-pin gather):
+// box, the WA sums along x (min_x, x_min_x, max_x, x_max_x) and y, and the net weight.
+struct __align__(16) NetRec {
+    int4 box;
+    float4 sx;
+    float4 sy;
+    float w;
+    int32_t pad[3];
 };
 
 struct NetsDev {
     int32_t num_nets;
     const int32_t* start;   // num_nets + 1, into the net-order pin arrays
-    const int32_t* pcell;   // per net-pin: cell or -1
+    int32_t* pcell;         // per net-pin: cell or -1 (renamed by a re-sort)
     const int2* poff;       // per net-pin: offset
     const int2* pxy0;       // per net-pin: position as init left it
-    const float* weight;
-    int4* box;
-    float4* sx;             // min_x, x_min_x, max_x, x_max_x
-    float4* sy;             // min_y, y_min_y, max_y, y_max_y
+    NetRec* rec;
     int32_t num_small;      // nets [0, num_small) take one thread each; the rest a block each
 };
 
@@ -87,16 +95,19 @@ struct Sums {
 };
 
 constexpr int kPartials = 2048;  // per-block partial slots per quantity
+constexpr int kPlaceChunk = 512;  // cells per block iteration of the scatter family
 
 // ---- launchers (stream-ordered, asynchronous) -----------------------------------------------
 void launch_zero_bins(hipStream_t s, const RegionDev& r);
 // init_density1's layout-inside + centring: cur_slp = prev_slp = cur = density centre; scatter.
-void launch_center_init(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* cur_slp, float2* prev_slp, float2* cur);
+void launch_center_init(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* cur_slp, float2* prev_slp, float2* cur,
+                        unsigned long long* global_adds);
+// (global_adds, when not null, counts the 64-bit atomic adds the scatter sent to the bins in HBM)
 // centre the region's cells at `xy` and scatter them into the bins.
-void launch_center_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy);
+void launch_center_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy, unsigned long long* global_adds);
 // nesterovUpdateCoordinates + centring + scatter.
 void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
-                           const float2* cur, float2* next, float2* next_slp, float step, float coeff);
+                           const float2* cur, float2* next, float2* next_slp, float step, float coeff, unsigned long long* global_adds);
 // per bin: density, overflow partials, accumulators cleared.
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials);
 // the Poisson solve of r.density into r.field_x / r.field_y (and r.phi when want_phi).
@@ -121,6 +132,34 @@ void launch_hpwl(hipStream_t s, const NetsDev& n, const CellsDev& c, unsigned lo
 // the objective (exact exponential, continuous coordinates): WA value and gradient per cell.
 void launch_objective(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, const float2* xy, float coef,
                       double* net_val, float2* grad_wl);
+
+// ---- spatial re-sort of the cells (device order only; results do not depend on it) ---------
+constexpr int kMaxRegions = 32;
+struct RegionTable {
+    int32_t n;
+    int32_t begin[kMaxRegions + 1];
+    int32_t lx[kMaxRegions], ly[kMaxRegions], cnt_x[kMaxRegions], cnt_y[kMaxRegions];
+    double size_x[kMaxRegions], size_y[kMaxRegions];
+};
+struct ResortBufs {
+    uint32_t* keys_in;
+    uint32_t* keys_out;
+    int32_t* idx_in;
+    int32_t* perm;   // new -> old
+    int32_t* inv;    // old -> new
+    int32_t* cnt;    // pins per new cell, then exclusive-scanned into the new pin starts (N + 1)
+    void* temp;
+    size_t temp_bytes;
+};
+size_t resort_temp_bytes(int n_cells);
+// keys (region, Morton code of the current bin) and the stable sort: perm and inv
+void launch_resort_sort(hipStream_t s, int n_cells, const int2* ctr, const RegionTable& t, ResortBufs& b);
+// every per-cell array gathered into `out` (pin CSR rebuilt), old_of composed
+void launch_resort_cells(hipStream_t s, int n_cells, const CellsDev& in, const CellsDev& out, ResortBufs& b);
+// one region's per-cell float pairs gathered: out[k] = in[perm[begin + k] - begin]
+void launch_gather_f2(hipStream_t s, int n, int begin, const int32_t* perm, const float2* in, float2* out);
+// net pins' cells renamed: pcell[k] = inv[pcell[k]]
+void launch_remap_pcell(hipStream_t s, int n_pins, int32_t* pcell, const int32_t* inv);
 
 int block_partials_bins(const RegionDev& r);
 int block_partials_cells(const RegionDev& r);

@@ -10,6 +10,8 @@
 
 #include <climits>
 
+#include <hipcub/hipcub.hpp>
+
 namespace gplk {
 
 constexpr int kBlock = 256;
@@ -81,24 +83,6 @@ __device__ __forceinline__ int4 centre_box(int4 d, float x, float y) {
     return make_int4(cx - hx, cy - hy, cx + hx, cy + hy);
 }
 
-// nesterov.rs:2225-2250: one cell's overlaps, truncated to int64, added into the bins.
-__device__ __forceinline__ void scatter_cell(const RegionDev& r, int4 d, float scale, uint8_t flags) {
-    int x0, x1, y0, y1;
-    bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
-    bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
-    const bool inst = flags & 1u, macro = inst && (flags & 2u);
-    unsigned long long* acc = inst ? r.inst : r.filler;
-    for (int y = y0; y < y1; ++y) {
-        int by0 = r.bin_y[y], by1 = r.bin_y[y + 1];
-        for (int x = x0; x < x1; ++x) {
-            float a = overlap_area(r.bin_x[x], by0, r.bin_x[x + 1], by1, d) * scale;
-            if (macro) a *= r.target_density;
-            long long v = (long long)a;
-            if (v != 0) atomicAdd(acc + (size_t)y * r.cnt_x + x, (unsigned long long)v);
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------------------------
 // Scatter family
 // ---------------------------------------------------------------------------------------------
@@ -111,171 +95,150 @@ __global__ void k_zero_bins(unsigned long long* a, unsigned long long* b, size_t
     }
 }
 
-// initDensity1 (nesterov.rs:2546-2570) per cell: the density box moved inside the grid
-// (updateDensityCoordiLayoutInside, 2173), its centre as cur_slp / prev_slp / cur, the box
-// centred there, and the scatter.
-__global__ void k_center_init(RegionDev r, CellsDev c, float2* cur_slp, float2* prev_slp, float2* cur) {
-    int n = r.cell_end - r.cell_begin;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int g = r.cell_begin + k;
-        int4 d = c.dbox[g];
-        int ddx = d.z - d.x, ddy = d.w - d.y;
-        float tlx = rmax((float)d.x, (float)r.lx), tly = rmax((float)d.y, (float)r.ly);
-        if (tlx + (float)ddx > (float)r.ux) tlx = (float)(r.ux - ddx);
-        if (tly + (float)ddy > (float)r.uy) tly = (float)(r.uy - ddy);
-        int lx = (int)tlx, ly = (int)tly;
-        d = make_int4(lx, ly, lx + ddx, ly + ddy);
-        float2 p = make_float2((float)((d.z + d.x) / 2), (float)((d.w + d.y) / 2));
-        cur_slp[k] = p;
-        prev_slp[k] = p;
-        cur[k] = p;
-        d = centre_box(d, p.x, p.y);
-        c.dbox[g] = d;
-        c.centered[g] = 1;
-        scatter_cell(r, d, c.dscale[g], c.flags[g]);
-    }
-}
-
-// updateGCellDensityCenterLocation (nesterov.rs:2268) + the scatter, at coordinates xy.
-__global__ void k_center_scatter(RegionDev r, CellsDev c, const float2* xy) {
-    int n = r.cell_end - r.cell_begin;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int g = r.cell_begin + k;
-        float2 p = xy[k];
-        int4 d = centre_box(c.dbox[g], p.x, p.y);
-        c.dbox[g] = d;
-        c.centered[g] = 1;
-        scatter_cell(r, d, c.dscale[g], c.flags[g]);
-    }
-}
-
-// nesterovUpdateCoordinates (nesterov.rs:2718-2745) fused with the centring and the scatter.
+// The scatter family: a cell's new density box from one of three rules, then its overlaps with
+// the bins added into the int64 accumulators (nesterov.rs:2225-2250).
+//   PLACE_INIT    initDensity1 (nesterov.rs:2546-2570): the box moved inside the grid
+//                 (updateDensityCoordiLayoutInside, 2173), its centre as cur_slp / prev_slp / cur,
+//                 the box centred there
+//   PLACE_CENTER  updateGCellDensityCenterLocation (nesterov.rs:2268) at coordinates xy
+//   PLACE_UPDATE  nesterovUpdateCoordinates (nesterov.rs:2718-2745): next and next_slp from the
+//                 step and the momentum, kept inside, the box centred at next_slp
 // A block takes kChunk consecutive cells (the device keeps each region's cells in Morton order of
-// their bins, so a chunk covers a compact window): the block's bin window is reduced first; when
-// it fits the LDS tile the chunk's overlaps are summed there with LDS atomics and each touched
-// bin gets one global atomic, else every overlap goes straight to global memory.  Integer sums:
-// the result does not depend on the path.
-constexpr int kCellsPerThread = 2;
-constexpr int kChunk = kBlock * kCellsPerThread;
-constexpr int kTile = 2048;
+// their bins, re-sorted when that order has gone stale, so a chunk touches few distinct bins).  Its
+// overlaps are summed in an LDS hash table keyed by bin (kSlots slots, linear probing, keys set
+// by compare-and-swap), then each occupied slot is added into the bins with one global atomic;
+// an overlap that finds no slot within kProbe probes goes straight to global memory.  The sums
+// are of integers: the result does not depend on the path.
+enum PlaceMode { PLACE_INIT = 0, PLACE_CENTER = 1, PLACE_UPDATE = 2 };
+constexpr int kCellsPerThread = kPlaceChunk / kBlock;
+constexpr int kChunk = kPlaceChunk;
+constexpr int kSlots = 2048;
+constexpr int kProbe = 16;
+constexpr unsigned kEmpty = 0xffffffffu;
 
-__global__ void __launch_bounds__(kBlock) k_update_scatter(RegionDev r, CellsDev c, const float2* cur_slp, const float2* cur_sum,
-                                                           const float2* cur, float2* next, float2* next_slp, float s, float coeff) {
-    __shared__ unsigned long long t_inst[kTile];
-    __shared__ unsigned long long t_fill[kTile];
-    __shared__ int s_win[4][kBlock / 64];
+struct PlaceArgs {
+    const float2* cur_slp;  // UPDATE: read; INIT: written
+    const float2* cur_sum;
+    const float2* cur;      // UPDATE: read; INIT: written (as cur)
+    float2* next;           // UPDATE: written; INIT: prev_slp written
+    float2* next_slp;       // UPDATE: written; INIT: cur_slp written
+    float2* cur_out;        // INIT: cur written
+    const float2* xy;       // CENTER
+    float s, coeff;
+    unsigned long long* global_adds;  // count of the atomic adds sent to HBM
+};
+
+// Adds v into the LDS table's slot for bin k; false when no slot was found.
+__device__ __forceinline__ bool table_add(unsigned* keys, unsigned long long* vals, unsigned k, unsigned long long v) {
+    unsigned slot = (k * 2654435761u) >> 21;  // 11 bits: kSlots = 2048
+#pragma unroll 1
+    for (int probe = 0; probe < kProbe; ++probe) {
+        unsigned cur = keys[slot];
+        if (cur == kEmpty) {
+            const unsigned old = atomicCAS(&keys[slot], kEmpty, k);
+            cur = old == kEmpty ? k : old;
+        }
+        if (cur == k) {
+            atomicAdd(&vals[slot], v);
+            return true;
+        }
+        slot = (slot + 1) & (kSlots - 1);
+    }
+    return false;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev c, PlaceArgs pa) {
+    __shared__ unsigned t_key[kSlots];
+    __shared__ unsigned long long t_inst[kSlots];
+    __shared__ unsigned long long t_fill[kSlots];
+    __shared__ unsigned s_adds;
     const int n = r.cell_end - r.cell_begin;
     const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_adds = 0;
     for (int base = blockIdx.x * kChunk; base < n; base += gridDim.x * kChunk) {
-        int4 box[kCellsPerThread];
-        int span[kCellsPerThread][4];
-        uint8_t fl[kCellsPerThread];
-        float sc[kCellsPerThread];
-        int wx0 = INT_MAX, wx1 = INT_MIN, wy0 = INT_MAX, wy1 = INT_MIN;
+        for (int i = threadIdx.x; i < kSlots; i += kBlock) {
+            t_key[i] = kEmpty;
+            t_inst[i] = 0ull;
+            t_fill[i] = 0ull;
+        }
+        __syncthreads();
+        unsigned adds = 0;
 #pragma unroll
         for (int j = 0; j < kCellsPerThread; ++j) {
             const int k = base + j * kBlock + threadIdx.x;
-            span[j][0] = span[j][1] = span[j][2] = span[j][3] = 0;
             if (k >= n) continue;
             const int g = r.cell_begin + k;
             int4 d = c.dbox[g];
             const uint8_t f = c.flags[g];
             float2 ns;
-            if (f & 4u) {
-                next[k] = cur[k];
-                ns = cur_slp[k];
+            if (MODE == PLACE_INIT) {
+                const int ddx = d.z - d.x, ddy = d.w - d.y;
+                float tlx = rmax((float)d.x, lx), tly = rmax((float)d.y, ly);
+                if (tlx + (float)ddx > ux) tlx = (float)(r.ux - ddx);
+                if (tly + (float)ddy > uy) tly = (float)(r.uy - ddy);
+                const int ilx = (int)tlx, ily = (int)tly;
+                d = make_int4(ilx, ily, ilx + ddx, ily + ddy);
+                ns = make_float2((float)((d.z + d.x) / 2), (float)((d.w + d.y) / 2));
+                pa.next_slp[k] = ns;  // cur_slp
+                pa.next[k] = ns;      // prev_slp
+                pa.cur_out[k] = ns;   // cur
+            } else if (MODE == PLACE_CENTER) {
+                ns = pa.xy[k];
             } else {
-                float2 a = cur_slp[k], sm = cur_sum[k], cu = cur[k];
-                float nx = a.x + s * sm.x;
-                float ny = a.y + s * sm.y;
-                float sx = nx + coeff * (nx - cu.x);
-                float sy = ny + coeff * (ny - cu.y);
-                float hx = (float)(d.z - d.x) / 2.0f, hy = (float)(d.w - d.y) / 2.0f;
-                next[k] = make_float2(inside(nx, hx, lx, ux), inside(ny, hy, ly, uy));
-                ns = make_float2(inside(sx, hx, lx, ux), inside(sy, hy, ly, uy));
+                if (f & 4u) {
+                    pa.next[k] = pa.cur[k];
+                    ns = pa.cur_slp[k];
+                } else {
+                    const float2 a = pa.cur_slp[k], sm = pa.cur_sum[k], cu = pa.cur[k];
+                    const float nx = a.x + pa.s * sm.x;
+                    const float ny = a.y + pa.s * sm.y;
+                    const float sx = nx + pa.coeff * (nx - cu.x);
+                    const float sy = ny + pa.coeff * (ny - cu.y);
+                    const float hx = (float)(d.z - d.x) / 2.0f, hy = (float)(d.w - d.y) / 2.0f;
+                    pa.next[k] = make_float2(inside(nx, hx, lx, ux), inside(ny, hy, ly, uy));
+                    ns = make_float2(inside(sx, hx, lx, ux), inside(sy, hy, ly, uy));
+                }
+                pa.next_slp[k] = ns;
             }
-            next_slp[k] = ns;
             d = centre_box(d, ns.x, ns.y);
             c.dbox[g] = d;
-            box[j] = d;
-            fl[j] = f;
-            sc[j] = c.dscale[g];
-            bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, span[j][0], span[j][1]);
-            bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, span[j][2], span[j][3]);
-            if (span[j][0] < span[j][1] && span[j][2] < span[j][3]) {
-                wx0 = min(wx0, span[j][0]);
-                wx1 = max(wx1, span[j][1]);
-                wy0 = min(wy0, span[j][2]);
-                wy1 = max(wy1, span[j][3]);
-            }
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            wx0 = min(wx0, __shfl_down(wx0, o, 64));
-            wx1 = max(wx1, __shfl_down(wx1, o, 64));
-            wy0 = min(wy0, __shfl_down(wy0, o, 64));
-            wy1 = max(wy1, __shfl_down(wy1, o, 64));
-        }
-        if (lane == 0) {
-            s_win[0][wv] = wx0;
-            s_win[1][wv] = wx1;
-            s_win[2][wv] = wy0;
-            s_win[3][wv] = wy1;
-        }
-        __syncthreads();
-        wx0 = s_win[0][0];
-        wx1 = s_win[1][0];
-        wy0 = s_win[2][0];
-        wy1 = s_win[3][0];
-        for (int i = 1; i < kBlock / 64; ++i) {
-            wx0 = min(wx0, s_win[0][i]);
-            wx1 = max(wx1, s_win[1][i]);
-            wy0 = min(wy0, s_win[2][i]);
-            wy1 = max(wy1, s_win[3][i]);
-        }
-        // no cell of the chunk touches a bin: the window stays empty (W = H = 0)
-        const int W = wx1 > wx0 ? wx1 - wx0 : 0, H = wy1 > wy0 ? wy1 - wy0 : 0;
-        const bool tiled = W > 0 && H > 0 && (long long)W * H <= kTile;
-        if (tiled) {
-            for (int i = threadIdx.x; i < W * H; i += kBlock) {
-                t_inst[i] = 0ull;
-                t_fill[i] = 0ull;
-            }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int j = 0; j < kCellsPerThread; ++j) {
-            if (span[j][0] >= span[j][1] || span[j][2] >= span[j][3]) continue;
-            const int4 d = box[j];
-            const bool inst = fl[j] & 1u, macro = inst && (fl[j] & 2u);
-            for (int y = span[j][2]; y < span[j][3]; ++y) {
+            c.ctr[g] = make_int2((d.x + d.z) / 2, (d.y + d.w) / 2);
+            int x0, x1, y0, y1;
+            bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
+            bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
+            const float sc = c.dscale[g];
+            const bool inst = f & 1u, macro = inst && (f & 2u);
+            unsigned long long* vals = inst ? t_inst : t_fill;
+            unsigned long long* glob = inst ? r.inst : r.filler;
+            for (int y = y0; y < y1; ++y) {
                 const int by0 = r.bin_y[y], by1 = r.bin_y[y + 1];
-                for (int x = span[j][0]; x < span[j][1]; ++x) {
-                    float a = overlap_area(r.bin_x[x], by0, r.bin_x[x + 1], by1, d) * sc[j];
+                for (int x = x0; x < x1; ++x) {
+                    float a = overlap_area(r.bin_x[x], by0, r.bin_x[x + 1], by1, d) * sc;
                     if (macro) a *= r.target_density;
                     const long long v = (long long)a;
                     if (v == 0) continue;
-                    if (tiled) {
-                        const int t = (y - wy0) * W + (x - wx0);
-                        atomicAdd(inst ? &t_inst[t] : &t_fill[t], (unsigned long long)v);
-                    } else {
-                        atomicAdd((inst ? r.inst : r.filler) + (size_t)y * r.cnt_x + x, (unsigned long long)v);
+                    const unsigned kb = (unsigned)y * (unsigned)r.cnt_x + (unsigned)x;
+                    if (!table_add(t_key, vals, kb, (unsigned long long)v)) {
+                        atomicAdd(glob + kb, (unsigned long long)v);
+                        ++adds;
                     }
                 }
             }
         }
-        if (tiled) {
-            __syncthreads();
-            for (int i = threadIdx.x; i < W * H; i += kBlock) {
-                const size_t kb = (size_t)(wy0 + i / W) * r.cnt_x + (wx0 + i % W);
-                const unsigned long long a = t_inst[i], b = t_fill[i];
-                if (a) atomicAdd(r.inst + kb, a);
-                if (b) atomicAdd(r.filler + kb, b);
-            }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kSlots; i += kBlock) {
+            const unsigned kb = t_key[i];
+            if (kb == kEmpty) continue;
+            const unsigned long long a = t_inst[i], b = t_fill[i];
+            if (a) atomicAdd(r.inst + kb, a);
+            if (b) atomicAdd(r.filler + kb, b);
+            adds += (a != 0) + (b != 0);
         }
+        if (adds) atomicAdd(&s_adds, adds);
         __syncthreads();
     }
+    if (threadIdx.x == 0 && s_adds && pa.global_adds) atomicAdd(pa.global_adds, (unsigned long long)s_adds);
 }
 
 // updateBinsGCellDensityArea's per-bin part (nesterov.rs:2252-2262): density, the overflow
@@ -561,169 +524,171 @@ __global__ void k_transpose(const float* in, float* out, int R, int C) {
 // A net pin's position: its cell's density centre plus its offset once the cell has been
 // centred, else where init left it (nesterov.rs:2150).
 __device__ __forceinline__ int2 pin_pos(const NetsDev& n, const CellsDev& c, int k) {
-    int cell = n.pcell[k];
-    if (cell >= 0 && c.centered[cell]) {
-        int4 d = c.dbox[cell];
-        int2 o = n.poff[k];
-        return make_int2((d.x + d.z) / 2 + o.x, (d.y + d.w) / 2 + o.y);
+    const int cell = n.pcell[k];
+    if (cell >= 0) {
+        const int2 m = c.ctr[cell];
+        if (m.x != INT_MIN) {
+            const int2 o = n.poff[k];
+            return make_int2(m.x + o.x, m.y + o.y);
+        }
     }
     return n.pxy0[k];
 }
 
-// One thread per net, the net's pins in its order (the reference's float sums, bit for bit).
-__global__ void k_wa_small(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl) {
-    __shared__ unsigned long long sh[kBlock / 64];
-    unsigned long long h = 0;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n.num_small; e += gridDim.x * blockDim.x) {
-        int s = n.start[e], t = n.start[e + 1];
-        int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
-        for (int k = s; k < t; ++k) {
-            int2 p = pin_pos(n, c, k);
-            lx = min(lx, p.x);
-            ly = min(ly, p.y);
-            ux = max(ux, p.x);
-            uy = max(uy, p.y);
-        }
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int k = s; k < t; ++k) {
-            int2 p = pin_pos(n, c, k);
-            float e0 = (float)(lx - p.x) * cx, e1 = (float)(p.x - ux) * cx;
-            float e2 = (float)(ly - p.y) * cy, e3 = (float)(p.y - uy) * cy;
-            if (e0 > -300.0f) {
-                float v = fast_exp(e0);
-                a.x += v;
-                a.y += (float)p.x * v;
-            }
-            if (e1 > -300.0f) {
-                float v = fast_exp(e1);
-                a.z += v;
-                a.w += (float)p.x * v;
-            }
-            if (e2 > -300.0f) {
-                float v = fast_exp(e2);
-                b.x += v;
-                b.y += (float)p.y * v;
-            }
-            if (e3 > -300.0f) {
-                float v = fast_exp(e3);
-                b.z += v;
-                b.w += (float)p.y * v;
-            }
-        }
-        n.box[e] = make_int4(lx, ly, ux, uy);
-        n.sx[e] = a;
-        n.sy[e] = b;
-        if (ux >= lx) h += (unsigned long long)((long long)ux - lx + (long long)uy - ly);
+// The four shifted exponentials of one pin (nesterov.rs:2354-2385) added into the net sums.
+__device__ __forceinline__ void wa_pin(int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float4& a, float4& b) {
+    const float e0 = (float)(lx - p.x) * cx, e1 = (float)(p.x - ux) * cx;
+    const float e2 = (float)(ly - p.y) * cy, e3 = (float)(p.y - uy) * cy;
+    if (e0 > -300.0f) {
+        const float v = fast_exp(e0);
+        a.x += v;
+        a.y += (float)p.x * v;
     }
-    h = wave_sum(h);
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) sh[w] = h;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
-        if (t) atomicAdd(hpwl, t);
+    if (e1 > -300.0f) {
+        const float v = fast_exp(e1);
+        a.z += v;
+        a.w += (float)p.x * v;
+    }
+    if (e2 > -300.0f) {
+        const float v = fast_exp(e2);
+        b.x += v;
+        b.y += (float)p.y * v;
+    }
+    if (e3 > -300.0f) {
+        const float v = fast_exp(e3);
+        b.z += v;
+        b.w += (float)p.y * v;
     }
 }
 
-// One block per high-fanout net: the box and the sums reduced across the block (FP64 partial
-// sums, rounded once to float).
-__global__ void k_wa_big(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl) {
+constexpr int kRegPins = 16;  // pins a thread keeps in registers between the box and the sums
+
+// updateWireLengthForceWA (nesterov.rs:2325-2388) and getHpwl (2297) in one launch: the first
+// (num_nets - num_small) blocks take one high-fanout net each (box and sums reduced across the
+// block, FP64 partial sums rounded once to float); the rest take one small net per thread, its
+// pins in the net's order (the reference's float sums, bit for bit), the first kRegPins pin
+// positions held in registers between the box pass and the sums pass.  SUMS false: the HPWL only.
+template <bool SUMS>
+__global__ void __launch_bounds__(kBlock) k_wa(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl) {
     __shared__ int shi[4][kBlock / 64];
-    __shared__ double shd[kBlock / 64];
-    const int e = n.num_small + blockIdx.x;
-    const int s = n.start[e], t = n.start[e + 1];
-    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
-    for (int k = s + threadIdx.x; k < t; k += blockDim.x) {
-        int2 p = pin_pos(n, c, k);
-        lx = min(lx, p.x);
-        ly = min(ly, p.y);
-        ux = max(ux, p.x);
-        uy = max(uy, p.y);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        lx = min(lx, __shfl_down(lx, o, 64));
-        ly = min(ly, __shfl_down(ly, o, 64));
-        ux = max(ux, __shfl_down(ux, o, 64));
-        uy = max(uy, __shfl_down(uy, o, 64));
-    }
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) {
-        shi[0][w] = lx;
-        shi[1][w] = ly;
-        shi[2][w] = ux;
-        shi[3][w] = uy;
-    }
-    __syncthreads();
-    lx = shi[0][0];
-    ly = shi[1][0];
-    ux = shi[2][0];
-    uy = shi[3][0];
-    for (int i = 1; i < kBlock / 64; ++i) {
-        lx = min(lx, shi[0][i]);
-        ly = min(ly, shi[1][i]);
-        ux = max(ux, shi[2][i]);
-        uy = max(uy, shi[3][i]);
-    }
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = s + threadIdx.x; k < t; k += blockDim.x) {
-        int2 p = pin_pos(n, c, k);
-        float e0 = (float)(lx - p.x) * cx, e1 = (float)(p.x - ux) * cx;
-        float e2 = (float)(ly - p.y) * cy, e3 = (float)(p.y - uy) * cy;
-        if (e0 > -300.0f) {
-            float v = fast_exp(e0);
-            acc[0] += v;
-            acc[1] += (double)((float)p.x * v);
-        }
-        if (e1 > -300.0f) {
-            float v = fast_exp(e1);
-            acc[2] += v;
-            acc[3] += (double)((float)p.x * v);
-        }
-        if (e2 > -300.0f) {
-            float v = fast_exp(e2);
-            acc[4] += v;
-            acc[5] += (double)((float)p.y * v);
-        }
-        if (e3 > -300.0f) {
-            float v = fast_exp(e3);
-            acc[6] += v;
-            acc[7] += (double)((float)p.y * v);
-        }
-    }
-    float r[8];
-    for (int q = 0; q < 8; ++q) r[q] = (float)block_sum(acc[q], shd);
-    if (threadIdx.x == 0) {
-        n.box[e] = make_int4(lx, ly, ux, uy);
-        n.sx[e] = make_float4(r[0], r[1], r[2], r[3]);
-        n.sy[e] = make_float4(r[4], r[5], r[6], r[7]);
-        if (ux >= lx) atomicAdd(hpwl, (unsigned long long)((long long)ux - lx + (long long)uy - ly));
-    }
-}
-
-// HPWL only (nesterov.rs:2297): boxes of every net.
-__global__ void k_hpwl(NetsDev n, CellsDev c, unsigned long long* hpwl) {
-    __shared__ unsigned long long sh[kBlock / 64];
-    unsigned long long h = 0;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n.num_nets; e += gridDim.x * blockDim.x) {
+    __shared__ double shd[8][kBlock / 64];
+    __shared__ unsigned long long shh[kBlock / 64];
+    const int nbig = n.num_nets - n.num_small;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if ((int)blockIdx.x < nbig) {
+        const int e = n.num_small + blockIdx.x;
+        const int s = n.start[e], t = n.start[e + 1];
         int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
-        for (int k = n.start[e]; k < n.start[e + 1]; ++k) {
-            int2 p = pin_pos(n, c, k);
+        for (int k = s + threadIdx.x; k < t; k += blockDim.x) {
+            const int2 p = pin_pos(n, c, k);
             lx = min(lx, p.x);
             ly = min(ly, p.y);
             ux = max(ux, p.x);
             uy = max(uy, p.y);
         }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lx = min(lx, __shfl_down(lx, o, 64));
+            ly = min(ly, __shfl_down(ly, o, 64));
+            ux = max(ux, __shfl_down(ux, o, 64));
+            uy = max(uy, __shfl_down(uy, o, 64));
+        }
+        if (lane == 0) {
+            shi[0][w] = lx;
+            shi[1][w] = ly;
+            shi[2][w] = ux;
+            shi[3][w] = uy;
+        }
+        __syncthreads();
+        lx = shi[0][0];
+        ly = shi[1][0];
+        ux = shi[2][0];
+        uy = shi[3][0];
+        for (int i = 1; i < kBlock / 64; ++i) {
+            lx = min(lx, shi[0][i]);
+            ly = min(ly, shi[1][i]);
+            ux = max(ux, shi[2][i]);
+            uy = max(uy, shi[3][i]);
+        }
+        if (SUMS) {
+            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int k = s + threadIdx.x; k < t; k += blockDim.x) {
+                const int2 p = pin_pos(n, c, k);
+                float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
+                wa_pin(p, lx, ly, ux, uy, cx, cy, a, b);
+                acc[0] += a.x;
+                acc[1] += a.y;
+                acc[2] += a.z;
+                acc[3] += a.w;
+                acc[4] += b.x;
+                acc[5] += b.y;
+                acc[6] += b.z;
+                acc[7] += b.w;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                double v = wave_sum(acc[q]);
+                if (lane == 0) shd[q][w] = v;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                float r[8];
+                for (int q = 0; q < 8; ++q) {
+                    double v = 0.0;
+                    for (int i = 0; i < kBlock / 64; ++i) v += shd[q][i];
+                    r[q] = (float)v;
+                }
+                NetRec& R = n.rec[e];
+                R.box = make_int4(lx, ly, ux, uy);
+                R.sx = make_float4(r[0], r[1], r[2], r[3]);
+                R.sy = make_float4(r[4], r[5], r[6], r[7]);
+            }
+        }
+        if (threadIdx.x == 0 && ux >= lx) atomicAdd(hpwl, (unsigned long long)((long long)ux - lx + (long long)uy - ly));
+        return;
+    }
+    unsigned long long h = 0;
+    const int stride = (gridDim.x - nbig) * blockDim.x;
+    for (int e = (blockIdx.x - nbig) * blockDim.x + threadIdx.x; e < n.num_small; e += stride) {
+        const int s = n.start[e], deg = n.start[e + 1] - s;
+        int2 pos[kRegPins];
+        int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+#pragma unroll
+        for (int k = 0; k < kRegPins; ++k) {
+            if (k < deg) {
+                pos[k] = pin_pos(n, c, s + k);
+                lx = min(lx, pos[k].x);
+                ly = min(ly, pos[k].y);
+                ux = max(ux, pos[k].x);
+                uy = max(uy, pos[k].y);
+            }
+        }
+        for (int k = kRegPins; k < deg; ++k) {
+            const int2 p = pin_pos(n, c, s + k);
+            lx = min(lx, p.x);
+            ly = min(ly, p.y);
+            ux = max(ux, p.x);
+            uy = max(uy, p.y);
+        }
+        if (SUMS) {
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < kRegPins; ++k)
+                if (k < deg) wa_pin(pos[k], lx, ly, ux, uy, cx, cy, a, b);
+            for (int k = kRegPins; k < deg; ++k) wa_pin(pin_pos(n, c, s + k), lx, ly, ux, uy, cx, cy, a, b);
+            NetRec& R = n.rec[e];
+            R.box = make_int4(lx, ly, ux, uy);
+            R.sx = a;
+            R.sy = b;
+        }
         if (ux >= lx) h += (unsigned long long)((long long)ux - lx + (long long)uy - ly);
     }
     h = wave_sum(h);
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) sh[w] = h;
+    if (lane == 0) shh[w] = h;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
-        for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
+        for (int i = 0; i < kBlock / 64; ++i) t += shh[i];
         if (t) atomicAdd(hpwl, t);
     }
 }
@@ -759,8 +724,9 @@ __global__ void k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float 
             const int e = c.cp_net[q];
             const int2 o = c.cp_off[q];
             const int px = dcx + o.x, py = dcy + o.y;
-            const int4 b = nt.box[e];
-            const float4 SX = nt.sx[e], SY = nt.sy[e];
+            const NetRec rec = nt.rec[e];
+            const int4 b = rec.box;
+            const float4 SX = rec.sx, SY = rec.sy;
             const float fpx = (float)px, fpy = (float)py;
             float gmin_x = 0.0f, gmax_x = 0.0f, gmin_y = 0.0f, gmax_y = 0.0f;
             float e0 = (float)(b.x - px) * cx, e1 = (float)(px - b.z) * cx;
@@ -769,7 +735,7 @@ __global__ void k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float 
             if (e1 > -300.0f) gmax_x = wa_term_max(SX.z, SX.w, fast_exp(e1), cx, fpx);
             if (e2 > -300.0f) gmin_y = wa_term_min(SY.x, SY.y, fast_exp(e2), cy, fpy);
             if (e3 > -300.0f) gmax_y = wa_term_max(SY.z, SY.w, fast_exp(e3), cy, fpy);
-            const float w = nt.weight[e];
+            const float w = rec.w;
             float tx = gmin_x - gmax_x, ty = gmin_y - gmax_y;
             tx *= w;
             ty *= w;
@@ -919,25 +885,48 @@ void launch_zero_bins(hipStream_t s, const RegionDev& r) {
     hipLaunchKernelGGL(k_zero_bins, dim3(grid_for(M)), dim3(kBlock), 0, s, r.inst, r.filler, M);
 }
 
-void launch_center_init(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* cur_slp, float2* prev_slp, float2* cur) {
-    int n = r.cell_end - r.cell_begin;
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_center_init, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, cur_slp, prev_slp, cur);
-}
-
-void launch_center_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy) {
-    int n = r.cell_end - r.cell_begin;
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_center_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, xy);
-}
-
-void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
-                           const float2* cur, float2* next, float2* next_slp, float step, float coeff) {
+static void place_scatter(hipStream_t s, int mode, const RegionDev& r, const CellsDev& c, const PlaceArgs& pa) {
     int n = r.cell_end - r.cell_begin;
     if (n <= 0) return;
     int chunks = (n + kChunk - 1) / kChunk;
     int grid = chunks < 4096 ? chunks : 4096;
-    hipLaunchKernelGGL(k_update_scatter, dim3(grid), dim3(kBlock), 0, s, r, c, cur_slp, cur_sum, cur, next, next_slp, step, coeff);
+    if (mode == PLACE_INIT)
+        hipLaunchKernelGGL(k_place_scatter<PLACE_INIT>, dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+    else if (mode == PLACE_CENTER)
+        hipLaunchKernelGGL(k_place_scatter<PLACE_CENTER>, dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+    else
+        hipLaunchKernelGGL(k_place_scatter<PLACE_UPDATE>, dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+}
+
+void launch_center_init(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* cur_slp, float2* prev_slp, float2* cur,
+                        unsigned long long* global_adds) {
+    PlaceArgs pa{};
+    pa.next_slp = cur_slp;
+    pa.next = prev_slp;
+    pa.cur_out = cur;
+    pa.global_adds = global_adds;
+    place_scatter(s, PLACE_INIT, r, c, pa);
+}
+
+void launch_center_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy, unsigned long long* global_adds) {
+    PlaceArgs pa{};
+    pa.xy = xy;
+    pa.global_adds = global_adds;
+    place_scatter(s, PLACE_CENTER, r, c, pa);
+}
+
+void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
+                           const float2* cur, float2* next, float2* next_slp, float step, float coeff, unsigned long long* global_adds) {
+    PlaceArgs pa{};
+    pa.cur_slp = cur_slp;
+    pa.cur_sum = cur_sum;
+    pa.cur = cur;
+    pa.next = next;
+    pa.next_slp = next_slp;
+    pa.s = step;
+    pa.coeff = coeff;
+    pa.global_adds = global_adds;
+    place_scatter(s, PLACE_UPDATE, r, c, pa);
 }
 
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials) {
@@ -1043,14 +1032,20 @@ void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi) {
     }
 }
 
+static int wa_grid(const NetsDev& n) {
+    int nbig = n.num_nets - n.num_small;
+    int small_blocks = (n.num_small + kBlock - 1) / kBlock;
+    if (small_blocks > 4096) small_blocks = 4096;
+    if (small_blocks < 1) small_blocks = 1;
+    return nbig + small_blocks;
+}
+
 void launch_wa_forward(hipStream_t s, const NetsDev& n, const CellsDev& c, float coef_x, float coef_y, unsigned long long* hpwl) {
-    if (n.num_small > 0) hipLaunchKernelGGL(k_wa_small, dim3(grid_for(n.num_small)), dim3(kBlock), 0, s, n, c, coef_x, coef_y, hpwl);
-    int big = n.num_nets - n.num_small;
-    if (big > 0) hipLaunchKernelGGL(k_wa_big, dim3(big), dim3(kBlock), 0, s, n, c, coef_x, coef_y, hpwl);
+    hipLaunchKernelGGL(k_wa<true>, dim3(wa_grid(n)), dim3(kBlock), 0, s, n, c, coef_x, coef_y, hpwl);
 }
 
 void launch_hpwl(hipStream_t s, const NetsDev& n, const CellsDev& c, unsigned long long* hpwl) {
-    hipLaunchKernelGGL(k_hpwl, dim3(grid_for(n.num_nets)), dim3(kBlock), 0, s, n, c, hpwl);
+    hipLaunchKernelGGL(k_wa<false>, dim3(wa_grid(n)), dim3(kBlock), 0, s, n, c, 0.0f, 0.0f, hpwl);
 }
 
 void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, float coef_x, float coef_y,
@@ -1081,6 +1076,110 @@ void launch_lock_fix(hipStream_t s, const RegionDev& r, const CellsDev& c, float
     if (n <= 0) return;
     hipLaunchKernelGGL(k_lock_fix, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, nv[0], nv[1], nv[2], nv[3], nv[4], cv[0], cv[1], cv[2],
                        cv[3], cv[4]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Spatial re-sort of the cells: keys from the current bins, a stable radix sort, gathers.
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t spread_bits(uint32_t v) {  // 11 bits -> every other bit
+    v &= 0x7ffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__global__ void k_resort_keys(int N, const int2* ctr, RegionTable t, uint32_t* keys, int32_t* idx) {
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < N; g += gridDim.x * blockDim.x) {
+        int b = 0;
+        while (b + 1 < t.n && g >= t.begin[b + 1]) ++b;
+        const int2 m = ctr[g];
+        uint32_t bx = 0, by = 0;
+        if (m.x != INT_MIN) {
+            double fx = (double)(m.x - t.lx[b]) / t.size_x[b], fy = (double)(m.y - t.ly[b]) / t.size_y[b];
+            int ix = (int)fx, iy = (int)fy;
+            ix = ix < 0 ? 0 : (ix >= t.cnt_x[b] ? t.cnt_x[b] - 1 : ix);
+            iy = iy < 0 ? 0 : (iy >= t.cnt_y[b] ? t.cnt_y[b] - 1 : iy);
+            bx = (uint32_t)ix;
+            by = (uint32_t)iy;
+        }
+        keys[g] = ((uint32_t)b << 22) | spread_bits(bx) | (spread_bits(by) << 1);
+        idx[g] = g;
+    }
+}
+
+__global__ void k_resort_inv(int N, const int32_t* perm, int32_t* inv) {
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < N; g += gridDim.x * blockDim.x) inv[perm[g]] = g;
+}
+
+__global__ void k_resort_gather(int N, const int32_t* perm, CellsDev in, CellsDev out, int32_t* cnt) {
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < N; g += gridDim.x * blockDim.x) {
+        const int o = perm[g];
+        out.dbox[g] = in.dbox[o];
+        out.ctr[g] = in.ctr[o];
+        out.dscale[g] = in.dscale[o];
+        out.flags[g] = in.flags[o];
+        out.pre[g] = in.pre[o];
+        out.old_of[g] = in.old_of[o];
+        cnt[g] = in.pin_start[o + 1] - in.pin_start[o];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[N] = 0;
+}
+
+__global__ void k_resort_pins(int N, const int32_t* perm, CellsDev in, CellsDev out) {
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < N; g += gridDim.x * blockDim.x) {
+        const int o = perm[g];
+        const int s = in.pin_start[o], e = in.pin_start[o + 1], d = out.pin_start[g];
+        for (int q = 0; q < e - s; ++q) {
+            out.cp_net[d + q] = in.cp_net[s + q];
+            out.cp_off[d + q] = in.cp_off[s + q];
+        }
+    }
+}
+
+__global__ void k_gather_f2(int n, int begin, const int32_t* perm, const float2* in, float2* out) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = in[perm[begin + k] - begin];
+}
+
+__global__ void k_remap_pcell(int P, int32_t* pcell, const int32_t* inv) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) {
+        const int c = pcell[k];
+        if (c >= 0) pcell[k] = inv[c];
+    }
+}
+
+size_t resort_temp_bytes(int N) {
+    size_t a = 0, b = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, a, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, N,
+                                             0, 27);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (int32_t*)nullptr, (int32_t*)nullptr, N + 1);
+    return a > b ? a : b;
+}
+
+void launch_resort_sort(hipStream_t s, int N, const int2* ctr, const RegionTable& t, ResortBufs& b) {
+    hipLaunchKernelGGL(k_resort_keys, dim3(grid_for(N)), dim3(kBlock), 0, s, N, ctr, t, b.keys_in, b.idx_in);
+    size_t bytes = b.temp_bytes;
+    (void)hipcub::DeviceRadixSort::SortPairs(b.temp, bytes, b.keys_in, b.keys_out, b.idx_in, b.perm, N, 0, 27, s);
+    hipLaunchKernelGGL(k_resort_inv, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, b.inv);
+}
+
+void launch_resort_cells(hipStream_t s, int N, const CellsDev& in, const CellsDev& out, ResortBufs& b) {
+    hipLaunchKernelGGL(k_resort_gather, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, in, out, b.cnt);
+    size_t bytes = b.temp_bytes;
+    (void)hipcub::DeviceScan::ExclusiveSum(b.temp, bytes, b.cnt, out.pin_start, N + 1, s);
+    hipLaunchKernelGGL(k_resort_pins, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, in, out);
+}
+
+void launch_gather_f2(hipStream_t s, int n, int begin, const int32_t* perm, const float2* in, float2* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_f2, dim3(grid_for(n)), dim3(kBlock), 0, s, n, begin, perm, in, out);
+}
+
+void launch_remap_pcell(hipStream_t s, int P, int32_t* pcell, const int32_t* inv) {
+    if (P <= 0) return;
+    hipLaunchKernelGGL(k_remap_pcell, dim3(grid_for(P)), dim3(kBlock), 0, s, P, pcell, inv);
 }
 
 }  // namespace gplk

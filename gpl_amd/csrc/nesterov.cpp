@@ -72,12 +72,25 @@ struct gpl_ctx {
     gplk::Sums* d_sums = nullptr;  // one per region
     gplk::Sums* h_sums = nullptr;  // pinned host copy
     unsigned long long* d_hpwl = nullptr;
+    unsigned long long* d_global_adds = nullptr;  // the scatter's atomic adds into the HBM bins
     uint8_t* d_flags = nullptr;
     std::vector<uint8_t> h_flags;
     std::vector<int32_t> net_sorted_of;  // original net -> sorted index
     std::vector<int32_t> net_orig_of;    // sorted index -> original net
     std::vector<int32_t> cell_inst, inst_box;
     std::vector<int32_t> cell_old_of, cell_new_of;  // device cell order <-> problem order
+    std::vector<int32_t> cell_inst_prob;            // placer instance of each problem cell
+    bool host_map_stale = false;
+    // spatial re-sort of the device cells
+    gplk::CellsDev cells_alt{};
+    gplk::ResortBufs rb{};
+    gplk::RegionTable rtab{};
+    std::vector<float2*> region_scratch;
+    int32_t net_pin_count = 0;
+    bool resort_enabled = true;
+    uint64_t last_global_adds = 0;
+    double adds_base = 0.0;  // global adds per trial in the first iteration after the last sort
+    int32_t last_resort_iter = -1000, resorts = 0;
     int32_t site_x = 0, dbu = 1000;
     gpl_opts o{};
     // NesterovPlace
@@ -265,7 +278,7 @@ struct gpl_ctx {
     void init_density1(int b) {
         Region& r = rg[b];
         mark(F_SCATTER);
-        gplk::launch_center_init(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_PREV_SLP], r.v[S_CUR]);
+        gplk::launch_center_init(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_PREV_SLP], r.v[S_CUR], d_global_adds);
         mark_end();
         bins_and_finish(r, b);
         r.prev_hpwl = hpwl_now();
@@ -332,7 +345,7 @@ struct gpl_ctx {
             // only region b's gradient is wanted here; later regions are recomputed below
             mark(F_SCATTER);
             gplk::launch_initial_prev(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], initial_prev_coordi_update_coef, r.v[S_PREV_SLP]);
-            gplk::launch_center_scatter(stream, r.dev, cells, r.v[S_PREV_SLP]);
+            gplk::launch_center_scatter(stream, r.dev, cells, r.v[S_PREV_SLP], d_global_adds);
             mark_end();
             bins_and_finish(r, int(b));
             solve(r);
@@ -364,7 +377,7 @@ struct gpl_ctx {
             if (r.is_converged) continue;
             mark(F_SCATTER);
             gplk::launch_update_scatter(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
-                                        r.step_length, coeff);
+                                        r.step_length, coeff, d_global_adds);
             mark_end();
             bins_and_finish(r, int(b));
             solve(r);
@@ -521,8 +534,70 @@ struct gpl_ctx {
         }
     }
 
+    // The problem's index of each device cell, refreshed after re-sorts.
+    void ensure_host_map() {
+        if (!host_map_stale) return;
+        check(hipMemcpyAsync(cell_old_of.data(), cells.old_of, sizeof(int32_t) * ncells, hipMemcpyDeviceToHost, stream), "copy");
+        check(hipStreamSynchronize(stream), "sync");
+        for (int32_t i = 0; i < ncells; ++i) {
+            cell_new_of[cell_old_of[i]] = i;
+            cell_inst[i] = cell_inst_prob[cell_old_of[i]];
+        }
+        host_map_stale = false;
+    }
+
+    // Re-sort every region's cells by the Morton code of their current bin: the cell arrays, the
+    // pin CSR and every per-cell vector are permuted and the net pins' cells renamed.  Only the
+    // device's storage order changes; the integer scatter is exact in any order and the FP64 sums
+    // stay deterministic.
+    void resort() {
+        const int32_t N = ncells;
+        mark(F_REDUCE);
+        gplk::launch_resort_sort(stream, N, cells.ctr, rtab, rb);
+        gplk::launch_resort_cells(stream, N, cells, cells_alt, rb);
+        std::swap(cells, cells_alt);
+        d_flags = cells.flags;
+        // the vectors the loop reads again: the coordinates, the preconditioned gradients and the
+        // snapshots (the wl / den vectors and the next-point slots are rewritten before any read)
+        static const int live[] = {S_CUR_SLP, S_CUR_SLP_SUM, S_PREV_SLP, S_PREV_SLP_SUM, S_CUR, S_SNAP, S_SNAP_SLP, S_SNAP_SLP_SUM,
+                                   S_SNAP_PREV_SLP_SUM};
+        for (size_t b = 0; b < rg.size(); ++b) {
+            Region& r = rg[b];
+            for (int q : live) {
+                gplk::launch_gather_f2(stream, r.n, r.R.cell_begin, rb.perm, r.v[q], region_scratch[b]);
+                std::swap(r.v[q], region_scratch[b]);
+            }
+        }
+        gplk::launch_remap_pcell(stream, net_pin_count, nets.pcell, rb.inv);
+        mark_end();
+        host_map_stale = true;
+        ++resorts;
+    }
+
+    // The scatter's cost follows its count of global atomic adds: one per distinct bin a chunk of
+    // cells touches.  A re-sort when that count per trial has grown a quarter above its level in the
+    // first iteration after the last sort (at most every 8 iterations).
+    void maybe_resort(int32_t iter) {
+        if (!resort_enabled) return;
+        if (inject_kind == 3 && inject_after >= 0 && iter >= inject_after) {
+            resort();
+            last_resort_iter = iter;
+            return;
+        }
+        uint64_t ga = 0;
+        std::memcpy(&ga, &h_sums[rg.size()].pad[0], sizeof ga);
+        const double rate = double(ga - last_global_adds) / double(std::max(trials, 1));
+        last_global_adds = ga;
+        if (iter == last_resort_iter + 1 || adds_base == 0.0) adds_base = rate;
+        if (rate > 1.25 * adds_base && iter - last_resort_iter >= 8) {
+            resort();
+            last_resort_iter = iter;
+        }
+    }
+
     // Per-cell float pairs of region b between the problem's order (host) and the device's.
     void upload_region_vec(int b, float2* dev, const float* host) {
+        ensure_host_map();
         Region& r = rg[b];
         std::vector<float2> tmp(r.n);
         for (int k = 0; k < r.n; ++k) {
@@ -533,6 +608,7 @@ struct gpl_ctx {
         check(hipStreamSynchronize(stream), "sync");
     }
     void download_region_vec(int b, const float2* dev, float* host) {
+        ensure_host_map();
         Region& r = rg[b];
         std::vector<float2> tmp(r.n);
         check(hipMemcpyAsync(tmp.data(), dev, sizeof(float2) * r.n, hipMemcpyDeviceToHost, stream), "copy");
@@ -568,7 +644,7 @@ struct gpl_ctx {
         r.density_penalty = r.snapshot_density_penalty;
         r.step_length = r.snapshot_step_length;
         mark(F_SCATTER);
-        gplk::launch_center_scatter(stream, r.dev, cells, r.v[S_CUR]);
+        gplk::launch_center_scatter(stream, r.dev, cells, r.v[S_CUR], d_global_adds);
         mark_end();
         bins_and_finish(r, b);
         solve(r);
@@ -763,6 +839,7 @@ struct gpl_ctx {
                 ls.done = true;
                 break;
             }
+            maybe_resort(ls.iter);
             ++ls.iter;
         }
     }
@@ -941,10 +1018,14 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         c->nets.pcell = c->upload(pcell.data(), pcell.size());
         c->nets.poff = c->upload(poff.data(), poff.size());
         c->nets.pxy0 = c->upload(pxy0.data(), pxy0.size());
-        c->nets.weight = c->upload(nweight.data(), nweight.size());
-        c->nets.box = c->dalloc<int4>(E);
-        c->nets.sx = c->dalloc<float4>(E);
-        c->nets.sy = c->dalloc<float4>(E);
+        {
+            std::vector<gplk::NetRec> rec(E);
+            for (int32_t i = 0; i < E; ++i) {
+                std::memset(&rec[i], 0, sizeof rec[i]);
+                rec[i].w = nweight[i];
+            }
+            c->nets.rec = c->upload(rec.data(), rec.size());
+        }
         // cells, in the new order
         std::vector<int4> dbox(N);
         std::vector<float2> pre(N);
@@ -975,14 +1056,41 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         c->cells.dscale = c->upload(dscale.data(), N);
         c->d_flags = c->upload(flags.data(), N);
         c->cells.flags = c->d_flags;
-        c->cells.centered = c->dalloc<uint8_t>(N);
-        check(hipMemsetAsync(c->cells.centered, 0, N, c->stream), "memset");
+        {
+            std::vector<int2> ctr(N, make_int2(INT_MIN, INT_MIN));
+            c->cells.ctr = c->upload(ctr.data(), N);
+        }
+
         c->cells.pin_start = c->upload(pin_start.data(), size_t(N) + 1);
         c->cells.cp_net = c->upload(cp_net.data(), cp_net.size());
         c->cells.cp_off = c->upload(cp_off.data(), cp_off.size());
         c->cells.pre = c->upload(pre.data(), N);
         c->cell_inst = cell_inst;
+        c->cell_inst_prob.assign(p->cell_inst, p->cell_inst + N);
         c->inst_box.assign(p->inst_box, p->inst_box + 4 * size_t(p->num_insts));
+        c->cells.old_of = c->upload(c->cell_old_of.data(), N);
+        // the re-sort's second copy of every cell array and its work buffers
+        {
+            gplk::CellsDev& a = c->cells_alt;
+            a.dbox = c->dalloc<int4>(N);
+            a.ctr = c->dalloc<int2>(N);
+            a.dscale = c->dalloc<float>(N);
+            a.flags = c->dalloc<uint8_t>(N);
+            a.pin_start = c->dalloc<int32_t>(size_t(N) + 1);
+            a.cp_net = c->dalloc<int32_t>(cp_net.size());
+            a.cp_off = c->dalloc<int2>(cp_off.size());
+            a.pre = c->dalloc<float2>(N);
+            a.old_of = c->dalloc<int32_t>(N);
+            c->rb.keys_in = c->dalloc<uint32_t>(N);
+            c->rb.keys_out = c->dalloc<uint32_t>(N);
+            c->rb.idx_in = c->dalloc<int32_t>(N);
+            c->rb.perm = c->dalloc<int32_t>(N);
+            c->rb.inv = c->dalloc<int32_t>(N);
+            c->rb.cnt = c->dalloc<int32_t>(size_t(N) + 1);
+            c->rb.temp_bytes = gplk::resort_temp_bytes(N);
+            c->rb.temp = c->dalloc<uint8_t>(c->rb.temp_bytes);
+            c->net_pin_count = int32_t(pcell.size());
+        }
         // regions
         c->rg.resize(p->num_regions);
         for (int32_t b = 0; b < p->num_regions; ++b) {
@@ -1044,6 +1152,22 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         check(hipMemsetAsync(c->d_sums, 0, sizeof(gplk::Sums) * (size_t(p->num_regions) + 1), c->stream), "memset");
         check(hipHostMalloc((void**)&c->h_sums, sizeof(gplk::Sums) * (size_t(p->num_regions) + 1), hipHostMallocDefault), "pinned");
         c->d_hpwl = &c->d_sums[p->num_regions].hpwl;
+        c->d_global_adds = (unsigned long long*)&c->d_sums[p->num_regions].pad[0];
+        // the region table of the re-sort; per-region scratch vectors for its gathers
+        c->resort_enabled = p->num_regions <= gplk::kMaxRegions;
+        c->rtab.n = std::min(p->num_regions, gplk::kMaxRegions);
+        for (int32_t b = 0; b < c->rtab.n; ++b) {
+            const gpl_region& R = p->regions[b];
+            c->rtab.begin[b] = R.cell_begin;
+            c->rtab.begin[b + 1] = R.cell_end;
+            c->rtab.lx[b] = R.lx;
+            c->rtab.ly[b] = R.ly;
+            c->rtab.cnt_x[b] = R.cnt_x;
+            c->rtab.cnt_y[b] = R.cnt_y;
+            c->rtab.size_x[b] = R.size_x;
+            c->rtab.size_y[b] = R.size_y;
+        }
+        for (int32_t b = 0; b < p->num_regions; ++b) c->region_scratch.push_back(c->dalloc<float2>(std::max(1, c->rg[b].n)));
         check(hipStreamSynchronize(c->stream), "sync");
     } catch (const HipError& e) {
         fprintf(stderr, "gpl_ctx_create: %s: %s\n", e.what, hipGetErrorString(e.e));
@@ -1180,6 +1304,7 @@ int32_t gpl_get_log(gpl_ctx* c, char* buf, int32_t cap) {
 int32_t gpl_placed_locations(gpl_ctx* c, int32_t* out, int32_t cap) {
     if (!c) return GPL_ERR_ARG;
     try {
+        c->ensure_host_map();
         std::vector<int4> dbox(c->ncells);
         check(hipMemcpyAsync(dbox.data(), c->cells.dbox, sizeof(int4) * c->ncells, hipMemcpyDeviceToHost, c->stream), "copy");
         check(hipStreamSynchronize(c->stream), "sync");
@@ -1251,7 +1376,7 @@ int32_t gpl_density_scatter(gpl_ctx* c, int32_t region, int32_t slot, int64_t* i
     try {
         Region& r = c->rg[region];
         size_t M = size_t(r.R.cnt_x) * r.R.cnt_y;
-        gplk::launch_center_scatter(c->stream, r.dev, c->cells, xy);
+        gplk::launch_center_scatter(c->stream, r.dev, c->cells, xy, c->d_global_adds);
         if (inst_area) check(hipMemcpyAsync(inst_area, r.dev.inst, 8 * M, hipMemcpyDeviceToHost, c->stream), "copy");
         if (filler_area) check(hipMemcpyAsync(filler_area, r.dev.filler, 8 * M, hipMemcpyDeviceToHost, c->stream), "copy");
         c->bins_and_finish(r, region);
@@ -1306,34 +1431,31 @@ int32_t gpl_wa_forward(gpl_ctx* c, float wl_coef_x, float wl_coef_y, float* net_
         c->wl_coef_y = wl_coef_y;
         c->wa_forward();
         const int32_t E = c->nnets;
-        std::vector<float4> sx(E), sy(E);
-        std::vector<int4> box(E);
+        std::vector<gplk::NetRec> rec(E);
         unsigned long long h = 0;
-        check(hipMemcpyAsync(sx.data(), c->nets.sx, sizeof(float4) * E, hipMemcpyDeviceToHost, c->stream), "copy");
-        check(hipMemcpyAsync(sy.data(), c->nets.sy, sizeof(float4) * E, hipMemcpyDeviceToHost, c->stream), "copy");
-        check(hipMemcpyAsync(box.data(), c->nets.box, sizeof(int4) * E, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipMemcpyAsync(rec.data(), c->nets.rec, sizeof(gplk::NetRec) * E, hipMemcpyDeviceToHost, c->stream), "copy");
         check(hipMemcpyAsync(&h, c->d_hpwl, sizeof h, hipMemcpyDeviceToHost, c->stream), "copy");
         check(hipStreamSynchronize(c->stream), "sync");
         c->collect();
         for (int32_t e = 0; e < E; ++e) {
-            int32_t i = c->net_sorted_of[e];
+            const gplk::NetRec& R = rec[c->net_sorted_of[e]];
             if (net_sums) {
-                float* s = &net_sums[8 * size_t(e)];
-                s[0] = sx[i].x;
-                s[1] = sx[i].y;
-                s[2] = sx[i].z;
-                s[3] = sx[i].w;
-                s[4] = sy[i].x;
-                s[5] = sy[i].y;
-                s[6] = sy[i].z;
-                s[7] = sy[i].w;
+                float* q = &net_sums[8 * size_t(e)];
+                q[0] = R.sx.x;
+                q[1] = R.sx.y;
+                q[2] = R.sx.z;
+                q[3] = R.sx.w;
+                q[4] = R.sy.x;
+                q[5] = R.sy.y;
+                q[6] = R.sy.z;
+                q[7] = R.sy.w;
             }
             if (net_box) {
                 int32_t* b = &net_box[4 * size_t(e)];
-                b[0] = box[i].x;
-                b[1] = box[i].y;
-                b[2] = box[i].z;
-                b[3] = box[i].w;
+                b[0] = R.box.x;
+                b[1] = R.box.y;
+                b[2] = R.box.z;
+                b[3] = R.box.w;
             }
         }
         if (hpwl) *hpwl = (int64_t)h;
@@ -1385,7 +1507,7 @@ int32_t gpl_nesterov_step(gpl_ctx* c, int32_t region, float step, float coeff, f
     try {
         Region& r = c->rg[region];
         gplk::launch_update_scatter(c->stream, r.dev, c->cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
-                                    step, coeff);
+                                    step, coeff, c->d_global_adds);
         c->bins_and_finish(r, region);
         c->sync_stats();
         if (next) c->download_region_vec(region, r.v[S_NEXT], next);
@@ -1397,7 +1519,7 @@ int32_t gpl_nesterov_step(gpl_ctx* c, int32_t region, float step, float coeff, f
 }
 
 int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter) {
-    if (!c || kind < 0 || kind > 2) return GPL_ERR_ARG;
+    if (!c || kind < 0 || kind > 3) return GPL_ERR_ARG;
     c->inject_kind = kind;
     c->inject_after = kind == 0 ? -1 : after_iter;
     return GPL_OK;
@@ -1429,6 +1551,18 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap) {
         std::snprintf(out[n].name, sizeof out[n].name, "%s", kFamName[f]);
         out[n].launches = c->fam_n[f];
         out[n].ms = c->fam_ms[f];
+    }
+    if (n < cap) {
+        // the scatter's atomic adds into the HBM bins since the context began (the count is in
+        // `ms`), and the re-sorts (in `launches`)
+        unsigned long long ga = 0;
+        if (hipMemcpy(&ga, c->d_global_adds, sizeof ga, hipMemcpyDeviceToHost) == hipSuccess) {
+            std::memset(&out[n], 0, sizeof out[n]);
+            std::snprintf(out[n].name, sizeof out[n].name, "scatter_global_adds");
+            out[n].launches = c->resorts;
+            out[n].ms = double(ga);
+            ++n;
+        }
     }
     return n;
 }

@@ -239,3 +239,28 @@ def test_divergence_is_reported(kind, code):
     assert f"[ERROR GPL-0{code}]" in g.log()
     g.close()
     d.close()
+
+
+def test_cell_resorts_do_not_change_the_placement(c1):
+    """Re-sorting the device cells every iteration (the storage order the scatter tiles rely on)
+    leaves the trajectory as it was: the same iterations, overflow and HPWL to FP64-sum rounding,
+    and the placed locations in the problem's order."""
+    runs = []
+    for kind in (0, 3):
+        d = G.Design.from_config("c1")
+        d.initial_place()
+        d.init_nesterov_place()
+        g = G.NesterovGPU(d)
+        g.init_nesterov_loop()
+        if kind:
+            g.debug_inject(3, 0)
+        g.run_nesterov_loop()
+        runs.append((g.stats(), g.history(), g.placed_locations(), g.get_coordinates(0, G.GPL_SLOT_CUR)))
+        g.close()
+        d.close()
+    (s0, h0, p0, c0), (s1, h1, p1, c1_) = runs
+    assert s0.iter == s1.iter and s1.converged
+    assert abs(s0.hpwl - s1.hpwl) / s0.hpwl < 1e-4
+    assert np.array_equal(p0[:, 0], p1[:, 0])           # same instances, same order
+    assert np.max(np.abs(p0[:, 1:] - p1[:, 1:])) <= 64  # positions within rounding drift
+    assert np.max(np.abs(c0 - c1_)) < 64.0
