@@ -61,6 +61,56 @@ def algorithmic_bytes(fam, N, P, E, M):
     }.get(fam)
 
 
+def copy_probe(mib):
+    """A plain copy of `mib` MiB (read mib, write mib) on cuda:0, three times: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE per dispatch against the known size give the counters' scale."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": 4 * n, "bytes_written": 4 * n}), flush=True)
+
+
+def summarize(paths):
+    """Text summary of rocprofv3 CSV files: per-kernel call count and durations from a
+    *_kernel_stats.csv, per-kernel average counter values per dispatch from a *_counter_collection.csv."""
+    import csv
+    from collections import defaultdict
+
+    def short(name):
+        name = name.split("(")[0]
+        return name.replace("void ", "").strip()
+
+    for path in paths:
+        with open(path, newline="") as f:
+            rows = list(csv.DictReader(f))
+        if not rows:
+            continue
+        if "AverageNs" in rows[0]:
+            print(f"# {os.path.basename(path)}")
+            print(f"{'kernel':60s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'pct':>6s}")
+            for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+                print(f"{short(r['Name'])[:60]:60s} {int(r['Calls']):7d} {float(r['TotalDurationNs']) / 1e6:10.3f} "
+                      f"{float(r['AverageNs']) / 1e3:9.2f} {float(r['MinNs']) / 1e3:9.2f} {float(r['MaxNs']) / 1e3:9.2f} "
+                      f"{float(r['Percentage']):6.2f}")
+        else:
+            acc = defaultdict(lambda: [0, 0.0])
+            disp = defaultdict(set)
+            for r in rows:
+                key = (short(r["Kernel_Name"]), r["Counter_Name"])
+                acc[key][1] += float(r["Counter_Value"])
+                disp[key].add(r.get("Dispatch_Id", r.get("Correlation_Id", "")))
+            print(f"# {os.path.basename(path)}")
+            print(f"{'kernel':60s} {'counter':>12s} {'dispatches':>10s} {'avg_per_dispatch':>18s}")
+            for (k, cn), (_, tot) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
+                nd = max(len(disp[(k, cn)]), 1)
+                print(f"{k[:60]:60s} {cn:>12s} {nd:10d} {tot / nd:18.1f}")
+        print()
+
+
 def cpu_info():
     model = platform.processor() or ""
     try:
@@ -83,7 +133,21 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline sampling")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the HPWL check")
     ap.add_argument("--hpwl-config", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--profile-only", action="store_true",
+                    help="np_init + W + K iterations only (the run rocprofv3 wraps); no convergence, no CPU leg")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run a plain device copy of this many MiB three times and exit (the HBM counter "
+                         "calibration under rocprofv3 --pmc)")
+    ap.add_argument("--summarize", nargs="+", metavar="CSV",
+                    help="print a text summary of rocprofv3 CSV output (kernel_stats.csv or counter_collection.csv) and exit")
     args = ap.parse_args()
+
+    if args.summarize:
+        summarize(args.summarize)
+        return
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -118,6 +182,12 @@ def main():
     t1 = time.perf_counter()
     steps = it - it0 if not done else it - it0 + 1
     log(f"[bench] timed {steps} iterations in {t1 - t0:.3f}s")
+    if args.profile_only:
+        g.close()
+        print(json.dumps({"profile_only": True, "config": args.config, "steps": steps,
+                          "iters_per_s": round(steps / (t1 - t0), 2)}), flush=True)
+        d.close()
+        return
     while not done:
         it, done = g.loop_step(1 << 20)
     rc, final_iter = g.loop_end(check=False)

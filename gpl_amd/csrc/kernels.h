@@ -1,13 +1,15 @@
 // Device kernels of the Nesterov engine (kernels.hip) and their launchers, called by the host
 // controller (nesterov.cpp).  Layout in HBM, per region r of the problem:
-//   cells   struct-of-arrays, each region's cells contiguous [cell_begin, cell_end):
-//           dbox int4 (dlx, dly, dux, duy: the density box), density_scale f32, flags u8,
-//           the coordinate slots float2 (cur_slp, next_slp, prev_slp, cur, next, snapshots),
-//           the gradient slots float2 (wl, den, sum per slp slot), pin CSR (start int32,
-//           per cell-pin: net int32, offset int2), preconditioner base (pins, area) float2.
-//   pins    in net order (net CSR): cell int32, offset int2, initial position int2.
-//   nets    sorted by degree (the host keeps the permutation): start int32, weight f32,
-//           box int4, sums 2 x float4 (min_x, x_min_x, max_x, x_max_x | the same in y).
+//   cells   struct-of-arrays, each region's cells contiguous [cell_begin, cell_end), in Morton
+//           order of their bins (re-sorted on the device as cells move): dbox int4 (the density
+//           box), centre int2, density_scale f32, flags u8, the coordinate slots float2 (cur_slp,
+//           next_slp, prev_slp, cur, next, snapshots), the gradient slots float2 (wl, den, sum per
+//           slp slot), pin CSR (start int32; per cell-pin: net-pin index int32, the pin's WA
+//           gradient float2), preconditioner base (pins, area) float2.
+//   pins    in net order (net CSR): cell int32, offset int2, initial position int2, cell-pin
+//           index int32.
+//   nets    sorted by degree, then spatially (the host keeps the permutation): start int32,
+//           weight f32; a 64-byte record (box, the WA sums) for the kernel-level entry.
 //   bins    y-major (k = y*cnt_x + x): inst/filler int64 accumulators, non_place (+unscaled)
 //           int64, area f32*target (scaled), density f32, field_x/field_y f32, scratch for the
 //           transforms.
@@ -59,8 +61,7 @@ struct CellsDev {
     float* dscale;
     uint8_t* flags;             // GPL_CELL_* (locked is updated on convergence)
     int32_t* pin_start;         // num_cells + 1
-    int32_t* cp_net;            // per cell-pin: the net (sorted net index)
-    int2* cp_off;               // per cell-pin: the offset
+    int32_t* cp_npin;           // per cell-pin: its index in the net-order pin arrays
     float2* pre;                // (pin count, dx*dy) per cell, as float
     int32_t* old_of;            // the problem's index of each device cell
 };
@@ -81,8 +82,15 @@ struct NetsDev {
     int32_t* pcell;         // per net-pin: cell or -1 (renamed by a re-sort)
     const int2* poff;       // per net-pin: offset
     const int2* pxy0;       // per net-pin: position as init left it
-    NetRec* rec;
+    const float* weight;    // per net
+    int32_t* cpi;           // per net-pin: its cell-pin index, or -1 (renumbered by a re-sort)
+    float2* pgrad;          // per cell-pin: the pin's weighted WA gradient, written by the WA kernel
+    NetRec* rec;            // box and sums per net, written when asked for (the kernel-level entry)
     int32_t num_small;      // nets [0, num_small) take one thread each; the rest a block each
+    // the small nets in 8 slices by their first cell's eighth of the cell order, [slice[x],
+    // slice[x+1]) each (the kernel gives slice x to XCD x)
+    int32_t slice[9];
+    int32_t slice_blocks;   // workgroups per slice
 };
 
 // Per-trial FP64 reductions of one region, finished into this struct by k_finish.
@@ -112,9 +120,13 @@ void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c,
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials);
 // the Poisson solve of r.density into r.field_x / r.field_y (and r.phi when want_phi).
 void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi);
-// WA forward over every net; HPWL partials.
-void launch_wa_forward(hipStream_t s, const NetsDev& n, const CellsDev& c, float coef_x, float coef_y, unsigned long long* hpwl);
-// per cell of a region: WA gradient, density gradient, preconditioned sum; partial sums of |wl|,
+// WA forward over every net, the HPWL, and each pin's weighted WA gradient into n.pgrad (at its
+// cell-pin index); the net records too when write_rec.  The high-fanout nets go to s_big (which
+// may be s; the caller orders s_big after the zeroed HPWL counter and joins it).
+void launch_wa_forward(hipStream_t s, hipStream_t s_big, const NetsDev& n, const CellsDev& c, float coef_x, float coef_y,
+                       unsigned long long* hpwl, bool write_rec);
+// per cell of a region: WA gradient (its pins' n.pgrad terms summed in pin order), density
+// gradient, preconditioned sum; partial sums of |wl|,
 // |den| and (when cmp_slp != nullptr) the squared distances to cmp_slp/cmp_sum.
 void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, float coef_x, float coef_y,
                      float penalty, float2* wl, float2* den, float2* sum, const float2* a_slp, const float2* b_slp,
@@ -154,8 +166,9 @@ struct ResortBufs {
 size_t resort_temp_bytes(int n_cells);
 // keys (region, Morton code of the current bin) and the stable sort: perm and inv
 void launch_resort_sort(hipStream_t s, int n_cells, const int2* ctr, const RegionTable& t, ResortBufs& b);
-// every per-cell array gathered into `out` (pin CSR rebuilt), old_of composed
-void launch_resort_cells(hipStream_t s, int n_cells, const CellsDev& in, const CellsDev& out, ResortBufs& b);
+// every per-cell array gathered into `out` (pin CSR rebuilt, the net pins' cell-pin indices
+// `cpi` renumbered), old_of composed
+void launch_resort_cells(hipStream_t s, int n_cells, const CellsDev& in, const CellsDev& out, ResortBufs& b, int32_t* cpi);
 // one region's per-cell float pairs gathered: out[k] = in[perm[begin + k] - begin]
 void launch_gather_f2(hipStream_t s, int n, int begin, const int32_t* perm, const float2* in, float2* out);
 // net pins' cells renamed: pcell[k] = inv[pcell[k]]

@@ -17,6 +17,17 @@ namespace gplk {
 constexpr int kBlock = 256;
 constexpr int kMaxBlocks = kPartials;
 
+// The dispatcher hands workgroups to the 8 XCDs in turn (workgroup b to XCD b % 8), and each XCD
+// has its own L2.  The scatter takes its chunks of Morton-ordered cells by the logical index below,
+// which gives XCD x the contiguous range [x G/8, (x+1) G/8) of a G-workgroup grid (G a multiple of
+// 8), so that the bins of neighbouring chunks meet in one L2.  The WA kernel gives XCD x the nets
+// whose first cell lies in the x-th eighth of the cell order (the host sorts the nets by that
+// slice first, then by degree), so that the cell centres it gathers and the pin gradients it
+// scatters stay within one L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int G) { return (b % kXcds) * (G / kXcds) + b / kXcds; }
+__host__ __device__ constexpr int round_xcds(int g) { return (g + kXcds - 1) / kXcds * kXcds; }
+
 // ---------------------------------------------------------------------------------------------
 // Shared device helpers
 // ---------------------------------------------------------------------------------------------
@@ -156,7 +167,12 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
     const int n = r.cell_end - r.cell_begin;
     const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
     if (threadIdx.x == 0) s_adds = 0;
-    for (int base = blockIdx.x * kChunk; base < n; base += gridDim.x * kChunk) {
+    const int chunks = (n + kChunk - 1) / kChunk;
+    const int per = (chunks + gridDim.x - 1) / gridDim.x;
+    const int L = xcd_block(blockIdx.x, gridDim.x);
+    const int ch_end = min((L + 1) * per, chunks);
+    for (int ch = L * per; ch < ch_end; ++ch) {
+        const int base = ch * kChunk;
         for (int i = threadIdx.x; i < kSlots; i += kBlock) {
             t_key[i] = kEmpty;
             t_inst[i] = 0ull;
@@ -561,95 +577,166 @@ __device__ __forceinline__ void wa_pin(int2 p, int lx, int ly, int ux, int uy, f
     }
 }
 
-constexpr int kRegPins = 16;  // pins a thread keeps in registers between the box and the sums
+// getWireLengthGradientPinWA (nesterov.rs:2392): one exponential's term; the numerator and the
+// quotient in double, the result a float.
+__device__ __forceinline__ float wa_term_min(float S0, float S1, float v, float coef, float pc) {
+    double num = (double)S0 * ((double)v * (1.0 - (double)(coef * pc))) + (double)(coef * v * S1);
+    return (float)(num / (double)(S0 * S0));
+}
+__device__ __forceinline__ float wa_term_max(float S0, float S1, float v, float coef, float pc) {
+    double num = (double)S0 * ((double)v * (1.0 + (double)(coef * pc))) - (double)(coef * v * S1);
+    return (float)(num / (double)(S0 * S0));
+}
 
-// updateWireLengthForceWA (nesterov.rs:2325-2388) and getHpwl (2297) in one launch: the first
-// (num_nets - num_small) blocks take one high-fanout net each (box and sums reduced across the
-// block, FP64 partial sums rounded once to float); the rest take one small net per thread, its
-// pins in the net's order (the reference's float sums, bit for bit), the first kRegPins pin
-// positions held in registers between the box pass and the sums pass.  SUMS false: the HPWL only.
+// One pin's weighted WA gradient (getWireLengthGradientPinWA, nesterov.rs:2392-2418, times the net
+// weight) stored at its cell-pin index.
+__device__ __forceinline__ void pin_grad(const NetsDev& n, int k, int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float4 SX,
+                                         float4 SY, float w) {
+    const int q = n.cpi[k];
+    if (q < 0) return;
+    const float fpx = (float)p.x, fpy = (float)p.y;
+    float gmin_x = 0.0f, gmax_x = 0.0f, gmin_y = 0.0f, gmax_y = 0.0f;
+    const float e0 = (float)(lx - p.x) * cx, e1 = (float)(p.x - ux) * cx;
+    const float e2 = (float)(ly - p.y) * cy, e3 = (float)(p.y - uy) * cy;
+    if (e0 > -300.0f) gmin_x = wa_term_min(SX.x, SX.y, fast_exp(e0), cx, fpx);
+    if (e1 > -300.0f) gmax_x = wa_term_max(SX.z, SX.w, fast_exp(e1), cx, fpx);
+    if (e2 > -300.0f) gmin_y = wa_term_min(SY.x, SY.y, fast_exp(e2), cy, fpy);
+    if (e3 > -300.0f) gmax_y = wa_term_max(SY.z, SY.w, fast_exp(e3), cy, fpy);
+    float tx = gmin_x - gmax_x, ty = gmin_y - gmax_y;
+    tx *= w;
+    ty *= w;
+    n.pgrad[q] = make_float2(tx, ty);
+}
+
+// updateWireLengthForceWA (nesterov.rs:2325-2388), getHpwl (2297) and the pins' gradient terms,
+// in two kernels.  SUMS false: the HPWL only.
+//
+// k_wa_big: one 1024-thread block per high-fanout net (num_small and up): the box and the sums
+// reduced across the block, the sums in FP64 and rounded once to float.  Each thread holds up to
+// kBigRegs of its pins in registers, loaded in one unrolled batch so that their gathers overlap.
+constexpr int kBigBlock = 1024;
+constexpr int kBigRegs = 8;
 template <bool SUMS>
-__global__ void __launch_bounds__(kBlock) k_wa(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl) {
-    __shared__ int shi[4][kBlock / 64];
-    __shared__ double shd[8][kBlock / 64];
-    __shared__ unsigned long long shh[kBlock / 64];
-    const int nbig = n.num_nets - n.num_small;
+__global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
+                                                      int write_rec) {
+    __shared__ int shi[4][kBigBlock / 64];
+    __shared__ double shd[8][kBigBlock / 64];
+    __shared__ float shr[8];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if ((int)blockIdx.x < nbig) {
-        const int e = n.num_small + blockIdx.x;
-        const int s = n.start[e], t = n.start[e + 1];
-        int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
-        for (int k = s + threadIdx.x; k < t; k += blockDim.x) {
-            const int2 p = pin_pos(n, c, k);
-            lx = min(lx, p.x);
-            ly = min(ly, p.y);
-            ux = max(ux, p.x);
-            uy = max(uy, p.y);
-        }
+    const int e = n.num_small + blockIdx.x;
+    const int s = n.start[e], t = n.start[e + 1];
+    const int rest = s + kBigRegs * kBigBlock;  // pins from here on are loaded again in each pass
+    int2 pos[kBigRegs];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            lx = min(lx, __shfl_down(lx, o, 64));
-            ly = min(ly, __shfl_down(ly, o, 64));
-            ux = max(ux, __shfl_down(ux, o, 64));
-            uy = max(uy, __shfl_down(uy, o, 64));
+    for (int i = 0; i < kBigRegs; ++i) {
+        const int k = s + threadIdx.x + i * kBigBlock;
+        pos[i] = k < t ? pin_pos(n, c, k) : make_int2(0, 0);
+    }
+    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+#pragma unroll
+    for (int i = 0; i < kBigRegs; ++i) {
+        if (s + (int)threadIdx.x + i * kBigBlock < t) {
+            lx = min(lx, pos[i].x);
+            ly = min(ly, pos[i].y);
+            ux = max(ux, pos[i].x);
+            uy = max(uy, pos[i].y);
         }
-        if (lane == 0) {
-            shi[0][w] = lx;
-            shi[1][w] = ly;
-            shi[2][w] = ux;
-            shi[3][w] = uy;
+    }
+    for (int k = rest + threadIdx.x; k < t; k += blockDim.x) {
+        const int2 p = pin_pos(n, c, k);
+        lx = min(lx, p.x);
+        ly = min(ly, p.y);
+        ux = max(ux, p.x);
+        uy = max(uy, p.y);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lx = min(lx, __shfl_down(lx, o, 64));
+        ly = min(ly, __shfl_down(ly, o, 64));
+        ux = max(ux, __shfl_down(ux, o, 64));
+        uy = max(uy, __shfl_down(uy, o, 64));
+    }
+    if (lane == 0) {
+        shi[0][w] = lx;
+        shi[1][w] = ly;
+        shi[2][w] = ux;
+        shi[3][w] = uy;
+    }
+    __syncthreads();
+    lx = shi[0][0];
+    ly = shi[1][0];
+    ux = shi[2][0];
+    uy = shi[3][0];
+    for (int i = 1; i < kBigBlock / 64; ++i) {
+        lx = min(lx, shi[0][i]);
+        ly = min(ly, shi[1][i]);
+        ux = max(ux, shi[2][i]);
+        uy = max(uy, shi[3][i]);
+    }
+    if (SUMS) {
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        auto add = [&](int2 p) {
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
+            wa_pin(p, lx, ly, ux, uy, cx, cy, a, b);
+            acc[0] += a.x;
+            acc[1] += a.y;
+            acc[2] += a.z;
+            acc[3] += a.w;
+            acc[4] += b.x;
+            acc[5] += b.y;
+            acc[6] += b.z;
+            acc[7] += b.w;
+        };
+#pragma unroll
+        for (int i = 0; i < kBigRegs; ++i)
+            if (s + (int)threadIdx.x + i * kBigBlock < t) add(pos[i]);
+        for (int k = rest + threadIdx.x; k < t; k += blockDim.x) add(pin_pos(n, c, k));
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double v = wave_sum(acc[q]);
+            if (lane == 0) shd[q][w] = v;
         }
         __syncthreads();
-        lx = shi[0][0];
-        ly = shi[1][0];
-        ux = shi[2][0];
-        uy = shi[3][0];
-        for (int i = 1; i < kBlock / 64; ++i) {
-            lx = min(lx, shi[0][i]);
-            ly = min(ly, shi[1][i]);
-            ux = max(ux, shi[2][i]);
-            uy = max(uy, shi[3][i]);
+        if (threadIdx.x < 8) {
+            double v = 0.0;
+            for (int i = 0; i < kBigBlock / 64; ++i) v += shd[threadIdx.x][i];
+            shr[threadIdx.x] = (float)v;
         }
-        if (SUMS) {
-            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (int k = s + threadIdx.x; k < t; k += blockDim.x) {
-                const int2 p = pin_pos(n, c, k);
-                float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
-                wa_pin(p, lx, ly, ux, uy, cx, cy, a, b);
-                acc[0] += a.x;
-                acc[1] += a.y;
-                acc[2] += a.z;
-                acc[3] += a.w;
-                acc[4] += b.x;
-                acc[5] += b.y;
-                acc[6] += b.z;
-                acc[7] += b.w;
-            }
+        __syncthreads();
+        const float4 SX = make_float4(shr[0], shr[1], shr[2], shr[3]);
+        const float4 SY = make_float4(shr[4], shr[5], shr[6], shr[7]);
+        if (threadIdx.x == 0 && write_rec) {
+            NetRec& R = n.rec[e];
+            R.box = make_int4(lx, ly, ux, uy);
+            R.sx = SX;
+            R.sy = SY;
+        }
+        const float wt = n.weight[e];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                double v = wave_sum(acc[q]);
-                if (lane == 0) shd[q][w] = v;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                float r[8];
-                for (int q = 0; q < 8; ++q) {
-                    double v = 0.0;
-                    for (int i = 0; i < kBlock / 64; ++i) v += shd[q][i];
-                    r[q] = (float)v;
-                }
-                NetRec& R = n.rec[e];
-                R.box = make_int4(lx, ly, ux, uy);
-                R.sx = make_float4(r[0], r[1], r[2], r[3]);
-                R.sy = make_float4(r[4], r[5], r[6], r[7]);
-            }
+        for (int i = 0; i < kBigRegs; ++i) {
+            const int k = s + threadIdx.x + i * kBigBlock;
+            if (k < t) pin_grad(n, k, pos[i], lx, ly, ux, uy, cx, cy, SX, SY, wt);
         }
-        if (threadIdx.x == 0 && ux >= lx) atomicAdd(hpwl, (unsigned long long)((long long)ux - lx + (long long)uy - ly));
-        return;
+        for (int k = rest + threadIdx.x; k < t; k += blockDim.x) pin_grad(n, k, pin_pos(n, c, k), lx, ly, ux, uy, cx, cy, SX, SY, wt);
     }
+    if (threadIdx.x == 0 && ux >= lx) atomicAdd(hpwl, (unsigned long long)((long long)ux - lx + (long long)uy - ly));
+}
+
+// k_wa_small: one net (below num_small) per thread, its pins in the net's order (the
+// reference's float sums, bit for bit), the first kRegPins pin positions held in registers
+// across the box, sums and gradient passes (nets of more pins gather the rest again).  XCD x
+// takes the nets of slice x.
+constexpr int kRegPins = 8;
+template <bool SUMS>
+__global__ void __launch_bounds__(kBlock) k_wa_small(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
+                                                     int write_rec) {
+    __shared__ unsigned long long shh[kBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned long long h = 0;
-    const int stride = (gridDim.x - nbig) * blockDim.x;
-    for (int e = (blockIdx.x - nbig) * blockDim.x + threadIdx.x; e < n.num_small; e += stride) {
+    const int x = blockIdx.x % kXcds;
+    const int e_end = n.slice[x + 1];
+    const int stride = n.slice_blocks * blockDim.x;
+    for (int e = n.slice[x] + (blockIdx.x / kXcds) * blockDim.x + threadIdx.x; e < e_end; e += stride) {
         const int s = n.start[e], deg = n.start[e + 1] - s;
         int2 pos[kRegPins];
         int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
@@ -676,10 +763,17 @@ __global__ void __launch_bounds__(kBlock) k_wa(NetsDev n, CellsDev c, float cx, 
             for (int k = 0; k < kRegPins; ++k)
                 if (k < deg) wa_pin(pos[k], lx, ly, ux, uy, cx, cy, a, b);
             for (int k = kRegPins; k < deg; ++k) wa_pin(pin_pos(n, c, s + k), lx, ly, ux, uy, cx, cy, a, b);
-            NetRec& R = n.rec[e];
-            R.box = make_int4(lx, ly, ux, uy);
-            R.sx = a;
-            R.sy = b;
+            if (write_rec) {
+                NetRec& R = n.rec[e];
+                R.box = make_int4(lx, ly, ux, uy);
+                R.sx = a;
+                R.sy = b;
+            }
+            const float wt = n.weight[e];
+#pragma unroll
+            for (int k = 0; k < kRegPins; ++k)
+                if (k < deg) pin_grad(n, s + k, pos[k], lx, ly, ux, uy, cx, cy, a, b, wt);
+            for (int k = kRegPins; k < deg; ++k) pin_grad(n, s + k, pin_pos(n, c, s + k), lx, ly, ux, uy, cx, cy, a, b, wt);
         }
         if (ux >= lx) h += (unsigned long long)((long long)ux - lx + (long long)uy - ly);
     }
@@ -693,19 +787,8 @@ __global__ void __launch_bounds__(kBlock) k_wa(NetsDev n, CellsDev c, float cx, 
     }
 }
 
-// getWireLengthGradientPinWA (nesterov.rs:2392): one exponential's term; the numerator and the
-// quotient in double, the result a float.
-__device__ __forceinline__ float wa_term_min(float S0, float S1, float v, float coef, float pc) {
-    double num = (double)S0 * ((double)v * (1.0 - (double)(coef * pc))) + (double)(coef * v * S1);
-    return (float)(num / (double)(S0 * S0));
-}
-__device__ __forceinline__ float wa_term_max(float S0, float S1, float v, float coef, float pc) {
-    double num = (double)S0 * ((double)v * (1.0 + (double)(coef * pc))) - (double)(coef * v * S1);
-    return (float)(num / (double)(S0 * S0));
-}
-
 // updateGradients (nesterov.rs:2455-2489) per cell: the WA gradient over the cell's pins in
-// their order, each times its net's weight (2418); the density gradient over the bins its box
+// their order (each pin's weighted term from the WA kernel); the density gradient over the bins its box
 // covers, x outer, y inner (2434); the preconditioned sum; FP64 partial sums of |wl|, |den|,
 // and, for a trial, the squared coordinate and gradient distances (nesterovUpdateStepLength,
 // 2755-2756) to a_slp/a_sum.
@@ -718,29 +801,11 @@ __global__ void k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float 
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const int g = r.cell_begin + k;
         const int4 d = c.dbox[g];
-        const int dcx = (d.x + d.z) / 2, dcy = (d.y + d.w) / 2;
         float wx = 0.0f, wy = 0.0f;
         for (int q = c.pin_start[g]; q < c.pin_start[g + 1]; ++q) {
-            const int e = c.cp_net[q];
-            const int2 o = c.cp_off[q];
-            const int px = dcx + o.x, py = dcy + o.y;
-            const NetRec rec = nt.rec[e];
-            const int4 b = rec.box;
-            const float4 SX = rec.sx, SY = rec.sy;
-            const float fpx = (float)px, fpy = (float)py;
-            float gmin_x = 0.0f, gmax_x = 0.0f, gmin_y = 0.0f, gmax_y = 0.0f;
-            float e0 = (float)(b.x - px) * cx, e1 = (float)(px - b.z) * cx;
-            float e2 = (float)(b.y - py) * cy, e3 = (float)(py - b.w) * cy;
-            if (e0 > -300.0f) gmin_x = wa_term_min(SX.x, SX.y, fast_exp(e0), cx, fpx);
-            if (e1 > -300.0f) gmax_x = wa_term_max(SX.z, SX.w, fast_exp(e1), cx, fpx);
-            if (e2 > -300.0f) gmin_y = wa_term_min(SY.x, SY.y, fast_exp(e2), cy, fpy);
-            if (e3 > -300.0f) gmax_y = wa_term_max(SY.z, SY.w, fast_exp(e3), cy, fpy);
-            const float w = rec.w;
-            float tx = gmin_x - gmax_x, ty = gmin_y - gmax_y;
-            tx *= w;
-            ty *= w;
-            wx += tx;
-            wy += ty;
+            const float2 t = nt.pgrad[q];
+            wx += t.x;
+            wy += t.y;
         }
         // density gradient
         int x0, x1, y0, y1;
@@ -874,7 +939,7 @@ static int grid_for(size_t n) {
     size_t b = (n + kBlock - 1) / kBlock;
     if (b < 1) b = 1;
     if (b > (size_t)kMaxBlocks) b = kMaxBlocks;
-    return (int)b;
+    return round_xcds((int)b);
 }
 
 int block_partials_bins(const RegionDev& r) { return grid_for((size_t)r.cnt_x * r.cnt_y); }
@@ -889,7 +954,7 @@ static void place_scatter(hipStream_t s, int mode, const RegionDev& r, const Cel
     int n = r.cell_end - r.cell_begin;
     if (n <= 0) return;
     int chunks = (n + kChunk - 1) / kChunk;
-    int grid = chunks < 4096 ? chunks : 4096;
+    int grid = round_xcds(chunks < 4096 ? chunks : 4096);
     if (mode == PLACE_INIT)
         hipLaunchKernelGGL(k_place_scatter<PLACE_INIT>, dim3(grid), dim3(kBlock), 0, s, r, c, pa);
     else if (mode == PLACE_CENTER)
@@ -1032,20 +1097,22 @@ void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi) {
     }
 }
 
-static int wa_grid(const NetsDev& n) {
-    int nbig = n.num_nets - n.num_small;
-    int small_blocks = (n.num_small + kBlock - 1) / kBlock;
-    if (small_blocks > 4096) small_blocks = 4096;
-    if (small_blocks < 1) small_blocks = 1;
-    return nbig + small_blocks;
+template <bool SUMS>
+static void wa_launch(hipStream_t s, hipStream_t s_big, const NetsDev& n, const CellsDev& c, float coef_x, float coef_y,
+                      unsigned long long* hpwl, int write_rec) {
+    const int nbig = n.num_nets - n.num_small;
+    if (nbig > 0) hipLaunchKernelGGL(k_wa_big<SUMS>, dim3(nbig), dim3(kBigBlock), 0, s_big, n, c, coef_x, coef_y, hpwl, write_rec);
+    if (n.num_small > 0)
+        hipLaunchKernelGGL(k_wa_small<SUMS>, dim3(kXcds * n.slice_blocks), dim3(kBlock), 0, s, n, c, coef_x, coef_y, hpwl, write_rec);
 }
 
-void launch_wa_forward(hipStream_t s, const NetsDev& n, const CellsDev& c, float coef_x, float coef_y, unsigned long long* hpwl) {
-    hipLaunchKernelGGL(k_wa<true>, dim3(wa_grid(n)), dim3(kBlock), 0, s, n, c, coef_x, coef_y, hpwl);
+void launch_wa_forward(hipStream_t s, hipStream_t s_big, const NetsDev& n, const CellsDev& c, float coef_x, float coef_y,
+                       unsigned long long* hpwl, bool write_rec) {
+    wa_launch<true>(s, s_big, n, c, coef_x, coef_y, hpwl, write_rec ? 1 : 0);
 }
 
 void launch_hpwl(hipStream_t s, const NetsDev& n, const CellsDev& c, unsigned long long* hpwl) {
-    hipLaunchKernelGGL(k_wa<false>, dim3(wa_grid(n)), dim3(kBlock), 0, s, n, c, 0.0f, 0.0f, hpwl);
+    wa_launch<false>(s, s, n, c, 0.0f, 0.0f, hpwl, 0);
 }
 
 void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, float coef_x, float coef_y,
@@ -1128,13 +1195,14 @@ __global__ void k_resort_gather(int N, const int32_t* perm, CellsDev in, CellsDe
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[N] = 0;
 }
 
-__global__ void k_resort_pins(int N, const int32_t* perm, CellsDev in, CellsDev out) {
+__global__ void k_resort_pins(int N, const int32_t* perm, CellsDev in, CellsDev out, int32_t* cpi) {
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < N; g += gridDim.x * blockDim.x) {
         const int o = perm[g];
         const int s = in.pin_start[o], e = in.pin_start[o + 1], d = out.pin_start[g];
         for (int q = 0; q < e - s; ++q) {
-            out.cp_net[d + q] = in.cp_net[s + q];
-            out.cp_off[d + q] = in.cp_off[s + q];
+            const int np = in.cp_npin[s + q];
+            out.cp_npin[d + q] = np;
+            cpi[np] = d + q;
         }
     }
 }
@@ -1165,11 +1233,11 @@ void launch_resort_sort(hipStream_t s, int N, const int2* ctr, const RegionTable
     hipLaunchKernelGGL(k_resort_inv, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, b.inv);
 }
 
-void launch_resort_cells(hipStream_t s, int N, const CellsDev& in, const CellsDev& out, ResortBufs& b) {
+void launch_resort_cells(hipStream_t s, int N, const CellsDev& in, const CellsDev& out, ResortBufs& b, int32_t* cpi) {
     hipLaunchKernelGGL(k_resort_gather, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, in, out, b.cnt);
     size_t bytes = b.temp_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(b.temp, bytes, b.cnt, out.pin_start, N + 1, s);
-    hipLaunchKernelGGL(k_resort_pins, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, in, out);
+    hipLaunchKernelGGL(k_resort_pins, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, in, out, cpi);
 }
 
 void launch_gather_f2(hipStream_t s, int n, int begin, const int32_t* perm, const float2* in, float2* out) {

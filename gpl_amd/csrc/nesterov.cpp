@@ -64,6 +64,16 @@ const char* kFamName[F_COUNT] = {"update_scatter", "bin_density", "dct_solve", "
 
 struct gpl_ctx {
     hipStream_t stream = nullptr;
+    // A second stream (default priority) for the bins and the Poisson solve of a trial, which run
+    // beside the WA kernels; the gradient waits on ev_join.  Off while profiling, so that each
+    // kernel family's events bracket that family alone.
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool overlap = true, join_pending = false;
+    // a third stream for the high-fanout nets' WA kernel, beside the small nets' one
+    hipStream_t aux2 = nullptr;
+    hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
+    bool join2_pending = false;
     std::vector<void*> allocs;
     int32_t ncells = 0, npins = 0, nnets = 0;
     gplk::CellsDev cells{};
@@ -147,6 +157,14 @@ struct gpl_ctx {
         if (h_sums) hipHostFree(h_sums);
         if (profiling)
             for (int i = 0; i < 64; ++i) hipEventDestroy(ev[i]);
+        if (aux) hipStreamSynchronize(aux);
+        if (aux2) hipStreamSynchronize(aux2);
+        if (ev_fork) hipEventDestroy(ev_fork);
+        if (ev_join) hipEventDestroy(ev_join);
+        if (ev_fork2) hipEventDestroy(ev_fork2);
+        if (ev_join2) hipEventDestroy(ev_join2);
+        if (aux) hipStreamDestroy(aux);
+        if (aux2) hipStreamDestroy(aux2);
         if (stream) hipStreamDestroy(stream);
     }
 
@@ -174,6 +192,7 @@ struct gpl_ctx {
     }
 
     void sync_stats() {
+        join();
         check(hipMemcpyAsync(h_sums, d_sums, sizeof(gplk::Sums) * (rg.size() + 1), hipMemcpyDeviceToHost, stream), "stats copy");
         check(hipStreamSynchronize(stream), "sync");
         collect();
@@ -184,7 +203,10 @@ struct gpl_ctx {
         return r.R.std_insts_area + (int64_t)std::round(float(r.R.macro_insts_area) * r.R.target_density);
     }
 
-    void zero_hpwl() { check(hipMemsetAsync(d_hpwl, 0, sizeof(unsigned long long), stream), "memset"); }
+    void zero_hpwl() {
+        join2();  // no high-fanout WA kernel may still be adding into the counter
+        check(hipMemsetAsync(d_hpwl, 0, sizeof(unsigned long long), stream), "memset");
+    }
 
     // getHpwl (nesterov.rs:2297), read back.
     int64_t hpwl_now() {
@@ -209,11 +231,29 @@ struct gpl_ctx {
         mark_end();
     }
 
-    void wa_forward() {
+    // updateWireLengthForceWA with the pins' gradient terms (read by the gradient kernel); the
+    // net records only for the kernel-level entry.
+    void wa_forward(bool write_rec = false) {
         mark(F_WA);
         zero_hpwl();
-        gplk::launch_wa_forward(stream, nets, cells, wl_coef_x, wl_coef_y, d_hpwl);
+        const bool ovl = overlap && !profiling && nets.num_small < nnets;
+        if (ovl) {
+            check(hipEventRecord(ev_fork2, stream), "event");
+            check(hipStreamWaitEvent(aux2, ev_fork2, 0), "wait");
+        }
+        gplk::launch_wa_forward(stream, ovl ? aux2 : stream, nets, cells, wl_coef_x, wl_coef_y, d_hpwl, write_rec);
+        if (ovl) {
+            check(hipEventRecord(ev_join2, aux2), "event");
+            join2_pending = true;
+        }
         mark_end();
+    }
+
+    // the main stream waits for the high-fanout nets' WA kernel
+    void join2() {
+        if (!join2_pending) return;
+        check(hipStreamWaitEvent(stream, ev_join2, 0), "wait");
+        join2_pending = false;
     }
 
     // The gradient of region b into `slot`; with `trial`, the distances of the step length too.
@@ -233,6 +273,7 @@ struct gpl_ctx {
     // updateGradients' sums read back; npUpdate*Gradient's zero-sum recursion and NaN check
     // (nesterov.rs:2512-2528).  Regions from `first` are (re)evaluated in order.
     void np_update_gradients(int first, int last, int slot, bool trial) {
+        join();
         for (int b = first; b < last; ++b) {
             if (rg[b].is_converged) continue;
             launch_gradient(b, slot, trial, slot == SLOT_NEXT && inject_now());
@@ -370,7 +411,7 @@ struct gpl_ctx {
         return GPL_OK;
     }
 
-    // nesterovUpdateCoordinates for every region, then the field.
+    // nesterovUpdateCoordinates for every region (the scatter); the field follows in field().
     void update_coordinates(float coeff) {
         for (size_t b = 0; b < rg.size(); ++b) {
             Region& r = rg[b];
@@ -379,9 +420,37 @@ struct gpl_ctx {
             gplk::launch_update_scatter(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
                                         r.step_length, coeff, d_global_adds);
             mark_end();
+        }
+        if (overlap && !profiling) check(hipEventRecord(ev_fork, stream), "event");
+    }
+
+    // The bins and the Poisson solve of every region after update_coordinates: on the second
+    // stream when overlapping (beside the WA kernels), joined before the gradient.
+    void field() {
+        const bool ovl = overlap && !profiling;
+        if (ovl) {
+            check(hipStreamWaitEvent(aux, ev_fork, 0), "wait");
+            std::swap(stream, aux);
+        }
+        for (size_t b = 0; b < rg.size(); ++b) {
+            Region& r = rg[b];
+            if (r.is_converged) continue;
             bins_and_finish(r, int(b));
             solve(r);
         }
+        if (ovl) {
+            std::swap(stream, aux);
+            check(hipEventRecord(ev_join, aux), "event");
+            join_pending = true;
+        }
+    }
+
+    // the main stream waits for the second stream's bins and solve
+    void join() {
+        join2();
+        if (!join_pending) return;
+        check(hipStreamWaitEvent(stream, ev_join, 0), "wait");
+        join_pending = false;
     }
 
     // nesterovUpdateStepLength (nesterov.rs:2750) from the trial's FP64 distance sums.
@@ -415,6 +484,7 @@ struct gpl_ctx {
         for (int nbt = 0; nbt < 10; ++nbt) {
             ++trials;
             update_coordinates(coeff);
+            field();
             wa_forward();
             num_region_diverged = 0;
             np_update_gradients(0, int(rg.size()), SLOT_NEXT, true);
@@ -554,7 +624,7 @@ struct gpl_ctx {
         const int32_t N = ncells;
         mark(F_REDUCE);
         gplk::launch_resort_sort(stream, N, cells.ctr, rtab, rb);
-        gplk::launch_resort_cells(stream, N, cells, cells_alt, rb);
+        gplk::launch_resort_cells(stream, N, cells, cells_alt, rb, nets.cpi);
         std::swap(cells, cells_alt);
         d_flags = cells.flags;
         // the vectors the loop reads again: the coordinates, the preconditioned gradients and the
@@ -942,6 +1012,12 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
     gpl_ctx* c = new gpl_ctx();
     try {
         check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
+        check(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking), "stream");
+        check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
+        check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "event");
+        check(hipStreamCreateWithFlags(&c->aux2, hipStreamNonBlocking), "stream");
+        check(hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming), "event");
+        check(hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming), "event");
         c->o = p->opts;
         c->max_iter = p->opts.max_iter;
         c->dbu = p->dbu;
@@ -989,23 +1065,44 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         };
         std::vector<int32_t> fc(E);
         for (int32_t e = 0; e < E; ++e) fc[e] = first_cell(e);
+        const int32_t kBig = 128;
+        // slice of a small net: the eighth of the cell order its first cell lies in; big nets last
+        auto slice_of = [&](int32_t e) -> int32_t {
+            if (deg(e) > kBig) return 8;
+            if (fc[e] == INT32_MAX || N == 0) return 7;
+            return int32_t((int64_t)fc[e] * 8 / N);
+        };
+        std::vector<int32_t> sl(E);
+        for (int32_t e = 0; e < E; ++e) sl[e] = slice_of(e);
         std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            if (sl[a] != sl[b]) return sl[a] < sl[b];
             return deg(a) != deg(b) ? deg(a) < deg(b) : fc[a] < fc[b];
         });
         c->net_orig_of = order;
         c->net_sorted_of.assign(E, 0);
         for (int32_t i = 0; i < E; ++i) c->net_sorted_of[order[i]] = i;
-        const int32_t kBig = 128;
         int32_t num_small = 0;
         while (num_small < E && deg(order[num_small]) <= kBig) ++num_small;
+        {
+            int32_t at = 0, most = 0;
+            for (int x = 0; x < 8; ++x) {
+                c->nets.slice[x] = at;
+                while (at < num_small && sl[order[at]] == x) ++at;
+                most = std::max(most, at - c->nets.slice[x]);
+            }
+            c->nets.slice[8] = at;
+            c->nets.slice_blocks = std::max(1, std::min((most + 255) / 256, 1024));
+        }
         std::vector<int32_t> nstart(E + 1, 0), pcell, np_net;
         std::vector<int2> poff, pxy0;
         std::vector<float> nweight(E);
+        std::vector<int32_t> npin_of(size_t(p->num_pins), -1);  // problem pin -> net-order pin index
         for (int32_t i = 0; i < E; ++i) {
             int32_t e = order[i];
             nweight[i] = p->net_weight[e];
             for (int32_t k = p->net_pin_start[e]; k < p->net_pin_start[e + 1]; ++k) {
                 int32_t q = p->net_pins[k];
+                npin_of[q] = int32_t(pcell.size());
                 pcell.push_back(newc(p->pin_cell[q]));
                 poff.push_back(make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]));
                 pxy0.push_back(make_int2(p->pin_xy[2 * q], p->pin_xy[2 * q + 1]));
@@ -1018,6 +1115,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         c->nets.pcell = c->upload(pcell.data(), pcell.size());
         c->nets.poff = c->upload(poff.data(), poff.size());
         c->nets.pxy0 = c->upload(pxy0.data(), pxy0.size());
+        c->nets.weight = c->upload(nweight.data(), nweight.size());
         {
             std::vector<gplk::NetRec> rec(E);
             for (int32_t i = 0; i < E; ++i) {
@@ -1032,8 +1130,8 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         std::vector<float> dscale(N);
         std::vector<uint8_t> flags(N);
         std::vector<int32_t> pin_start(size_t(N) + 1, 0), cell_inst(N);
-        std::vector<int32_t> cp_net(p->cell_pin_start[N]);
-        std::vector<int2> cp_off(p->cell_pin_start[N]);
+        std::vector<int32_t> cp_npin(p->cell_pin_start[N]);
+        std::vector<int32_t> cpi(pcell.size(), -1);
         for (int32_t i = 0; i < N; ++i) {
             int32_t g = c->cell_old_of[i];
             const int32_t* d = &p->cell_dbox[4 * g];
@@ -1046,8 +1144,8 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             int32_t at = pin_start[i];
             for (int32_t k = p->cell_pin_start[g]; k < p->cell_pin_start[g + 1]; ++k, ++at) {
                 int32_t q = p->cell_pins[k];
-                cp_net[at] = c->net_sorted_of[p->pin_net[q]];
-                cp_off[at] = make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]);
+                cp_npin[at] = npin_of[q];
+                cpi[npin_of[q]] = at;
             }
             pin_start[i + 1] = at;
         }
@@ -1062,8 +1160,9 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         }
 
         c->cells.pin_start = c->upload(pin_start.data(), size_t(N) + 1);
-        c->cells.cp_net = c->upload(cp_net.data(), cp_net.size());
-        c->cells.cp_off = c->upload(cp_off.data(), cp_off.size());
+        c->cells.cp_npin = c->upload(cp_npin.data(), cp_npin.size());
+        c->nets.cpi = c->upload(cpi.data(), cpi.size());
+        c->nets.pgrad = c->dalloc<float2>(std::max<size_t>(cp_npin.size(), 1));
         c->cells.pre = c->upload(pre.data(), N);
         c->cell_inst = cell_inst;
         c->cell_inst_prob.assign(p->cell_inst, p->cell_inst + N);
@@ -1077,8 +1176,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             a.dscale = c->dalloc<float>(N);
             a.flags = c->dalloc<uint8_t>(N);
             a.pin_start = c->dalloc<int32_t>(size_t(N) + 1);
-            a.cp_net = c->dalloc<int32_t>(cp_net.size());
-            a.cp_off = c->dalloc<int2>(cp_off.size());
+            a.cp_npin = c->dalloc<int32_t>(cp_npin.size());
             a.pre = c->dalloc<float2>(N);
             a.old_of = c->dalloc<int32_t>(N);
             c->rb.keys_in = c->dalloc<uint32_t>(N);
@@ -1429,7 +1527,8 @@ int32_t gpl_wa_forward(gpl_ctx* c, float wl_coef_x, float wl_coef_y, float* net_
     try {
         c->wl_coef_x = wl_coef_x;
         c->wl_coef_y = wl_coef_y;
-        c->wa_forward();
+        c->wa_forward(true);
+        c->join();
         const int32_t E = c->nnets;
         std::vector<gplk::NetRec> rec(E);
         unsigned long long h = 0;
@@ -1528,7 +1627,7 @@ int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter) {
 int32_t gpl_set_profiling(gpl_ctx* c, int32_t on) {
     if (!c) return GPL_ERR_ARG;
     if (on && !c->profiling) {
-        for (int i = 0; i < 64; ++i) hipEventCreate(&c->ev[i]);
+        for (int i = 0; i < 64; ++i) hipEventCreateWithFlags(&c->ev[i], hipEventDisableSystemFence);
     }
     if (!on && c->profiling) {
         hipStreamSynchronize(c->stream);

@@ -332,14 +332,16 @@ int32_t gpl_wa_forward(gpl_ctx* c, float wl_coef_x, float wl_coef_y, float* net_
                        int32_t* net_box, int64_t* hpwl);
 
 /* getWireLengthGradientWA for every cell of a region (nesterov.rs:2418) after gpl_wa_forward:
- * per cell x, y. */
+ * per cell x, y.  The pins' terms are those gpl_wa_forward computed (at the pin positions and
+ * coefficients of that call), summed in each cell's pin order. */
 int32_t gpl_wa_backward(gpl_ctx* c, int32_t region, float* wl_grad);
 
 /* getDensityGradient for every cell of a region (nesterov.rs:2434) after gpl_density_solve. */
 int32_t gpl_density_gather(gpl_ctx* c, int32_t region, float* den_grad);
 
-/* updateGradients (nesterov.rs:2455) into a slot, at the current state: per cell the WA and
- * density gradients and the preconditioned sum; the FP64 sums |wl| and |den|. */
+/* updateGradients (nesterov.rs:2455) into a slot, at the current state (the WA terms of the last
+ * gpl_wa_forward): per cell the WA and density gradients and the preconditioned sum; the FP64
+ * sums |wl| and |den|. */
 int32_t gpl_update_gradients(gpl_ctx* c, int32_t region, int32_t slot, float density_penalty,
                              float* wl, float* den, float* sum, double* abs_sums);
 
