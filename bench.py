@@ -38,27 +38,90 @@ WORKLOADS = {
 }
 
 
+# kernels of each family, the first one launched once per family launch
+FAMILY_KERNELS = {
+    "update_scatter": ["k_place_scatter<2>"],
+    "bin_density": ["k_bin_density"],
+    "dct_solve": ["k_rows<0>", "k_rows<1>", "k_rows<2>", "k_rows<3>", "k_transpose"],
+    "wa_forward": ["k_wa_small<true>", "k_wa_big<true>"],
+    "gradient": ["k_gradient"],
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c3.txt")
+
+
 def algorithmic_bytes(fam, N, P, E, M):
-    """Compulsory bytes per launch of each kernel family (every array touched once):
-    update_scatter  77 N + 16 M   per cell: read cur_slp, cur_sum, cur (24), box (16), flags (1),
-                                  scale (4); write next, next_slp (16), box (16); the two int64 bin
-                                  accumulators written once (16 per bin)
+    """Compulsory bytes per launch of each kernel family (every array touched once; N cells,
+    P pins, E nets, M bins):
+    update_scatter  85 N + 16 M   per cell: read cur_slp, cur_sum, cur (24), box (16), flags (1),
+                                  scale (4); write next, next_slp (16), box (16), centre (8); the two
+                                  int64 bin accumulators written once (16 per bin)
     bin_density     52 M          read inst, filler, non_place, non_place_unscaled (32); write the
                                   density (4); clear the accumulators (16)
     dct_solve       12 M          read the density (4); write field_x, field_y (8)
-    wa_forward      28 P + 52 E   per pin: cell (4), offset (8), its cell's box (16); per net:
-                                  start (4), box (16), sums (32)
-    gradient        80 N + 64 P + 8 M   per cell: box, pin start, preconditioner, scale (32),
-                                  3 slots read, 3 written (48); per cell pin: net (4), offset (8),
-                                  net box + sums + weight (52); the two fields once (8 per bin)
+    wa_forward      32 P + 8 E    per pin: cell (4), offset (8), cell-pin index (4), the cell's
+                                  centre (8), its gradient term written (8); per net: start (4),
+                                  weight (4)
+    gradient        80 N + 8 P + 8 M   per cell: box, pin start, preconditioner, scale (32), 3 slots
+                                  read, 3 written (48); per cell-pin: its gradient term (8); the two
+                                  fields once (8 per bin)
     """
     return {
-        "update_scatter": 77 * N + 16 * M,
+        "update_scatter": 85 * N + 16 * M,
         "bin_density": 52 * M,
         "dct_solve": 12 * M,
-        "wa_forward": 28 * P + 52 * E,
-        "gradient": 80 * N + 64 * P + 8 * M,
+        "wa_forward": 32 * P + 8 * E,
+        "gradient": 80 * N + 8 * P + 8 * M,
     }.get(fam)
+
+
+def read_traffic():
+    """HBM bytes per launch of each kernel family, from the committed rocprofv3 --pmc summary
+    (lines 'traffic <family> <bytes>'); {} when absent."""
+    out = {}
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                t = line.split()
+                if len(t) == 3 and t[0] == "traffic":
+                    out[t[1]] = float(t[2])
+    except OSError:
+        pass
+    return out
+
+
+def traffic_table(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib):
+    """Per kernel family: HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch), each scaled by what the counter reads for a plain copy of copy_mib MiB."""
+    import csv
+    from collections import defaultdict
+
+    def per_kernel(path):
+        tot, disp = defaultdict(float), defaultdict(set)
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                k = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("gplk::", "").strip()
+                tot[k] += float(r["Counter_Value"])
+                disp[k].add(r["Dispatch_Id"])
+        return tot, {k: len(v) for k, v in disp.items()}
+
+    known = copy_mib * 1024.0  # KiB read and written by the copy
+    cf, _ = per_kernel(copy_fetch_csv)
+    cw, cn = per_kernel(copy_write_csv)
+    probe = max(cw, key=lambda k: cw[k] / cn[k] if "neg" in k or "copy" in k.lower() else 0)
+    f_scale = known / (cf[probe] / cn[probe])
+    w_scale = known / (cw[probe] / cn[probe])
+    ft, fn = per_kernel(fetch_csv)
+    wt, wn = per_kernel(write_csv)
+    print(f"# copy probe {probe}: FETCH_SIZE x{f_scale:.3f}, WRITE_SIZE x{w_scale:.3f} (known {copy_mib} MiB each way)")
+    print(f"# {'family':16s} {'launches':>8s} {'fetch_MB':>10s} {'write_MB':>10s}")
+    for fam, ks in FAMILY_KERNELS.items():
+        n = fn.get(ks[0], 0)
+        if not n:
+            continue
+        fetch = sum(ft.get(k, 0.0) for k in ks) / n * f_scale * 1024.0
+        write = sum(wt.get(k, 0.0) for k in ks) / max(wn.get(ks[0], n), 1) * w_scale * 1024.0
+        print(f"# {fam:16s} {n:8d} {fetch / 1e6:10.2f} {write / 1e6:10.2f}")
+        print(f"traffic {fam} {fetch + write:.0f}")
 
 
 def copy_probe(mib):
@@ -134,16 +197,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the HPWL check")
     ap.add_argument("--hpwl-config", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--profile-only", action="store_true",
-                    help="np_init + W + K iterations only (the run rocprofv3 wraps); no convergence, no CPU leg")
+                    help="np_init + W + K iterations of the profiled (serial) path only, the run rocprofv3 wraps; "
+                         "no convergence, no CPU leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a plain device copy of this many MiB three times and exit (the HBM counter "
                          "calibration under rocprofv3 --pmc)")
     ap.add_argument("--summarize", nargs="+", metavar="CSV",
                     help="print a text summary of rocprofv3 CSV output (kernel_stats.csv or counter_collection.csv) and exit")
+    ap.add_argument("--traffic-table", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="print HBM bytes per launch of each kernel family from rocprofv3 --pmc runs of the loop "
+                         "and of --copy-probe-mib 256, and exit")
     args = ap.parse_args()
 
     if args.summarize:
         summarize(args.summarize)
+        return
+    if args.traffic_table:
+        traffic_table(*args.traffic_table, copy_mib=256)
         return
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
@@ -173,6 +243,8 @@ def main():
 
     # ---- the timed run ----
     g = G.NesterovGPU(d)
+    if args.profile_only:
+        g.set_profiling(True)  # the serial path the roofline leg measures, for rocprofv3 to wrap
     g.init_nesterov_loop()
     g.loop_begin(0)
     it, done = g.loop_step(args.warmup)
@@ -183,9 +255,12 @@ def main():
     steps = it - it0 if not done else it - it0 + 1
     log(f"[bench] timed {steps} iterations in {t1 - t0:.3f}s")
     if args.profile_only:
+        kt = g.kernel_times()
         g.close()
+        fam = {k: {"launches": v[0], "avg_us": round(1000.0 * v[1] / max(v[0], 1), 2)} for k, v in kt.items()
+               if k in FAMILY_KERNELS}
         print(json.dumps({"profile_only": True, "config": args.config, "steps": steps,
-                          "iters_per_s": round(steps / (t1 - t0), 2)}), flush=True)
+                          "iters_per_s": round(steps / (t1 - t0), 2), "families": fam}), flush=True)
         d.close()
         return
     while not done:
@@ -221,9 +296,12 @@ def main():
                     "achieved_GBs": (round(algorithmic_bytes(k, N, P, E, M) / (v[1] / 1000.0 / v[0]) / 1e9, 1)
                                      if algorithmic_bytes(k, N, P, E, M) and v[0] else None)}
                 for k, v in kt.items()}
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
-                "algorithmic_bytes_per_launch": bytes_launch, "families": families}
+    traffic = read_traffic().get(dom) if args.config == "c3" else None
+    roofline = {"bound": "hbm", "kernel": dom, "kernels": FAMILY_KERNELS[dom], "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(traffic) if traffic else None, "avg_launch_us": round(avg_s * 1e6, 2),
+                "algorithmic_bytes_per_launch": bytes_launch, "families": families,
+                "resorts": resorts, "scatter_global_adds": global_adds}
 
     # ---- CPU baseline (the oracle, one host thread, bounded sample of the same placement) ----
     cpu = None

@@ -146,6 +146,7 @@ SIGNATURES = [
     ("gpl_util_wavenumber_y", f32, [i32, i32, f32, f32]),
     ("gpl_util_union_area", i64, [P(i32), i32]),
     ("gpl_util_round_down_pow2", u32, [u32]),
+    ("gpl_util_struct_sizes", i32, [P(i64), i32]),
 ]
 
 _lib = None

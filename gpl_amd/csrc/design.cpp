@@ -1550,6 +1550,14 @@ int64_t gpl_util_union_area(const int32_t* rects, int32_t n) {
 
 uint32_t gpl_util_round_down_pow2(uint32_t x) { return gpl::round_down_to_power_of_two(x); }
 
+int32_t gpl_util_struct_sizes(int64_t* out, int32_t cap) {
+    const int64_t sz[] = {sizeof(gpl_opts),   sizeof(gpl_gen_config), sizeof(gpl_design_info), sizeof(gpl_region),
+                          sizeof(gpl_problem), sizeof(gpl_stats),     sizeof(gpl_iter_row),    sizeof(gpl_kernel_time)};
+    const int32_t n = int32_t(sizeof sz / sizeof sz[0]);
+    for (int32_t i = 0; i < n && i < cap && out; ++i) out[i] = sz[i];
+    return n;
+}
+
 int32_t gpl_design_problem(const gpl_design* d, gpl_problem* v) {
     if (!d || !v || !d->ni.built) return GPL_ERR_STATE;
     const gpl::NesterovInit& ni = d->ni;

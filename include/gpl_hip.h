@@ -154,6 +154,11 @@ float gpl_util_wavenumber_y(int32_t i, int32_t ny, float bin_x, float bin_y);
 int64_t gpl_util_union_area(const int32_t* rects, int32_t n);
 uint32_t gpl_util_round_down_pow2(uint32_t x);
 
+/* The byte sizes of the ABI's structs as this library was compiled (no reference counterpart: a
+ * binding's layout check), in the order gpl_opts, gpl_gen_config, gpl_design_info, gpl_region,
+ * gpl_problem, gpl_stats, gpl_iter_row, gpl_kernel_time; returns how many there are. */
+int32_t gpl_util_struct_sizes(int64_t* out, int32_t cap);
+
 /* ------------------------------------------------------------------------------------------ */
 /* The Nesterov placer problem: the state init_nesterov_place (nesterov.rs:787) leaves        */
 /* ------------------------------------------------------------------------------------------ */

@@ -13,7 +13,8 @@
  * Besides the faithful loop it holds FP64 versions of the per-kernel arithmetic (bin density,
  * the Poisson solve, WA forward and backward, the density gather) for the tolerance checks.
  *
- * Pinning (tests/test_oracle.py): the reference's known-answer tests (fast_exp nesterov.rs:4857,
+ * Pinning (tests/test_oracle_kat.py, test_oracle_dct.py, test_oracle_kernels.py, test_host.py):
+ * the reference's known-answer tests (fast_exp nesterov.rs:4857,
  * distance 4893, wavenumbers 4697, reference HPWL 4813, divergence and snapshot rules
  * 4627-4669), brute-force sums on tiny inputs, closed forms (2-pin WA, single-mode Poisson),
  * invariants (translation invariance of WA, the scatter's area conservation) and scipy.fft.

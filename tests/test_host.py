@@ -25,6 +25,15 @@ def test_library_exports_every_declared_function():
     assert declared == set(F.exported_symbols())
 
 
+def test_ctypes_structs_match_the_c_layout():
+    L = F.lib()
+    n = L.gpl_util_struct_sizes(None, 0)
+    out = (C.c_int64 * n)()
+    assert L.gpl_util_struct_sizes(out, n) == n == 8
+    mirror = [F.Opts, F.GenConfig, F.DesignInfo, F.Region, F.Problem, F.Stats, F.IterRow, F.KernelTime]
+    assert [C.sizeof(m) for m in mirror] == list(out)
+
+
 def test_mt19937_reference_output(kats):
     k = kats["mt19937"]
     out = (C.c_uint32 * k["nth"])()
