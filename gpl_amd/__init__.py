@@ -324,6 +324,23 @@ class NesterovGPU:
             "gpl_density_solve", self._h)
         return phi, fx, fy
 
+    def objective_forward(self, xy, wl_coef, lam):
+        """The objective WA + lam * 1/2 sum rho phi at region 0's centres xy: (value, wa, density)."""
+        a = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        assert a.size == 2 * self._n(0)
+        v, w, d = C.c_double(), C.c_double(), C.c_double()
+        _ok(_ffi.lib().gpl_objective_forward(self._h, _fp(a), wl_coef, lam, C.byref(v), C.byref(w), C.byref(d)),
+            "gpl_objective_forward", self._h)
+        return v.value, w.value, d.value
+
+    def objective_backward(self, xy, wl_coef, lam):
+        """d/dxy of the objective per cell of region 0 (problem order)."""
+        a = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        assert a.size == 2 * self._n(0)
+        g = np.zeros_like(a)
+        _ok(_ffi.lib().gpl_objective_backward(self._h, _fp(a), wl_coef, lam, _fp(g)), "gpl_objective_backward", self._h)
+        return g.reshape(-1, 2)
+
     def wa_forward(self, coef_x, coef_y):
         E = self.problem.num_nets
         sums = np.zeros(8 * E, np.float32)

@@ -141,9 +141,15 @@ void launch_initial_prev(hipStream_t s, const RegionDev& r, const CellsDev& c, c
 // lock the instance cells of a converged region; rotate locked cells' slots.
 void launch_lock_fix(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* const* next_vec, const float2* const* cur_vec, int nvec);
 void launch_hpwl(hipStream_t s, const NetsDev& n, const CellsDev& c, unsigned long long* hpwl);
-// the objective (exact exponential, continuous coordinates): WA value and gradient per cell.
-void launch_objective(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, const float2* xy, float coef,
-                      double* net_val, float2* grad_wl);
+// The differentiable objective over region r's cells at continuous centres xy (device order):
+// density: rho into r.density from the continuous boxes, then the solve with phi;
+// value: out2[0] = sum_e w_e WA_e (exact exponential), out2[1] = sum_b rho_b phi_b (FP64);
+// gradient: per cell d/dxy of WA + lambda * 1/2 sum rho phi (needs the density step first).
+void launch_objective_density(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy);
+void launch_objective_value(hipStream_t s, const RegionDev& r, const NetsDev& n, const float2* xy, float coef, double* partials,
+                            double* out2);
+void launch_objective_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, const float2* xy, float coef,
+                               float lambda, double* partials, float2* grad);
 
 // ---- spatial re-sort of the cells (device order only; results do not depend on it) ---------
 constexpr int kMaxRegions = 32;

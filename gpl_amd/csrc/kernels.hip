@@ -8,6 +8,7 @@
 // associative.
 #include "kernels.h"
 
+#include <cfloat>
 #include <climits>
 
 #include <hipcub/hipcub.hpp>
@@ -935,6 +936,194 @@ __global__ void k_lock_fix(RegionDev r, CellsDev c, float2* n0, float2* n1, floa
 // Launchers
 // ---------------------------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------------------------
+// The differentiable objective (no reference counterpart: the reference never evaluates it):
+// f = sum_e w_e WA_e + lambda * 1/2 sum_b rho_b phi_b over one region's cells at continuous
+// centres xy.  WA uses the exact exponential; pins of the region's cells sit at xy + offset, other
+// pins at their initial positions.  rho_b = (sum_i overlap_b(i) scale_i [target if macro] +
+// non_place_b) / (area_b target) with each density box centred at its point (not truncated);
+// phi = K rho from the region's Poisson solve.  K is symmetric, so d(1/2 rho^T K rho)/d rho = phi.
+// ---------------------------------------------------------------------------------------------
+constexpr double kObjFix = 65536.0;  // fixed point of the overlap accumulators: 2^-16 DBU^2
+
+__device__ __forceinline__ double obj_weight(const RegionDev& r, const CellsDev& c, int g) {
+    double s = (double)c.dscale[g];
+    const uint8_t f = c.flags[g];
+    if ((f & 1u) && (f & 2u)) s *= (double)r.target_density;
+    return s;
+}
+
+// bins [x0, x1] x [y0, y1] that may touch the box (one bin of margin, clamped to the grid)
+__device__ __forceinline__ void obj_span(const RegionDev& r, double lo, double hi, int org, double size, int cnt, int& a, int& b) {
+    a = (int)floor((lo - org) / size) - 1;
+    b = (int)floor((hi - org) / size) + 1;
+    a = max(a, 0);
+    b = min(b, cnt - 1);
+}
+
+__global__ void k_obj_scatter(RegionDev r, CellsDev c, const float2* xy, unsigned long long* acc) {
+    const int n = r.cell_end - r.cell_begin;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int g = r.cell_begin + k;
+        const int4 d = c.dbox[g];
+        const double hw = 0.5 * (double)(d.z - d.x), hh = 0.5 * (double)(d.w - d.y);
+        const float2 p = xy[k];
+        const double bl = (double)p.x - hw, bu = (double)p.x + hw, cl = (double)p.y - hh, cu = (double)p.y + hh;
+        const double s = obj_weight(r, c, g) * kObjFix;
+        int x0, x1, y0, y1;
+        obj_span(r, bl, bu, r.lx, r.size_x, r.cnt_x, x0, x1);
+        obj_span(r, cl, cu, r.ly, r.size_y, r.cnt_y, y0, y1);
+        for (int y = y0; y <= y1; ++y) {
+            const double oy = fmin(cu, (double)r.bin_y[y + 1]) - fmax(cl, (double)r.bin_y[y]);
+            if (oy <= 0.0) continue;
+            for (int x = x0; x <= x1; ++x) {
+                const double ox = fmin(bu, (double)r.bin_x[x + 1]) - fmax(bl, (double)r.bin_x[x]);
+                if (ox <= 0.0) continue;
+                atomicAdd(acc + (size_t)y * r.cnt_x + x, (unsigned long long)llrint(ox * oy * s));
+            }
+        }
+    }
+}
+
+// rho per bin into r.density (float, the solve's input); the accumulators cleared
+__global__ void k_obj_rho(RegionDev r, unsigned long long* acc) {
+    const size_t M = (size_t)r.cnt_x * r.cnt_y;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < M; k += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(k % r.cnt_x), y = (int)(k / r.cnt_x);
+        const double a = (double)(r.bin_x[x + 1] - r.bin_x[x]) * (double)(r.bin_y[y + 1] - r.bin_y[y]) * (double)r.target_density;
+        const double v = (double)(long long)acc[k] / kObjFix + (double)r.non_place[k];
+        r.density[k] = (float)(v / a);
+        acc[k] = 0ull;
+    }
+}
+
+// sum_b rho_b phi_b as FP64 block partials
+__global__ void k_obj_energy(RegionDev r, double* partials) {
+    __shared__ double sh[kBlock / 64];
+    const size_t M = (size_t)r.cnt_x * r.cnt_y;
+    double e = 0.0;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < M; k += (size_t)gridDim.x * blockDim.x)
+        e += (double)r.density[k] * (double)r.phi[k];
+    e = block_sum(e, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = e;
+}
+
+__device__ __forceinline__ float2 obj_pin(const NetsDev& n, const RegionDev& r, const float2* xy, int k) {
+    const int cell = n.pcell[k];
+    if (cell >= r.cell_begin && cell < r.cell_end) {
+        const float2 p = xy[cell - r.cell_begin];
+        const int2 o = n.poff[k];
+        return make_float2(p.x + (float)o.x, p.y + (float)o.y);
+    }
+    const int2 q = n.pxy0[k];
+    return make_float2((float)q.x, (float)q.y);
+}
+
+// One net per thread: the weighted WA value (FP64 block partials); with GRAD each pin's gradient
+// term dWA/dx (the reference's term expressions with the exact exponential, times the weight) at
+// its cell-pin index.
+template <bool GRAD>
+__global__ void k_obj_wa(NetsDev n, RegionDev r, const float2* xy, float cf, double* partials) {
+    __shared__ double sh[kBlock / 64];
+    double acc = 0.0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n.num_nets; e += gridDim.x * blockDim.x) {
+        const int s = n.start[e], t = n.start[e + 1];
+        if (t <= s) continue;
+        float lx = FLT_MAX, ly = FLT_MAX, ux = -FLT_MAX, uy = -FLT_MAX;
+        for (int k = s; k < t; ++k) {
+            const float2 p = obj_pin(n, r, xy, k);
+            lx = fminf(lx, p.x);
+            ux = fmaxf(ux, p.x);
+            ly = fminf(ly, p.y);
+            uy = fmaxf(uy, p.y);
+        }
+        float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f, c0 = 0.f, c1 = 0.f, d0 = 0.f, d1 = 0.f;
+        for (int k = s; k < t; ++k) {
+            const float2 p = obj_pin(n, r, xy, k);
+            const float ea = expf(cf * (p.x - ux)), eb = expf(cf * (lx - p.x));
+            const float ec = expf(cf * (p.y - uy)), ed = expf(cf * (ly - p.y));
+            a0 += ea;
+            a1 += p.x * ea;
+            b0 += eb;
+            b1 += p.x * eb;
+            c0 += ec;
+            c1 += p.y * ec;
+            d0 += ed;
+            d1 += p.y * ed;
+        }
+        const float w = n.weight[e];
+        acc += (double)w * (((double)a1 / a0 - (double)b1 / b0) + ((double)c1 / c0 - (double)d1 / d0));
+        if (GRAD) {
+            for (int k = s; k < t; ++k) {
+                const int q = n.cpi[k];
+                if (q < 0) continue;
+                const float2 p = obj_pin(n, r, xy, k);
+                const float ea = expf(cf * (p.x - ux)), eb = expf(cf * (lx - p.x));
+                const float ec = expf(cf * (p.y - uy)), ed = expf(cf * (ly - p.y));
+                float gx = wa_term_max(a0, a1, ea, cf, p.x) - wa_term_min(b0, b1, eb, cf, p.x);
+                float gy = wa_term_max(c0, c1, ec, cf, p.y) - wa_term_min(d0, d1, ed, cf, p.y);
+                n.pgrad[q] = make_float2(gx * w, gy * w);
+            }
+        }
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// Per cell: the WA gradient (its pins' terms) plus lambda times the density adjoint
+// sum_b phi_b d rho_b / dx, with d overlap / dx = (height of the overlap) x ([right edge in the
+// bin] - [left edge in the bin]).
+__global__ void k_obj_cell_grad(RegionDev r, CellsDev c, NetsDev nt, const float2* xy, float lambda, float2* grad) {
+    const int n = r.cell_end - r.cell_begin;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int g = r.cell_begin + k;
+        float wx = 0.0f, wy = 0.0f;
+        for (int q = c.pin_start[g]; q < c.pin_start[g + 1]; ++q) {
+            const float2 t = nt.pgrad[q];
+            wx += t.x;
+            wy += t.y;
+        }
+        const int4 d = c.dbox[g];
+        const double hw = 0.5 * (double)(d.z - d.x), hh = 0.5 * (double)(d.w - d.y);
+        const float2 p = xy[k];
+        const double bl = (double)p.x - hw, bu = (double)p.x + hw, cl = (double)p.y - hh, cu = (double)p.y + hh;
+        const double s = obj_weight(r, c, g);
+        int x0, x1, y0, y1;
+        obj_span(r, bl, bu, r.lx, r.size_x, r.cnt_x, x0, x1);
+        obj_span(r, cl, cu, r.ly, r.size_y, r.cnt_y, y0, y1);
+        double gx = 0.0, gy = 0.0;
+        for (int y = y0; y <= y1; ++y) {
+            const double by0 = r.bin_y[y], by1 = r.bin_y[y + 1];
+            const double oy = fmin(cu, by1) - fmax(cl, by0);
+            if (oy <= 0.0) continue;
+            const double dy = (cu > by0 && cu < by1 ? 1.0 : 0.0) - (cl > by0 && cl < by1 ? 1.0 : 0.0);
+            for (int x = x0; x <= x1; ++x) {
+                const double bx0 = r.bin_x[x], bx1 = r.bin_x[x + 1];
+                const double ox = fmin(bu, bx1) - fmax(bl, bx0);
+                if (ox <= 0.0) continue;
+                const double dx = (bu > bx0 && bu < bx1 ? 1.0 : 0.0) - (bl > bx0 && bl < bx1 ? 1.0 : 0.0);
+                const size_t kb = (size_t)y * r.cnt_x + x;
+                const double f = (double)r.phi[kb] * s / ((bx1 - bx0) * (by1 - by0) * (double)r.target_density);
+                gx += f * oy * dx;
+                gy += f * ox * dy;
+            }
+        }
+        grad[k] = make_float2(wx + lambda * (float)gx, wy + lambda * (float)gy);
+    }
+}
+
+// out[q] = sum of partials[q * stride + i], i < n, in order (one block)
+__global__ void k_sum_partials(const double* partials, int n, int stride, int nq, double* out) {
+    __shared__ double sh[kBlock / 64];
+    for (int q = 0; q < nq; ++q) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) v += partials[(size_t)q * stride + i];
+        v = block_sum(v, sh);
+        if (threadIdx.x == 0) out[q] = v;
+        __syncthreads();
+    }
+}
+
 static int grid_for(size_t n) {
     size_t b = (n + kBlock - 1) / kBlock;
     if (b < 1) b = 1;
@@ -1143,6 +1332,29 @@ void launch_lock_fix(hipStream_t s, const RegionDev& r, const CellsDev& c, float
     if (n <= 0) return;
     hipLaunchKernelGGL(k_lock_fix, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, nv[0], nv[1], nv[2], nv[3], nv[4], cv[0], cv[1], cv[2],
                        cv[3], cv[4]);
+}
+
+void launch_objective_density(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy) {
+    const int n = r.cell_end - r.cell_begin;
+    const size_t M = (size_t)r.cnt_x * r.cnt_y;
+    if (n > 0) hipLaunchKernelGGL(k_obj_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, xy, r.inst);
+    hipLaunchKernelGGL(k_obj_rho, dim3(grid_for(M)), dim3(kBlock), 0, s, r, r.inst);
+    launch_solve(s, r, true);
+}
+
+void launch_objective_value(hipStream_t s, const RegionDev& r, const NetsDev& n, const float2* xy, float coef, double* partials,
+                            double* out2) {
+    const int gw = grid_for(n.num_nets), gb = grid_for((size_t)r.cnt_x * r.cnt_y);
+    hipLaunchKernelGGL(k_obj_wa<false>, dim3(gw), dim3(kBlock), 0, s, n, r, xy, coef, partials);
+    hipLaunchKernelGGL(k_obj_energy, dim3(gb), dim3(kBlock), 0, s, r, partials + kPartials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, partials, gw > gb ? gw : gb, kPartials, 2, out2);
+}
+
+void launch_objective_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, const float2* xy, float coef,
+                               float lambda, double* partials, float2* grad) {
+    const int gw = grid_for(n.num_nets), nc = r.cell_end - r.cell_begin;
+    hipLaunchKernelGGL(k_obj_wa<true>, dim3(gw), dim3(kBlock), 0, s, n, r, xy, coef, partials);
+    if (nc > 0) hipLaunchKernelGGL(k_obj_cell_grad, dim3(grid_for(nc)), dim3(kBlock), 0, s, r, c, n, xy, lambda, grad);
 }
 
 // ---------------------------------------------------------------------------------------------

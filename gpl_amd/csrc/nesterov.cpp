@@ -67,6 +67,10 @@ struct gpl_ctx {
     // A second stream (default priority) for the bins and the Poisson solve of a trial, which run
     // beside the WA kernels; the gradient waits on ev_join.  Off while profiling, so that each
     // kernel family's events bracket that family alone.
+    // the objective entries' buffers (allocated on first use)
+    float2* obj_xy = nullptr;
+    float2* obj_grad = nullptr;
+    double* obj_out = nullptr;
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool overlap = true, join_pending = false;
@@ -1666,7 +1670,52 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap) {
     return n;
 }
 
-int32_t gpl_objective_forward(gpl_ctx*, const float*, float, float, double*, double*, double*) { return GPL_ERR_REFUSED; }
-int32_t gpl_objective_backward(gpl_ctx*, const float*, float, float, float*) { return GPL_ERR_REFUSED; }
+// The objective's common first step: region 0's points uploaded (device order) and its density
+// and potential computed from the continuous boxes.  The region's bin arrays are reused, so the
+// loop's fields are overwritten (every trial recomputes them).
+static void objective_density(gpl_ctx* c, const float* xy) {
+    Region& r = c->rg[0];
+    c->join();
+    if (!c->obj_xy) {
+        c->obj_xy = c->dalloc<float2>(std::max(r.n, 1));
+        c->obj_grad = c->dalloc<float2>(std::max(r.n, 1));
+        c->obj_out = c->dalloc<double>(2);
+    }
+    c->upload_region_vec(0, c->obj_xy, xy);
+    gplk::launch_objective_density(c->stream, r.dev, c->cells, c->obj_xy);
+}
+
+int32_t gpl_objective_forward(gpl_ctx* c, const float* xy, float wl_coef, float lambda, double* value, double* wa_part,
+                              double* density_part) {
+    if (!c || !xy || c->rg.empty()) return GPL_ERR_ARG;
+    try {
+        objective_density(c, xy);
+        Region& r = c->rg[0];
+        gplk::launch_objective_value(c->stream, r.dev, c->nets, c->obj_xy, wl_coef, r.partials, c->obj_out);
+        double h[2] = {0.0, 0.0};
+        check(hipMemcpyAsync(h, c->obj_out, sizeof h, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+        const double d = 0.5 * h[1];
+        if (wa_part) *wa_part = h[0];
+        if (density_part) *density_part = d;
+        if (value) *value = h[0] + (double)lambda * d;
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+int32_t gpl_objective_backward(gpl_ctx* c, const float* xy, float wl_coef, float lambda, float* grad) {
+    if (!c || !xy || !grad || c->rg.empty()) return GPL_ERR_ARG;
+    try {
+        objective_density(c, xy);
+        Region& r = c->rg[0];
+        gplk::launch_objective_gradient(c->stream, r.dev, c->cells, c->nets, c->obj_xy, wl_coef, lambda, r.partials, c->obj_grad);
+        c->download_region_vec(0, c->obj_grad, grad);
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
 
 }  // extern "C"

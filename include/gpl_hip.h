@@ -355,10 +355,17 @@ int32_t gpl_update_gradients(gpl_ctx* c, int32_t region, int32_t slot, float den
 int32_t gpl_nesterov_step(gpl_ctx* c, int32_t region, float step, float coeff, float* next,
                           float* next_slp);
 
-/* The differentiable objective (no reference counterpart: the reference never evaluates it):
- * f(x) = WA(x) + lambda * sum_bins phi*rho for the cell centres `xy` (region 0), with the
- * exact exponential and continuous coordinates.  forward returns f (FP64 reduction);
- * backward writes df/dxy per cell. */
+/* The differentiable objective (no reference counterpart: the reference evaluates only its
+ * gradient, nesterov.rs:2392-2449): f(x) = sum_e w_e WA_e(x) + lambda * 1/2 sum_b rho_b phi_b over
+ * region 0's cells at the continuous centres `xy` (2 floats per cell, problem order).  WA_e is the
+ * weighted-average wirelength of nesterov.rs:2325-2388 with the exact exponential (expf) and
+ * coefficient wl_coef; pins of region 0's cells sit at xy + offset, other pins at their initial
+ * positions.  rho_b = (sum_i overlap_b(i) density_scale_i [x target for macros] + non_place_b) /
+ * (area_b target) with each density box centred at its point (not truncated to integers), and
+ * phi = K rho with K the region's Poisson solve (symmetric, so d/d rho of 1/2 rho^T K rho = phi).
+ * forward: *value = wa + lambda * density, the two parts summed in FP64.  backward: df/dxy per
+ * cell (float), the WA terms with the reference's double quotients.  Both overwrite region 0's
+ * density, potential and fields (the loop recomputes them every trial). */
 int32_t gpl_objective_forward(gpl_ctx* c, const float* xy, float wl_coef, float lambda,
                               double* value, double* wa_part, double* density_part);
 int32_t gpl_objective_backward(gpl_ctx* c, const float* xy, float wl_coef, float lambda,

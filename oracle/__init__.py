@@ -70,6 +70,9 @@ def lib():
             ("orc_get_dbox", None, [C.c_void_p, i32p]),
             ("orc_get_pins", None, [C.c_void_p, i32p]),
             ("orc_wa_net_f64", C.c_double, [C.c_int32, f64p, C.c_double, C.c_int32, f64p]),
+            ("orc_objective_value", None, [C.c_void_p, f32p, C.c_float, f64p, f64p]),
+            ("orc_objective_fd", None, [C.c_void_p, f32p, C.c_float, C.c_float, C.c_int32, i32p, C.c_double, f64p]),
+            ("orc_solve_phi_f64", None, [C.c_int32, C.c_int32, f32p, f32p, f64p, f64p]),
         ]
         for name, res, args in sig:
             fn = getattr(L, name)
@@ -247,3 +250,29 @@ def solve_f32(nx, ny, wx, wy, density_ym):
                             f32(d), f32(phi), f32(fx), f32(fy))
     back = lambda a: a.reshape(nx, ny).T.reshape(-1).copy()
     return back(phi), back(fx), back(fy)
+
+
+def objective_value(problem, xy, wl_coef):
+    """FP64 value of the objective's two parts (wirelength, 1/2 sum rho phi) at region 0's cell
+    centres xy (problem order)."""
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1)
+    w, d = C.c_double(0.0), C.c_double(0.0)
+    lib().orc_objective_value(C.addressof(problem), f32(xy), wl_coef, C.byref(w), C.byref(d))
+    return w.value, d.value
+
+
+def objective_fd(problem, xy, wl_coef, lam, sample, h):
+    """FP64 central differences of the objective for the sampled cells: rows (d/dx, d/dy,
+    kink_x, kink_y)."""
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1)
+    sample = np.ascontiguousarray(sample, np.int32)
+    out = np.zeros(4 * len(sample), np.float64)
+    lib().orc_objective_fd(C.addressof(problem), f32(xy), wl_coef, lam, len(sample), i32(sample), h, f64(out))
+    return out.reshape(-1, 4)
+
+
+def solve_phi_f64(nx, ny, wx, wy, density):
+    d = np.ascontiguousarray(density, np.float64)
+    phi = np.zeros(nx * ny, np.float64)
+    lib().orc_solve_phi_f64(nx, ny, f32(np.ascontiguousarray(wx, np.float32)), f32(np.ascontiguousarray(wy, np.float32)), f64(d), f64(phi))
+    return phi

@@ -136,12 +136,14 @@ void orc_fft_solve_f32(int32_t nx, int32_t ny, const float* wx, const float* wy,
 
 /* FP64: bins in y-major layout (k = y*nx + x) in and out, the same arithmetic in double with
  * the float wavenumbers widened. */
-void orc_fft_solve_f64(int32_t nx, int32_t ny, const float* wx, const float* wy, const float* density_ym, double* phi_ym, double* fx_ym, double* fy_ym) {
+/* The solve of a double density (y-major), as orc_fft_solve_f64 below. */
+void orc_fft_solve_f64d(int32_t nx, int32_t ny, const float* wx, const float* wy, const double* density_ym, double* phi_ym, double* fx_ym,
+                        double* fy_ym) {
     size_t M = (size_t)nx * ny;
     double* a = (double*)malloc(sizeof(double) * M * 4);
     double *ph = a + M, *ex = a + 2 * M, *ey = a + 3 * M;
     for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) a[(size_t)x * ny + y] = (double)density_ym[(size_t)y * nx + x];
+        for (int x = 0; x < nx; ++x) a[(size_t)x * ny + y] = density_ym[(size_t)y * nx + x];
     orc_ddct2d_fwd_f64(nx, ny, a);
     for (int i = 1; i < nx; ++i) a[(size_t)i * ny] *= 0.5;
     for (int j = 1; j < ny; ++j) a[j] *= 0.5;
@@ -167,6 +169,15 @@ void orc_fft_solve_f64(int32_t nx, int32_t ny, const float* wx, const float* wy,
             if (fy_ym) fy_ym[d] = ey[s];
         }
     free(a);
+}
+
+/* Fft::solve (nesterov.rs:412-444) in double: phi and the two fields of a float density. */
+void orc_fft_solve_f64(int32_t nx, int32_t ny, const float* wx, const float* wy, const float* density_ym, double* phi_ym, double* fx_ym, double* fy_ym) {
+    size_t M = (size_t)nx * ny;
+    double* d = (double*)malloc(sizeof(double) * M);
+    for (size_t k = 0; k < M; ++k) d[k] = (double)density_ym[k];
+    orc_fft_solve_f64d(nx, ny, wx, wy, d, phi_ym, fx_ym, fy_ym);
+    free(d);
 }
 
 /* ---- the placer state ----------------------------------------------------------------------- */
@@ -1383,4 +1394,217 @@ double orc_wa_net_f64(int32_t n, const double* xs, double coef, int32_t use_fast
     }
     free(en);
     return s1x / s0x - s1n / s0n;
+}
+
+/* ---- the differentiable objective (no reference counterpart), in FP64 ----------------------- */
+/*
+ * f(x) = sum_e w_e (WA_e,x + WA_e,y) + lambda * 1/2 sum_b rho_b phi_b over region 0's cells at the
+ * continuous centres xy (problem order): WA with the exact exponential (the weighted-average
+ * wirelength of nesterov.rs:2325-2388 without fast_exp), pins at centre + offset (pins of other
+ * cells at their initial positions, ports fixed); rho_b = (sum_i overlap_b(i) scale_i [target if
+ * macro] + non_place_b) / (area_b target) with each cell's density box of the problem's size
+ * centred at its point (not truncated to integers); phi = K rho, K the Poisson solve.
+ */
+typedef struct {
+    const gpl_problem* p;
+    const gpl_region* R;
+    const float* xy;
+    double c;
+} obj_ctx;
+
+static void obj_pin(const obj_ctx* o, int32_t q, int32_t moved, int axis, double delta, double* px, double* py) {
+    int32_t g = o->p->pin_cell[q];
+    if (g >= o->R->cell_begin && g < o->R->cell_end) {
+        int32_t k = g - o->R->cell_begin;
+        *px = (double)o->xy[2 * k] + (double)o->p->pin_offset[2 * q];
+        *py = (double)o->xy[2 * k + 1] + (double)o->p->pin_offset[2 * q + 1];
+        if (g == moved) {
+            if (axis == 0) *px += delta;
+            else *py += delta;
+        }
+    } else {
+        *px = (double)o->p->pin_xy[2 * q];
+        *py = (double)o->p->pin_xy[2 * q + 1];
+    }
+}
+
+/* one net's WA (x plus y) in double, with cell `moved` shifted by delta along axis */
+static double obj_net_wa(const obj_ctx* o, int32_t e, int32_t moved, int axis, double delta) {
+    const gpl_problem* p = o->p;
+    int32_t s = p->net_pin_start[e], t = p->net_pin_start[e + 1];
+    if (t - s < 1) return 0.0;
+    double lx = DBL_MAX, ly = DBL_MAX, ux = -DBL_MAX, uy = -DBL_MAX;
+    for (int32_t k = s; k < t; ++k) {
+        double px, py;
+        obj_pin(o, p->net_pins[k], moved, axis, delta, &px, &py);
+        if (px < lx) lx = px;
+        if (px > ux) ux = px;
+        if (py < ly) ly = py;
+        if (py > uy) uy = py;
+    }
+    double a0 = 0, a1 = 0, b0 = 0, b1 = 0, c0 = 0, c1 = 0, d0 = 0, d1 = 0;
+    for (int32_t k = s; k < t; ++k) {
+        double px, py;
+        obj_pin(o, p->net_pins[k], moved, axis, delta, &px, &py);
+        double ea = exp(o->c * (px - ux)), eb = exp(o->c * (lx - px));
+        double ec = exp(o->c * (py - uy)), ed = exp(o->c * (ly - py));
+        a0 += ea;
+        a1 += px * ea;
+        b0 += eb;
+        b1 += px * eb;
+        c0 += ec;
+        c1 += py * ec;
+        d0 += ed;
+        d1 += py * ed;
+    }
+    return (a1 / a0 - b1 / b0) + (c1 / c0 - d1 / d0);
+}
+
+/* cell k's (region-local) density box at xy (+ delta along axis): lx, ly, ux, uy */
+static void obj_box(const obj_ctx* o, int32_t k, int axis, double delta, double* b) {
+    int32_t g = o->R->cell_begin + k;
+    const int32_t* d = &o->p->cell_dbox[4 * (size_t)g];
+    double w = (double)(d[2] - d[0]), h = (double)(d[3] - d[1]);
+    double x = (double)o->xy[2 * k] + (axis == 0 ? delta : 0.0), y = (double)o->xy[2 * k + 1] + (axis == 1 ? delta : 0.0);
+    b[0] = x - 0.5 * w;
+    b[1] = y - 0.5 * h;
+    b[2] = x + 0.5 * w;
+    b[3] = y + 0.5 * h;
+}
+
+static double obj_cell_weight(const obj_ctx* o, int32_t k) {
+    int32_t g = o->R->cell_begin + k;
+    double w = (double)o->p->density_scale[g];
+    uint8_t f = o->p->cell_flags[g];
+    if ((f & GPL_CELL_INSTANCE) && (f & GPL_CELL_MACRO)) w *= (double)o->R->target_density;
+    return w;
+}
+
+/* adds s * overlap(b, bin) / (area * target) into rho for every bin the box touches */
+static void obj_add_box(const obj_ctx* o, const double* b, double s, double* rho) {
+    const gpl_region* R = o->R;
+    int x0 = (int)floor((b[0] - R->lx) / R->size_x) - 1, x1 = (int)floor((b[2] - R->lx) / R->size_x) + 1;
+    int y0 = (int)floor((b[1] - R->ly) / R->size_y) - 1, y1 = (int)floor((b[3] - R->ly) / R->size_y) + 1;
+    if (x0 < 0) x0 = 0;
+    if (y0 < 0) y0 = 0;
+    if (x1 > R->cnt_x - 1) x1 = R->cnt_x - 1;
+    if (y1 > R->cnt_y - 1) y1 = R->cnt_y - 1;
+    for (int y = y0; y <= y1; ++y) {
+        double by0 = R->bin_y[y], by1 = R->bin_y[y + 1];
+        double oy = fmin(b[3], by1) - fmax(b[1], by0);
+        if (oy <= 0) continue;
+        for (int x = x0; x <= x1; ++x) {
+            double bx0 = R->bin_x[x], bx1 = R->bin_x[x + 1];
+            double ox = fmin(b[2], bx1) - fmax(b[0], bx0);
+            if (ox <= 0) continue;
+            rho[(size_t)y * R->cnt_x + x] += s * ox * oy / ((bx1 - bx0) * (by1 - by0) * (double)R->target_density);
+        }
+    }
+}
+
+static double* obj_rho(const obj_ctx* o) {
+    const gpl_region* R = o->R;
+    size_t M = (size_t)R->cnt_x * R->cnt_y;
+    double* rho = (double*)calloc(M, sizeof(double));
+    for (int y = 0; y < R->cnt_y; ++y)
+        for (int x = 0; x < R->cnt_x; ++x) {
+            double a = (double)(R->bin_x[x + 1] - R->bin_x[x]) * (double)(R->bin_y[y + 1] - R->bin_y[y]);
+            rho[(size_t)y * R->cnt_x + x] = (double)R->non_place[(size_t)y * R->cnt_x + x] / (a * (double)R->target_density);
+        }
+    for (int32_t k = 0; k < R->cell_end - R->cell_begin; ++k) {
+        double b[4];
+        obj_box(o, k, 0, 0.0, b);
+        obj_add_box(o, b, obj_cell_weight(o, k), rho);
+    }
+    return rho;
+}
+
+/* The objective's value in FP64: *wa = sum_e w_e WA_e, *den = 1/2 sum rho phi. */
+void orc_objective_value(const gpl_problem* p, const float* xy, float wl_coef, double* wa, double* den) {
+    obj_ctx o = {p, &p->regions[0], xy, (double)wl_coef};
+    double w = 0.0;
+    for (int32_t e = 0; e < p->num_nets; ++e) w += (double)p->net_weight[e] * obj_net_wa(&o, e, -1, 0, 0.0);
+    const gpl_region* R = o.R;
+    size_t M = (size_t)R->cnt_x * R->cnt_y;
+    double* rho = obj_rho(&o);
+    double* phi = (double*)malloc(sizeof(double) * M);
+    orc_fft_solve_f64d(R->cnt_x, R->cnt_y, R->wx, R->wy, rho, phi, NULL, NULL);
+    double d = 0.0;
+    for (size_t k = 0; k < M; ++k) d += rho[k] * phi[k];
+    free(rho);
+    free(phi);
+    *wa = w;
+    *den = 0.5 * d;
+}
+
+/*
+ * Central differences in FP64 of the objective for the sampled cells (region-local indices):
+ * out[4 i] = (f(x + h e_x) - f(x - h e_x)) / 2h, out[4 i + 1] the same along y, out[4 i + 2],
+ * out[4 i + 3] = 1 where a box edge lies within h of a bin edge along that axis (the density term
+ * has a kink there; the difference is not a derivative).  The wirelength part re-evaluates the
+ * nets on the cell's pins; the density part is phi^T (rho(x+h) - rho(x-h)), exact for the
+ * quadratic 1/2 rho^T K rho with K symmetric wherever rho is linear on [x-h, x+h] (no kink).
+ */
+void orc_objective_fd(const gpl_problem* p, const float* xy, float wl_coef, float lambda, int32_t ns, const int32_t* sample, double h,
+                      double* out) {
+    obj_ctx o = {p, &p->regions[0], xy, (double)wl_coef};
+    const gpl_region* R = o.R;
+    size_t M = (size_t)R->cnt_x * R->cnt_y;
+    double* rho = obj_rho(&o);
+    double* phi = (double*)malloc(sizeof(double) * M);
+    orc_fft_solve_f64d(R->cnt_x, R->cnt_y, R->wx, R->wy, rho, phi, NULL, NULL);
+    double* dr = (double*)calloc(M, sizeof(double));
+    for (int32_t i = 0; i < ns; ++i) {
+        int32_t k = sample[i], g = R->cell_begin + k;
+        for (int axis = 0; axis < 2; ++axis) {
+            double dw = 0.0;
+            for (int32_t q = p->cell_pin_start[g]; q < p->cell_pin_start[g + 1]; ++q) {
+                int32_t e = p->pin_net[p->cell_pins[q]];
+                /* a net with several pins on this cell is visited once per pin: count it once */
+                int32_t first = 1;
+                for (int32_t q2 = p->cell_pin_start[g]; q2 < q; ++q2)
+                    if (p->pin_net[p->cell_pins[q2]] == e) first = 0;
+                if (!first) continue;
+                dw += (double)p->net_weight[e] * (obj_net_wa(&o, e, g, axis, h) - obj_net_wa(&o, e, g, axis, -h));
+            }
+            double bp[4], bm[4], s = obj_cell_weight(&o, k);
+            obj_box(&o, k, axis, h, bp);
+            obj_box(&o, k, axis, -h, bm);
+            obj_add_box(&o, bp, s, dr);
+            obj_add_box(&o, bm, -s, dr);
+            double dd = 0.0;
+            int x0 = (int)floor((fmin(bp[0], bm[0]) - R->lx) / R->size_x) - 1, x1 = (int)floor((fmax(bp[2], bm[2]) - R->lx) / R->size_x) + 1;
+            int y0 = (int)floor((fmin(bp[1], bm[1]) - R->ly) / R->size_y) - 1, y1 = (int)floor((fmax(bp[3], bm[3]) - R->ly) / R->size_y) + 1;
+            if (x0 < 0) x0 = 0;
+            if (y0 < 0) y0 = 0;
+            if (x1 > R->cnt_x - 1) x1 = R->cnt_x - 1;
+            if (y1 > R->cnt_y - 1) y1 = R->cnt_y - 1;
+            for (int y = y0; y <= y1; ++y)
+                for (int x = x0; x <= x1; ++x) {
+                    size_t b = (size_t)y * R->cnt_x + x;
+                    dd += phi[b] * dr[b];
+                    dr[b] = 0.0;
+                }
+            out[4 * (size_t)i + axis] = (dw + (double)lambda * dd) / (2.0 * h);
+            /* kink: an edge of the box along this axis within h of a bin edge (or the grid's) */
+            const int32_t* edges = axis == 0 ? R->bin_x : R->bin_y;
+            int32_t ne = (axis == 0 ? R->cnt_x : R->cnt_y) + 1;
+            double lo = axis == 0 ? 0.5 * (bp[0] + bm[0]) : 0.5 * (bp[1] + bm[1]);
+            double hi = axis == 0 ? 0.5 * (bp[2] + bm[2]) : 0.5 * (bp[3] + bm[3]);
+            int kink = 0;
+            for (int32_t j = 0; j < ne && !kink; ++j) {
+                double ev = (double)edges[j];
+                if (fabs(ev - lo) <= h || fabs(ev - hi) <= h) kink = 1;
+            }
+            out[4 * (size_t)i + 2 + axis] = kink;
+        }
+    }
+    free(dr);
+    free(rho);
+    free(phi);
+}
+
+/* The Poisson solve as an operator on double densities (for the symmetry check). */
+void orc_solve_phi_f64(int32_t nx, int32_t ny, const float* wx, const float* wy, const double* density, double* phi) {
+    orc_fft_solve_f64d(nx, ny, wx, wy, density, phi, NULL, NULL);
 }
