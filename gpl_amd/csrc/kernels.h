@@ -62,6 +62,9 @@ struct CellsDev {
     uint8_t* flags;             // GPL_CELL_* (locked is updated on convergence)
     int32_t* pin_start;         // num_cells + 1
     int32_t* cp_npin;           // per cell-pin: its index in the net-order pin arrays
+    int2* cp_off;               // per cell-pin: the pin's offset from the cell centre
+    int2* ppos;                 // per net-pin (net order, shared by both cell orders): the pin's
+                                // position, written by the scatter with the cell's centre
     float2* pre;                // (pin count, dx*dy) per cell, as float
     int32_t* old_of;            // the problem's index of each device cell
 };
@@ -82,6 +85,7 @@ struct NetsDev {
     int32_t* pcell;         // per net-pin: cell or -1 (renamed by a re-sort)
     const int2* poff;       // per net-pin: offset
     const int2* pxy0;       // per net-pin: position as init left it
+    const int2* ppos;       // per net-pin: the current position (CellsDev::ppos)
     const float* weight;    // per net
     int32_t* cpi;           // per net-pin: its cell-pin index, or -1 (renumbered by a re-sort)
     float2* pgrad;          // per cell-pin: the pin's weighted WA gradient, written by the WA kernel

@@ -220,7 +220,14 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
             }
             d = centre_box(d, ns.x, ns.y);
             c.dbox[g] = d;
-            c.ctr[g] = make_int2((d.x + d.z) / 2, (d.y + d.w) / 2);
+            const int2 ce = make_int2((d.x + d.z) / 2, (d.y + d.w) / 2);
+            c.ctr[g] = ce;
+            // the cell's pins move with it (updatePinLocation, nesterov.rs:2268-2276), stored in
+            // net order for the WA kernels
+            for (int q = c.pin_start[g]; q < c.pin_start[g + 1]; ++q) {
+                const int2 o = c.cp_off[q];
+                c.ppos[c.cp_npin[q]] = make_int2(ce.x + o.x, ce.y + o.y);
+            }
             int x0, x1, y0, y1;
             bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
             bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
@@ -540,17 +547,7 @@ __global__ void k_transpose(const float* in, float* out, int R, int C) {
 
 // A net pin's position: its cell's density centre plus its offset once the cell has been
 // centred, else where init left it (nesterov.rs:2150).
-__device__ __forceinline__ int2 pin_pos(const NetsDev& n, const CellsDev& c, int k) {
-    const int cell = n.pcell[k];
-    if (cell >= 0) {
-        const int2 m = c.ctr[cell];
-        if (m.x != INT_MIN) {
-            const int2 o = n.poff[k];
-            return make_int2(m.x + o.x, m.y + o.y);
-        }
-    }
-    return n.pxy0[k];
-}
+__device__ __forceinline__ int2 pin_pos(const NetsDev& n, const CellsDev& c, int k) { return n.ppos[k]; }
 
 // The four shifted exponentials of one pin (nesterov.rs:2354-2385) added into the net sums.
 __device__ __forceinline__ void wa_pin(int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float4& a, float4& b) {
@@ -1414,6 +1411,7 @@ __global__ void k_resort_pins(int N, const int32_t* perm, CellsDev in, CellsDev 
         for (int q = 0; q < e - s; ++q) {
             const int np = in.cp_npin[s + q];
             out.cp_npin[d + q] = np;
+            out.cp_off[d + q] = in.cp_off[s + q];
             cpi[np] = d + q;
         }
     }

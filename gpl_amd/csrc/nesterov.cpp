@@ -150,9 +150,13 @@ struct gpl_ctx {
         return (T*)p;
     }
     template <typename T>
+    // a device copy of n host values; waits, so the source may go out of scope on return
     T* upload(const T* h, size_t n) {
-        T* d = dalloc<T>(n);
-        if (n) check(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, stream), "upload");
+        T* d = dalloc<T>(std::max<size_t>(n, 1));
+        if (n) {
+            check(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, stream), "upload");
+            check(hipStreamSynchronize(stream), "sync");
+        }
         return d;
     }
     ~gpl_ctx() {
@@ -1135,6 +1139,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         std::vector<uint8_t> flags(N);
         std::vector<int32_t> pin_start(size_t(N) + 1, 0), cell_inst(N);
         std::vector<int32_t> cp_npin(p->cell_pin_start[N]);
+        std::vector<int2> cp_off(p->cell_pin_start[N]);
         std::vector<int32_t> cpi(pcell.size(), -1);
         for (int32_t i = 0; i < N; ++i) {
             int32_t g = c->cell_old_of[i];
@@ -1149,6 +1154,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             for (int32_t k = p->cell_pin_start[g]; k < p->cell_pin_start[g + 1]; ++k, ++at) {
                 int32_t q = p->cell_pins[k];
                 cp_npin[at] = npin_of[q];
+                cp_off[at] = make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]);
                 cpi[npin_of[q]] = at;
             }
             pin_start[i + 1] = at;
@@ -1165,6 +1171,10 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
 
         c->cells.pin_start = c->upload(pin_start.data(), size_t(N) + 1);
         c->cells.cp_npin = c->upload(cp_npin.data(), cp_npin.size());
+        c->cells.cp_off = c->upload(cp_off.data(), cp_off.size());
+        // pin positions in net order, the initial ones until a scatter centres their cells
+        c->cells.ppos = c->upload(pxy0.data(), pxy0.size());
+        c->nets.ppos = c->cells.ppos;
         c->nets.cpi = c->upload(cpi.data(), cpi.size());
         c->nets.pgrad = c->dalloc<float2>(std::max<size_t>(cp_npin.size(), 1));
         c->cells.pre = c->upload(pre.data(), N);
@@ -1181,6 +1191,8 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             a.flags = c->dalloc<uint8_t>(N);
             a.pin_start = c->dalloc<int32_t>(size_t(N) + 1);
             a.cp_npin = c->dalloc<int32_t>(cp_npin.size());
+            a.cp_off = c->dalloc<int2>(cp_off.size());
+            a.ppos = c->cells.ppos;
             a.pre = c->dalloc<float2>(N);
             a.old_of = c->dalloc<int32_t>(N);
             c->rb.keys_in = c->dalloc<uint32_t>(N);
