@@ -275,6 +275,11 @@ class NesterovGPU:
     def set_profiling(self, on):
         _ok(_ffi.lib().gpl_set_profiling(self._h, 1 if on else 0), "gpl_set_profiling")
 
+    def set_launch_mode(self, mode):
+        """0: one HIP graph per back-tracking trial (default); 1: kernels one by one on the side
+        streams; 2: one by one on one stream.  The results do not depend on it."""
+        _ok(_ffi.lib().gpl_set_launch_mode(self._h, mode), "gpl_set_launch_mode", self._h)
+
     def kernel_times(self):
         arr = (KernelTime * 16)()
         n = _ffi.lib().gpl_kernel_times(self._h, arr, 16)

@@ -147,6 +147,7 @@ SIGNATURES = [
     ("gpl_util_union_area", i64, [P(i32), i32]),
     ("gpl_util_round_down_pow2", u32, [u32]),
     ("gpl_util_struct_sizes", i32, [P(i64), i32]),
+    ("gpl_set_launch_mode", i32, [C.c_void_p, i32]),
 ]
 
 _lib = None

@@ -27,6 +27,7 @@ struct RegionDev {
     double size_x, size_y;
     float target_density;
     int32_t cell_begin, cell_end;
+    int32_t narrow;  // 1: the scatter's LDS tables may hold 32-bit sums (kPlaceChunk x the largest bin area < 2^32)
     // bins
     const int32_t* bin_x;  // cnt_x + 1
     const int32_t* bin_y;  // cnt_y + 1
@@ -183,6 +184,28 @@ void launch_resort_cells(hipStream_t s, int n_cells, const CellsDev& in, const C
 void launch_gather_f2(hipStream_t s, int n, int begin, const int32_t* perm, const float2* in, float2* out);
 // net pins' cells renamed: pcell[k] = inv[pcell[k]]
 void launch_remap_pcell(hipStream_t s, int n_pins, int32_t* pcell, const int32_t* inv);
+
+void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v);
+
+// ---- a back-tracking trial as a HIP graph: the kernel nodes whose arguments change between
+// trials (the update scatter and the gradient per region, the two WA kernels), found by kernel
+// and region in a captured graph and given new arguments before each launch -----------------
+enum TrialKind { TK_SCATTER = 0, TK_WA_SMALL = 1, TK_WA_BIG = 2, TK_GRADIENT = 3 };
+struct TrialNode {
+    hipGraphNode_t node;
+    hipKernelNodeParams p;  // as captured (function, grid, block, LDS)
+    int kind;
+    int cell_begin;         // the node's region (scatter, gradient), by its first cell
+};
+int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap);  // count, or -1
+hipError_t set_scatter_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const float2* cur_slp,
+                            const float2* cur_sum, const float2* cur, float2* next, float2* next_slp, float step, float coeff,
+                            unsigned long long* global_adds);
+hipError_t set_wa_node(hipGraphExec_t e, const TrialNode& n, const NetsDev& nd, const CellsDev& c, float cx, float cy,
+                       unsigned long long* hpwl);
+hipError_t set_gradient_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const NetsDev& nd,
+                             float cx, float cy, float penalty, float2* wl, float2* den, float2* sum, const float2* a_slp,
+                             const float2* b_slp, const float2* a_sum, double* partials, int nan_inject);
 
 int block_partials_bins(const RegionDev& r);
 int block_partials_cells(const RegionDev& r);

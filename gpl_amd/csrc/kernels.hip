@@ -10,6 +10,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <vector>
 
 #include <hipcub/hipcub.hpp>
 
@@ -141,7 +142,8 @@ struct PlaceArgs {
 };
 
 // Adds v into the LDS table's slot for bin k; false when no slot was found.
-__device__ __forceinline__ bool table_add(unsigned* keys, unsigned long long* vals, unsigned k, unsigned long long v) {
+template <typename V>
+__device__ __forceinline__ bool table_add(unsigned* keys, V* vals, unsigned k, V v) {
     unsigned slot = (k * 2654435761u) >> 21;  // 11 bits: kSlots = 2048
 #pragma unroll 1
     for (int probe = 0; probe < kProbe; ++probe) {
@@ -159,11 +161,14 @@ __device__ __forceinline__ bool table_add(unsigned* keys, unsigned long long* va
     return false;
 }
 
-template <int MODE>
+// V: the LDS tables' sum type.  32 bits when a chunk's sum in one bin cannot reach 2^32 (every
+// overlap, times its scale and target, is at most the bin's area, so kChunk * the largest bin area
+// bounds it; the host checks that), which halves the tables and doubles the blocks per CU; else 64.
+template <int MODE, typename V>
 __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev c, PlaceArgs pa) {
     __shared__ unsigned t_key[kSlots];
-    __shared__ unsigned long long t_inst[kSlots];
-    __shared__ unsigned long long t_fill[kSlots];
+    __shared__ V t_inst[kSlots];
+    __shared__ V t_fill[kSlots];
     __shared__ unsigned s_adds;
     const int n = r.cell_end - r.cell_begin;
     const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
@@ -176,8 +181,8 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
         const int base = ch * kChunk;
         for (int i = threadIdx.x; i < kSlots; i += kBlock) {
             t_key[i] = kEmpty;
-            t_inst[i] = 0ull;
-            t_fill[i] = 0ull;
+            t_inst[i] = 0;
+            t_fill[i] = 0;
         }
         __syncthreads();
         unsigned adds = 0;
@@ -233,7 +238,7 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
             bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
             const float sc = c.dscale[g];
             const bool inst = f & 1u, macro = inst && (f & 2u);
-            unsigned long long* vals = inst ? t_inst : t_fill;
+            V* vals = inst ? t_inst : t_fill;
             unsigned long long* glob = inst ? r.inst : r.filler;
             for (int y = y0; y < y1; ++y) {
                 const int by0 = r.bin_y[y], by1 = r.bin_y[y + 1];
@@ -243,7 +248,7 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
                     const long long v = (long long)a;
                     if (v == 0) continue;
                     const unsigned kb = (unsigned)y * (unsigned)r.cnt_x + (unsigned)x;
-                    if (!table_add(t_key, vals, kb, (unsigned long long)v)) {
+                    if (!table_add<V>(t_key, vals, kb, (V)v)) {
                         atomicAdd(glob + kb, (unsigned long long)v);
                         ++adds;
                     }
@@ -254,9 +259,9 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
         for (int i = threadIdx.x; i < kSlots; i += kBlock) {
             const unsigned kb = t_key[i];
             if (kb == kEmpty) continue;
-            const unsigned long long a = t_inst[i], b = t_fill[i];
-            if (a) atomicAdd(r.inst + kb, a);
-            if (b) atomicAdd(r.filler + kb, b);
+            const V a = t_inst[i], b = t_fill[i];
+            if (a) atomicAdd(r.inst + kb, (unsigned long long)a);
+            if (b) atomicAdd(r.filler + kb, (unsigned long long)b);
             adds += (a != 0) + (b != 0);
         }
         if (adds) atomicAdd(&s_adds, adds);
@@ -1121,6 +1126,8 @@ __global__ void k_sum_partials(const double* partials, int n, int stride, int nq
     }
 }
 
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
 static int grid_for(size_t n) {
     size_t b = (n + kBlock - 1) / kBlock;
     if (b < 1) b = 1;
@@ -1141,12 +1148,18 @@ static void place_scatter(hipStream_t s, int mode, const RegionDev& r, const Cel
     if (n <= 0) return;
     int chunks = (n + kChunk - 1) / kChunk;
     int grid = round_xcds(chunks < 4096 ? chunks : 4096);
-    if (mode == PLACE_INIT)
-        hipLaunchKernelGGL(k_place_scatter<PLACE_INIT>, dim3(grid), dim3(kBlock), 0, s, r, c, pa);
-    else if (mode == PLACE_CENTER)
-        hipLaunchKernelGGL(k_place_scatter<PLACE_CENTER>, dim3(grid), dim3(kBlock), 0, s, r, c, pa);
-    else
-        hipLaunchKernelGGL(k_place_scatter<PLACE_UPDATE>, dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+    using N = unsigned;
+    using W = unsigned long long;
+    if (mode == PLACE_INIT) {
+        if (r.narrow) hipLaunchKernelGGL((k_place_scatter<PLACE_INIT, N>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+        else hipLaunchKernelGGL((k_place_scatter<PLACE_INIT, W>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+    } else if (mode == PLACE_CENTER) {
+        if (r.narrow) hipLaunchKernelGGL((k_place_scatter<PLACE_CENTER, N>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+        else hipLaunchKernelGGL((k_place_scatter<PLACE_CENTER, W>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+    } else {
+        if (r.narrow) hipLaunchKernelGGL((k_place_scatter<PLACE_UPDATE, N>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+        else hipLaunchKernelGGL((k_place_scatter<PLACE_UPDATE, W>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+    }
 }
 
 void launch_center_init(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* cur_slp, float2* prev_slp, float2* cur,
@@ -1458,6 +1471,94 @@ void launch_gather_f2(hipStream_t s, int n, int begin, const int32_t* perm, cons
 void launch_remap_pcell(hipStream_t s, int P, int32_t* pcell, const int32_t* inv) {
     if (P <= 0) return;
     hipLaunchKernelGGL(k_remap_pcell, dim3(grid_for(P)), dim3(kBlock), 0, s, P, pcell, inv);
+}
+
+void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v) {
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, p, v);
+}
+
+// ---- the trial graph's argument-dependent kernel nodes ---------------------------------------
+int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap) {
+    size_t n = 0;
+    if (hipGraphGetNodes(g, nullptr, &n) != hipSuccess) return -1;
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n && hipGraphGetNodes(g, nodes.data(), &n) != hipSuccess) return -1;
+    const void* f_scatter = reinterpret_cast<const void*>(k_place_scatter<PLACE_UPDATE, unsigned>);
+    const void* f_scatter_w = reinterpret_cast<const void*>(k_place_scatter<PLACE_UPDATE, unsigned long long>);
+    const void* f_small = reinterpret_cast<const void*>(k_wa_small<true>);
+    const void* f_big = reinterpret_cast<const void*>(k_wa_big<true>);
+    const void* f_grad = reinterpret_cast<const void*>(k_gradient);
+    int m = 0;
+    for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(nd, &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams p{};
+        if (hipGraphKernelNodeGetParams(nd, &p) != hipSuccess) return -1;
+        int kind = (p.func == f_scatter || p.func == f_scatter_w) ? TK_SCATTER
+                   : p.func == f_small                              ? TK_WA_SMALL
+                   : p.func == f_big                                ? TK_WA_BIG
+                   : p.func == f_grad                               ? TK_GRADIENT
+                                                                    : -1;
+        if (kind < 0) continue;
+        int begin = -1;
+        if (kind == TK_SCATTER || kind == TK_GRADIENT) {
+            if (!p.kernelParams || !p.kernelParams[0]) return -1;
+            begin = static_cast<const RegionDev*>(p.kernelParams[0])->cell_begin;  // the region of the node
+        }
+        if (m >= cap) return -1;
+        out[m].node = nd;
+        out[m].p = p;
+        out[m].kind = kind;
+        out[m].cell_begin = begin;
+        ++m;
+    }
+    return m;
+}
+
+hipError_t set_scatter_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const float2* cur_slp,
+                            const float2* cur_sum, const float2* cur, float2* next, float2* next_slp, float step, float coeff,
+                            unsigned long long* global_adds) {
+    PlaceArgs pa{};
+    pa.cur_slp = cur_slp;
+    pa.cur_sum = cur_sum;
+    pa.cur = cur;
+    pa.next = next;
+    pa.next_slp = next_slp;
+    pa.s = step;
+    pa.coeff = coeff;
+    pa.global_adds = global_adds;
+    RegionDev rr = r;
+    CellsDev cc = c;
+    void* args[] = {&rr, &cc, &pa};
+    hipKernelNodeParams p = n.p;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    return hipGraphExecKernelNodeSetParams(e, n.node, &p);
+}
+
+hipError_t set_wa_node(hipGraphExec_t e, const TrialNode& n, const NetsDev& nd, const CellsDev& c, float cx, float cy,
+                       unsigned long long* hpwl) {
+    NetsDev nn = nd;
+    CellsDev cc = c;
+    int write_rec = 0;
+    void* args[] = {&nn, &cc, &cx, &cy, &hpwl, &write_rec};
+    hipKernelNodeParams p = n.p;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    return hipGraphExecKernelNodeSetParams(e, n.node, &p);
+}
+
+hipError_t set_gradient_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const NetsDev& nd,
+                             float cx, float cy, float penalty, float2* wl, float2* den, float2* sum, const float2* a_slp,
+                             const float2* b_slp, const float2* a_sum, double* partials, int nan_inject) {
+    RegionDev rr = r;
+    CellsDev cc = c;
+    NetsDev nn = nd;
+    void* args[] = {&rr, &cc, &nn, &cx, &cy, &penalty, &wl, &den, &sum, &a_slp, &b_slp, &a_sum, &partials, &nan_inject};
+    hipKernelNodeParams p = n.p;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    return hipGraphExecKernelNodeSetParams(e, n.node, &p);
 }
 
 }  // namespace gplk

@@ -73,7 +73,14 @@ struct gpl_ctx {
     double* obj_out = nullptr;
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    bool overlap = true, join_pending = false;
+    bool overlap = true, join_pending = false;  // overlap: launch_mode < 2
+    // Launch mode: 0 a back-tracking trial as one HIP graph (side streams inside), 1 eager launches
+    // on the side streams, 2 eager launches on one stream.  Profiling runs as 2.
+    int launch_mode = 0;
+    hipGraph_t tgraph = nullptr;
+    hipGraphExec_t texec = nullptr;
+    uint64_t tgraph_key = ~0ull;
+    std::vector<gplk::TrialNode> tnodes;
     // a third stream for the high-fanout nets' WA kernel, beside the small nets' one
     hipStream_t aux2 = nullptr;
     hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
@@ -167,6 +174,8 @@ struct gpl_ctx {
             for (int i = 0; i < 64; ++i) hipEventDestroy(ev[i]);
         if (aux) hipStreamSynchronize(aux);
         if (aux2) hipStreamSynchronize(aux2);
+        if (texec) hipGraphExecDestroy(texec);
+        if (tgraph) hipGraphDestroy(tgraph);
         if (ev_fork) hipEventDestroy(ev_fork);
         if (ev_join) hipEventDestroy(ev_join);
         if (ev_fork2) hipEventDestroy(ev_fork2);
@@ -211,9 +220,10 @@ struct gpl_ctx {
         return r.R.std_insts_area + (int64_t)std::round(float(r.R.macro_insts_area) * r.R.target_density);
     }
 
+    // a kernel, not a memset, so that a captured trial graph holds no memset node
     void zero_hpwl() {
         join2();  // no high-fanout WA kernel may still be adding into the counter
-        check(hipMemsetAsync(d_hpwl, 0, sizeof(unsigned long long), stream), "memset");
+        gplk::launch_set_u64(stream, d_hpwl, 0ull);
     }
 
     // getHpwl (nesterov.rs:2297), read back.
@@ -282,11 +292,20 @@ struct gpl_ctx {
     // (nesterov.rs:2512-2528).  Regions from `first` are (re)evaluated in order.
     void np_update_gradients(int first, int last, int slot, bool trial) {
         join();
+        launch_gradients(first, last, slot, trial);
+        sync_stats();
+        gradients_read(first, last, slot, trial);
+    }
+
+    void launch_gradients(int first, int last, int slot, bool trial) {
         for (int b = first; b < last; ++b) {
             if (rg[b].is_converged) continue;
             launch_gradient(b, slot, trial, slot == SLOT_NEXT && inject_now());
         }
-        sync_stats();
+    }
+
+    // the host side of np_update_gradients once the statistics are back
+    void gradients_read(int first, int last, int slot, bool trial) {
         for (int b = first; b < last; ++b) {
             Region& r = rg[b];
             if (!r.is_converged) {
@@ -487,15 +506,117 @@ struct gpl_ctx {
     }
 
     // doBackTracking (nesterov.rs:2776).
+    // ---- one back-tracking trial as a HIP graph ----------------------------------------------
+    // The trial (scatter; bins and solve on the second stream beside the WA kernels; gradients,
+    // finishing reductions, the statistics copy) is captured once per set of live regions; before
+    // each launch the nodes whose arguments change (step, momentum, coefficients, penalty, the
+    // rotated slots, the cell arrays a re-sort swapped) get their new arguments.  One launch and
+    // one wait per trial instead of ~20 API calls.
+    void drop_graph() {
+        if (texec) hipGraphExecDestroy(texec);
+        if (tgraph) hipGraphDestroy(tgraph);
+        texec = nullptr;
+        tgraph = nullptr;
+        tnodes.clear();
+        tgraph_key = ~0ull;
+    }
+
+    uint64_t live_regions_key() const {
+        uint64_t k = 0;
+        for (size_t b = 0; b < rg.size(); ++b)
+            if (!rg[b].is_converged) k |= 1ull << b;
+        return k;
+    }
+
+    bool build_graph() {
+        drop_graph();
+        check(hipStreamSynchronize(stream), "sync");
+        check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+        hipError_t err = hipSuccess;
+        try {
+            update_coordinates(0.0f);
+            field();
+            wa_forward();
+            join();
+            launch_gradients(0, int(rg.size()), SLOT_NEXT, true);
+            check(hipMemcpyAsync(h_sums, d_sums, sizeof(gplk::Sums) * (rg.size() + 1), hipMemcpyDeviceToHost, stream), "stats copy");
+        } catch (const HipError& e) {
+            err = e.e;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(stream, &g);
+        join_pending = join2_pending = false;
+        if (err != hipSuccess || ec != hipSuccess || !g) {
+            if (g) hipGraphDestroy(g);
+            return false;
+        }
+        tgraph = g;
+        if (hipGraphInstantiate(&texec, tgraph, nullptr, nullptr, 0) != hipSuccess) {
+            drop_graph();
+            return false;
+        }
+        tnodes.resize(8 + 2 * rg.size());
+        int m = gplk::find_trial_nodes(tgraph, tnodes.data(), int(tnodes.size()));
+        int live = 0;
+        for (auto& r : rg) live += r.is_converged ? 0 : 1;
+        if (m != 2 * live + (nets.num_small > 0 ? 1 : 0) + (nets.num_small < nnets ? 1 : 0)) {
+            drop_graph();
+            return false;
+        }
+        tnodes.resize(m);
+        tgraph_key = live_regions_key();
+        return true;
+    }
+
+    Region* region_of(int cell_begin) {
+        for (auto& r : rg)
+            if (r.R.cell_begin == cell_begin) return &r;
+        return nullptr;
+    }
+
+    // one trial through the graph; false when graphs are off (the caller launches eagerly)
+    bool trial_by_graph(float coeff) {
+        if (launch_mode != 0 || profiling) return false;
+        if (!texec || tgraph_key != live_regions_key()) {
+            if (!build_graph()) {
+                launch_mode = 1;
+                return false;
+            }
+        }
+        const int nan = inject_now() ? 1 : 0;
+        for (auto& n : tnodes) {
+            hipError_t e = hipSuccess;
+            if (n.kind == gplk::TK_SCATTER) {
+                Region& r = *region_of(n.cell_begin);
+                e = gplk::set_scatter_node(texec, n, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT],
+                                           r.v[S_NEXT_SLP], r.step_length, coeff, d_global_adds);
+            } else if (n.kind == gplk::TK_GRADIENT) {
+                Region& r = *region_of(n.cell_begin);
+                e = gplk::set_gradient_node(texec, n, r.dev, cells, nets, wl_coef_x, wl_coef_y, r.density_penalty, r.v[S_NEXT_SLP + 1],
+                                            r.v[S_NEXT_SLP + 2], r.v[S_NEXT_SLP + 3], r.v[S_CUR_SLP], r.v[S_NEXT_SLP],
+                                            r.v[S_CUR_SLP_SUM], r.partials, nan);
+            } else {
+                e = gplk::set_wa_node(texec, n, nets, cells, wl_coef_x, wl_coef_y, d_hpwl);
+            }
+            check(e, "graph node");
+        }
+        check(hipGraphLaunch(texec, stream), "graph launch");
+        check(hipStreamSynchronize(stream), "sync");
+        gradients_read(0, int(rg.size()), SLOT_NEXT, true);
+        return true;
+    }
+
     void do_back_tracking(float coeff) {
         trials = 0;
         for (int nbt = 0; nbt < 10; ++nbt) {
             ++trials;
-            update_coordinates(coeff);
-            field();
-            wa_forward();
             num_region_diverged = 0;
-            np_update_gradients(0, int(rg.size()), SLOT_NEXT, true);
+            if (!trial_by_graph(coeff)) {
+                update_coordinates(coeff);
+                field();
+                wa_forward();
+                np_update_gradients(0, int(rg.size()), SLOT_NEXT, true);
+            }
             for (size_t b = 0; b < rg.size(); ++b) {
                 Region& r = rg[b];
                 if (!r.is_converged) {
@@ -1227,6 +1348,13 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             d.target_density = r.R.target_density;
             d.cell_begin = r.R.cell_begin;
             d.cell_end = r.R.cell_end;
+            {
+                // a chunk's sum in one bin is at most kPlaceChunk x the bin's area
+                int64_t wmax = 0, hmax = 0;
+                for (int32_t x = 0; x < r.R.cnt_x; ++x) wmax = std::max<int64_t>(wmax, int64_t(r.R.bin_x[x + 1]) - r.R.bin_x[x]);
+                for (int32_t y = 0; y < r.R.cnt_y; ++y) hmax = std::max<int64_t>(hmax, int64_t(r.R.bin_y[y + 1]) - r.R.bin_y[y]);
+                d.narrow = (double)gplk::kPlaceChunk * (double)wmax * (double)hmax < 4294967296.0 ? 1 : 0;
+            }
             size_t M = size_t(r.R.cnt_x) * r.R.cnt_y;
             d.bin_x = c->upload(r.R.bin_x, size_t(r.R.cnt_x) + 1);
             d.bin_y = c->upload(r.R.bin_y, size_t(r.R.cnt_y) + 1);
@@ -1637,6 +1765,20 @@ int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter) {
     if (!c || kind < 0 || kind > 3) return GPL_ERR_ARG;
     c->inject_kind = kind;
     c->inject_after = kind == 0 ? -1 : after_iter;
+    return GPL_OK;
+}
+
+int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode) {
+    if (!c || mode < 0 || mode > 2) return GPL_ERR_ARG;
+    try {
+        c->join();
+        check(hipStreamSynchronize(c->stream), "sync");
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    c->drop_graph();
+    c->launch_mode = mode;
+    c->overlap = mode < 2;
     return GPL_OK;
 }
 

@@ -64,6 +64,31 @@ def test_scatter_hand_computed_on_the_gpu():
     g.close()
 
 
+@pytest.mark.parametrize("bin_size", [1000, 100_000])
+def test_scatter_tables_of_both_widths_match_the_oracle(bin_size):
+    """3,000 random cells (several chunks, overlapping, fillers and macros) on 8 x 8 bins: bins of
+    1e3 DBU take the 32-bit LDS sums, bins of 1e5 DBU (sums up to ~1e13) the 64-bit ones."""
+    rng = np.random.default_rng(bin_size)
+    n, side = 3000, 8 * bin_size
+    cells = []
+    for i in range(n):
+        w, h = (rng.uniform(0.2, 2.0, 2) * bin_size).astype(int) + 1
+        flags = 1 | (2 if i % 97 == 0 else 0) if i % 5 else 0
+        cells.append(dict(box=(0, 0, int(w), int(h)), scale=float(rng.uniform(0.3, 1.0)), flags=flags))
+    t = Tiny(cells, [], regions=[dict(lx=0, ly=0, ux=side, uy=side, cnt_x=8, cnt_y=8, target_density=0.9)])
+    xy = rng.uniform(0.1 * side, 0.9 * side, (n, 2)).astype(np.float32)
+    g = _ctx(t)
+    g.set_coordinates(0, G.GPL_SLOT_CUR_SLP, xy)
+    inst, fill, dens, _ = g.density_scatter(0, G.GPL_SLOT_CUR_SLP)
+    g.close()
+    o = O.Oracle(t.problem)
+    oi, of_, od, _, _ = o.scatter(0, xy)
+    o.close()
+    assert np.array_equal(inst, oi) and np.array_equal(fill, of_) and np.array_equal(dens, od)
+    if bin_size == 100_000:
+        assert inst.max() > 2 ** 32  # the sums need 64 bits
+
+
 def _ctx(tiny):
     """A GPU context on a hand-built problem (no design object)."""
     import ctypes as C
@@ -224,6 +249,28 @@ def test_loop_is_deterministic(c1):
         d.close()
     assert runs[0][0] == runs[1][0]
     assert np.array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c2"])
+def test_launch_modes_give_the_same_placement(cfg):
+    """A trial as one HIP graph (replayed every trial, with the loop's eager work -- re-sorts,
+    snapshots, statistics -- between replays), as kernels on the side streams, and as kernels on
+    one stream: the same iterations, rows and placement."""
+    d = G.Design.from_config(cfg)
+    d.initial_place()
+    d.init_nesterov_place()
+    out = []
+    for mode in (0, 1, 2):
+        g = G.NesterovGPU(d)
+        g.set_launch_mode(mode)
+        it = g.do_nesterov_place()
+        out.append((it, g.history(), g.placed_locations(), g.stats().hpwl))
+        g.close()
+    d.close()
+    for o in out[1:]:
+        assert o[0] == out[0][0] and o[3] == out[0][3]
+        assert o[1] == out[0][1]
+        assert np.array_equal(o[2], out[0][2])
 
 
 @pytest.mark.parametrize("kind,code", [(1, 306), (2, 305)])
