@@ -35,6 +35,8 @@ WORKLOADS = {
     "c1": "2k-cell synthetic placement: 2,000 cells, 2,200 nets, 96 IO ports, 64x64 bins",
     "c2": "200k-cell synthetic placement: 200,000 cells, 210,000 nets (+20 of 500-2,000 pins), 512x512 bins",
     "c3": "1M-cell synthetic placement: 1,000,000 cells, 1,050,000 nets (+50 of 1k-5k pins), 5% fixed macros, 1024x1024 bins",
+    "c4s": "50k-cell synthetic placement: 50,000 cells, 52,000 nets, 256x256 bins",
+    "c4r": "8 region groups (5k-150k cells, 485k) + 20k top-level cells, per-region bin grids",
 }
 
 
@@ -175,6 +177,48 @@ def summarize(paths):
         print()
 
 
+def batched(cfg, k):
+    """K independent placements (seeds 1..K: their own perturbations) of one design: run one
+    after another, then at once from K host threads (one context each); one JSON line."""
+    import gpl_amd as G
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    t = time.perf_counter()
+    d = G.Design.from_config(cfg)
+    d.initial_place()
+
+    def contexts():
+        out = []
+        for seed in range(1, k + 1):
+            d.init_nesterov_place(G.default_opts(**dict(G.CONFIGS[cfg]["opts"], seed=seed)))
+            out.append(G.NesterovGPU(d))
+        return out
+
+    a = contexts()
+    p = d.problem()
+    log(f"[bench] {cfg} x {k} seeds: cells {p.num_cells} nets {p.num_nets}; setup {time.perf_counter() - t:.1f}s")
+    t0 = time.perf_counter()
+    it_seq = [g.do_nesterov_place() for g in a]
+    t_seq = time.perf_counter() - t0
+    b = contexts()
+    t0 = time.perf_counter()
+    it_par = G.place_concurrently(b)
+    t_par = time.perf_counter() - t0
+    same = it_seq == it_par and all(x.stats().hpwl == y.stats().hpwl for x, y in zip(a, b))
+    for g in a + b:
+        g.close()
+    d.close()
+    log(f"[bench] sequential {t_seq:.3f}s, concurrent {t_par:.3f}s, same results: {same}")
+    print(json.dumps({
+        "metric": f"placements/s, {k} seeds of one design, each to overflow 0.10, concurrent contexts",
+        "value": round(k / t_par, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
+        "sequential_value": round(k / t_seq, 3), "speedup_vs_sequential": round(t_seq / t_par, 2),
+        "iterations": it_par, "same_results_as_sequential": bool(same), "dtype": "f32", "data": "synthetic",
+        "config": {"workload": WORKLOADS.get(cfg, cfg), "name": cfg, "seeds": k}}), flush=True)
+
+
 def cpu_info():
     model = platform.processor() or ""
     try:
@@ -203,6 +247,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a plain device copy of this many MiB three times and exit (the HBM counter "
                          "calibration under rocprofv3 --pmc)")
+    ap.add_argument("--seeds", type=int, default=0,
+                    help="the batched form instead of the headline: this many seeds of --config (c4s: the "
+                         "50k-cell design) placed one after another, then all at once from concurrent contexts")
     ap.add_argument("--summarize", nargs="+", metavar="CSV",
                     help="print a text summary of rocprofv3 CSV output (kernel_stats.csv or counter_collection.csv) and exit")
     ap.add_argument("--traffic-table", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
@@ -218,6 +265,10 @@ def main():
         return
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
+        return
+
+    if args.seeds:
+        batched(args.config, args.seeds)
         return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

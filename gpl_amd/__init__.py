@@ -66,6 +66,10 @@ def default_opts(**kw):
 #       65 % utilisation, target density 0.70; 512 x 512 bins.
 #   c3: 1M cells, 1.05M nets capped at 128, 50 nets of 1k-5k pins, 5 % of the core in fixed
 #       macros of 50-400 rows; 1024 x 1024 bins.
+#   c4r: 8 region groups of 5k-150k cells (485k) plus 20k top-level cells (a region needs
+#       placeable cells), each region with the bin grid its size derives (32^2 .. 512^2).
+#   c4s: the 50k-cell, 52k-net design placed for many seeds (256 x 256 bins).
+#   c4r_small: c4r at 1/30 scale for the oracle comparison.
 CONFIGS = {
     "c1": dict(gen=dict(num_cells=2000, num_nets=2200, degree_p=0.55, degree_cap=32, num_io=96, utilization=0.70,
                         core_rows=36, core_sites=357),
@@ -77,6 +81,14 @@ CONFIGS = {
                         hf_min_pins=1000, hf_max_pins=5000, num_io=1024, utilization=0.65, macro_area_frac=0.05,
                         macro_rows_min=50, macro_rows_max=400),
                opts=dict(bin_cnt=1024, density=0.70, initial_place_max_iter=5)),
+    "c4r": dict(gen=dict(num_cells=20_000, num_nets=530_000, degree_p=0.5, degree_cap=64, num_io=256, utilization=0.65,
+                         group_cells=[5_000, 10_000, 20_000, 40_000, 60_000, 80_000, 120_000, 150_000]),
+                opts=dict(bin_cnt=0, density=0.70, initial_place_max_iter=5)),
+    "c4s": dict(gen=dict(num_cells=50_000, num_nets=52_000, degree_p=0.5, degree_cap=64, num_io=128, utilization=0.65),
+                opts=dict(bin_cnt=256, density=0.70, initial_place_max_iter=5)),
+    "c4r_small": dict(gen=dict(num_cells=1_000, num_nets=18_000, degree_p=0.5, degree_cap=64, num_io=96, utilization=0.65,
+                               group_cells=[250, 500, 1_000, 1_500, 2_000, 3_000, 4_000, 5_000]),
+                      opts=dict(bin_cnt=0, density=0.70, initial_place_max_iter=5)),
 }
 
 
@@ -192,6 +204,24 @@ class Design:
 
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def place_concurrently(contexts, max_workers=8, launch_mode=2):
+    """do_nesterov_place on each of several independent contexts from a pool of host threads (the
+    library releases the interpreter lock in every call), so that the GPU interleaves the
+    placements' kernels; returns the iteration counts in order.  The results are each context's
+    own (the same as running them one after another).  Each context launches on one stream
+    (launch_mode 2): many placements' graphs and side streams would contend for the device's few
+    hardware queues, one stream each interleaves best (16 seeds of the 50k-cell design: 3.3x the
+    sequential graph form with 8 threads, 1.8x with graphs and 16)."""
+    from concurrent.futures import ThreadPoolExecutor
+    if not contexts:
+        return []
+    if launch_mode is not None:
+        for g in contexts:
+            g.set_launch_mode(launch_mode)
+    with ThreadPoolExecutor(max_workers=min(max_workers or len(contexts), len(contexts))) as ex:
+        return list(ex.map(lambda g: g.do_nesterov_place(), contexts))
 
 
 class NesterovGPU:

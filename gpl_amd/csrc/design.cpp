@@ -1488,6 +1488,8 @@ void gpl_design_destroy(gpl_design* d) { delete d; }
 int32_t gpl_design_initial_place(gpl_design* d) {
     if (!d) return GPL_ERR_ARG;
     std::string err;
+    if (d->have_pl_before_init) d->pl = d->pl_before_init;
+    d->have_pl_before_init = false;
     int32_t rc = gpl::initial_place(d, &err);
     d->initial_hpwl = gpl::db_hpwl(d);
     return rc;
@@ -1523,6 +1525,12 @@ int32_t gpl_design_get_info(const gpl_design* d, gpl_design_info* out) {
 int32_t gpl_init_nesterov_place(gpl_design* d, const gpl_opts* o) {
     if (!d) return GPL_ERR_ARG;
     if (o) d->opts = *o;
+    if (d->have_pl_before_init) {
+        d->pl = d->pl_before_init;
+    } else {
+        d->pl_before_init = d->pl;
+        d->have_pl_before_init = true;
+    }
     std::string err;
     int32_t rc = gpl::init_nesterov(d, &err);
     if (rc != GPL_OK) fprintf(stderr, "gpl_init_nesterov_place: %s\n", err.c_str());

@@ -176,6 +176,10 @@ struct gpl_design {
     gpl_opts opts{};
     gpl::Design d;
     gpl::Placer pl;
+    // the placer as it stood before the first init_nesterov (which perturbs it): restored by every
+    // later one, so that re-initialising with another seed starts from the same placement
+    gpl::Placer pl_before_init;
+    bool have_pl_before_init = false;
     gpl::NesterovInit ni;
     gpl::Log log;
     int32_t ip_rounds = 0;

@@ -273,6 +273,59 @@ def test_launch_modes_give_the_same_placement(cfg):
         assert np.array_equal(o[2], out[0][2])
 
 
+def test_region_groups_match_the_oracle():
+    """Nine regions (eight region groups and the top level) with their own bin grids, 8^2 .. 128^2:
+    the GPU loop against the oracle's, and the trial graph (nine scatter and gradient nodes)
+    against launches one by one on one stream."""
+    d = G.Design.from_config("c4r_small")
+    d.initial_place()
+    d.init_nesterov_place()
+    p = d.problem()
+    assert p.num_regions == 9
+    runs = []
+    for mode in (0, 2):
+        g = G.NesterovGPU(d)
+        g.set_launch_mode(mode)
+        it = g.do_nesterov_place()
+        runs.append((it, g.history(), g.stats()))
+        g.close()
+    assert runs[0][0] == runs[1][0] and runs[0][1] == runs[1][1]
+    o = O.Oracle(p)
+    o.init()
+    o.run()
+    so = o.stats()
+    o.close()
+    s = runs[0][2]
+    assert s.converged and so.converged and s.overflow <= 0.1
+    assert abs(s.hpwl - so.hpwl) <= 0.005 * so.hpwl, (s.hpwl, so.hpwl)
+    assert abs(runs[0][0] - so.iter) <= 10, (runs[0][0], so.iter)
+    d.close()
+
+
+def test_independent_seeds_run_concurrently_with_the_same_results():
+    """Six placements of one design (seeds 1..6: their own perturbations) in six contexts run from
+    six host threads at once give each seed the result it gets alone."""
+    d = G.Design.from_config("c1")
+    d.initial_place()
+    sets = []
+    for _ in range(2):
+        ctxs = []
+        for seed in range(1, 7):
+            d.init_nesterov_place(G.default_opts(**dict(G.CONFIGS["c1"]["opts"], seed=seed)))
+            ctxs.append(G.NesterovGPU(d))
+        sets.append(ctxs)
+    seq = [g.do_nesterov_place() for g in sets[0]]
+    par = G.place_concurrently(sets[1])
+    assert seq == par
+    for a, b in zip(sets[0], sets[1]):
+        assert a.history() == b.history()
+        assert np.array_equal(a.placed_locations(), b.placed_locations())
+    assert len({a.stats().hpwl for a in sets[0]}) > 1  # the seeds differ
+    for g in sets[0] + sets[1]:
+        g.close()
+    d.close()
+
+
 @pytest.mark.parametrize("kind,code", [(1, 306), (2, 305)])
 def test_divergence_is_reported(kind, code):
     d = G.Design.from_config("c1")
