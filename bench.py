@@ -42,7 +42,7 @@ WORKLOADS = {
 
 # kernels of each family, the first one launched once per family launch
 FAMILY_KERNELS = {
-    "update_scatter": ["k_place_scatter<2>"],
+    "update_scatter": ["k_place_scatter<2, unsigned int>", "k_place_scatter<2, unsigned long long>"],
     "bin_density": ["k_bin_density"],
     "dct_solve": ["k_rows<0>", "k_rows<1>", "k_rows<2>", "k_rows<3>", "k_transpose"],
     "wa_forward": ["k_wa_small<true>", "k_wa_big<true>"],
@@ -118,11 +118,12 @@ def traffic_table(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib
     print(f"# copy probe {probe}: FETCH_SIZE x{f_scale:.3f}, WRITE_SIZE x{w_scale:.3f} (known {copy_mib} MiB each way)")
     print(f"# {'family':16s} {'launches':>8s} {'fetch_MB':>10s} {'write_MB':>10s}")
     for fam, ks in FAMILY_KERNELS.items():
-        n = fn.get(ks[0], 0)
-        if not n:
+        anchor = next((k for k in ks if fn.get(k, 0)), None)  # the family's kernel launched once per launch
+        if anchor is None:
             continue
+        n = fn[anchor]
         fetch = sum(ft.get(k, 0.0) for k in ks) / n * f_scale * 1024.0
-        write = sum(wt.get(k, 0.0) for k in ks) / max(wn.get(ks[0], n), 1) * w_scale * 1024.0
+        write = sum(wt.get(k, 0.0) for k in ks) / max(wn.get(anchor, n), 1) * w_scale * 1024.0
         print(f"# {fam:16s} {n:8d} {fetch / 1e6:10.2f} {write / 1e6:10.2f}")
         print(f"traffic {fam} {fetch + write:.0f}")
 
