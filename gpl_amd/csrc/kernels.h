@@ -63,9 +63,6 @@ struct CellsDev {
     uint8_t* flags;             // GPL_CELL_* (locked is updated on convergence)
     int32_t* pin_start;         // num_cells + 1
     int32_t* cp_npin;           // per cell-pin: its index in the net-order pin arrays
-    int2* cp_off;               // per cell-pin: the pin's offset from the cell centre
-    int2* ppos;                 // per net-pin (net order, shared by both cell orders): the pin's
-                                // position, written by the scatter with the cell's centre
     float2* pre;                // (pin count, dx*dy) per cell, as float
     int32_t* old_of;            // the problem's index of each device cell
 };
@@ -86,7 +83,6 @@ struct NetsDev {
     int32_t* pcell;         // per net-pin: cell or -1 (renamed by a re-sort)
     const int2* poff;       // per net-pin: offset
     const int2* pxy0;       // per net-pin: position as init left it
-    const int2* ppos;       // per net-pin: the current position (CellsDev::ppos)
     const float* weight;    // per net
     int32_t* cpi;           // per net-pin: its cell-pin index, or -1 (renumbered by a re-sort)
     float2* pgrad;          // per cell-pin: the pin's weighted WA gradient, written by the WA kernel
@@ -119,8 +115,10 @@ void launch_center_init(hipStream_t s, const RegionDev& r, const CellsDev& c, fl
 // centre the region's cells at `xy` and scatter them into the bins.
 void launch_center_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy, unsigned long long* global_adds);
 // nesterovUpdateCoordinates + centring + scatter.
+// (zero, when not null, is cleared first: the trial's HPWL counter)
 void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
-                           const float2* cur, float2* next, float2* next_slp, float step, float coeff, unsigned long long* global_adds);
+                           const float2* cur, float2* next, float2* next_slp, float step, float coeff, unsigned long long* global_adds,
+                           unsigned long long* zero);
 // per bin: density, overflow partials, accumulators cleared.
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials);
 // the Poisson solve of r.density into r.field_x / r.field_y (and r.phi when want_phi).
@@ -200,7 +198,7 @@ struct TrialNode {
 int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap);  // count, or -1
 hipError_t set_scatter_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const float2* cur_slp,
                             const float2* cur_sum, const float2* cur, float2* next, float2* next_slp, float step, float coeff,
-                            unsigned long long* global_adds);
+                            unsigned long long* global_adds, unsigned long long* zero);
 hipError_t set_wa_node(hipGraphExec_t e, const TrialNode& n, const NetsDev& nd, const CellsDev& c, float cx, float cy,
                        unsigned long long* hpwl);
 hipError_t set_gradient_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const NetsDev& nd,

@@ -139,6 +139,7 @@ struct PlaceArgs {
     const float2* xy;       // CENTER
     float s, coeff;
     unsigned long long* global_adds;  // count of the atomic adds sent to HBM
+    unsigned long long* zero;         // cleared by the first thread (the trial's HPWL counter)
 };
 
 // Adds v into the LDS table's slot for bin k; false when no slot was found.
@@ -173,6 +174,7 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
     const int n = r.cell_end - r.cell_begin;
     const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
     if (threadIdx.x == 0) s_adds = 0;
+    if (pa.zero && blockIdx.x == 0 && threadIdx.x == 0) *pa.zero = 0ull;
     const int chunks = (n + kChunk - 1) / kChunk;
     const int per = (chunks + gridDim.x - 1) / gridDim.x;
     const int L = xcd_block(blockIdx.x, gridDim.x);
@@ -225,14 +227,9 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
             }
             d = centre_box(d, ns.x, ns.y);
             c.dbox[g] = d;
-            const int2 ce = make_int2((d.x + d.z) / 2, (d.y + d.w) / 2);
-            c.ctr[g] = ce;
-            // the cell's pins move with it (updatePinLocation, nesterov.rs:2268-2276), stored in
-            // net order for the WA kernels
-            for (int q = c.pin_start[g]; q < c.pin_start[g + 1]; ++q) {
-                const int2 o = c.cp_off[q];
-                c.ppos[c.cp_npin[q]] = make_int2(ce.x + o.x, ce.y + o.y);
-            }
+            // the cell's centre; its pins sit at centre + offset (updatePinLocation,
+            // nesterov.rs:2268-2276), which the WA kernels read from here
+            c.ctr[g] = make_int2((d.x + d.z) / 2, (d.y + d.w) / 2);
             int x0, x1, y0, y1;
             bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
             bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
@@ -268,6 +265,32 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
         __syncthreads();
     }
     if (threadIdx.x == 0 && s_adds && pa.global_adds) atomicAdd(pa.global_adds, (unsigned long long)s_adds);
+}
+
+// The finishing reduction of a kernel's FP64 block partials into *out, in a fixed order (one
+// block): quantities 0-1 over nb_bins blocks (the overflow sums), 2-5 over nb_cells (the gradient
+// and step-length sums); the HPWL counter copied when given.
+__device__ void finish_sums(const double* partials, int nb_bins, int nb_cells, Sums* out, const unsigned long long* hpwl, double* sh) {
+    double v[6];
+    for (int q = 0; q < 6; ++q) {
+        int cnt = q < 2 ? nb_bins : nb_cells;
+        double s = 0.0;
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) s += partials[q * kPartials + i];
+        v[q] = block_sum(s, sh);
+    }
+    if (threadIdx.x == 0) {
+        if (nb_bins > 0) {
+            out->overflow = v[0];
+            out->overflow_unscaled = v[1];
+        }
+        if (nb_cells > 0) {
+            out->wl_abs = v[2];
+            out->den_abs = v[3];
+            out->coord_sq = v[4];
+            out->grad_sq = v[5];
+        }
+        if (hpwl) out->hpwl = *hpwl;
+    }
 }
 
 // updateBinsGCellDensityArea's per-bin part (nesterov.rs:2252-2262): density, the overflow
@@ -552,7 +575,19 @@ __global__ void k_transpose(const float* in, float* out, int R, int C) {
 
 // A net pin's position: its cell's density centre plus its offset once the cell has been
 // centred, else where init left it (nesterov.rs:2150).
-__device__ __forceinline__ int2 pin_pos(const NetsDev& n, const CellsDev& c, int k) { return n.ppos[k]; }
+// A net pin's position: its cell's centre plus its offset, or its initial position while the cell
+// has not been centred (or for a port).
+__device__ __forceinline__ int2 pin_pos(const NetsDev& n, const CellsDev& c, int k) {
+    const int cell = n.pcell[k];
+    if (cell >= 0) {
+        const int2 m = c.ctr[cell];
+        if (m.x != INT_MIN) {
+            const int2 o = n.poff[k];
+            return make_int2(m.x + o.x, m.y + o.y);
+        }
+    }
+    return n.pxy0[k];
+}
 
 // The four shifted exponentials of one pin (nesterov.rs:2354-2385) added into the net sums.
 __device__ __forceinline__ void wa_pin(int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float4& a, float4& b) {
@@ -894,28 +929,37 @@ __global__ void k_initial_prev(RegionDev r, CellsDev c, const float2* cur_slp, c
 
 // The finish: each quantity's block partials summed in index order by one block (a fixed
 // per-thread stride then a fixed tree).
-__global__ void k_finish(const double* partials, int nb_bins, int nb_cells, Sums* out, const unsigned long long* hpwl) {
-    __shared__ double sh[kBlock / 64];
+// One 1024-thread block: the six sums at once (each in a fixed order: thread t adds partials t,
+// t + 1024, ..., then the waves' totals in wave order).
+constexpr int kFinishBlock = 1024;
+__global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials, int nb_bins, int nb_cells, Sums* out,
+                                                         const unsigned long long* hpwl) {
+    __shared__ double sh[6][kFinishBlock / 64];
     double v[6];
+#pragma unroll
     for (int q = 0; q < 6; ++q) {
-        int cnt = q < 2 ? nb_bins : nb_cells;
+        const int cnt = q < 2 ? nb_bins : nb_cells;
         double s = 0.0;
-        for (int i = threadIdx.x; i < cnt; i += blockDim.x) s += partials[q * kPartials + i];
-        v[q] = block_sum(s, sh);
+        for (int i = threadIdx.x; i < cnt; i += kFinishBlock) s += partials[q * kPartials + i];
+        v[q] = wave_sum(s);
     }
-    if (threadIdx.x == 0) {
-        if (nb_bins > 0) {
-            out->overflow = v[0];
-            out->overflow_unscaled = v[1];
-        }
-        if (nb_cells > 0) {
-            out->wl_abs = v[2];
-            out->den_abs = v[3];
-            out->coord_sq = v[4];
-            out->grad_sq = v[5];
-        }
-        if (hpwl) out->hpwl = *hpwl;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sh[q][w] = v[q];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double t = 0.0;
+        for (int i = 0; i < kFinishBlock / 64; ++i) t += sh[threadIdx.x][i];
+        double* o = threadIdx.x == 0 ? &out->overflow
+                    : threadIdx.x == 1 ? &out->overflow_unscaled
+                    : threadIdx.x == 2 ? &out->wl_abs
+                    : threadIdx.x == 3 ? &out->den_abs
+                    : threadIdx.x == 4 ? &out->coord_sq
+                                       : &out->grad_sq;
+        if ((threadIdx.x < 2 && nb_bins > 0) || (threadIdx.x >= 2 && nb_cells > 0)) *o = t;
     }
+    if (threadIdx.x == 0 && hpwl) out->hpwl = *hpwl;
 }
 
 // nesterov.rs:2850-2859: a locked instance cell's next slots take its current values.
@@ -1180,8 +1224,10 @@ void launch_center_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c,
 }
 
 void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
-                           const float2* cur, float2* next, float2* next_slp, float step, float coeff, unsigned long long* global_adds) {
+                           const float2* cur, float2* next, float2* next_slp, float step, float coeff, unsigned long long* global_adds,
+                           unsigned long long* zero) {
     PlaceArgs pa{};
+    pa.zero = zero;
     pa.cur_slp = cur_slp;
     pa.cur_sum = cur_sum;
     pa.cur = cur;
@@ -1322,7 +1368,7 @@ void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const
 }
 
 void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int nparts_cells, Sums* out, const unsigned long long* hpwl) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, s, partials, nparts_bins, nparts_cells, out, hpwl);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishBlock), 0, s, partials, nparts_bins, nparts_cells, out, hpwl);
 }
 
 void launch_distance(hipStream_t s, int n, const float2* a, const float2* b, const float2* c, const float2* d, double* partials) {
@@ -1424,7 +1470,6 @@ __global__ void k_resort_pins(int N, const int32_t* perm, CellsDev in, CellsDev 
         for (int q = 0; q < e - s; ++q) {
             const int np = in.cp_npin[s + q];
             out.cp_npin[d + q] = np;
-            out.cp_off[d + q] = in.cp_off[s + q];
             cpi[np] = d + q;
         }
     }
@@ -1517,8 +1562,9 @@ int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap) {
 
 hipError_t set_scatter_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const float2* cur_slp,
                             const float2* cur_sum, const float2* cur, float2* next, float2* next_slp, float step, float coeff,
-                            unsigned long long* global_adds) {
+                            unsigned long long* global_adds, unsigned long long* zero) {
     PlaceArgs pa{};
+    pa.zero = zero;
     pa.cur_slp = cur_slp;
     pa.cur_sum = cur_sum;
     pa.cur = cur;

@@ -236,10 +236,12 @@ struct gpl_ctx {
         return (int64_t)h;
     }
 
-    void bins_and_finish(Region& r, int b) {
+    // the bin densities and their overflow sums (finished here, or with the gradient's sums when
+    // `finish` is false: a trial reads both only at its end)
+    void bins_and_finish(Region& r, int b, bool finish = true) {
         mark(F_BINS);
         gplk::launch_bin_density(stream, r.dev, r.partials);
-        gplk::launch_finish(stream, r.partials, gplk::block_partials_bins(r.dev), 0, d_sums + b, nullptr);
+        if (finish) gplk::launch_finish(stream, r.partials, gplk::block_partials_bins(r.dev), 0, d_sums + b, nullptr);
         mark_end();
     }
 
@@ -251,9 +253,10 @@ struct gpl_ctx {
 
     // updateWireLengthForceWA with the pins' gradient terms (read by the gradient kernel); the
     // net records only for the kernel-level entry.
-    void wa_forward(bool write_rec = false) {
+    void wa_forward(bool write_rec = false, bool zeroed = false) {
         mark(F_WA);
-        zero_hpwl();
+        if (zeroed) join2();  // (zero_hpwl joins too)
+        else zero_hpwl();
         const bool ovl = overlap && !profiling && nets.num_small < nnets;
         if (ovl) {
             check(hipEventRecord(ev_fork2, stream), "event");
@@ -274,7 +277,8 @@ struct gpl_ctx {
         join2_pending = false;
     }
 
-    // The gradient of region b into `slot`; with `trial`, the distances of the step length too.
+    // The gradient of region b into `slot`; with `trial`, the distances of the step length too, and
+    // the bins' overflow sums finished together with the gradient's (field(false) left them).
     void launch_gradient(int b, int slot, bool trial, int nan_inject = 0) {
         Region& r = rg[b];
         int base = slot == SLOT_PREV ? S_PREV_SLP : slot == SLOT_CUR ? S_CUR_SLP : S_NEXT_SLP;
@@ -282,7 +286,8 @@ struct gpl_ctx {
         gplk::launch_gradient(stream, r.dev, cells, nets, wl_coef_x, wl_coef_y, r.density_penalty, r.v[base + 1], r.v[base + 2],
                               r.v[base + 3], trial ? r.v[S_CUR_SLP] : nullptr, trial ? r.v[S_NEXT_SLP] : nullptr,
                               trial ? r.v[S_CUR_SLP_SUM] : nullptr, r.partials, nan_inject);
-        gplk::launch_finish(stream, r.partials, 0, gplk::block_partials_cells(r.dev), d_sums + b, d_hpwl);
+        gplk::launch_finish(stream, r.partials, trial ? gplk::block_partials_bins(r.dev) : 0, gplk::block_partials_cells(r.dev), d_sums + b,
+                            d_hpwl);
         mark_end();
     }
 
@@ -438,14 +443,22 @@ struct gpl_ctx {
         return GPL_OK;
     }
 
-    // nesterovUpdateCoordinates for every region (the scatter); the field follows in field().
+    // The first live region's scatter clears the HPWL counter for the trial's WA kernels.
+    unsigned long long* zero_for(size_t b) const {
+        for (size_t q = 0; q < rg.size(); ++q)
+            if (!rg[q].is_converged) return q == b ? d_hpwl : nullptr;
+        return nullptr;
+    }
+
+    // nesterovUpdateCoordinates for every region (the scatter); the field follows in field(), the
+    // WA kernels in wa_forward(false, true) (the counter cleared here).
     void update_coordinates(float coeff) {
         for (size_t b = 0; b < rg.size(); ++b) {
             Region& r = rg[b];
             if (r.is_converged) continue;
             mark(F_SCATTER);
             gplk::launch_update_scatter(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
-                                        r.step_length, coeff, d_global_adds);
+                                        r.step_length, coeff, d_global_adds, zero_for(b));
             mark_end();
         }
         if (overlap && !profiling) check(hipEventRecord(ev_fork, stream), "event");
@@ -462,7 +475,7 @@ struct gpl_ctx {
         for (size_t b = 0; b < rg.size(); ++b) {
             Region& r = rg[b];
             if (r.is_converged) continue;
-            bins_and_finish(r, int(b));
+            bins_and_finish(r, int(b), false);  // finished with the trial's gradient sums
             solve(r);
         }
         if (ovl) {
@@ -528,6 +541,7 @@ struct gpl_ctx {
         return k;
     }
 
+    // Captured with the side streams: the bins and the solve form a branch beside the WA kernels.
     bool build_graph() {
         drop_graph();
         check(hipStreamSynchronize(stream), "sync");
@@ -536,7 +550,7 @@ struct gpl_ctx {
         try {
             update_coordinates(0.0f);
             field();
-            wa_forward();
+            wa_forward(false, true);
             join();
             launch_gradients(0, int(rg.size()), SLOT_NEXT, true);
             check(hipMemcpyAsync(h_sums, d_sums, sizeof(gplk::Sums) * (rg.size() + 1), hipMemcpyDeviceToHost, stream), "stats copy");
@@ -589,7 +603,7 @@ struct gpl_ctx {
             if (n.kind == gplk::TK_SCATTER) {
                 Region& r = *region_of(n.cell_begin);
                 e = gplk::set_scatter_node(texec, n, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT],
-                                           r.v[S_NEXT_SLP], r.step_length, coeff, d_global_adds);
+                                           r.v[S_NEXT_SLP], r.step_length, coeff, d_global_adds, zero_for(size_t(&r - rg.data())));
             } else if (n.kind == gplk::TK_GRADIENT) {
                 Region& r = *region_of(n.cell_begin);
                 e = gplk::set_gradient_node(texec, n, r.dev, cells, nets, wl_coef_x, wl_coef_y, r.density_penalty, r.v[S_NEXT_SLP + 1],
@@ -614,7 +628,7 @@ struct gpl_ctx {
             if (!trial_by_graph(coeff)) {
                 update_coordinates(coeff);
                 field();
-                wa_forward();
+                wa_forward(false, true);
                 np_update_gradients(0, int(rg.size()), SLOT_NEXT, true);
             }
             for (size_t b = 0; b < rg.size(); ++b) {
@@ -1260,7 +1274,6 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         std::vector<uint8_t> flags(N);
         std::vector<int32_t> pin_start(size_t(N) + 1, 0), cell_inst(N);
         std::vector<int32_t> cp_npin(p->cell_pin_start[N]);
-        std::vector<int2> cp_off(p->cell_pin_start[N]);
         std::vector<int32_t> cpi(pcell.size(), -1);
         for (int32_t i = 0; i < N; ++i) {
             int32_t g = c->cell_old_of[i];
@@ -1275,7 +1288,6 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             for (int32_t k = p->cell_pin_start[g]; k < p->cell_pin_start[g + 1]; ++k, ++at) {
                 int32_t q = p->cell_pins[k];
                 cp_npin[at] = npin_of[q];
-                cp_off[at] = make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]);
                 cpi[npin_of[q]] = at;
             }
             pin_start[i + 1] = at;
@@ -1292,10 +1304,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
 
         c->cells.pin_start = c->upload(pin_start.data(), size_t(N) + 1);
         c->cells.cp_npin = c->upload(cp_npin.data(), cp_npin.size());
-        c->cells.cp_off = c->upload(cp_off.data(), cp_off.size());
-        // pin positions in net order, the initial ones until a scatter centres their cells
-        c->cells.ppos = c->upload(pxy0.data(), pxy0.size());
-        c->nets.ppos = c->cells.ppos;
+
         c->nets.cpi = c->upload(cpi.data(), cpi.size());
         c->nets.pgrad = c->dalloc<float2>(std::max<size_t>(cp_npin.size(), 1));
         c->cells.pre = c->upload(pre.data(), N);
@@ -1312,8 +1321,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             a.flags = c->dalloc<uint8_t>(N);
             a.pin_start = c->dalloc<int32_t>(size_t(N) + 1);
             a.cp_npin = c->dalloc<int32_t>(cp_npin.size());
-            a.cp_off = c->dalloc<int2>(cp_off.size());
-            a.ppos = c->cells.ppos;
+
             a.pre = c->dalloc<float2>(N);
             a.old_of = c->dalloc<int32_t>(N);
             c->rb.keys_in = c->dalloc<uint32_t>(N);
@@ -1395,6 +1403,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         check(hipHostMalloc((void**)&c->h_sums, sizeof(gplk::Sums) * (size_t(p->num_regions) + 1), hipHostMallocDefault), "pinned");
         c->d_hpwl = &c->d_sums[p->num_regions].hpwl;
         c->d_global_adds = (unsigned long long*)&c->d_sums[p->num_regions].pad[0];
+
         // the region table of the re-sort; per-region scratch vectors for its gathers
         c->resort_enabled = p->num_regions <= gplk::kMaxRegions;
         c->rtab.n = std::min(p->num_regions, gplk::kMaxRegions);
@@ -1750,7 +1759,7 @@ int32_t gpl_nesterov_step(gpl_ctx* c, int32_t region, float step, float coeff, f
     try {
         Region& r = c->rg[region];
         gplk::launch_update_scatter(c->stream, r.dev, c->cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
-                                    step, coeff, c->d_global_adds);
+                                    step, coeff, c->d_global_adds, nullptr);
         c->bins_and_finish(r, region);
         c->sync_stats();
         if (next) c->download_region_vec(region, r.v[S_NEXT], next);
