@@ -54,25 +54,24 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c3.txt")
 def algorithmic_bytes(fam, N, P, E, M):
     """Compulsory bytes per launch of each kernel family (every array touched once; N cells,
     P pins, E nets, M bins):
-    update_scatter  85 N + 16 P + 16 M   per cell: read cur_slp, cur_sum, cur (24), box (16), flags
-                                  (1), scale (4); write next, next_slp (16), box (16), centre (8);
-                                  per cell-pin: read its net-pin index and offset (12), write the
-                                  pin's position in net order (8) -- counted 16 with the offset
-                                  read; the two int64 bin accumulators written once (16 per bin)
+    update_scatter  85 N + 16 M   per cell: read cur_slp, cur_sum, cur (24), box (16), flags (1),
+                                  scale (4); write next, next_slp (16), box (16), centre (8); the two
+                                  int64 bin accumulators written once (16 per bin)
     bin_density     52 M          read inst, filler, non_place, non_place_unscaled (32); write the
                                   density (4); clear the accumulators (16)
     dct_solve       12 M          read the density (4); write field_x, field_y (8)
-    wa_forward      20 P + 8 E    per pin: its position (8), cell-pin index (4), its gradient term
-                                  written (8); per net: start (4), weight (4)
+    wa_forward      32 P + 8 E    per pin: cell (4), offset (8), the cell's centre (8), cell-pin
+                                  index (4), its gradient term written (8); per net: start (4),
+                                  weight (4)
     gradient        80 N + 8 P + 8 M   per cell: box, pin start, preconditioner, scale (32), 3 slots
                                   read, 3 written (48); per cell-pin: its gradient term (8); the two
                                   fields once (8 per bin)
     """
     return {
-        "update_scatter": 85 * N + 16 * P + 16 * M,
+        "update_scatter": 85 * N + 16 * M,
         "bin_density": 52 * M,
         "dct_solve": 12 * M,
-        "wa_forward": 20 * P + 8 * E,
+        "wa_forward": 32 * P + 8 * E,
         "gradient": 80 * N + 8 * P + 8 * M,
     }.get(fam)
 
