@@ -31,3 +31,15 @@ def c1_design():
     d.init_nesterov_place()
     yield d
     d.close()
+
+
+@pytest.fixture(scope="session")
+def c3_design():
+    """The 1M-cell workload (1024 x 1024 bins), initially placed, Nesterov placer built (GPU tests
+    only: about ten seconds of host work)."""
+    import gpl_amd as G
+    d = G.Design.from_config("c3")
+    d.initial_place()
+    d.init_nesterov_place()
+    yield d
+    d.close()

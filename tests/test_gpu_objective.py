@@ -79,11 +79,9 @@ def test_objective_gradient_matches_central_differences(c1):
     _check(c1, spread_points(c1), 600, 4, wl_tol=2e-3, den_tol=1e-4)
 
 
-def test_objective_gradient_at_1m_cells():
+def test_objective_gradient_at_1m_cells(c3_design):
     """2,000 seeded cells of the 1M-cell workload at the loop's points after 100 iterations."""
-    d = G.Design.from_config("c3")
-    d.initial_place()
-    d.init_nesterov_place()
+    d = c3_design
     g = G.NesterovGPU(d)
     g.init_nesterov_loop()
     g.loop_begin(0)
@@ -92,4 +90,3 @@ def test_objective_gradient_at_1m_cells():
     g.close()
     lam, e_wl, e_den = _check(d, xy, 2000, 7, wl_tol=5e-3, den_tol=1e-4)
     print(f"1M cells: lambda {lam:.3g}, worst wirelength error {e_wl:.2e}, worst total error {e_den:.2e}")
-    d.close()
