@@ -81,7 +81,7 @@ struct NetsDev {
     int32_t num_nets;
     const int32_t* start;   // num_nets + 1, into the net-order pin arrays
     int32_t* pcell;         // per net-pin: cell or -1 (renamed by a re-sort)
-    const int2* poff;       // per net-pin: offset
+    const int2* poff;       // per net-pin: offset from the cell centre; a pin on no movable cell: its position
     const int2* pxy0;       // per net-pin: position as init left it
     const float* weight;    // per net
     int32_t* cpi;           // per net-pin: its cell-pin index, or -1 (renumbered by a re-sort)

@@ -195,6 +195,7 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
             const int g = r.cell_begin + k;
             int4 d = c.dbox[g];
             const uint8_t f = c.flags[g];
+            const float sc = c.dscale[g];
             float2 ns;
             if (MODE == PLACE_INIT) {
                 const int ddx = d.z - d.x, ddy = d.w - d.y;
@@ -210,19 +211,17 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
             } else if (MODE == PLACE_CENTER) {
                 ns = pa.xy[k];
             } else {
-                if (f & 4u) {
-                    pa.next[k] = pa.cur[k];
-                    ns = pa.cur_slp[k];
-                } else {
-                    const float2 a = pa.cur_slp[k], sm = pa.cur_sum[k], cu = pa.cur[k];
-                    const float nx = a.x + pa.s * sm.x;
-                    const float ny = a.y + pa.s * sm.y;
-                    const float sx = nx + pa.coeff * (nx - cu.x);
-                    const float sy = ny + pa.coeff * (ny - cu.y);
-                    const float hx = (float)(d.z - d.x) / 2.0f, hy = (float)(d.w - d.y) / 2.0f;
-                    pa.next[k] = make_float2(inside(nx, hx, lx, ux), inside(ny, hy, ly, uy));
-                    ns = make_float2(inside(sx, hx, lx, ux), inside(sy, hy, ly, uy));
-                }
+                // computed for every cell and selected, so that the three reads are issued with
+                // the box's rather than after a branch on the flags; a locked cell (flag 4) holds
+                const float2 a = pa.cur_slp[k], sm = pa.cur_sum[k], cu = pa.cur[k];
+                const float nx = a.x + pa.s * sm.x;
+                const float ny = a.y + pa.s * sm.y;
+                const float sx = nx + pa.coeff * (nx - cu.x);
+                const float sy = ny + pa.coeff * (ny - cu.y);
+                const float hx = (float)(d.z - d.x) / 2.0f, hy = (float)(d.w - d.y) / 2.0f;
+                const bool locked = f & 4u;
+                pa.next[k] = locked ? cu : make_float2(inside(nx, hx, lx, ux), inside(ny, hy, ly, uy));
+                ns = locked ? a : make_float2(inside(sx, hx, lx, ux), inside(sy, hy, ly, uy));
                 pa.next_slp[k] = ns;
             }
             d = centre_box(d, ns.x, ns.y);
@@ -233,22 +232,42 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
             int x0, x1, y0, y1;
             bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
             bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
-            const float sc = c.dscale[g];
             const bool inst = f & 1u, macro = inst && (f & 2u);
             V* vals = inst ? t_inst : t_fill;
             unsigned long long* glob = inst ? r.inst : r.filler;
-            for (int y = y0; y < y1; ++y) {
-                const int by0 = r.bin_y[y], by1 = r.bin_y[y + 1];
-                for (int x = x0; x < x1; ++x) {
-                    float a = overlap_area(r.bin_x[x], by0, r.bin_x[x + 1], by1, d) * sc;
-                    if (macro) a *= r.target_density;
-                    const long long v = (long long)a;
-                    if (v == 0) continue;
-                    const unsigned kb = (unsigned)y * (unsigned)r.cnt_x + (unsigned)x;
-                    if (!table_add<V>(t_key, vals, kb, (V)v)) {
-                        atomicAdd(glob + kb, (unsigned long long)v);
-                        ++adds;
+            auto add = [&](int x, int y, int bx0, int by0, int bx1, int by1) {
+                float a = overlap_area(bx0, by0, bx1, by1, d) * sc;
+                if (macro) a *= r.target_density;
+                const long long v = (long long)a;
+                if (v == 0) return;
+                const unsigned kb = (unsigned)y * (unsigned)r.cnt_x + (unsigned)x;
+                if (!table_add<V>(t_key, vals, kb, (V)v)) {
+                    atomicAdd(glob + kb, (unsigned long long)v);
+                    ++adds;
+                }
+            };
+            if (x1 - x0 <= 3 && y1 - y0 <= 3) {
+                // the edges of a 3 x 3 bin window read at once (indices clamped, no guards, so
+                // that the compiler waits once): the span of any cell up to twice a bin's size
+                int ex[4], ey[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    ex[i] = r.bin_x[min(x0 + i, r.cnt_x)];
+                    ey[i] = r.bin_y[min(y0 + i, r.cnt_y)];
+                }
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    if (j >= y1 - y0) break;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        if (i >= x1 - x0) break;
+                        add(x0 + i, y0 + j, ex[i], ey[j], ex[i + 1], ey[j + 1]);
                     }
+                }
+            } else {
+                for (int y = y0; y < y1; ++y) {
+                    const int by0 = r.bin_y[y], by1 = r.bin_y[y + 1];
+                    for (int x = x0; x < x1; ++x) add(x, y, r.bin_x[x], by0, r.bin_x[x + 1], by1);
                 }
             }
         }
@@ -573,20 +592,47 @@ __global__ void k_transpose(const float* in, float* out, int R, int C) {
 // WA wirelength (nesterov.rs:2325-2430)
 // ---------------------------------------------------------------------------------------------
 
-// A net pin's position: its cell's density centre plus its offset once the cell has been
-// centred, else where init left it (nesterov.rs:2150).
-// A net pin's position: its cell's centre plus its offset, or its initial position while the cell
-// has not been centred (or for a port).
+// A net pin's position (nesterov.rs:2150, 2268-2276) from its cell index, the cell's centre and
+// the pin's offset: its cell's centre plus the offset; a pin on no movable cell (port, fixed
+// instance) holds its fixed position in the offset (the centre, read from cell 0, is then
+// unused); a cell not yet centred leaves its pins where init put them.  The centre is read
+// unconditionally so that the callers can issue every pin's reads before waiting for any.
+__device__ __forceinline__ int2 pin_at(const NetsDev& n, int k, int cell, int2 m, int2 o) {
+    if (cell < 0) return o;
+    if (m.x == INT_MIN) return n.pxy0[k];
+    return make_int2(m.x + o.x, m.y + o.y);
+}
 __device__ __forceinline__ int2 pin_pos(const NetsDev& n, const CellsDev& c, int k) {
     const int cell = n.pcell[k];
-    if (cell >= 0) {
-        const int2 m = c.ctr[cell];
-        if (m.x != INT_MIN) {
-            const int2 o = n.poff[k];
-            return make_int2(m.x + o.x, m.y + o.y);
+    const int2 o = n.poff[k];
+    return pin_at(n, k, cell, c.ctr[max(cell, 0)], o);
+}
+
+// Pins s + i * stride (i < R, i * stride < deg) of a net: cell indices, offsets and, with GRAD,
+// cell-pin indices read in one batch, then every centre in a second, so that a net waits for two
+// memory round trips instead of two or three per pin.
+template <int R, bool GRAD>
+__device__ __forceinline__ void load_pins(const NetsDev& n, const CellsDev& c, int s, int deg, int stride, int2 (&pos)[R],
+                                          int (&cq)[R]) {
+    int cl[R];
+    int2 of[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        cl[i] = -1;
+        of[i] = make_int2(0, 0);
+        cq[i] = -1;
+        if (i * stride < deg) {
+            cl[i] = n.pcell[s + i * stride];
+            of[i] = n.poff[s + i * stride];
+            if (GRAD) cq[i] = n.cpi[s + i * stride];
         }
     }
-    return n.pxy0[k];
+    // unguarded (an absent pin reads cell 0): a guard per read makes the compiler wait after each
+#pragma unroll
+    for (int i = 0; i < R; ++i) pos[i] = c.ctr[max(cl[i], 0)];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+        if (i * stride < deg) pos[i] = pin_at(n, s + i * stride, cl[i], pos[i], of[i]);
 }
 
 // The four shifted exponentials of one pin (nesterov.rs:2354-2385) added into the net sums.
@@ -628,9 +674,8 @@ __device__ __forceinline__ float wa_term_max(float S0, float S1, float v, float 
 
 // One pin's weighted WA gradient (getWireLengthGradientPinWA, nesterov.rs:2392-2418, times the net
 // weight) stored at its cell-pin index.
-__device__ __forceinline__ void pin_grad(const NetsDev& n, int k, int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float4 SX,
+__device__ __forceinline__ void pin_grad(const NetsDev& n, int q, int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float4 SX,
                                          float4 SY, float w) {
-    const int q = n.cpi[k];
     if (q < 0) return;
     const float fpx = (float)p.x, fpy = (float)p.y;
     float gmin_x = 0.0f, gmax_x = 0.0f, gmin_y = 0.0f, gmax_y = 0.0f;
@@ -665,11 +710,8 @@ __global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, flo
     const int s = n.start[e], t = n.start[e + 1];
     const int rest = s + kBigRegs * kBigBlock;  // pins from here on are loaded again in each pass
     int2 pos[kBigRegs];
-#pragma unroll
-    for (int i = 0; i < kBigRegs; ++i) {
-        const int k = s + threadIdx.x + i * kBigBlock;
-        pos[i] = k < t ? pin_pos(n, c, k) : make_int2(0, 0);
-    }
+    int cq[kBigRegs];
+    load_pins<kBigRegs, SUMS>(n, c, s + (int)threadIdx.x, t - s - (int)threadIdx.x, kBigBlock, pos, cq);
     int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
 #pragma unroll
     for (int i = 0; i < kBigRegs; ++i) {
@@ -753,9 +795,10 @@ __global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, flo
 #pragma unroll
         for (int i = 0; i < kBigRegs; ++i) {
             const int k = s + threadIdx.x + i * kBigBlock;
-            if (k < t) pin_grad(n, k, pos[i], lx, ly, ux, uy, cx, cy, SX, SY, wt);
+            if (k < t) pin_grad(n, cq[i], pos[i], lx, ly, ux, uy, cx, cy, SX, SY, wt);
         }
-        for (int k = rest + threadIdx.x; k < t; k += blockDim.x) pin_grad(n, k, pin_pos(n, c, k), lx, ly, ux, uy, cx, cy, SX, SY, wt);
+        for (int k = rest + threadIdx.x; k < t; k += blockDim.x)
+            pin_grad(n, n.cpi[k], pin_pos(n, c, k), lx, ly, ux, uy, cx, cy, SX, SY, wt);
     }
     if (threadIdx.x == 0 && ux >= lx) atomicAdd(hpwl, (unsigned long long)((long long)ux - lx + (long long)uy - ly));
 }
@@ -777,11 +820,12 @@ __global__ void __launch_bounds__(kBlock) k_wa_small(NetsDev n, CellsDev c, floa
     for (int e = n.slice[x] + (blockIdx.x / kXcds) * blockDim.x + threadIdx.x; e < e_end; e += stride) {
         const int s = n.start[e], deg = n.start[e + 1] - s;
         int2 pos[kRegPins];
+        int cq[kRegPins];
+        load_pins<kRegPins, SUMS>(n, c, s, deg, 1, pos, cq);
         int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
 #pragma unroll
         for (int k = 0; k < kRegPins; ++k) {
             if (k < deg) {
-                pos[k] = pin_pos(n, c, s + k);
                 lx = min(lx, pos[k].x);
                 ly = min(ly, pos[k].y);
                 ux = max(ux, pos[k].x);
@@ -810,8 +854,8 @@ __global__ void __launch_bounds__(kBlock) k_wa_small(NetsDev n, CellsDev c, floa
             const float wt = n.weight[e];
 #pragma unroll
             for (int k = 0; k < kRegPins; ++k)
-                if (k < deg) pin_grad(n, s + k, pos[k], lx, ly, ux, uy, cx, cy, a, b, wt);
-            for (int k = kRegPins; k < deg; ++k) pin_grad(n, s + k, pin_pos(n, c, s + k), lx, ly, ux, uy, cx, cy, a, b, wt);
+                if (k < deg) pin_grad(n, cq[k], pos[k], lx, ly, ux, uy, cx, cy, a, b, wt);
+            for (int k = kRegPins; k < deg; ++k) pin_grad(n, n.cpi[s + k], pin_pos(n, c, s + k), lx, ly, ux, uy, cx, cy, a, b, wt);
         }
         if (ux >= lx) h += (unsigned long long)((long long)ux - lx + (long long)uy - ly);
     }
@@ -825,6 +869,7 @@ __global__ void __launch_bounds__(kBlock) k_wa_small(NetsDev n, CellsDev c, floa
     }
 }
 
+constexpr int kGradPins = 8;
 // updateGradients (nesterov.rs:2455-2489) per cell: the WA gradient over the cell's pins in
 // their order (each pin's weighted term from the WA kernel); the density gradient over the bins its box
 // covers, x outer, y inner (2434); the preconditioned sum; FP64 partial sums of |wl|, |den|,
@@ -839,28 +884,74 @@ __global__ void k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float 
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const int g = r.cell_begin + k;
         const int4 d = c.dbox[g];
-        float wx = 0.0f, wy = 0.0f;
-        for (int q = c.pin_start[g]; q < c.pin_start[g + 1]; ++q) {
-            const float2 t = nt.pgrad[q];
-            wx += t.x;
-            wy += t.y;
-        }
-        // density gradient
+        const int q0 = c.pin_start[g], q1 = c.pin_start[g + 1];
+        const float scale = c.dscale[g];
+        const float2 pr = c.pre[g];
         int x0, x1, y0, y1;
         bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
         bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
-        const float scale = c.dscale[g];
-        float fx = 0.0f, fy = 0.0f;
-        for (int x = x0; x < x1; ++x) {
-            int bx0 = r.bin_x[x], bx1 = r.bin_x[x + 1];
-            for (int y = y0; y < y1; ++y) {
-                float a = overlap_area(bx0, r.bin_y[y], bx1, r.bin_y[y + 1], d) * scale;
-                size_t kb = (size_t)y * r.cnt_x + x;
-                fx += a * r.field_x[kb];
-                fy += a * r.field_y[kb];
+        // Every read of the cell's first kGradPins pin terms and of a 3 x 3 bin window is issued
+        // before any is used (indices clamped into the arrays, no guards, so that the compiler
+        // does not wait after each); cells with more pins or a wider span finish in the loops.
+        float2 t[kGradPins];
+        const int qlast = max(q1 - 1, 0);
+#pragma unroll
+        for (int j = 0; j < kGradPins; ++j) t[j] = nt.pgrad[min(q0 + j, qlast)];
+        int ex[4], ey[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ex[i] = r.bin_x[min(x0 + i, r.cnt_x)];
+            ey[i] = r.bin_y[min(y0 + i, r.cnt_y)];
+        }
+        float wfx[9], wfy[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const size_t kb = (size_t)min(y0 + j, r.cnt_y - 1) * r.cnt_x + min(x0 + i, r.cnt_x - 1);
+                wfx[3 * i + j] = r.field_x[kb];
+                wfy[3 * i + j] = r.field_y[kb];
             }
         }
-        float2 pr = c.pre[g];
+        // the WA gradient: the pins' terms in the cell's pin order
+        float wx = 0.0f, wy = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kGradPins; ++j) {
+            if (q0 + j < q1) {
+                wx += t[j].x;
+                wy += t[j].y;
+            }
+        }
+        for (int q = q0 + kGradPins; q < q1; ++q) {
+            const float2 u = nt.pgrad[q];
+            wx += u.x;
+            wy += u.y;
+        }
+        // density gradient, x outer, y inner
+        float fx = 0.0f, fy = 0.0f;
+        if (x1 - x0 <= 3 && y1 - y0 <= 3) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                if (i >= x1 - x0) break;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    if (j >= y1 - y0) break;
+                    float a = overlap_area(ex[i], ey[j], ex[i + 1], ey[j + 1], d) * scale;
+                    fx += a * wfx[3 * i + j];
+                    fy += a * wfy[3 * i + j];
+                }
+            }
+        } else {
+            for (int x = x0; x < x1; ++x) {
+                int bx0 = r.bin_x[x], bx1 = r.bin_x[x + 1];
+                for (int y = y0; y < y1; ++y) {
+                    float a = overlap_area(bx0, r.bin_y[y], bx1, r.bin_y[y + 1], d) * scale;
+                    size_t kb = (size_t)y * r.cnt_x + x;
+                    fx += a * r.field_x[kb];
+                    fy += a * r.field_y[kb];
+                }
+            }
+        }
         float sx = wx + penalty * fx;
         float sy = wy + penalty * fy;
         float pre = rmax(pr.x + penalty * pr.y, 1.0f);

@@ -1193,7 +1193,9 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         }
         for (int32_t i = 0; i < N; ++i) c->cell_new_of[c->cell_old_of[i]] = i;
         auto newc = [&](int32_t old) { return old >= 0 ? c->cell_new_of[old] : -1; };
-        // nets sorted by degree, then by their first pin's cell (spatial order within a degree);
+        // nets sorted by the XCD slice of their first pin's cell, then by windows of 4096 cells of
+        // it, then by degree (so a wave's lanes loop equally), then by the first cell: the nets in
+        // flight at once touch one window's cells, whose centres and gradient terms stay in L2;
         // nets above 128 pins last, one block each
         std::vector<int32_t> order(E);
         std::iota(order.begin(), order.end(), 0);
@@ -1219,6 +1221,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         for (int32_t e = 0; e < E; ++e) sl[e] = slice_of(e);
         std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
             if (sl[a] != sl[b]) return sl[a] < sl[b];
+            if ((fc[a] >> 12) != (fc[b] >> 12)) return fc[a] < fc[b];
             return deg(a) != deg(b) ? deg(a) < deg(b) : fc[a] < fc[b];
         });
         c->net_orig_of = order;
@@ -1247,8 +1250,9 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
                 int32_t q = p->net_pins[k];
                 npin_of[q] = int32_t(pcell.size());
                 pcell.push_back(newc(p->pin_cell[q]));
-                poff.push_back(make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]));
                 pxy0.push_back(make_int2(p->pin_xy[2 * q], p->pin_xy[2 * q + 1]));
+                // a pin on no movable cell keeps its fixed position in the offset slot
+                poff.push_back(pcell.back() >= 0 ? make_int2(p->pin_offset[2 * q], p->pin_offset[2 * q + 1]) : pxy0.back());
             }
             nstart[i + 1] = int32_t(pcell.size());
         }
