@@ -42,7 +42,8 @@ WORKLOADS = {
 
 # kernels of each family, the first one launched once per family launch
 FAMILY_KERNELS = {
-    "update_scatter": ["k_place_scatter<2, unsigned int>", "k_place_scatter<2, unsigned long long>"],
+    "update": ["k_place_update"],
+    "scatter": ["k_place_scatter<2, unsigned int>", "k_place_scatter<2, unsigned long long>"],
     "bin_density": ["k_bin_density"],
     "dct_solve": ["k_rows<0>", "k_rows<1>", "k_rows<2>", "k_rows<3>", "k_transpose"],
     "wa_forward": ["k_wa_small<true>", "k_wa_big<true>"],
@@ -54,9 +55,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c3.txt")
 def algorithmic_bytes(fam, N, P, E, M):
     """Compulsory bytes per launch of each kernel family (every array touched once; N cells,
     P pins, E nets, M bins):
-    update_scatter  85 N + 16 M   per cell: read cur_slp, cur_sum, cur (24), box (16), flags (1),
-                                  scale (4); write next, next_slp (16), box (16), centre (8); the two
-                                  int64 bin accumulators written once (16 per bin)
+    update          81 N          per cell: read cur_slp, cur_sum, cur (24), box (16), flags (1);
+                                  write next, next_slp (16), box (16), centre (8)
+    scatter         21 N + 16 M   per cell: read box (16), flags (1), scale (4); the two int64 bin
+                                  accumulators written once (16 per bin)
     bin_density     52 M          read inst, filler, non_place, non_place_unscaled (32); write the
                                   density (4); clear the accumulators (16)
     dct_solve       12 M          read the density (4); write field_x, field_y (8)
@@ -68,7 +70,8 @@ def algorithmic_bytes(fam, N, P, E, M):
                                   fields once (8 per bin)
     """
     return {
-        "update_scatter": 85 * N + 16 * M,
+        "update": 81 * N,
+        "scatter": 21 * N + 16 * M,
         "bin_density": 52 * M,
         "dct_solve": 12 * M,
         "wa_forward": 32 * P + 8 * E,

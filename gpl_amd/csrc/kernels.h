@@ -88,8 +88,8 @@ struct NetsDev {
     float2* pgrad;          // per cell-pin: the pin's weighted WA gradient, written by the WA kernel
     NetRec* rec;            // box and sums per net, written when asked for (the kernel-level entry)
     int32_t num_small;      // nets [0, num_small) take one thread each; the rest a block each
-    // the small nets in 8 slices by their first cell's eighth of the cell order, [slice[x],
-    // slice[x+1]) each (the kernel gives slice x to XCD x)
+    // the small nets in 8 contiguous slices of equal pin counts, [slice[x], slice[x+1]) each (the
+    // kernel gives slice x to XCD x)
     int32_t slice[9];
     int32_t slice_blocks;   // workgroups per slice
 };
@@ -114,11 +114,12 @@ void launch_center_init(hipStream_t s, const RegionDev& r, const CellsDev& c, fl
 // (global_adds, when not null, counts the 64-bit atomic adds the scatter sent to the bins in HBM)
 // centre the region's cells at `xy` and scatter them into the bins.
 void launch_center_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy, unsigned long long* global_adds);
-// nesterovUpdateCoordinates + centring + scatter.
+// nesterovUpdateCoordinates + centring: next, next_slp, the density boxes and the centres.
 // (zero, when not null, is cleared first: the trial's HPWL counter)
-void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
-                           const float2* cur, float2* next, float2* next_slp, float step, float coeff, unsigned long long* global_adds,
-                           unsigned long long* zero);
+void launch_update(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum, const float2* cur,
+                   float2* next, float2* next_slp, float step, float coeff, unsigned long long* zero);
+// the scatter of the density boxes launch_update left into the bins.
+void launch_density_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, unsigned long long* global_adds);
 // per bin: density, overflow partials, accumulators cleared.
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials);
 // the Poisson solve of r.density into r.field_x / r.field_y (and r.phi when want_phi).
@@ -186,19 +187,21 @@ void launch_remap_pcell(hipStream_t s, int n_pins, int32_t* pcell, const int32_t
 void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v);
 
 // ---- a back-tracking trial as a HIP graph: the kernel nodes whose arguments change between
-// trials (the update scatter and the gradient per region, the two WA kernels), found by kernel
-// and region in a captured graph and given new arguments before each launch -----------------
-enum TrialKind { TK_SCATTER = 0, TK_WA_SMALL = 1, TK_WA_BIG = 2, TK_GRADIENT = 3 };
+// trials (the update, the density scatter and the gradient per region, the two WA kernels),
+// found by kernel and region in a captured graph and given new arguments before each launch ---
+enum TrialKind { TK_SCATTER = 0, TK_WA_SMALL = 1, TK_WA_BIG = 2, TK_GRADIENT = 3, TK_UPDATE = 4 };
 struct TrialNode {
     hipGraphNode_t node;
     hipKernelNodeParams p;  // as captured (function, grid, block, LDS)
     int kind;
-    int cell_begin;         // the node's region (scatter, gradient), by its first cell
+    int cell_begin;         // the node's region (update, scatter, gradient), by its first cell
 };
 int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap);  // count, or -1
-hipError_t set_scatter_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const float2* cur_slp,
-                            const float2* cur_sum, const float2* cur, float2* next, float2* next_slp, float step, float coeff,
-                            unsigned long long* global_adds, unsigned long long* zero);
+hipError_t set_update_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const float2* cur_slp,
+                           const float2* cur_sum, const float2* cur, float2* next, float2* next_slp, float step, float coeff,
+                           unsigned long long* zero);
+hipError_t set_scatter_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c,
+                            unsigned long long* global_adds);
 hipError_t set_wa_node(hipGraphExec_t e, const TrialNode& n, const NetsDev& nd, const CellsDev& c, float cx, float cy,
                        unsigned long long* hpwl);
 hipError_t set_gradient_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const NetsDev& nd,

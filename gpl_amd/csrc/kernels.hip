@@ -22,10 +22,10 @@ constexpr int kMaxBlocks = kPartials;
 // The dispatcher hands workgroups to the 8 XCDs in turn (workgroup b to XCD b % 8), and each XCD
 // has its own L2.  The scatter takes its chunks of Morton-ordered cells by the logical index below,
 // which gives XCD x the contiguous range [x G/8, (x+1) G/8) of a G-workgroup grid (G a multiple of
-// 8), so that the bins of neighbouring chunks meet in one L2.  The WA kernel gives XCD x the nets
-// whose first cell lies in the x-th eighth of the cell order (the host sorts the nets by that
-// slice first, then by degree), so that the cell centres it gathers and the pin gradients it
-// scatters stay within one L2.
+// 8), so that the bins of neighbouring chunks meet in one L2.  The WA kernel gives XCD x the x-th
+// of 8 contiguous slices of the nets (the host orders the nets by their first cell in windows of
+// 4096 cells, and cuts the slices at equal pin counts), so that the cell centres it gathers and
+// the pin gradients it scatters stay within one L2 and the XCDs get equal work.
 constexpr int kXcds = 8;
 __device__ __forceinline__ int xcd_block(int b, int G) { return (b % kXcds) * (G / kXcds) + b / kXcds; }
 __host__ __device__ constexpr int round_xcds(int g) { return (g + kXcds - 1) / kXcds * kXcds; }
@@ -108,21 +108,22 @@ __global__ void k_zero_bins(unsigned long long* a, unsigned long long* b, size_t
     }
 }
 
-// The scatter family: a cell's new density box from one of three rules, then its overlaps with
-// the bins added into the int64 accumulators (nesterov.rs:2225-2250).
+// The scatter family: a cell's density box by one of three rules, then its overlaps with the bins
+// added into the int64 accumulators (nesterov.rs:2225-2250).
 //   PLACE_INIT    initDensity1 (nesterov.rs:2546-2570): the box moved inside the grid
 //                 (updateDensityCoordiLayoutInside, 2173), its centre as cur_slp / prev_slp / cur,
 //                 the box centred there
 //   PLACE_CENTER  updateGCellDensityCenterLocation (nesterov.rs:2268) at coordinates xy
-//   PLACE_UPDATE  nesterovUpdateCoordinates (nesterov.rs:2718-2745): next and next_slp from the
-//                 step and the momentum, kept inside, the box centred at next_slp
+//   PLACE_STORED  the box as k_place_update left it (the trial's Nesterov update runs as its own
+//                 kernel so that the WA kernels, which need only the cells' centres, can start
+//                 beside this scatter)
 // A block takes kChunk consecutive cells (the device keeps each region's cells in Morton order of
 // their bins, re-sorted when that order has gone stale, so a chunk touches few distinct bins).  Its
 // overlaps are summed in an LDS hash table keyed by bin (kSlots slots, linear probing, keys set
 // by compare-and-swap), then each occupied slot is added into the bins with one global atomic;
 // an overlap that finds no slot within kProbe probes goes straight to global memory.  The sums
 // are of integers: the result does not depend on the path.
-enum PlaceMode { PLACE_INIT = 0, PLACE_CENTER = 1, PLACE_UPDATE = 2 };
+enum PlaceMode { PLACE_INIT = 0, PLACE_CENTER = 1, PLACE_STORED = 2 };
 constexpr int kCellsPerThread = kPlaceChunk / kBlock;
 constexpr int kChunk = kPlaceChunk;
 constexpr int kSlots = 2048;
@@ -162,6 +163,48 @@ __device__ __forceinline__ bool table_add(unsigned* keys, V* vals, unsigned k, V
     return false;
 }
 
+// nesterovUpdateCoordinates (nesterov.rs:2718-2745) for one unlocked cell: next and next_slp from
+// the step and the momentum, each kept inside the region (`inside`, 2188-2213) by the box's half
+// size.
+__device__ __forceinline__ void nesterov_point(float2 a, float2 sm, float2 cu, float s, float coeff, int4 d, float lx, float ly,
+                                               float ux, float uy, float2& nxt, float2& nsl) {
+    const float nx = a.x + s * sm.x;
+    const float ny = a.y + s * sm.y;
+    const float sx = nx + coeff * (nx - cu.x);
+    const float sy = ny + coeff * (ny - cu.y);
+    const float hx = (float)(d.z - d.x) / 2.0f, hy = (float)(d.w - d.y) / 2.0f;
+    nxt = make_float2(inside(nx, hx, lx, ux), inside(ny, hy, ly, uy));
+    nsl = make_float2(inside(sx, hx, lx, ux), inside(sy, hy, ly, uy));
+}
+
+// The trial's placement step for every cell of a region: the Nesterov update (a locked cell,
+// flag 4, holds: next = cur, next_slp = cur_slp), the density box centred at next_slp
+// (updateGCellDensityCenterLocation, nesterov.rs:2139-2153) and the cell's centre, where its pins
+// sit at centre + offset (updatePinLocation, 2268-2276) for the WA kernels.  The first thread
+// also clears the trial's HPWL counter.
+__global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c, PlaceArgs pa) {
+    const int n = r.cell_end - r.cell_begin;
+    const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
+    if (pa.zero && blockIdx.x == 0 && threadIdx.x == 0) *pa.zero = 0ull;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int g = r.cell_begin + k;
+        int4 d = c.dbox[g];
+        const uint8_t f = c.flags[g];
+        const float2 a = pa.cur_slp[k], sm = pa.cur_sum[k], cu = pa.cur[k];
+        float2 nxt, nsl;
+        nesterov_point(a, sm, cu, pa.s, pa.coeff, d, lx, ly, ux, uy, nxt, nsl);
+        if (f & 4u) {
+            nxt = cu;
+            nsl = a;
+        }
+        pa.next[k] = nxt;
+        pa.next_slp[k] = nsl;
+        d = centre_box(d, nsl.x, nsl.y);
+        c.dbox[g] = d;
+        c.ctr[g] = make_int2((d.x + d.z) / 2, (d.y + d.w) / 2);
+    }
+}
+
 // V: the LDS tables' sum type.  32 bits when a chunk's sum in one bin cannot reach 2^32 (every
 // overlap, times its scale and target, is at most the bin's area, so kChunk * the largest bin area
 // bounds it; the host checks that), which halves the tables and doubles the blocks per CU; else 64.
@@ -196,7 +239,7 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
             int4 d = c.dbox[g];
             const uint8_t f = c.flags[g];
             const float sc = c.dscale[g];
-            float2 ns;
+            float2 ns = make_float2(0.0f, 0.0f);
             if (MODE == PLACE_INIT) {
                 const int ddx = d.z - d.x, ddy = d.w - d.y;
                 float tlx = rmax((float)d.x, lx), tly = rmax((float)d.y, ly);
@@ -210,25 +253,14 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
                 pa.cur_out[k] = ns;   // cur
             } else if (MODE == PLACE_CENTER) {
                 ns = pa.xy[k];
-            } else {
-                // computed for every cell and selected, so that the three reads are issued with
-                // the box's rather than after a branch on the flags; a locked cell (flag 4) holds
-                const float2 a = pa.cur_slp[k], sm = pa.cur_sum[k], cu = pa.cur[k];
-                const float nx = a.x + pa.s * sm.x;
-                const float ny = a.y + pa.s * sm.y;
-                const float sx = nx + pa.coeff * (nx - cu.x);
-                const float sy = ny + pa.coeff * (ny - cu.y);
-                const float hx = (float)(d.z - d.x) / 2.0f, hy = (float)(d.w - d.y) / 2.0f;
-                const bool locked = f & 4u;
-                pa.next[k] = locked ? cu : make_float2(inside(nx, hx, lx, ux), inside(ny, hy, ly, uy));
-                ns = locked ? a : make_float2(inside(sx, hx, lx, ux), inside(sy, hy, ly, uy));
-                pa.next_slp[k] = ns;
             }
-            d = centre_box(d, ns.x, ns.y);
-            c.dbox[g] = d;
-            // the cell's centre; its pins sit at centre + offset (updatePinLocation,
-            // nesterov.rs:2268-2276), which the WA kernels read from here
-            c.ctr[g] = make_int2((d.x + d.z) / 2, (d.y + d.w) / 2);
+            if (MODE != PLACE_STORED) {
+                d = centre_box(d, ns.x, ns.y);
+                c.dbox[g] = d;
+                // the cell's centre; its pins sit at centre + offset (updatePinLocation,
+                // nesterov.rs:2268-2276), which the WA kernels read from here
+                c.ctr[g] = make_int2((d.x + d.z) / 2, (d.y + d.w) / 2);
+            }
             int x0, x1, y0, y1;
             bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
             bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
@@ -661,6 +693,34 @@ __device__ __forceinline__ void wa_pin(int2 p, int lx, int ly, int ux, int uy, f
     }
 }
 
+// wa_pin that keeps the four exponentials in ev for the gradient pass (-1 where the term is
+// absent, its exponent at or below -300; an exponential itself is never negative).
+__device__ __forceinline__ void wa_pin_keep(int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float4& a, float4& b, float4& ev) {
+    const float e0 = (float)(lx - p.x) * cx, e1 = (float)(p.x - ux) * cx;
+    const float e2 = (float)(ly - p.y) * cy, e3 = (float)(p.y - uy) * cy;
+    ev = make_float4(-1.0f, -1.0f, -1.0f, -1.0f);
+    if (e0 > -300.0f) {
+        ev.x = fast_exp(e0);
+        a.x += ev.x;
+        a.y += (float)p.x * ev.x;
+    }
+    if (e1 > -300.0f) {
+        ev.y = fast_exp(e1);
+        a.z += ev.y;
+        a.w += (float)p.x * ev.y;
+    }
+    if (e2 > -300.0f) {
+        ev.z = fast_exp(e2);
+        b.x += ev.z;
+        b.y += (float)p.y * ev.z;
+    }
+    if (e3 > -300.0f) {
+        ev.w = fast_exp(e3);
+        b.z += ev.w;
+        b.w += (float)p.y * ev.w;
+    }
+}
+
 // getWireLengthGradientPinWA (nesterov.rs:2392): one exponential's term; the numerator and the
 // quotient in double, the result a float.
 __device__ __forceinline__ float wa_term_min(float S0, float S1, float v, float coef, float pc) {
@@ -685,6 +745,22 @@ __device__ __forceinline__ void pin_grad(const NetsDev& n, int q, int2 p, int lx
     if (e1 > -300.0f) gmax_x = wa_term_max(SX.z, SX.w, fast_exp(e1), cx, fpx);
     if (e2 > -300.0f) gmin_y = wa_term_min(SY.x, SY.y, fast_exp(e2), cy, fpy);
     if (e3 > -300.0f) gmax_y = wa_term_max(SY.z, SY.w, fast_exp(e3), cy, fpy);
+    float tx = gmin_x - gmax_x, ty = gmin_y - gmax_y;
+    tx *= w;
+    ty *= w;
+    n.pgrad[q] = make_float2(tx, ty);
+}
+
+// pin_grad from the exponentials wa_pin_keep kept.
+__device__ __forceinline__ void pin_grad_kept(const NetsDev& n, int q, int2 p, float cx, float cy, float4 SX, float4 SY, float w,
+                                              float4 ev) {
+    if (q < 0) return;
+    const float fpx = (float)p.x, fpy = (float)p.y;
+    float gmin_x = 0.0f, gmax_x = 0.0f, gmin_y = 0.0f, gmax_y = 0.0f;
+    if (ev.x >= 0.0f) gmin_x = wa_term_min(SX.x, SX.y, ev.x, cx, fpx);
+    if (ev.y >= 0.0f) gmax_x = wa_term_max(SX.z, SX.w, ev.y, cx, fpx);
+    if (ev.z >= 0.0f) gmin_y = wa_term_min(SY.x, SY.y, ev.z, cy, fpy);
+    if (ev.w >= 0.0f) gmax_y = wa_term_max(SY.z, SY.w, ev.w, cy, fpy);
     float tx = gmin_x - gmax_x, ty = gmin_y - gmax_y;
     tx *= w;
     ty *= w;
@@ -808,6 +884,7 @@ __global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, flo
 // across the box, sums and gradient passes (nets of more pins gather the rest again).  XCD x
 // takes the nets of slice x.
 constexpr int kRegPins = 8;
+constexpr int kKeepPins = 8;  // of those, the pins whose four exponentials are kept in registers
 template <bool SUMS>
 __global__ void __launch_bounds__(kBlock) k_wa_small(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
                                                      int write_rec) {
@@ -841,9 +918,14 @@ __global__ void __launch_bounds__(kBlock) k_wa_small(NetsDev n, CellsDev c, floa
         }
         if (SUMS) {
             float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 ev[kKeepPins > 0 ? kKeepPins : 1];
 #pragma unroll
-            for (int k = 0; k < kRegPins; ++k)
-                if (k < deg) wa_pin(pos[k], lx, ly, ux, uy, cx, cy, a, b);
+            for (int k = 0; k < kRegPins; ++k) {
+                if (k < deg) {
+                    if (k < kKeepPins) wa_pin_keep(pos[k], lx, ly, ux, uy, cx, cy, a, b, ev[k < kKeepPins ? k : 0]);
+                    else wa_pin(pos[k], lx, ly, ux, uy, cx, cy, a, b);
+                }
+            }
             for (int k = kRegPins; k < deg; ++k) wa_pin(pin_pos(n, c, s + k), lx, ly, ux, uy, cx, cy, a, b);
             if (write_rec) {
                 NetRec& R = n.rec[e];
@@ -854,7 +936,10 @@ __global__ void __launch_bounds__(kBlock) k_wa_small(NetsDev n, CellsDev c, floa
             const float wt = n.weight[e];
 #pragma unroll
             for (int k = 0; k < kRegPins; ++k)
-                if (k < deg) pin_grad(n, cq[k], pos[k], lx, ly, ux, uy, cx, cy, a, b, wt);
+                if (k < deg) {
+                    if (k < kKeepPins) pin_grad_kept(n, cq[k], pos[k], cx, cy, a, b, wt, ev[k < kKeepPins ? k : 0]);
+                    else pin_grad(n, cq[k], pos[k], lx, ly, ux, uy, cx, cy, a, b, wt);
+                }
             for (int k = kRegPins; k < deg; ++k) pin_grad(n, n.cpi[s + k], pin_pos(n, c, s + k), lx, ly, ux, uy, cx, cy, a, b, wt);
         }
         if (ux >= lx) h += (unsigned long long)((long long)ux - lx + (long long)uy - ly);
@@ -1292,8 +1377,8 @@ static void place_scatter(hipStream_t s, int mode, const RegionDev& r, const Cel
         if (r.narrow) hipLaunchKernelGGL((k_place_scatter<PLACE_CENTER, N>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
         else hipLaunchKernelGGL((k_place_scatter<PLACE_CENTER, W>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
     } else {
-        if (r.narrow) hipLaunchKernelGGL((k_place_scatter<PLACE_UPDATE, N>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
-        else hipLaunchKernelGGL((k_place_scatter<PLACE_UPDATE, W>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+        if (r.narrow) hipLaunchKernelGGL((k_place_scatter<PLACE_STORED, N>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+        else hipLaunchKernelGGL((k_place_scatter<PLACE_STORED, W>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
     }
 }
 
@@ -1314,9 +1399,8 @@ void launch_center_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c,
     place_scatter(s, PLACE_CENTER, r, c, pa);
 }
 
-void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,
-                           const float2* cur, float2* next, float2* next_slp, float step, float coeff, unsigned long long* global_adds,
-                           unsigned long long* zero) {
+static PlaceArgs update_args(const float2* cur_slp, const float2* cur_sum, const float2* cur, float2* next, float2* next_slp, float step,
+                             float coeff, unsigned long long* zero) {
     PlaceArgs pa{};
     pa.zero = zero;
     pa.cur_slp = cur_slp;
@@ -1326,8 +1410,28 @@ void launch_update_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c,
     pa.next_slp = next_slp;
     pa.s = step;
     pa.coeff = coeff;
+    return pa;
+}
+
+static int update_grid(const RegionDev& r) {
+    const int n = r.cell_end - r.cell_begin;
+    return std::max(1, std::min((n + kBlock - 1) / kBlock, 4096));
+}
+
+void launch_update(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum, const float2* cur,
+                   float2* next, float2* next_slp, float step, float coeff, unsigned long long* zero) {
+    if (r.cell_end <= r.cell_begin) {
+        if (zero) launch_set_u64(s, zero, 0ull);
+        return;
+    }
+    const PlaceArgs pa = update_args(cur_slp, cur_sum, cur, next, next_slp, step, coeff, zero);
+    hipLaunchKernelGGL(k_place_update, dim3(update_grid(r)), dim3(kBlock), 0, s, r, c, pa);
+}
+
+void launch_density_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, unsigned long long* global_adds) {
+    PlaceArgs pa{};
     pa.global_adds = global_adds;
-    place_scatter(s, PLACE_UPDATE, r, c, pa);
+    place_scatter(s, PLACE_STORED, r, c, pa);
 }
 
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials) {
@@ -1619,8 +1723,9 @@ int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap) {
     if (hipGraphGetNodes(g, nullptr, &n) != hipSuccess) return -1;
     std::vector<hipGraphNode_t> nodes(n);
     if (n && hipGraphGetNodes(g, nodes.data(), &n) != hipSuccess) return -1;
-    const void* f_scatter = reinterpret_cast<const void*>(k_place_scatter<PLACE_UPDATE, unsigned>);
-    const void* f_scatter_w = reinterpret_cast<const void*>(k_place_scatter<PLACE_UPDATE, unsigned long long>);
+    const void* f_update = reinterpret_cast<const void*>(k_place_update);
+    const void* f_scatter = reinterpret_cast<const void*>(k_place_scatter<PLACE_STORED, unsigned>);
+    const void* f_scatter_w = reinterpret_cast<const void*>(k_place_scatter<PLACE_STORED, unsigned long long>);
     const void* f_small = reinterpret_cast<const void*>(k_wa_small<true>);
     const void* f_big = reinterpret_cast<const void*>(k_wa_big<true>);
     const void* f_grad = reinterpret_cast<const void*>(k_gradient);
@@ -1630,14 +1735,15 @@ int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap) {
         if (hipGraphNodeGetType(nd, &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
         hipKernelNodeParams p{};
         if (hipGraphKernelNodeGetParams(nd, &p) != hipSuccess) return -1;
-        int kind = (p.func == f_scatter || p.func == f_scatter_w) ? TK_SCATTER
+        int kind = p.func == f_update                               ? TK_UPDATE
+                   : (p.func == f_scatter || p.func == f_scatter_w) ? TK_SCATTER
                    : p.func == f_small                              ? TK_WA_SMALL
                    : p.func == f_big                                ? TK_WA_BIG
                    : p.func == f_grad                               ? TK_GRADIENT
                                                                     : -1;
         if (kind < 0) continue;
         int begin = -1;
-        if (kind == TK_SCATTER || kind == TK_GRADIENT) {
+        if (kind == TK_UPDATE || kind == TK_SCATTER || kind == TK_GRADIENT) {
             if (!p.kernelParams || !p.kernelParams[0]) return -1;
             begin = static_cast<const RegionDev*>(p.kernelParams[0])->cell_begin;  // the region of the node
         }
@@ -1651,18 +1757,22 @@ int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap) {
     return m;
 }
 
-hipError_t set_scatter_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const float2* cur_slp,
-                            const float2* cur_sum, const float2* cur, float2* next, float2* next_slp, float step, float coeff,
-                            unsigned long long* global_adds, unsigned long long* zero) {
+hipError_t set_update_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const float2* cur_slp,
+                           const float2* cur_sum, const float2* cur, float2* next, float2* next_slp, float step, float coeff,
+                           unsigned long long* zero) {
+    PlaceArgs pa = update_args(cur_slp, cur_sum, cur, next, next_slp, step, coeff, zero);
+    RegionDev rr = r;
+    CellsDev cc = c;
+    void* args[] = {&rr, &cc, &pa};
+    hipKernelNodeParams p = n.p;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    return hipGraphExecKernelNodeSetParams(e, n.node, &p);
+}
+
+hipError_t set_scatter_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c,
+                            unsigned long long* global_adds) {
     PlaceArgs pa{};
-    pa.zero = zero;
-    pa.cur_slp = cur_slp;
-    pa.cur_sum = cur_sum;
-    pa.cur = cur;
-    pa.next = next;
-    pa.next_slp = next_slp;
-    pa.s = step;
-    pa.coeff = coeff;
     pa.global_adds = global_adds;
     RegionDev rr = r;
     CellsDev cc = c;

@@ -57,8 +57,8 @@ struct Region {
 };
 
 // Device time per kernel family (profiling mode).
-enum Fam { F_SCATTER, F_BINS, F_DCT, F_WA, F_GRAD, F_REDUCE, F_COUNT };
-const char* kFamName[F_COUNT] = {"update_scatter", "bin_density", "dct_solve", "wa_forward", "gradient", "reduce"};
+enum Fam { F_UPDATE, F_SCATTER, F_BINS, F_DCT, F_WA, F_GRAD, F_REDUCE, F_COUNT };
+const char* kFamName[F_COUNT] = {"update", "scatter", "bin_density", "dct_solve", "wa_forward", "gradient", "reduce"};
 
 }  // namespace
 
@@ -147,6 +147,13 @@ struct gpl_ctx {
     int ev_fam[32];
     double fam_ms[F_COUNT] = {};
     int fam_n[F_COUNT] = {};
+    // host time of the graph-launched trials (ms): node arguments, the launch call, the wait, and
+    // the host's work between one trial's return and the next's start
+    enum { H_PARAMS, H_LAUNCH, H_WAIT, H_BETWEEN, H_COUNT };
+    double host_ms[H_COUNT] = {};
+    int host_n = 0;
+    std::chrono::steady_clock::time_point host_last{};
+    bool host_have_last = false;
 
     template <typename T>
     T* dalloc(size_t n) {
@@ -450,15 +457,16 @@ struct gpl_ctx {
         return nullptr;
     }
 
-    // nesterovUpdateCoordinates for every region (the scatter); the field follows in field(), the
-    // WA kernels in wa_forward(false, true) (the counter cleared here).
+    // nesterovUpdateCoordinates for every region (new points, boxes and centres); field() then
+    // scatters the boxes and solves on the side stream while wa_forward(false, true) runs the WA
+    // kernels on the centres (the HPWL counter cleared here).
     void update_coordinates(float coeff) {
         for (size_t b = 0; b < rg.size(); ++b) {
             Region& r = rg[b];
             if (r.is_converged) continue;
-            mark(F_SCATTER);
-            gplk::launch_update_scatter(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
-                                        r.step_length, coeff, d_global_adds, zero_for(b));
+            mark(F_UPDATE);
+            gplk::launch_update(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
+                                r.step_length, coeff, zero_for(b));
             mark_end();
         }
         if (overlap && !profiling) check(hipEventRecord(ev_fork, stream), "event");
@@ -475,6 +483,9 @@ struct gpl_ctx {
         for (size_t b = 0; b < rg.size(); ++b) {
             Region& r = rg[b];
             if (r.is_converged) continue;
+            mark(F_SCATTER);
+            gplk::launch_density_scatter(stream, r.dev, cells, d_global_adds);
+            mark_end();
             bins_and_finish(r, int(b), false);  // finished with the trial's gradient sums
             solve(r);
         }
@@ -569,11 +580,11 @@ struct gpl_ctx {
             drop_graph();
             return false;
         }
-        tnodes.resize(8 + 2 * rg.size());
+        tnodes.resize(8 + 3 * rg.size());
         int m = gplk::find_trial_nodes(tgraph, tnodes.data(), int(tnodes.size()));
         int live = 0;
         for (auto& r : rg) live += r.is_converged ? 0 : 1;
-        if (m != 2 * live + (nets.num_small > 0 ? 1 : 0) + (nets.num_small < nnets ? 1 : 0)) {
+        if (m != 3 * live + (nets.num_small > 0 ? 1 : 0) + (nets.num_small < nnets ? 1 : 0)) {
             drop_graph();
             return false;
         }
@@ -597,13 +608,20 @@ struct gpl_ctx {
                 return false;
             }
         }
+        using clk = std::chrono::steady_clock;
+        auto ms_since = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        const clk::time_point t0 = clk::now();
+        if (host_have_last) host_ms[H_BETWEEN] += ms_since(host_last, t0);
         const int nan = inject_now() ? 1 : 0;
         for (auto& n : tnodes) {
             hipError_t e = hipSuccess;
-            if (n.kind == gplk::TK_SCATTER) {
+            if (n.kind == gplk::TK_UPDATE) {
                 Region& r = *region_of(n.cell_begin);
-                e = gplk::set_scatter_node(texec, n, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT],
-                                           r.v[S_NEXT_SLP], r.step_length, coeff, d_global_adds, zero_for(size_t(&r - rg.data())));
+                e = gplk::set_update_node(texec, n, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT],
+                                          r.v[S_NEXT_SLP], r.step_length, coeff, zero_for(size_t(&r - rg.data())));
+            } else if (n.kind == gplk::TK_SCATTER) {
+                Region& r = *region_of(n.cell_begin);
+                e = gplk::set_scatter_node(texec, n, r.dev, cells, d_global_adds);
             } else if (n.kind == gplk::TK_GRADIENT) {
                 Region& r = *region_of(n.cell_begin);
                 e = gplk::set_gradient_node(texec, n, r.dev, cells, nets, wl_coef_x, wl_coef_y, r.density_penalty, r.v[S_NEXT_SLP + 1],
@@ -614,9 +632,18 @@ struct gpl_ctx {
             }
             check(e, "graph node");
         }
+        const clk::time_point t1 = clk::now();
         check(hipGraphLaunch(texec, stream), "graph launch");
+        const clk::time_point t2 = clk::now();
         check(hipStreamSynchronize(stream), "sync");
+        const clk::time_point t3 = clk::now();
         gradients_read(0, int(rg.size()), SLOT_NEXT, true);
+        host_ms[H_PARAMS] += ms_since(t0, t1);
+        host_ms[H_LAUNCH] += ms_since(t1, t2);
+        host_ms[H_WAIT] += ms_since(t2, t3);
+        ++host_n;
+        host_last = clk::now();
+        host_have_last = true;
         return true;
     }
 
@@ -1193,10 +1220,11 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         }
         for (int32_t i = 0; i < N; ++i) c->cell_new_of[c->cell_old_of[i]] = i;
         auto newc = [&](int32_t old) { return old >= 0 ? c->cell_new_of[old] : -1; };
-        // nets sorted by the XCD slice of their first pin's cell, then by windows of 4096 cells of
-        // it, then by degree (so a wave's lanes loop equally), then by the first cell: the nets in
-        // flight at once touch one window's cells, whose centres and gradient terms stay in L2;
-        // nets above 128 pins last, one block each
+        // nets sorted by windows of 4096 cells of their first pin's cell, then by degree (so that
+        // a wave's lanes loop equally), then by the first cell: the nets in flight at once touch
+        // one window's cells, whose centres and gradient terms stay in L2; nets above 128 pins
+        // last, one block each.  The small nets are then cut into 8 contiguous slices of equal pin
+        // counts, one per XCD.
         std::vector<int32_t> order(E);
         std::iota(order.begin(), order.end(), 0);
         auto deg = [&](int32_t e) { return p->net_pin_start[e + 1] - p->net_pin_start[e]; };
@@ -1211,16 +1239,9 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         std::vector<int32_t> fc(E);
         for (int32_t e = 0; e < E; ++e) fc[e] = first_cell(e);
         const int32_t kBig = 128;
-        // slice of a small net: the eighth of the cell order its first cell lies in; big nets last
-        auto slice_of = [&](int32_t e) -> int32_t {
-            if (deg(e) > kBig) return 8;
-            if (fc[e] == INT32_MAX || N == 0) return 7;
-            return int32_t((int64_t)fc[e] * 8 / N);
-        };
-        std::vector<int32_t> sl(E);
-        for (int32_t e = 0; e < E; ++e) sl[e] = slice_of(e);
         std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-            if (sl[a] != sl[b]) return sl[a] < sl[b];
+            const bool ba = deg(a) > kBig, bb = deg(b) > kBig;
+            if (ba != bb) return bb;
             if ((fc[a] >> 12) != (fc[b] >> 12)) return fc[a] < fc[b];
             return deg(a) != deg(b) ? deg(a) < deg(b) : fc[a] < fc[b];
         });
@@ -1230,10 +1251,14 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         int32_t num_small = 0;
         while (num_small < E && deg(order[num_small]) <= kBig) ++num_small;
         {
+            int64_t small_pins = 0;
+            for (int32_t i = 0; i < num_small; ++i) small_pins += deg(order[i]);
             int32_t at = 0, most = 0;
+            int64_t pins = 0;
             for (int x = 0; x < 8; ++x) {
                 c->nets.slice[x] = at;
-                while (at < num_small && sl[order[at]] == x) ++at;
+                const int64_t goal = small_pins * (x + 1) / 8;
+                while (at < num_small && (x == 7 || pins < goal)) pins += deg(order[at++]);
                 most = std::max(most, at - c->nets.slice[x]);
             }
             c->nets.slice[8] = at;
@@ -1762,8 +1787,9 @@ int32_t gpl_nesterov_step(gpl_ctx* c, int32_t region, float step, float coeff, f
     if (bad_region(c, region)) return GPL_ERR_ARG;
     try {
         Region& r = c->rg[region];
-        gplk::launch_update_scatter(c->stream, r.dev, c->cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
-                                    step, coeff, c->d_global_adds, nullptr);
+        gplk::launch_update(c->stream, r.dev, c->cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP], step,
+                            coeff, nullptr);
+        gplk::launch_density_scatter(c->stream, r.dev, c->cells, c->d_global_adds);
         c->bins_and_finish(r, region);
         c->sync_stats();
         if (next) c->download_region_vec(region, r.v[S_NEXT], next);
@@ -1822,6 +1848,9 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap) {
         out[n].launches = c->fam_n[f];
         out[n].ms = c->fam_ms[f];
     }
+    // the graph-launched trials' host time (after the HBM atomic count: callers that take a fixed
+    // number of entries see the kernel families first)
+    static const char* kHostName[] = {"host_params", "host_launch", "host_wait", "host_between"};
     if (n < cap) {
         // the scatter's atomic adds into the HBM bins since the context began (the count is in
         // `ms`), and the re-sorts (in `launches`)
@@ -1833,6 +1862,12 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap) {
             out[n].ms = double(ga);
             ++n;
         }
+    }
+    for (int h = 0; h < gpl_ctx::H_COUNT && n < cap && c->host_n > 0; ++h, ++n) {
+        std::memset(&out[n], 0, sizeof out[n]);
+        std::snprintf(out[n].name, sizeof out[n].name, "%s", kHostName[h]);
+        out[n].launches = c->host_n;
+        out[n].ms = c->host_ms[h];
     }
     return n;
 }

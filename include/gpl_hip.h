@@ -377,9 +377,13 @@ int32_t gpl_objective_backward(gpl_ctx* c, const float* xy, float wl_coef, float
  * storage order must not change the results).  kind 0 clears it. */
 int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter);
 
-/* Device timing of the kernels of the last run (milliseconds summed per kernel family and
- * launch counts), for the roofline report.  names: scatter, bin_density, dct, wa_forward,
- * gradient, update, reduce.  Returns the number of families written. */
+/* Device timing of the kernels while profiling (gpl_set_profiling; milliseconds summed per
+ * kernel family and launch counts), for the roofline report: update_scatter, bin_density,
+ * dct_solve, wa_forward, gradient, reduce; then "scatter_global_adds" (the scatter's atomic adds
+ * into HBM in `ms`, the re-sorts in `launches`); then, once trials have run as graphs, the host's
+ * time per phase of those trials: host_params (node arguments), host_launch (the launch call),
+ * host_wait (the wait for the trial), host_between (from one trial's return to the next's
+ * start), with the trial count in `launches`.  Returns the number of entries written. */
 typedef struct gpl_kernel_time {
     char name[24];
     int32_t launches;
