@@ -142,6 +142,7 @@ SIGNATURES = [
     ("gpl_set_profiling", i32, [C.c_void_p, i32]),
     ("gpl_util_mt19937", None, [u32, i32, P(u32)]),
     ("gpl_util_uniform_int", None, [u32, i32, P(i32), P(i32), P(i32)]),
+    ("gpl_util_normal", u32, [u32, C.c_double, C.c_double, i32, P(f64)]),
     ("gpl_util_wavenumber_x", f32, [i32, i32]),
     ("gpl_util_wavenumber_y", f32, [i32, i32, f32, f32]),
     ("gpl_util_union_area", i64, [P(i32), i32]),

@@ -6,6 +6,7 @@
 // Every step follows the reference's rules and evaluation order, in code of its own.  Where this
 // build departs (the normal variate's generator, the BiCGSTAB sums) the function says so.
 #include "host.h"
+#include "ziggurat_tables.h"
 
 #include <algorithm>
 #include <cmath>
@@ -112,14 +113,100 @@ double uniform01(Mt19937& eng) {
     return (double(a) * 67108864.0 + double(b)) * (1.0 / 9007199254740992.0);
 }
 
-double normal(Mt19937& eng, double mean, double sigma) {
-    // Marsaglia's polar method; one of the pair is used so a draw has no carried state.
+// ------------------------------------------------------------------------------------------
+// boost::random::normal_distribution<double> on mt19937, as the reference draws it
+// (normal.rs:144-214, Boost 1.89): a 128-layer ziggurat of exp(-x^2/2) whose tail samples two
+// exponentials from a 256-layer ziggurat of exp(-x). The layer tables (ziggurat_tables.h) are
+// derived from the ziggurat's definition by gen_ziggurat.py. What must agree with the reference
+// is the count of engine words per draw and the double arithmetic on them.
+// ------------------------------------------------------------------------------------------
+namespace {
+
+struct ZigTables {
+    double nx[129], ny[129], ex[257], ey[257];
+    ZigTables() {
+        std::memcpy(nx, gplz::kNormalX, sizeof nx);
+        std::memcpy(ny, gplz::kNormalY, sizeof ny);
+        std::memcpy(ex, gplz::kExpX, sizeof ex);
+        std::memcpy(ey, gplz::kExpY, sizeof ey);
+    }
+};
+const ZigTables kZig;
+
+// Two engine words: the layer byte is the low 8 bits of the first; the fraction in [0, 1) is its
+// high 24 bits over 2^24 plus the second's low 29 bits, all over 2^29 (normal.rs:150-160).
+double layer_and_fraction(Mt19937& eng, uint32_t& layer) {
+    uint32_t w = eng.next();
+    layer = w & 0xffu;
+    double r = double(w >> 8) * (1.0 / 16777216.0);
+    w = eng.next();
+    r += double(w & 0x1fffffffu);
+    r *= 0.5 / 268435456.0;
+    return r;
+}
+
+// One engine word over 2^32 (uniform_01 on a 32-bit engine, normal.rs:162-166).
+double one_word_uniform(Mt19937& eng) { return double(eng.next()) * (1.0 / 4294967296.0); }
+
+// Unit exponential (normal.rs:196-214): a point under layer i's rectangle is accepted at once;
+// layer 0 overflows into the tail, which is the body again shifted by x[1]; otherwise one more
+// word picks a height, tested against the chord, the tangent and then exp(-x) itself.
+double unit_exponential(Mt19937& eng) {
+    const double* X = kZig.ex;
+    const double* Y = kZig.ey;
+    double shift = 0.0;
     for (;;) {
-        double u = 2.0 * uniform01(eng) - 1.0, v = 2.0 * uniform01(eng) - 1.0;
-        double s = u * u + v * v;
-        if (s > 0.0 && s < 1.0) return u * std::sqrt(-2.0 * std::log(s) / s) * sigma + mean;
+        uint32_t i;
+        double r = layer_and_fraction(eng, i);
+        double x = r * X[i];
+        if (x < X[i + 1]) return shift + x;
+        if (i == 0) {
+            shift += X[1];
+            continue;
+        }
+        double t = one_word_uniform(eng);
+        double y = Y[i] + t * (Y[i + 1] - Y[i]);
+        double over_chord = (X[i] - X[i + 1]) * t - (X[i] - x);
+        double over_tangent = y - (Y[i + 1] + (X[i + 1] - x) * Y[i + 1]);
+        if (over_chord < 0.0 && (over_tangent < 0.0 || y < std::exp(-x))) return x + shift;
     }
 }
+
+// The unit normal (normal.rs:168-194): the layer byte's bit 0 is the sign, bits 1..7 the layer.
+// Layer 0 beyond the rectangle draws the tail R + x with x ~ Exp(R), y ~ Exp(1) until 2y > x^2.
+// In a wedge, the chord bounds the density from above where the curve is convex (x >= 1) and the
+// tangent where it is concave; the other one bounds it from below.
+double unit_normal(Mt19937& eng) {
+    const double* X = kZig.nx;
+    const double* Y = kZig.ny;
+    for (;;) {
+        uint32_t b;
+        double r = layer_and_fraction(eng, b);
+        double sign = double(int(b & 1u) * 2 - 1);
+        uint32_t i = b >> 1;
+        double x = r * X[i];
+        if (x < X[i + 1]) return x * sign;
+        if (i == 0) {
+            const double R = X[1];
+            for (;;) {
+                double tx = unit_exponential(eng) / R;
+                double ty = unit_exponential(eng) / 1.0;
+                if (2.0 * ty > tx * tx) return (tx + R) * sign;
+            }
+        }
+        double t = one_word_uniform(eng);
+        double y = Y[i] + t * (Y[i + 1] - Y[i]);
+        double chord = (X[i] - X[i + 1]) * t - (X[i] - x);
+        double tangent = y - (Y[i] + (X[i] - x) * Y[i] * X[i]);
+        double upper = X[i] >= 1.0 ? chord : tangent;
+        double lower = X[i] >= 1.0 ? tangent : chord;
+        if (upper < 0.0 && (lower < 0.0 || y < std::exp(-(x * x / 2.0)))) return x * sign;
+    }
+}
+
+}  // namespace
+
+double normal(Mt19937& eng, double mean, double sigma) { return unit_normal(eng) * sigma + mean; }
 
 // ------------------------------------------------------------------------------------------
 // The synthetic generator
@@ -1545,6 +1632,12 @@ void gpl_util_mt19937(uint32_t seed, int32_t n, uint32_t* out) {
 void gpl_util_uniform_int(uint32_t seed, int32_t n, const int32_t* lo, const int32_t* hi, int32_t* out) {
     gpl::Mt19937 e(seed);
     for (int32_t i = 0; i < n; ++i) out[i] = gpl::uniform_int(e, lo[i], hi[i]);
+}
+
+uint32_t gpl_util_normal(uint32_t seed, double mean, double sigma, int32_t n, double* out) {
+    gpl::Mt19937 e(seed);
+    for (int32_t i = 0; i < n; ++i) out[i] = gpl::normal(e, mean, sigma);
+    return e.next();
 }
 
 float gpl_util_wavenumber_x(int32_t i, int32_t nx) { return gpl::wavenumber_x(i, nx); }

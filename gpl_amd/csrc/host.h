@@ -48,8 +48,8 @@ private:
 };
 
 int32_t uniform_int(Mt19937& eng, int32_t lo, int32_t hi);
-// A unit-normal variate scaled and shifted: the polar method on two 53-bit uniforms drawn from
-// the engine (the reference draws boost's ziggurat, normal.rs:144; the draws differ).
+// One draw of boost's normal_distribution<double>(mean, sigma) from the engine: its ziggurat unit
+// normal times sigma plus mean (normal.rs:120-124), word for word as the reference draws it.
 double normal(Mt19937& eng, double mean, double sigma);
 double uniform01(Mt19937& eng);  // 53-bit uniform in [0, 1)
 

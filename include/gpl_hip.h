@@ -144,11 +144,15 @@ int32_t gpl_design_get_info(const gpl_design* d, gpl_design_info* out);
 
 /* Host helpers the construction uses, exported so their rules can be pinned by the reference's
  * known-answer tests: mt19937 outputs (normal.rs:38-120), boost uniform_int draws
- * (normal.rs:127; n draws from one engine, draw i in [lo[i], hi[i]]), the x87 wavenumbers (nesterov.rs:351, 359), the fixed-area union
+ * (normal.rs:127; n draws from one engine, draw i in [lo[i], hi[i]]), boost normal draws
+ * (normal.rs:120-214: n draws of normal(mean, sigma) from mt19937(seed), each with a fresh
+ * distribution as the perturbation draws them; returns the engine's next word after them, so
+ * the count of words the draws took is checked too), the x87 wavenumbers (nesterov.rs:351, 359), the fixed-area union
  * (nesterov.rs:1901) and roundDownToPowerOfTwo (nesterov.rs:771).  `rects` holds 4 int32 per
  * rectangle (lx, ly, ux, uy). */
 void gpl_util_mt19937(uint32_t seed, int32_t n, uint32_t* out);
 void gpl_util_uniform_int(uint32_t seed, int32_t n, const int32_t* lo, const int32_t* hi, int32_t* out);
+uint32_t gpl_util_normal(uint32_t seed, double mean, double sigma, int32_t n, double* out);
 float gpl_util_wavenumber_x(int32_t i, int32_t nx);
 float gpl_util_wavenumber_y(int32_t i, int32_t ny, float bin_x, float bin_y);
 int64_t gpl_util_union_area(const int32_t* rects, int32_t n);

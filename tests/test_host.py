@@ -54,6 +54,39 @@ def test_uniform_int_draws_as_boost_does(kats):
     assert list(out) == want
 
 
+def test_normal_draws_match_boost_bit_for_bit(kats):
+    """The perturbation's normal: the reference's 16 draws of Boost 1.89 (one through a wedge) as
+    bit patterns, and the engine's next word after 1000 draws (any draw taking a wrong count of
+    engine words, the tail and the density test included, shifts it)."""
+    import struct
+    k = kats["normal_seed1_bits"]
+    sigma = 0.3071963263271184 * 100.0
+    n = len(k["bits"])
+    out = (C.c_double * n)()
+    F.lib().gpl_util_normal(k["seed"], 0.0, sigma, n, out)
+    got = [struct.pack(">d", v).hex() for v in out]
+    assert got == k["bits"]
+    many = (C.c_double * 1000)()
+    assert F.lib().gpl_util_normal(k["seed"], 0.0, sigma, 1000, many) == k["engine_word_after_1000_draws"]
+
+
+def test_normal_draws_follow_the_unit_normal():
+    """Moments and tail mass of 400k draws (the ziggurat's tail and wedges included): mean 0,
+    variance sigma^2, P(|z| > 3.4426) (beyond the first layer's edge) as the normal's."""
+    import math
+    import numpy as np
+    n = 400_000
+    out = (C.c_double * n)()
+    F.lib().gpl_util_normal(7, 2.0, 3.0, n, out)
+    z = (np.frombuffer(out, np.float64) - 2.0) / 3.0
+    assert abs(z.mean()) < 5 / math.sqrt(n)
+    assert abs(z.var() - 1.0) < 0.01
+    tail = np.mean(np.abs(z) > 3.442619855896652)
+    want = math.erfc(3.442619855896652 / math.sqrt(2.0))
+    assert abs(tail - want) < 5 * math.sqrt(want / n)
+    assert np.mean(z > 0) == pytest.approx(0.5, abs=0.005)
+
+
 def test_wavenumbers_in_extended_precision(kats):
     L = F.lib()
     named = {"0": 0.0, "FRAC_PI_2": np.float32(math.pi / 2), "PI": np.float32(math.pi), "FRAC_PI_4": np.float32(math.pi / 4)}
