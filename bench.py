@@ -45,8 +45,8 @@ FAMILY_KERNELS = {
     "update": ["k_place_update"],
     "scatter": ["k_place_scatter<2, unsigned int>", "k_place_scatter<2, unsigned long long>"],
     "bin_density": ["k_bin_density"],
-    "dct_solve": ["k_rows<0>", "k_rows<1>", "k_rows<2>", "k_rows<3>", "k_transpose"],
-    "wa_forward": ["k_wa_small<true>", "k_wa_big<true>"],
+    "dct_solve": ["k_rows<0>", "k_rows<6>", "k_rows<3>"],
+    "wa_forward": ["k_wa_tiles<true>", "k_wa_big<true>"],
     "gradient": ["k_gradient"],
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c3.txt")
@@ -180,6 +180,52 @@ def summarize(paths):
         print()
 
 
+def timeline(path):
+    """Median offsets (us from the trial's first kernel) of each kernel of a back-tracking trial,
+    from a rocprofv3 --kernel-trace CSV of the timed (graph) path: a trial runs from
+    k_place_update to k_finish; the gap to the next trial is the host's turn (statistics copy,
+    decisions, graph update, launch).  The last 35 complete trials but 5 are used."""
+    import csv
+    import statistics as st
+    from collections import defaultdict
+
+    def short(name):
+        return name.split("(")[0].replace("void ", "").replace("gplk::", "").strip()
+
+    with open(path, newline="") as f:
+        ev = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), short(r["Kernel_Name"]), r["Queue_Id"])
+                    for r in csv.DictReader(f))
+    trials, cur = [], None
+    for e in ev:
+        if e[2] == "k_place_update":
+            if cur:
+                trials.append(cur)
+            cur = [e]
+        elif cur is not None:
+            cur.append(e)
+    trials = [t for t in trials if any(e[2] == "k_finish" for e in t)][-40:-5]
+    span, gaps, at = [], [], defaultdict(list)
+    for i, t in enumerate(trials):
+        t0 = t[0][0]
+        t1 = next(e[1] for e in t if e[2] == "k_finish")
+        span.append(t1 - t0)
+        if i + 1 < len(trials):
+            gaps.append(trials[i + 1][0][0] - t1)
+        seen = defaultdict(int)
+        for e in t:
+            if e[1] <= t1:
+                at[(e[2], seen[e[2]])].append((e[0] - t0, e[1] - t0, e[3]))
+                seen[e[2]] += 1
+    busy = [sum(e[1] - e[0] for e in t if e[1] <= next(x[1] for x in t if x[2] == "k_finish")) for t in trials]
+    print(f"# {os.path.basename(path)}: {len(trials)} trials")
+    print(f"trial span (k_place_update start to k_finish end): median {st.median(span) / 1e3:.1f} us; "
+          f"kernel-busy {st.median(busy) / 1e3:.1f} us; host turn between trials: median {st.median(gaps) / 1e3:.1f} us")
+    print(f"{'kernel':44s} {'start_us':>9s} {'end_us':>9s} {'dur_us':>9s} {'queue':>6s}")
+    for (k, n), v in sorted(at.items(), key=lambda kv: st.median(x[0] for x in kv[1])):
+        print(f"{(k + (f' #{n + 1}' if n else ''))[:44]:44s} {st.median(x[0] for x in v) / 1e3:9.1f} "
+              f"{st.median(x[1] for x in v) / 1e3:9.1f} {st.median(x[1] - x[0] for x in v) / 1e3:9.1f} {v[0][2]:>6s}")
+
+
 def batched(cfg, k):
     """K independent placements (seeds 1..K: their own perturbations) of one design: run one
     after another, then at once from K host threads (one context each); one JSON line."""
@@ -255,6 +301,9 @@ def main():
                          "50k-cell design) placed one after another, then all at once from concurrent contexts")
     ap.add_argument("--summarize", nargs="+", metavar="CSV",
                     help="print a text summary of rocprofv3 CSV output (kernel_stats.csv or counter_collection.csv) and exit")
+    ap.add_argument("--timeline", metavar="KERNEL_TRACE_CSV",
+                    help="print the median kernel offsets within a back-tracking trial from a rocprofv3 "
+                         "--kernel-trace CSV of the timed (graph) path, and exit")
     ap.add_argument("--traffic-table", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="print HBM bytes per launch of each kernel family from rocprofv3 --pmc runs of the loop "
                          "and of --copy-probe-mib 256, and exit")
@@ -262,6 +311,9 @@ def main():
 
     if args.summarize:
         summarize(args.summarize)
+        return
+    if args.timeline:
+        timeline(args.timeline)
         return
     if args.traffic_table:
         traffic_table(*args.traffic_table, copy_mib=256)

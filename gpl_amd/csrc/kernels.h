@@ -87,11 +87,13 @@ struct NetsDev {
     int32_t* cpi;           // per net-pin: its cell-pin index, or -1 (renumbered by a re-sort)
     float2* pgrad;          // per cell-pin: the pin's weighted WA gradient, written by the WA kernel
     NetRec* rec;            // box and sums per net, written when asked for (the kernel-level entry)
-    int32_t num_small;      // nets [0, num_small) take one thread each; the rest a block each
-    // the small nets in 8 contiguous slices of equal pin counts, [slice[x], slice[x+1]) each (the
-    // kernel gives slice x to XCD x)
+    int32_t num_small;      // nets [0, num_small) go by tiles; the rest take a block each
+    // the small nets in 8 contiguous slices of equal pin counts, slice x cut into tiles
+    // [tile_slice[x], tile_slice[x+1]) of consecutive nets (the kernel gives slice x to XCD x)
     int32_t slice[9];
-    int32_t slice_blocks;   // workgroups per slice
+    const int32_t* tile;    // tile t holds nets [tile[t], tile[t+1])
+    int32_t tile_slice[9];
+    int32_t tile_blocks;    // the most tiles of a slice: workgroups per slice
 };
 
 // Per-trial FP64 reductions of one region, finished into this struct by k_finish.
@@ -105,6 +107,9 @@ struct Sums {
 
 constexpr int kPartials = 2048;  // per-block partial slots per quantity
 constexpr int kPlaceChunk = 512;  // cells per block iteration of the scatter family
+constexpr int kTilePins = 2048;   // pins of a WA tile (at least the largest small net, 128)
+constexpr int kTileNets = 512;    // nets of a WA tile
+static_assert(kTilePins >= 128 && kTilePins % 256 == 0, "a WA tile holds any small net, staged by 256 threads");
 
 // ---- launchers (stream-ordered, asynchronous) -----------------------------------------------
 void launch_zero_bins(hipStream_t s, const RegionDev& r);

@@ -384,7 +384,14 @@ __global__ void k_bin_density(RegionDev r, double* partials) {
 //   DST-III  S[k] = sum_{j=1..L} A[j] sin(pi j (k+1/2) / L), A[L] in a[0]
 // ---------------------------------------------------------------------------------------------
 
-enum RowMode { ROW_DCT2 = 0, ROW_DCT2_SCALE = 1, ROW_POISSON = 2, ROW_INV_FIELDS = 3, ROW_INV_PHI = 4 };
+enum RowMode {
+    ROW_DCT2 = 0,
+    ROW_DCT2_SCALE = 1,
+    ROW_POISSON = 2,
+    ROW_INV_FIELDS = 3,
+    ROW_INV_PHI = 4,
+    ROW_SPECTRUM_POISSON = 6
+};
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
@@ -535,9 +542,11 @@ struct RowArgs {
     const float* wcol;   // wavenumber of the element index
     double scale;
     int want_phi;
+    int transpose_out;   // outputs written transposed: out[k * nrows + row]
 };
 
-// Row transforms, L/8 threads per row, 256 / (L/8) rows per block.  Modes:
+// Row transforms, L/8 threads per row, 256 / (L/8) rows per block (rows of one XCD contiguous,
+// so that the blocks writing the same lines of a transposed output share an L2).  Modes:
 //  ROW_DCT2        out0[row] = DCT-II(in0[row])
 //  ROW_DCT2_SCALE  the same, then the halving of row/column 0 and x 4/(nx ny) (nesterov.rs:415-424)
 //  ROW_POISSON     in0 = scaled spectrum rows A[u][v] (u = row): Ex = p wx[u], Ey = p wy[v],
@@ -545,10 +554,13 @@ struct RowArgs {
 //                  DST-III(Ey), out2 = DCT-III(p) when want_phi
 //  ROW_INV_FIELDS  out0 = DST-III(in0), out1 = DCT-III(in1)
 //  ROW_INV_PHI     out0 = DCT-III(in0)
+//  ROW_SPECTRUM_POISSON  ROW_DCT2_SCALE kept in LDS, then ROW_POISSON on it
+// With transpose_out the outputs go out transposed, so that the next pass reads rows.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     extern __shared__ float2 lds2[];
     const int T = a.T;
+    const int rpb = blockDim.x / T;
     const int sub = threadIdx.x / T;
     RowCtx R;
     R.L = a.L;
@@ -561,47 +573,57 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     R.dw = a.dw;
     R.A = lds2 + (size_t)sub * a.L;
     R.B = R.A + R.h;
-    const int row = blockIdx.x * (blockDim.x / T) + sub;
+    // ROW_SPECTRUM_POISSON: one more row of L floats per row (the spectrum), after the FFT buffers
+    float* extra = reinterpret_cast<float*>(lds2 + (size_t)rpb * a.L) + (size_t)sub * a.L;
+    const int lb = gridDim.x % kXcds == 0 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int row = lb * rpb + sub;
     const bool active = row < a.nrows;
     const int L = a.L;
     const size_t off = (size_t)(active ? row : 0) * L;
+    const int nrows = a.nrows;
+    const bool tr = a.transpose_out != 0;
+    auto put = [&](float* o, int k, float v) {
+        if (tr) o[(size_t)k * nrows + row] = v;
+        else o[off + k] = v;
+    };
     if (MODE == ROW_DCT2 || MODE == ROW_DCT2_SCALE) {
-        float* o = a.out0 + off;
         const double scale = a.scale;
         dct2_row(R, active, a.in0 + off, [&](int k, float v) {
             if (MODE == ROW_DCT2_SCALE) {
                 if ((row >= 1 && k == 0) || (row == 0 && k >= 1)) v = (float)((double)v * 0.5);
                 v = (float)((double)v * scale);
             }
-            o[k] = v;
+            put(a.out0, k, v);
         });
-    } else if (MODE == ROW_POISSON) {
-        const float wu = active ? a.wrow[row] : 0.0f, wu2 = wu * wu;
+    } else if (MODE == ROW_POISSON || MODE == ROW_SPECTRUM_POISSON) {
         const float* A = a.in0 + off;
+        if (MODE == ROW_SPECTRUM_POISSON) {
+            const double scale = a.scale;
+            dct2_row(R, active, a.in0 + off, [&](int k, float v) {
+                if ((row >= 1 && k == 0) || (row == 0 && k >= 1)) v = (float)((double)v * 0.5);
+                v = (float)((double)v * scale);
+                extra[k] = v;
+            });
+            __syncthreads();
+            A = extra;
+        }
+        const float wu = active ? a.wrow[row] : 0.0f, wu2 = wu * wu;
         auto pot = [&](int v) -> float {
             if (row == 0 && v == 0) return 0.0f;
             float wv = a.wcol[v];
             return A[v] / (wu2 + wv * wv);
         };
-        float* o0 = a.out0 + off;
-        float* o1 = a.out1 + off;
-        dct3_row(R, active, [&](int v) { return pot(v) * wu; }, false, [&](int j, float y) { o0[j] = y; });
-        dct3_row(R, active, [&](int v) { return pot(v) * a.wcol[v]; }, true, [&](int j, float y) { o1[j] = y; });
-        if (a.want_phi) {
-            float* o2 = a.out2 + off;
-            dct3_row(R, active, pot, false, [&](int j, float y) { o2[j] = y; });
-        }
+        dct3_row(R, active, [&](int v) { return pot(v) * wu; }, false, [&](int j, float y) { put(a.out0, j, y); });
+        dct3_row(R, active, [&](int v) { return pot(v) * a.wcol[v]; }, true, [&](int j, float y) { put(a.out1, j, y); });
+        if (a.want_phi) dct3_row(R, active, pot, false, [&](int j, float y) { put(a.out2, j, y); });
     } else if (MODE == ROW_INV_FIELDS) {
         const float* X = a.in0 + off;
         const float* Y = a.in1 + off;
-        float* o0 = a.out0 + off;
-        float* o1 = a.out1 + off;
-        dct3_row(R, active, [&](int v) { return X[v]; }, true, [&](int j, float y) { o0[j] = y; });
-        dct3_row(R, active, [&](int v) { return Y[v]; }, false, [&](int j, float y) { o1[j] = y; });
+        dct3_row(R, active, [&](int v) { return X[v]; }, true, [&](int j, float y) { put(a.out0, j, y); });
+        dct3_row(R, active, [&](int v) { return Y[v]; }, false, [&](int j, float y) { put(a.out1, j, y); });
     } else if (MODE == ROW_INV_PHI) {
         const float* X = a.in0 + off;
-        float* o0 = a.out0 + off;
-        dct3_row(R, active, [&](int v) { return X[v]; }, false, [&](int j, float y) { o0[j] = y; });
+        dct3_row(R, active, [&](int v) { return X[v]; }, false, [&](int j, float y) { put(a.out0, j, y); });
     }
 }
 
@@ -879,78 +901,97 @@ __global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, flo
     if (threadIdx.x == 0 && ux >= lx) atomicAdd(hpwl, (unsigned long long)((long long)ux - lx + (long long)uy - ly));
 }
 
-// k_wa_small: one net (below num_small) per thread, its pins in the net's order (the
-// reference's float sums, bit for bit), the first kRegPins pin positions held in registers
-// across the box, sums and gradient passes (nets of more pins gather the rest again).  XCD x
-// takes the nets of slice x.
-constexpr int kRegPins = 8;
-constexpr int kKeepPins = 8;  // of those, the pins whose four exponentials are kept in registers
+// k_wa_tiles: the small nets (below num_small) in tiles of consecutive nets (at most kTilePins
+// pins and kTileNets nets each; XCD x takes the tiles of slice x).  A block first stages its
+// tile's pin positions (and cell-pin indices) in LDS, pin-parallel: the pin records are read
+// coalesced and every cell-centre gather of the tile is in flight at once.  Then one net per
+// thread reads its pins from LDS in the net's order: the box, the four shifted exponentials
+// summed in float in pin order (the reference's sums, bit for bit), the first kKeepPins pins'
+// exponentials kept in registers for the gradient pass, each pin's weighted gradient term stored
+// at its cell-pin index.  The nets of a tile have close degrees (nets are sorted by degree within
+// their window), so a wave's lanes loop alike.
+constexpr int kKeepPins = 8;  // pins whose four exponentials are kept in registers
+constexpr int kTileStage = kTilePins / kBlock;
 template <bool SUMS>
-__global__ void __launch_bounds__(kBlock) k_wa_small(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
+__global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
                                                      int write_rec) {
+    __shared__ int2 spos[kTilePins];
+    __shared__ int scq[SUMS ? kTilePins : 1];
     __shared__ unsigned long long shh[kBlock / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned long long h = 0;
     const int x = blockIdx.x % kXcds;
-    const int e_end = n.slice[x + 1];
-    const int stride = n.slice_blocks * blockDim.x;
-    for (int e = n.slice[x] + (blockIdx.x / kXcds) * blockDim.x + threadIdx.x; e < e_end; e += stride) {
-        const int s = n.start[e], deg = n.start[e + 1] - s;
-        int2 pos[kRegPins];
-        int cq[kRegPins];
-        load_pins<kRegPins, SUMS>(n, c, s, deg, 1, pos, cq);
-        int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+    const int t = n.tile_slice[x] + (int)(blockIdx.x / kXcds);
+    unsigned long long h = 0;
+    if (t < n.tile_slice[x + 1]) {
+        const int e0 = n.tile[t], e1 = n.tile[t + 1];
+        const int s0 = n.start[e0], np = n.start[e1] - s0;
+        {
+            int cl[kTileStage], cq[kTileStage];
+            int2 of[kTileStage], m[kTileStage];
 #pragma unroll
-        for (int k = 0; k < kRegPins; ++k) {
-            if (k < deg) {
-                lx = min(lx, pos[k].x);
-                ly = min(ly, pos[k].y);
-                ux = max(ux, pos[k].x);
-                uy = max(uy, pos[k].y);
-            }
-        }
-        for (int k = kRegPins; k < deg; ++k) {
-            const int2 p = pin_pos(n, c, s + k);
-            lx = min(lx, p.x);
-            ly = min(ly, p.y);
-            ux = max(ux, p.x);
-            uy = max(uy, p.y);
-        }
-        if (SUMS) {
-            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
-            float4 ev[kKeepPins > 0 ? kKeepPins : 1];
-#pragma unroll
-            for (int k = 0; k < kRegPins; ++k) {
-                if (k < deg) {
-                    if (k < kKeepPins) wa_pin_keep(pos[k], lx, ly, ux, uy, cx, cy, a, b, ev[k < kKeepPins ? k : 0]);
-                    else wa_pin(pos[k], lx, ly, ux, uy, cx, cy, a, b);
+            for (int i = 0; i < kTileStage; ++i) {
+                const int j = (int)threadIdx.x + i * kBlock;
+                cl[i] = -1;
+                cq[i] = -1;
+                of[i] = make_int2(0, 0);
+                if (j < np) {
+                    cl[i] = n.pcell[s0 + j];
+                    of[i] = n.poff[s0 + j];
+                    if (SUMS) cq[i] = n.cpi[s0 + j];
                 }
             }
-            for (int k = kRegPins; k < deg; ++k) wa_pin(pin_pos(n, c, s + k), lx, ly, ux, uy, cx, cy, a, b);
-            if (write_rec) {
-                NetRec& R = n.rec[e];
-                R.box = make_int4(lx, ly, ux, uy);
-                R.sx = a;
-                R.sy = b;
-            }
-            const float wt = n.weight[e];
+            // unguarded (an absent pin reads cell 0): a guard per read makes the compiler wait after each
 #pragma unroll
-            for (int k = 0; k < kRegPins; ++k)
-                if (k < deg) {
-                    if (k < kKeepPins) pin_grad_kept(n, cq[k], pos[k], cx, cy, a, b, wt, ev[k < kKeepPins ? k : 0]);
-                    else pin_grad(n, cq[k], pos[k], lx, ly, ux, uy, cx, cy, a, b, wt);
+            for (int i = 0; i < kTileStage; ++i) m[i] = c.ctr[max(cl[i], 0)];
+#pragma unroll
+            for (int i = 0; i < kTileStage; ++i) {
+                const int j = (int)threadIdx.x + i * kBlock;
+                if (j < np) {
+                    spos[j] = pin_at(n, s0 + j, cl[i], m[i], of[i]);
+                    if (SUMS) scq[j] = cq[i];
                 }
-            for (int k = kRegPins; k < deg; ++k) pin_grad(n, n.cpi[s + k], pin_pos(n, c, s + k), lx, ly, ux, uy, cx, cy, a, b, wt);
+            }
         }
-        if (ux >= lx) h += (unsigned long long)((long long)ux - lx + (long long)uy - ly);
+        __syncthreads();
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+            const int s = n.start[e] - s0, deg = n.start[e + 1] - n.start[e];
+            int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+            for (int k = 0; k < deg; ++k) {
+                const int2 p = spos[s + k];
+                lx = min(lx, p.x);
+                ly = min(ly, p.y);
+                ux = max(ux, p.x);
+                uy = max(uy, p.y);
+            }
+            if (SUMS) {
+                float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
+                float4 ev[kKeepPins];
+#pragma unroll
+                for (int k = 0; k < kKeepPins; ++k)
+                    if (k < deg) wa_pin_keep(spos[s + k], lx, ly, ux, uy, cx, cy, a, b, ev[k]);
+                for (int k = kKeepPins; k < deg; ++k) wa_pin(spos[s + k], lx, ly, ux, uy, cx, cy, a, b);
+                if (write_rec) {
+                    NetRec& R = n.rec[e];
+                    R.box = make_int4(lx, ly, ux, uy);
+                    R.sx = a;
+                    R.sy = b;
+                }
+                const float wt = n.weight[e];
+#pragma unroll
+                for (int k = 0; k < kKeepPins; ++k)
+                    if (k < deg) pin_grad_kept(n, scq[s + k], spos[s + k], cx, cy, a, b, wt, ev[k]);
+                for (int k = kKeepPins; k < deg; ++k) pin_grad(n, scq[s + k], spos[s + k], lx, ly, ux, uy, cx, cy, a, b, wt);
+            }
+            if (ux >= lx) h += (unsigned long long)((long long)ux - lx + (long long)uy - ly);
+        }
     }
     h = wave_sum(h);
     if (lane == 0) shh[w] = h;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int i = 0; i < kBlock / 64; ++i) t += shh[i];
-        if (t) atomicAdd(hpwl, t);
+        unsigned long long tot = 0;
+        for (int i = 0; i < kBlock / 64; ++i) tot += shh[i];
+        if (tot) atomicAdd(hpwl, tot);
     }
 }
 
@@ -1449,28 +1490,102 @@ static void transpose(hipStream_t s, const float* in, float* out, int R, int C) 
     hipLaunchKernelGGL(k_transpose, g, b, 0, s, in, out, R, C);
 }
 
+static int rows_per_block(int L) {
+    const int h = L >> 1;
+    const int T = h >= 4 ? h >> 2 : 1;
+    return 256 / T > 0 ? 256 / T : 1;
+}
+
 static void launch_rows(hipStream_t s, int mode, RowArgs a) {
     const int h = a.L >> 1;
     a.T = h >= 4 ? h >> 2 : 1;  // L/8 threads per row (one for the tiny lengths)
     a.logh = ilog2(h);
-    const int rpb = 256 / a.T > 0 ? 256 / a.T : 1;
+    const int rpb = rows_per_block(a.L);
     const int block = a.T * rpb;
     const int grid = (a.nrows + rpb - 1) / rpb;
-    const size_t lds = sizeof(float2) * (size_t)a.L * rpb;
+    const bool extra = mode == ROW_SPECTRUM_POISSON;
+    const size_t lds = (extra ? 12 : 8) * (size_t)a.L * rpb;
     switch (mode) {
     case ROW_DCT2: hipLaunchKernelGGL(k_rows<ROW_DCT2>, dim3(grid), dim3(block), lds, s, a); break;
     case ROW_DCT2_SCALE: hipLaunchKernelGGL(k_rows<ROW_DCT2_SCALE>, dim3(grid), dim3(block), lds, s, a); break;
     case ROW_POISSON: hipLaunchKernelGGL(k_rows<ROW_POISSON>, dim3(grid), dim3(block), lds, s, a); break;
     case ROW_INV_FIELDS: hipLaunchKernelGGL(k_rows<ROW_INV_FIELDS>, dim3(grid), dim3(block), lds, s, a); break;
+    case ROW_SPECTRUM_POISSON: hipLaunchKernelGGL(k_rows<ROW_SPECTRUM_POISSON>, dim3(grid), dim3(block), lds, s, a); break;
     default: hipLaunchKernelGGL(k_rows<ROW_INV_PHI>, dim3(grid), dim3(block), lds, s, a); break;
     }
 }
 
-// Bins are y-major D[y][x].  P1: rows along x, DCT-II -> B[y][u]; transpose -> [u][y];
+// The three-pass solve applies when every pass runs 256-thread blocks (L/8 threads per row, so
+// rows of up to 2048 bins).
+static bool three_pass(const RegionDev& r) { return r.cnt_x <= 2048 && r.cnt_y <= 2048; }
+
+// Bins are y-major D[y][x].  Three passes, each writing its output transposed for the next:
+// Q1 rows along x: DCT-II -> [u][y];
+// Q2 rows along y (u = row): DCT-II, halving and scale (nesterov.rs:415-424) kept in LDS, the
+//    Poisson divide (425-440) and the inverse transforms along y -> Ex'[y][u] (cos), Ey'[y][u]
+//    (sin), phi'[y][u];
+// Q3 rows along x: field_x = sin along x, field_y = cos along x (and phi = cos along x).
+static void solve_three_pass(hipStream_t s, const RegionDev& r, bool want_phi) {
+    const int nx = r.cnt_x, ny = r.cnt_y;
+    RowArgs a{};
+    a.in0 = r.density;
+    a.out0 = r.work1;  // [u][y]
+    a.L = nx;
+    a.nrows = ny;
+    a.tw = r.tw_x;
+    a.th = r.th_x;
+    a.dw = r.dw_x;
+    a.transpose_out = 1;
+    launch_rows(s, ROW_DCT2, a);
+    a = RowArgs{};
+    a.in0 = r.work1;
+    a.out0 = r.work0;  // Ex' [y][u]
+    a.out1 = r.work2;  // Ey' [y][u]
+    a.out2 = r.work3;  // phi' [y][u]
+    a.L = ny;
+    a.nrows = nx;
+    a.tw = r.tw_y;
+    a.th = r.th_y;
+    a.dw = r.dw_y;
+    a.scale = 4.0 / (double)nx / (double)ny;
+    a.wrow = r.wx;
+    a.wcol = r.wy;
+    a.want_phi = want_phi ? 1 : 0;
+    a.transpose_out = 1;
+    launch_rows(s, ROW_SPECTRUM_POISSON, a);
+    a = RowArgs{};
+    a.in0 = r.work0;
+    a.in1 = r.work2;
+    a.out0 = r.field_x;
+    a.out1 = r.field_y;
+    a.L = nx;
+    a.nrows = ny;
+    a.tw = r.tw_x;
+    a.th = r.th_x;
+    a.dw = r.dw_x;
+    launch_rows(s, ROW_INV_FIELDS, a);
+    if (want_phi) {
+        a = RowArgs{};
+        a.in0 = r.work3;
+        a.out0 = r.phi;
+        a.L = nx;
+        a.nrows = ny;
+        a.tw = r.tw_x;
+        a.th = r.th_x;
+        a.dw = r.dw_x;
+        launch_rows(s, ROW_INV_PHI, a);
+    }
+}
+
+// Grids beyond the three-pass form: bins are y-major D[y][x].  P1: rows along x, DCT-II -> B[y][u]; transpose -> [u][y];
 // P2: rows along y, DCT-II + halving + scale -> A[u][v]; P3: rows along v, the Poisson divide and
 // the inverse transforms along y -> Ex'[u][y] (cos), Ey'[u][y] (sin), phi'[u][y]; transpose ->
 // [y][u]; P4: rows along u -> field_x = sin along x, field_y = cos along x, phi = cos along x.
 void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi) {
+    if (three_pass(r)) {
+        solve_three_pass(s, r, want_phi);
+        return;
+    }
     const int nx = r.cnt_x, ny = r.cnt_y;
     RowArgs a{};
     // P1
@@ -1543,7 +1658,7 @@ static void wa_launch(hipStream_t s, hipStream_t s_big, const NetsDev& n, const 
     const int nbig = n.num_nets - n.num_small;
     if (nbig > 0) hipLaunchKernelGGL(k_wa_big<SUMS>, dim3(nbig), dim3(kBigBlock), 0, s_big, n, c, coef_x, coef_y, hpwl, write_rec);
     if (n.num_small > 0)
-        hipLaunchKernelGGL(k_wa_small<SUMS>, dim3(kXcds * n.slice_blocks), dim3(kBlock), 0, s, n, c, coef_x, coef_y, hpwl, write_rec);
+        hipLaunchKernelGGL(k_wa_tiles<SUMS>, dim3(kXcds * n.tile_blocks), dim3(kBlock), 0, s, n, c, coef_x, coef_y, hpwl, write_rec);
 }
 
 void launch_wa_forward(hipStream_t s, hipStream_t s_big, const NetsDev& n, const CellsDev& c, float coef_x, float coef_y,
@@ -1726,7 +1841,7 @@ int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap) {
     const void* f_update = reinterpret_cast<const void*>(k_place_update);
     const void* f_scatter = reinterpret_cast<const void*>(k_place_scatter<PLACE_STORED, unsigned>);
     const void* f_scatter_w = reinterpret_cast<const void*>(k_place_scatter<PLACE_STORED, unsigned long long>);
-    const void* f_small = reinterpret_cast<const void*>(k_wa_small<true>);
+    const void* f_small = reinterpret_cast<const void*>(k_wa_tiles<true>);
     const void* f_big = reinterpret_cast<const void*>(k_wa_big<true>);
     const void* f_grad = reinterpret_cast<const void*>(k_gradient);
     int m = 0;

@@ -252,6 +252,13 @@ struct gpl_ctx {
         mark_end();
     }
 
+    // the bins' per-bin pass and the solve (the overflow sums finished here, or with the
+    // gradient's sums when `finish` is false)
+    void bins_and_solve(Region& r, int b, bool finish = true, bool want_phi = false) {
+        bins_and_finish(r, b, finish);
+        solve(r, want_phi);
+    }
+
     void solve(Region& r, bool want_phi = false) {
         mark(F_DCT);
         gplk::launch_solve(stream, r.dev, want_phi);
@@ -360,9 +367,8 @@ struct gpl_ctx {
         mark(F_SCATTER);
         gplk::launch_center_init(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_PREV_SLP], r.v[S_CUR], d_global_adds);
         mark_end();
-        bins_and_finish(r, b);
+        bins_and_solve(r, b);
         r.prev_hpwl = hpwl_now();
-        solve(r);
         sync_stats();
         r.overflow = (int64_t)h_sums[b].overflow;
         r.overflow_unscaled = (int64_t)h_sums[b].overflow_unscaled;
@@ -427,8 +433,7 @@ struct gpl_ctx {
             gplk::launch_initial_prev(stream, r.dev, cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], initial_prev_coordi_update_coef, r.v[S_PREV_SLP]);
             gplk::launch_center_scatter(stream, r.dev, cells, r.v[S_PREV_SLP], d_global_adds);
             mark_end();
-            bins_and_finish(r, int(b));
-            solve(r);
+            bins_and_solve(r, int(b));
             sync_stats();
             r.overflow = (int64_t)h_sums[b].overflow;
             r.overflow_unscaled = (int64_t)h_sums[b].overflow_unscaled;
@@ -486,8 +491,7 @@ struct gpl_ctx {
             mark(F_SCATTER);
             gplk::launch_density_scatter(stream, r.dev, cells, d_global_adds);
             mark_end();
-            bins_and_finish(r, int(b), false);  // finished with the trial's gradient sums
-            solve(r);
+            bins_and_solve(r, int(b), false);  // finished with the trial's gradient sums
         }
         if (ovl) {
             std::swap(stream, aux);
@@ -886,8 +890,7 @@ struct gpl_ctx {
         mark(F_SCATTER);
         gplk::launch_center_scatter(stream, r.dev, cells, r.v[S_CUR], d_global_adds);
         mark_end();
-        bins_and_finish(r, b);
-        solve(r);
+        bins_and_solve(r, b);
         sync_stats();
         r.overflow = (int64_t)h_sums[b].overflow;
         r.overflow_unscaled = (int64_t)h_sums[b].overflow_unscaled;
@@ -1262,7 +1265,30 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
                 most = std::max(most, at - c->nets.slice[x]);
             }
             c->nets.slice[8] = at;
-            c->nets.slice_blocks = std::max(1, std::min((most + 255) / 256, 1024));
+            (void)most;
+        }
+        {
+            // each slice cut into tiles of consecutive nets for the WA kernel's LDS staging
+            std::vector<int32_t> tile;
+            int32_t most = 1;
+            for (int x = 0; x < 8; ++x) {
+                c->nets.tile_slice[x] = int32_t(tile.size());
+                int32_t e = c->nets.slice[x];
+                while (e < c->nets.slice[x + 1]) {
+                    tile.push_back(e);
+                    int32_t pins = 0, cnt = 0;
+                    while (e < c->nets.slice[x + 1] && cnt < gplk::kTileNets && pins + deg(order[e]) <= gplk::kTilePins) {
+                        pins += deg(order[e]);
+                        ++cnt;
+                        ++e;
+                    }
+                }
+                most = std::max(most, int32_t(tile.size()) - c->nets.tile_slice[x]);
+            }
+            c->nets.tile_slice[8] = int32_t(tile.size());
+            tile.push_back(c->nets.slice[8]);
+            c->nets.tile = c->upload(tile.data(), tile.size());
+            c->nets.tile_blocks = most;
         }
         std::vector<int32_t> nstart(E + 1, 0), pcell, np_net;
         std::vector<int2> poff, pxy0;
