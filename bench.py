@@ -290,6 +290,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline sampling")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the HPWL check")
     ap.add_argument("--hpwl-config", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--launch-mode", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="0: each trial one HIP graph (branches on side streams); 1: kernels launched one by "
+                         "one on the side streams; 2: one by one on one stream; 3: one graph captured on one stream")
     ap.add_argument("--profile-only", action="store_true",
                     help="np_init + W + K iterations of the profiled (serial) path only, the run rocprofv3 wraps; "
                          "no convergence, no CPU leg")
@@ -350,6 +353,7 @@ def main():
 
     # ---- the timed run ----
     g = G.NesterovGPU(d)
+    g.set_launch_mode(args.launch_mode)
     if args.profile_only:
         g.set_profiling(True)  # the serial path the roofline leg measures, for rocprofv3 to wrap
     g.init_nesterov_loop()
@@ -375,6 +379,10 @@ def main():
     rc, final_iter = g.loop_end(check=False)
     s = g.stats()
     hist = g.history()
+    # the host's share of each graph-launched trial: node argument updates, the launch call, the
+    # wait for the graph, and the host's own work between two trials
+    host = {k: round(1000.0 * v[1] / v[0], 2) for k, v in g.kernel_times().items()
+            if k.startswith("host_") and v[0] > 0}
     g.close()
     timed_s = t1 - t0
     value = steps / timed_s if timed_s > 0 else 0.0
@@ -468,6 +476,8 @@ def main():
         "final_hpwl": s.hpwl,
         "trials_per_iteration": round(trials, 3),
         "np_init_s": round(s.init_seconds, 4),
+        "host_us_per_trial": host,
+        "launch_mode": args.launch_mode,
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
         "cpu_baseline": cpu,

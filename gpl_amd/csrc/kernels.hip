@@ -10,6 +10,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <type_traits>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -208,14 +209,26 @@ __global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c
 // V: the LDS tables' sum type.  32 bits when a chunk's sum in one bin cannot reach 2^32 (every
 // overlap, times its scale and target, is at most the bin's area, so kChunk * the largest bin area
 // bounds it; the host checks that), which halves the tables and doubles the blocks per CU; else 64.
+//
+// A chunk first finds the box of bins its cells cover.  When that box holds at most kDense bins
+// (cells in Morton order: most chunks), the overlaps are added into a dense LDS tile indexed by
+// the bin's place in the box, one LDS atomic per overlap; otherwise into the hash table.
 template <int MODE, typename V>
 __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev c, PlaceArgs pa) {
-    __shared__ unsigned t_key[kSlots];
-    __shared__ V t_inst[kSlots];
-    __shared__ V t_fill[kSlots];
+    constexpr int kDense = sizeof(V) == 4 ? 3072 : 2048;
+    constexpr int kHashBytes = kSlots * (int)(sizeof(unsigned) + 2 * sizeof(V));
+    constexpr int kDenseBytes = 2 * kDense * (int)sizeof(V);
+    __shared__ __align__(16) unsigned char lds[kHashBytes > kDenseBytes ? kHashBytes : kDenseBytes];
+    __shared__ int s_box[4][kBlock / 64];
     __shared__ unsigned s_adds;
+    unsigned* t_key = reinterpret_cast<unsigned*>(lds);
+    V* t_inst = reinterpret_cast<V*>(lds + kSlots * sizeof(unsigned));
+    V* t_fill = t_inst + kSlots;
+    V* d_inst = reinterpret_cast<V*>(lds);
+    V* d_fill = d_inst + kDense;
     const int n = r.cell_end - r.cell_begin;
     const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (threadIdx.x == 0) s_adds = 0;
     if (pa.zero && blockIdx.x == 0 && threadIdx.x == 0) *pa.zero = 0ull;
     const int chunks = (n + kChunk - 1) / kChunk;
@@ -224,21 +237,23 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
     const int ch_end = min((L + 1) * per, chunks);
     for (int ch = L * per; ch < ch_end; ++ch) {
         const int base = ch * kChunk;
-        for (int i = threadIdx.x; i < kSlots; i += kBlock) {
-            t_key[i] = kEmpty;
-            t_inst[i] = 0;
-            t_fill[i] = 0;
-        }
-        __syncthreads();
-        unsigned adds = 0;
+        int4 dd[kCellsPerThread];
+        int x0[kCellsPerThread], x1[kCellsPerThread], y0[kCellsPerThread], y1[kCellsPerThread];
+        float scv[kCellsPerThread];
+        uint8_t fl[kCellsPerThread];
+        int bx0 = INT_MAX, by0 = INT_MAX, bx1 = INT_MIN, by1 = INT_MIN;
 #pragma unroll
         for (int j = 0; j < kCellsPerThread; ++j) {
             const int k = base + j * kBlock + threadIdx.x;
+            x0[j] = x1[j] = y0[j] = y1[j] = 0;
+            fl[j] = 0;
+            scv[j] = 0.0f;
+            dd[j] = make_int4(0, 0, 0, 0);
             if (k >= n) continue;
             const int g = r.cell_begin + k;
             int4 d = c.dbox[g];
-            const uint8_t f = c.flags[g];
-            const float sc = c.dscale[g];
+            fl[j] = c.flags[g];
+            scv[j] = c.dscale[g];
             float2 ns = make_float2(0.0f, 0.0f);
             if (MODE == PLACE_INIT) {
                 const int ddx = d.z - d.x, ddy = d.w - d.y;
@@ -261,56 +276,123 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
                 // nesterov.rs:2268-2276), which the WA kernels read from here
                 c.ctr[g] = make_int2((d.x + d.z) / 2, (d.y + d.w) / 2);
             }
-            int x0, x1, y0, y1;
-            bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
-            bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
-            const bool inst = f & 1u, macro = inst && (f & 2u);
-            V* vals = inst ? t_inst : t_fill;
+            dd[j] = d;
+            bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0[j], x1[j]);
+            bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0[j], y1[j]);
+            if (x1[j] > x0[j] && y1[j] > y0[j]) {
+                bx0 = min(bx0, x0[j]);
+                by0 = min(by0, y0[j]);
+                bx1 = max(bx1, x1[j]);
+                by1 = max(by1, y1[j]);
+            }
+        }
+        // the chunk's box of bins
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            bx0 = min(bx0, __shfl_xor(bx0, o, 64));
+            by0 = min(by0, __shfl_xor(by0, o, 64));
+            bx1 = max(bx1, __shfl_xor(bx1, o, 64));
+            by1 = max(by1, __shfl_xor(by1, o, 64));
+        }
+        if (lane == 0) {
+            s_box[0][w] = bx0;
+            s_box[1][w] = by0;
+            s_box[2][w] = bx1;
+            s_box[3][w] = by1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kBlock / 64; ++i) {
+            bx0 = min(bx0, s_box[0][i]);
+            by0 = min(by0, s_box[1][i]);
+            bx1 = max(bx1, s_box[2][i]);
+            by1 = max(by1, s_box[3][i]);
+        }
+        const int BW = bx1 - bx0, BH = by1 - by0;
+        const bool dense = BW > 0 && BH > 0 && BW * BH <= kDense;
+        const int nd = dense ? BW * BH : 0;
+        if (dense) {
+            for (int i = threadIdx.x; i < nd; i += kBlock) {
+                d_inst[i] = 0;
+                d_fill[i] = 0;
+            }
+        } else {
+            for (int i = threadIdx.x; i < kSlots; i += kBlock) {
+                t_key[i] = kEmpty;
+                t_inst[i] = 0;
+                t_fill[i] = 0;
+            }
+        }
+        __syncthreads();
+        unsigned adds = 0;
+#pragma unroll
+        for (int j = 0; j < kCellsPerThread; ++j) {
+            if (x1[j] <= x0[j] || y1[j] <= y0[j]) continue;
+            const int4 d = dd[j];
+            const bool inst = fl[j] & 1u, macro = inst && (fl[j] & 2u);
+            const float sc = scv[j];
+            V* vals = dense ? (inst ? d_inst : d_fill) : (inst ? t_inst : t_fill);
             unsigned long long* glob = inst ? r.inst : r.filler;
-            auto add = [&](int x, int y, int bx0, int by0, int bx1, int by1) {
-                float a = overlap_area(bx0, by0, bx1, by1, d) * sc;
+            auto add = [&](int x, int y, int ex0, int ey0, int ex1, int ey1) {
+                float a = overlap_area(ex0, ey0, ex1, ey1, d) * sc;
                 if (macro) a *= r.target_density;
                 const long long v = (long long)a;
                 if (v == 0) return;
+                if (dense) {
+                    atomicAdd(&vals[(y - by0) * BW + (x - bx0)], (V)v);
+                    return;
+                }
                 const unsigned kb = (unsigned)y * (unsigned)r.cnt_x + (unsigned)x;
                 if (!table_add<V>(t_key, vals, kb, (V)v)) {
                     atomicAdd(glob + kb, (unsigned long long)v);
                     ++adds;
                 }
             };
-            if (x1 - x0 <= 3 && y1 - y0 <= 3) {
+            if (x1[j] - x0[j] <= 3 && y1[j] - y0[j] <= 3) {
                 // the edges of a 3 x 3 bin window read at once (indices clamped, no guards, so
                 // that the compiler waits once): the span of any cell up to twice a bin's size
                 int ex[4], ey[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    ex[i] = r.bin_x[min(x0 + i, r.cnt_x)];
-                    ey[i] = r.bin_y[min(y0 + i, r.cnt_y)];
+                    ex[i] = r.bin_x[min(x0[j] + i, r.cnt_x)];
+                    ey[i] = r.bin_y[min(y0[j] + i, r.cnt_y)];
                 }
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    if (j >= y1 - y0) break;
+                for (int q = 0; q < 3; ++q) {
+                    if (q >= y1[j] - y0[j]) break;
 #pragma unroll
                     for (int i = 0; i < 3; ++i) {
-                        if (i >= x1 - x0) break;
-                        add(x0 + i, y0 + j, ex[i], ey[j], ex[i + 1], ey[j + 1]);
+                        if (i >= x1[j] - x0[j]) break;
+                        add(x0[j] + i, y0[j] + q, ex[i], ey[q], ex[i + 1], ey[q + 1]);
                     }
                 }
             } else {
-                for (int y = y0; y < y1; ++y) {
-                    const int by0 = r.bin_y[y], by1 = r.bin_y[y + 1];
-                    for (int x = x0; x < x1; ++x) add(x, y, r.bin_x[x], by0, r.bin_x[x + 1], by1);
+                for (int y = y0[j]; y < y1[j]; ++y) {
+                    const int ey0 = r.bin_y[y], ey1 = r.bin_y[y + 1];
+                    for (int x = x0[j]; x < x1[j]; ++x) add(x, y, r.bin_x[x], ey0, r.bin_x[x + 1], ey1);
                 }
             }
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < kSlots; i += kBlock) {
-            const unsigned kb = t_key[i];
-            if (kb == kEmpty) continue;
-            const V a = t_inst[i], b = t_fill[i];
-            if (a) atomicAdd(r.inst + kb, (unsigned long long)a);
-            if (b) atomicAdd(r.filler + kb, (unsigned long long)b);
-            adds += (a != 0) + (b != 0);
+        if (dense) {
+            for (int i = threadIdx.x; i < nd; i += kBlock) {
+                const V a = d_inst[i], b = d_fill[i];
+                if (!(a | b)) continue;
+                const int yy = i / BW;
+                const unsigned kb = (unsigned)(by0 + yy) * (unsigned)r.cnt_x + (unsigned)(bx0 + i - yy * BW);
+                if (a) atomicAdd(r.inst + kb, (unsigned long long)a);
+                if (b) atomicAdd(r.filler + kb, (unsigned long long)b);
+                adds += (a != 0) + (b != 0);
+            }
+        } else {
+            for (int i = threadIdx.x; i < kSlots; i += kBlock) {
+                const unsigned kb = t_key[i];
+                if (kb == kEmpty) continue;
+                const V a = t_inst[i], b = t_fill[i];
+                if (a) atomicAdd(r.inst + kb, (unsigned long long)a);
+                if (b) atomicAdd(r.filler + kb, (unsigned long long)b);
+                adds += (a != 0) + (b != 0);
+            }
         }
         if (adds) atomicAdd(&s_adds, adds);
         __syncthreads();
@@ -396,37 +478,57 @@ enum RowMode {
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
+// LDS pointers typed as such, so that every access compiles to ds_read / ds_write (a generic
+// pointer into LDS makes flat accesses, which wait like global loads).
+#define GPL_LDS __attribute__((address_space(3)))
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef GPL_LDS vf2 lds_float2;
+typedef GPL_LDS float lds_float;
+__device__ __forceinline__ float2 ld2(const lds_float2* p) {
+    const vf2 v = *p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float2 ld2(const float2* p) { return *p; }
+__device__ __forceinline__ void st2(lds_float2* p, float2 v) {
+    vf2 t;
+    t.x = v.x;
+    t.y = v.y;
+    *p = t;
+}
+
 // Complex FFT of N points (a power of two) held in LDS buffer x, by T = max(N/4, 1) threads of a
 // row group (tid < T), Stockham autosort: radix-2 first when log2 N is odd, then radix-4 stages;
-// natural order in and out.  tw[m] = e^{-2 pi i m / N}, m < N; dir -1 forward, +1 inverse
-// (unnormalised).  Every thread of the block calls it (block barriers inside); returns the buffer
-// that holds the result.
-__device__ float2* fft_stockham(float2* x, float2* y, int N, int logN, int T, int tid, const float2* tw, int dir) {
-    int Ns = 1;
+// natural order in and out.  tw[m] = e^{-2 pi i m / N}, m < N (TP: an LDS or a global pointer);
+// dir -1 forward, +1 inverse (unnormalised).  Every thread of the block calls it (block barriers
+// inside); returns the buffer that holds the result.
+template <typename TP>
+__device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N, int logN, int T, int tid, TP tw, int dir) {
+    int lNs = 0;  // log2 of the current sub-transform length
     __syncthreads();
     if (logN & 1) {
         for (int j = tid; j < (N >> 1); j += T) {
-            float2 v0 = x[j], v1 = x[j + (N >> 1)];
+            float2 v0 = ld2(x + j), v1 = ld2(x + j + (N >> 1));
             // Ns = 1: no twiddle
             int d = j * 2;
-            y[d] = make_float2(v0.x + v1.x, v0.y + v1.y);
-            y[d + 1] = make_float2(v0.x - v1.x, v0.y - v1.y);
+            st2(y + d, make_float2(v0.x + v1.x, v0.y + v1.y));
+            st2(y + d + 1, make_float2(v0.x - v1.x, v0.y - v1.y));
         }
-        Ns = 2;
-        float2* t = x;
+        lNs = 1;
+        lds_float2* t = x;
         x = y;
         y = t;
         __syncthreads();
     }
     const int q = N >> 2;
-    for (; Ns < N; Ns <<= 2) {
+    for (; lNs < logN; lNs += 2) {
+        const int Ns = 1 << lNs;
         if (tid < q) {
             const int j = tid;
             const int k = j & (Ns - 1);
-            float2 v0 = x[j], v1 = x[j + q], v2 = x[j + 2 * q], v3 = x[j + 3 * q];
+            float2 v0 = ld2(x + j), v1 = ld2(x + j + q), v2 = ld2(x + j + 2 * q), v3 = ld2(x + j + 3 * q);
             if (Ns > 1) {
-                const int step = N / (Ns << 2);  // twiddle index stride
-                float2 w1 = tw[k * step], w2 = tw[2 * k * step], w3 = tw[3 * k * step];
+                const int step = N >> (lNs + 2);  // twiddle index stride
+                float2 w1 = ld2(tw + k * step), w2 = ld2(tw + 2 * k * step), w3 = ld2(tw + 3 * k * step);
                 if (dir > 0) {
                     w1 = cconj(w1);
                     w2 = cconj(w2);
@@ -440,13 +542,13 @@ __device__ float2* fft_stockham(float2* x, float2* y, int N, int logN, int T, in
             float2 t2 = make_float2(v1.x + v3.x, v1.y + v3.y), t3 = make_float2(v1.x - v3.x, v1.y - v3.y);
             // forward: X1 = t1 - i t3, X3 = t1 + i t3; inverse the other way
             float2 it3 = dir < 0 ? make_float2(t3.y, -t3.x) : make_float2(-t3.y, t3.x);  // -i t3 or +i t3
-            const int d = (j / Ns) * Ns * 4 + k;
-            y[d] = make_float2(t0.x + t2.x, t0.y + t2.y);
-            y[d + Ns] = make_float2(t1.x + it3.x, t1.y + it3.y);
-            y[d + 2 * Ns] = make_float2(t0.x - t2.x, t0.y - t2.y);
-            y[d + 3 * Ns] = make_float2(t1.x - it3.x, t1.y - it3.y);
+            const int d = ((j >> lNs) << (lNs + 2)) + k;
+            st2(y + d, make_float2(t0.x + t2.x, t0.y + t2.y));
+            st2(y + d + Ns, make_float2(t1.x + it3.x, t1.y + it3.y));
+            st2(y + d + 2 * Ns, make_float2(t0.x - t2.x, t0.y - t2.y));
+            st2(y + d + 3 * Ns, make_float2(t1.x - it3.x, t1.y - it3.y));
         }
-        float2* t = x;
+        lds_float2* t = x;
         x = y;
         y = t;
         __syncthreads();
@@ -454,38 +556,44 @@ __device__ float2* fft_stockham(float2* x, float2* y, int N, int logN, int T, in
     return x;
 }
 
+template <typename TP>
 struct RowCtx {
     int L, h, logh, T, tid;
-    const float2* tw;  // e^{-2 pi i m / h}, m < h
-    const float2* th;  // e^{-2 pi i k / L}, k <= h
-    const float2* dw;  // (cos, -sin)(pi k / 2L), k < L
-    float2* A;
-    float2* B;
+    TP tw;  // e^{-2 pi i m / h}, m < h
+    TP th;  // e^{-2 pi i k / L}, k <= h
+    TP dw;  // (cos, -sin)(pi k / 2L), k < L
+    lds_float2* A;
+    lds_float2* B;
 };
 
-// DCT-II of the real row at src (Makhoul: v = (x[0], x[2], ..., x[3], x[1]), its DFT through an
-// L/2-point complex FFT of the pairs, C[k] = Re(e^{-i pi k / 2L} V[k])); calls out(k, C[k]).
-template <typename OUT>
-__device__ void dct2_row(const RowCtx& R, bool active, const float* src, OUT out) {
+// DCT-II of the real row given by src(j), j < L (Makhoul: v = (x[0], x[2], ..., x[3], x[1]), its
+// DFT through an L/2-point complex FFT of the pairs, C[k] = Re(e^{-i pi k / 2L} V[k])); calls
+// out(k, C[k]).
+template <typename TP, typename SRC, typename OUT>
+__device__ void dct2_row(const RowCtx<TP>& R, bool active, SRC src, OUT out) {
     const int L = R.L, h = R.h;
     if (active)
         for (int m = R.tid; m < h; m += R.T) {
             int q0 = 2 * m, q1 = 2 * m + 1;
-            float a = q0 < h ? src[2 * q0] : src[2 * (L - 1 - q0) + 1];
-            float b = q1 < h ? src[2 * q1] : src[2 * (L - 1 - q1) + 1];
-            R.A[m] = make_float2(a, b);
+            float a = q0 < h ? src(2 * q0) : src(2 * (L - 1 - q0) + 1);
+            float b = q1 < h ? src(2 * q1) : src(2 * (L - 1 - q1) + 1);
+            st2(R.A + m, make_float2(a, b));
         }
-    float2* Z = fft_stockham(R.A, R.B, h, R.logh, R.T, R.tid, R.tw, -1);
+    lds_float2* Z = fft_stockham(R.A, R.B, h, R.logh, R.T, R.tid, R.tw, -1);
     if (active)
         for (int k = R.tid; k <= h; k += R.T) {
-            float2 zk = Z[k == h ? 0 : k], zc = cconj(Z[k == 0 ? 0 : h - k]);
+            float2 zk = ld2(Z + (k == h ? 0 : k)), zc = cconj(ld2(Z + (k == 0 ? 0 : h - k)));
             float2 E = make_float2((zk.x + zc.x) * 0.5f, (zk.y + zc.y) * 0.5f);
             float2 Op = make_float2((zk.x - zc.x) * 0.5f, (zk.y - zc.y) * 0.5f);
             float2 O = make_float2(Op.y, -Op.x);  // -i Op
-            float2 wO = cmul(R.th[k], O);
+            float2 wO = cmul(ld2(R.th + k), O);
             float2 V = make_float2(E.x + wO.x, E.y + wO.y);
-            out(k, R.dw[k].x * V.x - R.dw[k].y * V.y);
-            if (k > 0 && k < h) out(L - k, R.dw[L - k].x * V.x + R.dw[L - k].y * V.y);
+            const float2 d0 = ld2(R.dw + k);
+            out(k, d0.x * V.x - d0.y * V.y);
+            if (k > 0 && k < h) {
+                const float2 d1 = ld2(R.dw + (L - k));
+                out(L - k, d1.x * V.x + d1.y * V.y);
+            }
         }
 }
 
@@ -494,8 +602,8 @@ __device__ void dct2_row(const RowCtx& R, bool active, const float* src, OUT out
 // c from b[0] = a[0], b[j] = a[L-j]); the real inverse DFT through an L/2-point complex IFFT of
 // Z[k] = (V[k] + conj V[h-k]) + i e^{2 pi i k / L} (V[k] - conj V[h-k]); y = v / 2 unshuffled,
 // odd outputs negated for the sine.
-template <typename F, typename OUT>
-__device__ void dct3_row(const RowCtx& R, bool active, F coef, bool sine, OUT out) {
+template <typename TP, typename F, typename OUT>
+__device__ void dct3_row(const RowCtx<TP>& R, bool active, F coef, bool sine, OUT out) {
     const int L = R.L, h = R.h;
     auto V = [&](int q) -> float2 {
         float ck, cl;
@@ -506,21 +614,22 @@ __device__ void dct3_row(const RowCtx& R, bool active, F coef, bool sine, OUT ou
             ck = q == 0 ? 2.0f * coef(0) : coef(L - q);
             cl = q == 0 ? 0.0f : coef(q);
         }
-        float cs = R.dw[q].x, sn = -R.dw[q].y;
+        const float2 dq = ld2(R.dw + q);
+        float cs = dq.x, sn = -dq.y;
         return make_float2(cs * ck + sn * cl, sn * ck - cs * cl);
     };
     if (active)
         for (int k = R.tid; k < h; k += R.T) {
             float2 vk = V(k), vc = cconj(V(h - k));
             float2 E = make_float2(vk.x + vc.x, vk.y + vc.y), D = make_float2(vk.x - vc.x, vk.y - vc.y);
-            float2 wD = cmul(cconj(R.th[k]), D);
-            R.A[k] = make_float2(E.x - wD.y, E.y + wD.x);
+            float2 wD = cmul(cconj(ld2(R.th + k)), D);
+            st2(R.A + k, make_float2(E.x - wD.y, E.y + wD.x));
         }
-    float2* z = fft_stockham(R.A, R.B, h, R.logh, R.T, R.tid, R.tw, +1);
+    lds_float2* z = fft_stockham(R.A, R.B, h, R.logh, R.T, R.tid, R.tw, +1);
     if (active)
         for (int j = R.tid; j < L; j += R.T) {
             int m = (j & 1) ? (L - 1 - (j >> 1)) : (j >> 1);
-            float2 zz = z[m >> 1];
+            float2 zz = ld2(z + (m >> 1));
             float v = ((m & 1) ? zz.y : zz.x) * 0.5f;
             if (sine && (j & 1)) v = -v;
             out(j, v);
@@ -556,29 +665,55 @@ struct RowArgs {
 //  ROW_INV_PHI     out0 = DCT-III(in0)
 //  ROW_SPECTRUM_POISSON  ROW_DCT2_SCALE kept in LDS, then ROW_POISSON on it
 // With transpose_out the outputs go out transposed, so that the next pass reads rows.
-template <int MODE>
+// TL: the twiddle tables (and the element wavenumbers) are first copied to LDS (the launcher
+// picks it when they fit beside the rows' buffers); else they are read from global memory.
+// LDS: per row the FFT's two buffers (L float2), [ROW_SPECTRUM_POISSON: the spectrum, L floats],
+// then [TL: tw h, th h + 1, dw L float2, the element wavenumbers L floats].
+template <int MODE, bool TL>
 __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     extern __shared__ float2 lds2[];
+    lds_float2* const lbase = (lds_float2*)lds2;
+    using TP = typename std::conditional<TL, const lds_float2*, const float2*>::type;
     const int T = a.T;
     const int rpb = blockDim.x / T;
     const int sub = threadIdx.x / T;
-    RowCtx R;
-    R.L = a.L;
-    R.h = a.L >> 1;
+    const int L = a.L;
+    RowCtx<TP> R;
+    R.L = L;
+    R.h = L >> 1;
     R.logh = a.logh;
     R.T = T;
     R.tid = threadIdx.x - sub * T;
-    R.tw = a.tw;
-    R.th = a.th;
-    R.dw = a.dw;
-    R.A = lds2 + (size_t)sub * a.L;
+    R.A = lbase + (size_t)sub * L;
     R.B = R.A + R.h;
-    // ROW_SPECTRUM_POISSON: one more row of L floats per row (the spectrum), after the FFT buffers
-    float* extra = reinterpret_cast<float*>(lds2 + (size_t)rpb * a.L) + (size_t)sub * a.L;
+    lds_float* extra = (lds_float*)(lbase + (size_t)rpb * L) + (size_t)sub * L;
+    const bool spectrum = MODE == ROW_SPECTRUM_POISSON;
+    const bool wav = MODE == ROW_POISSON || MODE == ROW_SPECTRUM_POISSON;
+    lds_float2* tab = lbase + (size_t)rpb * L + (spectrum ? (size_t)rpb * L / 2 : 0);
+    lds_float* wcol_l = (lds_float*)(tab + 2 * R.h + 1 + L);
+    if constexpr (TL) {
+        const int h = R.h;
+        for (int i = threadIdx.x; i < h; i += blockDim.x) st2(tab + i, a.tw[i]);
+        for (int i = threadIdx.x; i <= h; i += blockDim.x) st2(tab + h + i, a.th[i]);
+        for (int i = threadIdx.x; i < L; i += blockDim.x) st2(tab + 2 * h + 1 + i, a.dw[i]);
+        if (wav)
+            for (int i = threadIdx.x; i < L; i += blockDim.x) wcol_l[i] = a.wcol[i];
+        R.tw = tab;
+        R.th = tab + h;
+        R.dw = tab + 2 * h + 1;
+        __syncthreads();
+    } else {
+        R.tw = a.tw;
+        R.th = a.th;
+        R.dw = a.dw;
+    }
+    auto wcol = [&](int v) -> float {
+        if constexpr (TL) return wcol_l[v];
+        else return a.wcol[v];
+    };
     const int lb = gridDim.x % kXcds == 0 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int row = lb * rpb + sub;
     const bool active = row < a.nrows;
-    const int L = a.L;
     const size_t off = (size_t)(active ? row : 0) * L;
     const int nrows = a.nrows;
     const bool tr = a.transpose_out != 0;
@@ -588,7 +723,8 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     };
     if (MODE == ROW_DCT2 || MODE == ROW_DCT2_SCALE) {
         const double scale = a.scale;
-        dct2_row(R, active, a.in0 + off, [&](int k, float v) {
+        const float* src = a.in0 + off;
+        dct2_row(R, active, [&](int j) { return src[j]; }, [&](int k, float v) {
             if (MODE == ROW_DCT2_SCALE) {
                 if ((row >= 1 && k == 0) || (row == 0 && k >= 1)) v = (float)((double)v * 0.5);
                 v = (float)((double)v * scale);
@@ -596,25 +732,28 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
             put(a.out0, k, v);
         });
     } else if (MODE == ROW_POISSON || MODE == ROW_SPECTRUM_POISSON) {
-        const float* A = a.in0 + off;
-        if (MODE == ROW_SPECTRUM_POISSON) {
+        const float* Ag = a.in0 + off;
+        if (spectrum) {
             const double scale = a.scale;
-            dct2_row(R, active, a.in0 + off, [&](int k, float v) {
+            dct2_row(R, active, [&](int j) { return Ag[j]; }, [&](int k, float v) {
                 if ((row >= 1 && k == 0) || (row == 0 && k >= 1)) v = (float)((double)v * 0.5);
                 v = (float)((double)v * scale);
                 extra[k] = v;
             });
             __syncthreads();
-            A = extra;
         }
+        auto A = [&](int v) -> float {
+            if (spectrum) return extra[v];
+            return Ag[v];
+        };
         const float wu = active ? a.wrow[row] : 0.0f, wu2 = wu * wu;
         auto pot = [&](int v) -> float {
             if (row == 0 && v == 0) return 0.0f;
-            float wv = a.wcol[v];
-            return A[v] / (wu2 + wv * wv);
+            float wv = wcol(v);
+            return A(v) / (wu2 + wv * wv);
         };
         dct3_row(R, active, [&](int v) { return pot(v) * wu; }, false, [&](int j, float y) { put(a.out0, j, y); });
-        dct3_row(R, active, [&](int v) { return pot(v) * a.wcol[v]; }, true, [&](int j, float y) { put(a.out1, j, y); });
+        dct3_row(R, active, [&](int v) { return pot(v) * wcol(v); }, true, [&](int j, float y) { put(a.out1, j, y); });
         if (a.want_phi) dct3_row(R, active, pot, false, [&](int j, float y) { put(a.out2, j, y); });
     } else if (MODE == ROW_INV_FIELDS) {
         const float* X = a.in0 + off;
@@ -1150,7 +1289,8 @@ __global__ void k_initial_prev(RegionDev r, CellsDev c, const float2* cur_slp, c
 // t + 1024, ..., then the waves' totals in wave order).
 constexpr int kFinishBlock = 1024;
 __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials, int nb_bins, int nb_cells, Sums* out,
-                                                         const unsigned long long* hpwl) {
+                                                         const unsigned long long* hpwl, Sums* host_out,
+                                                         const Sums* dev_tail, Sums* host_tail) {
     __shared__ double sh[6][kFinishBlock / 64];
     double v[6];
 #pragma unroll
@@ -1174,9 +1314,23 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials,
                     : threadIdx.x == 3 ? &out->den_abs
                     : threadIdx.x == 4 ? &out->coord_sq
                                        : &out->grad_sq;
-        if ((threadIdx.x < 2 && nb_bins > 0) || (threadIdx.x >= 2 && nb_cells > 0)) *o = t;
+        if ((threadIdx.x < 2 && nb_bins > 0) || (threadIdx.x >= 2 && nb_cells > 0)) {
+            *o = t;
+            // the same value straight into the host's pinned copy (no copy node after the trial)
+            if (host_out) *(reinterpret_cast<double*>(host_out) + (o - &out->overflow)) = t;
+        }
     }
-    if (threadIdx.x == 0 && hpwl) out->hpwl = *hpwl;
+    if (threadIdx.x == 0 && hpwl) {
+        const unsigned long long h = *hpwl;
+        out->hpwl = h;
+        if (host_out) host_out->hpwl = h;
+    }
+    if (threadIdx.x == 0 && host_tail) {
+        // the HPWL counter and the scatter's global-add count
+        host_tail->hpwl = dev_tail->hpwl;
+        host_tail->pad[0] = dev_tail->pad[0];
+        host_tail->pad[1] = dev_tail->pad[1];
+    }
 }
 
 // nesterov.rs:2850-2859: a locked instance cell's next slots take its current values.
@@ -1503,16 +1657,26 @@ static void launch_rows(hipStream_t s, int mode, RowArgs a) {
     const int rpb = rows_per_block(a.L);
     const int block = a.T * rpb;
     const int grid = (a.nrows + rpb - 1) / rpb;
-    const bool extra = mode == ROW_SPECTRUM_POISSON;
-    const size_t lds = (extra ? 12 : 8) * (size_t)a.L * rpb;
+    const bool spectrum = mode == ROW_SPECTRUM_POISSON;
+    const bool wav = mode == ROW_POISSON || mode == ROW_SPECTRUM_POISSON;
+    const size_t rows_lds = (spectrum ? 12 : 8) * (size_t)a.L * rpb;
+    const size_t tab_lds = sizeof(float2) * (2 * (size_t)a.L + 1) + (wav ? 4 * (size_t)a.L : 0);
+    const bool tl = rows_lds + tab_lds <= 64 * 1024;
+    const size_t lds = rows_lds + (tl ? tab_lds : 0);
+#define GPL_ROWS(M)                                                                                     \
+    do {                                                                                                \
+        if (tl) hipLaunchKernelGGL((k_rows<M, true>), dim3(grid), dim3(block), lds, s, a);              \
+        else hipLaunchKernelGGL((k_rows<M, false>), dim3(grid), dim3(block), lds, s, a);                \
+    } while (0)
     switch (mode) {
-    case ROW_DCT2: hipLaunchKernelGGL(k_rows<ROW_DCT2>, dim3(grid), dim3(block), lds, s, a); break;
-    case ROW_DCT2_SCALE: hipLaunchKernelGGL(k_rows<ROW_DCT2_SCALE>, dim3(grid), dim3(block), lds, s, a); break;
-    case ROW_POISSON: hipLaunchKernelGGL(k_rows<ROW_POISSON>, dim3(grid), dim3(block), lds, s, a); break;
-    case ROW_INV_FIELDS: hipLaunchKernelGGL(k_rows<ROW_INV_FIELDS>, dim3(grid), dim3(block), lds, s, a); break;
-    case ROW_SPECTRUM_POISSON: hipLaunchKernelGGL(k_rows<ROW_SPECTRUM_POISSON>, dim3(grid), dim3(block), lds, s, a); break;
-    default: hipLaunchKernelGGL(k_rows<ROW_INV_PHI>, dim3(grid), dim3(block), lds, s, a); break;
+    case ROW_DCT2: GPL_ROWS(ROW_DCT2); break;
+    case ROW_DCT2_SCALE: GPL_ROWS(ROW_DCT2_SCALE); break;
+    case ROW_POISSON: GPL_ROWS(ROW_POISSON); break;
+    case ROW_INV_FIELDS: GPL_ROWS(ROW_INV_FIELDS); break;
+    case ROW_SPECTRUM_POISSON: GPL_ROWS(ROW_SPECTRUM_POISSON); break;
+    default: GPL_ROWS(ROW_INV_PHI); break;
     }
+#undef GPL_ROWS
 }
 
 // The three-pass solve applies when every pass runs 256-thread blocks (L/8 threads per row, so
@@ -1677,8 +1841,10 @@ void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const
                        a_slp, b_slp, a_sum, partials, nan_inject);
 }
 
-void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int nparts_cells, Sums* out, const unsigned long long* hpwl) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishBlock), 0, s, partials, nparts_bins, nparts_cells, out, hpwl);
+void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int nparts_cells, Sums* out, const unsigned long long* hpwl,
+                   Sums* host_out, const Sums* dev_tail, Sums* host_tail) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishBlock), 0, s, partials, nparts_bins, nparts_cells, out, hpwl, host_out, dev_tail,
+                       host_tail);
 }
 
 void launch_distance(hipStream_t s, int n, const float2* a, const float2* b, const float2* c, const float2* d, double* partials) {

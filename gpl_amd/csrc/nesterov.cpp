@@ -150,6 +150,7 @@ struct gpl_ctx {
     // host time of the graph-launched trials (ms): node arguments, the launch call, the wait, and
     // the host's work between one trial's return and the next's start
     enum { H_PARAMS, H_LAUNCH, H_WAIT, H_BETWEEN, H_COUNT };
+    bool finish_to_host = false;  // k_finish also writes the pinned host statistics (graph capture)
     double host_ms[H_COUNT] = {};
     int host_n = 0;
     std::chrono::steady_clock::time_point host_last{};
@@ -301,7 +302,8 @@ struct gpl_ctx {
                               r.v[base + 3], trial ? r.v[S_CUR_SLP] : nullptr, trial ? r.v[S_NEXT_SLP] : nullptr,
                               trial ? r.v[S_CUR_SLP_SUM] : nullptr, r.partials, nan_inject);
         gplk::launch_finish(stream, r.partials, trial ? gplk::block_partials_bins(r.dev) : 0, gplk::block_partials_cells(r.dev), d_sums + b,
-                            d_hpwl);
+                            d_hpwl, finish_to_host ? h_sums + b : nullptr, finish_to_host ? d_sums + rg.size() : nullptr,
+                            finish_to_host ? h_sums + rg.size() : nullptr);
         mark_end();
     }
 
@@ -567,8 +569,10 @@ struct gpl_ctx {
             field();
             wa_forward(false, true);
             join();
+            // the trial's statistics go straight to the pinned host copy: the graph holds no copy node
+            finish_to_host = true;
             launch_gradients(0, int(rg.size()), SLOT_NEXT, true);
-            check(hipMemcpyAsync(h_sums, d_sums, sizeof(gplk::Sums) * (rg.size() + 1), hipMemcpyDeviceToHost, stream), "stats copy");
+            finish_to_host = false;
         } catch (const HipError& e) {
             err = e.e;
         }
@@ -1834,7 +1838,7 @@ int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter) {
 }
 
 int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode) {
-    if (!c || mode < 0 || mode > 2) return GPL_ERR_ARG;
+    if (!c || mode < 0 || mode > 3) return GPL_ERR_ARG;
     try {
         c->join();
         check(hipStreamSynchronize(c->stream), "sync");
@@ -1842,7 +1846,7 @@ int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode) {
         return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
     }
     c->drop_graph();
-    c->launch_mode = mode;
+    c->launch_mode = mode == 3 ? 0 : mode;
     c->overlap = mode < 2;
     return GPL_OK;
 }

@@ -398,7 +398,7 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap);
 /* How a back-tracking trial is launched (no reference counterpart; the results are the same in
  * every mode): 0 (default) one HIP graph per trial, the bins and the Poisson solve on a branch
  * beside the WA kernels, the high-fanout nets on a third; 1 the same kernels launched one by one
- * on those streams; 2 one by one on one stream. */
+ * on those streams; 2 one by one on one stream; 3 one HIP graph per trial captured on one stream. */
 int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode);
 int32_t gpl_set_profiling(gpl_ctx* c, int32_t on);
 
