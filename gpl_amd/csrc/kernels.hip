@@ -499,20 +499,34 @@ __device__ __forceinline__ void st2(lds_float2* p, float2 v) {
 // Complex FFT of N points (a power of two) held in LDS buffer x, by T = max(N/4, 1) threads of a
 // row group (tid < T), Stockham autosort: radix-2 first when log2 N is odd, then radix-4 stages;
 // natural order in and out.  tw[m] = e^{-2 pi i m / N}, m < N (TP: an LDS or a global pointer);
-// dir -1 forward, +1 inverse (unnormalised).  Every thread of the block calls it (block barriers
-// inside); returns the buffer that holds the result.
-template <typename TP>
-__device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N, int logN, int T, int tid, TP tw, int dir) {
+// dir -1 forward, +1 inverse (unnormalised).  LOGN > 0 fixes N = 2^LOGN at compile time (the
+// stages unroll, the index arithmetic folds); 0 takes N and logN as given.  Every thread of the
+// block calls it (block barriers inside); returns the buffer that holds the result.
+template <int LOGN, typename TP>
+__device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N_, int logN_, int T_, int tid, TP tw, int dir) {
+    const int logN = LOGN > 0 ? LOGN : logN_;
+    const int N = LOGN > 0 ? (1 << LOGN) : N_;
+    const int T = LOGN > 0 ? (LOGN >= 2 ? (1 << (LOGN - 2)) : 1) : T_;
     int lNs = 0;  // log2 of the current sub-transform length
     __syncthreads();
     if (logN & 1) {
-        for (int j = tid; j < (N >> 1); j += T) {
-            float2 v0 = ld2(x + j), v1 = ld2(x + j + (N >> 1));
-            // Ns = 1: no twiddle
-            int d = j * 2;
-            st2(y + d, make_float2(v0.x + v1.x, v0.y + v1.y));
-            st2(y + d + 1, make_float2(v0.x - v1.x, v0.y - v1.y));
+        const int half = N >> 1;
+        // LOGN > 0: half / T = 2 passes (1 when N = 2); LOGN == 0: the loop after this one
+#pragma unroll
+        for (int i = 0; i < (LOGN >= 2 ? 2 : 1); ++i) {
+            const int j = tid + i * T;
+            if (LOGN > 0 && j < half) {
+                float2 v0 = ld2(x + j), v1 = ld2(x + j + half);
+                st2(y + 2 * j, make_float2(v0.x + v1.x, v0.y + v1.y));
+                st2(y + 2 * j + 1, make_float2(v0.x - v1.x, v0.y - v1.y));
+            }
         }
+        if (LOGN == 0)
+            for (int j = tid; j < half; j += T) {
+                float2 v0 = ld2(x + j), v1 = ld2(x + j + half);
+                st2(y + 2 * j, make_float2(v0.x + v1.x, v0.y + v1.y));
+                st2(y + 2 * j + 1, make_float2(v0.x - v1.x, v0.y - v1.y));
+            }
         lNs = 1;
         lds_float2* t = x;
         x = y;
@@ -520,7 +534,9 @@ __device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N, int log
         __syncthreads();
     }
     const int q = N >> 2;
-    for (; lNs < logN; lNs += 2) {
+#pragma unroll
+    for (int st = 0; st < (LOGN > 0 ? LOGN / 2 : 16); ++st) {
+        if (LOGN == 0 && lNs >= logN) break;
         const int Ns = 1 << lNs;
         if (tid < q) {
             const int j = tid;
@@ -548,6 +564,7 @@ __device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N, int log
             st2(y + d + 2 * Ns, make_float2(t0.x - t2.x, t0.y - t2.y));
             st2(y + d + 3 * Ns, make_float2(t1.x - it3.x, t1.y - it3.y));
         }
+        lNs += 2;
         lds_float2* t = x;
         x = y;
         y = t;
@@ -556,32 +573,41 @@ __device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N, int log
     return x;
 }
 
-template <typename TP>
+// A row's transform context.  LOGL > 0 fixes the row length L = 2^LOGL (and with it h = L/2, the
+// T = h/4 threads per row, the FFT's stages) at compile time; 0 takes them from the fields.
+template <typename TP, int LOGL>
 struct RowCtx {
-    int L, h, logh, T, tid;
+    int L_, h_, logh_, T_, tid;
     TP tw;  // e^{-2 pi i m / h}, m < h
     TP th;  // e^{-2 pi i k / L}, k <= h
     TP dw;  // (cos, -sin)(pi k / 2L), k < L
     lds_float2* A;
     lds_float2* B;
+    __device__ __forceinline__ int L() const { return LOGL > 0 ? (1 << LOGL) : L_; }
+    __device__ __forceinline__ int h() const { return LOGL > 0 ? (1 << (LOGL - 1)) : h_; }
+    __device__ __forceinline__ int logh() const { return LOGL > 0 ? LOGL - 1 : logh_; }
+    __device__ __forceinline__ int T() const { return LOGL > 0 ? (LOGL >= 3 ? (1 << (LOGL - 3)) : 1) : T_; }
 };
 
 // DCT-II of the real row given by src(j), j < L (Makhoul: v = (x[0], x[2], ..., x[3], x[1]), its
 // DFT through an L/2-point complex FFT of the pairs, C[k] = Re(e^{-i pi k / 2L} V[k])); calls
-// out(k, C[k]).
-template <typename TP, typename SRC, typename OUT>
-__device__ void dct2_row(const RowCtx<TP>& R, bool active, SRC src, OUT out) {
-    const int L = R.L, h = R.h;
-    if (active)
-        for (int m = R.tid; m < h; m += R.T) {
+// out(k, C[k]).  (h is a multiple of T: every loop below runs h / T or L / T times per thread.)
+template <typename TP, int LOGL, typename SRC, typename OUT>
+__device__ void dct2_row(const RowCtx<TP, LOGL>& R, bool active, SRC src, OUT out) {
+    const int L = R.L(), h = R.h(), T = R.T();
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < h / T; ++i) {
+            const int m = R.tid + i * T;
             int q0 = 2 * m, q1 = 2 * m + 1;
             float a = q0 < h ? src(2 * q0) : src(2 * (L - 1 - q0) + 1);
             float b = q1 < h ? src(2 * q1) : src(2 * (L - 1 - q1) + 1);
             st2(R.A + m, make_float2(a, b));
         }
-    lds_float2* Z = fft_stockham(R.A, R.B, h, R.logh, R.T, R.tid, R.tw, -1);
-    if (active)
-        for (int k = R.tid; k <= h; k += R.T) {
+    }
+    lds_float2* Z = fft_stockham<(LOGL > 0 ? LOGL - 1 : 0)>(R.A, R.B, h, R.logh(), T, R.tid, R.tw, -1);
+    if (active) {
+        auto post = [&](int k) {
             float2 zk = ld2(Z + (k == h ? 0 : k)), zc = cconj(ld2(Z + (k == 0 ? 0 : h - k)));
             float2 E = make_float2((zk.x + zc.x) * 0.5f, (zk.y + zc.y) * 0.5f);
             float2 Op = make_float2((zk.x - zc.x) * 0.5f, (zk.y - zc.y) * 0.5f);
@@ -594,7 +620,11 @@ __device__ void dct2_row(const RowCtx<TP>& R, bool active, SRC src, OUT out) {
                 const float2 d1 = ld2(R.dw + (L - k));
                 out(L - k, d1.x * V.x + d1.y * V.y);
             }
-        }
+        };
+#pragma unroll
+        for (int i = 0; i < h / T; ++i) post(R.tid + i * T);
+        if (R.tid == 0) post(h);
+    }
 }
 
 // DCT-III (sine: DST-III) of the row given by coef(j), j < L (Ooura's conventions); calls
@@ -602,9 +632,9 @@ __device__ void dct2_row(const RowCtx<TP>& R, bool active, SRC src, OUT out) {
 // c from b[0] = a[0], b[j] = a[L-j]); the real inverse DFT through an L/2-point complex IFFT of
 // Z[k] = (V[k] + conj V[h-k]) + i e^{2 pi i k / L} (V[k] - conj V[h-k]); y = v / 2 unshuffled,
 // odd outputs negated for the sine.
-template <typename TP, typename F, typename OUT>
-__device__ void dct3_row(const RowCtx<TP>& R, bool active, F coef, bool sine, OUT out) {
-    const int L = R.L, h = R.h;
+template <typename TP, int LOGL, typename F, typename OUT>
+__device__ void dct3_row(const RowCtx<TP, LOGL>& R, bool active, F coef, bool sine, OUT out) {
+    const int L = R.L(), h = R.h(), T = R.T();
     auto V = [&](int q) -> float2 {
         float ck, cl;
         if (!sine) {
@@ -618,22 +648,28 @@ __device__ void dct3_row(const RowCtx<TP>& R, bool active, F coef, bool sine, OU
         float cs = dq.x, sn = -dq.y;
         return make_float2(cs * ck + sn * cl, sn * ck - cs * cl);
     };
-    if (active)
-        for (int k = R.tid; k < h; k += R.T) {
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < h / T; ++i) {
+            const int k = R.tid + i * T;
             float2 vk = V(k), vc = cconj(V(h - k));
             float2 E = make_float2(vk.x + vc.x, vk.y + vc.y), D = make_float2(vk.x - vc.x, vk.y - vc.y);
             float2 wD = cmul(cconj(ld2(R.th + k)), D);
             st2(R.A + k, make_float2(E.x - wD.y, E.y + wD.x));
         }
-    lds_float2* z = fft_stockham(R.A, R.B, h, R.logh, R.T, R.tid, R.tw, +1);
-    if (active)
-        for (int j = R.tid; j < L; j += R.T) {
+    }
+    lds_float2* z = fft_stockham<(LOGL > 0 ? LOGL - 1 : 0)>(R.A, R.B, h, R.logh(), T, R.tid, R.tw, +1);
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < L / T; ++i) {
+            const int j = R.tid + i * T;
             int m = (j & 1) ? (L - 1 - (j >> 1)) : (j >> 1);
             float2 zz = ld2(z + (m >> 1));
             float v = ((m & 1) ? zz.y : zz.x) * 0.5f;
             if (sine && (j & 1)) v = -v;
             out(j, v);
         }
+    }
     __syncthreads();
 }
 
@@ -669,30 +705,28 @@ struct RowArgs {
 // picks it when they fit beside the rows' buffers); else they are read from global memory.
 // LDS: per row the FFT's two buffers (L float2), [ROW_SPECTRUM_POISSON: the spectrum, L floats],
 // then [TL: tw h, th h + 1, dw L float2, the element wavenumbers L floats].
-template <int MODE, bool TL>
+template <int MODE, bool TL, int LOGL>
 __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     extern __shared__ float2 lds2[];
     lds_float2* const lbase = (lds_float2*)lds2;
     using TP = typename std::conditional<TL, const lds_float2*, const float2*>::type;
-    const int T = a.T;
-    const int rpb = blockDim.x / T;
+    RowCtx<TP, LOGL> R;
+    R.L_ = a.L;
+    R.h_ = a.L >> 1;
+    R.logh_ = a.logh;
+    R.T_ = a.T;
+    const int T = R.T(), L = R.L(), h = R.h();
+    const int rpb = LOGL > 0 ? 256 / T : (int)blockDim.x / T;
     const int sub = threadIdx.x / T;
-    const int L = a.L;
-    RowCtx<TP> R;
-    R.L = L;
-    R.h = L >> 1;
-    R.logh = a.logh;
-    R.T = T;
     R.tid = threadIdx.x - sub * T;
     R.A = lbase + (size_t)sub * L;
-    R.B = R.A + R.h;
+    R.B = R.A + h;
     lds_float* extra = (lds_float*)(lbase + (size_t)rpb * L) + (size_t)sub * L;
     const bool spectrum = MODE == ROW_SPECTRUM_POISSON;
     const bool wav = MODE == ROW_POISSON || MODE == ROW_SPECTRUM_POISSON;
     lds_float2* tab = lbase + (size_t)rpb * L + (spectrum ? (size_t)rpb * L / 2 : 0);
-    lds_float* wcol_l = (lds_float*)(tab + 2 * R.h + 1 + L);
+    lds_float* wcol_l = (lds_float*)(tab + 2 * h + 1 + L);
     if constexpr (TL) {
-        const int h = R.h;
         for (int i = threadIdx.x; i < h; i += blockDim.x) st2(tab + i, a.tw[i]);
         for (int i = threadIdx.x; i <= h; i += blockDim.x) st2(tab + h + i, a.th[i]);
         for (int i = threadIdx.x; i < L; i += blockDim.x) st2(tab + 2 * h + 1 + i, a.dw[i]);
@@ -731,26 +765,37 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
             }
             put(a.out0, k, v);
         });
-    } else if (MODE == ROW_POISSON || MODE == ROW_SPECTRUM_POISSON) {
+    } else if (MODE == ROW_SPECTRUM_POISSON) {
+        // the spectrum row, halved and scaled, then the potential p = A / (wx^2 + wy^2) of each
+        // element (DC 0) in place: both field transforms read it
+        const double scale = a.scale;
         const float* Ag = a.in0 + off;
-        if (spectrum) {
-            const double scale = a.scale;
-            dct2_row(R, active, [&](int j) { return Ag[j]; }, [&](int k, float v) {
-                if ((row >= 1 && k == 0) || (row == 0 && k >= 1)) v = (float)((double)v * 0.5);
-                v = (float)((double)v * scale);
-                extra[k] = v;
-            });
-            __syncthreads();
+        dct2_row(R, active, [&](int j) { return Ag[j]; }, [&](int k, float v) {
+            if ((row >= 1 && k == 0) || (row == 0 && k >= 1)) v = (float)((double)v * 0.5);
+            v = (float)((double)v * scale);
+            extra[k] = v;
+        });
+        __syncthreads();
+        const float wu = active ? a.wrow[row] : 0.0f, wu2 = wu * wu;
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < L / T; ++i) {
+                const int v = R.tid + i * T;
+                const float wv = wcol(v);
+                extra[v] = (row == 0 && v == 0) ? 0.0f : extra[v] / (wu2 + wv * wv);
+            }
         }
-        auto A = [&](int v) -> float {
-            if (spectrum) return extra[v];
-            return Ag[v];
-        };
+        __syncthreads();
+        dct3_row(R, active, [&](int v) { return extra[v] * wu; }, false, [&](int j, float y) { put(a.out0, j, y); });
+        dct3_row(R, active, [&](int v) { return extra[v] * wcol(v); }, true, [&](int j, float y) { put(a.out1, j, y); });
+        if (a.want_phi) dct3_row(R, active, [&](int v) { return extra[v]; }, false, [&](int j, float y) { put(a.out2, j, y); });
+    } else if (MODE == ROW_POISSON) {
+        const float* A = a.in0 + off;
         const float wu = active ? a.wrow[row] : 0.0f, wu2 = wu * wu;
         auto pot = [&](int v) -> float {
             if (row == 0 && v == 0) return 0.0f;
             float wv = wcol(v);
-            return A(v) / (wu2 + wv * wv);
+            return A[v] / (wu2 + wv * wv);
         };
         dct3_row(R, active, [&](int v) { return pot(v) * wu; }, false, [&](int j, float y) { put(a.out0, j, y); });
         dct3_row(R, active, [&](int v) { return pot(v) * wcol(v); }, true, [&](int j, float y) { put(a.out1, j, y); });
@@ -1663,10 +1708,13 @@ static void launch_rows(hipStream_t s, int mode, RowArgs a) {
     const size_t tab_lds = sizeof(float2) * (2 * (size_t)a.L + 1) + (wav ? 4 * (size_t)a.L : 0);
     const bool tl = rows_lds + tab_lds <= 64 * 1024;
     const size_t lds = rows_lds + (tl ? tab_lds : 0);
-#define GPL_ROWS(M)                                                                                     \
-    do {                                                                                                \
-        if (tl) hipLaunchKernelGGL((k_rows<M, true>), dim3(grid), dim3(block), lds, s, a);              \
-        else hipLaunchKernelGGL((k_rows<M, false>), dim3(grid), dim3(block), lds, s, a);                \
+#define GPL_ROWS(M)                                                                                        \
+    do {                                                                                                   \
+        if (tl && a.L == 1024) hipLaunchKernelGGL((k_rows<M, true, 10>), dim3(grid), dim3(block), lds, s, a); \
+        else if (tl && a.L == 512) hipLaunchKernelGGL((k_rows<M, true, 9>), dim3(grid), dim3(block), lds, s, a); \
+        else if (tl && a.L == 256) hipLaunchKernelGGL((k_rows<M, true, 8>), dim3(grid), dim3(block), lds, s, a); \
+        else if (tl) hipLaunchKernelGGL((k_rows<M, true, 0>), dim3(grid), dim3(block), lds, s, a);          \
+        else hipLaunchKernelGGL((k_rows<M, false, 0>), dim3(grid), dim3(block), lds, s, a);                 \
     } while (0)
     switch (mode) {
     case ROW_DCT2: GPL_ROWS(ROW_DCT2); break;
