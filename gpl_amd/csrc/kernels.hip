@@ -938,19 +938,50 @@ __device__ __forceinline__ float wa_term_max(float S0, float S1, float v, float 
     return (float)(num / (double)(S0 * S0));
 }
 
+// The quotient num / den of getWireLengthGradientPinWA's terms, correctly rounded in double as
+// the division gives it, from the net's correctly rounded reciprocal y = 1 / den: q = num y, then
+// one correction q + (num - q den) y (the residual exact by FMA), which is the correctly rounded
+// quotient (Markstein's theorem; normal operands: den = S0^2 with 1 <= S0 <= the net's degree).
+// One division per net and sum instead of one per pin and term.
+struct WaRcp {
+    double min_x, max_x, min_y, max_y;  // 1 / (double)(S0 * S0) of the four exponential sums
+};
+__device__ __forceinline__ WaRcp wa_rcp(float4 SX, float4 SY) {
+    WaRcp r;
+    r.min_x = 1.0 / (double)(SX.x * SX.x);
+    r.max_x = 1.0 / (double)(SX.z * SX.z);
+    r.min_y = 1.0 / (double)(SY.x * SY.x);
+    r.max_y = 1.0 / (double)(SY.z * SY.z);
+    return r;
+}
+__device__ __forceinline__ float quot(double num, float S0, double y) {
+    const double den = (double)(S0 * S0);
+    const double q = num * y;
+    const double e = fma(-q, den, num);
+    return (float)fma(e, y, q);
+}
+__device__ __forceinline__ float wa_term_min_r(float S0, float S1, float v, float coef, float pc, double y) {
+    double num = (double)S0 * ((double)v * (1.0 - (double)(coef * pc))) + (double)(coef * v * S1);
+    return quot(num, S0, y);
+}
+__device__ __forceinline__ float wa_term_max_r(float S0, float S1, float v, float coef, float pc, double y) {
+    double num = (double)S0 * ((double)v * (1.0 + (double)(coef * pc))) - (double)(coef * v * S1);
+    return quot(num, S0, y);
+}
+
 // One pin's weighted WA gradient (getWireLengthGradientPinWA, nesterov.rs:2392-2418, times the net
 // weight) stored at its cell-pin index.
 __device__ __forceinline__ void pin_grad(const NetsDev& n, int q, int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float4 SX,
-                                         float4 SY, float w) {
+                                         float4 SY, float w, const WaRcp& rc) {
     if (q < 0) return;
     const float fpx = (float)p.x, fpy = (float)p.y;
     float gmin_x = 0.0f, gmax_x = 0.0f, gmin_y = 0.0f, gmax_y = 0.0f;
     const float e0 = (float)(lx - p.x) * cx, e1 = (float)(p.x - ux) * cx;
     const float e2 = (float)(ly - p.y) * cy, e3 = (float)(p.y - uy) * cy;
-    if (e0 > -300.0f) gmin_x = wa_term_min(SX.x, SX.y, fast_exp(e0), cx, fpx);
-    if (e1 > -300.0f) gmax_x = wa_term_max(SX.z, SX.w, fast_exp(e1), cx, fpx);
-    if (e2 > -300.0f) gmin_y = wa_term_min(SY.x, SY.y, fast_exp(e2), cy, fpy);
-    if (e3 > -300.0f) gmax_y = wa_term_max(SY.z, SY.w, fast_exp(e3), cy, fpy);
+    if (e0 > -300.0f) gmin_x = wa_term_min_r(SX.x, SX.y, fast_exp(e0), cx, fpx, rc.min_x);
+    if (e1 > -300.0f) gmax_x = wa_term_max_r(SX.z, SX.w, fast_exp(e1), cx, fpx, rc.max_x);
+    if (e2 > -300.0f) gmin_y = wa_term_min_r(SY.x, SY.y, fast_exp(e2), cy, fpy, rc.min_y);
+    if (e3 > -300.0f) gmax_y = wa_term_max_r(SY.z, SY.w, fast_exp(e3), cy, fpy, rc.max_y);
     float tx = gmin_x - gmax_x, ty = gmin_y - gmax_y;
     tx *= w;
     ty *= w;
@@ -959,14 +990,14 @@ __device__ __forceinline__ void pin_grad(const NetsDev& n, int q, int2 p, int lx
 
 // pin_grad from the exponentials wa_pin_keep kept.
 __device__ __forceinline__ void pin_grad_kept(const NetsDev& n, int q, int2 p, float cx, float cy, float4 SX, float4 SY, float w,
-                                              float4 ev) {
+                                              float4 ev, const WaRcp& rc) {
     if (q < 0) return;
     const float fpx = (float)p.x, fpy = (float)p.y;
     float gmin_x = 0.0f, gmax_x = 0.0f, gmin_y = 0.0f, gmax_y = 0.0f;
-    if (ev.x >= 0.0f) gmin_x = wa_term_min(SX.x, SX.y, ev.x, cx, fpx);
-    if (ev.y >= 0.0f) gmax_x = wa_term_max(SX.z, SX.w, ev.y, cx, fpx);
-    if (ev.z >= 0.0f) gmin_y = wa_term_min(SY.x, SY.y, ev.z, cy, fpy);
-    if (ev.w >= 0.0f) gmax_y = wa_term_max(SY.z, SY.w, ev.w, cy, fpy);
+    if (ev.x >= 0.0f) gmin_x = wa_term_min_r(SX.x, SX.y, ev.x, cx, fpx, rc.min_x);
+    if (ev.y >= 0.0f) gmax_x = wa_term_max_r(SX.z, SX.w, ev.y, cx, fpx, rc.max_x);
+    if (ev.z >= 0.0f) gmin_y = wa_term_min_r(SY.x, SY.y, ev.z, cy, fpy, rc.min_y);
+    if (ev.w >= 0.0f) gmax_y = wa_term_max_r(SY.z, SY.w, ev.w, cy, fpy, rc.max_y);
     float tx = gmin_x - gmax_x, ty = gmin_y - gmax_y;
     tx *= w;
     ty *= w;
@@ -1074,13 +1105,14 @@ __global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, flo
             R.sy = SY;
         }
         const float wt = n.weight[e];
+        const WaRcp rc = wa_rcp(SX, SY);
 #pragma unroll
         for (int i = 0; i < kBigRegs; ++i) {
             const int k = s + threadIdx.x + i * kBigBlock;
-            if (k < t) pin_grad(n, cq[i], pos[i], lx, ly, ux, uy, cx, cy, SX, SY, wt);
+            if (k < t) pin_grad(n, cq[i], pos[i], lx, ly, ux, uy, cx, cy, SX, SY, wt, rc);
         }
         for (int k = rest + threadIdx.x; k < t; k += blockDim.x)
-            pin_grad(n, n.cpi[k], pin_pos(n, c, k), lx, ly, ux, uy, cx, cy, SX, SY, wt);
+            pin_grad(n, n.cpi[k], pin_pos(n, c, k), lx, ly, ux, uy, cx, cy, SX, SY, wt, rc);
     }
     if (threadIdx.x == 0 && ux >= lx) atomicAdd(hpwl, (unsigned long long)((long long)ux - lx + (long long)uy - ly));
 }
@@ -1161,10 +1193,11 @@ __global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, floa
                     R.sy = b;
                 }
                 const float wt = n.weight[e];
+                const WaRcp rc = wa_rcp(a, b);
 #pragma unroll
                 for (int k = 0; k < kKeepPins; ++k)
-                    if (k < deg) pin_grad_kept(n, scq[s + k], spos[s + k], cx, cy, a, b, wt, ev[k]);
-                for (int k = kKeepPins; k < deg; ++k) pin_grad(n, scq[s + k], spos[s + k], lx, ly, ux, uy, cx, cy, a, b, wt);
+                    if (k < deg) pin_grad_kept(n, scq[s + k], spos[s + k], cx, cy, a, b, wt, ev[k], rc);
+                for (int k = kKeepPins; k < deg; ++k) pin_grad(n, scq[s + k], spos[s + k], lx, ly, ux, uy, cx, cy, a, b, wt, rc);
             }
             if (ux >= lx) h += (unsigned long long)((long long)ux - lx + (long long)uy - ly);
         }
