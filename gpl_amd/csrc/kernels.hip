@@ -336,7 +336,10 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
             auto add = [&](int x, int y, int ex0, int ey0, int ex1, int ey1) {
                 float a = overlap_area(ex0, ey0, ex1, ey1, d) * sc;
                 if (macro) a *= r.target_density;
-                const long long v = (long long)a;
+                // the reference's `as i64` (truncation); with 32-bit tables every overlap is below
+                // 2^32 (at most its bin's area), so the one-instruction unsigned conversion gives
+                // the same integer
+                const long long v = sizeof(V) == 4 ? (long long)(unsigned)a : (long long)a;
                 if (v == 0) return;
                 if (dense) {
                     atomicAdd(&vals[(y - by0) * BW + (x - bx0)], (V)v);
