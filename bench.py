@@ -65,9 +65,9 @@ def algorithmic_bytes(fam, N, P, E, M):
     wa_forward      32 P + 8 E    per pin: cell (4), offset (8), the cell's centre (8), cell-pin
                                   index (4), its gradient term written (8); per net: start (4),
                                   weight (4)
-    gradient        80 N + 8 P + 8 M   per cell: box, pin start, preconditioner, scale (32), 3 slots
-                                  read, 3 written (48); per cell-pin: its gradient term (8); the two
-                                  fields once (8 per bin)
+    gradient        64 N + 8 P + 8 M   per cell: box, pin start, preconditioner, scale (32), the 3
+                                  slots the step length reads (24), the preconditioned sum written (8);
+                                  per cell-pin: its gradient term (8); the two fields once (8 per bin)
     """
     return {
         "update": 81 * N,
@@ -75,7 +75,7 @@ def algorithmic_bytes(fam, N, P, E, M):
         "bin_density": 52 * M,
         "dct_solve": 12 * M,
         "wa_forward": 32 * P + 8 * E,
-        "gradient": 80 * N + 8 * P + 8 * M,
+        "gradient": 64 * N + 8 * P + 8 * M,
     }.get(fam)
 
 

@@ -1303,8 +1303,10 @@ __global__ void k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float 
         float pre = rmax(pr.x + penalty * pr.y, 1.0f);
         sx /= pre;
         sy /= pre;
-        wl_out[k] = make_float2(wx, wy);
-        den_out[k] = make_float2(fx, fy);
+        if (wl_out) {
+            wl_out[k] = make_float2(wx, wy);
+            den_out[k] = make_float2(fx, fy);
+        }
         sum_out[k] = make_float2(sx, sy);
         s_wl += (double)fabsf(wx) + (double)fabsf(wy);
         s_den += (double)fabsf(fx) + (double)fabsf(fy);

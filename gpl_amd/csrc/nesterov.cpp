@@ -293,12 +293,15 @@ struct gpl_ctx {
     }
 
     // The gradient of region b into `slot`; with `trial`, the distances of the step length too, and
-    // the bins' overflow sums finished together with the gradient's (field(false) left them).
-    void launch_gradient(int b, int slot, bool trial, int nan_inject = 0) {
+    // the bins' overflow sums finished together with the gradient's (field(false) left them).  The
+    // WA and density parts are stored apart only for the kernel-level entries (`parts`): the loop
+    // reads the preconditioned sum alone.
+    void launch_gradient(int b, int slot, bool trial, int nan_inject = 0, bool parts = false) {
         Region& r = rg[b];
         int base = slot == SLOT_PREV ? S_PREV_SLP : slot == SLOT_CUR ? S_CUR_SLP : S_NEXT_SLP;
         mark(F_GRAD);
-        gplk::launch_gradient(stream, r.dev, cells, nets, wl_coef_x, wl_coef_y, r.density_penalty, r.v[base + 1], r.v[base + 2],
+        gplk::launch_gradient(stream, r.dev, cells, nets, wl_coef_x, wl_coef_y, r.density_penalty, parts ? r.v[base + 1] : nullptr,
+                              parts ? r.v[base + 2] : nullptr,
                               r.v[base + 3], trial ? r.v[S_CUR_SLP] : nullptr, trial ? r.v[S_NEXT_SLP] : nullptr,
                               trial ? r.v[S_CUR_SLP_SUM] : nullptr, r.partials, nan_inject);
         gplk::launch_finish(stream, r.partials, trial ? gplk::block_partials_bins(r.dev) : 0, gplk::block_partials_cells(r.dev), d_sums + b,
@@ -632,8 +635,8 @@ struct gpl_ctx {
                 e = gplk::set_scatter_node(texec, n, r.dev, cells, d_global_adds);
             } else if (n.kind == gplk::TK_GRADIENT) {
                 Region& r = *region_of(n.cell_begin);
-                e = gplk::set_gradient_node(texec, n, r.dev, cells, nets, wl_coef_x, wl_coef_y, r.density_penalty, r.v[S_NEXT_SLP + 1],
-                                            r.v[S_NEXT_SLP + 2], r.v[S_NEXT_SLP + 3], r.v[S_CUR_SLP], r.v[S_NEXT_SLP],
+                e = gplk::set_gradient_node(texec, n, r.dev, cells, nets, wl_coef_x, wl_coef_y, r.density_penalty, nullptr,
+                                            nullptr, r.v[S_NEXT_SLP + 3], r.v[S_CUR_SLP], r.v[S_NEXT_SLP],
                                             r.v[S_CUR_SLP_SUM], r.partials, nan);
             } else {
                 e = gplk::set_wa_node(texec, n, nets, cells, wl_coef_x, wl_coef_y, d_hpwl);
@@ -1784,7 +1787,7 @@ static int32_t gradients_out(gpl_ctx* c, int32_t region, float penalty, float* w
         Region& r = c->rg[region];
         float saved = r.density_penalty;
         r.density_penalty = penalty;
-        c->launch_gradient(region, slot, false);
+        c->launch_gradient(region, slot, false, 0, true);
         r.density_penalty = saved;
         int base = slot == SLOT_PREV ? S_PREV_SLP : slot == SLOT_CUR ? S_CUR_SLP : S_NEXT_SLP;
         c->sync_stats();
