@@ -143,9 +143,11 @@ void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const
 // finish: sums of the block partials into *out (fixed order).
 // the FP64 partials finished into *out (and *hpwl copied); with host_out (pinned host memory) the
 // same values also written there, and with host_tail the HPWL counter and the global-add count of
-// dev_tail (the Sums record that holds them) copied to host_tail
+// dev_tail (the Sums record that holds them) copied to host_tail; with host_seq, after all of
+// that is visible to the host, *dev_seq is incremented and its new value written to *host_seq
 void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int nparts_cells, Sums* out, const unsigned long long* hpwl,
-                   Sums* host_out = nullptr, const Sums* dev_tail = nullptr, Sums* host_tail = nullptr);
+                   Sums* host_out = nullptr, const Sums* dev_tail = nullptr, Sums* host_tail = nullptr,
+                   unsigned long long* dev_seq = nullptr, unsigned long long* host_seq = nullptr);
 // squared distance between two coordinate sets (prev/cur at init).
 void launch_distance(hipStream_t s, int n, const float2* a, const float2* b, const float2* c, const float2* d, double* partials);
 // initial prev_slp: cur_slp - c * cur_sum, kept inside.

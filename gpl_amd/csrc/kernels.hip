@@ -1373,7 +1373,8 @@ __global__ void k_initial_prev(RegionDev r, CellsDev c, const float2* cur_slp, c
 constexpr int kFinishBlock = 1024;
 __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials, int nb_bins, int nb_cells, Sums* out,
                                                          const unsigned long long* hpwl, Sums* host_out,
-                                                         const Sums* dev_tail, Sums* host_tail) {
+                                                         const Sums* dev_tail, Sums* host_tail, unsigned long long* dev_seq,
+                                                         unsigned long long* host_seq) {
     __shared__ double sh[6][kFinishBlock / 64];
     double v[6];
 #pragma unroll
@@ -1413,6 +1414,18 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials,
         host_tail->hpwl = dev_tail->hpwl;
         host_tail->pad[0] = dev_tail->pad[0];
         host_tail->pad[1] = dev_tail->pad[1];
+    }
+    if (host_seq) {
+        // the trial's last finishing reduction: once every host write above is visible system-wide,
+        // the next sequence number tells the waiting host that the statistics are complete
+        if (threadIdx.x < 6) __threadfence_system();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long q = *dev_seq + 1ull;
+            *dev_seq = q;
+            __threadfence_system();
+            *reinterpret_cast<volatile unsigned long long*>(host_seq) = q;
+        }
     }
 }
 
@@ -1928,9 +1941,9 @@ void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const
 }
 
 void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int nparts_cells, Sums* out, const unsigned long long* hpwl,
-                   Sums* host_out, const Sums* dev_tail, Sums* host_tail) {
+                   Sums* host_out, const Sums* dev_tail, Sums* host_tail, unsigned long long* dev_seq, unsigned long long* host_seq) {
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishBlock), 0, s, partials, nparts_bins, nparts_cells, out, hpwl, host_out, dev_tail,
-                       host_tail);
+                       host_tail, dev_seq, host_seq);
 }
 
 void launch_distance(hipStream_t s, int n, const float2* a, const float2* b, const float2* c, const float2* d, double* partials) {

@@ -92,6 +92,13 @@ struct gpl_ctx {
     std::vector<Region> rg;
     gplk::Sums* d_sums = nullptr;  // one per region
     gplk::Sums* h_sums = nullptr;  // pinned host copy
+    // the trial graph's completion signal: its last k_finish bumps d_seq and writes the value to
+    // the pinned h_seq after the statistics; the host spins on h_seq instead of sleeping in a
+    // stream synchronisation
+    unsigned long long* d_seq = nullptr;
+    unsigned long long* h_seq = nullptr;
+    unsigned long long seq_want = 0;
+    int signal_region = -1;  // the region whose gradient closes the trial graph
     unsigned long long* d_hpwl = nullptr;
     unsigned long long* d_global_adds = nullptr;  // the scatter's atomic adds into the HBM bins
     uint8_t* d_flags = nullptr;
@@ -178,6 +185,7 @@ struct gpl_ctx {
         if (stream) hipStreamSynchronize(stream);
         for (void* p : allocs) hipFree(p);
         if (h_sums) hipHostFree(h_sums);
+        if (h_seq) hipHostFree(h_seq);
         if (profiling)
             for (int i = 0; i < 64; ++i) hipEventDestroy(ev[i]);
         if (aux) hipStreamSynchronize(aux);
@@ -304,9 +312,10 @@ struct gpl_ctx {
                               parts ? r.v[base + 2] : nullptr,
                               r.v[base + 3], trial ? r.v[S_CUR_SLP] : nullptr, trial ? r.v[S_NEXT_SLP] : nullptr,
                               trial ? r.v[S_CUR_SLP_SUM] : nullptr, r.partials, nan_inject);
+        const bool sig = finish_to_host && b == signal_region;
         gplk::launch_finish(stream, r.partials, trial ? gplk::block_partials_bins(r.dev) : 0, gplk::block_partials_cells(r.dev), d_sums + b,
                             d_hpwl, finish_to_host ? h_sums + b : nullptr, finish_to_host ? d_sums + rg.size() : nullptr,
-                            finish_to_host ? h_sums + rg.size() : nullptr);
+                            finish_to_host ? h_sums + rg.size() : nullptr, sig ? d_seq : nullptr, sig ? h_seq : nullptr);
         mark_end();
     }
 
@@ -545,6 +554,20 @@ struct gpl_ctx {
     // each launch the nodes whose arguments change (step, momentum, coefficients, penalty, the
     // rotated slots, the cell arrays a re-sort swapped) get their new arguments.  One launch and
     // one wait per trial instead of ~20 API calls.
+    // Waits for the trial graph's completion signal (spinning: the statistics are then complete in
+    // pinned memory, and the stream's later work queues behind the graph as usual); after 2 s
+    // without it, a stream synchronisation reports whatever went wrong.
+    void wait_trial_signal() {
+        const unsigned long long want = ++seq_want;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t k = 0;; ++k) {
+            if (*reinterpret_cast<volatile unsigned long long*>(h_seq) >= want) return;
+            if ((k & 1023u) == 1023u && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) break;
+        }
+        check(hipStreamSynchronize(stream), "sync");
+        if (*reinterpret_cast<volatile unsigned long long*>(h_seq) < want) throw HipError{hipErrorUnknown, "trial signal"};
+    }
+
     void drop_graph() {
         if (texec) hipGraphExecDestroy(texec);
         if (tgraph) hipGraphDestroy(tgraph);
@@ -574,6 +597,9 @@ struct gpl_ctx {
             join();
             // the trial's statistics go straight to the pinned host copy: the graph holds no copy node
             finish_to_host = true;
+            signal_region = -1;
+            for (size_t b = 0; b < rg.size(); ++b)
+                if (!rg[b].is_converged) signal_region = int(b);
             launch_gradients(0, int(rg.size()), SLOT_NEXT, true);
             finish_to_host = false;
         } catch (const HipError& e) {
@@ -646,7 +672,7 @@ struct gpl_ctx {
         const clk::time_point t1 = clk::now();
         check(hipGraphLaunch(texec, stream), "graph launch");
         const clk::time_point t2 = clk::now();
-        check(hipStreamSynchronize(stream), "sync");
+        wait_trial_signal();
         const clk::time_point t3 = clk::now();
         gradients_read(0, int(rg.size()), SLOT_NEXT, true);
         host_ms[H_PARAMS] += ms_since(t0, t1);
@@ -1464,6 +1490,10 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         check(hipMemsetAsync(c->d_sums, 0, sizeof(gplk::Sums) * (size_t(p->num_regions) + 1), c->stream), "memset");
         check(hipHostMalloc((void**)&c->h_sums, sizeof(gplk::Sums) * (size_t(p->num_regions) + 1), hipHostMallocDefault), "pinned");
         c->d_hpwl = &c->d_sums[p->num_regions].hpwl;
+        c->d_seq = c->dalloc<unsigned long long>(1);
+        check(hipMemsetAsync(c->d_seq, 0, sizeof(unsigned long long), c->stream), "memset");
+        check(hipHostMalloc((void**)&c->h_seq, sizeof(unsigned long long), hipHostMallocDefault), "pinned");
+        *c->h_seq = 0ull;
         c->d_global_adds = (unsigned long long*)&c->d_sums[p->num_regions].pad[0];
 
         // the region table of the re-sort; per-region scratch vectors for its gathers
