@@ -45,7 +45,7 @@ FAMILY_KERNELS = {
     "update": ["k_place_update"],
     "scatter": ["k_place_scatter<2, unsigned int>", "k_place_scatter<2, unsigned long long>"],
     "bin_density": ["k_bin_density"],
-    "dct_solve": ["k_rows<0, true>", "k_rows<6, true>", "k_rows<3, true>"],
+    "dct_solve": ["k_rows<0, true, 10>", "k_rows<6, true, 10>", "k_rows<3, true, 10>"],
     "wa_forward": ["k_wa_tiles<true>", "k_wa_big<true>"],
     "gradient": ["k_gradient"],
 }
