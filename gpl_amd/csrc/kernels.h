@@ -168,11 +168,14 @@ void launch_objective_gradient(hipStream_t s, const RegionDev& r, const CellsDev
 
 // ---- spatial re-sort of the cells (device order only; results do not depend on it) ---------
 constexpr int kMaxRegions = 32;
+constexpr int kResortShift = 2;  // the sort key's Morton code runs over 4 x 4-bin squares
 struct RegionTable {
     int32_t n;
     int32_t begin[kMaxRegions + 1];
     int32_t lx[kMaxRegions], ly[kMaxRegions], cnt_x[kMaxRegions], cnt_y[kMaxRegions];
     double size_x[kMaxRegions], size_y[kMaxRegions];
+    int32_t morton_bits;  // bits of the Morton part of a key (the region index above them)
+    int32_t end_bit;      // key bits the radix sort looks at
 };
 struct ResortBufs {
     uint32_t* keys_in;
@@ -192,6 +195,14 @@ void launch_resort_sort(hipStream_t s, int n_cells, const int2* ctr, const Regio
 void launch_resort_cells(hipStream_t s, int n_cells, const CellsDev& in, const CellsDev& out, ResortBufs& b, int32_t* cpi);
 // one region's per-cell float pairs gathered: out[k] = in[perm[begin + k] - begin]
 void launch_gather_f2(hipStream_t s, int n, int begin, const int32_t* perm, const float2* in, float2* out);
+// up to kGatherMax such vectors of one region in one launch
+constexpr int kGatherMax = 9;
+struct GatherSet {
+    const float2* in[kGatherMax];
+    float2* out[kGatherMax];
+    int32_t count;
+};
+void launch_gather_f2_set(hipStream_t s, int n, int begin, const int32_t* perm, const GatherSet& g);
 // net pins' cells renamed: pcell[k] = inv[pcell[k]]
 void launch_remap_pcell(hipStream_t s, int n_pins, int32_t* pcell, const int32_t* inv);
 

@@ -2015,7 +2015,9 @@ __global__ void k_resort_keys(int N, const int2* ctr, RegionTable t, uint32_t* k
             bx = (uint32_t)ix;
             by = (uint32_t)iy;
         }
-        keys[g] = ((uint32_t)b << 22) | spread_bits(bx) | (spread_bits(by) << 1);
+        bx >>= kResortShift;
+        by >>= kResortShift;
+        keys[g] = ((uint32_t)b << t.morton_bits) | spread_bits(bx) | (spread_bits(by) << 1);
         idx[g] = g;
     }
 }
@@ -2064,7 +2066,7 @@ __global__ void k_remap_pcell(int P, int32_t* pcell, const int32_t* inv) {
 size_t resort_temp_bytes(int N) {
     size_t a = 0, b = 0;
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, a, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, N,
-                                             0, 27);
+                                             0, 32);
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (int32_t*)nullptr, (int32_t*)nullptr, N + 1);
     return a > b ? a : b;
 }
@@ -2072,7 +2074,7 @@ size_t resort_temp_bytes(int N) {
 void launch_resort_sort(hipStream_t s, int N, const int2* ctr, const RegionTable& t, ResortBufs& b) {
     hipLaunchKernelGGL(k_resort_keys, dim3(grid_for(N)), dim3(kBlock), 0, s, N, ctr, t, b.keys_in, b.idx_in);
     size_t bytes = b.temp_bytes;
-    (void)hipcub::DeviceRadixSort::SortPairs(b.temp, bytes, b.keys_in, b.keys_out, b.idx_in, b.perm, N, 0, 27, s);
+    (void)hipcub::DeviceRadixSort::SortPairs(b.temp, bytes, b.keys_in, b.keys_out, b.idx_in, b.perm, N, 0, t.end_bit, s);
     hipLaunchKernelGGL(k_resort_inv, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, b.inv);
 }
 
@@ -2086,6 +2088,20 @@ void launch_resort_cells(hipStream_t s, int N, const CellsDev& in, const CellsDe
 void launch_gather_f2(hipStream_t s, int n, int begin, const int32_t* perm, const float2* in, float2* out) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_gather_f2, dim3(grid_for(n)), dim3(kBlock), 0, s, n, begin, perm, in, out);
+}
+
+__global__ void k_gather_f2_set(int n, int begin, const int32_t* perm, GatherSet g) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int from = perm[begin + k] - begin;
+#pragma unroll
+        for (int q = 0; q < kGatherMax; ++q)
+            if (q < g.count) g.out[q][k] = g.in[q][from];
+    }
+}
+
+void launch_gather_f2_set(hipStream_t s, int n, int begin, const int32_t* perm, const GatherSet& g) {
+    if (n <= 0 || g.count <= 0) return;
+    hipLaunchKernelGGL(k_gather_f2_set, dim3(grid_for(n)), dim3(kBlock), 0, s, n, begin, perm, g);
 }
 
 void launch_remap_pcell(hipStream_t s, int P, int32_t* pcell, const int32_t* inv) {

@@ -113,7 +113,7 @@ struct gpl_ctx {
     gplk::CellsDev cells_alt{};
     gplk::ResortBufs rb{};
     gplk::RegionTable rtab{};
-    std::vector<float2*> region_scratch;
+    std::vector<std::vector<float2*>> region_scratch;  // per region, kGatherMax vectors
     int32_t net_pin_count = 0;
     bool resort_enabled = true;
     uint64_t last_global_adds = 0;
@@ -834,16 +834,22 @@ struct gpl_ctx {
         gplk::launch_resort_cells(stream, N, cells, cells_alt, rb, nets.cpi);
         std::swap(cells, cells_alt);
         d_flags = cells.flags;
-        // the vectors the loop reads again: the coordinates, the preconditioned gradients and the
-        // snapshots (the wl / den vectors and the next-point slots are rewritten before any read)
+        // the vectors the loop reads again: the coordinates, the preconditioned gradients and, while
+        // a snapshot can still be restored, the snapshots (the wl / den vectors and the next-point
+        // slots are rewritten before any read); all of a region's in one launch
         static const int live[] = {S_CUR_SLP, S_CUR_SLP_SUM, S_PREV_SLP, S_PREV_SLP_SUM, S_CUR, S_SNAP, S_SNAP_SLP, S_SNAP_SLP_SUM,
                                    S_SNAP_PREV_SLP_SUM};
+        const int nlive = (diverge_saved || diverge_reverts > 0) ? 9 : 5;
         for (size_t b = 0; b < rg.size(); ++b) {
             Region& r = rg[b];
-            for (int q : live) {
-                gplk::launch_gather_f2(stream, r.n, r.R.cell_begin, rb.perm, r.v[q], region_scratch[b]);
-                std::swap(r.v[q], region_scratch[b]);
+            gplk::GatherSet gs{};
+            gs.count = nlive;
+            for (int i = 0; i < nlive; ++i) {
+                gs.in[i] = r.v[live[i]];
+                gs.out[i] = region_scratch[b][i];
             }
+            gplk::launch_gather_f2_set(stream, r.n, r.R.cell_begin, rb.perm, gs);
+            for (int i = 0; i < nlive; ++i) std::swap(r.v[live[i]], region_scratch[b][i]);
         }
         gplk::launch_remap_pcell(stream, net_pin_count, nets.pcell, rb.inv);
         mark_end();
@@ -1510,7 +1516,22 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             c->rtab.size_x[b] = R.size_x;
             c->rtab.size_y[b] = R.size_y;
         }
-        for (int32_t b = 0; b < p->num_regions; ++b) c->region_scratch.push_back(c->dalloc<float2>(std::max(1, c->rg[b].n)));
+        {
+            // the sort key: the region index above the Morton code of the cell's 4 x 4-bin square
+            int32_t maxc = 1;
+            for (int32_t b = 0; b < c->rtab.n; ++b) maxc = std::max({maxc, c->rtab.cnt_x[b], c->rtab.cnt_y[b]});
+            int32_t side = (maxc + (1 << gplk::kResortShift) - 1) >> gplk::kResortShift, bits = 1;
+            while ((1 << bits) < side) ++bits;
+            int32_t rbits = 0;
+            while ((1 << rbits) < c->rtab.n) ++rbits;
+            c->rtab.morton_bits = 2 * bits;
+            c->rtab.end_bit = 2 * bits + rbits;
+            if (c->rtab.end_bit > 32 || bits > 11) c->resort_enabled = false;
+        }
+        for (int32_t b = 0; b < p->num_regions; ++b) {
+            c->region_scratch.emplace_back();
+            for (int i = 0; i < gplk::kGatherMax; ++i) c->region_scratch.back().push_back(c->dalloc<float2>(std::max(1, c->rg[b].n)));
+        }
         check(hipStreamSynchronize(c->stream), "sync");
     } catch (const HipError& e) {
         fprintf(stderr, "gpl_ctx_create: %s: %s\n", e.what, hipGetErrorString(e.e));
