@@ -88,12 +88,12 @@ struct NetsDev {
     float2* pgrad;          // per cell-pin: the pin's weighted WA gradient, written by the WA kernel
     NetRec* rec;            // box and sums per net, written when asked for (the kernel-level entry)
     int32_t num_small;      // nets [0, num_small) go by tiles; the rest take a block each
-    // the small nets in 8 contiguous slices of equal pin counts, slice x cut into tiles
-    // [tile_slice[x], tile_slice[x+1]) of consecutive nets (the kernel gives slice x to XCD x)
-    int32_t slice[9];
-    const int32_t* tile;    // tile t holds nets [tile[t], tile[t+1])
-    int32_t tile_slice[9];
-    int32_t tile_blocks;    // the most tiles of a slice: workgroups per slice
+    // Tile t holds the small nets whose first pin lies in [t kTileSpan, (t+1) kTileSpan), nets
+    // [tile[t], tile[t+1]) (at most kTileSpan + 127 < kTilePins pins); XCD x takes the tiles
+    // [x tile_blocks, (x+1) tile_blocks)
+    const int32_t* tile;    // num_tiles + 1
+    int32_t num_tiles;
+    int32_t tile_blocks;
 };
 
 // Per-trial FP64 reductions of one region, finished into this struct by k_finish.
@@ -107,9 +107,9 @@ struct Sums {
 
 constexpr int kPartials = 2048;  // per-block partial slots per quantity
 constexpr int kPlaceChunk = 512;  // cells per block iteration of the scatter family
-constexpr int kTilePins = 2048;   // pins of a WA tile (at least the largest small net, 128)
-constexpr int kTileNets = 512;    // nets of a WA tile
-static_assert(kTilePins >= 128 && kTilePins % 256 == 0, "a WA tile holds any small net, staged by 256 threads");
+constexpr int kTilePins = 2048;   // pins a WA tile stages in LDS
+constexpr int kTileSpan = kTilePins - 128;  // pin range whose nets form a tile (a small net has <= 128 pins)
+static_assert(kTilePins >= 256 && kTilePins % 256 == 0, "a WA tile holds any small net, staged by 256 threads");
 
 // ---- launchers (stream-ordered, asynchronous) -----------------------------------------------
 void launch_zero_bins(hipStream_t s, const RegionDev& r);
@@ -205,6 +205,35 @@ struct GatherSet {
 void launch_gather_f2_set(hipStream_t s, int n, int begin, const int32_t* perm, const GatherSet& g);
 // net pins' cells renamed: pcell[k] = inv[pcell[k]]
 void launch_remap_pcell(hipStream_t s, int n_pins, int32_t* pcell, const int32_t* inv);
+
+// ---- net re-sort (after a cell re-sort): the small nets ordered again by (window of 4096 cells of
+// their first cell, degree), stably, as at construction, so that a WA tile's nets gather the
+// centres of nearby cells; the high-fanout nets keep their places at the end
+struct NetResortBufs {
+    unsigned long long* keys_in;
+    unsigned long long* keys_out;
+    int32_t* idx_in;
+    int32_t* perm;     // new -> old (small nets)
+    int32_t* deg;      // degrees in the new order, scanned into the new starts
+    void* temp;
+    size_t temp_bytes;
+    int32_t end_bit;   // key bits the sort looks at
+    // the alternate net arrays the permutation writes (swapped with NetsDev's after)
+    int32_t* start;
+    int32_t* pcell;
+    int2* poff;
+    int2* pxy0;
+    int32_t* cpi;
+    float* weight;
+    int32_t* orig;     // the problem's index of each net (device order)
+};
+size_t net_resort_temp_bytes(int num_small);
+// writes the permuted arrays into b (start, pcell, poff, pxy0, cpi, weight, orig) and the cells'
+// cell-pin -> net-pin map; the caller swaps b's arrays with the NetsDev's and then calls
+// launch_tile_bounds
+void launch_net_resort(hipStream_t s, const NetsDev& n, int num_pins, int num_cells, const int32_t* orig, CellsDev& c, NetResortBufs& b);
+// tile[t] = the first small net whose first pin is at or past t kTileSpan, t <= num_tiles
+void launch_tile_bounds(hipStream_t s, const NetsDev& n, int32_t* tile);
 
 void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v);
 

@@ -1120,8 +1120,9 @@ __global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, flo
     if (threadIdx.x == 0 && ux >= lx) atomicAdd(hpwl, (unsigned long long)((long long)ux - lx + (long long)uy - ly));
 }
 
-// k_wa_tiles: the small nets (below num_small) in tiles of consecutive nets (at most kTilePins
-// pins and kTileNets nets each; XCD x takes the tiles of slice x).  A block first stages its
+// k_wa_tiles: the small nets (below num_small) in tiles of consecutive nets (the nets whose first
+// pin lies in one range of kTileSpan pins: at most kTilePins pins; XCD x takes a contiguous
+// eighth of the tiles).  A block first stages its
 // tile's pin positions (and cell-pin indices) in LDS, pin-parallel: the pin records are read
 // coalesced and every cell-centre gather of the tile is in flight at once.  Then one net per
 // thread reads its pins from LDS in the net's order: the box, the four shifted exponentials
@@ -1139,9 +1140,9 @@ __global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, floa
     __shared__ unsigned long long shh[kBlock / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int x = blockIdx.x % kXcds;
-    const int t = n.tile_slice[x] + (int)(blockIdx.x / kXcds);
+    const int t = x * n.tile_blocks + (int)(blockIdx.x / kXcds);
     unsigned long long h = 0;
-    if (t < n.tile_slice[x + 1]) {
+    if (t < n.num_tiles) {
         const int e0 = n.tile[t], e1 = n.tile[t + 1];
         const int s0 = n.start[e0], np = n.start[e1] - s0;
         {
@@ -2061,6 +2062,102 @@ __global__ void k_remap_pcell(int P, int32_t* pcell, const int32_t* inv) {
         const int c = pcell[k];
         if (c >= 0) pcell[k] = inv[c];
     }
+}
+
+// ---- net re-sort ----------------------------------------------------------------------------
+__global__ void k_net_keys(int E, NetsDev n, int num_cells, unsigned long long* keys, int32_t* idx) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const int s = n.start[e], t = n.start[e + 1];
+        int fc = num_cells;  // a net on no movable cell sorts after the others
+        for (int k = s; k < t; ++k) {
+            const int c = n.pcell[k];
+            if (c >= 0 && c < fc) fc = c;
+        }
+        keys[e] = ((unsigned long long)(unsigned)(fc >> 12) << 8) | (unsigned long long)(unsigned)(t - s);
+        idx[e] = e;
+    }
+}
+
+__global__ void k_net_deg(int E, NetsDev n, const int32_t* perm, int32_t* deg) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const int o = perm[e];
+        deg[e] = n.start[o + 1] - n.start[o];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) deg[E] = 0;
+}
+
+// new small net e (old perm[e]) copies its pins to their new places; each moved pin's cell-pin
+// slot learns the pin's new index
+__global__ void k_net_move(int E, NetsDev n, const int32_t* perm, const int32_t* orig, NetResortBufs b, int32_t* cp_npin) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const int o = perm[e];
+        const int s = n.start[o], t = n.start[o + 1], d = b.start[e];
+        for (int q = 0; q < t - s; ++q) {
+            const int cp = n.cpi[s + q];
+            b.pcell[d + q] = n.pcell[s + q];
+            b.poff[d + q] = n.poff[s + q];
+            b.pxy0[d + q] = n.pxy0[s + q];
+            b.cpi[d + q] = cp;
+            if (cp >= 0) cp_npin[cp] = d + q;
+        }
+        b.weight[e] = n.weight[o];
+        b.orig[e] = orig[o];
+    }
+}
+
+// the high-fanout nets' pins and records keep their places
+__global__ void k_net_tail(int E, int num_nets, int P, NetsDev n, const int32_t* orig, NetResortBufs b) {
+    const int p0 = n.start[E];
+    for (int k = p0 + blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) {
+        b.pcell[k] = n.pcell[k];
+        b.poff[k] = n.poff[k];
+        b.pxy0[k] = n.pxy0[k];
+        b.cpi[k] = n.cpi[k];
+    }
+    for (int e = E + blockIdx.x * blockDim.x + threadIdx.x; e < num_nets; e += gridDim.x * blockDim.x) {
+        b.start[e + 1] = n.start[e + 1];
+        b.weight[e] = n.weight[e];
+        b.orig[e] = orig[e];
+    }
+}
+
+size_t net_resort_temp_bytes(int E) {
+    size_t a = 0, b = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, a, (unsigned long long*)nullptr, (unsigned long long*)nullptr, (int32_t*)nullptr,
+                                             (int32_t*)nullptr, E, 0, 64);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (int32_t*)nullptr, (int32_t*)nullptr, E + 1);
+    return a > b ? a : b;
+}
+
+void launch_net_resort(hipStream_t s, const NetsDev& n, int P, int num_cells, const int32_t* orig, CellsDev& c, NetResortBufs& b) {
+    const int E = n.num_small;
+    if (E <= 0) return;
+    hipLaunchKernelGGL(k_net_keys, dim3(grid_for(E)), dim3(kBlock), 0, s, E, n, num_cells, b.keys_in, b.idx_in);
+    size_t bytes = b.temp_bytes;
+    (void)hipcub::DeviceRadixSort::SortPairs(b.temp, bytes, b.keys_in, b.keys_out, b.idx_in, b.perm, E, 0, b.end_bit, s);
+    hipLaunchKernelGGL(k_net_deg, dim3(grid_for(E)), dim3(kBlock), 0, s, E, n, b.perm, b.deg);
+    bytes = b.temp_bytes;
+    (void)hipcub::DeviceScan::ExclusiveSum(b.temp, bytes, b.deg, b.start, E + 1, s);
+    hipLaunchKernelGGL(k_net_move, dim3(grid_for(E)), dim3(kBlock), 0, s, E, n, b.perm, orig, b, c.cp_npin);
+    hipLaunchKernelGGL(k_net_tail, dim3(grid_for(P - 0)), dim3(kBlock), 0, s, E, n.num_nets, P, n, orig, b);
+}
+
+// tile[t] = the first net e with start[e] >= t kTileSpan (E where none): net e is the first of
+// every tile t in (start[e-1] / kTileSpan, start[e] / kTileSpan]; tiles past the last net's start
+// are empty
+__global__ void k_tile_bounds(int E, const int32_t* start, int T, int32_t* tile) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const int te = start[e] / kTileSpan;
+        const int tp = e == 0 ? -1 : start[e - 1] / kTileSpan;
+        for (int t = tp + 1; t <= te && t < T; ++t) tile[t] = e;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int t = (E > 0 ? start[E - 1] / kTileSpan + 1 : 0); t <= T; ++t) tile[t] = E;
+}
+
+void launch_tile_bounds(hipStream_t s, const NetsDev& n, int32_t* tile) {
+    if (n.num_small <= 0) return;
+    hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(n.num_small)), dim3(kBlock), 0, s, n.num_small, n.start, n.num_tiles, tile);
 }
 
 size_t resort_temp_bytes(int N) {

@@ -105,6 +105,10 @@ struct gpl_ctx {
     std::vector<uint8_t> h_flags;
     std::vector<int32_t> net_sorted_of;  // original net -> sorted index
     std::vector<int32_t> net_orig_of;    // sorted index -> original net
+    int32_t* d_net_orig = nullptr;       // the same on the device (net re-sorts permute it)
+    bool net_map_stale = false;          // the host maps lag the device's order
+    int32_t* d_tile = nullptr;           // the WA tiles' first nets (nets.tile)
+    gplk::NetResortBufs nrb{};
     std::vector<int32_t> cell_inst, inst_box;
     std::vector<int32_t> cell_old_of, cell_new_of;  // device cell order <-> problem order
     std::vector<int32_t> cell_inst_prob;            // placer instance of each problem cell
@@ -811,6 +815,15 @@ struct gpl_ctx {
         }
     }
 
+    // The problem's index of each device net, refreshed after net re-sorts.
+    void ensure_net_map() {
+        if (!net_map_stale) return;
+        check(hipMemcpyAsync(net_orig_of.data(), d_net_orig, sizeof(int32_t) * net_orig_of.size(), hipMemcpyDeviceToHost, stream), "copy");
+        check(hipStreamSynchronize(stream), "sync");
+        for (size_t i = 0; i < net_orig_of.size(); ++i) net_sorted_of[net_orig_of[i]] = int32_t(i);
+        net_map_stale = false;
+    }
+
     // The problem's index of each device cell, refreshed after re-sorts.
     void ensure_host_map() {
         if (!host_map_stale) return;
@@ -852,6 +865,27 @@ struct gpl_ctx {
             for (int i = 0; i < nlive; ++i) std::swap(r.v[live[i]], region_scratch[b][i]);
         }
         gplk::launch_remap_pcell(stream, net_pin_count, nets.pcell, rb.inv);
+        // the nets follow the cells: the small nets ordered again by their first cell's window
+        gplk::launch_net_resort(stream, nets, net_pin_count, ncells, d_net_orig, cells, nrb);
+        {
+            int32_t* t = const_cast<int32_t*>(nets.start);
+            nets.start = nrb.start;
+            nrb.start = t;
+            std::swap(nets.pcell, nrb.pcell);
+            std::swap(nets.cpi, nrb.cpi);
+            int2* o = const_cast<int2*>(nets.poff);
+            nets.poff = nrb.poff;
+            nrb.poff = o;
+            o = const_cast<int2*>(nets.pxy0);
+            nets.pxy0 = nrb.pxy0;
+            nrb.pxy0 = o;
+            float* w = const_cast<float*>(nets.weight);
+            nets.weight = nrb.weight;
+            nrb.weight = w;
+            std::swap(d_net_orig, nrb.orig);
+        }
+        gplk::launch_tile_bounds(stream, nets, d_tile);
+        net_map_stale = true;
         mark_end();
         host_map_stale = true;
         ++resorts;
@@ -1296,41 +1330,14 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         int32_t num_small = 0;
         while (num_small < E && deg(order[num_small]) <= kBig) ++num_small;
         {
+            // the WA tiles: ranges of kTileSpan small-net pins (bounds found on the device, where a
+            // net re-sort recomputes them), XCD x taking a contiguous eighth
             int64_t small_pins = 0;
             for (int32_t i = 0; i < num_small; ++i) small_pins += deg(order[i]);
-            int32_t at = 0, most = 0;
-            int64_t pins = 0;
-            for (int x = 0; x < 8; ++x) {
-                c->nets.slice[x] = at;
-                const int64_t goal = small_pins * (x + 1) / 8;
-                while (at < num_small && (x == 7 || pins < goal)) pins += deg(order[at++]);
-                most = std::max(most, at - c->nets.slice[x]);
-            }
-            c->nets.slice[8] = at;
-            (void)most;
-        }
-        {
-            // each slice cut into tiles of consecutive nets for the WA kernel's LDS staging
-            std::vector<int32_t> tile;
-            int32_t most = 1;
-            for (int x = 0; x < 8; ++x) {
-                c->nets.tile_slice[x] = int32_t(tile.size());
-                int32_t e = c->nets.slice[x];
-                while (e < c->nets.slice[x + 1]) {
-                    tile.push_back(e);
-                    int32_t pins = 0, cnt = 0;
-                    while (e < c->nets.slice[x + 1] && cnt < gplk::kTileNets && pins + deg(order[e]) <= gplk::kTilePins) {
-                        pins += deg(order[e]);
-                        ++cnt;
-                        ++e;
-                    }
-                }
-                most = std::max(most, int32_t(tile.size()) - c->nets.tile_slice[x]);
-            }
-            c->nets.tile_slice[8] = int32_t(tile.size());
-            tile.push_back(c->nets.slice[8]);
-            c->nets.tile = c->upload(tile.data(), tile.size());
-            c->nets.tile_blocks = most;
+            c->nets.num_tiles = int32_t((small_pins + gplk::kTileSpan - 1) / gplk::kTileSpan);
+            c->nets.tile_blocks = std::max(1, (c->nets.num_tiles + 7) / 8);
+            c->d_tile = c->dalloc<int32_t>(size_t(c->nets.num_tiles) + 1);
+            c->nets.tile = c->d_tile;
         }
         std::vector<int32_t> nstart(E + 1, 0), pcell, np_net;
         std::vector<int2> poff, pxy0;
@@ -1404,6 +1411,31 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
 
         c->nets.cpi = c->upload(cpi.data(), cpi.size());
         c->nets.pgrad = c->dalloc<float2>(std::max<size_t>(cp_npin.size(), 1));
+        gplk::launch_tile_bounds(c->stream, c->nets, c->d_tile);
+        // the net re-sort's buffers and second copy of every net array
+        {
+            gplk::NetResortBufs& b = c->nrb;
+            const int32_t Es = std::max(1, num_small);
+            const size_t P = std::max<size_t>(pcell.size(), 1);
+            b.keys_in = c->dalloc<unsigned long long>(Es);
+            b.keys_out = c->dalloc<unsigned long long>(Es);
+            b.idx_in = c->dalloc<int32_t>(Es);
+            b.perm = c->dalloc<int32_t>(Es);
+            b.deg = c->dalloc<int32_t>(size_t(Es) + 1);
+            b.temp_bytes = gplk::net_resort_temp_bytes(Es);
+            b.temp = c->dalloc<uint8_t>(b.temp_bytes);
+            int32_t wbits = 1;
+            while ((int64_t(1) << wbits) <= (int64_t(N) >> 12) + 1) ++wbits;
+            b.end_bit = 8 + wbits;
+            b.start = c->dalloc<int32_t>(size_t(E) + 1);
+            b.pcell = c->dalloc<int32_t>(P);
+            b.poff = c->dalloc<int2>(P);
+            b.pxy0 = c->dalloc<int2>(P);
+            b.cpi = c->dalloc<int32_t>(P);
+            b.weight = c->dalloc<float>(std::max(1, E));
+            b.orig = c->dalloc<int32_t>(std::max(1, E));
+            c->d_net_orig = c->upload(order.data(), order.size());
+        }
         c->cells.pre = c->upload(pre.data(), N);
         c->cell_inst = cell_inst;
         c->cell_inst_prob.assign(p->cell_inst, p->cell_inst + N);
@@ -1805,6 +1837,7 @@ int32_t gpl_wa_forward(gpl_ctx* c, float wl_coef_x, float wl_coef_y, float* net_
         check(hipMemcpyAsync(&h, c->d_hpwl, sizeof h, hipMemcpyDeviceToHost, c->stream), "copy");
         check(hipStreamSynchronize(c->stream), "sync");
         c->collect();
+        c->ensure_net_map();
         for (int32_t e = 0; e < E; ++e) {
             const gplk::NetRec& R = rec[c->net_sorted_of[e]];
             if (net_sums) {

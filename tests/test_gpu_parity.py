@@ -342,10 +342,11 @@ def test_divergence_is_reported(kind, code):
 
 
 def test_cell_resorts_do_not_change_the_placement(c1):
-    """Re-sorting the device cells every iteration (the storage order the scatter tiles rely on)
-    leaves the trajectory as it was: the same iterations, overflow and HPWL to FP64-sum rounding,
-    and the placed locations in the problem's order."""
-    runs = []
+    """Re-sorting the device cells (and with them the nets) every iteration (the storage orders the
+    scatter and WA tiles rely on) leaves the trajectory as it was: the same iterations, overflow
+    and HPWL to FP64-sum rounding, the placed locations in the problem's order, and every net's WA
+    box and sums in the problem's order."""
+    runs, ctxs = [], []
     for kind in (0, 3):
         d = G.Design.from_config("c1")
         d.initial_place()
@@ -356,11 +357,24 @@ def test_cell_resorts_do_not_change_the_placement(c1):
             g.debug_inject(3, 0)
         g.run_nesterov_loop()
         runs.append((g.stats(), g.history(), g.placed_locations(), g.get_coordinates(0, G.GPL_SLOT_CUR)))
-        g.close()
-        d.close()
+        ctxs.append((g, d))
     (s0, h0, p0, c0), (s1, h1, p1, c1_) = runs
-    assert s0.iter == s1.iter and s1.converged
-    assert abs(s0.hpwl - s1.hpwl) / s0.hpwl < 1e-4
-    assert np.array_equal(p0[:, 0], p1[:, 0])           # same instances, same order
-    assert np.max(np.abs(p0[:, 1:] - p1[:, 1:])) <= 64  # positions within rounding drift
-    assert np.max(np.abs(c0 - c1_)) < 64.0
+    try:
+        assert s0.iter == s1.iter and s1.converged
+        assert abs(s0.hpwl - s1.hpwl) / s0.hpwl < 1e-4
+        assert np.array_equal(p0[:, 0], p1[:, 0])           # same instances, same order
+        assert np.max(np.abs(p0[:, 1:] - p1[:, 1:])) <= 64  # positions within rounding drift
+        assert np.max(np.abs(c0 - c1_)) < 64.0
+        # the nets were re-sorted with the cells: at the same points, every net's box and WA sums
+        # come back in the problem's order, bit for bit
+        outs = []
+        for g, _ in ctxs:
+            g.set_coordinates(0, G.GPL_SLOT_CUR_SLP, c0)
+            g.density_scatter(0, G.GPL_SLOT_CUR_SLP)
+            outs.append(g.wa_forward(np.float32(1e-3), np.float32(1e-3)))
+        (a0, b0, q0), (a1, b1, q1) = outs
+        assert q0 == q1 and np.array_equal(b0, b1) and np.array_equal(a0, a1)
+    finally:
+        for g, d in ctxs:
+            g.close()
+            d.close()
