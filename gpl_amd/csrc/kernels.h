@@ -28,6 +28,7 @@ struct RegionDev {
     float target_density;
     int32_t cell_begin, cell_end;
     int32_t narrow;  // 1: the scatter's LDS tables may hold 32-bit sums (kPlaceChunk x the largest bin area < 2^32)
+    int32_t edges_rounded;  // 1: bin_x[i] = lx + round(i size_x) and bin_y alike (init_bins' edges)
     // bins
     const int32_t* bin_x;  // cnt_x + 1
     const int32_t* bin_y;  // cnt_y + 1

@@ -51,6 +51,13 @@ __device__ __forceinline__ void bin_span(int lo, int hi, int org, double size, i
     b = u < cnt ? u : cnt;
 }
 
+// Bin edge i along one axis: the grid's origin plus the rounded multiple of the bin size
+// (init_bins, nesterov.rs:1879-1881), when the region's tables hold exactly these (edges_rounded);
+// else the table's entry.
+__device__ __forceinline__ int bin_edge(const int32_t* table, int rounded, int org, double size, int cnt, int i) {
+    return rounded ? org + (int)round((double)i * size) : table[min(i, cnt)];
+}
+
 // nesterov.rs:298 getOverlapDensityArea.
 __device__ __forceinline__ float overlap_area(int blx, int bly, int bux, int buy, int4 d) {
     int lx = max(blx, d.x), ly = max(bly, d.y), ux = min(bux, d.z), uy = min(buy, d.w);
@@ -1217,18 +1224,33 @@ __global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, floa
 }
 
 constexpr int kGradPins = 8;
+constexpr int kGradStage = 2048;  // pin terms of a block's cells staged in LDS
 // updateGradients (nesterov.rs:2455-2489) per cell: the WA gradient over the cell's pins in
 // their order (each pin's weighted term from the WA kernel); the density gradient over the bins its box
 // covers, x outer, y inner (2434); the preconditioned sum; FP64 partial sums of |wl|, |den|,
 // and, for a trial, the squared coordinate and gradient distances (nesterovUpdateStepLength,
 // 2755-2756) to a_slp/a_sum.
-__global__ void k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float cy, float penalty, float2* wl_out, float2* den_out,
+// (six waves per SIMD: 80 registers, no spill)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float cy, float penalty, float2* wl_out, float2* den_out,
                            float2* sum_out, const float2* a_slp, const float2* b_slp, const float2* a_sum, double* partials,
                            int nan_inject) {
     __shared__ double sh[kBlock / 64];
+    __shared__ float2 spg[kGradStage];
     const int n = r.cell_end - r.cell_begin;
     double s_wl = 0.0, s_den = 0.0, s_c = 0.0, s_g = 0.0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    // the block's cells in groups of kBlock consecutive cells (the trip count is the block's own)
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        // the group's pin terms are one contiguous range (cells in order, each cell's pins in
+        // order): read coalesced into LDS when it fits
+        const int qb = c.pin_start[r.cell_begin + base];
+        const int qe = c.pin_start[r.cell_begin + min(base + kBlock, n)];
+        const bool staged = qe - qb <= kGradStage;
+        __syncthreads();
+        if (staged)
+            for (int i = threadIdx.x; i < qe - qb; i += kBlock) spg[i] = nt.pgrad[qb + i];
+        __syncthreads();
+        const int k = base + (int)threadIdx.x;
+        if (k >= n) continue;
         const int g = r.cell_begin + k;
         const int4 d = c.dbox[g];
         const int q0 = c.pin_start[g], q1 = c.pin_start[g + 1];
@@ -1242,13 +1264,18 @@ __global__ void k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float 
         // does not wait after each); cells with more pins or a wider span finish in the loops.
         float2 t[kGradPins];
         const int qlast = max(q1 - 1, 0);
+        if (staged) {
 #pragma unroll
-        for (int j = 0; j < kGradPins; ++j) t[j] = nt.pgrad[min(q0 + j, qlast)];
+            for (int j = 0; j < kGradPins; ++j) t[j] = spg[min(q0 + j, qlast) - qb];
+        } else {
+#pragma unroll
+            for (int j = 0; j < kGradPins; ++j) t[j] = nt.pgrad[min(q0 + j, qlast)];
+        }
         int ex[4], ey[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            ex[i] = r.bin_x[min(x0 + i, r.cnt_x)];
-            ey[i] = r.bin_y[min(y0 + i, r.cnt_y)];
+            ex[i] = bin_edge(r.bin_x, r.edges_rounded, r.lx, r.size_x, r.cnt_x, x0 + i);
+            ey[i] = bin_edge(r.bin_y, r.edges_rounded, r.ly, r.size_y, r.cnt_y, y0 + i);
         }
         float wfx[9], wfy[9];
 #pragma unroll
@@ -1270,7 +1297,7 @@ __global__ void k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float 
             }
         }
         for (int q = q0 + kGradPins; q < q1; ++q) {
-            const float2 u = nt.pgrad[q];
+            const float2 u = staged ? spg[q - qb] : nt.pgrad[q];
             wx += u.x;
             wy += u.y;
         }
@@ -1648,7 +1675,23 @@ static int grid_for(size_t n) {
 }
 
 int block_partials_bins(const RegionDev& r) { return grid_for((size_t)r.cnt_x * r.cnt_y); }
-int block_partials_cells(const RegionDev& r) { return grid_for((size_t)(r.cell_end - r.cell_begin)); }
+// The grid of the per-cell kernels that leave FP64 partials (the gradient, the distances): no more
+// blocks than the device holds at once at the gradient's occupancy, so that no second, part-filled
+// round of blocks follows the first.  Fixed per device (and so is the partials' summation order).
+static int g_cells_cap = 0;
+static int cells_grid(size_t n) {
+    if (g_cells_cap == 0) {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_gradient), kBlock, 0) == hipSuccess &&
+            cus > 0 && per > 0)
+            g_cells_cap = std::max(kXcds, std::min(kMaxBlocks, cus * per / kXcds * kXcds));
+        else
+            g_cells_cap = kMaxBlocks;
+    }
+    return std::min(grid_for(n), g_cells_cap);
+}
+int block_partials_cells(const RegionDev& r) { return cells_grid((size_t)(r.cell_end - r.cell_begin)); }
 
 void launch_zero_bins(hipStream_t s, const RegionDev& r) {
     size_t M = (size_t)r.cnt_x * r.cnt_y;
@@ -1948,7 +1991,7 @@ void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int n
 }
 
 void launch_distance(hipStream_t s, int n, const float2* a, const float2* b, const float2* c, const float2* d, double* partials) {
-    hipLaunchKernelGGL(k_distance, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, b, c, d, partials);
+    hipLaunchKernelGGL(k_distance, dim3(cells_grid(n)), dim3(kBlock), 0, s, n, a, b, c, d, partials);
 }
 
 void launch_initial_prev(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* cur_slp, const float2* cur_sum,

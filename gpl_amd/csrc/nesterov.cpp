@@ -1492,6 +1492,15 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
                 for (int32_t y = 0; y < r.R.cnt_y; ++y) hmax = std::max<int64_t>(hmax, int64_t(r.R.bin_y[y + 1]) - r.R.bin_y[y]);
                 d.narrow = (double)gplk::kPlaceChunk * (double)wmax * (double)hmax < 4294967296.0 ? 1 : 0;
             }
+            {
+                // the kernels compute the edges themselves when the tables are init_bins' own
+                bool rounded = true;
+                for (int32_t x = 0; x <= r.R.cnt_x; ++x)
+                    rounded = rounded && r.R.bin_x[x] == d.lx + (int32_t)std::round(double(x) * d.size_x);
+                for (int32_t y = 0; y <= r.R.cnt_y; ++y)
+                    rounded = rounded && r.R.bin_y[y] == d.ly + (int32_t)std::round(double(y) * d.size_y);
+                d.edges_rounded = rounded ? 1 : 0;
+            }
             size_t M = size_t(r.R.cnt_x) * r.R.cnt_y;
             d.bin_x = c->upload(r.R.bin_x, size_t(r.R.cnt_x) + 1);
             d.bin_y = c->upload(r.R.bin_y, size_t(r.R.cnt_y) + 1);
