@@ -485,6 +485,9 @@ enum RowMode {
     ROW_SPECTRUM_POISSON = 6
 };
 
+// The modes that run two inverse transforms side by side (a second pair of FFT buffers per row).
+__host__ __device__ constexpr bool row_mode_dual(int m) { return m == ROW_POISSON || m == ROW_INV_FIELDS || m == ROW_SPECTRUM_POISSON; }
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
@@ -506,18 +509,35 @@ __device__ __forceinline__ void st2(lds_float2* p, float2 v) {
     *p = t;
 }
 
-// Complex FFT of N points (a power of two) held in LDS buffer x, by T = max(N/4, 1) threads of a
+// Complex FFTs of N points (a power of two), NB of them side by side (each held in LDS buffer
+// x[b]; the NB transforms share the twiddles and the barriers), by T = max(N/4, 1) threads of a
 // row group (tid < T), Stockham autosort: radix-2 first when log2 N is odd, then radix-4 stages;
 // natural order in and out.  tw[m] = e^{-2 pi i m / N}, m < N (TP: an LDS or a global pointer);
 // dir -1 forward, +1 inverse (unnormalised).  LOGN > 0 fixes N = 2^LOGN at compile time (the
 // stages unroll, the index arithmetic folds); 0 takes N and logN as given.  Every thread of the
-// block calls it (block barriers inside); returns the buffer that holds the result.
-template <int LOGN, typename TP>
-__device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N_, int logN_, int T_, int tid, TP tw, int dir) {
+// block calls it (block barriers inside); on return x[b] is the buffer that holds result b.
+template <int LOGN, int NB, typename TP>
+__device__ void fft_stockham_n(lds_float2* (&x)[NB], lds_float2* (&y)[NB], int N_, int logN_, int T_, int tid, TP tw, int dir) {
     const int logN = LOGN > 0 ? LOGN : logN_;
     const int N = LOGN > 0 ? (1 << LOGN) : N_;
     const int T = LOGN > 0 ? (LOGN >= 2 ? (1 << (LOGN - 2)) : 1) : T_;
     int lNs = 0;  // log2 of the current sub-transform length
+    auto swap_all = [&]() {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            lds_float2* t = x[b];
+            x[b] = y[b];
+            y[b] = t;
+        }
+    };
+    auto radix2 = [&](int j, int half) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            float2 v0 = ld2(x[b] + j), v1 = ld2(x[b] + j + half);
+            st2(y[b] + 2 * j, make_float2(v0.x + v1.x, v0.y + v1.y));
+            st2(y[b] + 2 * j + 1, make_float2(v0.x - v1.x, v0.y - v1.y));
+        }
+    };
     __syncthreads();
     if (logN & 1) {
         const int half = N >> 1;
@@ -525,22 +545,12 @@ __device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N_, int lo
 #pragma unroll
         for (int i = 0; i < (LOGN >= 2 ? 2 : 1); ++i) {
             const int j = tid + i * T;
-            if (LOGN > 0 && j < half) {
-                float2 v0 = ld2(x + j), v1 = ld2(x + j + half);
-                st2(y + 2 * j, make_float2(v0.x + v1.x, v0.y + v1.y));
-                st2(y + 2 * j + 1, make_float2(v0.x - v1.x, v0.y - v1.y));
-            }
+            if (LOGN > 0 && j < half) radix2(j, half);
         }
         if (LOGN == 0)
-            for (int j = tid; j < half; j += T) {
-                float2 v0 = ld2(x + j), v1 = ld2(x + j + half);
-                st2(y + 2 * j, make_float2(v0.x + v1.x, v0.y + v1.y));
-                st2(y + 2 * j + 1, make_float2(v0.x - v1.x, v0.y - v1.y));
-            }
+            for (int j = tid; j < half; j += T) radix2(j, half);
         lNs = 1;
-        lds_float2* t = x;
-        x = y;
-        y = t;
+        swap_all();
         __syncthreads();
     }
     const int q = N >> 2;
@@ -551,7 +561,14 @@ __device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N_, int lo
         if (tid < q) {
             const int j = tid;
             const int k = j & (Ns - 1);
-            float2 v0 = ld2(x + j), v1 = ld2(x + j + q), v2 = ld2(x + j + 2 * q), v3 = ld2(x + j + 3 * q);
+            float2 v[NB][4];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                v[b][0] = ld2(x[b] + j);
+                v[b][1] = ld2(x[b] + j + q);
+                v[b][2] = ld2(x[b] + j + 2 * q);
+                v[b][3] = ld2(x[b] + j + 3 * q);
+            }
             if (Ns > 1) {
                 const int step = N >> (lNs + 2);  // twiddle index stride
                 float2 w1 = ld2(tw + k * step), w2 = ld2(tw + 2 * k * step), w3 = ld2(tw + 3 * k * step);
@@ -560,27 +577,39 @@ __device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N_, int lo
                     w2 = cconj(w2);
                     w3 = cconj(w3);
                 }
-                v1 = cmul(v1, w1);
-                v2 = cmul(v2, w2);
-                v3 = cmul(v3, w3);
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    v[b][1] = cmul(v[b][1], w1);
+                    v[b][2] = cmul(v[b][2], w2);
+                    v[b][3] = cmul(v[b][3], w3);
+                }
             }
-            float2 t0 = make_float2(v0.x + v2.x, v0.y + v2.y), t1 = make_float2(v0.x - v2.x, v0.y - v2.y);
-            float2 t2 = make_float2(v1.x + v3.x, v1.y + v3.y), t3 = make_float2(v1.x - v3.x, v1.y - v3.y);
-            // forward: X1 = t1 - i t3, X3 = t1 + i t3; inverse the other way
-            float2 it3 = dir < 0 ? make_float2(t3.y, -t3.x) : make_float2(-t3.y, t3.x);  // -i t3 or +i t3
             const int d = ((j >> lNs) << (lNs + 2)) + k;
-            st2(y + d, make_float2(t0.x + t2.x, t0.y + t2.y));
-            st2(y + d + Ns, make_float2(t1.x + it3.x, t1.y + it3.y));
-            st2(y + d + 2 * Ns, make_float2(t0.x - t2.x, t0.y - t2.y));
-            st2(y + d + 3 * Ns, make_float2(t1.x - it3.x, t1.y - it3.y));
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float2 v0 = v[b][0], v1 = v[b][1], v2 = v[b][2], v3 = v[b][3];
+                float2 t0 = make_float2(v0.x + v2.x, v0.y + v2.y), t1 = make_float2(v0.x - v2.x, v0.y - v2.y);
+                float2 t2 = make_float2(v1.x + v3.x, v1.y + v3.y), t3 = make_float2(v1.x - v3.x, v1.y - v3.y);
+                // forward: X1 = t1 - i t3, X3 = t1 + i t3; inverse the other way
+                float2 it3 = dir < 0 ? make_float2(t3.y, -t3.x) : make_float2(-t3.y, t3.x);  // -i t3 or +i t3
+                st2(y[b] + d, make_float2(t0.x + t2.x, t0.y + t2.y));
+                st2(y[b] + d + Ns, make_float2(t1.x + it3.x, t1.y + it3.y));
+                st2(y[b] + d + 2 * Ns, make_float2(t0.x - t2.x, t0.y - t2.y));
+                st2(y[b] + d + 3 * Ns, make_float2(t1.x - it3.x, t1.y - it3.y));
+            }
         }
         lNs += 2;
-        lds_float2* t = x;
-        x = y;
-        y = t;
+        swap_all();
         __syncthreads();
     }
-    return x;
+}
+
+template <int LOGN, typename TP>
+__device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N_, int logN_, int T_, int tid, TP tw, int dir) {
+    lds_float2* xs[1] = {x};
+    lds_float2* ys[1] = {y};
+    fft_stockham_n<LOGN, 1>(xs, ys, N_, logN_, T_, tid, tw, dir);
+    return xs[0];
 }
 
 // A row's transform context.  LOGL > 0 fixes the row length L = 2^LOGL (and with it h = L/2, the
@@ -642,8 +671,8 @@ __device__ void dct2_row(const RowCtx<TP, LOGL>& R, bool active, SRC src, OUT ou
 // c from b[0] = a[0], b[j] = a[L-j]); the real inverse DFT through an L/2-point complex IFFT of
 // Z[k] = (V[k] + conj V[h-k]) + i e^{2 pi i k / L} (V[k] - conj V[h-k]); y = v / 2 unshuffled,
 // odd outputs negated for the sine.
-template <typename TP, int LOGL, typename F, typename OUT>
-__device__ void dct3_row(const RowCtx<TP, LOGL>& R, bool active, F coef, bool sine, OUT out) {
+template <typename TP, int LOGL, typename F>
+__device__ __forceinline__ void dct3_pre(const RowCtx<TP, LOGL>& R, lds_float2* dst, F coef, bool sine) {
     const int L = R.L(), h = R.h(), T = R.T();
     auto V = [&](int q) -> float2 {
         float ck, cl;
@@ -658,27 +687,57 @@ __device__ void dct3_row(const RowCtx<TP, LOGL>& R, bool active, F coef, bool si
         float cs = dq.x, sn = -dq.y;
         return make_float2(cs * ck + sn * cl, sn * ck - cs * cl);
     };
-    if (active) {
 #pragma unroll
-        for (int i = 0; i < h / T; ++i) {
-            const int k = R.tid + i * T;
-            float2 vk = V(k), vc = cconj(V(h - k));
-            float2 E = make_float2(vk.x + vc.x, vk.y + vc.y), D = make_float2(vk.x - vc.x, vk.y - vc.y);
-            float2 wD = cmul(cconj(ld2(R.th + k)), D);
-            st2(R.A + k, make_float2(E.x - wD.y, E.y + wD.x));
-        }
+    for (int i = 0; i < h / T; ++i) {
+        const int k = R.tid + i * T;
+        float2 vk = V(k), vc = cconj(V(h - k));
+        float2 E = make_float2(vk.x + vc.x, vk.y + vc.y), D = make_float2(vk.x - vc.x, vk.y - vc.y);
+        float2 wD = cmul(cconj(ld2(R.th + k)), D);
+        st2(dst + k, make_float2(E.x - wD.y, E.y + wD.x));
     }
-    lds_float2* z = fft_stockham<(LOGL > 0 ? LOGL - 1 : 0)>(R.A, R.B, h, R.logh(), T, R.tid, R.tw, +1);
-    if (active) {
+}
+template <typename TP, int LOGL, typename OUT>
+__device__ __forceinline__ void dct3_post(const RowCtx<TP, LOGL>& R, const lds_float2* z, bool sine, OUT out) {
+    const int L = R.L(), T = R.T();
 #pragma unroll
-        for (int i = 0; i < L / T; ++i) {
-            const int j = R.tid + i * T;
-            int m = (j & 1) ? (L - 1 - (j >> 1)) : (j >> 1);
-            float2 zz = ld2(z + (m >> 1));
-            float v = ((m & 1) ? zz.y : zz.x) * 0.5f;
-            if (sine && (j & 1)) v = -v;
-            out(j, v);
-        }
+    for (int i = 0; i < L / T; ++i) {
+        const int j = R.tid + i * T;
+        int m = (j & 1) ? (L - 1 - (j >> 1)) : (j >> 1);
+        float2 zz = ld2(z + (m >> 1));
+        float v = ((m & 1) ? zz.y : zz.x) * 0.5f;
+        if (sine && (j & 1)) v = -v;
+        out(j, v);
+    }
+}
+
+// DCT-III (sine: DST-III) of the row given by coef(j), j < L (Ooura's conventions); calls
+// out(j, y[j]).  V[q] = e^{i pi q / 2L} (c[q] - i c[L-q]) with c[0] = 2 a[0], c[L] = 0 (the sine:
+// c from b[0] = a[0], b[j] = a[L-j]); the real inverse DFT through an L/2-point complex IFFT of
+// Z[k] = (V[k] + conj V[h-k]) + i e^{2 pi i k / L} (V[k] - conj V[h-k]); y = v / 2 unshuffled,
+// odd outputs negated for the sine.
+template <typename TP, int LOGL, typename F, typename OUT>
+__device__ void dct3_row(const RowCtx<TP, LOGL>& R, bool active, F coef, bool sine, OUT out) {
+    if (active) dct3_pre(R, R.A, coef, sine);
+    lds_float2* z = fft_stockham<(LOGL > 0 ? LOGL - 1 : 0)>(R.A, R.B, R.h(), R.logh(), R.T(), R.tid, R.tw, +1);
+    if (active) dct3_post(R, z, sine, out);
+    __syncthreads();
+}
+
+// Two DCT-III / DST-III of one row side by side (the second in the buffers A1, B1): one FFT stage
+// loop, one barrier per stage for both.  Each output is the same as dct3_row's.
+template <typename TP, int LOGL, typename F0, typename OUT0, typename F1, typename OUT1>
+__device__ void dct3_row2(const RowCtx<TP, LOGL>& R, lds_float2* A1, lds_float2* B1, bool active, F0 coef0, bool sine0, OUT0 out0,
+                          F1 coef1, bool sine1, OUT1 out1) {
+    if (active) {
+        dct3_pre(R, R.A, coef0, sine0);
+        dct3_pre(R, A1, coef1, sine1);
+    }
+    lds_float2* x[2] = {R.A, A1};
+    lds_float2* y[2] = {R.B, B1};
+    fft_stockham_n<(LOGL > 0 ? LOGL - 1 : 0), 2>(x, y, R.h(), R.logh(), R.T(), R.tid, R.tw, +1);
+    if (active) {
+        dct3_post(R, x[0], sine0, out0);
+        dct3_post(R, x[1], sine1, out1);
     }
     __syncthreads();
 }
@@ -713,7 +772,8 @@ struct RowArgs {
 // With transpose_out the outputs go out transposed, so that the next pass reads rows.
 // TL: the twiddle tables (and the element wavenumbers) are first copied to LDS (the launcher
 // picks it when they fit beside the rows' buffers); else they are read from global memory.
-// LDS: per row the FFT's two buffers (L float2), [ROW_SPECTRUM_POISSON: the spectrum, L floats],
+// LDS: per row the FFT's two buffers (L float2), [the modes with two inverse transforms side by
+// side: the second transform's two buffers, L float2 per row], [ROW_SPECTRUM_POISSON: the spectrum, L floats],
 // then [TL: tw h, th h + 1, dw L float2, the element wavenumbers L floats].
 template <int MODE, bool TL, int LOGL>
 __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
@@ -731,10 +791,15 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     R.tid = threadIdx.x - sub * T;
     R.A = lbase + (size_t)sub * L;
     R.B = R.A + h;
-    lds_float* extra = (lds_float*)(lbase + (size_t)rpb * L) + (size_t)sub * L;
+    constexpr bool dual = row_mode_dual(MODE);
+    lds_float2* const dbase = lbase + (size_t)rpb * L;  // [dual] the second transform's buffers
+    lds_float2* const A1 = dbase + (size_t)sub * L;
+    lds_float2* const B1 = A1 + h;
+    lds_float2* const after = dual ? dbase + (size_t)rpb * L : dbase;
+    lds_float* extra = (lds_float*)after + (size_t)sub * L;
     const bool spectrum = MODE == ROW_SPECTRUM_POISSON;
     const bool wav = MODE == ROW_POISSON || MODE == ROW_SPECTRUM_POISSON;
-    lds_float2* tab = lbase + (size_t)rpb * L + (spectrum ? (size_t)rpb * L / 2 : 0);
+    lds_float2* tab = after + (spectrum ? (size_t)rpb * L / 2 : 0);
     lds_float* wcol_l = (lds_float*)(tab + 2 * h + 1 + L);
     if constexpr (TL) {
         for (int i = threadIdx.x; i < h; i += blockDim.x) st2(tab + i, a.tw[i]);
@@ -765,6 +830,34 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
         if (tr) o[(size_t)k * nrows + row] = v;
         else o[off + k] = v;
     };
+    // Transposed outputs when every row of every block is present: each row's outputs staged in
+    // its FFT buffer that the transform left free (the result lies in B after an odd number of
+    // Stockham swaps), then the block's rpb consecutive rows written together, rpb floats per
+    // output column (a 4-byte store per element into a different line otherwise).
+    const bool group = tr && rpb >= 2 && (rpb & 1) == 0 && nrows % rpb == 0;
+    const int lh = R.logh();
+    const bool res_b = (((lh & 1) + (lh >> 1)) & 1) != 0;
+    auto stage_of = [&](bool second, int s) -> lds_float* {
+        lds_float2* base = (second ? dbase : lbase) + (size_t)s * L;
+        return (lds_float*)(res_b ? base : base + h);
+    };
+    lds_float* const st0 = stage_of(false, sub);
+    lds_float* const st1 = stage_of(true, sub);
+    auto emit = [&](float* o, lds_float* st, int k, float v) {
+        if (group) st[k] = v;
+        else put(o, k, v);
+    };
+    auto flush = [&](float* o, bool second) {
+        if (!group) return;
+        __syncthreads();
+        const int row0 = lb * rpb, np = rpb / 2;
+        for (int i = threadIdx.x; i < L * np; i += blockDim.x) {
+            const int k = i / np, pr = i - k * np;
+            const float2 v = make_float2(stage_of(second, 2 * pr)[k], stage_of(second, 2 * pr + 1)[k]);
+            *reinterpret_cast<float2*>(o + (size_t)k * nrows + row0 + 2 * pr) = v;
+        }
+        __syncthreads();
+    };
     if (MODE == ROW_DCT2 || MODE == ROW_DCT2_SCALE) {
         const double scale = a.scale;
         const float* src = a.in0 + off;
@@ -773,8 +866,9 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
                 if ((row >= 1 && k == 0) || (row == 0 && k >= 1)) v = (float)((double)v * 0.5);
                 v = (float)((double)v * scale);
             }
-            put(a.out0, k, v);
+            emit(a.out0, st0, k, v);
         });
+        flush(a.out0, false);
     } else if (MODE == ROW_SPECTRUM_POISSON) {
         // the spectrum row, halved and scaled, then the potential p = A / (wx^2 + wy^2) of each
         // element (DC 0) in place: both field transforms read it
@@ -796,9 +890,15 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
             }
         }
         __syncthreads();
-        dct3_row(R, active, [&](int v) { return extra[v] * wu; }, false, [&](int j, float y) { put(a.out0, j, y); });
-        dct3_row(R, active, [&](int v) { return extra[v] * wcol(v); }, true, [&](int j, float y) { put(a.out1, j, y); });
-        if (a.want_phi) dct3_row(R, active, [&](int v) { return extra[v]; }, false, [&](int j, float y) { put(a.out2, j, y); });
+        dct3_row2(
+            R, A1, B1, active, [&](int v) { return extra[v] * wu; }, false, [&](int j, float y) { emit(a.out0, st0, j, y); },
+            [&](int v) { return extra[v] * wcol(v); }, true, [&](int j, float y) { emit(a.out1, st1, j, y); });
+        flush(a.out0, false);
+        flush(a.out1, true);
+        if (a.want_phi) {
+            dct3_row(R, active, [&](int v) { return extra[v]; }, false, [&](int j, float y) { emit(a.out2, st0, j, y); });
+            flush(a.out2, false);
+        }
     } else if (MODE == ROW_POISSON) {
         const float* A = a.in0 + off;
         const float wu = active ? a.wrow[row] : 0.0f, wu2 = wu * wu;
@@ -807,17 +907,27 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
             float wv = wcol(v);
             return A[v] / (wu2 + wv * wv);
         };
-        dct3_row(R, active, [&](int v) { return pot(v) * wu; }, false, [&](int j, float y) { put(a.out0, j, y); });
-        dct3_row(R, active, [&](int v) { return pot(v) * wcol(v); }, true, [&](int j, float y) { put(a.out1, j, y); });
-        if (a.want_phi) dct3_row(R, active, pot, false, [&](int j, float y) { put(a.out2, j, y); });
+        dct3_row2(
+            R, A1, B1, active, [&](int v) { return pot(v) * wu; }, false, [&](int j, float y) { emit(a.out0, st0, j, y); },
+            [&](int v) { return pot(v) * wcol(v); }, true, [&](int j, float y) { emit(a.out1, st1, j, y); });
+        flush(a.out0, false);
+        flush(a.out1, true);
+        if (a.want_phi) {
+            dct3_row(R, active, pot, false, [&](int j, float y) { emit(a.out2, st0, j, y); });
+            flush(a.out2, false);
+        }
     } else if (MODE == ROW_INV_FIELDS) {
         const float* X = a.in0 + off;
         const float* Y = a.in1 + off;
-        dct3_row(R, active, [&](int v) { return X[v]; }, true, [&](int j, float y) { put(a.out0, j, y); });
-        dct3_row(R, active, [&](int v) { return Y[v]; }, false, [&](int j, float y) { put(a.out1, j, y); });
+        dct3_row2(
+            R, A1, B1, active, [&](int v) { return X[v]; }, true, [&](int j, float y) { emit(a.out0, st0, j, y); },
+            [&](int v) { return Y[v]; }, false, [&](int j, float y) { emit(a.out1, st1, j, y); });
+        flush(a.out0, false);
+        flush(a.out1, true);
     } else if (MODE == ROW_INV_PHI) {
         const float* X = a.in0 + off;
-        dct3_row(R, active, [&](int v) { return X[v]; }, false, [&](int j, float y) { put(a.out0, j, y); });
+        dct3_row(R, active, [&](int v) { return X[v]; }, false, [&](int j, float y) { emit(a.out0, st0, j, y); });
+        flush(a.out0, false);
     }
 }
 
@@ -1799,7 +1909,7 @@ static void launch_rows(hipStream_t s, int mode, RowArgs a) {
     const int grid = (a.nrows + rpb - 1) / rpb;
     const bool spectrum = mode == ROW_SPECTRUM_POISSON;
     const bool wav = mode == ROW_POISSON || mode == ROW_SPECTRUM_POISSON;
-    const size_t rows_lds = (spectrum ? 12 : 8) * (size_t)a.L * rpb;
+    const size_t rows_lds = ((spectrum ? 12 : 8) + (row_mode_dual(mode) ? 8 : 0)) * (size_t)a.L * rpb;
     const size_t tab_lds = sizeof(float2) * (2 * (size_t)a.L + 1) + (wav ? 4 * (size_t)a.L : 0);
     const bool tl = rows_lds + tab_lds <= 64 * 1024;
     const size_t lds = rows_lds + (tl ? tab_lds : 0);
