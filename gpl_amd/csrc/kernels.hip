@@ -1514,6 +1514,17 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials,
                                                          const Sums* dev_tail, Sums* host_tail, unsigned long long* dev_seq,
                                                          unsigned long long* host_seq) {
     __shared__ double sh[6][kFinishBlock / 64];
+    // thread 0's scalars read first, so that their latency overlaps the partials'
+    unsigned long long seq = 0, h = 0, tail_h = 0, tail_p0 = 0, tail_p1 = 0;
+    if (threadIdx.x == 0) {
+        if (host_seq) seq = *dev_seq;
+        if (hpwl) h = *hpwl;
+        if (host_tail) {
+            tail_h = dev_tail->hpwl;
+            tail_p0 = dev_tail->pad[0];
+            tail_p1 = dev_tail->pad[1];
+        }
+    }
     double v[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
@@ -1543,15 +1554,14 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials,
         }
     }
     if (threadIdx.x == 0 && hpwl) {
-        const unsigned long long h = *hpwl;
         out->hpwl = h;
         if (host_out) host_out->hpwl = h;
     }
     if (threadIdx.x == 0 && host_tail) {
         // the HPWL counter and the scatter's global-add count
-        host_tail->hpwl = dev_tail->hpwl;
-        host_tail->pad[0] = dev_tail->pad[0];
-        host_tail->pad[1] = dev_tail->pad[1];
+        host_tail->hpwl = tail_h;
+        host_tail->pad[0] = tail_p0;
+        host_tail->pad[1] = tail_p1;
     }
     if (host_seq) {
         // the trial's last finishing reduction: once every host write above is visible system-wide,
@@ -1559,9 +1569,9 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials,
         if (threadIdx.x < 6) __threadfence_system();
         __syncthreads();
         if (threadIdx.x == 0) {
-            const unsigned long long q = *dev_seq + 1ull;
+            // (the host reads only host_seq: the device copy needs no ordering against it)
+            const unsigned long long q = seq + 1ull;
             *dev_seq = q;
-            __threadfence_system();
             *reinterpret_cast<volatile unsigned long long*>(host_seq) = q;
         }
     }
