@@ -40,7 +40,7 @@ struct RegionDev {
     float* field_x;
     float* field_y;
     float* phi;
-    float* work0;  // transform scratch, cnt_x*cnt_y each
+    float* work0;  // transform scratch: 2 cnt_x*cnt_y (the three-pass solve's field pairs), the others cnt_x*cnt_y
     float* work1;
     float* work2;
     float* work3;

@@ -757,6 +757,8 @@ struct RowArgs {
     double scale;
     int want_phi;
     int transpose_out;   // outputs written transposed: out[k * nrows + row]
+    int interleave;      // ROW_SPECTRUM_POISSON: the two fields as pairs (Ex', Ey') in out0, [k][row][2];
+                         // ROW_INV_FIELDS: in0 holds such pairs, [row][v][2]
 };
 
 // Row transforms, L/8 threads per row, 256 / (L/8) rows per block (rows of one XCD contiguous,
@@ -890,11 +892,37 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
             }
         }
         __syncthreads();
+        const bool il = a.interleave != 0;
         dct3_row2(
-            R, A1, B1, active, [&](int v) { return extra[v] * wu; }, false, [&](int j, float y) { emit(a.out0, st0, j, y); },
-            [&](int v) { return extra[v] * wcol(v); }, true, [&](int j, float y) { emit(a.out1, st1, j, y); });
-        flush(a.out0, false);
-        flush(a.out1, true);
+            R, A1, B1, active, [&](int v) { return extra[v] * wu; }, false,
+            [&](int j, float y) {
+                if (group) st0[j] = y;
+                else if (il) a.out0[((size_t)j * nrows + row) * 2] = y;
+                else put(a.out0, j, y);
+            },
+            [&](int v) { return extra[v] * wcol(v); }, true,
+            [&](int j, float y) {
+                if (group) st1[j] = y;
+                else if (il) a.out0[((size_t)j * nrows + row) * 2 + 1] = y;
+                else put(a.out1, j, y);
+            });
+        if (a.interleave) {
+            if (group) {
+                // the block's rows' (Ex', Ey') pairs, 2 rpb floats per output column
+                __syncthreads();
+                const int row0 = lb * rpb, np = rpb / 2;
+                for (int i = threadIdx.x; i < L * np; i += blockDim.x) {
+                    const int k = i / np, pr = i - k * np;
+                    const float4 v = make_float4(stage_of(false, 2 * pr)[k], stage_of(true, 2 * pr)[k], stage_of(false, 2 * pr + 1)[k],
+                                                 stage_of(true, 2 * pr + 1)[k]);
+                    *reinterpret_cast<float4*>(a.out0 + ((size_t)k * nrows + row0 + 2 * pr) * 2) = v;
+                }
+                __syncthreads();
+            }
+        } else {
+            flush(a.out0, false);
+            flush(a.out1, true);
+        }
         if (a.want_phi) {
             dct3_row(R, active, [&](int v) { return extra[v]; }, false, [&](int j, float y) { emit(a.out2, st0, j, y); });
             flush(a.out2, false);
@@ -917,11 +945,12 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
             flush(a.out2, false);
         }
     } else if (MODE == ROW_INV_FIELDS) {
-        const float* X = a.in0 + off;
-        const float* Y = a.in1 + off;
+        const int il = a.interleave ? 2 : 1;
+        const float* X = il == 2 ? a.in0 + 2 * off : a.in0 + off;
+        const float* Y = il == 2 ? a.in0 + 2 * off + 1 : a.in1 + off;
         dct3_row2(
-            R, A1, B1, active, [&](int v) { return X[v]; }, true, [&](int j, float y) { emit(a.out0, st0, j, y); },
-            [&](int v) { return Y[v]; }, false, [&](int j, float y) { emit(a.out1, st1, j, y); });
+            R, A1, B1, active, [&](int v) { return X[il * v]; }, true, [&](int j, float y) { emit(a.out0, st0, j, y); },
+            [&](int v) { return Y[il * v]; }, false, [&](int j, float y) { emit(a.out1, st1, j, y); });
         flush(a.out0, false);
         flush(a.out1, true);
     } else if (MODE == ROW_INV_PHI) {
@@ -1966,9 +1995,9 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, bool want_phi) {
     launch_rows(s, ROW_DCT2, a);
     a = RowArgs{};
     a.in0 = r.work1;
-    a.out0 = r.work0;  // Ex' [y][u]
-    a.out1 = r.work2;  // Ey' [y][u]
+    a.out0 = r.work0;  // (Ex', Ey') pairs [y][u][2] (work0 holds 2 cnt_x cnt_y floats)
     a.out2 = r.work3;  // phi' [y][u]
+    a.interleave = 1;
     a.L = ny;
     a.nrows = nx;
     a.tw = r.tw_y;
@@ -1982,7 +2011,7 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, bool want_phi) {
     launch_rows(s, ROW_SPECTRUM_POISSON, a);
     a = RowArgs{};
     a.in0 = r.work0;
-    a.in1 = r.work2;
+    a.interleave = 1;
     a.out0 = r.field_x;
     a.out1 = r.field_y;
     a.L = nx;

@@ -1512,7 +1512,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             d.field_x = c->dalloc<float>(M);
             d.field_y = c->dalloc<float>(M);
             d.phi = c->dalloc<float>(M);
-            d.work0 = c->dalloc<float>(M);
+            d.work0 = c->dalloc<float>(2 * M);  // the solve's (Ex', Ey') pairs
             d.work1 = c->dalloc<float>(M);
             d.work2 = c->dalloc<float>(M);
             d.work3 = c->dalloc<float>(M);
