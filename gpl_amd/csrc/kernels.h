@@ -11,7 +11,7 @@
 //   nets    sorted by degree, then spatially (the host keeps the permutation): start int32,
 //           weight f32; a 64-byte record (box, the WA sums) for the kernel-level entry.
 //   bins    y-major (k = y*cnt_x + x): inst/filler int64 accumulators, non_place (+unscaled)
-//           int64, area f32*target (scaled), density f32, field_x/field_y f32, scratch for the
+//           int64, area f32*target (scaled), density f32, the field as (x, y) f32 pairs, scratch for the
 //           transforms.
 #pragma once
 
@@ -37,8 +37,7 @@ struct RegionDev {
     const long long* non_place;
     const long long* non_place_unscaled;
     float* density;
-    float* field_x;
-    float* field_y;
+    float2* field;  // the electric field per bin as (x, y) pairs
     float* phi;
     float* work0;  // transform scratch: 2 cnt_x*cnt_y (the three-pass solve's field pairs), the others cnt_x*cnt_y
     float* work1;
@@ -128,7 +127,7 @@ void launch_update(hipStream_t s, const RegionDev& r, const CellsDev& c, const f
 void launch_density_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, unsigned long long* global_adds);
 // per bin: density, overflow partials, accumulators cleared.
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials);
-// the Poisson solve of r.density into r.field_x / r.field_y (and r.phi when want_phi).
+// the Poisson solve of r.density into r.field (and r.phi when want_phi).
 void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi);
 // WA forward over every net, the HPWL, and each pin's weighted WA gradient into n.pgrad (at its
 // cell-pin index); the net records too when write_rec.  The high-fanout nets go to s_big (which

@@ -759,6 +759,7 @@ struct RowArgs {
     int transpose_out;   // outputs written transposed: out[k * nrows + row]
     int interleave;      // ROW_SPECTRUM_POISSON: the two fields as pairs (Ex', Ey') in out0, [k][row][2];
                          // ROW_INV_FIELDS: in0 holds such pairs, [row][v][2]
+    float2* pairs;       // ROW_INV_FIELDS: the two outputs as (out0, out1) pairs, [row][j] (out0/out1 unused)
 };
 
 // Row transforms, L/8 threads per row, 256 / (L/8) rows per block (rows of one XCD contiguous,
@@ -948,11 +949,20 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
         const int il = a.interleave ? 2 : 1;
         const float* X = il == 2 ? a.in0 + 2 * off : a.in0 + off;
         const float* Y = il == 2 ? a.in0 + 2 * off + 1 : a.in1 + off;
-        dct3_row2(
-            R, A1, B1, active, [&](int v) { return X[il * v]; }, true, [&](int j, float y) { emit(a.out0, st0, j, y); },
-            [&](int v) { return Y[il * v]; }, false, [&](int j, float y) { emit(a.out1, st1, j, y); });
-        flush(a.out0, false);
-        flush(a.out1, true);
+        if (a.pairs) {
+            // both outputs staged in the freed FFT buffers, then the row written as (x, y) pairs
+            dct3_row2(
+                R, A1, B1, active, [&](int v) { return X[il * v]; }, true, [&](int j, float y) { st0[j] = y; },
+                [&](int v) { return Y[il * v]; }, false, [&](int j, float y) { st1[j] = y; });
+            if (active)
+                for (int j = R.tid; j < L; j += T) a.pairs[off + j] = make_float2(st0[j], st1[j]);
+        } else {
+            dct3_row2(
+                R, A1, B1, active, [&](int v) { return X[il * v]; }, true, [&](int j, float y) { emit(a.out0, st0, j, y); },
+                [&](int v) { return Y[il * v]; }, false, [&](int j, float y) { emit(a.out1, st1, j, y); });
+            flush(a.out0, false);
+            flush(a.out1, true);
+        }
     } else if (MODE == ROW_INV_PHI) {
         const float* X = a.in0 + off;
         dct3_row(R, active, [&](int v) { return X[v]; }, false, [&](int j, float y) { emit(a.out0, st0, j, y); });
@@ -1416,14 +1426,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
             ex[i] = bin_edge(r.bin_x, r.edges_rounded, r.lx, r.size_x, r.cnt_x, x0 + i);
             ey[i] = bin_edge(r.bin_y, r.edges_rounded, r.ly, r.size_y, r.cnt_y, y0 + i);
         }
-        float wfx[9], wfy[9];
+        float2 wf[9];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const size_t kb = (size_t)min(y0 + j, r.cnt_y - 1) * r.cnt_x + min(x0 + i, r.cnt_x - 1);
-                wfx[3 * i + j] = r.field_x[kb];
-                wfy[3 * i + j] = r.field_y[kb];
+                wf[3 * i + j] = r.field[kb];
             }
         }
         // the WA gradient: the pins' terms in the cell's pin order
@@ -1450,8 +1459,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
                 for (int j = 0; j < 3; ++j) {
                     if (j >= y1 - y0) break;
                     float a = overlap_area(ex[i], ey[j], ex[i + 1], ey[j + 1], d) * scale;
-                    fx += a * wfx[3 * i + j];
-                    fy += a * wfy[3 * i + j];
+                    fx += a * wf[3 * i + j].x;
+                    fy += a * wf[3 * i + j].y;
                 }
             }
         } else {
@@ -1460,8 +1469,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
                 for (int y = y0; y < y1; ++y) {
                     float a = overlap_area(bx0, r.bin_y[y], bx1, r.bin_y[y + 1], d) * scale;
                     size_t kb = (size_t)y * r.cnt_x + x;
-                    fx += a * r.field_x[kb];
-                    fy += a * r.field_y[kb];
+                    const float2 f = r.field[kb];
+                    fx += a * f.x;
+                    fy += a * f.y;
                 }
             }
         }
@@ -1980,7 +1990,7 @@ static bool three_pass(const RegionDev& r) { return r.cnt_x <= 2048 && r.cnt_y <
 // Q2 rows along y (u = row): DCT-II, halving and scale (nesterov.rs:415-424) kept in LDS, the
 //    Poisson divide (425-440) and the inverse transforms along y -> Ex'[y][u] (cos), Ey'[y][u]
 //    (sin), phi'[y][u];
-// Q3 rows along x: field_x = sin along x, field_y = cos along x (and phi = cos along x).
+// Q3 rows along x: the field pairs (sin along x, cos along x) (and phi = cos along x).
 static void solve_three_pass(hipStream_t s, const RegionDev& r, bool want_phi) {
     const int nx = r.cnt_x, ny = r.cnt_y;
     RowArgs a{};
@@ -2012,8 +2022,7 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, bool want_phi) {
     a = RowArgs{};
     a.in0 = r.work0;
     a.interleave = 1;
-    a.out0 = r.field_x;
-    a.out1 = r.field_y;
+    a.pairs = r.field;
     a.L = nx;
     a.nrows = ny;
     a.tw = r.tw_x;
@@ -2036,7 +2045,7 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, bool want_phi) {
 // Grids beyond the three-pass form: bins are y-major D[y][x].  P1: rows along x, DCT-II -> B[y][u]; transpose -> [u][y];
 // P2: rows along y, DCT-II + halving + scale -> A[u][v]; P3: rows along v, the Poisson divide and
 // the inverse transforms along y -> Ex'[u][y] (cos), Ey'[u][y] (sin), phi'[u][y]; transpose ->
-// [y][u]; P4: rows along u -> field_x = sin along x, field_y = cos along x, phi = cos along x.
+// [y][u]; P4: rows along u -> the field pairs (sin along x, cos along x), phi = cos along x.
 void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi) {
     if (three_pass(r)) {
         solve_three_pass(s, r, want_phi);
@@ -2087,8 +2096,7 @@ void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi) {
     a = RowArgs{};
     a.in0 = r.work0;
     a.in1 = r.work1;
-    a.out0 = r.field_x;
-    a.out1 = r.field_y;
+    a.pairs = r.field;
     a.L = nx;
     a.nrows = ny;
     a.tw = r.tw_x;

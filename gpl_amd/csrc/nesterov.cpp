@@ -1509,8 +1509,7 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             d.non_place = (const long long*)c->upload(r.R.non_place, M);
             d.non_place_unscaled = (const long long*)c->upload(r.R.non_place_unscaled, M);
             d.density = c->dalloc<float>(M);
-            d.field_x = c->dalloc<float>(M);
-            d.field_y = c->dalloc<float>(M);
+            d.field = c->dalloc<float2>(M);
             d.phi = c->dalloc<float>(M);
             d.work0 = c->dalloc<float>(2 * M);  // the solve's (Ex', Ey') pairs
             d.work1 = c->dalloc<float>(M);
@@ -1822,9 +1821,16 @@ int32_t gpl_density_solve(gpl_ctx* c, int32_t region, float* phi, float* field_x
         size_t M = size_t(r.R.cnt_x) * r.R.cnt_y;
         c->solve(r, phi != nullptr);
         if (phi) check(hipMemcpyAsync(phi, r.dev.phi, 4 * M, hipMemcpyDeviceToHost, c->stream), "copy");
-        if (field_x) check(hipMemcpyAsync(field_x, r.dev.field_x, 4 * M, hipMemcpyDeviceToHost, c->stream), "copy");
-        if (field_y) check(hipMemcpyAsync(field_y, r.dev.field_y, 4 * M, hipMemcpyDeviceToHost, c->stream), "copy");
+        std::vector<float2> f;
+        if (field_x || field_y) {
+            f.resize(M);
+            check(hipMemcpyAsync(f.data(), r.dev.field, sizeof(float2) * M, hipMemcpyDeviceToHost, c->stream), "copy");
+        }
         check(hipStreamSynchronize(c->stream), "sync");
+        for (size_t k = 0; k < f.size(); ++k) {
+            if (field_x) field_x[k] = f[k].x;
+            if (field_y) field_y[k] = f[k].y;
+        }
         c->collect();
     } catch (const HipError& e) {
         return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
