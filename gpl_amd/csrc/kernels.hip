@@ -1372,14 +1372,14 @@ __global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, floa
     }
 }
 
-constexpr int kGradPins = 8;
+constexpr int kGradPins = 6;
 constexpr int kGradStage = 2048;  // pin terms of a block's cells staged in LDS
 // updateGradients (nesterov.rs:2455-2489) per cell: the WA gradient over the cell's pins in
 // their order (each pin's weighted term from the WA kernel); the density gradient over the bins its box
 // covers, x outer, y inner (2434); the preconditioned sum; FP64 partial sums of |wl|, |den|,
 // and, for a trial, the squared coordinate and gradient distances (nesterovUpdateStepLength,
 // 2755-2756) to a_slp/a_sum.
-// (six waves per SIMD: 80 registers, no spill)
+// (six waves per SIMD: 76 registers, no spill)
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float cy, float penalty, float2* wl_out, float2* den_out,
                            float2* sum_out, const float2* a_slp, const float2* b_slp, const float2* a_sum, double* partials,
                            int nan_inject) {
