@@ -29,6 +29,7 @@ struct RegionDev {
     int32_t cell_begin, cell_end;
     int32_t narrow;  // 1: the scatter's LDS tables may hold 32-bit sums (kPlaceChunk x the largest bin area < 2^32)
     int32_t edges_rounded;  // 1: bin_x[i] = lx + round(i size_x) and bin_y alike (init_bins' edges)
+    int32_t grad_stage;     // the gradient stages a block's pin terms in LDS up to this many (at most kGradStage)
     // bins
     const int32_t* bin_x;  // cnt_x + 1
     const int32_t* bin_y;  // cnt_y + 1
@@ -106,6 +107,7 @@ struct Sums {
 };
 
 constexpr int kPartials = 2048;  // per-block partial slots per quantity
+constexpr int kGradStage = 2048;  // pin terms of a gradient block's cells staged in LDS, at most
 constexpr int kPlaceChunk = 512;  // cells per block iteration of the scatter family
 constexpr int kTilePins = 2048;   // pins a WA tile stages in LDS
 constexpr int kTileSpan = kTilePins - 128;  // pin range whose nets form a tile (a small net has <= 128 pins)

@@ -1373,7 +1373,6 @@ __global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, floa
 }
 
 constexpr int kGradPins = 6;
-constexpr int kGradStage = 2048;  // pin terms of a block's cells staged in LDS
 // updateGradients (nesterov.rs:2455-2489) per cell: the WA gradient over the cell's pins in
 // their order (each pin's weighted term from the WA kernel); the density gradient over the bins its box
 // covers, x outer, y inner (2434); the preconditioned sum; FP64 partial sums of |wl|, |den|,
@@ -1393,7 +1392,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
         // order): read coalesced into LDS when it fits
         const int qb = c.pin_start[r.cell_begin + base];
         const int qe = c.pin_start[r.cell_begin + min(base + kBlock, n)];
-        const bool staged = qe - qb <= kGradStage;
+        const bool staged = qe - qb <= min(r.grad_stage, kGradStage);
         __syncthreads();
         if (staged)
             for (int i = threadIdx.x; i < qe - qb; i += kBlock) spg[i] = nt.pgrad[qb + i];

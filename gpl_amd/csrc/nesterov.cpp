@@ -54,6 +54,7 @@ struct Region {
     float coordi_distance = 0, peak_coordi_distance = 0, max_phi_coef = 0;
     bool max_phi_coef_changed = false, is_converged = false, is_diverged = false, reprint_header = false;
     int64_t nia = 0;  // nesterov_insts_area
+    int32_t edges_rounded = 0;  // dev.edges_rounded as found at construction
 };
 
 // Device time per kernel family (profiling mode).
@@ -1500,6 +1501,8 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
                 for (int32_t y = 0; y <= r.R.cnt_y; ++y)
                     rounded = rounded && r.R.bin_y[y] == d.ly + (int32_t)std::round(double(y) * d.size_y);
                 d.edges_rounded = rounded ? 1 : 0;
+                r.edges_rounded = d.edges_rounded;
+                d.grad_stage = gplk::kGradStage;
             }
             size_t M = size_t(r.R.cnt_x) * r.R.cnt_y;
             d.bin_x = c->upload(r.R.bin_x, size_t(r.R.cnt_x) + 1);
@@ -1936,7 +1939,16 @@ int32_t gpl_nesterov_step(gpl_ctx* c, int32_t region, float step, float coeff, f
 }
 
 int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter) {
-    if (!c || kind < 0 || kind > 3) return GPL_ERR_ARG;
+    if (!c || kind < 0 || kind > 4) return GPL_ERR_ARG;
+    if (kind == 0 || kind == 4) {
+        // kind 4: the kernels' general paths (bin edges from the tables, the gradient's per-cell
+        // gathers of pin terms); they give the same results as the fast ones
+        for (auto& r : c->rg) {
+            r.dev.edges_rounded = kind == 4 ? 0 : r.edges_rounded;
+            r.dev.grad_stage = kind == 4 ? 0 : gplk::kGradStage;
+        }
+        if (kind == 4) return GPL_OK;
+    }
     c->inject_kind = kind;
     c->inject_after = kind == 0 ? -1 : after_iter;
     return GPL_OK;

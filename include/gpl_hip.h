@@ -378,7 +378,10 @@ int32_t gpl_objective_backward(gpl_ctx* c, const float* xy, float wl_coef, float
 /* Fault injection for host-driven failure tests: after `after_iter` iterations the next
  * gradient sum reads NaN (GPL-0306) -- kind 1 -- or the next step length is NaN (GPL-0305) --
  * kind 2; kind 3 re-sorts the device cells after every iteration from `after_iter` on (the
- * storage order must not change the results).  kind 0 clears it. */
+ * storage order must not change the results); kind 4 (from now on, `after_iter` unused) runs
+ * the kernels' general paths where they have fast ones (bin edges read from the region's tables,
+ * the gradient's per-cell gathers of the pin terms), which must give the same results.  kind 0
+ * clears it. */
 int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter);
 
 /* Device timing of the kernels while profiling (gpl_set_profiling; milliseconds summed per

@@ -273,6 +273,28 @@ def test_launch_modes_give_the_same_placement(cfg):
         assert np.array_equal(o[2], out[0][2])
 
 
+def test_general_kernel_paths_give_the_same_placement():
+    """The gradient's per-cell gathers of the pin terms (the path for blocks whose cells hold more
+    pins than the LDS stage) and the bin edges read from the region's tables (the path for bin
+    grids not built by init_bins), forced by debug kind 4, against the fast paths: the same
+    iterations, rows and placement, bit for bit."""
+    d = G.Design.from_config("c2")
+    d.initial_place()
+    d.init_nesterov_place()
+    out = []
+    for general in (False, True):
+        g = G.NesterovGPU(d)
+        if general:
+            g.debug_inject(4, 0)
+        it = g.do_nesterov_place()
+        out.append((it, g.history(), g.placed_locations(), g.stats().hpwl))
+        g.close()
+    d.close()
+    assert out[1][0] == out[0][0] and out[1][3] == out[0][3]
+    assert out[1][1] == out[0][1]
+    assert np.array_equal(out[1][2], out[0][2])
+
+
 def test_region_groups_match_the_oracle():
     """Nine regions (eight region groups and the top level) with their own bin grids, 8^2 .. 128^2:
     the GPU loop against the oracle's, and the trial graph (nine scatter and gradient nodes)
