@@ -1836,18 +1836,17 @@ int block_partials_bins(const RegionDev& r) { return grid_for((size_t)r.cnt_x * 
 // The grid of the per-cell kernels that leave FP64 partials (the gradient, the distances): no more
 // blocks than the device holds at once at the gradient's occupancy, so that no second, part-filled
 // round of blocks follows the first.  Fixed per device (and so is the partials' summation order).
-static int g_cells_cap = 0;
+// (computed once per process, thread-safely: contexts may run from several host threads)
 static int cells_grid(size_t n) {
-    if (g_cells_cap == 0) {
+    static const int cap = [] {
         int dev = 0, cus = 0, per = 0;
         if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_gradient), kBlock, 0) == hipSuccess &&
             cus > 0 && per > 0)
-            g_cells_cap = std::max(kXcds, std::min(kMaxBlocks, cus * per / kXcds * kXcds));
-        else
-            g_cells_cap = kMaxBlocks;
-    }
-    return std::min(grid_for(n), g_cells_cap);
+            return std::max(kXcds, std::min(kMaxBlocks, cus * per / kXcds * kXcds));
+        return kMaxBlocks;
+    }();
+    return std::min(grid_for(n), cap);
 }
 int block_partials_cells(const RegionDev& r) { return cells_grid((size_t)(r.cell_end - r.cell_begin)); }
 
