@@ -222,6 +222,9 @@ __global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c
 // the bin's place in the box, one LDS atomic per overlap; otherwise into the hash table.
 template <int MODE, typename V>
 __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev c, PlaceArgs pa) {
+    // the trial's critical chain (scatter -> bins -> solve) shares the CUs with the WA kernels on
+    // the side streams: its waves take the SIMDs' issue slots first
+    __builtin_amdgcn_s_setprio(2);
     constexpr int kDense = sizeof(V) == 4 ? 3072 : 2048;
     constexpr int kHashBytes = kSlots * (int)(sizeof(unsigned) + 2 * sizeof(V));
     constexpr int kDenseBytes = 2 * kDense * (int)sizeof(V);
@@ -439,6 +442,7 @@ __device__ void finish_sums(const double* partials, int nb_bins, int nb_cells, S
 // updateBinsGCellDensityArea's per-bin part (nesterov.rs:2252-2262): density, the overflow
 // terms (summed here in FP64), and the accumulators cleared for the next scatter.
 __global__ void k_bin_density(RegionDev r, double* partials) {
+    __builtin_amdgcn_s_setprio(2);  // on the trial's critical chain (as the scatter)
     __shared__ double sh[kBlock / 64];
     const size_t M = (size_t)r.cnt_x * r.cnt_y;
     double o = 0.0, ou = 0.0;
@@ -780,6 +784,7 @@ struct RowArgs {
 // then [TL: tw h, th h + 1, dw L float2, the element wavenumbers L floats].
 template <int MODE, bool TL, int LOGL>
 __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
+    __builtin_amdgcn_s_setprio(2);  // on the trial's critical chain (as the scatter)
     extern __shared__ float2 lds2[];
     lds_float2* const lbase = (lds_float2*)lds2;
     using TP = typename std::conditional<TL, const lds_float2*, const float2*>::type;
