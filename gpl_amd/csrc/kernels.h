@@ -10,8 +10,8 @@
 //           index int32.
 //   nets    sorted by degree, then spatially (the host keeps the permutation): start int32,
 //           weight f32; a 64-byte record (box, the WA sums) for the kernel-level entry.
-//   bins    y-major (k = y*cnt_x + x): inst/filler int64 accumulators, non_place (+unscaled)
-//           int64, area f32*target (scaled), density f32, the field as (x, y) f32 pairs, scratch for the
+//   bins    y-major (k = y*cnt_x + x): inst/filler int64 accumulators, non_place
+//           int64 and (with the unscaled one) as f32 pairs, density f32, the field as (x, y) f32 pairs, scratch for the
 //           transforms.
 #pragma once
 
@@ -36,7 +36,7 @@ struct RegionDev {
     unsigned long long* inst;    // int64 accumulators (two's complement in u64)
     unsigned long long* filler;
     const long long* non_place;
-    const long long* non_place_unscaled;
+    const float2* non_place_f;  // per bin ((float)non_place, (float)non_place_unscaled), as the bins pass uses them
     float* density;
     float2* field;  // the electric field per bin as (x, y) pairs
     float* phi;

@@ -450,12 +450,12 @@ __global__ void k_bin_density(RegionDev r, double* partials) {
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < M; k += (size_t)gridDim.x * blockDim.x) {
         int x = (int)(k % r.cnt_x), y = (int)(k / r.cnt_x);
         long long inst = (long long)r.inst[k], fill = (long long)r.filler[k];
-        long long np = r.non_place[k], npu = r.non_place_unscaled[k];
+        const float2 npf = r.non_place_f[k];  // the two int64 constants, converted once on the host
         long long area = (long long)(r.bin_x[x + 1] - r.bin_x[x]) * (long long)(r.bin_y[y + 1] - r.bin_y[y]);
         float scaled = (float)area * td;
-        r.density[k] = ((float)inst + (float)fill + (float)np) / scaled;
-        float t = rmax((float)inst + (float)np - scaled, 0.0f);
-        float tu = rmax((float)inst + (float)npu - scaled, 0.0f);
+        r.density[k] = ((float)inst + (float)fill + npf.x) / scaled;
+        float t = rmax((float)inst + npf.x - scaled, 0.0f);
+        float tu = rmax((float)inst + npf.y - scaled, 0.0f);
         if (!(t >= 0.0f)) t = 0.0f;
         if (!(tu >= 0.0f)) tu = 0.0f;
         o += (double)t;

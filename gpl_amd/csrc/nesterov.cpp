@@ -1510,7 +1510,13 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             d.inst = c->dalloc<unsigned long long>(M);
             d.filler = c->dalloc<unsigned long long>(M);
             d.non_place = (const long long*)c->upload(r.R.non_place, M);
-            d.non_place_unscaled = (const long long*)c->upload(r.R.non_place_unscaled, M);
+            {
+                // int64 -> float rounds to nearest even on the host as on the device
+                std::vector<float2> npf(M);
+                for (size_t k = 0; k < M; ++k)
+                    npf[k] = make_float2((float)(long long)r.R.non_place[k], (float)(long long)r.R.non_place_unscaled[k]);
+                d.non_place_f = c->upload(npf.data(), M);
+            }
             d.density = c->dalloc<float>(M);
             d.field = c->dalloc<float2>(M);
             d.phi = c->dalloc<float>(M);
