@@ -40,6 +40,11 @@ struct RegionDev {
     float* density;
     float2* field;  // the electric field per bin as (x, y) pairs
     float* phi;
+};
+
+// The Poisson solve's buffers and tables of a region (host side: the row kernels get them through
+// their own arguments, so the kernels that take a RegionDev by value do not carry them).
+struct SolveBufs {
     float* work0;  // transform scratch: 2 cnt_x*cnt_y (the three-pass solve's field pairs), the others cnt_x*cnt_y
     float* work1;
     float* work2;
@@ -130,7 +135,7 @@ void launch_density_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c
 // per bin: density, overflow partials, accumulators cleared.
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials);
 // the Poisson solve of r.density into r.field (and r.phi when want_phi).
-void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi);
+void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi);
 // WA forward over every net, the HPWL, and each pin's weighted WA gradient into n.pgrad (at its
 // cell-pin index); the net records too when write_rec.  The high-fanout nets go to s_big (which
 // may be s; the caller orders s_big after the zeroed HPWL counter and joins it).
@@ -162,7 +167,7 @@ void launch_hpwl(hipStream_t s, const NetsDev& n, const CellsDev& c, unsigned lo
 // density: rho into r.density from the continuous boxes, then the solve with phi;
 // value: out2[0] = sum_e w_e WA_e (exact exponential), out2[1] = sum_b rho_b phi_b (FP64);
 // gradient: per cell d/dxy of WA + lambda * 1/2 sum rho phi (needs the density step first).
-void launch_objective_density(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy);
+void launch_objective_density(hipStream_t s, const RegionDev& r, const SolveBufs& sb, const CellsDev& c, const float2* xy);
 void launch_objective_value(hipStream_t s, const RegionDev& r, const NetsDev& n, const float2* xy, float coef, double* partials,
                             double* out2);
 void launch_objective_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, const float2* xy, float coef,

@@ -1994,53 +1994,53 @@ static bool three_pass(const RegionDev& r) { return r.cnt_x <= 2048 && r.cnt_y <
 //    Poisson divide (425-440) and the inverse transforms along y -> Ex'[y][u] (cos), Ey'[y][u]
 //    (sin), phi'[y][u];
 // Q3 rows along x: the field pairs (sin along x, cos along x) (and phi = cos along x).
-static void solve_three_pass(hipStream_t s, const RegionDev& r, bool want_phi) {
+static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi) {
     const int nx = r.cnt_x, ny = r.cnt_y;
     RowArgs a{};
     a.in0 = r.density;
-    a.out0 = r.work1;  // [u][y]
+    a.out0 = sb.work1;  // [u][y]
     a.L = nx;
     a.nrows = ny;
-    a.tw = r.tw_x;
-    a.th = r.th_x;
-    a.dw = r.dw_x;
+    a.tw = sb.tw_x;
+    a.th = sb.th_x;
+    a.dw = sb.dw_x;
     a.transpose_out = 1;
     launch_rows(s, ROW_DCT2, a);
     a = RowArgs{};
-    a.in0 = r.work1;
-    a.out0 = r.work0;  // (Ex', Ey') pairs [y][u][2] (work0 holds 2 cnt_x cnt_y floats)
-    a.out2 = r.work3;  // phi' [y][u]
+    a.in0 = sb.work1;
+    a.out0 = sb.work0;  // (Ex', Ey') pairs [y][u][2] (work0 holds 2 cnt_x cnt_y floats)
+    a.out2 = sb.work3;  // phi' [y][u]
     a.interleave = 1;
     a.L = ny;
     a.nrows = nx;
-    a.tw = r.tw_y;
-    a.th = r.th_y;
-    a.dw = r.dw_y;
+    a.tw = sb.tw_y;
+    a.th = sb.th_y;
+    a.dw = sb.dw_y;
     a.scale = 4.0 / (double)nx / (double)ny;
-    a.wrow = r.wx;
-    a.wcol = r.wy;
+    a.wrow = sb.wx;
+    a.wcol = sb.wy;
     a.want_phi = want_phi ? 1 : 0;
     a.transpose_out = 1;
     launch_rows(s, ROW_SPECTRUM_POISSON, a);
     a = RowArgs{};
-    a.in0 = r.work0;
+    a.in0 = sb.work0;
     a.interleave = 1;
     a.pairs = r.field;
     a.L = nx;
     a.nrows = ny;
-    a.tw = r.tw_x;
-    a.th = r.th_x;
-    a.dw = r.dw_x;
+    a.tw = sb.tw_x;
+    a.th = sb.th_x;
+    a.dw = sb.dw_x;
     launch_rows(s, ROW_INV_FIELDS, a);
     if (want_phi) {
         a = RowArgs{};
-        a.in0 = r.work3;
+        a.in0 = sb.work3;
         a.out0 = r.phi;
         a.L = nx;
         a.nrows = ny;
-        a.tw = r.tw_x;
-        a.th = r.th_x;
-        a.dw = r.dw_x;
+        a.tw = sb.tw_x;
+        a.th = sb.th_x;
+        a.dw = sb.dw_x;
         launch_rows(s, ROW_INV_PHI, a);
     }
 }
@@ -2049,72 +2049,72 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, bool want_phi) {
 // P2: rows along y, DCT-II + halving + scale -> A[u][v]; P3: rows along v, the Poisson divide and
 // the inverse transforms along y -> Ex'[u][y] (cos), Ey'[u][y] (sin), phi'[u][y]; transpose ->
 // [y][u]; P4: rows along u -> the field pairs (sin along x, cos along x), phi = cos along x.
-void launch_solve(hipStream_t s, const RegionDev& r, bool want_phi) {
+void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi) {
     if (three_pass(r)) {
-        solve_three_pass(s, r, want_phi);
+        solve_three_pass(s, r, sb, want_phi);
         return;
     }
     const int nx = r.cnt_x, ny = r.cnt_y;
     RowArgs a{};
     // P1
     a.in0 = r.density;
-    a.out0 = r.work0;
+    a.out0 = sb.work0;
     a.L = nx;
     a.nrows = ny;
-    a.tw = r.tw_x;
-    a.th = r.th_x;
-    a.dw = r.dw_x;
+    a.tw = sb.tw_x;
+    a.th = sb.th_x;
+    a.dw = sb.dw_x;
     launch_rows(s, ROW_DCT2, a);
-    transpose(s, r.work0, r.work1, ny, nx);  // [u][y]
+    transpose(s, sb.work0, sb.work1, ny, nx);  // [u][y]
     // P2
     a = RowArgs{};
-    a.in0 = r.work1;
-    a.out0 = r.work0;
+    a.in0 = sb.work1;
+    a.out0 = sb.work0;
     a.L = ny;
     a.nrows = nx;
-    a.tw = r.tw_y;
-    a.th = r.th_y;
-    a.dw = r.dw_y;
+    a.tw = sb.tw_y;
+    a.th = sb.th_y;
+    a.dw = sb.dw_y;
     a.scale = 4.0 / (double)nx / (double)ny;
     launch_rows(s, ROW_DCT2_SCALE, a);  // A[u][v]
     // P3
     a = RowArgs{};
-    a.in0 = r.work0;
-    a.out0 = r.work1;  // Ex' [u][y]
-    a.out1 = r.work2;  // Ey' [u][y]
-    a.out2 = r.work3;  // phi' [u][y]
+    a.in0 = sb.work0;
+    a.out0 = sb.work1;  // Ex' [u][y]
+    a.out1 = sb.work2;  // Ey' [u][y]
+    a.out2 = sb.work3;  // phi' [u][y]
     a.L = ny;
     a.nrows = nx;
-    a.tw = r.tw_y;
-    a.th = r.th_y;
-    a.dw = r.dw_y;
-    a.wrow = r.wx;
-    a.wcol = r.wy;
+    a.tw = sb.tw_y;
+    a.th = sb.th_y;
+    a.dw = sb.dw_y;
+    a.wrow = sb.wx;
+    a.wcol = sb.wy;
     a.want_phi = want_phi ? 1 : 0;
     launch_rows(s, ROW_POISSON, a);
-    transpose(s, r.work1, r.work0, nx, ny);  // Ex [y][u]
-    transpose(s, r.work2, r.work1, nx, ny);  // Ey [y][u]
-    if (want_phi) transpose(s, r.work3, r.work2, nx, ny);
+    transpose(s, sb.work1, sb.work0, nx, ny);  // Ex [y][u]
+    transpose(s, sb.work2, sb.work1, nx, ny);  // Ey [y][u]
+    if (want_phi) transpose(s, sb.work3, sb.work2, nx, ny);
     // P4
     a = RowArgs{};
-    a.in0 = r.work0;
-    a.in1 = r.work1;
+    a.in0 = sb.work0;
+    a.in1 = sb.work1;
     a.pairs = r.field;
     a.L = nx;
     a.nrows = ny;
-    a.tw = r.tw_x;
-    a.th = r.th_x;
-    a.dw = r.dw_x;
+    a.tw = sb.tw_x;
+    a.th = sb.th_x;
+    a.dw = sb.dw_x;
     launch_rows(s, ROW_INV_FIELDS, a);
     if (want_phi) {
         RowArgs p{};
-        p.in0 = r.work2;
+        p.in0 = sb.work2;
         p.out0 = r.phi;
         p.L = nx;
         p.nrows = ny;
-        p.tw = r.tw_x;
-        p.th = r.th_x;
-        p.dw = r.dw_x;
+        p.tw = sb.tw_x;
+        p.th = sb.th_x;
+        p.dw = sb.dw_x;
         launch_rows(s, ROW_INV_PHI, p);
     }
 }
@@ -2169,12 +2169,12 @@ void launch_lock_fix(hipStream_t s, const RegionDev& r, const CellsDev& c, float
                        cv[3], cv[4]);
 }
 
-void launch_objective_density(hipStream_t s, const RegionDev& r, const CellsDev& c, const float2* xy) {
+void launch_objective_density(hipStream_t s, const RegionDev& r, const SolveBufs& sb, const CellsDev& c, const float2* xy) {
     const int n = r.cell_end - r.cell_begin;
     const size_t M = (size_t)r.cnt_x * r.cnt_y;
     if (n > 0) hipLaunchKernelGGL(k_obj_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, r, c, xy, r.inst);
     hipLaunchKernelGGL(k_obj_rho, dim3(grid_for(M)), dim3(kBlock), 0, s, r, r.inst);
-    launch_solve(s, r, true);
+    launch_solve(s, r, sb, true);
 }
 
 void launch_objective_value(hipStream_t s, const RegionDev& r, const NetsDev& n, const float2* xy, float coef, double* partials,

@@ -43,6 +43,7 @@ struct Region {
     gpl_region R{};
     int32_t n = 0;
     gplk::RegionDev dev{};
+    gplk::SolveBufs sb{};
     float2* v[S_COUNT] = {};
     double* partials = nullptr;
     bool has_locked = false;
@@ -275,7 +276,7 @@ struct gpl_ctx {
 
     void solve(Region& r, bool want_phi = false) {
         mark(F_DCT);
-        gplk::launch_solve(stream, r.dev, want_phi);
+        gplk::launch_solve(stream, r.dev, r.sb, want_phi);
         mark_end();
     }
 
@@ -1520,21 +1521,21 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             d.density = c->dalloc<float>(M);
             d.field = c->dalloc<float2>(M);
             d.phi = c->dalloc<float>(M);
-            d.work0 = c->dalloc<float>(2 * M);  // the solve's (Ex', Ey') pairs
-            d.work1 = c->dalloc<float>(M);
-            d.work2 = c->dalloc<float>(M);
-            d.work3 = c->dalloc<float>(M);
-            d.wx = c->upload(r.R.wx, r.R.cnt_x);
-            d.wy = c->upload(r.R.wy, r.R.cnt_y);
+            r.sb.work0 = c->dalloc<float>(2 * M);  // the solve's (Ex', Ey') pairs
+            r.sb.work1 = c->dalloc<float>(M);
+            r.sb.work2 = c->dalloc<float>(M);
+            r.sb.work3 = c->dalloc<float>(M);
+            r.sb.wx = c->upload(r.R.wx, r.R.cnt_x);
+            r.sb.wy = c->upload(r.R.wy, r.R.cnt_y);
             std::vector<float2> tw, th, dw;
             make_tables(r.R.cnt_x, tw, th, dw);
-            d.tw_x = c->upload(tw.data(), tw.size());
-            d.th_x = c->upload(th.data(), th.size());
-            d.dw_x = c->upload(dw.data(), dw.size());
+            r.sb.tw_x = c->upload(tw.data(), tw.size());
+            r.sb.th_x = c->upload(th.data(), th.size());
+            r.sb.dw_x = c->upload(dw.data(), dw.size());
             make_tables(r.R.cnt_y, tw, th, dw);
-            d.tw_y = c->upload(tw.data(), tw.size());
-            d.th_y = c->upload(th.data(), th.size());
-            d.dw_y = c->upload(dw.data(), dw.size());
+            r.sb.tw_y = c->upload(tw.data(), tw.size());
+            r.sb.th_y = c->upload(th.data(), th.size());
+            r.sb.dw_y = c->upload(dw.data(), dw.size());
             for (int s = 0; s < S_COUNT; ++s) {
                 r.v[s] = c->dalloc<float2>(r.n);
                 check(hipMemsetAsync(r.v[s], 0, sizeof(float2) * std::max(1, r.n), c->stream), "memset");
@@ -2037,7 +2038,7 @@ static void objective_density(gpl_ctx* c, const float* xy) {
         c->obj_out = c->dalloc<double>(2);
     }
     c->upload_region_vec(0, c->obj_xy, xy);
-    gplk::launch_objective_density(c->stream, r.dev, c->cells, c->obj_xy);
+    gplk::launch_objective_density(c->stream, r.dev, r.sb, c->cells, c->obj_xy);
 }
 
 int32_t gpl_objective_forward(gpl_ctx* c, const float* xy, float wl_coef, float lambda, double* value, double* wa_part,
