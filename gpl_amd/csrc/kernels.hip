@@ -2291,20 +2291,42 @@ __global__ void k_net_deg(int E, NetsDev n, const int32_t* perm, int32_t* deg) {
 
 // new small net e (old perm[e]) copies its pins to their new places; each moved pin's cell-pin
 // slot learns the pin's new index
-__global__ void k_net_move(int E, NetsDev n, const int32_t* perm, const int32_t* orig, NetResortBufs b, int32_t* cp_npin) {
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
-        const int o = perm[e];
-        const int s = n.start[o], t = n.start[o + 1], d = b.start[e];
-        for (int q = 0; q < t - s; ++q) {
-            const int cp = n.cpi[s + q];
-            b.pcell[d + q] = n.pcell[s + q];
-            b.poff[d + q] = n.poff[s + q];
-            b.pxy0[d + q] = n.pxy0[s + q];
-            b.cpi[d + q] = cp;
-            if (cp >= 0) cp_npin[cp] = d + q;
-        }
+// The small nets' pins moved into the new order, pin-parallel: a block takes kBlock consecutive
+// new nets, stages their new and old pin starts in LDS, and copies their (contiguous) new pin
+// range with consecutive threads on consecutive pins; each thread finds its pin's net by a binary
+// search of the staged starts (the last net starting at or before the pin: empty nets share
+// their start with the next one).
+__global__ void __launch_bounds__(kBlock) k_net_move(int E, NetsDev n, const int32_t* perm, const int32_t* orig, NetResortBufs b,
+                                                      int32_t* cp_npin) {
+    __shared__ int s_dst[kBlock + 1];
+    __shared__ int s_src[kBlock];
+    const int e0 = (int)blockIdx.x * kBlock;
+    const int ne = min(kBlock, E - e0);
+    if (ne <= 0) return;  // (uniform per block)
+    if ((int)threadIdx.x < ne) {
+        const int e = e0 + (int)threadIdx.x, o = perm[e];
+        s_dst[threadIdx.x] = b.start[e];
+        s_src[threadIdx.x] = n.start[o];
         b.weight[e] = n.weight[o];
         b.orig[e] = orig[o];
+    }
+    if (threadIdx.x == 0) s_dst[ne] = b.start[e0 + ne];
+    __syncthreads();
+    const int d1 = s_dst[ne];
+    for (int d = s_dst[0] + (int)threadIdx.x; d < d1; d += kBlock) {
+        int lo = 0, hi = ne - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_dst[mid] <= d) lo = mid;
+            else hi = mid - 1;
+        }
+        const int src = s_src[lo] + (d - s_dst[lo]);
+        const int cp = n.cpi[src];
+        b.pcell[d] = n.pcell[src];
+        b.poff[d] = n.poff[src];
+        b.pxy0[d] = n.pxy0[src];
+        b.cpi[d] = cp;
+        if (cp >= 0) cp_npin[cp] = d;
     }
 }
 
@@ -2341,7 +2363,7 @@ void launch_net_resort(hipStream_t s, const NetsDev& n, int P, int num_cells, co
     hipLaunchKernelGGL(k_net_deg, dim3(grid_for(E)), dim3(kBlock), 0, s, E, n, b.perm, b.deg);
     bytes = b.temp_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(b.temp, bytes, b.deg, b.start, E + 1, s);
-    hipLaunchKernelGGL(k_net_move, dim3(grid_for(E)), dim3(kBlock), 0, s, E, n, b.perm, orig, b, c.cp_npin);
+    hipLaunchKernelGGL(k_net_move, dim3((E + kBlock - 1) / kBlock), dim3(kBlock), 0, s, E, n, b.perm, orig, b, c.cp_npin);
     hipLaunchKernelGGL(k_net_tail, dim3(grid_for(P - 0)), dim3(kBlock), 0, s, E, n.num_nets, P, n, orig, b);
 }
 
