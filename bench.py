@@ -331,8 +331,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: one placement runs on one MI355X; this build measures N=1 only "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        print("bench.py: one placement runs on one MI355X; only N=1 is measured "
+              "(placements spread across GPUs one per device; sharding one placement is not built)", file=sys.stderr)
         sys.exit(2)
 
     import gpl_amd as G
