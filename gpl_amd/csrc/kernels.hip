@@ -2244,15 +2244,40 @@ __global__ void k_resort_gather(int N, const int32_t* perm, CellsDev in, CellsDe
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[N] = 0;
 }
 
-__global__ void k_resort_pins(int N, const int32_t* perm, CellsDev in, CellsDev out, int32_t* cpi) {
-    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < N; g += gridDim.x * blockDim.x) {
-        const int o = perm[g];
-        const int s = in.pin_start[o], e = in.pin_start[o + 1], d = out.pin_start[g];
-        for (int q = 0; q < e - s; ++q) {
-            const int np = in.cp_npin[s + q];
-            out.cp_npin[d + q] = np;
-            cpi[np] = d + q;
-        }
+// The segment of a block's staged segment starts (nondecreasing, ne of them) that holds element
+// d: the last one starting at or before d (an empty segment shares its start with the next one).
+__device__ __forceinline__ int segment_of(const int* s_dst, int ne, int d) {
+    int lo = 0, hi = ne - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_dst[mid] <= d) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The cells' pin lists moved into the new cell order, pin-parallel: a block takes kBlock
+// consecutive new cells, stages their new and old pin starts in LDS and copies their contiguous
+// new pin range with consecutive threads on consecutive pins.
+__global__ void __launch_bounds__(kBlock) k_resort_pins(int N, const int32_t* perm, CellsDev in, CellsDev out, int32_t* cpi) {
+    __shared__ int s_dst[kBlock + 1];
+    __shared__ int s_src[kBlock];
+    const int g0 = (int)blockIdx.x * kBlock;
+    const int ne = min(kBlock, N - g0);
+    if (ne <= 0) return;  // (uniform per block)
+    if ((int)threadIdx.x < ne) {
+        const int g = g0 + (int)threadIdx.x;
+        s_dst[threadIdx.x] = out.pin_start[g];
+        s_src[threadIdx.x] = in.pin_start[perm[g]];
+    }
+    if (threadIdx.x == 0) s_dst[ne] = out.pin_start[g0 + ne];
+    __syncthreads();
+    const int d1 = s_dst[ne];
+    for (int d = s_dst[0] + (int)threadIdx.x; d < d1; d += kBlock) {
+        const int i = segment_of(s_dst, ne, d);
+        const int np = in.cp_npin[s_src[i] + (d - s_dst[i])];
+        out.cp_npin[d] = np;
+        cpi[np] = d;
     }
 }
 
@@ -2291,11 +2316,8 @@ __global__ void k_net_deg(int E, NetsDev n, const int32_t* perm, int32_t* deg) {
 
 // new small net e (old perm[e]) copies its pins to their new places; each moved pin's cell-pin
 // slot learns the pin's new index
-// The small nets' pins moved into the new order, pin-parallel: a block takes kBlock consecutive
-// new nets, stages their new and old pin starts in LDS, and copies their (contiguous) new pin
-// range with consecutive threads on consecutive pins; each thread finds its pin's net by a binary
-// search of the staged starts (the last net starting at or before the pin: empty nets share
-// their start with the next one).
+// The small nets' pins moved into the new order, pin-parallel as k_resort_pins: a block takes
+// kBlock consecutive new nets and copies their (contiguous) new pin range.
 __global__ void __launch_bounds__(kBlock) k_net_move(int E, NetsDev n, const int32_t* perm, const int32_t* orig, NetResortBufs b,
                                                       int32_t* cp_npin) {
     __shared__ int s_dst[kBlock + 1];
@@ -2314,12 +2336,7 @@ __global__ void __launch_bounds__(kBlock) k_net_move(int E, NetsDev n, const int
     __syncthreads();
     const int d1 = s_dst[ne];
     for (int d = s_dst[0] + (int)threadIdx.x; d < d1; d += kBlock) {
-        int lo = 0, hi = ne - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_dst[mid] <= d) lo = mid;
-            else hi = mid - 1;
-        }
+        const int lo = segment_of(s_dst, ne, d);
         const int src = s_src[lo] + (d - s_dst[lo]);
         const int cp = n.cpi[src];
         b.pcell[d] = n.pcell[src];
@@ -2404,7 +2421,7 @@ void launch_resort_cells(hipStream_t s, int N, const CellsDev& in, const CellsDe
     hipLaunchKernelGGL(k_resort_gather, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, in, out, b.cnt);
     size_t bytes = b.temp_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(b.temp, bytes, b.cnt, out.pin_start, N + 1, s);
-    hipLaunchKernelGGL(k_resort_pins, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, in, out, cpi);
+    hipLaunchKernelGGL(k_resort_pins, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, s, N, b.perm, in, out, cpi);
 }
 
 void launch_gather_f2(hipStream_t s, int n, int begin, const int32_t* perm, const float2* in, float2* out) {
