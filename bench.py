@@ -59,9 +59,9 @@ def algorithmic_bytes(fam, N, P, E, M):
                                   write next, next_slp (16), box (16), centre (8)
     scatter         21 N + 16 M   per cell: read box (16), flags (1), scale (4); the two int64 bin
                                   accumulators written once (16 per bin)
-    bin_density     52 M          read inst, filler, non_place, non_place_unscaled (32); write the
-                                  density (4); clear the accumulators (16)
-    dct_solve       12 M          read the density (4); write field_x, field_y (8)
+    bin_density     44 M          read inst, filler (16), the non-placeable areas as a float pair
+                                  (8); write the density (4); clear the accumulators (16)
+    dct_solve       12 M          read the density (4); write the field pairs (8)
     wa_forward      32 P + 8 E    per pin: cell (4), offset (8), the cell's centre (8), cell-pin
                                   index (4), its gradient term written (8); per net: start (4),
                                   weight (4)
@@ -72,7 +72,7 @@ def algorithmic_bytes(fam, N, P, E, M):
     return {
         "update": 81 * N,
         "scatter": 21 * N + 16 * M,
-        "bin_density": 52 * M,
+        "bin_density": 44 * M,
         "dct_solve": 12 * M,
         "wa_forward": 32 * P + 8 * E,
         "gradient": 64 * N + 8 * P + 8 * M,
