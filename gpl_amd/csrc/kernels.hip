@@ -472,9 +472,10 @@ __global__ void k_bin_density(RegionDev r, double* partials) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// The Poisson solve: 1-D cosine / sine transforms along rows, one 64-lane block per row, the
-// row in LDS; complex radix-2 FFT of the row's length L (a power of two), Makhoul's mapping
-// between the real transform and the FFT.  Conventions are Ooura's (ooura.rs:158-255):
+// The Poisson solve: 1-D cosine / sine transforms along rows, L/8 threads per row and the row in
+// LDS; an L/2-point complex Stockham FFT (radix 4, one radix-2 pass when log2(L/2) is odd) with
+// Makhoul's mapping between the real transform and the FFT.  Conventions are Ooura's
+// (ooura.rs:158-255):
 //   DCT-II   C[k] = sum_j a[j] cos(pi (j+1/2) k / L)
 //   DCT-III  C[k] = sum_j a[j] cos(pi j (k+1/2) / L)
 //   DST-III  S[k] = sum_{j=1..L} A[j] sin(pi j (k+1/2) / L), A[L] in a[0]
