@@ -219,7 +219,11 @@ __global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c
 //
 // A chunk first finds the box of bins its cells cover.  When that box holds at most kDense bins
 // (cells in Morton order: most chunks), the overlaps are added into a dense LDS tile indexed by
-// the bin's place in the box, one LDS atomic per overlap; otherwise into the hash table.
+// the bin's place in the box, one LDS atomic per overlap.  When it does not, but the box of the
+// chunk's middle half (its cells 128-383 in the storage order) does, the tile covers that box,
+// widened evenly while it fits, and the overlaps outside it go straight to the bins in global
+// memory: between two re-sorts the storage order decays as cells move, and a few far cells
+// would otherwise send the whole chunk to the hash table.  Otherwise the hash table.
 template <int MODE, typename V>
 __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev c, PlaceArgs pa) {
     // the trial's critical chain (scatter -> bins -> solve) shares the CUs with the WA kernels on
@@ -229,7 +233,7 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
     constexpr int kHashBytes = kSlots * (int)(sizeof(unsigned) + 2 * sizeof(V));
     constexpr int kDenseBytes = 2 * kDense * (int)sizeof(V);
     __shared__ __align__(16) unsigned char lds[kHashBytes > kDenseBytes ? kHashBytes : kDenseBytes];
-    __shared__ int s_box[4][kBlock / 64];
+    __shared__ int s_box[8][kBlock / 64];
     __shared__ unsigned s_adds;
     unsigned* t_key = reinterpret_cast<unsigned*>(lds);
     V* t_inst = reinterpret_cast<V*>(lds + kSlots * sizeof(unsigned));
@@ -252,6 +256,7 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
         float scv[kCellsPerThread];
         uint8_t fl[kCellsPerThread];
         int bx0 = INT_MAX, by0 = INT_MAX, bx1 = INT_MIN, by1 = INT_MIN;
+        int cx0 = INT_MAX, cy0 = INT_MAX, cx1 = INT_MIN, cy1 = INT_MIN;  // the chunk's middle half
 #pragma unroll
         for (int j = 0; j < kCellsPerThread; ++j) {
             const int k = base + j * kBlock + threadIdx.x;
@@ -294,6 +299,13 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
                 by0 = min(by0, y0[j]);
                 bx1 = max(bx1, x1[j]);
                 by1 = max(by1, y1[j]);
+                const int q = j * kBlock + (int)threadIdx.x;
+                if (q >= kChunk / 4 && q < 3 * kChunk / 4) {
+                    cx0 = min(cx0, x0[j]);
+                    cy0 = min(cy0, y0[j]);
+                    cx1 = max(cx1, x1[j]);
+                    cy1 = max(cy1, y1[j]);
+                }
             }
         }
         // the chunk's box of bins
@@ -303,12 +315,20 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
             by0 = min(by0, __shfl_xor(by0, o, 64));
             bx1 = max(bx1, __shfl_xor(bx1, o, 64));
             by1 = max(by1, __shfl_xor(by1, o, 64));
+            cx0 = min(cx0, __shfl_xor(cx0, o, 64));
+            cy0 = min(cy0, __shfl_xor(cy0, o, 64));
+            cx1 = max(cx1, __shfl_xor(cx1, o, 64));
+            cy1 = max(cy1, __shfl_xor(cy1, o, 64));
         }
         if (lane == 0) {
             s_box[0][w] = bx0;
             s_box[1][w] = by0;
             s_box[2][w] = bx1;
             s_box[3][w] = by1;
+            s_box[4][w] = cx0;
+            s_box[5][w] = cy0;
+            s_box[6][w] = cx1;
+            s_box[7][w] = cy1;
         }
         __syncthreads();
 #pragma unroll
@@ -317,9 +337,26 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
             by0 = min(by0, s_box[1][i]);
             bx1 = max(bx1, s_box[2][i]);
             by1 = max(by1, s_box[3][i]);
+            cx0 = min(cx0, s_box[4][i]);
+            cy0 = min(cy0, s_box[5][i]);
+            cx1 = max(cx1, s_box[6][i]);
+            cy1 = max(cy1, s_box[7][i]);
         }
-        const int BW = bx1 - bx0, BH = by1 - by0;
-        const bool dense = BW > 0 && BH > 0 && BW * BH <= kDense;
+        int BW = bx1 - bx0, BH = by1 - by0;
+        bool dense = BW > 0 && BH > 0 && BW * BH <= kDense;
+        if (!dense && cx1 > cx0 && cy1 > cy0 && (cx1 - cx0) * (cy1 - cy0) <= kDense) {
+            // the tile over the middle half's box, widened evenly within the chunk's box while it
+            // fits; overlaps outside it go straight to the bins in global memory
+            int m = 0;
+            while (m < 64 && (cx1 - cx0 + 2 * (m + 1)) * (cy1 - cy0 + 2 * (m + 1)) <= kDense) ++m;
+            bx0 = max(bx0, cx0 - m);
+            by0 = max(by0, cy0 - m);
+            bx1 = min(bx1, cx1 + m);
+            by1 = min(by1, cy1 + m);
+            BW = bx1 - bx0;
+            BH = by1 - by0;
+            dense = true;
+        }
         const int nd = dense ? BW * BH : 0;
         if (dense) {
             for (int i = threadIdx.x; i < nd; i += kBlock) {
@@ -352,7 +389,13 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
                 const long long v = sizeof(V) == 4 ? (long long)(unsigned)a : (long long)a;
                 if (v == 0) return;
                 if (dense) {
-                    atomicAdd(&vals[(y - by0) * BW + (x - bx0)], (V)v);
+                    const unsigned ux = (unsigned)(x - bx0), uy = (unsigned)(y - by0);
+                    if (ux < (unsigned)BW && uy < (unsigned)BH) {
+                        atomicAdd(&vals[uy * (unsigned)BW + ux], (V)v);
+                    } else {
+                        atomicAdd(glob + ((unsigned)y * (unsigned)r.cnt_x + (unsigned)x), (unsigned long long)v);
+                        ++adds;
+                    }
                     return;
                 }
                 const unsigned kb = (unsigned)y * (unsigned)r.cnt_x + (unsigned)x;
