@@ -15,6 +15,7 @@ f32p = C.POINTER(C.c_float)
 f64p = C.POINTER(C.c_double)
 i32p = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
+u8p = C.POINTER(C.c_uint8)
 
 _lib = None
 
@@ -73,6 +74,15 @@ def lib():
             ("orc_objective_value", None, [C.c_void_p, f32p, C.c_float, f64p, f64p]),
             ("orc_objective_fd", None, [C.c_void_p, f32p, C.c_float, C.c_float, C.c_int32, i32p, C.c_double, f64p]),
             ("orc_solve_phi_f64", None, [C.c_int32, C.c_int32, f32p, f32p, f64p, f64p]),
+            ("orc_inside", C.c_float, [C.c_float, C.c_int32, C.c_int32, C.c_int32]),
+            ("orc_nesterov_points", None, [C.c_int32, f32p, f32p, f32p, u8p, i32p, i32p, C.c_float, C.c_float, f32p, f32p]),
+            ("orc_step_length", C.c_int32, [C.c_int32, f32p, f32p, f32p, f32p, f32p, f32p, i32p]),
+            ("orc_precondition", None, [C.c_int32, f32p, f32p, i32p, i32p, C.c_float, f32p, f32p, f32p]),
+            ("orc_init_penalty", C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
+            ("orc_init_step", C.c_float, [C.c_int32, f32p, f32p, f32p, f32p, f32p]),
+            ("orc_check_divergence", C.c_int32, [C.c_float, C.c_float, C.c_float, C.c_int64, C.c_float, C.c_int64]),
+            ("orc_momentum", C.c_float, [C.c_float, f32p]),
+            ("orc_set_inject_control", None, [C.c_void_p, C.c_int32, C.c_int32]),
         ]
         for name, res, args in sig:
             fn = getattr(L, name)
@@ -221,6 +231,11 @@ class Oracle:
     def set_inject(self, after_iter):
         lib().orc_set_inject(self._h, after_iter)
 
+    def set_inject_control(self, kind, at):
+        """Kind 5: the WA gradient sum read as zero once at iteration `at`; kind 6: a divergence
+        reported at iteration `at` (twice for two iterations); kind 0 clears them."""
+        lib().orc_set_inject_control(self._h, kind, at)
+
     def close(self):
         if getattr(self, "_h", None):
             lib().orc_destroy(self._h)
@@ -276,3 +291,68 @@ def solve_phi_f64(nx, ny, wx, wy, density):
     phi = np.zeros(nx * ny, np.float64)
     lib().orc_solve_phi_f64(nx, ny, f32(np.ascontiguousarray(wx, np.float32)), f32(np.ascontiguousarray(wy, np.float32)), f64(d), f64(phi))
     return phi
+
+
+# ---- the loop control's arithmetic on plain arrays (pinned by tests/test_oracle_control.py) ----
+
+def _f2(a):
+    return np.ascontiguousarray(a, np.float32).reshape(-1)
+
+
+def inside(c, dd, lo, hi):
+    return lib().orc_inside(np.float32(c), int(dd), int(lo), int(hi))
+
+
+def nesterov_points(cur_slp, sum_, cur, locked, dd, box, step, coeff):
+    """next, next_slp of nesterovUpdateCoordinates (n x 2 float32 each)."""
+    cs, sm, cu = _f2(cur_slp), _f2(sum_), _f2(cur)
+    n = len(cs) // 2
+    lk = np.ascontiguousarray(locked, np.uint8)
+    d = np.ascontiguousarray(dd, np.int32).reshape(-1)
+    b = np.ascontiguousarray(box, np.int32)
+    nx, ns = np.zeros(2 * n, np.float32), np.zeros(2 * n, np.float32)
+    lib().orc_nesterov_points(n, f32(cs), f32(sm), f32(cu), lk.ctypes.data_as(u8p), i32(d), i32(b), np.float32(step),
+                              np.float32(coeff), f32(nx), f32(ns))
+    return nx.reshape(-1, 2), ns.reshape(-1, 2)
+
+
+def step_length(cur_slp, next_slp, cur_sum, next_sum, old_step):
+    """(accepted, new step, coordinate distance, diverged) of nesterovUpdateStepLength."""
+    a, b, c, d = _f2(cur_slp), _f2(next_slp), _f2(cur_sum), _f2(next_sum)
+    st, cd, dv = C.c_float(old_step), C.c_float(0.0), C.c_int32(0)
+    ok = lib().orc_step_length(len(a) // 2, f32(a), f32(b), f32(c), f32(d), C.byref(st), C.byref(cd), C.byref(dv))
+    return bool(ok), st.value, cd.value, bool(dv.value)
+
+
+def precondition(wl, den, npins, dxy, penalty):
+    """(sum n x 2, |wl| sum, |den| sum) of updateGradients' arithmetic."""
+    w, d = _f2(wl), _f2(den)
+    n = len(w) // 2
+    s = np.zeros(2 * n, np.float32)
+    wa, da = C.c_float(0.0), C.c_float(0.0)
+    lib().orc_precondition(n, f32(w), f32(d), i32(np.ascontiguousarray(npins, np.int32)),
+                           i32(np.ascontiguousarray(dxy, np.int32).reshape(-1)), np.float32(penalty), f32(s), C.byref(wa), C.byref(da))
+    return s.reshape(-1, 2), wa.value, da.value
+
+
+def init_penalty(wl_sum, den_sum, cur, init):
+    return lib().orc_init_penalty(np.float32(wl_sum), np.float32(den_sum), np.float32(cur), np.float32(init))
+
+
+def init_step(prev_slp, cur_slp, prev_sum, cur_sum):
+    a, b, c, d = _f2(prev_slp), _f2(cur_slp), _f2(prev_sum), _f2(cur_sum)
+    cd = C.c_float(0.0)
+    s = lib().orc_init_step(len(a) // 2, f32(a), f32(b), f32(c), f32(d), C.byref(cd))
+    return s, cd.value
+
+
+def check_divergence(ovf, min_ovf, hpwl_at_min, prev_hpwl, prev_reported_ovf, prev_reported_hpwl):
+    return lib().orc_check_divergence(np.float32(ovf), np.float32(min_ovf), np.float32(hpwl_at_min), int(prev_hpwl),
+                                      np.float32(prev_reported_ovf), int(prev_reported_hpwl))
+
+
+def momentum(prev_a):
+    """(a', coeff) of the loop's momentum step."""
+    c = C.c_float(0.0)
+    a = lib().orc_momentum(np.float32(prev_a), C.byref(c))
+    return a, c.value

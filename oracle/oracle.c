@@ -212,6 +212,7 @@ typedef struct orc_np {
     int32_t ncells, npins, nnets, nregions;
     /* cells */
     int32_t *lx, *ly, *ux, *uy, *dlx, *dly, *dux, *duy, *pin_start, *pins;
+    int32_t *cnpins, *dxy; /* per cell: its pin count; its box's dx, dy (the preconditioner's inputs) */
     float* dscale;
     uint8_t* flags;
     /* pins */
@@ -242,6 +243,11 @@ typedef struct orc_np {
     gpl_iter_row* hist;
     int32_t nhist, cap_hist;
     int32_t inject_nan_after;
+    /* host-forced cases of the loop control (the GPU's gpl_debug_inject kinds 5 and 6): the WA
+     * gradient sum read as zero once at iteration inject_zero_wl_at (the halving recursion), and a
+     * divergence reported at iterations inject_diverge_at / inject_diverge_at2 (the min-HPWL
+     * snapshot revert); -1: off */
+    int32_t inject_zero_wl_at, inject_diverge_at, inject_diverge_at2, iter_now;
 } orc_np;
 
 static void* xcalloc(size_t n, size_t s) {
@@ -285,6 +291,13 @@ orc_np* orc_create(const gpl_problem* p) {
         np->flags[i] = p->cell_flags[i];
     }
     memcpy(np->pin_start, p->cell_pin_start, sizeof(int32_t) * (N + 1));
+    np->cnpins = (int32_t*)xcalloc(N, 4);
+    np->dxy = (int32_t*)xcalloc(2 * (size_t)N, 4);
+    for (int32_t i = 0; i < N; ++i) {
+        np->cnpins[i] = p->cell_pin_start[i + 1] - p->cell_pin_start[i];
+        np->dxy[2 * i] = np->ux[i] - np->lx[i];
+        np->dxy[2 * i + 1] = np->uy[i] - np->ly[i];
+    }
     memcpy(np->pins, p->cell_pins, sizeof(int32_t) * p->cell_pin_start[N]);
     np->pcx = (int32_t*)xcalloc(P, 4);
     np->pcy = (int32_t*)xcalloc(P, 4);
@@ -349,6 +362,10 @@ orc_np* orc_create(const gpl_problem* p) {
     np->max_iter = np->o.max_iter;
     np->min_hpwl = INT64_MAX;
     np->inject_nan_after = -1;
+    np->inject_zero_wl_at = -1;
+    np->inject_diverge_at = -1;
+    np->inject_diverge_at2 = -1;
+    np->iter_now = -1;
     return np;
 }
 
@@ -373,7 +390,7 @@ void orc_destroy(orc_np* np) {
     }
     free(np->nbs);
     int32_t* ia[] = {np->lx, np->ly, np->ux, np->uy, np->dlx, np->dly, np->dux, np->duy, np->pin_start, np->pins, np->pcx, np->pcy,
-                     np->poff_x, np->poff_y, np->pcell, np->pnet, np->net_start, np->net_pins, np->nbox};
+                     np->poff_x, np->poff_y, np->pcell, np->pnet, np->net_start, np->net_pins, np->nbox, np->cnpins, np->dxy};
     for (size_t i = 0; i < sizeof ia / sizeof ia[0]; ++i) free(ia[i]);
     free(np->dscale);
     free(np->flags);
@@ -421,19 +438,17 @@ static void set_density_location(orc_np* np, int g, int lx, int ly) {
     }
 }
 
-/* nesterov.rs:2188 inside_x / 2202 inside_y. */
-static float inside_x(const orc_np* np, const orc_nb* nb, int g, float cx) {
-    float half = (float)(np->dux[g] - np->dlx[g]) / 2.0f, adj = cx;
-    if (cx - half < (float)nb->R.lx) adj = (float)nb->R.lx + half;
-    if (cx + half > (float)nb->R.ux) adj = (float)nb->R.ux - half;
+/* nesterov.rs:2188-2213 getDensityCoordiLayoutInside{X,Y}: a centre c of a density box of integer
+ * width `dd` kept inside [lo, hi] (the upper rule is applied last, so a box wider than the grid
+ * ends at hi).  Pinned by hand-computed cases (tests/test_oracle_control.py). */
+float orc_inside(float c, int32_t dd, int32_t lo, int32_t hi) {
+    float half = (float)dd / 2.0f, adj = c;
+    if (c - half < (float)lo) adj = (float)lo + half;
+    if (c + half > (float)hi) adj = (float)hi - half;
     return adj;
 }
-static float inside_y(const orc_np* np, const orc_nb* nb, int g, float cy) {
-    float half = (float)(np->duy[g] - np->dly[g]) / 2.0f, adj = cy;
-    if (cy - half < (float)nb->R.ly) adj = (float)nb->R.ly + half;
-    if (cy + half > (float)nb->R.uy) adj = (float)nb->R.uy - half;
-    return adj;
-}
+static float inside_x(const orc_np* np, const orc_nb* nb, int g, float cx) { return orc_inside(cx, np->dux[g] - np->dlx[g], nb->R.lx, nb->R.ux); }
+static float inside_y(const orc_np* np, const orc_nb* nb, int g, float cy) { return orc_inside(cy, np->duy[g] - np->dly[g], nb->R.ly, nb->R.uy); }
 
 static void didx_x(const orc_nb* nb, int32_t lo, int32_t hi, int* x0, int* x1) {
     int a = (int)((double)(lo - nb->R.lx) / nb->R.size_x);
@@ -653,6 +668,35 @@ static fpt density_gradient(const orc_np* np, const orc_nb* nb, int g) {
 
 enum { SLOT_PREV, SLOT_CUR, SLOT_NEXT };
 
+/* nesterov.rs:2469-2489, updateGradients' arithmetic on given gradients: per cell the
+ * preconditioned sum (wl + p den) / max(pins + p (dx dy), 1) -- the floor taken as Rust's
+ * f32::max, which returns 1 for a NaN preconditioner -- and the float sums of |wl| and |den| in
+ * cell order, x before y.  wl, den, sum: n points (x, y interleaved); dxy: 2 ints per cell (the
+ * cell box's dx, dy).  Pinned by hand-computed cases and the convex-quadratic run
+ * (tests/test_oracle_control.py). */
+void orc_precondition(int32_t n, const float* wl, const float* den, const int32_t* npins, const int32_t* dxy, float p, float* sum,
+                      float* wl_abs, float* den_abs) {
+    for (int32_t i = 0; i < n; ++i) {
+        float sx = wl[2 * i] + p * den[2 * i];
+        float sy = wl[2 * i + 1] + p * den[2 * i + 1];
+        float wl_pre = (float)npins[i];
+        float area = (float)dxy[2 * i] * (float)dxy[2 * i + 1];
+        float pre = wl_pre + p * area;
+        if (!(pre >= 1.0f)) pre = 1.0f;
+        sum[2 * i] = sx / pre;
+        sum[2 * i + 1] = sy / pre;
+    }
+    float a = 0.0f, d = 0.0f;
+    for (int32_t i = 0; i < n; ++i) {
+        a += fabsf(wl[2 * i]);
+        a += fabsf(wl[2 * i + 1]);
+        d += fabsf(den[2 * i]);
+        d += fabsf(den[2 * i + 1]);
+    }
+    *wl_abs = a;
+    *den_abs = d;
+}
+
 /* nesterov.rs:2455 updateGradients into a slot. */
 static void update_gradients(orc_np* np, orc_nb* nb, int slot) {
     if (nb->is_converged) return;
@@ -660,26 +704,8 @@ static void update_gradients(orc_np* np, orc_nb* nb, int slot) {
     fpt *wl = nb->v[base + 1], *den = nb->v[base + 2], *sum = nb->v[base + 3];
     for (int i = 0; i < nb->n; ++i) wl[i] = wl_gradient(np, nb->begin + i);
     for (int i = 0; i < nb->n; ++i) den[i] = density_gradient(np, nb, nb->begin + i);
-    float p = nb->density_penalty;
-    for (int i = 0; i < nb->n; ++i) {
-        int g = nb->begin + i;
-        sum[i].x = wl[i].x + p * den[i].x;
-        sum[i].y = wl[i].y + p * den[i].y;
-        float wl_pre = (float)(np->pin_start[g + 1] - np->pin_start[g]);
-        float area = (float)(np->ux[g] - np->lx[g]) * (float)(np->uy[g] - np->ly[g]);
-        float pre = wl_pre + p * area;
-        if (!(pre >= 1.0f)) pre = pre > 1.0f ? pre : 1.0f;
-        sum[i].x /= pre;
-        sum[i].y /= pre;
-    }
-    nb->wl_grad_sum = 0.0f;
-    nb->density_grad_sum = 0.0f;
-    for (int i = 0; i < nb->n; ++i) {
-        nb->wl_grad_sum += fabsf(wl[i].x);
-        nb->wl_grad_sum += fabsf(wl[i].y);
-        nb->density_grad_sum += fabsf(den[i].x);
-        nb->density_grad_sum += fabsf(den[i].y);
-    }
+    orc_precondition(nb->n, (const float*)wl, (const float*)den, np->cnpins + nb->begin, np->dxy + 2 * (size_t)nb->begin, nb->density_penalty,
+                     (float*)sum, &nb->wl_grad_sum, &nb->density_grad_sum);
 }
 
 static int bad_f(float v) { return isnan(v) || isinf(v); }
@@ -689,6 +715,10 @@ static void np_update_gradient(orc_np* np, int b, int slot) {
     orc_nb* nb = &np->nbs[b];
     update_gradients(np, nb, slot);
     if (np->inject_nan_after >= 0 && np->iter_done >= np->inject_nan_after && slot == SLOT_NEXT) nb->wl_grad_sum = NAN;
+    if (np->inject_zero_wl_at >= 0 && np->iter_now == np->inject_zero_wl_at && slot == SLOT_NEXT) {
+        nb->wl_grad_sum = 0.0f;
+        np->inject_zero_wl_at = -1;
+    }
     if (nb->wl_grad_sum == 0.0f && np->recursion_wl < 10) {
         np->wl_coef_x = (float)((double)np->wl_coef_x * 0.5);
         np->wl_coef_y = (float)((double)np->wl_coef_y * 0.5);
@@ -754,6 +784,22 @@ static void update_initial_prev_slp(orc_np* np, int b) {
     }
 }
 
+/* nesterov.rs:2608-2611 initDensity2's first penalty: (WA sum / density sum) x the initial factor;
+ * with a zero WA sum the penalty is left as it is (`cur`).  Pinned by hand-computed cases
+ * (tests/test_oracle_control.py). */
+float orc_init_penalty(float wl_grad_sum, float density_grad_sum, float cur, float init_density_penalty) {
+    if (wl_grad_sum != 0.0f) return (wl_grad_sum / density_grad_sum) * init_density_penalty;
+    return cur;
+}
+
+/* nesterov.rs:2615-2617 initDensity2's first step: the RMS distance of the two points over that of
+ * their preconditioned gradients.  Pinned as orc_step_length. */
+float orc_init_step(int32_t n, const float* prev_slp, const float* cur_slp, const float* prev_sum, const float* cur_sum, float* coordi_distance) {
+    *coordi_distance = orc_distance(prev_slp, cur_slp, n);
+    float grad = orc_distance(prev_sum, cur_sum, n);
+    return *coordi_distance / grad;
+}
+
 /* nesterov.rs:2600 initDensity2. */
 static float init_density2(orc_np* np, int b) {
     orc_nb* nb = &np->nbs[b];
@@ -761,13 +807,12 @@ static float init_density2(orc_np* np, int b) {
         nb->density_penalty = np->o.init_density_penalty;
         update_gradients(np, nb, SLOT_PREV);
     }
-    if (nb->wl_grad_sum != 0.0f) nb->density_penalty = (nb->wl_grad_sum / nb->density_grad_sum) * np->o.init_density_penalty;
+    nb->density_penalty = orc_init_penalty(nb->wl_grad_sum, nb->density_grad_sum, nb->density_penalty, np->o.init_density_penalty);
     float nia = (float)nesterov_insts_area(nb);
     nb->sum_overflow = (float)nb->overflow / nia;
     nb->sum_overflow_unscaled = (float)nb->overflow_unscaled / nia;
-    nb->coordi_distance = orc_distance((const float*)nb->v[S_PREV_SLP], (const float*)nb->v[S_CUR_SLP], nb->n);
-    float grad = orc_distance((const float*)nb->v[S_PREV_SLP_SUM], (const float*)nb->v[S_CUR_SLP_SUM], nb->n);
-    nb->step_length = nb->coordi_distance / grad;
+    nb->step_length = orc_init_step(nb->n, (const float*)nb->v[S_PREV_SLP], (const float*)nb->v[S_CUR_SLP], (const float*)nb->v[S_PREV_SLP_SUM],
+                                    (const float*)nb->v[S_CUR_SLP_SUM], &nb->coordi_distance);
     return nb->step_length;
 }
 
@@ -819,53 +864,86 @@ int32_t orc_init(orc_np* np) {
     return rc;
 }
 
+/* nesterov.rs:2718-2746 nesterovUpdateCoordinates' per-cell arithmetic: next = slp + s sum and
+ * next_slp = next + coeff (next - cur), each kept inside the grid box (lx, ly, ux, uy) by the
+ * cell's density box size dd (2 ints per cell); a cell with a nonzero `locked` entry keeps cur and
+ * cur_slp.  Points are n (x, y) pairs.  Pinned by hand-computed cases and the convex-quadratic
+ * run (tests/test_oracle_control.py). */
+void orc_nesterov_points(int32_t n, const float* cur_slp, const float* sum, const float* cur, const uint8_t* locked, const int32_t* dd,
+                         const int32_t* box, float s, float coeff, float* next, float* next_slp) {
+    for (int32_t k = 0; k < n; ++k) {
+        if (locked[k]) {
+            next[2 * k] = cur[2 * k];
+            next[2 * k + 1] = cur[2 * k + 1];
+            next_slp[2 * k] = cur_slp[2 * k];
+            next_slp[2 * k + 1] = cur_slp[2 * k + 1];
+            continue;
+        }
+        float nx = cur_slp[2 * k] + s * sum[2 * k];
+        float ny = cur_slp[2 * k + 1] + s * sum[2 * k + 1];
+        float sx = nx + coeff * (nx - cur[2 * k]);
+        float sy = ny + coeff * (ny - cur[2 * k + 1]);
+        next[2 * k] = orc_inside(nx, dd[2 * k], box[0], box[2]);
+        next[2 * k + 1] = orc_inside(ny, dd[2 * k + 1], box[1], box[3]);
+        next_slp[2 * k] = orc_inside(sx, dd[2 * k], box[0], box[2]);
+        next_slp[2 * k + 1] = orc_inside(sy, dd[2 * k + 1], box[1], box[3]);
+    }
+}
+
 /* nesterov.rs:2718 nesterovUpdateCoordinates. */
 static void nesterov_update_coordinates(orc_np* np, int b, float coeff) {
     orc_nb* nb = &np->nbs[b];
     if (nb->is_converged) return;
-    float s = nb->step_length;
-    fpt *cur_slp = nb->v[S_CUR_SLP], *sum = nb->v[S_CUR_SLP_SUM], *cur = nb->v[S_CUR];
-    fpt *next = nb->v[S_NEXT], *next_slp = nb->v[S_NEXT_SLP];
+    uint8_t* locked = (uint8_t*)xcalloc(nb->n, 1);
+    int32_t* dd = (int32_t*)xcalloc(2 * (size_t)nb->n, 4);
     for (int k = 0; k < nb->n; ++k) {
         int g = nb->begin + k;
-        if (np->flags[g] & GPL_CELL_LOCKED) {
-            next[k] = cur[k];
-            next_slp[k] = cur_slp[k];
-            continue;
-        }
-        float nx = cur_slp[k].x + s * sum[k].x;
-        float ny = cur_slp[k].y + s * sum[k].y;
-        float sx = nx + coeff * (nx - cur[k].x);
-        float sy = ny + coeff * (ny - cur[k].y);
-        next[k].x = inside_x(np, nb, g, nx);
-        next[k].y = inside_y(np, nb, g, ny);
-        next_slp[k].x = inside_x(np, nb, g, sx);
-        next_slp[k].y = inside_y(np, nb, g, sy);
+        locked[k] = (np->flags[g] & GPL_CELL_LOCKED) ? 1 : 0;
+        dd[2 * k] = np->dux[g] - np->dlx[g];
+        dd[2 * k + 1] = np->duy[g] - np->dly[g];
     }
-    update_center_location(np, nb, next_slp);
+    const int32_t box[4] = {nb->R.lx, nb->R.ly, nb->R.ux, nb->R.uy};
+    orc_nesterov_points(nb->n, (const float*)nb->v[S_CUR_SLP], (const float*)nb->v[S_CUR_SLP_SUM], (const float*)nb->v[S_CUR], locked, dd,
+                        box, nb->step_length, coeff, (float*)nb->v[S_NEXT], (float*)nb->v[S_NEXT_SLP]);
+    free(locked);
+    free(dd);
+    update_center_location(np, nb, nb->v[S_NEXT_SLP]);
     update_density_field_bin(nb);
+}
+
+/* nesterov.rs:2750-2772 nesterovUpdateStepLength on given points: the new step is the RMS distance
+ * of the look-ahead points over that of their preconditioned gradients (getDistance, 319); a NaN
+ * or infinite one sets *diverged and is refused; one above 0.95 of the old (compared in double) is
+ * taken but not accepted; one under 0.01 becomes 0.01, not accepted; else accepted.  Returns 1
+ * when accepted; *step is the old step in and the new out.  Pinned by hand-computed cases, the
+ * isotropic-quadratic closed form (the step equals the preconditioner over the curvature) and the
+ * convex-quadratic run (tests/test_oracle_control.py). */
+int32_t orc_step_length(int32_t n, const float* cur_slp, const float* next_slp, const float* cur_sum, const float* next_sum, float* step,
+                        float* coordi_distance, int32_t* diverged) {
+    *coordi_distance = orc_distance(cur_slp, next_slp, n);
+    float grad = orc_distance(cur_sum, next_sum, n);
+    float nw = *coordi_distance / grad;
+    if (bad_f(nw)) {
+        *diverged = 1;
+        return 0;
+    }
+    if ((double)nw > (double)*step * 0.95) {
+        *step = nw;
+        return 0;
+    }
+    if ((double)nw < 0.01) {
+        *step = (float)0.01;
+        return 0;
+    }
+    *step = nw;
+    return 1;
 }
 
 /* nesterov.rs:2750 nesterovUpdateStepLength. */
 static int nesterov_update_step_length(orc_np* np, orc_nb* nb) {
     if (nb->is_converged) return 1;
-    nb->coordi_distance = orc_distance((const float*)nb->v[S_CUR_SLP], (const float*)nb->v[S_NEXT_SLP], nb->n);
-    float grad = orc_distance((const float*)nb->v[S_CUR_SLP_SUM], (const float*)nb->v[S_NEXT_SLP_SUM], nb->n);
-    float nw = nb->coordi_distance / grad;
-    if (bad_f(nw)) {
-        nb->is_diverged = 1;
-        return 0;
-    }
-    if ((double)nw > (double)nb->step_length * 0.95) {
-        nb->step_length = nw;
-        return 0;
-    }
-    if ((double)nw < 0.01) {
-        nb->step_length = (float)0.01;
-        return 0;
-    }
-    nb->step_length = nw;
-    return 1;
+    return orc_step_length(nb->n, (const float*)nb->v[S_CUR_SLP], (const float*)nb->v[S_NEXT_SLP], (const float*)nb->v[S_CUR_SLP_SUM],
+                           (const float*)nb->v[S_NEXT_SLP_SUM], &nb->step_length, &nb->coordi_distance, &nb->is_diverged);
 }
 
 /* nesterov.rs:2776 doBackTracking. */
@@ -1000,16 +1078,31 @@ static void revert_to_snapshot(orc_np* np, orc_nb* nb) {
     nb->prev_reported_overflow_unscaled = 0.0f;
 }
 
-/* nesterov.rs:2976 checkDivergence. */
-static int check_divergence(orc_nb* nb) {
-    if (nb->sum_overflow_unscaled < 0.2f && nb->sum_overflow_unscaled - nb->min_sum_overflow >= 0.02f &&
-        nb->hpwl_with_min_sum_overflow * 1.2f < (float)nb->prev_hpwl)
-        nb->is_diverged = 1;
-    if (nb->prev_reported_overflow_unscaled > 0.0f && nb->prev_reported_hpwl > 0) {
-        float change = nb->sum_overflow_unscaled - nb->prev_reported_overflow_unscaled;
-        float increase = (float)(nb->prev_hpwl - nb->prev_reported_hpwl) / (float)nb->prev_reported_hpwl;
-        if (change >= 0.05f && increase >= 0.25f) nb->is_diverged = 1;
+/* nesterov.rs:2976-2991 checkDivergence's two rules: bit 0 (GPL-0323) the overflow is under 0.2,
+ * at least 0.02 above its minimum, and the HPWL more than 1.2x the HPWL at that minimum; bit 1
+ * (GPL-0324) since the last progress row the overflow rose by at least 0.05 and the HPWL by at
+ * least 25 %.  Pinned by hand-computed cases on both sides of every bound
+ * (tests/test_oracle_control.py). */
+int32_t orc_check_divergence(float sum_overflow_unscaled, float min_sum_overflow, float hpwl_with_min_sum_overflow, int64_t prev_hpwl,
+                             float prev_reported_overflow_unscaled, int64_t prev_reported_hpwl) {
+    int32_t r = 0;
+    if (sum_overflow_unscaled < 0.2f && sum_overflow_unscaled - min_sum_overflow >= 0.02f && hpwl_with_min_sum_overflow * 1.2f < (float)prev_hpwl)
+        r |= 1;
+    if (prev_reported_overflow_unscaled > 0.0f && prev_reported_hpwl > 0) {
+        float change = sum_overflow_unscaled - prev_reported_overflow_unscaled;
+        float increase = (float)(prev_hpwl - prev_reported_hpwl) / (float)prev_reported_hpwl;
+        if (change >= 0.05f && increase >= 0.25f) r |= 2;
     }
+    return r;
+}
+
+/* nesterov.rs:2976 checkDivergence. */
+static int check_divergence(orc_np* np, orc_nb* nb) {
+    if (orc_check_divergence(nb->sum_overflow_unscaled, nb->min_sum_overflow, nb->hpwl_with_min_sum_overflow, nb->prev_hpwl,
+                             nb->prev_reported_overflow_unscaled, nb->prev_reported_hpwl))
+        nb->is_diverged = 1;
+    if (np->inject_diverge_at >= 0 && np->iter_now == np->inject_diverge_at) nb->is_diverged = 1;
+    if (np->inject_diverge_at2 >= 0 && np->iter_now == np->inject_diverge_at2) nb->is_diverged = 1;
     return nb->is_diverged;
 }
 
@@ -1027,7 +1120,7 @@ static void revert_to_diverge_snapshot(orc_np* np) {
 /* nesterov.rs:3009 isDiverged (routability off): 0 none, 1 resume, 2 stop. */
 static int np_is_diverged(orc_np* np, float* cur_a) {
     np->num_region_diverged = 0;
-    for (int b = 0; b < np->nregions; ++b) np->num_region_diverged += check_divergence(&np->nbs[b]);
+    for (int b = 0; b < np->nregions; ++b) np->num_region_diverged += check_divergence(np, &np->nbs[b]);
     if (np->num_region_diverged == 0 && np->is_min_hpwl && !np->o.disable_revert_if_diverge) {
         np->diverge_coef_x = np->wl_coef_x;
         np->diverge_coef_y = np->wl_coef_y;
@@ -1082,6 +1175,15 @@ static void push_history(orc_np* np, int iter) {
     r->hpwl = np->nbs[0].prev_hpwl;
 }
 
+/* nesterov.rs:4484-4486 the loop's momentum: a' = (1 + sqrt(4 a^2 + 1)) / 2 in double, and the
+ * look-ahead coefficient (a - 1) / a'.  Pinned by the closed form of Nesterov's sequence
+ * (a_k^2 - a_k = a_{k-1}^2, tests/test_oracle_control.py). */
+float orc_momentum(float prev_a, float* coeff) {
+    float a = (float)((1.0 + sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
+    *coeff = (float)(((double)prev_a - 1.0) / (double)a);
+    return a;
+}
+
 /* nesterov.rs:4440 resume_nesterov_place (no routability, timing or penalty guard).  Runs at most
  * `iter_budget` iterations (-1: no limit) and stops early once `time_budget_s` of wall time has
  * passed (<= 0: no limit) -- the bench's bounded CPU sample.  Returns the status; *iter_out is
@@ -1105,9 +1207,9 @@ int32_t orc_run(orc_np* np, int32_t start_iter, int32_t iter_budget, double time
     while (iter < np->max_iter) {
         if (iter_budget >= 0 && ran >= iter_budget) break;
         if (time_budget_s > 0.0 && now_s() - t0 > time_budget_s) break;
-        float prev_a = cur_a;
-        cur_a = (float)((1.0 + sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
-        float coeff = (float)(((double)prev_a - 1.0) / (double)cur_a);
+        float coeff = 0.0f;
+        cur_a = orc_momentum(cur_a, &coeff);
+        np->iter_now = iter;
         do_back_tracking(np, coeff);
         np->total_trials += np->trials;
         for (int b = 0; b < np->nregions; ++b) nesterov_adjust_phi(&np->nbs[b]);
@@ -1204,6 +1306,19 @@ void orc_get_scalars(orc_np* np, int32_t b, float* out) {
     out[9] = nb->coordi_distance;
 }
 void orc_set_inject(orc_np* np, int32_t after_iter) { np->inject_nan_after = after_iter; }
+
+/* The forced loop-control cases (see orc_np): kind 5 the zero WA sum at iteration `at`, kind 6 a
+ * divergence at iteration `at` (a second call sets a second iteration); kind 0 clears both. */
+void orc_set_inject_control(orc_np* np, int32_t kind, int32_t at) {
+    if (kind == 0) {
+        np->inject_zero_wl_at = np->inject_diverge_at = np->inject_diverge_at2 = -1;
+    } else if (kind == 5) {
+        np->inject_zero_wl_at = at;
+    } else if (kind == 6) {
+        if (np->inject_diverge_at < 0) np->inject_diverge_at = at;
+        else np->inject_diverge_at2 = at;
+    }
+}
 
 /* ---- per-kernel oracles on a caller-given state --------------------------------------------- */
 

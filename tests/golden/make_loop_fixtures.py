@@ -1,0 +1,80 @@
+"""Writes the end-to-end loop fixtures: the oracle's full run to overflow 0.10 on the 200k-cell
+(512 x 512 bins) and 1M-cell (1024 x 1024 bins) synthetic placements, seed 1.
+
+The oracle is the single-thread C restatement of the reference's loop (oracle/); a 1M-cell run
+takes about eleven minutes on one core, too long for a test, so its outcome is kept here as data:
+final HPWL, iteration, overflow, the first progress rows, the CPU time, and a hash of the
+generated problem (the GPU test checks that it places the same problem).
+
+    python tests/golden/make_loop_fixtures.py c2 c3
+"""
+import hashlib
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+
+def problem_hash(design):
+    """sha256 over every array of the Nesterov problem (cells, pins, nets, regions' bins)."""
+    a = design.arrays()
+    h = hashlib.sha256()
+    for k in sorted(k for k in a if k != "regions"):
+        v = a[k]
+        h.update(k.encode())
+        h.update(np.ascontiguousarray(v).tobytes() if isinstance(v, np.ndarray) else str(v).encode())
+    for r in a["regions"]:
+        for k in sorted(r):
+            v = r[k]
+            h.update(k.encode())
+            h.update(np.ascontiguousarray(v).tobytes() if isinstance(v, np.ndarray) else repr(v).encode())
+    return h.hexdigest()
+
+
+def run(name, seed=1):
+    import gpl_amd as G
+    import oracle as O
+    t0 = time.time()
+    d = G.Design.from_config(name, seed=seed)
+    d.initial_place()
+    d.init_nesterov_place()
+    setup = time.time() - t0
+    ph = problem_hash(d)
+    o = O.Oracle(d.problem())
+    t1 = time.time()
+    rc_init = o.init()
+    t_init = time.time() - t1
+    rc, it = o.run()
+    t_run = time.time() - t1
+    s = o.stats()
+    hist = o.history()
+    out = dict(
+        config=name, seed=seed, problem_sha256=ph,
+        cells=int(d.problem().num_cells), pins=int(d.problem().num_pins), nets=int(d.problem().num_nets),
+        status=int(rc), init_status=int(rc_init), converged=bool(s.converged), final_iter=int(it),
+        final_hpwl=int(s.hpwl), final_overflow=float(s.overflow), total_trials=int(s.total_trials),
+        first_below_target_seconds=float(s.first_below_target_seconds), init_seconds=float(t_init),
+        cpu_seconds_total=float(t_run), setup_seconds=float(setup),
+        cpu=platform.processor() or platform.machine(), threads=1,
+        rows=[dict(iter=r["iter"], trials=r["trials"], overflow=float(r["overflow"]), hpwl=int(r["hpwl"]),
+                   density_penalty=float(r["density_penalty"]), step_length=float(r["step_length"]))
+              for r in hist[:10]],
+        every_50th=[dict(iter=r["iter"], overflow=float(r["overflow"]), hpwl=int(r["hpwl"])) for r in hist[::50]],
+    )
+    o.close()
+    d.close()
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"loop_{name}.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"{name}: iter {it} hpwl {s.hpwl} overflow {s.overflow:.4f} cpu {t_run:.1f}s -> {path}", flush=True)
+
+
+if __name__ == "__main__":
+    for n in sys.argv[1:] or ["c2", "c3"]:
+        run(n)
