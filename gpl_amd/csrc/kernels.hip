@@ -1460,7 +1460,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
         // before any is used (indices clamped into the arrays, no guards, so that the compiler
         // does not wait after each); cells with more pins or a wider span finish in the loops.
         float2 t[kGradPins];
-        const int qlast = max(q1 - 1, 0);
+        // a pinless cell (q0 == q1) clamps to a term inside the staged range (or the array), so
+        // every read stays in bounds; its values are not used
+        const int qlast = staged ? max(q1 - 1, qb) : max(q1 - 1, 0);
         if (staged) {
 #pragma unroll
             for (int j = 0; j < kGradPins; ++j) t[j] = spg[min(q0 + j, qlast) - qb];
@@ -1656,10 +1658,12 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials,
         if (threadIdx.x < 6) __threadfence_system();
         __syncthreads();
         if (threadIdx.x == 0) {
-            // (the host reads only host_seq: the device copy needs no ordering against it)
+            // (the host reads only host_seq: the device copy needs no ordering against it).  The
+            // signal is a system-scope release store, so every write this block made before the
+            // barrier is visible to the host before the new number is.
             const unsigned long long q = seq + 1ull;
             *dev_seq = q;
-            *reinterpret_cast<volatile unsigned long long*>(host_seq) = q;
+            __hip_atomic_store(host_seq, q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 }

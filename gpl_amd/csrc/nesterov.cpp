@@ -153,6 +153,11 @@ struct gpl_ctx {
     gpl::Log log;
     std::string err;
     int32_t inject_kind = 0, inject_after = -1, iters_run = 0;
+    // host-forced loop-control cases (gpl_debug_inject kinds 5 and 6): the WA gradient sum read as
+    // zero once in iteration inject_zero_wl_at (the coefficient-halving recursion), a divergence
+    // reported in iterations inject_diverge_at / inject_diverge_at2 (the min-HPWL snapshot revert);
+    // -1: off.  iter_now is the iteration being run.
+    int32_t inject_zero_wl_at = -1, inject_diverge_at = -1, inject_diverge_at2 = -1, iter_now = -1;
     // profiling
     bool profiling = false;
     hipEvent_t ev[64];
@@ -350,6 +355,10 @@ struct gpl_ctx {
             if (!r.is_converged) {
                 r.wl_grad_sum = (float)h_sums[b].wl_abs;
                 r.density_grad_sum = (float)h_sums[b].den_abs;
+                if (inject_zero_wl_at >= 0 && iter_now == inject_zero_wl_at && slot == SLOT_NEXT) {
+                    r.wl_grad_sum = 0.0f;
+                    inject_zero_wl_at = -1;
+                }
             }
             if (r.wl_grad_sum == 0.0f && recursion_wl < 10) {
                 wl_coef_x = (float)((double)wl_coef_x * 0.5);
@@ -991,6 +1000,7 @@ struct gpl_ctx {
                 log.warn(324, strfmt("Divergence detected between reported values. Overflow change: %g, HPWL increase: %g%%.", change, increase * 100.0f));
             }
         }
+        if (iter_now >= 0 && (iter_now == inject_diverge_at || iter_now == inject_diverge_at2)) r.is_diverged = true;
         return r.is_diverged;
     }
 
@@ -1132,6 +1142,7 @@ struct gpl_ctx {
             float prev_a = ls.cur_a;
             ls.cur_a = (float)((1.0 + std::sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
             float coeff = (float)(((double)prev_a - 1.0) / (double)ls.cur_a);
+            iter_now = ls.iter;
             do_back_tracking(coeff);
             total_trials += trials;
             for (auto& r : rg) adjust_phi(r);
@@ -1946,7 +1957,17 @@ int32_t gpl_nesterov_step(gpl_ctx* c, int32_t region, float step, float coeff, f
 }
 
 int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter) {
-    if (!c || kind < 0 || kind > 4) return GPL_ERR_ARG;
+    if (!c || kind < 0 || kind > 6) return GPL_ERR_ARG;
+    if (kind == 5) {
+        c->inject_zero_wl_at = after_iter;
+        return GPL_OK;
+    }
+    if (kind == 6) {
+        if (c->inject_diverge_at < 0) c->inject_diverge_at = after_iter;
+        else c->inject_diverge_at2 = after_iter;
+        return GPL_OK;
+    }
+    if (kind == 0) c->inject_zero_wl_at = c->inject_diverge_at = c->inject_diverge_at2 = -1;
     if (kind == 0 || kind == 4) {
         // kind 4: the kernels' general paths (bin edges from the tables, the gradient's per-cell
         // gathers of pin terms); they give the same results as the fast ones

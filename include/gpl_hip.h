@@ -380,8 +380,11 @@ int32_t gpl_objective_backward(gpl_ctx* c, const float* xy, float wl_coef, float
  * kind 2; kind 3 re-sorts the device cells after every iteration from `after_iter` on (the
  * storage order must not change the results); kind 4 (from now on, `after_iter` unused) runs
  * the kernels' general paths where they have fast ones (bin edges read from the region's tables,
- * the gradient's per-cell gathers of the pin terms), which must give the same results.  kind 0
- * clears it. */
+ * the gradient's per-cell gathers of the pin terms), which must give the same results; kind 5:
+ * in iteration `after_iter` the WA gradient sum is read as zero once (npUpdate*Gradient's
+ * coefficient-halving recursion, nesterov.rs:2516-2522); kind 6: a divergence is reported in
+ * iteration `after_iter` (a second call adds a second iteration), driving isDiverged's min-HPWL
+ * snapshot revert and resume or stop (nesterov.rs:3009-3051).  Kind 0 clears every kind. */
 int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter);
 
 /* Device timing of the kernels while profiling (gpl_set_profiling; milliseconds summed per

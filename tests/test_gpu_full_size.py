@@ -53,7 +53,16 @@ def test_scatter_and_wa_bit_exact_at_1m_cells(c3_design):
         assert rel(sums[~small], os64[~small]) < 1e-5
         # WA backward and the density gather
         _, fx, fy = g.density_solve(0)
-        wl, den, _, _ = g.update_gradients(0, G.GPL_SLOT_PREV_SLP, np.float32(1e-8))
+        pen = np.float32(1e-8)
+        wl, den, sm, ab = g.update_gradients(0, G.GPL_SLOT_PREV_SLP, pen)
+        # the preconditioned sum: the oracle's arithmetic on the GPU's own parts, bit for bit; the
+        # FP64 |wl| and |den| sums against numpy's
+        box = a["cell_box"]
+        dxy = np.stack([box[:, 2] - box[:, 0], box[:, 3] - box[:, 1]], 1)
+        osm, _, _ = O.precondition(wl, den, np.diff(a["cell_pin_start"]), dxy, pen)
+        assert np.array_equal(sm, osm)
+        w64, d64 = np.abs(wl.astype(np.float64)).sum(), np.abs(den.astype(np.float64)).sum()
+        assert abs(ab[0] - w64) <= 1e-12 * w64 and abs(ab[1] - d64) <= 1e-12 * d64
         owl, ow64 = o.wa_backward(0, os64)
         big_cells = np.unique(a["pin_cell"][np.concatenate([a["net_pins"][a["net_pin_start"][e]:a["net_pin_start"][e + 1]]
                                                             for e in np.nonzero(~small)[0]])])

@@ -173,40 +173,69 @@ def test_gather_matches_fp64_field(c1):
 
 
 def test_nesterov_step_is_the_reference_arithmetic():
-    cells = [dict(box=(0, 0, 100, 100)), dict(box=(0, 0, 300, 100)), dict(box=(0, 0, 100, 100), flags=1 | 4)]
+    """nesterovUpdateCoordinates on a hand-built region from a nonzero preconditioned gradient (the
+    density part of updateGradients, penalty 1): next and next_slp bit-exact with the oracle's
+    update, a locked cell holding still and a cell pushed against the grid's edge."""
+    cells = [dict(box=(0, 0, 100, 100)), dict(box=(0, 0, 300, 100)), dict(box=(0, 0, 100, 100), flags=1 | 4),
+             dict(box=(0, 0, 700, 500)), dict(box=(0, 0, 200, 200), flags=0)]
     t = Tiny(cells, [], regions=[dict(lx=0, ly=0, ux=10000, uy=10000, cnt_x=8, cnt_y=8)])
     g = _ctx(t)
-    cur_slp = np.array([[500.5, 600.25], [9900.0, 50.0], [700.0, 700.0]], np.float32)
-    cur = np.array([[400.0, 650.0], [9800.0, 100.0], [650.0, 690.0]], np.float32)
-    g.set_coordinates(0, G.GPL_SLOT_CUR_SLP, cur_slp)   # also centres the boxes
+    cur_slp = np.array([[500.5, 600.25], [9900.0, 50.0], [700.0, 700.0], [800.0, 900.0], [650.0, 580.0]], np.float32)
+    cur = np.array([[400.0, 650.0], [9800.0, 100.0], [650.0, 690.0], [700.0, 950.0], [600.0, 600.0]], np.float32)
+    g.set_coordinates(0, G.GPL_SLOT_CUR_SLP, cur_slp)
     g.density_scatter(0, G.GPL_SLOT_CUR_SLP)
+    g.density_solve(0)
+    _, _, sm, _ = g.update_gradients(0, G.GPL_SLOT_CUR_SLP, np.float32(1.0))  # fills the cur_slp sum slot
+    assert np.abs(sm).max() > 0
     g.set_coordinates(0, G.GPL_SLOT_CUR, cur)
-    # the preconditioned gradient slot: set through a gradient evaluation is not needed -- zero sum
-    s, coeff = np.float32(3.5), np.float32(0.25)
-    nx, ns = g.nesterov_step(0, s, coeff)
-    sm = np.zeros_like(cur_slp)  # the cur_slp_sum slot starts at zero
-    f = np.float32
-
-    def inside(c, half, lo, hi):  # nesterov.rs:2188
-        adj = c
-        if c - half < f(lo):
-            adj = f(lo) + half
-        if c + half > f(hi):
-            adj = f(hi) - half
-        return adj
-
-    for k in range(3):
-        if k == 2:  # locked: stays
-            assert nx[k].tolist() == cur[k].tolist() and ns[k].tolist() == cur_slp[k].tolist()
-            continue
-        bx = cells[k]["box"]
-        halves = (f(bx[2] - bx[0]) / f(2.0), f(bx[3] - bx[1]) / f(2.0))
-        for ax in range(2):
-            n_ = f(cur_slp[k, ax] + s * sm[k, ax])
-            sl = f(n_ + coeff * f(n_ - cur[k, ax]))
-            assert nx[k, ax] == inside(n_, halves[ax], 0, 10000), (k, ax)
-            assert ns[k, ax] == inside(sl, halves[ax], 0, 10000), (k, ax)
+    dd = np.array([[c["box"][2] - c["box"][0], c["box"][3] - c["box"][1]] for c in cells], np.int32)
+    locked = np.array([bool(c.get("flags", 1) & 4) for c in cells])
+    for s in (np.float32(3.5), np.float32(5e4 / np.abs(sm).max())):
+        coeff = np.float32(0.25)
+        nx, ns = g.nesterov_step(0, s, coeff)
+        ox, osl = O.nesterov_points(cur_slp, sm, cur, locked, dd, (0, 0, 10000, 10000), s, coeff)
+        assert np.array_equal(nx, ox) and np.array_equal(ns, osl)
+        assert nx[2].tolist() == cur[2].tolist() and ns[2].tolist() == cur_slp[2].tolist()
+    assert np.any(ns - dd / 2 <= 0) or np.any(ns + dd / 2 >= 10000)  # the large step reached an edge
     g.close()
+
+
+@pytest.mark.parametrize("bin_size", [1000, 100_000])
+@pytest.mark.parametrize("layout", ["dense", "middle", "hash"])
+def test_scatter_chunk_paths_match_the_oracle(layout, bin_size):
+    """Two 512-cell chunks on 128 x 128 bins, laid out for each path of the scatter: every cell in
+    one 20 x 20-bin patch (the dense tile over the chunk's box); the middle half of each chunk in
+    the patch and its first and last quarters spread over the grid (the tile over the middle half,
+    the rest sent straight to the bins); every cell spread (the LDS hash table).  Both LDS table
+    widths (bins of 1e3 and 1e5 DBU).  The cells start at one point, so the device keeps the
+    problem's order and the chunks are the problem's cells 0-511 and 512-1023."""
+    rng = np.random.default_rng(11)
+    n, nb = 1024, 128
+    side = nb * bin_size
+    cells = []
+    for i in range(n):
+        w, h = (rng.uniform(0.3, 1.4, 2) * bin_size).astype(int) + 1
+        flags = 0 if i % 7 == 0 else (1 | (2 if i % 53 == 0 else 0))
+        cells.append(dict(box=(0, 0, int(w), int(h)), scale=float(rng.uniform(0.4, 1.0)), flags=flags))
+    t = Tiny(cells, [], regions=[dict(lx=0, ly=0, ux=side, uy=side, cnt_x=nb, cnt_y=nb, target_density=0.8)])
+    patch = rng.uniform(50 * bin_size, 70 * bin_size, (n, 2))
+    spread = rng.uniform(0.02 * side, 0.98 * side, (n, 2))
+    k = np.arange(n) % 512
+    middle = (k >= 128) & (k < 384)
+    xy = {"dense": patch, "middle": np.where(middle[:, None], patch, spread), "hash": spread}[layout].astype(np.float32)
+    g = _ctx(t)
+    before = g.kernel_times()["scatter_global_adds"][1]
+    g.set_coordinates(0, G.GPL_SLOT_CUR_SLP, xy)
+    inst, fill, dens, _ = g.density_scatter(0, G.GPL_SLOT_CUR_SLP)
+    adds = g.kernel_times()["scatter_global_adds"][1] - before
+    g.close()
+    o = O.Oracle(t.problem)
+    oi, of_, od, _, _ = o.scatter(0, xy)
+    o.close()
+    assert np.array_equal(inst, oi) and np.array_equal(fill, of_) and np.array_equal(dens, od)
+    touched = int(np.count_nonzero(oi) + np.count_nonzero(of_))
+    print(f"{layout} {bin_size}: {adds:.0f} global adds, {touched} nonzero bin accumulators")
+    assert adds >= 1
 
 
 def test_loop_matches_the_oracle_on_the_small_workload(c1):
