@@ -1,0 +1,53 @@
+"""The whole GPU loop to overflow 0.10 on the 200k-cell (512 x 512 bins) and 1M-cell
+(1024 x 1024 bins) workloads against the oracle's full runs on the same problems, kept as fixtures
+(tests/golden/loop_c2.json, loop_c3.json, written by tests/golden/make_loop_fixtures.py: the 1M-cell
+oracle run takes about eleven minutes on one core).
+
+Bars (the headline's): both runs reach overflow 0.10, the final pin HPWL within 0.5 % of the
+oracle's, the iteration count within 2 % (the GPU's FP64 reductions and FFT rounding move the
+trajectory slightly), and the first progress row within 1e-4 (HPWL) and 1e-4 (overflow)."""
+import json
+import os
+
+import pytest
+
+import gpl_amd as G
+from tests.golden.make_loop_fixtures import problem_hash
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.parametrize("cfg", ["c2", "c3"])
+def test_loop_matches_the_oracles_full_run(cfg, request):
+    with open(os.path.join(HERE, "golden", f"loop_{cfg}.json")) as f:
+        fx = json.load(f)
+    if cfg == "c3":
+        d = request.getfixturevalue("c3_design")
+        own = False
+    else:
+        d = G.Design.from_config(cfg)
+        d.initial_place()
+        d.init_nesterov_place()
+        own = True
+    try:
+        assert problem_hash(d) == fx["problem_sha256"], "the generator built a different problem"
+        g = G.NesterovGPU(d)
+        it = g.do_nesterov_place()
+        s, hist = g.stats(), g.history()
+        g.close()
+    finally:
+        if own:
+            d.close()
+    delta = (s.hpwl - fx["final_hpwl"]) / fx["final_hpwl"]
+    print(f"{cfg}: GPU iteration {it} HPWL {s.hpwl} overflow {s.overflow:.4f}; oracle iteration {fx['final_iter']} "
+          f"HPWL {fx['final_hpwl']} overflow {fx['final_overflow']:.4f}; delta {100 * delta:+.4f} %")
+    assert s.converged and fx["converged"]
+    assert s.overflow <= 0.1 and fx["final_overflow"] <= 0.1
+    assert abs(delta) <= 0.005
+    assert abs(it - fx["final_iter"]) <= max(10, 0.02 * fx["final_iter"])
+    r0 = fx["rows"][0]
+    assert hist[0]["iter"] == r0["iter"]
+    assert abs(hist[0]["hpwl"] - r0["hpwl"]) <= 1e-4 * r0["hpwl"]
+    assert abs(hist[0]["overflow"] - r0["overflow"]) <= 1e-4
