@@ -103,6 +103,17 @@ class KernelTime(C.Structure):
 
 
 # (name, restype, argtypes) of every exported function declared in include/gpl_hip.h
+class DensityIO(C.Structure):
+    """gpl_density_io: device pointers (as integers) of gpl_density_scatter_noalloc's arrays."""
+    _fields_ = [
+        ("num_cells", i32), ("dbox", C.c_void_p), ("density_scale", C.c_void_p), ("cell_flags", C.c_void_p),
+        ("lx", i32), ("ly", i32), ("ux", i32), ("uy", i32), ("cnt_x", i32), ("cnt_y", i32),
+        ("size_x", f64), ("size_y", f64), ("target_density", f32), ("bin_x", C.c_void_p), ("bin_y", C.c_void_p),
+        ("non_place_f", C.c_void_p), ("inst_area", C.c_void_p), ("filler_area", C.c_void_p), ("density", C.c_void_p),
+        ("overflow", C.c_void_p), ("stream", C.c_void_p),
+    ]
+
+
 SIGNATURES = [
     ("gpl_opts_default", None, [P(Opts)]),
     ("gpl_design_generate", i32, [P(GenConfig), P(Opts), P(C.c_void_p)]),
@@ -149,6 +160,7 @@ SIGNATURES = [
     ("gpl_util_round_down_pow2", u32, [u32]),
     ("gpl_util_struct_sizes", i32, [P(i64), i32]),
     ("gpl_set_launch_mode", i32, [C.c_void_p, i32]),
+    ("gpl_density_scatter_noalloc", i32, [P(DensityIO)]),
 ]
 
 _lib = None

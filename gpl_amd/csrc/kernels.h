@@ -132,8 +132,10 @@ void launch_update(hipStream_t s, const RegionDev& r, const CellsDev& c, const f
                    float2* next, float2* next_slp, float step, float coeff, unsigned long long* zero);
 // the scatter of the density boxes launch_update left into the bins.
 void launch_density_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c, unsigned long long* global_adds);
-// per bin: density, overflow partials, accumulators cleared.
-void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials);
+// per bin: density, overflow partials, the accumulators cleared (unless `clear` is false).
+void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials, bool clear = true);
+// b[i] = a[i], i < n <= 64 (device to device, one small kernel: no copy node in a captured graph)
+void launch_copy_f64(hipStream_t s, const double* a, double* b, int n);
 // the Poisson solve of r.density into r.field (and r.phi when want_phi).
 void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi);
 // WA forward over every net, the HPWL, and each pin's weighted WA gradient into n.pgrad (at its

@@ -484,7 +484,7 @@ __device__ void finish_sums(const double* partials, int nb_bins, int nb_cells, S
 
 // updateBinsGCellDensityArea's per-bin part (nesterov.rs:2252-2262): density, the overflow
 // terms (summed here in FP64), and the accumulators cleared for the next scatter.
-__global__ void k_bin_density(RegionDev r, double* partials) {
+__global__ void k_bin_density(RegionDev r, double* partials, int clear) {
     __builtin_amdgcn_s_setprio(2);  // on the trial's critical chain (as the scatter)
     __shared__ double sh[kBlock / 64];
     const size_t M = (size_t)r.cnt_x * r.cnt_y;
@@ -503,8 +503,10 @@ __global__ void k_bin_density(RegionDev r, double* partials) {
         if (!(tu >= 0.0f)) tu = 0.0f;
         o += (double)t;
         ou += (double)tu;
-        r.inst[k] = 0ull;
-        r.filler[k] = 0ull;
+        if (clear) {
+            r.inst[k] = 0ull;
+            r.filler[k] = 0ull;
+        }
     }
     double so = block_sum(o, sh);
     double sou = block_sum(ou, sh);
@@ -1979,8 +1981,16 @@ void launch_density_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c
     place_scatter(s, PLACE_STORED, r, c, pa);
 }
 
-void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials) {
-    hipLaunchKernelGGL(k_bin_density, dim3(block_partials_bins(r)), dim3(kBlock), 0, s, r, partials);
+void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials, bool clear) {
+    hipLaunchKernelGGL(k_bin_density, dim3(block_partials_bins(r)), dim3(kBlock), 0, s, r, partials, clear ? 1 : 0);
+}
+
+__global__ void k_copy_f64(const double* a, double* b, int n) {
+    if ((int)threadIdx.x < n) b[threadIdx.x] = a[threadIdx.x];
+}
+
+void launch_copy_f64(hipStream_t s, const double* a, double* b, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_copy_f64, dim3(1), dim3(64), 0, s, a, b, n);
 }
 
 static int ilog2(int L) {

@@ -375,6 +375,37 @@ int32_t gpl_objective_forward(gpl_ctx* c, const float* xy, float wl_coef, float 
 int32_t gpl_objective_backward(gpl_ctx* c, const float* xy, float wl_coef, float lambda,
                                float* grad);
 
+/* The density map of gpl_density_scatter on arrays the caller keeps in device memory, with no
+ * allocation (no reference counterpart: it serves callers that cannot call hipMalloc -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it).  The scatter of the density
+ * boxes as given (updateBinsGCellDensityArea, nesterov.rs:2218-2250) into inst_area / filler_area
+ * (int64 per bin, cleared first), the density per bin (2252-2263) and the two overflows
+ * (overflow[0] scaled, overflow[1] unscaled; FP64 sums), all written on `stream` (a hipStream_t;
+ * NULL: the null stream), asynchronously.  Its only other device memory is a fixed 33 KB array of
+ * the library's (the overflow reduction's partials) per GPU, held until the process ends; calls
+ * that overlap on one GPU race on it.  Limits: bin grids up to 4096 x 4096; dbox 16-byte aligned,
+ * non_place_f 8-byte aligned; a larger grid or a misaligned array is refused with GPL_ERR_ARG and
+ * a message on stderr.  Every pointer is a device pointer. */
+typedef struct gpl_density_io {
+    int32_t num_cells;
+    const int32_t* dbox;          /* 4 per cell: dlx, dly, dux, duy (the density box) */
+    const float* density_scale;   /* per cell */
+    const uint8_t* cell_flags;    /* GPL_CELL_* per cell */
+    int32_t lx, ly, ux, uy;       /* the bin grid (BinGrid, nesterov.rs:260) */
+    int32_t cnt_x, cnt_y;
+    double size_x, size_y;
+    float target_density;
+    const int32_t* bin_x;         /* cnt_x + 1 bin edges */
+    const int32_t* bin_y;         /* cnt_y + 1 */
+    const float* non_place_f;     /* 2 per bin: (float)non_place, (float)non_place_unscaled */
+    int64_t* inst_area;           /* out: cnt_x * cnt_y, bin k = y * cnt_x + x */
+    int64_t* filler_area;         /* out */
+    float* density;               /* out */
+    double* overflow;             /* out: 2 */
+    void* stream;
+} gpl_density_io;
+int32_t gpl_density_scatter_noalloc(const gpl_density_io* io);
+
 /* Fault injection for host-driven failure tests: after `after_iter` iterations the next
  * gradient sum reads NaN (GPL-0306) -- kind 1 -- or the next step length is NaN (GPL-0305) --
  * kind 2; kind 3 re-sorts the device cells after every iteration from `after_iter` on (the
