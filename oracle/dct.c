@@ -11,7 +11,8 @@
  * first, then columns of length n1, on a row-major n1 x n2 grid.
  *
  * The arithmetic is this file's own: Makhoul's DCT through a half-length complex FFT (radix-2,
- * twiddles from the C library in double), not Ooura's split-radix butterflies, so results agree
+ * twiddles from the C library in double, tabled once per length as Ooura's makewt/makect tables
+ * are, ooura.rs:260-382), not Ooura's split-radix butterflies, so results agree
  * with the reference's to rounding, not bit for bit (the reference pins its own bits on macOS
  * only, ooura.rs:3124-3159).  Powers of two use the FFT; any other length falls back to the
  * direct O(n^2) sum with double accumulation.
@@ -19,6 +20,7 @@
  * Pinned by tests/test_oracle.py: brute-force O(n^2) FP64 sums, scipy.fft, forward-inverse
  * identity, and the single-cosine-mode eigenfunction of the Poisson solve.
  */
+#include <alloca.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -30,8 +32,47 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 /* ---- generic body, instantiated for float (f32) and double (f64) ---------------------------- */
 #define DEFINE_DCT(T, SFX)                                                                          \
+    /* The transforms' twiddles of one length n (a power of two), computed once and kept: each     \
+     * entry is the cosine or sine of the same double angle the transforms' expressions name       \
+     * (2 pi k / len for the FFT's stage of length len, 2 pi k / n, pi k / 2n), rounded to T once.  \
+     * fft: stage len = 2 h holds h pairs at [h - 1, 2 h - 1); wn: k <= n/2; w4n: k < n. */         \
+    typedef struct {                                                                                \
+        int n;                                                                                      \
+        T *fft, *wn, *w4n;                                                                          \
+    } tab_t_##SFX;                                                                                  \
+    static tab_t_##SFX tabs_##SFX[32];                                                               \
+    static const tab_t_##SFX* tab_##SFX(int n) {                                                   \
+        int lg = 0;                                                                                 \
+        while ((1 << lg) < n) ++lg;                                                                 \
+        tab_t_##SFX* t = &tabs_##SFX[lg];                                                           \
+        if (t->n == n) return t;                                                                    \
+        T* buf = (T*)malloc(sizeof(T) * (2 * (size_t)n + 2 * ((size_t)n / 2 + 1) + 2 * (size_t)n + 2)); \
+        t->fft = buf;                                                                               \
+        t->wn = buf + 2 * (size_t)n;                                                                \
+        t->w4n = t->wn + 2 * ((size_t)n / 2 + 1);                                                   \
+        for (int len = 2; len <= n; len <<= 1) {                                                    \
+            int h = len >> 1;                                                                       \
+            for (int k = 0; k < h; ++k) {                                                           \
+                double ang = 2.0 * ORC_PI * k / len;                                                \
+                t->fft[2 * (h - 1 + k)] = (T)cos(ang);                                              \
+                t->fft[2 * (h - 1 + k) + 1] = (T)sin(ang);                                          \
+            }                                                                                       \
+        }                                                                                           \
+        for (int k = 0; k <= n / 2; ++k) {                                                          \
+            double ang = 2.0 * ORC_PI * k / n;                                                      \
+            t->wn[2 * k] = (T)cos(ang);                                                             \
+            t->wn[2 * k + 1] = (T)sin(ang);                                                         \
+        }                                                                                           \
+        for (int k = 0; k < n; ++k) {                                                               \
+            double ang = ORC_PI * k / (2.0 * n);                                                    \
+            t->w4n[2 * k] = (T)cos(ang);                                                            \
+            t->w4n[2 * k + 1] = (T)sin(ang);                                                        \
+        }                                                                                           \
+        t->n = n;                                                                                   \
+        return t;                                                                                   \
+    }                                                                                               \
     /* in-place radix-2 complex FFT of m points (re, im interleaved), sign -1 forward, +1 inverse  \
-     * (no scaling); twiddles exp(sign 2 pi i k / m) from double cos/sin */                         \
+     * (no scaling); twiddles exp(sign 2 pi i k / len) from the length's table */                   \
     static void fft_##SFX(int m, T* z, int sign) {                                                  \
         for (int i = 1, j = 0; i < m; ++i) {                                                        \
             int bit = m >> 1;                                                                       \
@@ -45,19 +86,26 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
                 z[2 * j + 1] = ti;                                                                  \
             }                                                                                       \
         }                                                                                           \
+        const T* st = tab_##SFX(m)->fft;                                                            \
+        /* the stage's twiddles with the sign on the sine, then its butterflies (each butterfly's  \
+         * arithmetic is independent of the order they run in) */                                   \
+        T* w = (T*)alloca(sizeof(T) * (size_t)m);                                                   \
         for (int len = 2; len <= m; len <<= 1) {                                                    \
             int h = len >> 1;                                                                       \
             for (int k = 0; k < h; ++k) {                                                           \
-                double ang = sign * 2.0 * ORC_PI * k / len;                                         \
-                T wr = (T)cos(ang), wi = (T)sin(ang);                                               \
-                for (int s = 0; s < m; s += len) {                                                  \
-                    T* a = z + 2 * (s + k);                                                         \
-                    T* b = z + 2 * (s + k + h);                                                     \
-                    T br = b[0] * wr - b[1] * wi, bi = b[0] * wi + b[1] * wr;                       \
-                    b[0] = a[0] - br;                                                               \
-                    b[1] = a[1] - bi;                                                               \
-                    a[0] = a[0] + br;                                                               \
-                    a[1] = a[1] + bi;                                                               \
+                w[2 * k] = st[2 * (h - 1 + k)];                                                     \
+                w[2 * k + 1] = sign < 0 ? -st[2 * (h - 1 + k) + 1] : st[2 * (h - 1 + k) + 1];       \
+            }                                                                                       \
+            for (int s = 0; s < m; s += len) {                                                      \
+                T* a = z + 2 * s;                                                                   \
+                T* b = z + 2 * (s + h);                                                             \
+                for (int k = 0; k < h; ++k) {                                                       \
+                    const T wr = w[2 * k], wi = w[2 * k + 1];                                       \
+                    T br = b[2 * k] * wr - b[2 * k + 1] * wi, bi = b[2 * k] * wi + b[2 * k + 1] * wr; \
+                    b[2 * k] = a[2 * k] - br;                                                       \
+                    b[2 * k + 1] = a[2 * k + 1] - bi;                                               \
+                    a[2 * k] = a[2 * k] + br;                                                       \
+                    a[2 * k + 1] = a[2 * k + 1] + bi;                                               \
                 }                                                                                   \
             }                                                                                       \
         }                                                                                           \
@@ -76,8 +124,8 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
             T ar = z[2 * k1], ai = z[2 * k1 + 1], br = z[2 * k2], bi = -z[2 * k2 + 1];              \
             T er = (ar + br) * (T)0.5, ei = (ai + bi) * (T)0.5; /* even part */                     \
             T orr = (ar - br) * (T)0.5, oi = (ai - bi) * (T)0.5; /* i * odd part */                 \
-            double ang = -2.0 * ORC_PI * k / n;                                                     \
-            T wr = (T)cos(ang), wi = (T)sin(ang);                                                   \
+            const T* wn = tab_##SFX(n)->wn; /* cos, sin of 2 pi k / n */                            \
+            T wr = wn[2 * k], wi = -wn[2 * k + 1];                                                  \
             /* V = E + w * O, O = -i * (orr + i oi) = oi - i orr */                                 \
             T xr = oi, xi = -orr;                                                                   \
             out[2 * k] = er + (wr * xr - wi * xi);                                                  \
@@ -128,8 +176,8 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
                 vr = V[2 * (n - k)];                                                                \
                 vi = -V[2 * (n - k) + 1];                                                           \
             }                                                                                       \
-            double ang = -ORC_PI * k / (2.0 * n);                                                   \
-            a[k] = (T)cos(ang) * vr - (T)sin(ang) * vi;                                             \
+            const T* w4 = tab_##SFX(n)->w4n; /* cos, sin of pi k / 2n */                            \
+            a[k] = w4[2 * k] * vr - (-w4[2 * k + 1]) * vi;                                          \
         }                                                                                           \
         free(v);                                                                                    \
     }                                                                                               \
@@ -150,8 +198,8 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
             T ck = (k == 0) ? (T)2 * a[0] : a[k % n];                                               \
             T cnk = (k == 0) ? (T)0 : a[n - k];                                                     \
             if (k == h) ck = a[h], cnk = a[h];                                                      \
-            double ang = ORC_PI * k / (2.0 * n);                                                    \
-            T wr = (T)cos(ang), wi = (T)sin(ang);                                                   \
+            const T* w4 = tab_##SFX(n)->w4n;                                                        \
+            T wr = w4[2 * k], wi = w4[2 * k + 1];                                                   \
             V[2 * k] = wr * ck + wi * cnk;                                                          \
             V[2 * k + 1] = wi * ck - wr * cnk;                                                      \
         }                                                                                           \
@@ -160,8 +208,8 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
         for (int k = 0; k < h; ++k) {                                                               \
             T ar = V[2 * k], ai = V[2 * k + 1], br = V[2 * (h - k)], bi = -V[2 * (h - k) + 1];      \
             T er = ar + br, ei = ai + bi, dr = ar - br, di = ai - bi;                               \
-            double ang = 2.0 * ORC_PI * k / n;                                                      \
-            T wr = (T)cos(ang), wi = (T)sin(ang);                                                   \
+            const T* wn = tab_##SFX(n)->wn;                                                         \
+            T wr = wn[2 * k], wi = wn[2 * k + 1];                                                   \
             T tr = wr * dr - wi * di, ti = wr * di + wi * dr; /* w * d */                           \
             z[2 * k] = er - ti;                                                                     \
             z[2 * k + 1] = ei + tr;                                                                 \
@@ -195,11 +243,16 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
     /* 2-D: rows (length n2) with `row`, then columns (length n1) with `col` */                     \
     static void xt2d_##SFX(int n1, int n2, T* a, void (*row)(int, T*), void (*col)(int, T*)) {      \
         for (int i = 0; i < n1; ++i) row(n2, a + (size_t)i * n2);                                   \
-        T* t = (T*)malloc(sizeof(T) * n1);                                                          \
-        for (int j = 0; j < n2; ++j) {                                                              \
-            for (int i = 0; i < n1; ++i) t[i] = a[(size_t)i * n2 + j];                              \
-            col(n1, t);                                                                             \
-            for (int i = 0; i < n1; ++i) a[(size_t)i * n2 + j] = t[i];                              \
+        /* columns in blocks of 16 gathered together (runs of 16 consecutive values per row) */     \
+        enum { B = 16 };                                                                            \
+        T* t = (T*)malloc(sizeof(T) * (size_t)n1 * B);                                              \
+        for (int j0 = 0; j0 < n2; j0 += B) {                                                        \
+            const int nb = n2 - j0 < B ? n2 - j0 : B;                                               \
+            for (int i = 0; i < n1; ++i)                                                            \
+                for (int q = 0; q < nb; ++q) t[(size_t)q * n1 + i] = a[(size_t)i * n2 + j0 + q];    \
+            for (int q = 0; q < nb; ++q) col(n1, t + (size_t)q * n1);                               \
+            for (int i = 0; i < n1; ++i)                                                            \
+                for (int q = 0; q < nb; ++q) a[(size_t)i * n2 + j0 + q] = t[(size_t)q * n1 + i];    \
         }                                                                                           \
         free(t);                                                                                    \
     }                                                                                               \
