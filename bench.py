@@ -15,7 +15,14 @@ Besides the throughput the line carries:
   cpu_baseline  the oracle (the C restatement of the reference's loop, one host thread) on a bounded
                 sample of the same placement: np_init and the first iterations within --cpu-budget
                 seconds;
-  hpwl_delta    GPU vs CPU final HPWL on the small (2k-cell) placement, run to convergence on both.
+  steady_state  iterations/s from iteration 50 to convergence of the same run (the whole run's
+                rate, not only the timed window), and the time to overflow 0.10;
+  hpwl_delta    GPU vs CPU final HPWL on the 1M-cell placement: the GPU run above against the
+                oracle's full run on the same problem (tests/golden/loop_c3.json: one core, about
+                twelve minutes, too long to repeat here; the problems' fingerprints must match),
+                and a fresh GPU-vs-oracle run to convergence on the 2k-cell placement;
+  batched       16 seeds of the 50k-cell design placed one after another and from concurrent
+                contexts (each its own placement and control), with the results compared.
 """
 import argparse
 import json
@@ -226,9 +233,19 @@ def timeline(path):
               f"{st.median(x[1] for x in v) / 1e3:9.1f} {st.median(x[1] - x[0] for x in v) / 1e3:9.1f} {v[0][2]:>6s}")
 
 
-def batched(cfg, k):
+def load_fixture(cfg):
+    """The oracle's full run of a workload (tests/golden/loop_<cfg>.json), or None."""
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", f"loop_{cfg}.json")) as f:
+            return json.load(f)
+    except OSError:
+        return None
+
+
+def batched(cfg, k, emit=True):
     """K independent placements (seeds 1..K: their own perturbations) of one design: run one
-    after another, then at once from K host threads (one context each); one JSON line."""
+    after another, then at once from K host threads (one context each); one JSON line when
+    `emit`, and the record returned."""
     import gpl_amd as G
 
     def log(*a):
@@ -260,12 +277,15 @@ def batched(cfg, k):
         g.close()
     d.close()
     log(f"[bench] sequential {t_seq:.3f}s, concurrent {t_par:.3f}s, same results: {same}")
-    print(json.dumps({
+    line = {
         "metric": f"placements/s, {k} seeds of one design, each to overflow 0.10, concurrent contexts",
         "value": round(k / t_par, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
         "sequential_value": round(k / t_seq, 3), "speedup_vs_sequential": round(t_seq / t_par, 2),
         "iterations": it_par, "same_results_as_sequential": bool(same), "dtype": "f32", "data": "synthetic",
-        "config": {"workload": WORKLOADS.get(cfg, cfg), "name": cfg, "seeds": k}}), flush=True)
+        "config": {"workload": WORKLOADS.get(cfg, cfg), "name": cfg, "seeds": k}}
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 def cpu_info():
@@ -290,6 +310,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline sampling")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the HPWL check")
     ap.add_argument("--hpwl-config", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-batched", action="store_true", help="skip the 16-seed batched comparison")
     ap.add_argument("--launch-mode", type=int, default=0, choices=(0, 1, 2, 3),
                     help="0: each trial one HIP graph (branches on side streams); 1: kernels launched one by "
                          "one on the side streams; 2: one by one on one stream; 3: one graph captured on one stream")
@@ -328,6 +349,8 @@ def main():
     if args.seeds:
         batched(args.config, args.seeds)
         return
+    if args.config != "c3":
+        args.no_batched = True
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -374,8 +397,15 @@ def main():
                           "iters_per_s": round(steps / (t1 - t0), 2), "families": fam}), flush=True)
         d.close()
         return
+    # on to iteration 50, then to convergence: the steady-state window
+    t50, it50 = None, None
+    if not done and it < 50:
+        it, done = g.loop_step(50 - it)
+    if not done:
+        t50, it50 = time.perf_counter(), it
     while not done:
         it, done = g.loop_step(1 << 20)
+    t_end = time.perf_counter()
     rc, final_iter = g.loop_end(check=False)
     s = g.stats()
     hist = g.history()
@@ -395,7 +425,10 @@ def main():
     gp.set_profiling(True)
     gp.init_nesterov_loop()
     gp.loop_begin(0)
-    gp.loop_step(args.warmup + args.steps)
+    # the whole run, np_init to convergence: the families' averages over every phase of the run
+    pdone = False
+    while not pdone:
+        _, pdone = gp.loop_step(1 << 20)
     kt = gp.kernel_times()
     gp.close()
     resorts, global_adds = kt.pop("scatter_global_adds", (0, 0.0))
@@ -436,6 +469,19 @@ def main():
                "init_seconds": round(os_.init_seconds, 3)}
         o.close()
         log(f"[bench] cpu baseline {cpu_val:.3f} it/s over {n_it} iterations")
+        fx = load_fixture(args.config)
+        if fx:
+            cpu["full_run"] = {"time_to_overflow_s": round(fx["first_below_target_seconds"], 1), "iterations": fx["final_iter"],
+                               "iters_per_s": round(fx["final_iter"] / fx["cpu_seconds_total"], 4), "cores": 1,
+                               "host": fx["cpu"], "source": f"tests/golden/loop_{args.config}.json (the oracle's full run, "
+                                                            "one core of the build container)"}
+            same_problem = d.problem_sha256() == fx["problem_sha256"]
+            hpwl_delta = {"config": args.config, "gpu_hpwl": s.hpwl, "cpu_hpwl": fx["final_hpwl"],
+                          "delta": round((s.hpwl - fx["final_hpwl"]) / fx["final_hpwl"], 6), "gpu_iter": final_iter,
+                          "cpu_iter": fx["final_iter"], "gpu_overflow": round(s.overflow, 4),
+                          "cpu_overflow": round(fx["final_overflow"], 4), "same_problem": same_problem,
+                          "cpu_run": f"tests/golden/loop_{args.config}.json"}
+            log(f"[bench] HPWL vs the oracle's full run: {100 * hpwl_delta['delta']:+.4f} % (same problem: {same_problem})")
         # full-run HPWL parity on the small placement
         dh = G.Design.from_config(args.hpwl_config)
         dh.initial_place()
@@ -449,10 +495,18 @@ def main():
         oh.run()
         so = oh.stats()
         oh.close()
-        hpwl_delta = {"config": args.hpwl_config, "gpu_hpwl": sg.hpwl, "cpu_hpwl": so.hpwl,
-                      "delta": round((sg.hpwl - so.hpwl) / so.hpwl, 6), "gpu_iter": sg.iter, "cpu_iter": so.iter,
-                      "gpu_overflow": round(sg.overflow, 4), "cpu_overflow": round(so.overflow, 4)}
+        small = {"config": args.hpwl_config, "gpu_hpwl": sg.hpwl, "cpu_hpwl": so.hpwl,
+                 "delta": round((sg.hpwl - so.hpwl) / so.hpwl, 6), "gpu_iter": sg.iter, "cpu_iter": so.iter,
+                 "gpu_overflow": round(sg.overflow, 4), "cpu_overflow": round(so.overflow, 4)}
+        if hpwl_delta is None:
+            hpwl_delta = small
+        else:
+            hpwl_delta["small"] = small
         dh.close()
+
+    batched_line = None
+    if not args.no_batched:
+        batched_line = batched("c4s", 16, emit=False)
 
     out = {
         "metric": METRIC,
@@ -483,6 +537,10 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu and cpu["value"] > 0 else None,
         "hpwl_delta": hpwl_delta,
+        "steady_state": ({"iters_per_s": round((final_iter - it50) / (t_end - t50), 2), "window": [it50, final_iter],
+                          "note": "iteration 50 to convergence of the timed run"}
+                         if t50 is not None and t_end > t50 and final_iter > it50 else None),
+        "batched": batched_line,
     }
     print(json.dumps(out), flush=True)
     d.close()

@@ -190,6 +190,26 @@ class Design:
         out["regions"] = regs
         return out
 
+    def problem_sha256(self):
+        """A fingerprint of the Nesterov problem: sha256 over every array of arrays() (cells, pins,
+        nets, the regions' bins), so that two builds of a design can be told apart."""
+        import hashlib
+        a = self.arrays()
+        h = hashlib.sha256()
+
+        def feed(k, v):
+            h.update(k.encode())
+            h.update(np.ascontiguousarray(v).tobytes() if isinstance(v, np.ndarray) else repr(v).encode())
+
+        for k in sorted(k for k in a if k != "regions"):
+            v = a[k]
+            h.update(k.encode())
+            h.update(np.ascontiguousarray(v).tobytes() if isinstance(v, np.ndarray) else str(v).encode())
+        for r in a["regions"]:
+            for k in sorted(r):
+                feed(k, r[k])
+        return h.hexdigest()
+
     def close(self):
         if getattr(self, "_h", None):
             _ffi.lib().gpl_design_destroy(self._h)

@@ -8,7 +8,6 @@ generated problem (the GPU test checks that it places the same problem).
 
     python tests/golden/make_loop_fixtures.py c2 c3
 """
-import hashlib
 import json
 import os
 import platform
@@ -23,18 +22,18 @@ import numpy as np  # noqa: E402
 
 def problem_hash(design):
     """sha256 over every array of the Nesterov problem (cells, pins, nets, regions' bins)."""
-    a = design.arrays()
-    h = hashlib.sha256()
-    for k in sorted(k for k in a if k != "regions"):
-        v = a[k]
-        h.update(k.encode())
-        h.update(np.ascontiguousarray(v).tobytes() if isinstance(v, np.ndarray) else str(v).encode())
-    for r in a["regions"]:
-        for k in sorted(r):
-            v = r[k]
-            h.update(k.encode())
-            h.update(np.ascontiguousarray(v).tobytes() if isinstance(v, np.ndarray) else repr(v).encode())
-    return h.hexdigest()
+    return design.problem_sha256()
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def run(name, seed=1):
@@ -61,7 +60,7 @@ def run(name, seed=1):
         final_hpwl=int(s.hpwl), final_overflow=float(s.overflow), total_trials=int(s.total_trials),
         first_below_target_seconds=float(s.first_below_target_seconds), init_seconds=float(t_init),
         cpu_seconds_total=float(t_run), setup_seconds=float(setup),
-        cpu=platform.processor() or platform.machine(), threads=1,
+        cpu=cpu_model(), threads=1,
         rows=[dict(iter=r["iter"], trials=r["trials"], overflow=float(r["overflow"]), hpwl=int(r["hpwl"]),
                    density_penalty=float(r["density_penalty"]), step_length=float(r["step_length"]))
               for r in hist[:10]],

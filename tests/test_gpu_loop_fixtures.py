@@ -1,7 +1,9 @@
-"""The whole GPU loop to overflow 0.10 on the 200k-cell (512 x 512 bins) and 1M-cell
-(1024 x 1024 bins) workloads against the oracle's full runs on the same problems, kept as fixtures
-(tests/golden/loop_c2.json, loop_c3.json, written by tests/golden/make_loop_fixtures.py: the 1M-cell
-oracle run takes about eleven minutes on one core).
+"""The whole GPU loop to overflow 0.10 against the oracle's full runs on the same problems, kept as
+fixtures (tests/golden/loop_<config>.json, written by tests/golden/make_loop_fixtures.py: the
+1M-cell oracle run takes about twelve minutes on one core): the 200k-cell (512 x 512 bins) and
+1M-cell (1024 x 1024 bins) workloads, the eight region groups of 5k-150k cells (485k, grids
+32^2..512^2) with 20k top-level cells, and seed 1 of the 50k-cell design (256 x 256 bins); and the
+16 seeds of that design placed concurrently against one after another.
 
 Bars (the headline's): both runs reach overflow 0.10, the final pin HPWL within 0.5 % of the
 oracle's, the iteration count within 2 % (the GPU's FP64 reductions and FFT rounding move the
@@ -19,7 +21,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-@pytest.mark.parametrize("cfg", ["c2", "c3"])
+@pytest.mark.parametrize("cfg", ["c2", "c3", "c4r", "c4s"])
 def test_loop_matches_the_oracles_full_run(cfg, request):
     with open(os.path.join(HERE, "golden", f"loop_{cfg}.json")) as f:
         fx = json.load(f)
@@ -37,6 +39,13 @@ def test_loop_matches_the_oracles_full_run(cfg, request):
         it = g.do_nesterov_place()
         s, hist = g.stats(), g.history()
         g.close()
+        if cfg == "c4r":
+            # nine regions in one trial graph against the kernels launched one by one on one stream
+            g = G.NesterovGPU(d)
+            g.set_launch_mode(2)
+            assert g.do_nesterov_place() == it
+            assert g.history() == hist
+            g.close()
     finally:
         if own:
             d.close()
@@ -51,3 +60,29 @@ def test_loop_matches_the_oracles_full_run(cfg, request):
     assert hist[0]["iter"] == r0["iter"]
     assert abs(hist[0]["hpwl"] - r0["hpwl"]) <= 1e-4 * r0["hpwl"]
     assert abs(hist[0]["overflow"] - r0["overflow"]) <= 1e-4
+
+
+def test_sixteen_seeds_concurrently_give_each_seed_its_own_result():
+    """Seeds 1..16 of the 50k-cell design (each its own perturbation and control) from concurrent
+    contexts: every seed's rows, iterations and placement as when it runs alone."""
+    d = G.Design.from_config("c4s")
+    d.initial_place()
+    sets = []
+    for _ in range(2):
+        ctxs = []
+        for seed in range(1, 17):
+            d.init_nesterov_place(G.default_opts(**dict(G.CONFIGS["c4s"]["opts"], seed=seed)))
+            ctxs.append(G.NesterovGPU(d))
+        sets.append(ctxs)
+    try:
+        seq = [g.do_nesterov_place() for g in sets[0]]
+        par = G.place_concurrently(sets[1])
+        assert seq == par
+        for a, b in zip(sets[0], sets[1]):
+            assert a.history() == b.history()
+            assert (a.placed_locations() == b.placed_locations()).all()
+            assert a.stats().converged
+    finally:
+        for g in sets[0] + sets[1]:
+            g.close()
+        d.close()
