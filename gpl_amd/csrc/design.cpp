@@ -1692,3 +1692,112 @@ int32_t gpl_design_problem(const gpl_design* d, gpl_problem* v) {
 }
 
 }  // extern "C"
+
+// ---- placed locations into the design, and the design as DEF ----------------------------------
+
+extern "C" int32_t gpl_design_write_locations(gpl_design* d, const int32_t* loc, int32_t n, int64_t* hpwl_out) {
+    if (!d || (n > 0 && !loc)) return GPL_ERR_ARG;
+    gpl::Placer& pl = d->pl;
+    gpl::Design& D = d->d;
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t k = loc[3 * i], lx = loc[3 * i + 1], ly = loc[3 * i + 2];
+        if (k < 0 || k >= (int32_t)pl.insts.size()) return GPL_ERR_ARG;
+        const int32_t di = pl.insts[k].db;
+        if (di < 0) continue;  // a dummy: no design instance
+        gpl::Rect& b = D.insts[di].bbox;
+        const int32_t w = b.dx(), h = b.dy();
+        b = gpl::Rect{lx, ly, lx + w, ly + h};
+        D.insts[di].placed = true;
+        if (di < (int32_t)pl.db_bbox.size()) pl.db_bbox[di] = b;
+    }
+    if (hpwl_out) {
+        // the nets' terminal boxes (db_router.rs:98-112): each instance terminal at its instance's
+        // centre plus its offset, each port its box
+        int64_t total = 0;
+        for (auto& net : D.nets) {
+            int32_t lx = INT32_MAX, ly = INT32_MAX, ux = INT32_MIN, uy = INT32_MIN;
+            for (auto& t : net.iterms) {
+                const gpl::Rect& b = D.insts[t.inst].bbox;
+                const int32_t x = (b.lx + b.ux) / 2 + t.off_x, y = (b.ly + b.uy) / 2 + t.off_y;
+                lx = std::min(lx, x);
+                ly = std::min(ly, y);
+                ux = std::max(ux, x);
+                uy = std::max(uy, y);
+            }
+            for (auto& q : net.bterms) {
+                lx = std::min(lx, q.bbox.lx);
+                ly = std::min(ly, q.bbox.ly);
+                ux = std::max(ux, q.bbox.ux);
+                uy = std::max(uy, q.bbox.uy);
+            }
+            if (ux >= lx && uy >= ly) total += int64_t(ux - lx) + int64_t(uy - ly);
+        }
+        *hpwl_out = total;
+    }
+    return GPL_OK;
+}
+
+extern "C" int32_t gpl_design_bases(const gpl_design* d, int32_t* out, int32_t cap) {
+    if (!d) return GPL_ERR_ARG;
+    const int32_t n = (int32_t)d->pl.bases.size();
+    for (int32_t b = 0; b < n && out && b < cap; ++b) {
+        const gpl::Base& B = d->pl.bases[b];
+        out[4 * b] = B.group;
+        out[4 * b + 1] = (int32_t)B.place_insts.size();
+        out[4 * b + 2] = (int32_t)B.fixed_insts.size();
+        out[4 * b + 3] = (int32_t)B.dummies.size();
+    }
+    return n;
+}
+
+// DEF 5.8 of the design as it stands: the die, the rows, every instance (PLACED where the placer
+// moved it, FIXED where it is fixed), the IO ports with their boxes and every net's terminals.  The
+// synthetic designs have no library: a master is named by its size (W<sites>H<rows> for standard
+// cells, BLOCK_<w>x<h> for blocks) and an instance terminal by its place among the instance's.
+extern "C" int32_t gpl_design_write_def(const gpl_design* d, const char* path) {
+    if (!d || !path) return GPL_ERR_ARG;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return GPL_ERR_ARG;
+    const gpl::Design& D = d->d;
+    const int32_t sx = std::max(1, D.site_x), sy = std::max(1, D.site_y);
+    std::fprintf(f, "VERSION 5.8 ;\nDIVIDERCHAR \"/\" ;\nBUSBITCHARS \"[]\" ;\nDESIGN synthetic_%u ;\n", d->cfg.seed);
+    std::fprintf(f, "UNITS DISTANCE MICRONS %d ;\n\n", D.dbu);
+    std::fprintf(f, "DIEAREA ( %d %d ) ( %d %d ) ;\n\n", D.die.lx, D.die.ly, D.die.ux, D.die.uy);
+    for (size_t r = 0; r < D.rows.size(); ++r) {
+        const gpl::Rect& R = D.rows[r];
+        std::fprintf(f, "ROW ROW_%zu core_site %d %d %s DO %d BY 1 STEP %d 0 ;\n", r, R.lx, R.ly, (r % 2) ? "FS" : "N", R.dx() / sx, sx);
+    }
+    std::fprintf(f, "\nCOMPONENTS %zu ;\n", D.insts.size());
+    for (size_t i = 0; i < D.insts.size(); ++i) {
+        const gpl::DbInst& I = D.insts[i];
+        const gpl::Rect& b = i < d->pl.db_bbox.size() ? d->pl.db_bbox[i] : I.bbox;
+        char master[64];
+        if (I.block)
+            std::snprintf(master, sizeof master, "BLOCK_%dx%d", b.dx(), b.dy());
+        else
+            std::snprintf(master, sizeof master, "W%dH%d", b.dx() / sx, std::max(1, b.dy() / sy));
+        std::fprintf(f, "- i%zu %s + %s ( %d %d ) N ;\n", i, master, I.fixed ? "FIXED" : "PLACED", b.lx, b.ly);
+    }
+    std::fprintf(f, "END COMPONENTS\n\n");
+    size_t nports = 0;
+    for (auto& n : D.nets) nports += n.bterms.size();
+    std::fprintf(f, "PINS %zu ;\n", nports);
+    size_t port = 0;
+    for (size_t e = 0; e < D.nets.size(); ++e)
+        for (auto& p : D.nets[e].bterms) {
+            const int32_t cx = (p.bbox.lx + p.bbox.ux) / 2, cy = (p.bbox.ly + p.bbox.uy) / 2;
+            std::fprintf(f, "- io%zu + NET n%zu + DIRECTION INOUT + USE SIGNAL\n  + LAYER metal1 ( %d %d ) ( %d %d )\n  + FIXED ( %d %d ) N ;\n",
+                         port++, e, p.bbox.lx - cx, p.bbox.ly - cy, p.bbox.ux - cx, p.bbox.uy - cy, cx, cy);
+        }
+    std::fprintf(f, "END PINS\n\nNETS %zu ;\n", D.nets.size());
+    port = 0;
+    for (size_t e = 0; e < D.nets.size(); ++e) {
+        std::fprintf(f, "- n%zu", e);
+        for (auto& t : D.nets[e].iterms) std::fprintf(f, " ( i%d p%d )", t.inst, t.order);
+        for (size_t q = 0; q < D.nets[e].bterms.size(); ++q) std::fprintf(f, " ( PIN io%zu )", port++);
+        std::fprintf(f, " ;\n");
+    }
+    std::fprintf(f, "END NETS\n\nEND DESIGN\n");
+    const bool ok = std::ferror(f) == 0;
+    return std::fclose(f) == 0 && ok ? GPL_OK : GPL_ERR_ARG;
+}

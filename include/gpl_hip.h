@@ -163,6 +163,20 @@ uint32_t gpl_util_round_down_pow2(uint32_t x);
  * gpl_problem, gpl_stats, gpl_iter_row, gpl_kernel_time; returns how many there are. */
 int32_t gpl_util_struct_sizes(int64_t* out, int32_t cap);
 
+/* main.rs:24-41 (updateDbGCells, the job's report of the design HPWL): each placed instance's
+ * lower-left (the triples gpl_placed_locations returns: placer instance, lx, ly) written into the
+ * design's instance boxes, marked placed; *hpwl_out the HPWL of the nets' terminal boxes afterwards
+ * (db_router.rs:98-112's GPL-1018 measure: an instance terminal at its instance's centre plus its
+ * offset, a port its box). */
+int32_t gpl_design_write_locations(gpl_design* d, const int32_t* loc, int32_t n, int64_t* hpwl_out);
+/* The job report's `bases` (main.rs:362-364): per placer base 4 int32 -- group (-1 the top level),
+ * instances placed, fixed, dummies; returns the number of bases. */
+int32_t gpl_design_bases(const gpl_design* d, int32_t* out, int32_t cap);
+/* The job's write_def (main.rs:357-358, libodb's DEF writer there): the design as it stands as a
+ * DEF 5.8 file -- die, rows, components (PLACED / FIXED), IO pins, nets.  The synthetic designs have
+ * no library, so masters are named by size and terminals by their order on the instance. */
+int32_t gpl_design_write_def(const gpl_design* d, const char* path);
+
 /* ------------------------------------------------------------------------------------------ */
 /* The Nesterov placer problem: the state init_nesterov_place (nesterov.rs:787) leaves        */
 /* ------------------------------------------------------------------------------------------ */

@@ -83,6 +83,7 @@ def lib():
             ("orc_check_divergence", C.c_int32, [C.c_float, C.c_float, C.c_float, C.c_int64, C.c_float, C.c_int64]),
             ("orc_momentum", C.c_float, [C.c_float, f32p]),
             ("orc_set_inject_control", None, [C.c_void_p, C.c_int32, C.c_int32]),
+            ("orc_region_rows", C.c_int32, [C.c_void_p, f32p, C.c_int32]),
         ]
         for name, res, args in sig:
             fn = getattr(L, name)
@@ -230,6 +231,14 @@ class Oracle:
 
     def set_inject(self, after_iter):
         lib().orc_set_inject(self._h, after_iter)
+
+    def region_rows(self):
+        """Every region's state after each iteration: rows (iter, region, overflow, penalty, step,
+        converged)."""
+        n = lib().orc_region_rows(self._h, None, 0)
+        a = np.zeros(6 * max(n, 1), np.float32)
+        lib().orc_region_rows(self._h, f32(a), n)
+        return a[:6 * n].reshape(-1, 6)
 
     def set_inject_control(self, kind, at):
         """Kind 5: the WA gradient sum read as zero once at iteration `at`; kind 6: a divergence

@@ -242,6 +242,8 @@ typedef struct orc_np {
     double t_init, t_loop, t_converge;
     gpl_iter_row* hist;
     int32_t nhist, cap_hist;
+    float* rrows; /* per iteration and region: iter, region, sum_overflow_unscaled, density_penalty, step, converged */
+    int32_t nrrows, cap_rrows;
     int32_t inject_nan_after;
     /* host-forced cases of the loop control (the GPU's gpl_debug_inject kinds 5 and 6): the WA
      * gradient sum read as zero once at iteration inject_zero_wl_at (the halving recursion), and a
@@ -399,6 +401,7 @@ void orc_destroy(orc_np* np) {
     free(np->nw);
     free(np->nsum);
     free(np->hist);
+    free(np->rrows);
     free(np);
 }
 
@@ -1173,6 +1176,27 @@ static void push_history(orc_np* np, int iter) {
     r->step_length = np->nbs[0].step_length;
     r->wl_coef = np->wl_coef_x;
     r->hpwl = np->nbs[0].prev_hpwl;
+    for (int b = 0; b < np->nregions; ++b) {
+        if (np->nrrows == np->cap_rrows) {
+            np->cap_rrows = np->cap_rrows ? 2 * np->cap_rrows : 4096;
+            np->rrows = (float*)realloc(np->rrows, sizeof(float) * 6 * (size_t)np->cap_rrows);
+        }
+        float* q = np->rrows + 6 * (size_t)np->nrrows++;
+        q[0] = (float)iter;
+        q[1] = (float)b;
+        q[2] = np->nbs[b].sum_overflow_unscaled;
+        q[3] = np->nbs[b].density_penalty;
+        q[4] = np->nbs[b].step_length;
+        q[5] = (float)np->nbs[b].is_converged;
+    }
+}
+
+/* Every region's state after each iteration (6 floats a row: iteration, region, overflow, penalty,
+ * step length, converged); returns the number of rows. */
+int32_t orc_region_rows(orc_np* np, float* out, int32_t cap) {
+    int32_t n = np->nrrows < cap ? np->nrrows : cap;
+    if (out && n > 0) memcpy(out, np->rrows, sizeof(float) * 6 * (size_t)n);
+    return np->nrrows;
 }
 
 /* nesterov.rs:4484-4486 the loop's momentum: a' = (1 + sqrt(4 a^2 + 1)) / 2 in double, and the
