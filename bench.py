@@ -413,6 +413,8 @@ def main():
     # wait for the graph, and the host's own work between two trials
     host = {k: round(1000.0 * v[1] / v[0], 2) for k, v in g.kernel_times().items()
             if k.startswith("host_") and v[0] > 0}
+    tuning = g.tuning()
+    tune_runs, tune_ms = g.kernel_times().get("autotune", (0, 0.0))
     g.close()
     timed_s = t1 - t0
     value = steps / timed_s if timed_s > 0 else 0.0
@@ -439,11 +441,11 @@ def main():
     bytes_launch = algorithmic_bytes(dom, N, P, E, M)
     achieved = bytes_launch / avg_s / 1e9
     families = {k: {"launches": v[0], "avg_us": round(1000.0 * v[1] / max(v[0], 1), 2),
-                    "share": round(v[1] / max(sum(x[1] for x in kt.values()), 1e-12), 4),
+                    "share": round(v[1] / max(sum(x[1] for k2, x in kt.items() if k2 != "autotune"), 1e-12), 4),
                     "algorithmic_bytes": algorithmic_bytes(k, N, P, E, M),
                     "achieved_GBs": (round(algorithmic_bytes(k, N, P, E, M) / (v[1] / 1000.0 / v[0]) / 1e9, 1)
                                      if algorithmic_bytes(k, N, P, E, M) and v[0] else None)}
-                for k, v in kt.items()}
+                for k, v in kt.items() if k != "autotune"}
     traffic = read_traffic().get(dom) if args.config == "c3" else None
     roofline = {"bound": "hbm", "kernel": dom, "kernels": FAMILY_KERNELS[dom], "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -532,6 +534,7 @@ def main():
         "np_init_s": round(s.init_seconds, 4),
         "host_us_per_trial": host,
         "launch_mode": args.launch_mode,
+        "autotune": {"regions": tuning[:4], "measurements": tune_runs, "host_ms": round(tune_ms, 2)},
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -330,6 +330,20 @@ class NesterovGPU:
         streams; 2: one by one on one stream.  The results do not depend on it."""
         _ok(_ffi.lib().gpl_set_launch_mode(self._h, mode), "gpl_set_launch_mode", self._h)
 
+    def set_tuning(self, region, chunk, table_bits, stage):
+        """Forces region's kernel variants (scatter chunk 256/512, LDS sum width 32/64, gradient
+        staging 0/1) and turns the autotuner off; region -1: the tuner on (chunk 1) or off (0)."""
+        _ok(_ffi.lib().gpl_set_tuning(self._h, region, chunk, table_bits, stage), "gpl_set_tuning", self._h)
+
+    def tuning(self):
+        """Per region: dict(chunk, table_bits, stage, source, scatter_us, gradient_us)."""
+        n = _ffi.lib().gpl_get_tuning(self._h, None, 0)
+        out = np.zeros(6 * max(n, 1), np.int32)
+        _ffi.lib().gpl_get_tuning(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), n)
+        src = {0: "default", 1: "measured", 2: "cached"}
+        return [dict(chunk=int(a), table_bits=int(b), stage=int(c), source=src.get(int(d), str(d)), scatter_us=e / 100.0,
+                     gradient_us=f / 100.0) for a, b, c, d, e, f in out[:6 * n].reshape(-1, 6)]
+
     def kernel_times(self):
         arr = (KernelTime * 16)()
         n = _ffi.lib().gpl_kernel_times(self._h, arr, 16)

@@ -30,6 +30,7 @@ struct RegionDev {
     int32_t narrow;  // 1: the scatter's LDS tables may hold 32-bit sums (kPlaceChunk x the largest bin area < 2^32)
     int32_t edges_rounded;  // 1: bin_x[i] = lx + round(i size_x) and bin_y alike (init_bins' edges)
     int32_t grad_stage;     // the gradient stages a block's pin terms in LDS up to this many (at most kGradStage)
+    int32_t chunk;          // cells a scatter block takes at a time: 256 or kPlaceChunk (0: kPlaceChunk); the autotuner's pick
     // bins
     const int32_t* bin_x;  // cnt_x + 1
     const int32_t* bin_y;  // cnt_y + 1

@@ -132,8 +132,6 @@ __global__ void k_zero_bins(unsigned long long* a, unsigned long long* b, size_t
 // an overlap that finds no slot within kProbe probes goes straight to global memory.  The sums
 // are of integers: the result does not depend on the path.
 enum PlaceMode { PLACE_INIT = 0, PLACE_CENTER = 1, PLACE_STORED = 2 };
-constexpr int kCellsPerThread = kPlaceChunk / kBlock;
-constexpr int kChunk = kPlaceChunk;
 constexpr int kSlots = 2048;
 constexpr int kProbe = 16;
 constexpr unsigned kEmpty = 0xffffffffu;
@@ -224,8 +222,11 @@ __global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c
 // widened evenly while it fits, and the overlaps outside it go straight to the bins in global
 // memory: between two re-sorts the storage order decays as cells move, and a few far cells
 // would otherwise send the whole chunk to the hash table.  Otherwise the hash table.
-template <int MODE, typename V>
+// CH: the cells a block takes at a time (a tile shape the autotuner picks per region: 256 or 512).
+template <int MODE, typename V, int CH>
 __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev c, PlaceArgs pa) {
+    constexpr int kChunk = CH;
+    constexpr int kCellsPerThread = CH / kBlock;
     // the trial's critical chain (scatter -> bins -> solve) shares the CUs with the WA kernels on
     // the side streams: its waves take the SIMDs' issue slots first
     __builtin_amdgcn_s_setprio(2);
@@ -1910,23 +1911,27 @@ void launch_zero_bins(hipStream_t s, const RegionDev& r) {
     hipLaunchKernelGGL(k_zero_bins, dim3(grid_for(M)), dim3(kBlock), 0, s, r.inst, r.filler, M);
 }
 
+template <int MODE, int CH>
+static void place_scatter_ch(hipStream_t s, const RegionDev& r, const CellsDev& c, const PlaceArgs& pa, int grid) {
+    if (r.narrow) hipLaunchKernelGGL((k_place_scatter<MODE, unsigned, CH>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+    else hipLaunchKernelGGL((k_place_scatter<MODE, unsigned long long, CH>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
+}
+
+template <int MODE>
+static void place_scatter_mode(hipStream_t s, const RegionDev& r, const CellsDev& c, const PlaceArgs& pa) {
+    const int n = r.cell_end - r.cell_begin;
+    const int ch = r.chunk == 256 ? 256 : kPlaceChunk;
+    const int chunks = (n + ch - 1) / ch;
+    const int grid = round_xcds(chunks < 4096 ? chunks : 4096);
+    if (ch == 256) place_scatter_ch<MODE, 256>(s, r, c, pa, grid);
+    else place_scatter_ch<MODE, kPlaceChunk>(s, r, c, pa, grid);
+}
+
 static void place_scatter(hipStream_t s, int mode, const RegionDev& r, const CellsDev& c, const PlaceArgs& pa) {
-    int n = r.cell_end - r.cell_begin;
-    if (n <= 0) return;
-    int chunks = (n + kChunk - 1) / kChunk;
-    int grid = round_xcds(chunks < 4096 ? chunks : 4096);
-    using N = unsigned;
-    using W = unsigned long long;
-    if (mode == PLACE_INIT) {
-        if (r.narrow) hipLaunchKernelGGL((k_place_scatter<PLACE_INIT, N>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
-        else hipLaunchKernelGGL((k_place_scatter<PLACE_INIT, W>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
-    } else if (mode == PLACE_CENTER) {
-        if (r.narrow) hipLaunchKernelGGL((k_place_scatter<PLACE_CENTER, N>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
-        else hipLaunchKernelGGL((k_place_scatter<PLACE_CENTER, W>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
-    } else {
-        if (r.narrow) hipLaunchKernelGGL((k_place_scatter<PLACE_STORED, N>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
-        else hipLaunchKernelGGL((k_place_scatter<PLACE_STORED, W>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
-    }
+    if (r.cell_end <= r.cell_begin) return;
+    if (mode == PLACE_INIT) place_scatter_mode<PLACE_INIT>(s, r, c, pa);
+    else if (mode == PLACE_CENTER) place_scatter_mode<PLACE_CENTER>(s, r, c, pa);
+    else place_scatter_mode<PLACE_STORED>(s, r, c, pa);
 }
 
 void launch_center_init(hipStream_t s, const RegionDev& r, const CellsDev& c, float2* cur_slp, float2* prev_slp, float2* cur,
@@ -2517,8 +2522,11 @@ int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap) {
     std::vector<hipGraphNode_t> nodes(n);
     if (n && hipGraphGetNodes(g, nodes.data(), &n) != hipSuccess) return -1;
     const void* f_update = reinterpret_cast<const void*>(k_place_update);
-    const void* f_scatter = reinterpret_cast<const void*>(k_place_scatter<PLACE_STORED, unsigned>);
-    const void* f_scatter_w = reinterpret_cast<const void*>(k_place_scatter<PLACE_STORED, unsigned long long>);
+    const void* f_scatters[] = {reinterpret_cast<const void*>(k_place_scatter<PLACE_STORED, unsigned, kPlaceChunk>),
+                                reinterpret_cast<const void*>(k_place_scatter<PLACE_STORED, unsigned long long, kPlaceChunk>),
+                                reinterpret_cast<const void*>(k_place_scatter<PLACE_STORED, unsigned, 256>),
+                                reinterpret_cast<const void*>(k_place_scatter<PLACE_STORED, unsigned long long, 256>)};
+    auto is_scatter = [&](const void* f) { return f == f_scatters[0] || f == f_scatters[1] || f == f_scatters[2] || f == f_scatters[3]; };
     const void* f_small = reinterpret_cast<const void*>(k_wa_tiles<true>);
     const void* f_big = reinterpret_cast<const void*>(k_wa_big<true>);
     const void* f_grad = reinterpret_cast<const void*>(k_gradient);
@@ -2529,7 +2537,7 @@ int find_trial_nodes(hipGraph_t g, TrialNode* out, int cap) {
         hipKernelNodeParams p{};
         if (hipGraphKernelNodeGetParams(nd, &p) != hipSuccess) return -1;
         int kind = p.func == f_update                               ? TK_UPDATE
-                   : (p.func == f_scatter || p.func == f_scatter_w) ? TK_SCATTER
+                   : is_scatter(p.func)                             ? TK_SCATTER
                    : p.func == f_small                              ? TK_WA_SMALL
                    : p.func == f_big                                ? TK_WA_BIG
                    : p.func == f_grad                               ? TK_GRADIENT

@@ -13,8 +13,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "host.h"
@@ -56,7 +59,29 @@ struct Region {
     bool max_phi_coef_changed = false, is_converged = false, is_diverged = false, reprint_header = false;
     int64_t nia = 0;  // nesterov_insts_area
     int32_t edges_rounded = 0;  // dev.edges_rounded as found at construction
+    int32_t narrow_ok = 0;      // 32-bit scatter sums are exact for this region's bins
 };
+
+// The autotuner's pick for one region (kernel variants with bit-identical results: the scatter's
+// chunk of cells per block and its LDS sum width, the gradient's LDS staging of the pin terms) and
+// where it came from: 0 the defaults or a caller's choice, 1 measured here, 2 a cached measurement
+// of the same shape.
+struct Tune {
+    int32_t chunk = gplk::kPlaceChunk, narrow = 0, stage = 1, source = 0;
+    double scatter_us = 0.0, gradient_us = 0.0;
+    bool decayed = false;  // the scatter measured on a decayed storage order (just before a re-sort)
+};
+
+// Measured picks by problem shape (cells, bins, pins of the region), shared by every context of the
+// process so that a shape is measured once.
+struct TuneKey {
+    int32_t n, cnt_x, cnt_y, pins;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(n, cnt_x, cnt_y, pins) < std::tie(o.n, o.cnt_x, o.cnt_y, o.pins);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, Tune> g_tune_cache;
 
 // Device time per kernel family (profiling mode).
 enum Fam { F_UPDATE, F_SCATTER, F_BINS, F_DCT, F_WA, F_GRAD, F_REDUCE, F_COUNT };
@@ -158,6 +183,12 @@ struct gpl_ctx {
     // reported in iterations inject_diverge_at / inject_diverge_at2 (the min-HPWL snapshot revert);
     // -1: off.  iter_now is the iteration being run.
     int32_t inject_zero_wl_at = -1, inject_diverge_at = -1, inject_diverge_at2 = -1, iter_now = -1;
+    // the autotuner: 1 measure at the loop's init and again after every re-sort, 0 keep the current
+    // choices (the defaults, or those gpl_set_tuning forced)
+    int autotune = 1;
+    std::vector<Tune> tune;
+    double tune_ms = 0.0;
+    int32_t tune_runs = 0;
     // profiling
     bool profiling = false;
     hipEvent_t ev[64];
@@ -921,6 +952,7 @@ struct gpl_ctx {
         last_global_adds = ga;
         if (iter == last_resort_iter + 1 || adds_base == 0.0) adds_base = rate;
         if (rate > 1.5 * adds_base && iter - last_resort_iter >= 50) {
+            autotune_before_resort();
             resort();
             last_resort_iter = iter;
         }
@@ -1053,6 +1085,146 @@ struct gpl_ctx {
     }
 
     void lock_region(int b);
+
+    // ---- the autotuner ---------------------------------------------------------------------------
+    void apply_tune(int b) {
+        Region& r = rg[b];
+        const Tune& t = tune[b];
+        r.dev.chunk = t.chunk;
+        r.dev.narrow = t.narrow && r.narrow_ok ? 1 : 0;
+        r.dev.grad_stage = t.stage ? gplk::kGradStage : 0;
+    }
+
+    // The shortest of `reps` device timings of f (HIP events on the stream), in microseconds; `after`
+    // runs untimed after each (the scatter's bins pass, which clears the accumulators).
+    template <typename F, typename A>
+    double time_us(F f, A after, int reps) {
+        hipEvent_t e0, e1;
+        check(hipEventCreate(&e0), "event");
+        check(hipEventCreate(&e1), "event");
+        double best = 1e30;
+        for (int k = 0; k < reps; ++k) {
+            check(hipEventRecord(e0, stream), "event");
+            f();
+            check(hipEventRecord(e1, stream), "event");
+            after();
+            check(hipEventSynchronize(e1), "sync");
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, e0, e1);
+            best = std::min(best, 1000.0 * (double)ms);
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return best;
+    }
+
+    // The scatter's variants timed on the boxes the cells hold now, then the bins pass (clearing the
+    // accumulators; the density it recomputes is that of the same boxes): the fastest is kept.
+    void tune_scatter(int b, int reps) {
+        Region& r = rg[b];
+        if (r.n <= 0) return;
+        Tune& t = tune[b];
+        const Tune before = t;
+        double best = 1e30;
+        for (int chunk : {gplk::kPlaceChunk, 256})
+            for (int nar : {1, 0}) {
+                if (nar && !r.narrow_ok) continue;
+                r.dev.chunk = chunk;
+                r.dev.narrow = nar;
+                const double us = time_us([&] { gplk::launch_density_scatter(stream, r.dev, cells, nullptr); },
+                                          [&] { gplk::launch_bin_density(stream, r.dev, r.partials); }, reps);
+                if (us < best) {
+                    best = us;
+                    t.chunk = chunk;
+                    t.narrow = nar;
+                }
+            }
+        t.scatter_us = best;
+        t.source = 1;
+        apply_tune(b);
+        if (t.chunk != before.chunk || t.narrow != before.narrow) drop_graph();  // the graph holds the old variant
+    }
+
+    // The gradient with and without its LDS staging of the pin terms, into the next slot (free
+    // before np_init and between trials).
+    void tune_gradient(int b) {
+        Region& r = rg[b];
+        if (r.n <= 0) return;
+        Tune& t = tune[b];
+        double best = 1e30;
+        for (int stage : {1, 0}) {
+            r.dev.grad_stage = stage ? gplk::kGradStage : 0;
+            const double us = time_us(
+                [&] {
+                    gplk::launch_gradient(stream, r.dev, cells, nets, wl_coef_x, wl_coef_y, r.density_penalty, nullptr, nullptr,
+                                          r.v[S_NEXT_SLP_SUM], nullptr, nullptr, nullptr, r.partials, 0);
+                },
+                [] {}, 3);
+            if (us < best) {
+                best = us;
+                t.stage = stage;
+            }
+        }
+        t.gradient_us = best;
+        apply_tune(b);
+    }
+
+    // Before np_init: every region measured on the initial boxes (or its shape's cached pick taken).
+    void autotune_init() {
+        if (!autotune) return;
+        const auto t0 = Clock::now();
+        join();
+        bool any = false;
+        for (size_t b = 0; b < rg.size(); ++b) {
+            Region& r = rg[b];
+            const TuneKey key{r.n, r.R.cnt_x, r.R.cnt_y, net_pin_count};
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune_cache.find(key);
+            if (it != g_tune_cache.end()) {
+                tune[b] = it->second;
+                tune[b].source = 2;
+                apply_tune(int(b));
+                continue;
+            }
+            any = true;
+            tune_scatter(int(b), 3);
+            // a field and pin terms for the gradient's timing
+            gplk::launch_density_scatter(stream, r.dev, cells, nullptr);
+            gplk::launch_bin_density(stream, r.dev, r.partials);
+            gplk::launch_solve(stream, r.dev, r.sb, false);
+            wa_forward();
+            join();
+            tune_gradient(int(b));
+            g_tune_cache[key] = tune[b];
+        }
+        check(hipStreamSynchronize(stream), "sync");
+        if (any) ++tune_runs;
+        tune_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+        drop_graph();
+    }
+
+    // Just before a shape's first re-sort the storage order is at its most decayed, where the
+    // scatter's variants differ most: they are measured there once and that pick is kept (and
+    // cached for the shape, so later contexts start with it).
+    void autotune_before_resort() {
+        if (!autotune) return;
+        bool need = false;
+        for (size_t b = 0; b < rg.size(); ++b) need = need || (!rg[b].is_converged && !tune[b].decayed);
+        if (!need) return;
+        const auto t0 = Clock::now();
+        join();
+        for (size_t b = 0; b < rg.size(); ++b) {
+            Region& r = rg[b];
+            if (r.is_converged || tune[b].decayed) continue;
+            tune_scatter(int(b), 2);
+            tune[b].decayed = true;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune_cache[TuneKey{r.n, r.R.cnt_x, r.R.cnt_y, net_pin_count}] = tune[b];
+        }
+        check(hipStreamSynchronize(stream), "sync");
+        ++tune_runs;
+        tune_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    }
 
     // checkConvergence (nesterov.rs:3107).
     bool check_convergence(int b, int iter) {
@@ -1504,6 +1676,8 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
                 for (int32_t x = 0; x < r.R.cnt_x; ++x) wmax = std::max<int64_t>(wmax, int64_t(r.R.bin_x[x + 1]) - r.R.bin_x[x]);
                 for (int32_t y = 0; y < r.R.cnt_y; ++y) hmax = std::max<int64_t>(hmax, int64_t(r.R.bin_y[y + 1]) - r.R.bin_y[y]);
                 d.narrow = (double)gplk::kPlaceChunk * (double)wmax * (double)hmax < 4294967296.0 ? 1 : 0;
+                r.narrow_ok = d.narrow;
+                d.chunk = gplk::kPlaceChunk;
             }
             {
                 // the kernels compute the edges themselves when the tables are init_bins' own
@@ -1555,6 +1729,8 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             check(hipMemsetAsync(r.partials, 0, sizeof(double) * 6 * gplk::kPartials, c->stream), "memset");
             gplk::launch_zero_bins(c->stream, d);
         }
+        c->tune.assign(p->num_regions, Tune{});
+        for (int32_t b = 0; b < p->num_regions; ++b) c->tune[b].narrow = c->rg[b].narrow_ok;
         // one Sums per region, then one whose hpwl field is the HPWL counter the WA kernels add into
         c->d_sums = c->dalloc<gplk::Sums>(size_t(p->num_regions) + 1);
         check(hipMemsetAsync(c->d_sums, 0, sizeof(gplk::Sums) * (size_t(p->num_regions) + 1), c->stream), "memset");
@@ -1613,6 +1789,7 @@ const char* gpl_last_error(const gpl_ctx* c) { return c ? c->err.c_str() : "null
 int32_t gpl_init_nesterov_loop(gpl_ctx* c) {
     if (!c) return GPL_ERR_ARG;
     try {
+        c->autotune_init();  // kernel variants picked per region (untimed: part of the set-up)
         auto t0 = Clock::now();
         int64_t design_hpwl = c->hpwl_now();
         float user = c->o.reference_hpwl;
@@ -1970,16 +2147,64 @@ int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter) {
     if (kind == 0) c->inject_zero_wl_at = c->inject_diverge_at = c->inject_diverge_at2 = -1;
     if (kind == 0 || kind == 4) {
         // kind 4: the kernels' general paths (bin edges from the tables, the gradient's per-cell
-        // gathers of pin terms); they give the same results as the fast ones
-        for (auto& r : c->rg) {
+        // gathers of pin terms); they give the same results as the fast ones.  The autotuner then
+        // keeps its hands off the gradient's staging.
+        for (size_t b = 0; b < c->rg.size(); ++b) {
+            Region& r = c->rg[b];
             r.dev.edges_rounded = kind == 4 ? 0 : r.edges_rounded;
-            r.dev.grad_stage = kind == 4 ? 0 : gplk::kGradStage;
+            if (kind == 4) c->tune[b].stage = 0;
+            c->apply_tune(int(b));
         }
-        if (kind == 4) return GPL_OK;
+        if (kind == 4) {
+            c->autotune = 0;
+            c->drop_graph();
+            return GPL_OK;
+        }
     }
     c->inject_kind = kind;
     c->inject_after = kind == 0 ? -1 : after_iter;
     return GPL_OK;
+}
+
+int32_t gpl_set_tuning(gpl_ctx* c, int32_t region, int32_t chunk, int32_t table_bits, int32_t stage) {
+    if (!c) return GPL_ERR_ARG;
+    if (region < 0) {
+        c->autotune = chunk != 0 ? 1 : 0;  // region -1: the tuner on (chunk 1) or off (chunk 0)
+        return GPL_OK;
+    }
+    if (bad_region(c, region) || (chunk != 256 && chunk != gplk::kPlaceChunk) || (table_bits != 32 && table_bits != 64) ||
+        (stage != 0 && stage != 1))
+        return GPL_ERR_ARG;
+    try {
+        c->join();
+        check(hipStreamSynchronize(c->stream), "sync");
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    Tune& t = c->tune[region];
+    t.chunk = chunk;
+    t.narrow = table_bits == 32 ? 1 : 0;
+    t.stage = stage;
+    t.source = 0;
+    c->apply_tune(region);
+    c->autotune = 0;
+    c->drop_graph();
+    return GPL_OK;
+}
+
+int32_t gpl_get_tuning(gpl_ctx* c, int32_t* out, int32_t cap) {
+    if (!c) return GPL_ERR_ARG;
+    const int32_t n = int32_t(c->rg.size());
+    for (int32_t b = 0; b < n && out && b < cap; ++b) {
+        const Region& r = c->rg[b];
+        out[6 * b] = r.dev.chunk;
+        out[6 * b + 1] = r.dev.narrow ? 32 : 64;
+        out[6 * b + 2] = r.dev.grad_stage ? 1 : 0;
+        out[6 * b + 3] = c->tune[b].source;
+        out[6 * b + 4] = int32_t(std::lround(100.0 * c->tune[b].scatter_us));
+        out[6 * b + 5] = int32_t(std::lround(100.0 * c->tune[b].gradient_us));
+    }
+    return n;
 }
 
 int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode) {
@@ -2037,6 +2262,14 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap) {
             out[n].ms = double(ga);
             ++n;
         }
+    }
+    if (n < cap) {
+        // the autotuner's measurements: runs in `launches`, their host time in `ms`
+        std::memset(&out[n], 0, sizeof out[n]);
+        std::snprintf(out[n].name, sizeof out[n].name, "autotune");
+        out[n].launches = c->tune_runs;
+        out[n].ms = c->tune_ms;
+        ++n;
     }
     for (int h = 0; h < gpl_ctx::H_COUNT && n < cap && c->host_n > 0; ++h, ++n) {
         std::memset(&out[n], 0, sizeof out[n]);

@@ -451,6 +451,20 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap);
  * beside the WA kernels, the high-fanout nets on a third; 1 the same kernels launched one by one
  * on those streams; 2 one by one on one stream; 3 one HIP graph per trial captured on one stream. */
 int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode);
+
+/* The autotuner (no reference counterpart; north star: tile shapes picked by an autotuner the project
+ * writes).  Per region it times the kernel variants whose results are bit-identical -- the scatter's
+ * chunk of cells per block (256 or 512) and LDS sum width (32 bits where the bins allow, or 64), the
+ * gradient's LDS staging of the pin terms (on or off) -- at gpl_init_nesterov_loop (before np_init,
+ * outside its timing) and the scatter's again just before the shape's first re-sort (the storage
+ * order at its most decayed, where the variants differ most), keeps the fastest, and caches the pick
+ * per shape (cells, bin grid, pins) for the process.  gpl_set_tuning forces a region's
+ * variant (chunk 256 or 512, table_bits 32 or 64, stage 0 or 1) and turns the tuner off; region -1
+ * turns it on (chunk 1) or off (chunk 0).  gpl_get_tuning: per region 6 int32 -- chunk, table bits,
+ * stage, source (0 default or forced, 1 measured, 2 cached), the scatter's and the gradient's
+ * measured time in 1/100 us; returns the number of regions. */
+int32_t gpl_set_tuning(gpl_ctx* c, int32_t region, int32_t chunk, int32_t table_bits, int32_t stage);
+int32_t gpl_get_tuning(gpl_ctx* c, int32_t* out, int32_t cap);
 int32_t gpl_set_profiling(gpl_ctx* c, int32_t on);
 
 #ifdef __cplusplus

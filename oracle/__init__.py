@@ -84,6 +84,7 @@ def lib():
             ("orc_momentum", C.c_float, [C.c_float, f32p]),
             ("orc_set_inject_control", None, [C.c_void_p, C.c_int32, C.c_int32]),
             ("orc_region_rows", C.c_int32, [C.c_void_p, f32p, C.c_int32]),
+            ("orc_set_field_noise", None, [C.c_void_p, C.c_float, C.c_uint64]),
         ]
         for name, res, args in sig:
             fn = getattr(L, name)
@@ -234,11 +235,16 @@ class Oracle:
 
     def region_rows(self):
         """Every region's state after each iteration: rows (iter, region, overflow, penalty, step,
-        converged)."""
+        converged, WA gradient sum, density gradient sum)."""
         n = lib().orc_region_rows(self._h, None, 0)
-        a = np.zeros(6 * max(n, 1), np.float32)
+        a = np.zeros(8 * max(n, 1), np.float32)
         lib().orc_region_rows(self._h, f32(a), n)
-        return a[:6 * n].reshape(-1, 6)
+        return a[:8 * n].reshape(-1, 8)
+
+    def set_field_noise(self, eps, seed=1):
+        """The sensitivity probe: every field value of every solve scaled by (1 + eps u), u uniform
+        in [-1, 1); eps = 0 turns it off."""
+        lib().orc_set_field_noise(self._h, eps, seed)
 
     def set_inject_control(self, kind, at):
         """Kind 5: the WA gradient sum read as zero once at iteration `at`; kind 6: a divergence

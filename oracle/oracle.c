@@ -242,7 +242,8 @@ typedef struct orc_np {
     double t_init, t_loop, t_converge;
     gpl_iter_row* hist;
     int32_t nhist, cap_hist;
-    float* rrows; /* per iteration and region: iter, region, sum_overflow_unscaled, density_penalty, step, converged */
+    float* rrows; /* per iteration and region: iter, region, sum_overflow_unscaled, density_penalty, step, converged,
+                   * wl_grad_sum, density_grad_sum */
     int32_t nrrows, cap_rrows;
     int32_t inject_nan_after;
     /* host-forced cases of the loop control (the GPU's gpl_debug_inject kinds 5 and 6): the WA
@@ -250,6 +251,11 @@ typedef struct orc_np {
      * divergence reported at iterations inject_diverge_at / inject_diverge_at2 (the min-HPWL
      * snapshot revert); -1: off */
     int32_t inject_zero_wl_at, inject_diverge_at, inject_diverge_at2, iter_now;
+    /* a sensitivity probe: every field value of every solve scaled by (1 + field_noise u), u uniform
+     * in [-1, 1) from a xorshift stream (0: off) -- rounding-level perturbations of the size the
+     * GPU's transforms differ by, to measure how far the loop's outcome moves under them */
+    float field_noise;
+    uint64_t noise_state;
 } orc_np;
 
 static void* xcalloc(size_t n, size_t s) {
@@ -522,7 +528,16 @@ static void update_center_location(orc_np* np, orc_nb* nb, const fpt* c) {
 }
 
 /* nesterov.rs:2279 updateDensityFieldBin. */
-static void update_density_field_bin(orc_nb* nb) {
+static float noise_factor(orc_np* np) {
+    uint64_t x = np->noise_state;
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    np->noise_state = x;
+    return 1.0f + np->field_noise * ((float)(x & 0xffffu) / 32768.0f - 1.0f);
+}
+
+static void update_density_field_bin(orc_np* np, orc_nb* nb) {
     int nx = nb->R.cnt_x, ny = nb->R.cnt_y;
     float *a = nb->fft_a, *ph = nb->fft_b, *fx = nb->fft_c, *fy = nb->fft_d;
     for (int y = 0; y < ny; ++y)
@@ -534,7 +549,17 @@ static void update_density_field_bin(orc_nb* nb) {
             nb->field_x[d] = fx[s];
             nb->field_y[d] = fy[s];
             nb->phi[d] = ph[s];
+            if (np && np->field_noise != 0.0f) {
+                nb->field_x[d] *= noise_factor(np);
+                nb->field_y[d] *= noise_factor(np);
+            }
         }
+}
+
+/* Turns the sensitivity probe on (eps > 0, stream seeded with `seed`) or off (eps = 0). */
+void orc_set_field_noise(orc_np* np, float eps, uint64_t seed) {
+    np->field_noise = eps;
+    np->noise_state = seed ? seed : 88172645463325252ull;
 }
 
 /* nesterov.rs:2307 GNet::updateBox. */
@@ -763,7 +788,7 @@ static void init_density1(orc_np* np, int b) {
     }
     update_center_location(np, nb, nb->v[S_CUR_SLP]);
     nb->prev_hpwl = orc_np_hpwl(np);
-    update_density_field_bin(nb);
+    update_density_field_bin(np, nb);
     nb->base_wl_coef_nb = (float)((double)np->o.init_wirelength_coef / ((double)(float)(nb->R.size_x + nb->R.size_y) * 0.5));
     float nia = (float)nesterov_insts_area(nb);
     nb->sum_overflow = (float)nb->overflow / nia;
@@ -837,7 +862,7 @@ static int32_t np_init(orc_np* np) {
         np_update_gradient(np, b, SLOT_CUR);
         update_initial_prev_slp(np, b);
         update_center_location(np, nb, nb->v[S_PREV_SLP]);
-        update_density_field_bin(nb);
+        update_density_field_bin(np, nb);
     }
     update_wirelength_force_wa(np);
     for (int b = 0; b < np->nregions; ++b) np_update_gradient(np, b, SLOT_PREV);
@@ -911,7 +936,7 @@ static void nesterov_update_coordinates(orc_np* np, int b, float coeff) {
     free(locked);
     free(dd);
     update_center_location(np, nb, nb->v[S_NEXT_SLP]);
-    update_density_field_bin(nb);
+    update_density_field_bin(np, nb);
 }
 
 /* nesterov.rs:2750-2772 nesterovUpdateStepLength on given points: the new step is the RMS distance
@@ -1075,7 +1100,7 @@ static void revert_to_snapshot(orc_np* np, orc_nb* nb) {
     nb->density_penalty = nb->snapshot_density_penalty;
     nb->step_length = nb->snapshot_step_length;
     update_center_location(np, nb, nb->v[S_CUR]);
-    update_density_field_bin(nb);
+    update_density_field_bin(np, nb);
     nb->is_diverged = 0;
     nb->prev_reported_hpwl = 0;
     nb->prev_reported_overflow_unscaled = 0.0f;
@@ -1179,23 +1204,25 @@ static void push_history(orc_np* np, int iter) {
     for (int b = 0; b < np->nregions; ++b) {
         if (np->nrrows == np->cap_rrows) {
             np->cap_rrows = np->cap_rrows ? 2 * np->cap_rrows : 4096;
-            np->rrows = (float*)realloc(np->rrows, sizeof(float) * 6 * (size_t)np->cap_rrows);
+            np->rrows = (float*)realloc(np->rrows, sizeof(float) * 8 * (size_t)np->cap_rrows);
         }
-        float* q = np->rrows + 6 * (size_t)np->nrrows++;
+        float* q = np->rrows + 8 * (size_t)np->nrrows++;
         q[0] = (float)iter;
         q[1] = (float)b;
         q[2] = np->nbs[b].sum_overflow_unscaled;
         q[3] = np->nbs[b].density_penalty;
         q[4] = np->nbs[b].step_length;
         q[5] = (float)np->nbs[b].is_converged;
+        q[6] = np->nbs[b].wl_grad_sum;
+        q[7] = np->nbs[b].density_grad_sum;
     }
 }
 
-/* Every region's state after each iteration (6 floats a row: iteration, region, overflow, penalty,
- * step length, converged); returns the number of rows. */
+/* Every region's state after each iteration (8 floats a row: iteration, region, overflow, penalty,
+ * step length, converged, WA gradient sum, density gradient sum); returns the number of rows. */
 int32_t orc_region_rows(orc_np* np, float* out, int32_t cap) {
     int32_t n = np->nrrows < cap ? np->nrrows : cap;
-    if (out && n > 0) memcpy(out, np->rrows, sizeof(float) * 6 * (size_t)n);
+    if (out && n > 0) memcpy(out, np->rrows, sizeof(float) * 8 * (size_t)n);
     return np->nrrows;
 }
 

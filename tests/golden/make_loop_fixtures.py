@@ -7,6 +7,12 @@ final HPWL, iteration, overflow, the first progress rows, the CPU time, and a ha
 generated problem (the GPU test checks that it places the same problem).
 
     python tests/golden/make_loop_fixtures.py c2 c3
+    python tests/golden/make_loop_fixtures.py --perturb 3 c4r   # adds the sensitivity runs
+
+--perturb N adds N more oracle runs of the same problem with every field value scaled by
+(1 + 2e-7 u), u uniform in [-1, 1) (the oracle's sensitivity probe: the size by which the GPU's
+transforms differ from the oracle's), recording how far the loop's outcome moves under
+rounding-level perturbations.
 """
 import json
 import os
@@ -34,6 +40,32 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor() or platform.machine()
+
+
+def perturbed(name, n, eps=2e-7, seed=1):
+    import gpl_amd as G
+    import oracle as O
+    d = G.Design.from_config(name, seed=seed)
+    d.initial_place()
+    d.init_nesterov_place()
+    out = []
+    for k in range(1, n + 1):
+        o = O.Oracle(d.problem())
+        o.set_field_noise(eps, k)
+        o.init()
+        rc, it = o.run()
+        s = o.stats()
+        out.append(dict(field_noise=eps, noise_seed=k, status=int(rc), converged=bool(s.converged), final_iter=int(it),
+                        final_hpwl=int(s.hpwl), final_overflow=float(s.overflow)))
+        print(f"{name} perturbed {k}: iter {it} hpwl {s.hpwl}", flush=True)
+        o.close()
+    d.close()
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"loop_{name}.json")
+    with open(path) as f:
+        fx = json.load(f)
+    fx["perturbed"] = out
+    with open(path, "w") as f:
+        json.dump(fx, f, indent=1)
 
 
 def run(name, seed=1):
@@ -75,5 +107,11 @@ def run(name, seed=1):
 
 
 if __name__ == "__main__":
-    for n in sys.argv[1:] or ["c2", "c3"]:
-        run(n)
+    args = sys.argv[1:]
+    if args[:1] == ["--perturb"]:
+        k = int(args[1])
+        for n in args[2:]:
+            perturbed(n, k)
+    else:
+        for n in args or ["c2", "c3"]:
+            run(n)
