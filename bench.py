@@ -366,9 +366,13 @@ def main():
     # ---- the workload, built on the host (untimed) ----
     t = time.perf_counter()
     d = G.Design.from_config(args.config)
-    d.initial_place()
+    t_gen = time.perf_counter()
+    d.initial_place(device=True)  # the B2B rounds on the GPU: the host's placement bit for bit
+    t_ip = time.perf_counter()
     d.init_nesterov_place()
     setup_s = time.perf_counter() - t
+    setup_split = {"generate_s": round(t_gen - t, 2), "initial_place_gpu_s": round(t_ip - t_gen, 2),
+                   "init_nesterov_s": round(time.perf_counter() - t_ip, 2)}
     p = d.problem()
     N, P, E = p.num_cells, p.num_pins, p.num_nets
     M = sum(p.regions[b].cnt_x * p.regions[b].cnt_y for b in range(p.num_regions))
@@ -536,6 +540,7 @@ def main():
         "launch_mode": args.launch_mode,
         "autotune": {"regions": tuning[:4], "measurements": tune_runs, "host_ms": round(tune_ms, 2)},
         "setup_s": round(setup_s, 2),
+        "setup": setup_split,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu and cpu["value"] > 0 else None,

@@ -133,8 +133,13 @@ class Design:
         o.update(opt_over)
         return cls(gen_config(name, seed=seed, **(gen_over or {})), default_opts(**o))
 
-    def initial_place(self):
-        _ok(_ffi.lib().gpl_design_initial_place(self._h), "gpl_design_initial_place")
+    def initial_place(self, device=False):
+        """B2B + BiCGSTAB initial placement; device=True runs its rounds on the GPU (the same
+        placement bit for bit)."""
+        if device:
+            _ok(_ffi.lib().gpl_design_initial_place_gpu(self._h), "gpl_design_initial_place_gpu")
+        else:
+            _ok(_ffi.lib().gpl_design_initial_place(self._h), "gpl_design_initial_place")
 
     def init_nesterov_place(self, opts=None):
         if opts is not None:

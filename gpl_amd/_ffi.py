@@ -119,6 +119,7 @@ SIGNATURES = [
     ("gpl_design_generate", i32, [P(GenConfig), P(Opts), P(C.c_void_p)]),
     ("gpl_design_destroy", None, [C.c_void_p]),
     ("gpl_design_initial_place", i32, [C.c_void_p]),
+    ("gpl_design_initial_place_gpu", i32, [C.c_void_p]),
     ("gpl_design_get_info", i32, [C.c_void_p, P(DesignInfo)]),
     ("gpl_init_nesterov_place", i32, [C.c_void_p, P(Opts)]),
     ("gpl_design_problem", i32, [C.c_void_p, P(Problem)]),

@@ -782,15 +782,9 @@ double dot(const std::vector<float>& a, const std::vector<float>& b) {
     return s;
 }
 
-struct Solve {
-    int32_t iterations;
-    float error;
-    bool converged;
-};
-
 // BiCGSTAB with the identity preconditioner and the restart rule of bicgstab.rs:148-197; the
 // vectors are f32 as there, the dot products FP64 (the reference sums in Eigen's packet order).
-Solve bicgstab(const Csr& mat, const std::vector<float>& rhs, std::vector<float>& x, int32_t max_iters) {
+IpSolve bicgstab(const Csr& mat, const std::vector<float>& rhs, std::vector<float>& x, int32_t max_iters) {
     const double tol = 1.1920928955078125e-07;
     const size_t n = size_t(mat.n);
     std::vector<float> r(n), r0, v(n, 0.0f), p(n, 0.0f), s(n), t(n), mx;
@@ -800,7 +794,7 @@ Solve bicgstab(const Csr& mat, const std::vector<float>& rhs, std::vector<float>
     double r0_sqnorm = dot(r0, r0), rhs_sqnorm = dot(rhs, rhs);
     if (rhs_sqnorm == 0.0) {
         std::fill(x.begin(), x.end(), 0.0f);
-        return Solve{max_iters, float(tol), true};
+        return IpSolve{max_iters, float(tol), true};
     }
     double rho = 1.0, alpha = 1.0, w = 1.0;
     const double tol2 = tol * tol * rhs_sqnorm, eps2 = tol * tol;
@@ -838,7 +832,7 @@ Solve bicgstab(const Csr& mat, const std::vector<float>& rhs, std::vector<float>
         ++i;
     }
     float error = float(std::sqrt(dot(r, r) / rhs_sqnorm));
-    return Solve{i, error, error <= float(tol)};
+    return IpSolve{i, error, error <= float(tol)};
 }
 
 }  // namespace
@@ -868,7 +862,6 @@ int64_t db_hpwl(const gpl_design* g) {
 }
 
 int32_t initial_place(gpl_design* g, std::string* err) {
-    (void)err;
     const Design& d = g->d;
     Placer& pl = g->pl;
     const gpl_opts& o = g->opts;
@@ -897,109 +890,116 @@ int32_t initial_place(gpl_design* g, std::string* err) {
     for (int32_t e = 0; e < n; ++e) ext[pl.place[e]] = e;
     g->ip_rounds = 0;
     for (int32_t iter = 1; iter <= o.initial_place_max_iter; ++iter) {
-        // updatePinInfo (initial_place.rs:203): the first pin at each strict extreme
-        for (auto& p : pl.pins) p.min_x = p.max_x = p.min_y = p.max_y = false;
-        for (auto& net : pl.net_pins) {
-            int32_t lx = INT32_MAX, ly = INT32_MAX, ux = INT32_MIN, uy = INT32_MIN;
-            int32_t pminx = -1, pminy = -1, pmaxx = -1, pmaxy = -1;
-            for (int32_t k : net) {
-                Pin& p = pl.pins[k];
-                if (lx > p.cx) {
-                    if (pminx >= 0) pl.pins[pminx].min_x = false;
-                    lx = p.cx;
-                    pminx = k;
-                    p.min_x = true;
-                }
-                if (ux < p.cx) {
-                    if (pmaxx >= 0) pl.pins[pmaxx].max_x = false;
-                    ux = p.cx;
-                    pmaxx = k;
-                    p.max_x = true;
-                }
-                if (ly > p.cy) {
-                    if (pminy >= 0) pl.pins[pminy].min_y = false;
-                    ly = p.cy;
-                    pminy = k;
-                    p.min_y = true;
-                }
-                if (uy < p.cy) {
-                    if (pmaxy >= 0) pl.pins[pmaxy].max_y = false;
-                    uy = p.cy;
-                    pmaxy = k;
-                    p.max_y = true;
+        std::vector<float> x(size_t(n), 0.0f), y(size_t(n), 0.0f);
+        IpSolve ex{}, ey{};
+        if (g->ip_device) {
+            const int32_t rc = ip_round_device(g, ext, x, y, &ex, &ey, err);
+            if (rc != GPL_OK) return rc;
+        } else {
+            // updatePinInfo (initial_place.rs:203): the first pin at each strict extreme
+            for (auto& p : pl.pins) p.min_x = p.max_x = p.min_y = p.max_y = false;
+            for (auto& net : pl.net_pins) {
+                int32_t lx = INT32_MAX, ly = INT32_MAX, ux = INT32_MIN, uy = INT32_MIN;
+                int32_t pminx = -1, pminy = -1, pmaxx = -1, pmaxy = -1;
+                for (int32_t k : net) {
+                    Pin& p = pl.pins[k];
+                    if (lx > p.cx) {
+                        if (pminx >= 0) pl.pins[pminx].min_x = false;
+                        lx = p.cx;
+                        pminx = k;
+                        p.min_x = true;
+                    }
+                    if (ux < p.cx) {
+                        if (pmaxx >= 0) pl.pins[pmaxx].max_x = false;
+                        ux = p.cx;
+                        pmaxx = k;
+                        p.max_x = true;
+                    }
+                    if (ly > p.cy) {
+                        if (pminy >= 0) pl.pins[pminy].min_y = false;
+                        ly = p.cy;
+                        pminy = k;
+                        p.min_y = true;
+                    }
+                    if (uy < p.cy) {
+                        if (pmaxy >= 0) pl.pins[pmaxy].max_y = false;
+                        uy = p.cy;
+                        pmaxy = k;
+                        p.max_y = true;
+                    }
                 }
             }
-        }
-        // createSparseMatrix (initial_place.rs:279)
-        std::vector<Triplet> tx, ty;
-        std::vector<float> bx(size_t(n), 0.0f), by(size_t(n), 0.0f), x(size_t(n), 0.0f), y(size_t(n), 0.0f);
-        for (int32_t e = 0; e < n; ++e) {
-            const Instance& i = pl.insts[pl.place[e]];
-            x[e] = float(i.cx());
-            y[e] = float(i.cy());
-            if (i.locked) {
-                tx.push_back({e, e, 1.0f});
-                ty.push_back({e, e, 1.0f});
-                bx[e] = float(i.cx());
-                by[e] = float(i.cy());
+            // createSparseMatrix (initial_place.rs:279)
+            std::vector<Triplet> tx, ty;
+            std::vector<float> bx(size_t(n), 0.0f), by(size_t(n), 0.0f);
+            for (int32_t e = 0; e < n; ++e) {
+                const Instance& i = pl.insts[pl.place[e]];
+                x[e] = float(i.cx());
+                y[e] = float(i.cy());
+                if (i.locked) {
+                    tx.push_back({e, e, 1.0f});
+                    ty.push_back({e, e, 1.0f});
+                    bx[e] = float(i.cx());
+                    by[e] = float(i.cy());
+                }
             }
-        }
-        auto placed = [&](const Pin& p) -> int32_t { return (p.inst >= 0 && !pl.insts[p.inst].fixed) ? p.inst : -1; };
-        for (auto& net : pl.net_pins) {
-            if (net.size() <= 1 || (int32_t)net.size() >= o.initial_place_max_fanout) continue;
-            float net_weight = o.initial_place_net_weight_scale / float(net.size() - 1);
-            for (size_t i1 = 1; i1 < net.size(); ++i1) {
-                const Pin& p1 = pl.pins[net[i1]];
-                for (size_t i2 = 0; i2 < i1; ++i2) {
-                    const Pin& p2 = pl.pins[net[i2]];
-                    if (p1.inst == p2.inst) continue;
-                    for (int axis = 0; axis < 2; ++axis) {
-                        bool marked = axis == 0 ? (p1.min_x || p1.max_x || p2.min_x || p2.max_x) : (p1.min_y || p1.max_y || p2.min_y || p2.max_y);
-                        if (!marked) continue;
-                        int32_t c1 = axis == 0 ? p1.cx : p1.cy, c2 = axis == 0 ? p2.cx : p2.cy;
-                        auto& trip = axis == 0 ? tx : ty;
-                        auto& rhs = axis == 0 ? bx : by;
-                        auto center = [&](int32_t k) { return axis == 0 ? pl.insts[k].cx() : pl.insts[k].cy(); };
-                        int32_t diff = std::abs(c1 - c2);
-                        float weight = diff > o.initial_place_min_diff_length ? net_weight / float(diff) : net_weight / float(o.initial_place_min_diff_length);
-                        int32_t a = placed(p1), b = placed(p2);
-                        if (a >= 0 && b >= 0) {
-                            bool la = pl.insts[a].locked, lb = pl.insts[b].locked;
-                            if (!la && !lb) {
-                                int32_t e1 = ext[a], e2 = ext[b];
-                                trip.push_back({e1, e1, weight});
-                                trip.push_back({e2, e2, weight});
-                                trip.push_back({e1, e2, -weight});
-                                trip.push_back({e2, e1, -weight});
-                                rhs[e1] += -weight * float((c1 - center(a)) - (c2 - center(b)));
-                                rhs[e2] += -weight * float((c2 - center(b)) - (c1 - center(a)));
-                            } else if (la && !lb) {
+            auto placed = [&](const Pin& p) -> int32_t { return (p.inst >= 0 && !pl.insts[p.inst].fixed) ? p.inst : -1; };
+            for (auto& net : pl.net_pins) {
+                if (net.size() <= 1 || (int32_t)net.size() >= o.initial_place_max_fanout) continue;
+                float net_weight = o.initial_place_net_weight_scale / float(net.size() - 1);
+                for (size_t i1 = 1; i1 < net.size(); ++i1) {
+                    const Pin& p1 = pl.pins[net[i1]];
+                    for (size_t i2 = 0; i2 < i1; ++i2) {
+                        const Pin& p2 = pl.pins[net[i2]];
+                        if (p1.inst == p2.inst) continue;
+                        for (int axis = 0; axis < 2; ++axis) {
+                            bool marked = axis == 0 ? (p1.min_x || p1.max_x || p2.min_x || p2.max_x) : (p1.min_y || p1.max_y || p2.min_y || p2.max_y);
+                            if (!marked) continue;
+                            int32_t c1 = axis == 0 ? p1.cx : p1.cy, c2 = axis == 0 ? p2.cx : p2.cy;
+                            auto& trip = axis == 0 ? tx : ty;
+                            auto& rhs = axis == 0 ? bx : by;
+                            auto center = [&](int32_t k) { return axis == 0 ? pl.insts[k].cx() : pl.insts[k].cy(); };
+                            int32_t diff = std::abs(c1 - c2);
+                            float weight = diff > o.initial_place_min_diff_length ? net_weight / float(diff) : net_weight / float(o.initial_place_min_diff_length);
+                            int32_t a = placed(p1), b = placed(p2);
+                            if (a >= 0 && b >= 0) {
+                                bool la = pl.insts[a].locked, lb = pl.insts[b].locked;
+                                if (!la && !lb) {
+                                    int32_t e1 = ext[a], e2 = ext[b];
+                                    trip.push_back({e1, e1, weight});
+                                    trip.push_back({e2, e2, weight});
+                                    trip.push_back({e1, e2, -weight});
+                                    trip.push_back({e2, e1, -weight});
+                                    rhs[e1] += -weight * float((c1 - center(a)) - (c2 - center(b)));
+                                    rhs[e2] += -weight * float((c2 - center(b)) - (c1 - center(a)));
+                                } else if (la && !lb) {
+                                    int32_t e2 = ext[b];
+                                    trip.push_back({e2, e2, weight});
+                                    rhs[e2] += weight * float(c1 - (c2 - center(b)));
+                                } else if (!la && lb) {
+                                    int32_t e1 = ext[a];
+                                    trip.push_back({e1, e1, weight});
+                                    rhs[e1] += weight * float(c2 - (c1 - center(a)));
+                                }
+                            } else if (a < 0 && b >= 0) {
                                 int32_t e2 = ext[b];
                                 trip.push_back({e2, e2, weight});
                                 rhs[e2] += weight * float(c1 - (c2 - center(b)));
-                            } else if (!la && lb) {
+                            } else if (a >= 0 && b < 0) {
                                 int32_t e1 = ext[a];
                                 trip.push_back({e1, e1, weight});
                                 rhs[e1] += weight * float(c2 - (c1 - center(a)));
                             }
-                        } else if (a < 0 && b >= 0) {
-                            int32_t e2 = ext[b];
-                            trip.push_back({e2, e2, weight});
-                            rhs[e2] += weight * float(c1 - (c2 - center(b)));
-                        } else if (a >= 0 && b < 0) {
-                            int32_t e1 = ext[a];
-                            trip.push_back({e1, e1, weight});
-                            rhs[e1] += weight * float(c2 - (c1 - center(a)));
                         }
                     }
                 }
             }
+            Csr mx = csr_from_triplets(n, tx), my = csr_from_triplets(n, ty);
+            tx.clear();
+            ty.clear();
+            ex = bicgstab(mx, bx, x, o.initial_place_max_solver_iter);
+            ey = bicgstab(my, by, y, o.initial_place_max_solver_iter);
         }
-        Csr mx = csr_from_triplets(n, tx), my = csr_from_triplets(n, ty);
-        tx.clear();
-        ty.clear();
-        Solve ex = bicgstab(mx, bx, x, o.initial_place_max_solver_iter);
-        Solve ey = bicgstab(my, by, y, o.initial_place_max_solver_iter);
         g->ip_rounds = iter;
         if (std::isnan(ex.error) || std::isnan(ey.error)) {
             g->log.warn(325, strfmt("Conjugate gradient initial placement solver failed at iteration %d. ", iter));
@@ -1579,6 +1579,14 @@ int32_t gpl_design_initial_place(gpl_design* d) {
     d->have_pl_before_init = false;
     int32_t rc = gpl::initial_place(d, &err);
     d->initial_hpwl = gpl::db_hpwl(d);
+    return rc;
+}
+
+int32_t gpl_design_initial_place_gpu(gpl_design* d) {
+    if (!d) return GPL_ERR_ARG;
+    d->ip_device = 1;
+    const int32_t rc = gpl_design_initial_place(d);
+    d->ip_device = 0;
     return rc;
 }
 

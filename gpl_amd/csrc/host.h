@@ -186,9 +186,20 @@ struct gpl_design {
     float ip_error = 0.0f;
     int64_t initial_hpwl = 0;
     std::vector<int32_t> group_cells;
+    int32_t ip_device = 0;  // 1: the initial placement's rounds on the GPU (gpl_design_initial_place_gpu)
 };
 
 namespace gpl {
+// One BiCGSTAB solve's outcome (bicgstab.rs:143-197).
+struct IpSolve {
+    int32_t iterations;
+    float error;
+    bool converged;
+};
+// One initial-placement round's pin flags, B2B system and both solves on the device, bit for bit
+// as the host's (ip_gpu.hip); x, y in and out.
+int32_t ip_round_device(gpl_design* g, const std::vector<int32_t>& ext, std::vector<float>& x, std::vector<float>& y, IpSolve* ex,
+                        IpSolve* ey, std::string* err);
 // placer_base.rs:301 placer_base_common + 430 placer_base + initial_place.rs:100 pins
 int32_t build_placer(gpl_design* g, std::string* err);
 // initial_place.rs:420

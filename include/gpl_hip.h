@@ -122,6 +122,10 @@ void gpl_design_destroy(gpl_design* d);
 
 /* B2B + BiCGSTAB initial placement (initial_place.rs:420 do_bicgstab_place), host side. */
 int32_t gpl_design_initial_place(gpl_design* d);
+/* The same rounds with the pin flags, the B2B system (createSparseMatrix, initial_place.rs:264-355;
+ * its CSR, bicgstab.rs:29-69) and the BiCGSTAB solves (bicgstab.rs:72-197) on the GPU, in the host's
+ * evaluation orders: the placement it leaves is the host's bit for bit. */
+int32_t gpl_design_initial_place_gpu(gpl_design* d);
 
 /* Design statistics for reports: counts and areas. */
 typedef struct gpl_design_info {
