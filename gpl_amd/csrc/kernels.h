@@ -21,6 +21,9 @@
 
 namespace gplk {
 
+struct CtlRegion;
+struct CtlHead;
+
 struct RegionDev {
     int32_t lx, ly, ux, uy;
     int32_t cnt_x, cnt_y;
@@ -41,6 +44,11 @@ struct RegionDev {
     float* density;
     float2* field;  // the electric field per bin as (x, y) pairs
     float* phi;
+    // device-driven trials (null otherwise): the region's control record and the global one; the
+    // kernels then read their step, penalty, coefficients and slot pointers from them and do
+    // nothing while ctl->skip is set
+    const CtlRegion* ctl;
+    const CtlHead* cg;
 };
 
 // The Poisson solve's buffers and tables of a region (host side: the row kernels get them through
@@ -101,6 +109,7 @@ struct NetsDev {
     const int32_t* tile;    // num_tiles + 1
     int32_t num_tiles;
     int32_t tile_blocks;
+    const CtlHead* cg;      // device-driven trials: the WA coefficients from here, nothing while halted
 };
 
 // Per-trial FP64 reductions of one region, finished into this struct by k_finish.
@@ -112,12 +121,135 @@ struct Sums {
     int32_t pad[2];
 };
 
+// ---- device-driven trials ----------------------------------------------------------------------
+// The loop's control record in device memory.  The trial graph's last kernel (k_control) takes the
+// host's decisions of nesterov.cpp on the device after every back-tracking trial -- step length and
+// its acceptance, another trial or the iteration's end (phi, penalty, WA coefficient, the divergence
+// watermarks, the momentum) -- so that trials follow each other with no host round trip; it halts
+// (every kernel of later trials doing nothing) on the rare events the host handles: a zero or
+// non-finite gradient sum or step, a divergence, a new min-HPWL snapshot, a region's convergence,
+// a re-sort, the iteration limit.
+constexpr int kCtlMaxRegions = 16;
+constexpr int kCtlRing = 1024;  // history and progress rows kept between two halts
+
+// The arguments a region's trial kernels read (one 64-byte line, loaded at once): set by the host's
+// upload and refreshed by the control after every trial.
+struct alignas(64) CtlHot {
+    // bit 0: the region's kernels do nothing (converged, or the device halted); bit 1: the gradient
+    // puts a NaN into its sums (a host-forced case)
+    int32_t flags;
+    float step_length, density_penalty, coeff;
+    float2* cur_slp;
+    float2* cur_sum;
+    float2* cur;
+    float2* next;
+    float2* next_slp;
+    float2* next_sum;
+};
+
+struct alignas(64) CtlRegion {
+    CtlHot hot;
+    float2* sets[3][4];  // the three slp point sets (coordinates, wl, den, preconditioned sum)
+    float2* cn[2];       // cur / next
+    int32_t rot, cnp;    // cur_slp = sets[rot], next_slp = sets[(rot+1)%3], prev_slp = sets[(rot+2)%3]; cur = cn[cnp]
+    int32_t is_converged, max_phi_coef_changed, n, group;
+    float step_length, density_penalty, max_phi_coef, sum_overflow, sum_overflow_unscaled, min_sum_overflow;
+    float hpwl_with_min_sum_overflow, coordi_distance, peak_coordi_distance, prev_reported_overflow_unscaled, wl_grad_sum;
+    float density_grad_sum;
+    long long prev_hpwl, prev_reported_hpwl, overflow, overflow_unscaled, nia, non_place_insts_area;
+};
+
+struct CtlHist {  // one iteration's history row (gpl_iter_row)
+    int32_t iter, trials;
+    float overflow, density_penalty, step_length, wl_coef;
+    long long hpwl;
+};
+
+struct CtlLog {  // one region's progress row (every tenth iteration)
+    int32_t iter, region;
+    float overflow, pct, penalty;
+    int32_t pad;
+    long long hpwl;
+};
+
+enum CtlEvent {
+    CTL_RUN = 0,
+    CTL_ZERO_WL = 1,   // stage 1: a zero WA gradient sum (the coefficient-halving recursion)
+    CTL_NAN = 2,       // stage 1: a non-finite gradient sum (GPL-0306)
+    CTL_STEP = 3,      // stage 1: a non-finite step length (GPL-0305)
+    CTL_DIVERGE = 4,   // stage 2: a region's divergence rule fired
+    CTL_SNAPSHOT = 5,  // stage 2: a new min-HPWL snapshot is due
+    CTL_CONVERGE = 6,  // stage 2: a region reached the target overflow
+    CTL_RESORT = 7,    // stage 2: the re-sort trigger fired
+    CTL_STOP = 8,      // stage 3: the iteration limit, the caller's stop, or the rows are full
+};
+
+// The loop's scalars and the regions' records (the part the control stages in LDS).
+struct alignas(64) CtlHead {
+    // read by the trial kernels: the halt event, the momentum coefficient, the WA coefficients
+    int32_t halt;
+    float coeff, wl_coef_x, wl_coef_y;
+    int32_t stage;  // where the device halted: 1 after a trial's gradient sums, 2 after an
+                    // iteration's bookkeeping, 3 before the next iteration
+    int32_t nreg, iter, iter_stop, max_iter, trials, total_trials, recursion_wl, disable_revert;
+    float base_wl_coef, cur_a, reference_hpwl, average_overflow, average_overflow_unscaled;
+    float overflow_target, min_phi, diverge_snapshot_overflow;
+    int32_t is_min_hpwl, diverge_snapshot_iter, resort_enabled, last_resort_iter, iters_run;
+    long long min_hpwl, cur_hpwl;
+    unsigned long long last_global_adds;
+    double adds_base;
+    int32_t nhist, nlog;
+    // host-forced cases (-1 off): a NaN in the gradient sums from iters_run >= nan_from, a NaN step
+    // length from iters_run >= step_nan_from, a re-sort at every iteration >= resort_from, the WA sum
+    // read as zero in iteration zero_wl_at, a divergence in iterations diverge_at / diverge_at2
+    int32_t nan_from, step_nan_from, resort_from, zero_wl_at, diverge_at, diverge_at2;
+    CtlRegion reg[kCtlMaxRegions];
+};
+
+struct CtlGlobal {
+    CtlHead h;
+    // rows since the last halt (the host reads them there)
+    CtlHist hist[kCtlRing];
+    CtlLog log[kCtlRing];
+};
+
+// The control's arguments to the trial's last finishing reduction (g null: no control).
+struct CtlArgs {
+    CtlGlobal* g;
+    const Sums* sums;  // every region's, then the tail (HPWL and global-add counters)
+    Sums* tail;
+    unsigned long long* dev_seq;
+    unsigned long long* host_seq;  // pinned: [0] sequence number, [1] halt event
+    int32_t nreg;
+};
+
+// the hot fields of every region from its canonical ones (skip: halted or converged)
+__host__ __device__ inline void ctl_refresh(CtlHead& g) {
+    for (int b = 0; b < g.nreg; ++b) {
+        CtlRegion& q = g.reg[b];
+        const int ns = q.rot == 2 ? 0 : q.rot + 1;
+        q.hot.flags = ((g.halt != 0 || q.is_converged) ? 1 : 0) | (g.nan_from >= 0 && g.iters_run >= g.nan_from ? 2 : 0);
+        q.hot.step_length = q.step_length;
+        q.hot.density_penalty = q.density_penalty;
+        q.hot.coeff = g.coeff;
+        q.hot.cur_slp = q.sets[q.rot][0];
+        q.hot.cur_sum = q.sets[q.rot][3];
+        q.hot.cur = q.cn[q.cnp];
+        q.hot.next = q.cn[q.cnp ^ 1];
+        q.hot.next_slp = q.sets[ns][0];
+        q.hot.next_sum = q.sets[ns][3];
+    }
+}
+
 constexpr int kPartials = 2048;  // per-block partial slots per quantity
 constexpr int kGradStage = 2048;  // pin terms of a gradient block's cells staged in LDS, at most
 constexpr int kPlaceChunk = 512;  // cells per block iteration of the scatter family
 constexpr int kTilePins = 2048;   // pins a WA tile stages in LDS
 constexpr int kTileSpan = kTilePins - 128;  // pin range whose nets form a tile (a small net has <= 128 pins)
 static_assert(kTilePins >= 256 && kTilePins % 256 == 0, "a WA tile holds any small net, staged by 256 threads");
+static_assert(sizeof(CtlHot) == 64 && sizeof(Sums) == 64, "one 64-byte line each");
+static_assert(sizeof(CtlHead) % 8 == 0 && sizeof(CtlHead) / 8 <= 1024 && (kCtlMaxRegions + 1) * sizeof(Sums) / 8 <= 1024,
+              "the control stages one word per thread of the 1024-thread finishing block");
 
 // ---- launchers (stream-ordered, asynchronous) -----------------------------------------------
 void launch_zero_bins(hipStream_t s, const RegionDev& r);
@@ -137,8 +269,9 @@ void launch_density_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials, bool clear = true);
 // b[i] = a[i], i < n <= 64 (device to device, one small kernel: no copy node in a captured graph)
 void launch_copy_f64(hipStream_t s, const double* a, double* b, int n);
-// the Poisson solve of r.density into r.field (and r.phi when want_phi).
-void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi);
+// the Poisson solve of r.density into r.field (and r.phi when want_phi); nothing while *skip (when
+// given) is set.
+void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi, const int* skip = nullptr);
 // WA forward over every net, the HPWL, and each pin's weighted WA gradient into n.pgrad (at its
 // cell-pin index); the net records too when write_rec.  The high-fanout nets go to s_big (which
 // may be s; the caller orders s_big after the zeroed HPWL counter and joins it).
@@ -155,9 +288,16 @@ void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const
 // same values also written there, and with host_tail the HPWL counter and the global-add count of
 // dev_tail (the Sums record that holds them) copied to host_tail; with host_seq, after all of
 // that is visible to the host, *dev_seq is incremented and its new value written to *host_seq
+// With skip, nothing while *skip is set -- except, with ctl.g, the device-driven trial's control,
+// which then runs after the sums (k_finish of the trial's last region): it stages the control
+// record and the sums in LDS, takes the loop's decisions after the trial (the host's expressions of
+// nesterov.cpp), writes the record back, clears the HPWL counter for the next trial (unless it
+// halted) and signals the host (host_seq[1] = the halt event, then host_seq[0] = the sequence
+// number, a system-scope release store).
 void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int nparts_cells, Sums* out, const unsigned long long* hpwl,
                    Sums* host_out = nullptr, const Sums* dev_tail = nullptr, Sums* host_tail = nullptr,
-                   unsigned long long* dev_seq = nullptr, unsigned long long* host_seq = nullptr);
+                   unsigned long long* dev_seq = nullptr, unsigned long long* host_seq = nullptr, const int* skip = nullptr,
+                   const CtlArgs& ctl = CtlArgs{});
 // squared distance between two coordinate sets (prev/cur at init).
 void launch_distance(hipStream_t s, int n, const float2* a, const float2* b, const float2* c, const float2* d, double* partials);
 // initial prev_slp: cur_slp - c * cur_sum, kept inside.

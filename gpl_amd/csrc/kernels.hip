@@ -189,7 +189,22 @@ __device__ __forceinline__ void nesterov_point(float2 a, float2 sm, float2 cu, f
 // sit at centre + offset (updatePinLocation, 2268-2276) for the WA kernels.  The first thread
 // also clears the trial's HPWL counter.
 __global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c, PlaceArgs pa) {
-    const int n = r.cell_end - r.cell_begin;
+    int n = r.cell_end - r.cell_begin;
+    if (r.ctl) {
+        // a device-driven trial: the step, the momentum and the rotated slots from the control
+        // record's line (read at once; a skipped region has no cells to move, so the loads are not
+        // held back behind a branch)
+        const CtlHot h = r.ctl->hot;
+        pa.cur_slp = h.cur_slp;
+        pa.cur_sum = h.cur_sum;
+        pa.cur = h.cur;
+        pa.next = h.next;
+        pa.next_slp = h.next_slp;
+        pa.s = h.step_length;
+        pa.coeff = h.coeff;
+        pa.zero = nullptr;
+        if (h.flags & 1) n = 0;
+    }
     const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
     if (pa.zero && blockIdx.x == 0 && threadIdx.x == 0) *pa.zero = 0ull;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
@@ -225,6 +240,7 @@ __global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c
 // CH: the cells a block takes at a time (a tile shape the autotuner picks per region: 256 or 512).
 template <int MODE, typename V, int CH>
 __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev c, PlaceArgs pa) {
+    if (r.ctl && (r.ctl->hot.flags & 1)) return;
     constexpr int kChunk = CH;
     constexpr int kCellsPerThread = CH / kBlock;
     // the trial's critical chain (scatter -> bins -> solve) shares the CUs with the WA kernels on
@@ -486,6 +502,7 @@ __device__ void finish_sums(const double* partials, int nb_bins, int nb_cells, S
 // updateBinsGCellDensityArea's per-bin part (nesterov.rs:2252-2262): density, the overflow
 // terms (summed here in FP64), and the accumulators cleared for the next scatter.
 __global__ void k_bin_density(RegionDev r, double* partials, int clear) {
+    if (r.ctl && (r.ctl->hot.flags & 1)) return;
     __builtin_amdgcn_s_setprio(2);  // on the trial's critical chain (as the scatter)
     __shared__ double sh[kBlock / 64];
     const size_t M = (size_t)r.cnt_x * r.cnt_y;
@@ -807,6 +824,7 @@ struct RowArgs {
     const float* wcol;   // wavenumber of the element index
     double scale;
     int want_phi;
+    const int* skip;     // when set and *skip: the launch does nothing (a device-driven trial's halted region)
     int transpose_out;   // outputs written transposed: out[k * nrows + row]
     int interleave;      // ROW_SPECTRUM_POISSON: the two fields as pairs (Ex', Ey') in out0, [k][row][2];
                          // ROW_INV_FIELDS: in0 holds such pairs, [row][v][2]
@@ -831,6 +849,7 @@ struct RowArgs {
 // then [TL: tw h, th h + 1, dw L float2, the element wavenumbers L floats].
 template <int MODE, bool TL, int LOGL>
 __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
+    if (a.skip && (*a.skip & 1)) return;
     __builtin_amdgcn_s_setprio(2);  // on the trial's critical chain (as the scatter)
     extern __shared__ float2 lds2[];
     lds_float2* const lbase = (lds_float2*)lds2;
@@ -1226,6 +1245,12 @@ constexpr int kBigRegs = 8;
 template <bool SUMS>
 __global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
                                                       int write_rec) {
+    if (n.cg) {
+        const int halt = n.cg->halt;
+        cx = n.cg->wl_coef_x;
+        cy = n.cg->wl_coef_y;
+        if (halt) return;
+    }
     __shared__ int shi[4][kBigBlock / 64];
     __shared__ double shd[8][kBigBlock / 64];
     __shared__ float shr[8];
@@ -1343,6 +1368,12 @@ constexpr int kTileStage = kTilePins / kBlock;
 template <bool SUMS>
 __global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
                                                      int write_rec) {
+    if (n.cg) {
+        const int halt = n.cg->halt;
+        cx = n.cg->wl_coef_x;
+        cy = n.cg->wl_coef_y;
+        if (halt) return;
+    }
     __shared__ int2 spos[kTilePins];
     __shared__ int scq[SUMS ? kTilePins : 1];
     __shared__ unsigned long long shh[kBlock / 64];
@@ -1434,9 +1465,23 @@ constexpr int kGradPins = 6;
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float cy, float penalty, float2* wl_out, float2* den_out,
                            float2* sum_out, const float2* a_slp, const float2* b_slp, const float2* a_sum, double* partials,
                            int nan_inject) {
+    int n = r.cell_end - r.cell_begin;
+    if (r.ctl) {
+        // a device-driven trial: the penalty and the rotated slots from the control record's line
+        // (a skipped region: no cells, its partials zero and unread)
+        const CtlHot h = r.ctl->hot;
+        penalty = h.density_penalty;
+        sum_out = h.next_sum;
+        a_slp = h.cur_slp;
+        b_slp = h.next_slp;
+        a_sum = h.cur_sum;
+        wl_out = nullptr;
+        den_out = nullptr;
+        nan_inject = (h.flags >> 1) & 1;
+        if (h.flags & 1) n = 0;
+    }
     __shared__ double sh[kBlock / 64];
     __shared__ float2 spg[kGradStage];
-    const int n = r.cell_end - r.cell_begin;
     double s_wl = 0.0, s_den = 0.0, s_c = 0.0, s_g = 0.0;
     // the block's cells in groups of kBlock consecutive cells (the trip count is the block's own)
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -1600,11 +1645,233 @@ __global__ void k_initial_prev(RegionDev r, CellsDev c, const float2* cur_slp, c
 // per-thread stride then a fixed tree).
 // One 1024-thread block: the six sums at once (each in a fixed order: thread t adds partials t,
 // t + 1024, ..., then the waves' totals in wave order).
+// ---------------------------------------------------------------------------------------------
+// Device-driven trials: the loop's decisions after a trial, on the device (one thread).  Every
+// expression is the host's (nesterov.cpp: gradients_read, update_step_length, do_back_tracking,
+// adjust_phi, nb_/np_update_next_iter, update_wirelength_coef, phi_coef, the momentum of loop_step,
+// maybe_resort's bookkeeping), in the same float and double arithmetic, so that the two modes give
+// the same rows; the events the host handles halt the device before any of their effects.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool ctl_bad(float v) { return isnan(v) || isinf(v); }
+
+__device__ void ctl_halt(CtlHead& g, int event, int stage) {
+    g.halt = event;  // (ctl_refresh then sets every region's skip)
+    g.stage = stage;
+}
+
+__device__ void ctl_wl_coef(CtlHead& g, float overflow) {
+#pragma clang fp contract(off)
+    float c;
+    if (overflow > 1.0f)
+        c = (float)0.1;
+    else if ((double)overflow < 0.1)
+        c = 10.0f;
+    else
+        c = (float)(1.0 / pow(10.0, ((double)overflow - 0.1) * 20.0 / 9.0 - 1.0));
+    g.wl_coef_x = c * g.base_wl_coef;
+    g.wl_coef_y = c * g.base_wl_coef;
+}
+
+// g and the sums (every region's, then the tail) staged in LDS; history and progress rows appended
+// to G's rings; nstep / ncd: LDS scratch per region
+__device__ void ctl_trial(CtlHead& g, const Sums* sums, CtlGlobal* G, float* nstep, float* ncd) {
+#pragma clang fp contract(off)
+    const Sums* tail = sums + g.nreg;
+    const int R = g.nreg;
+    // gradients_read: a zero WA sum or a non-finite sum is the host's (nothing changed yet); a
+    // converged region's are its last values, as on the host
+    for (int b = 0; b < R; ++b) {
+        const CtlRegion& q = g.reg[b];
+        const float wl = q.is_converged ? q.wl_grad_sum : (float)sums[b].wl_abs;
+        const float den = q.is_converged ? q.density_grad_sum : (float)sums[b].den_abs;
+        if (wl == 0.0f && g.recursion_wl < 10) return ctl_halt(g, CTL_ZERO_WL, 1);
+        if (g.iter == g.zero_wl_at && !q.is_converged) return ctl_halt(g, CTL_ZERO_WL, 1);
+        if (ctl_bad(wl) || ctl_bad(den)) return ctl_halt(g, CTL_NAN, 1);
+    }
+    // nesterovUpdateStepLength of every live region, checked before any is kept
+    int ok = 0;
+    for (int b = 0; b < R; ++b) {
+        const CtlRegion& q = g.reg[b];
+        if (q.is_converged) {
+            ++ok;
+            continue;
+        }
+        const double n2 = 2.0 * (double)q.n;
+        const float cd = q.n ? (float)sqrt(sums[b].coord_sq / n2) : 0.0f;
+        const float gd = q.n ? (float)sqrt(sums[b].grad_sq / n2) : 0.0f;
+        const float nw = cd / gd;
+        if (ctl_bad(nw) || (g.step_nan_from >= 0 && g.iters_run >= g.step_nan_from)) return ctl_halt(g, CTL_STEP, 1);
+        ncd[b] = cd;
+        if ((double)nw > (double)q.step_length * 0.95) {
+            nstep[b] = nw;
+        } else if ((double)nw < 0.01) {
+            nstep[b] = (float)0.01;
+        } else {
+            nstep[b] = nw;
+            ++ok;
+        }
+    }
+    for (int b = 0; b < R; ++b) {
+        CtlRegion& q = g.reg[b];
+        if (q.is_converged) continue;
+        q.wl_grad_sum = (float)sums[b].wl_abs;
+        q.density_grad_sum = (float)sums[b].den_abs;
+        q.overflow = (long long)sums[b].overflow;
+        q.overflow_unscaled = (long long)sums[b].overflow_unscaled;
+        q.step_length = nstep[b];
+        q.coordi_distance = ncd[b];
+    }
+    g.cur_hpwl = (long long)tail->hpwl;
+    g.trials += 1;
+    if (ok == R && g.trials < 10) return;  // every step shrank: another trial
+    // the iteration's end
+    const int iter = g.iter;
+    const int trials = g.trials;
+    g.total_trials += trials;
+    for (int b = 0; b < R; ++b) {  // nesterovAdjustPhi
+        CtlRegion& q = g.reg[b];
+        if (q.is_converged) continue;
+        if (!q.max_phi_coef_changed && q.sum_overflow_unscaled < 0.35f) {
+            q.max_phi_coef_changed = 1;
+            q.max_phi_coef = (float)((double)q.max_phi_coef * 0.99);
+        }
+        if (q.max_phi_coef <= 1.0f) q.max_phi_coef = 1.01f;
+    }
+    float total = 0.0f, total_unscaled = 0.0f;
+    for (int b = 0; b < R; ++b) {  // NesterovBase::updateNextIter
+        CtlRegion& q = g.reg[b];
+        if (!q.is_converged) {
+            q.rot = q.rot == 2 ? 0 : q.rot + 1;
+            q.cnp ^= 1;
+            const float fraction = (float)iter / (float)g.max_iter;
+            const float a = (float)q.nia, c = fraction * (float)q.non_place_insts_area * 0.05f;
+            const float denom = a > c ? a : c;
+            q.sum_overflow = (float)q.overflow / denom;
+            q.sum_overflow_unscaled = (float)q.overflow_unscaled / denom;
+            const long long hpwl = g.cur_hpwl;
+            if (iter == 0 || iter % 10 == 0) {
+                float pct = 0.0f;
+                if (q.prev_reported_hpwl != 0)
+                    pct = (float)((double)(hpwl - q.prev_reported_hpwl) / (double)q.prev_reported_hpwl * 100.0);
+                if (g.nlog < kCtlRing) {
+                    CtlLog& L = G->log[g.nlog++];
+                    L.iter = iter;
+                    L.region = b;
+                    L.overflow = q.sum_overflow_unscaled;
+                    L.pct = pct;
+                    L.penalty = q.density_penalty;
+                    L.hpwl = hpwl;
+                }
+                q.prev_reported_hpwl = hpwl;
+                q.prev_reported_overflow_unscaled = q.sum_overflow_unscaled;
+            }
+            const float scaled = (float)(hpwl - q.prev_hpwl) / g.reference_hpwl;
+            const float m = q.max_phi_coef;
+            const float v = scaled < 0.0f ? m : (float)((double)m * pow((double)m, -(double)scaled));
+            const float phi = g.min_phi > v ? g.min_phi : v;
+            q.density_penalty *= phi;
+            q.prev_hpwl = hpwl;
+            q.peak_coordi_distance = q.peak_coordi_distance > q.coordi_distance ? q.peak_coordi_distance : q.coordi_distance;
+            if (iter > 50 && q.min_sum_overflow > q.sum_overflow_unscaled) {
+                q.min_sum_overflow = q.sum_overflow_unscaled;
+                q.hpwl_with_min_sum_overflow = (float)q.prev_hpwl;
+            }
+        }
+        total += q.sum_overflow;
+        total_unscaled += q.sum_overflow_unscaled;
+    }
+    const float nr = (float)R;
+    g.average_overflow = total / nr;
+    g.average_overflow_unscaled = total_unscaled / nr;
+    ctl_wl_coef(g, g.average_overflow);
+    if (!g.disable_revert) {
+        const long long hpwl = g.cur_hpwl;
+        if (hpwl < g.min_hpwl && (double)g.average_overflow_unscaled <= 0.25) {
+            g.min_hpwl = hpwl;
+            g.diverge_snapshot_overflow = g.average_overflow_unscaled;
+            g.diverge_snapshot_iter = iter + 1;
+            g.is_min_hpwl = 1;
+        } else {
+            g.is_min_hpwl = 0;
+        }
+    }
+    if (g.nhist < kCtlRing) {  // push_history
+        CtlHist& H = G->hist[g.nhist++];
+        H.iter = iter;
+        H.trials = trials;
+        H.overflow = g.average_overflow_unscaled;
+        H.density_penalty = g.reg[0].density_penalty;
+        H.step_length = g.reg[0].step_length;
+        H.wl_coef = g.wl_coef_x;
+        H.hpwl = g.reg[0].prev_hpwl;
+    }
+    g.iters_run += 1;
+    g.trials = 0;
+    // the host takes over for isDiverged's effects, a new snapshot and a convergence
+    for (int b = 0; b < R; ++b) {  // checkDivergence's rules (every region, as on the host)
+        const CtlRegion& q = g.reg[b];
+        bool d = q.sum_overflow_unscaled < 0.2f && q.sum_overflow_unscaled - q.min_sum_overflow >= 0.02f &&
+                 q.hpwl_with_min_sum_overflow * 1.2f < (float)q.prev_hpwl;
+        if (q.prev_reported_overflow_unscaled > 0.0f && q.prev_reported_hpwl > 0) {
+            const float change = q.sum_overflow_unscaled - q.prev_reported_overflow_unscaled;
+            const float increase = (float)(q.prev_hpwl - q.prev_reported_hpwl) / (float)q.prev_reported_hpwl;
+            d = d || (change >= 0.05f && increase >= 0.25f);
+        }
+        if (d || iter == g.diverge_at || iter == g.diverge_at2) return ctl_halt(g, CTL_DIVERGE, 2);
+    }
+    if (g.is_min_hpwl && !g.disable_revert) return ctl_halt(g, CTL_SNAPSHOT, 2);
+    for (int b = 0; b < R; ++b)
+        if (!g.reg[b].is_converged && !(g.reg[b].sum_overflow_unscaled > g.overflow_target)) return ctl_halt(g, CTL_CONVERGE, 2);
+    // maybe_resort's bookkeeping; a due re-sort is the host's
+    if (g.resort_enabled) {
+        if (g.resort_from >= 0 && iter >= g.resort_from) return ctl_halt(g, CTL_RESORT, 2);
+        const unsigned long long ga = (unsigned long long)(unsigned)tail->pad[0] | ((unsigned long long)(unsigned)tail->pad[1] << 32);
+        const double rate = double(ga - g.last_global_adds) / double(trials > 1 ? trials : 1);
+        double base = g.adds_base;
+        if (iter == g.last_resort_iter + 1 || base == 0.0) base = rate;
+        if (rate > 1.5 * base && iter - g.last_resort_iter >= 50) return ctl_halt(g, CTL_RESORT, 2);
+        g.last_global_adds = ga;
+        g.adds_base = base;
+    }
+    g.iter = iter + 1;
+    if (g.iter >= g.max_iter || g.iter >= g.iter_stop || g.nhist >= kCtlRing || g.nlog + R > kCtlRing)
+        return ctl_halt(g, CTL_STOP, 3);
+    // the next iteration's momentum
+    const float prev_a = g.cur_a;
+    g.cur_a = (float)((1.0 + sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
+    g.coeff = (float)(((double)prev_a - 1.0) / (double)g.cur_a);
+}
+
+// The control of a device-driven trial, run by every thread of k_finish's block once the record's
+// header and regions (st) and every region's sums (ss) are in LDS: the decisions by one thread, the
+// record written back, the host signalled.
+__device__ void ctl_decide(const CtlArgs& ca, CtlHead& st, Sums* ss, float* nstep, float* ncd, int words, unsigned long long seq) {
+    __shared__ int run;
+    if (threadIdx.x == 0) {
+        run = st.halt == 0;
+        if (run) {
+            ctl_trial(st, ss, ca.g, nstep, ncd);
+            ctl_refresh(st);
+        }
+    }
+    __syncthreads();
+    unsigned long long* src = reinterpret_cast<unsigned long long*>(&ca.g->h);
+    const unsigned long long* dst = reinterpret_cast<const unsigned long long*>(&st);
+    if (run && (int)threadIdx.x < words) src[threadIdx.x] = dst[threadIdx.x];
+    if (threadIdx.x == 0) {
+        // the next trial's WA kernels add into a cleared counter; a halted trial's HPWL stays for the host
+        if (run && !st.halt) ca.tail->hpwl = 0ull;
+        const unsigned long long q = seq + 1ull;
+        *ca.dev_seq = q;
+        ca.host_seq[1] = (unsigned long long)(unsigned)st.halt;
+        __hip_atomic_store(ca.host_seq, q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 constexpr int kFinishBlock = 1024;
-__global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials, int nb_bins, int nb_cells, Sums* out,
-                                                         const unsigned long long* hpwl, Sums* host_out,
-                                                         const Sums* dev_tail, Sums* host_tail, unsigned long long* dev_seq,
-                                                         unsigned long long* host_seq) {
+__device__ __forceinline__ void finish_body(const double* partials, int nb_bins, int nb_cells, Sums* out, const unsigned long long* hpwl,
+                                            Sums* host_out, const Sums* dev_tail, Sums* host_tail, unsigned long long* dev_seq,
+                                            unsigned long long* host_seq, Sums* lds_out) {
     __shared__ double sh[6][kFinishBlock / 64];
     // thread 0's scalars read first, so that their latency overlaps the partials'
     unsigned long long seq = 0, h = 0, tail_h = 0, tail_p0 = 0, tail_p1 = 0;
@@ -1643,11 +1910,13 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials,
             *o = t;
             // the same value straight into the host's pinned copy (no copy node after the trial)
             if (host_out) *(reinterpret_cast<double*>(host_out) + (o - &out->overflow)) = t;
+            if (lds_out) *(reinterpret_cast<double*>(lds_out) + (o - &out->overflow)) = t;
         }
     }
     if (threadIdx.x == 0 && hpwl) {
         out->hpwl = h;
         if (host_out) host_out->hpwl = h;
+        if (lds_out) lds_out->hpwl = h;
     }
     if (threadIdx.x == 0 && host_tail) {
         // the HPWL counter and the scatter's global-add count
@@ -1668,6 +1937,40 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials,
             *dev_seq = q;
             __hip_atomic_store(host_seq, q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
+    }
+}
+
+__global__ void __launch_bounds__(kFinishBlock) k_finish(const double* partials, int nb_bins, int nb_cells, Sums* out,
+                                                         const unsigned long long* hpwl, Sums* host_out,
+                                                         const Sums* dev_tail, Sums* host_tail, unsigned long long* dev_seq,
+                                                         unsigned long long* host_seq, const int* skip, CtlArgs ctl) {
+    __shared__ CtlHead st;
+    __shared__ Sums ss[kCtlMaxRegions + 1];
+    __shared__ float nstep[kCtlMaxRegions], ncd[kCtlMaxRegions];
+    const bool skipped = skip && (*skip & 1);
+    if (skipped && !ctl.g) return;
+    // the control's staging loads are issued first (one word per thread: the header and regions
+    // take at most 8 x 84 words, the sums 8 x 17), so that they fly beside the reduction's
+    int words = 0, swords = 0;
+    unsigned long long v0 = 0, v1 = 0, seq = 0;
+    if (ctl.g) {
+        words = int((offsetof(CtlHead, reg) + size_t(ctl.nreg) * sizeof(CtlRegion)) / 8);
+        swords = int(size_t(ctl.nreg + 1) * sizeof(Sums) / 8);
+        if ((int)threadIdx.x < words) v0 = reinterpret_cast<const unsigned long long*>(&ctl.g->h)[threadIdx.x];
+        if ((int)threadIdx.x < swords) v1 = reinterpret_cast<const unsigned long long*>(ctl.sums)[threadIdx.x];
+        if (threadIdx.x == 0) seq = *ctl.dev_seq;
+    }
+    if (!skipped)
+        finish_body(partials, nb_bins, nb_cells, out, hpwl, host_out, dev_tail, host_tail, dev_seq, host_seq,
+                    ctl.g ? &ss[ctl.nreg - 1] : nullptr);
+    if (ctl.g) {
+        // the staged words into LDS, but not those of this region's sums that the reduction wrote
+        const int w = (int)threadIdx.x - 8 * (ctl.nreg - 1);
+        const bool fresh = !skipped && w >= 0 && w < 8 && ((w < 2 && nb_bins > 0) || (w >= 2 && w < 6 && nb_cells > 0) || (w == 6 && hpwl));
+        if ((int)threadIdx.x < words) reinterpret_cast<unsigned long long*>(&st)[threadIdx.x] = v0;
+        if ((int)threadIdx.x < swords && !fresh) reinterpret_cast<unsigned long long*>(ss)[threadIdx.x] = v1;
+        __syncthreads();
+        ctl_decide(ctl, st, ss, nstep, ncd, words, seq);
     }
 }
 
@@ -2057,7 +2360,7 @@ static bool three_pass(const RegionDev& r) { return r.cnt_x <= 2048 && r.cnt_y <
 //    Poisson divide (425-440) and the inverse transforms along y -> Ex'[y][u] (cos), Ey'[y][u]
 //    (sin), phi'[y][u];
 // Q3 rows along x: the field pairs (sin along x, cos along x) (and phi = cos along x).
-static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi) {
+static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi, const int* skip) {
     const int nx = r.cnt_x, ny = r.cnt_y;
     RowArgs a{};
     a.in0 = r.density;
@@ -2068,6 +2371,7 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs&
     a.th = sb.th_x;
     a.dw = sb.dw_x;
     a.transpose_out = 1;
+    a.skip = skip;
     launch_rows(s, ROW_DCT2, a);
     a = RowArgs{};
     a.in0 = sb.work1;
@@ -2084,6 +2388,7 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs&
     a.wcol = sb.wy;
     a.want_phi = want_phi ? 1 : 0;
     a.transpose_out = 1;
+    a.skip = skip;
     launch_rows(s, ROW_SPECTRUM_POISSON, a);
     a = RowArgs{};
     a.in0 = sb.work0;
@@ -2094,6 +2399,7 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs&
     a.tw = sb.tw_x;
     a.th = sb.th_x;
     a.dw = sb.dw_x;
+    a.skip = skip;
     launch_rows(s, ROW_INV_FIELDS, a);
     if (want_phi) {
         a = RowArgs{};
@@ -2104,6 +2410,7 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs&
         a.tw = sb.tw_x;
         a.th = sb.th_x;
         a.dw = sb.dw_x;
+        a.skip = skip;
         launch_rows(s, ROW_INV_PHI, a);
     }
 }
@@ -2112,9 +2419,9 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs&
 // P2: rows along y, DCT-II + halving + scale -> A[u][v]; P3: rows along v, the Poisson divide and
 // the inverse transforms along y -> Ex'[u][y] (cos), Ey'[u][y] (sin), phi'[u][y]; transpose ->
 // [y][u]; P4: rows along u -> the field pairs (sin along x, cos along x), phi = cos along x.
-void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi) {
+void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi, const int* skip) {
     if (three_pass(r)) {
-        solve_three_pass(s, r, sb, want_phi);
+        solve_three_pass(s, r, sb, want_phi, skip);
         return;
     }
     const int nx = r.cnt_x, ny = r.cnt_y;
@@ -2127,6 +2434,7 @@ void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool w
     a.tw = sb.tw_x;
     a.th = sb.th_x;
     a.dw = sb.dw_x;
+    a.skip = skip;
     launch_rows(s, ROW_DCT2, a);
     transpose(s, sb.work0, sb.work1, ny, nx);  // [u][y]
     // P2
@@ -2139,6 +2447,7 @@ void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool w
     a.th = sb.th_y;
     a.dw = sb.dw_y;
     a.scale = 4.0 / (double)nx / (double)ny;
+    a.skip = skip;
     launch_rows(s, ROW_DCT2_SCALE, a);  // A[u][v]
     // P3
     a = RowArgs{};
@@ -2154,6 +2463,7 @@ void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool w
     a.wrow = sb.wx;
     a.wcol = sb.wy;
     a.want_phi = want_phi ? 1 : 0;
+    a.skip = skip;
     launch_rows(s, ROW_POISSON, a);
     transpose(s, sb.work1, sb.work0, nx, ny);  // Ex [y][u]
     transpose(s, sb.work2, sb.work1, nx, ny);  // Ey [y][u]
@@ -2168,6 +2478,7 @@ void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool w
     a.tw = sb.tw_x;
     a.th = sb.th_x;
     a.dw = sb.dw_x;
+    a.skip = skip;
     launch_rows(s, ROW_INV_FIELDS, a);
     if (want_phi) {
         RowArgs p{};
@@ -2178,6 +2489,7 @@ void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool w
         p.tw = sb.tw_x;
         p.th = sb.th_x;
         p.dw = sb.dw_x;
+        p.skip = skip;
         launch_rows(s, ROW_INV_PHI, p);
     }
 }
@@ -2208,9 +2520,10 @@ void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const
 }
 
 void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int nparts_cells, Sums* out, const unsigned long long* hpwl,
-                   Sums* host_out, const Sums* dev_tail, Sums* host_tail, unsigned long long* dev_seq, unsigned long long* host_seq) {
+                   Sums* host_out, const Sums* dev_tail, Sums* host_tail, unsigned long long* dev_seq, unsigned long long* host_seq,
+                   const int* skip, const CtlArgs& ctl) {
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishBlock), 0, s, partials, nparts_bins, nparts_cells, out, hpwl, host_out, dev_tail,
-                       host_tail, dev_seq, host_seq);
+                       host_tail, dev_seq, host_seq, skip, ctl);
 }
 
 void launch_distance(hipStream_t s, int n, const float2* a, const float2* b, const float2* c, const float2* d, double* partials) {

@@ -3,8 +3,12 @@
 //
 // The per-cell, per-pin, per-net and per-bin work runs on the device; the scalar control (step
 // length acceptance, the penalty and WA-coefficient schedule, divergence, convergence) runs
-// here on values the device reduces in FP64 and hands back once per back-tracking trial (one
-// 128-byte copy per region).  Slot rotation (prev <- cur <- next) is a swap of device pointers.
+// here on values the device reduces in FP64 and hands back once per back-tracking trial (the
+// sums straight into pinned memory).  Slot rotation (prev <- cur <- next) is a swap of device
+// pointers.  In launch modes 4-6 the same decisions are taken on the device by the trial's last
+// kernel (device-driven trials): the host then steps in only where the device halts, on the
+// events that need it (a zero or non-finite sum, a divergence, a new snapshot, a convergence, a
+// re-sort, the iteration limit), and the rows are those of the host-driven loop, bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -108,6 +112,22 @@ struct gpl_ctx {
     hipGraphExec_t texec = nullptr;
     uint64_t tgraph_key = ~0ull;
     std::vector<gplk::TrialNode> tnodes;
+    // Device-driven trials (launch mode 4): the trial graph ends in k_control, which takes the
+    // loop's decisions on the device from the control record d_ctl, so that trials run back to
+    // back (ctl_depth graphs in flight) and the host steps in only when the device halts on an
+    // event it handles.  h_ctl is the record's pinned staging copy.
+    bool device_trials = false, ctl_failed = false;
+    gplk::CtlGlobal* d_ctl = nullptr;
+    gplk::CtlGlobal* h_ctl = nullptr;
+    hipGraph_t cgraph = nullptr;
+    hipGraphExec_t cexec = nullptr;
+    int32_t ctl_depth = 2;        // trials in flight
+    int32_t ctl_launch = 0;       // 0 graphs, 1 kernels one by one on the side streams, 2 on one stream
+    bool ctl_ready = false;
+    std::vector<gplk::RegionDev> ctl_dv;
+    gplk::NetsDev ctl_nn{};
+    int64_t ctl_graphs = 0, ctl_halts = 0;
+    double ctl_host_ms = 0.0;
     // a third stream for the high-fanout nets' WA kernel, beside the small nets' one
     hipStream_t aux2 = nullptr;
     hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
@@ -228,6 +248,9 @@ struct gpl_ctx {
         for (void* p : allocs) hipFree(p);
         if (h_sums) hipHostFree(h_sums);
         if (h_seq) hipHostFree(h_seq);
+        if (h_ctl) hipHostFree(h_ctl);
+        if (cexec) hipGraphExecDestroy(cexec);
+        if (cgraph) hipGraphDestroy(cgraph);
         if (profiling)
             for (int i = 0; i < 64; ++i) hipEventDestroy(ev[i]);
         if (aux) hipStreamSynchronize(aux);
@@ -603,8 +626,9 @@ struct gpl_ctx {
     // Waits for the trial graph's completion signal (spinning: the statistics are then complete in
     // pinned memory, and the stream's later work queues behind the graph as usual); after 2 s
     // without it, a stream synchronisation reports whatever went wrong.
-    void wait_trial_signal() {
-        const unsigned long long want = ++seq_want;
+    void wait_trial_signal() { wait_seq(++seq_want); }
+
+    void wait_seq(unsigned long long want) {
         const auto t0 = std::chrono::steady_clock::now();
         for (uint32_t k = 0;; ++k) {
             if (*reinterpret_cast<volatile unsigned long long*>(h_seq) >= want) return;
@@ -614,6 +638,297 @@ struct gpl_ctx {
         if (*reinterpret_cast<volatile unsigned long long*>(h_seq) < want) throw HipError{hipErrorUnknown, "trial signal"};
     }
 
+    // ---- device-driven trials ----------------------------------------------------------------
+    // Whether the next iteration's trials may run on the device: launch mode 4, no profiling, no
+    // live region with locked cells (their next slots are fixed by a host-launched kernel) and few
+    // enough regions for the control record.
+    bool ctl_usable() const {
+        if (!device_trials || ctl_failed || launch_mode != 0 || profiling) return false;
+        if (rg.size() > size_t(gplk::kCtlMaxRegions)) return false;
+        for (auto& r : rg)
+            if (!r.is_converged && r.has_locked) return false;
+        return true;
+    }
+
+    // The device-driven trial: every region's update, scatter, bins and solve (second stream),
+    // the WA kernels (third stream for the high-fanout nets), gradients and finishing reductions,
+    // the last of which runs the control.  The kernels read their step, penalty, coefficients and
+    // slots from the control record, so the same launches serve every trial: captured once as a
+    // graph (launch mode 4), or launched one by one on the side streams (5) or on one stream (6).  Their arguments (ctl_dv, ctl_nn) are taken again after a re-sort or
+    // a new autotuner pick (the cell and net arrays, the kernel variants).
+    void ctl_prepare() {
+        ctl_nn = nets;
+        ctl_nn.cg = &d_ctl->h;
+        ctl_dv.resize(rg.size());
+        for (size_t b = 0; b < rg.size(); ++b) {
+            ctl_dv[b] = rg[b].dev;
+            ctl_dv[b].ctl = &d_ctl->h.reg[b];
+            ctl_dv[b].cg = &d_ctl->h;
+        }
+        ctl_ready = true;
+    }
+
+    void enqueue_ctl_trial(bool side) {
+        const size_t R = rg.size();
+        hipStream_t s1 = side ? aux : stream;
+        for (size_t b = 0; b < R; ++b) {
+            Region& r = rg[b];
+            gplk::launch_update(stream, ctl_dv[b], cells, r.v[S_CUR_SLP], r.v[S_CUR_SLP_SUM], r.v[S_CUR], r.v[S_NEXT], r.v[S_NEXT_SLP],
+                                0.0f, 0.0f, nullptr);
+        }
+        if (side) {
+            check(hipEventRecord(ev_fork, stream), "event");
+            check(hipStreamWaitEvent(aux, ev_fork, 0), "wait");
+        }
+        for (size_t b = 0; b < R; ++b) {
+            Region& r = rg[b];
+            gplk::launch_density_scatter(s1, ctl_dv[b], cells, d_global_adds);
+            gplk::launch_bin_density(s1, ctl_dv[b], r.partials);
+            gplk::launch_solve(s1, ctl_dv[b], r.sb, false, &d_ctl->h.reg[b].hot.flags);
+        }
+        if (side) check(hipEventRecord(ev_join, aux), "event");
+        const bool big = side && nets.num_small < nnets;
+        if (big) {
+            check(hipEventRecord(ev_fork2, stream), "event");
+            check(hipStreamWaitEvent(aux2, ev_fork2, 0), "wait");
+        }
+        gplk::launch_wa_forward(stream, big ? aux2 : stream, ctl_nn, cells, 0.0f, 0.0f, d_hpwl, false);
+        if (big) {
+            check(hipEventRecord(ev_join2, aux2), "event");
+            check(hipStreamWaitEvent(stream, ev_join2, 0), "wait");
+        }
+        if (side) check(hipStreamWaitEvent(stream, ev_join, 0), "wait");
+        gplk::CtlArgs ca{d_ctl, d_sums, d_sums + R, d_seq, h_seq, int32_t(R)};
+        for (size_t b = 0; b < R; ++b) {
+            Region& r = rg[b];
+            gplk::launch_gradient(stream, ctl_dv[b], cells, ctl_nn, 0.0f, 0.0f, 0.0f, nullptr, nullptr, r.v[S_NEXT_SLP_SUM],
+                                  r.v[S_CUR_SLP], r.v[S_NEXT_SLP], r.v[S_CUR_SLP_SUM], r.partials, 0);
+            gplk::launch_finish(stream, r.partials, gplk::block_partials_bins(r.dev), gplk::block_partials_cells(r.dev), d_sums + b, d_hpwl,
+                                nullptr, nullptr, nullptr, nullptr, nullptr, &d_ctl->h.reg[b].hot.flags, b + 1 == R ? ca : gplk::CtlArgs{});
+        }
+    }
+
+    bool build_ctl_graph() {
+        drop_ctl_graph();
+        join();
+        check(hipStreamSynchronize(stream), "sync");
+        ctl_prepare();
+        check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+        hipError_t err = hipSuccess;
+        try {
+            enqueue_ctl_trial(true);
+        } catch (const HipError& e) {
+            err = e.e;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(stream, &g);
+        if (err != hipSuccess || ec != hipSuccess || !g) {
+            if (g) hipGraphDestroy(g);
+            return false;
+        }
+        cgraph = g;
+        if (hipGraphInstantiate(&cexec, cgraph, nullptr, nullptr, 0) != hipSuccess) {
+            drop_ctl_graph();
+            return false;
+        }
+        return true;
+    }
+
+    // The host's loop state into the control record (pinned staging copy, then an asynchronous
+    // copy on the stream): the point sets as rotation 0, the iteration `ls.iter` about to start with
+    // momentum coefficient `coeff`, at most up to iteration `stop_at`.
+    void ctl_upload(float coeff, int32_t stop_at) {
+        gplk::CtlHead& g = h_ctl->h;
+        g.halt = 0;
+        g.stage = 0;
+        g.nreg = int32_t(rg.size());
+        g.iter = ls.iter;
+        g.iter_stop = stop_at;
+        g.max_iter = max_iter;
+        g.trials = 0;
+        g.total_trials = total_trials;
+        g.recursion_wl = recursion_wl;
+        g.disable_revert = o.disable_revert_if_diverge ? 1 : 0;
+        g.wl_coef_x = wl_coef_x;
+        g.wl_coef_y = wl_coef_y;
+        g.base_wl_coef = base_wl_coef;
+        g.cur_a = ls.cur_a;
+        g.coeff = coeff;
+        g.reference_hpwl = reference_hpwl;
+        g.average_overflow = average_overflow;
+        g.average_overflow_unscaled = average_overflow_unscaled;
+        g.overflow_target = o.overflow;
+        g.min_phi = o.min_phi_coef;
+        g.diverge_snapshot_overflow = diverge_snapshot_overflow;
+        g.is_min_hpwl = is_min_hpwl ? 1 : 0;
+        g.diverge_snapshot_iter = diverge_snapshot_iter;
+        g.resort_enabled = resort_enabled ? 1 : 0;
+        g.last_resort_iter = last_resort_iter;
+        g.iters_run = iters_run;
+        g.min_hpwl = min_hpwl;
+        g.cur_hpwl = cur_hpwl;
+        g.last_global_adds = last_global_adds;
+        g.adds_base = adds_base;
+        g.nhist = 0;
+        g.nlog = 0;
+        const bool after = inject_after >= 0;
+        g.nan_from = inject_kind == 1 && after ? inject_after : -1;
+        g.step_nan_from = inject_kind == 2 && after ? inject_after : -1;
+        g.resort_from = inject_kind == 3 && after ? inject_after : -1;
+        g.zero_wl_at = inject_zero_wl_at;
+        g.diverge_at = inject_diverge_at;
+        g.diverge_at2 = inject_diverge_at2;
+        for (size_t b = 0; b < rg.size(); ++b) {
+            const Region& r = rg[b];
+            gplk::CtlRegion& q = g.reg[b];
+            for (int k = 0; k < 4; ++k) {
+                q.sets[0][k] = r.v[S_CUR_SLP + k];
+                q.sets[1][k] = r.v[S_NEXT_SLP + k];
+                q.sets[2][k] = r.v[S_PREV_SLP + k];
+            }
+            q.cn[0] = r.v[S_CUR];
+            q.cn[1] = r.v[S_NEXT];
+            q.rot = 0;
+            q.cnp = 0;
+            q.is_converged = r.is_converged ? 1 : 0;
+            q.max_phi_coef_changed = r.max_phi_coef_changed ? 1 : 0;
+            q.n = r.n;
+            q.group = r.R.group;
+            q.step_length = r.step_length;
+            q.density_penalty = r.density_penalty;
+            q.max_phi_coef = r.max_phi_coef;
+            q.sum_overflow = r.sum_overflow;
+            q.sum_overflow_unscaled = r.sum_overflow_unscaled;
+            q.min_sum_overflow = r.min_sum_overflow;
+            q.hpwl_with_min_sum_overflow = r.hpwl_with_min_sum_overflow;
+            q.coordi_distance = r.coordi_distance;
+            q.peak_coordi_distance = r.peak_coordi_distance;
+            q.prev_reported_overflow_unscaled = r.prev_reported_overflow_unscaled;
+            q.wl_grad_sum = r.wl_grad_sum;
+            q.density_grad_sum = r.density_grad_sum;
+            q.prev_hpwl = r.prev_hpwl;
+            q.prev_reported_hpwl = r.prev_reported_hpwl;
+            q.overflow = r.overflow;
+            q.overflow_unscaled = r.overflow_unscaled;
+            q.nia = r.nia;
+            q.non_place_insts_area = r.R.non_place_insts_area;
+        }
+        gplk::ctl_refresh(g);
+        check(hipMemcpyAsync(d_ctl, h_ctl, sizeof(gplk::CtlHead), hipMemcpyHostToDevice, stream), "control upload");
+    }
+
+    // The control record back into the host's state after a halt: the progress rows logged and the
+    // history rows kept in order, every scalar, and each region's point sets in their rotation.
+    void ctl_apply() {
+        const gplk::CtlHead& g = h_ctl->h;
+        for (int i = 0; i < g.nlog; ++i) {
+            const gplk::CtlLog& L = h_ctl->log[i];
+            report_row(rg[L.region], L.iter, L.overflow, L.hpwl, L.pct, L.penalty);
+        }
+        for (int i = 0; i < g.nhist; ++i) {
+            const gplk::CtlHist& H = h_ctl->hist[i];
+            gpl_iter_row row{};
+            row.iter = H.iter;
+            row.trials = H.trials;
+            row.overflow = H.overflow;
+            row.density_penalty = H.density_penalty;
+            row.step_length = H.step_length;
+            row.wl_coef = H.wl_coef;
+            row.hpwl = H.hpwl;
+            hist.push_back(row);
+        }
+        if (g.nhist > 0) last_iter = h_ctl->hist[g.nhist - 1].iter;
+        wl_coef_x = g.wl_coef_x;
+        wl_coef_y = g.wl_coef_y;
+        average_overflow = g.average_overflow;
+        average_overflow_unscaled = g.average_overflow_unscaled;
+        diverge_snapshot_overflow = g.diverge_snapshot_overflow;
+        is_min_hpwl = g.is_min_hpwl != 0;
+        diverge_snapshot_iter = g.diverge_snapshot_iter;
+        iters_run = g.iters_run;
+        total_trials = g.total_trials;
+        min_hpwl = g.min_hpwl;
+        cur_hpwl = g.cur_hpwl;
+        last_global_adds = g.last_global_adds;
+        adds_base = g.adds_base;
+        ls.iter = g.iter;
+        ls.cur_a = g.cur_a;
+        trials = g.stage == 1 ? g.trials : (g.nhist > 0 ? h_ctl->hist[g.nhist - 1].trials : trials);
+        for (size_t b = 0; b < rg.size(); ++b) {
+            Region& r = rg[b];
+            const gplk::CtlRegion& q = g.reg[b];
+            const int ns = q.rot == 2 ? 0 : q.rot + 1, ps = ns == 2 ? 0 : ns + 1;
+            for (int k = 0; k < 4; ++k) {
+                r.v[S_CUR_SLP + k] = q.sets[q.rot][k];
+                r.v[S_NEXT_SLP + k] = q.sets[ns][k];
+                r.v[S_PREV_SLP + k] = q.sets[ps][k];
+            }
+            r.v[S_CUR] = q.cn[q.cnp];
+            r.v[S_NEXT] = q.cn[q.cnp ^ 1];
+            r.max_phi_coef_changed = q.max_phi_coef_changed != 0;
+            r.step_length = q.step_length;
+            r.density_penalty = q.density_penalty;
+            r.max_phi_coef = q.max_phi_coef;
+            r.sum_overflow = q.sum_overflow;
+            r.sum_overflow_unscaled = q.sum_overflow_unscaled;
+            r.min_sum_overflow = q.min_sum_overflow;
+            r.hpwl_with_min_sum_overflow = q.hpwl_with_min_sum_overflow;
+            r.coordi_distance = q.coordi_distance;
+            r.peak_coordi_distance = q.peak_coordi_distance;
+            r.prev_reported_overflow_unscaled = q.prev_reported_overflow_unscaled;
+            r.wl_grad_sum = q.wl_grad_sum;
+            r.density_grad_sum = q.density_grad_sum;
+            r.prev_hpwl = q.prev_hpwl;
+            r.prev_reported_hpwl = q.prev_reported_hpwl;
+            r.overflow = q.overflow;
+            r.overflow_unscaled = q.overflow_unscaled;
+        }
+    }
+
+    // Runs trials on the device from iteration ls.iter (its momentum `coeff` taken) until the device
+    // halts, with ctl_depth graphs in flight; the host's state then follows the device's.  Returns
+    // the halt's stage (1: the host finishes the halted trial and the iteration; 2: the host takes
+    // the iteration from isDiverged; 3: the iteration ended), or 0 when the graph cannot be built
+    // (the host runs the iteration).
+    int run_device_trials(float coeff, int32_t stop_at) {
+        if (!d_ctl) {
+            d_ctl = dalloc<gplk::CtlGlobal>(1);
+            check(hipHostMalloc((void**)&h_ctl, sizeof(gplk::CtlGlobal), hipHostMallocDefault), "pinned");
+        }
+        if (ctl_launch == 0 && !cexec && !build_ctl_graph()) {
+            ctl_failed = true;
+            return 0;
+        }
+        if (ctl_launch != 0 && !ctl_ready) ctl_prepare();
+        const auto t0 = std::chrono::steady_clock::now();
+        join();
+        ctl_upload(coeff, stop_at);
+        gplk::launch_set_u64(stream, d_hpwl, 0ull);  // the first trial's HPWL counter
+        const unsigned long long base = seq_want;
+        unsigned long long launched = 0, done = 0;
+        bool halted = false;
+        for (;;) {
+            while (!halted && launched - done < (unsigned long long)ctl_depth) {
+                if (ctl_launch == 0) check(hipGraphLaunch(cexec, stream), "graph launch");
+                else enqueue_ctl_trial(ctl_launch == 1);
+                ++launched;
+            }
+            if (done == launched) break;
+            wait_seq(base + done + 1);
+            ++done;
+            if (!halted && *reinterpret_cast<volatile unsigned long long*>(h_seq + 1) != 0ull) halted = true;
+        }
+        seq_want = base + launched;
+        ctl_graphs += (int64_t)launched;
+        ++ctl_halts;
+        check(hipMemcpyAsync(h_ctl, d_ctl, sizeof(gplk::CtlGlobal), hipMemcpyDeviceToHost, stream), "control download");
+        sync_stats();
+        ctl_apply();
+        ctl_host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return h_ctl->h.stage;
+    }
+
+
     void drop_graph() {
         if (texec) hipGraphExecDestroy(texec);
         if (tgraph) hipGraphDestroy(tgraph);
@@ -621,6 +936,15 @@ struct gpl_ctx {
         tgraph = nullptr;
         tnodes.clear();
         tgraph_key = ~0ull;
+        drop_ctl_graph();
+    }
+
+    void drop_ctl_graph() {
+        if (cexec) hipGraphExecDestroy(cexec);
+        if (cgraph) hipGraphDestroy(cgraph);
+        cexec = nullptr;
+        cgraph = nullptr;
+        ctl_ready = false;
     }
 
     uint64_t live_regions_key() const {
@@ -730,12 +1054,17 @@ struct gpl_ctx {
         return true;
     }
 
-    void do_back_tracking(float coeff) {
-        trials = 0;
-        for (int nbt = 0; nbt < 10; ++nbt) {
+    // `resume` > 0: the iteration's first resume - 1 trials ran on the device and trial `resume`
+    // did too, up to its gradient sums (in d_sums): the host takes it from there.
+    void do_back_tracking(float coeff, int resume = 0) {
+        trials = resume > 0 ? resume - 1 : 0;
+        for (int nbt = trials; nbt < 10; ++nbt) {
             ++trials;
             num_region_diverged = 0;
-            if (!trial_by_graph(coeff)) {
+            if (resume > 0 && nbt == resume - 1) {
+                sync_stats();
+                gradients_read(0, int(rg.size()), SLOT_NEXT, true);
+            } else if (!trial_by_graph(coeff)) {
                 update_coordinates(coeff);
                 field();
                 wa_forward(false, true);
@@ -790,6 +1119,18 @@ struct gpl_ctx {
         log.report("---------------------------------------------------------------");
     }
 
+    // A region's progress row (every tenth iteration), after the table's header at the first.
+    void report_row(Region& r, int iter, float overflow, int64_t hpwl, float pct, float penalty) {
+        std::string group = r.R.group >= 0 ? strfmt(" (group%d)", r.R.group) : std::string();
+        if ((iter == 0 || r.reprint_header) && r.R.group < 0) {
+            if (iter == 0) log.info(31, "HPWL: Half-Perimeter Wirelength");
+            report_header();
+            r.reprint_header = false;
+        }
+        log.report(strfmt("%9d | %8.4f | %13.6e | %+7.2f%% | %9.2e | %5s", iter, overflow, (double)hpwl / (double)dbu, pct, (double)penalty,
+                          group.c_str()));
+    }
+
     // NesterovBase::updateNextIter (nesterov.rs:2837).
     void nb_update_next_iter(int b, int iter) {
         Region& r = rg[b];
@@ -812,14 +1153,7 @@ struct gpl_ctx {
             if (r.prev_reported_hpwl != 0) pct = (float)((double)(hpwl - r.prev_reported_hpwl) / (double)r.prev_reported_hpwl * 100.0);
             r.prev_reported_hpwl = hpwl;
             r.prev_reported_overflow_unscaled = r.sum_overflow_unscaled;
-            std::string group = r.R.group >= 0 ? strfmt(" (group%d)", r.R.group) : std::string();
-            if ((iter == 0 || r.reprint_header) && r.R.group < 0) {
-                if (iter == 0) log.info(31, "HPWL: Half-Perimeter Wirelength");
-                report_header();
-                r.reprint_header = false;
-            }
-            log.report(strfmt("%9d | %8.4f | %13.6e | %+7.2f%% | %9.2e | %5s", iter, r.sum_overflow_unscaled, (double)hpwl / (double)dbu, pct,
-                              (double)r.density_penalty, group.c_str()));
+            report_row(r, iter, r.sum_overflow_unscaled, hpwl, pct, r.density_penalty);
         }
         float scaled = (float)(hpwl - r.prev_hpwl) / reference_hpwl;
         float phi = phi_coef(r, scaled);
@@ -931,6 +1265,7 @@ struct gpl_ctx {
         mark_end();
         host_map_stale = true;
         ++resorts;
+        drop_ctl_graph();  // it holds the old cell and net arrays
     }
 
     // The scatter's cost follows its count of global atomic adds: one per distinct bin a chunk of
@@ -1304,9 +1639,12 @@ struct gpl_ctx {
         return GPL_OK;
     }
 
-    // At most n iterations of the loop body (nesterov.rs:4478-4575).
+    // At most n iterations of the loop body (nesterov.rs:4478-4575).  Each pass is one iteration
+    // (ls.iter + 1 at its end, or the loop ends); with device-driven trials the device runs whole
+    // iterations until it halts, and the host finishes the one it halted in.
     void loop_step(int32_t n) {
-        for (int32_t k = 0; k < n && ls.active && !ls.done; ++k) {
+        const int32_t stop_at = n >= INT32_MAX - ls.iter ? INT32_MAX : ls.iter + n;
+        while (ls.active && !ls.done && ls.iter < stop_at) {
             if (ls.iter >= max_iter) {
                 ls.done = true;
                 break;
@@ -1315,17 +1653,31 @@ struct gpl_ctx {
             ls.cur_a = (float)((1.0 + std::sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
             float coeff = (float)(((double)prev_a - 1.0) / (double)ls.cur_a);
             iter_now = ls.iter;
-            do_back_tracking(coeff);
-            total_trials += trials;
-            for (auto& r : rg) adjust_phi(r);
-            if (num_region_diverged > 0) {
-                ls.done = true;
-                break;
+            int resume = 0;
+            bool booked = false;  // the iteration's bookkeeping (up to the history row) is done
+            if (ctl_usable()) {
+                const int stage = run_device_trials(coeff, stop_at);
+                iter_now = ls.iter;
+                if (stage == 3) continue;
+                if (stage == 2) booked = true;
+                if (stage == 1) {
+                    resume = h_ctl->h.trials + 1;
+                    coeff = h_ctl->h.coeff;
+                }
             }
-            np_update_next_iter(ls.iter);
-            push_history(ls.iter);
-            last_iter = ls.iter;
-            ++iters_run;
+            if (!booked) {
+                do_back_tracking(coeff, resume);
+                total_trials += trials;
+                for (auto& r : rg) adjust_phi(r);
+                if (num_region_diverged > 0) {
+                    ls.done = true;
+                    break;
+                }
+                np_update_next_iter(ls.iter);
+                push_history(ls.iter);
+                last_iter = ls.iter;
+                ++iters_run;
+            }
             int act = np_is_diverged(&ls.cur_a);
             if (act == 2) {
                 ls.done = true;
@@ -1738,8 +2090,9 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
         c->d_hpwl = &c->d_sums[p->num_regions].hpwl;
         c->d_seq = c->dalloc<unsigned long long>(1);
         check(hipMemsetAsync(c->d_seq, 0, sizeof(unsigned long long), c->stream), "memset");
-        check(hipHostMalloc((void**)&c->h_seq, sizeof(unsigned long long), hipHostMallocDefault), "pinned");
-        *c->h_seq = 0ull;
+        // [0] the sequence number, [1] the device-driven trials' halt event
+        check(hipHostMalloc((void**)&c->h_seq, 2 * sizeof(unsigned long long), hipHostMallocDefault), "pinned");
+        c->h_seq[0] = c->h_seq[1] = 0ull;
         c->d_global_adds = (unsigned long long*)&c->d_sums[p->num_regions].pad[0];
 
         // the region table of the re-sort; per-region scratch vectors for its gathers
@@ -2155,9 +2508,9 @@ int32_t gpl_debug_inject(gpl_ctx* c, int32_t kind, int32_t after_iter) {
             if (kind == 4) c->tune[b].stage = 0;
             c->apply_tune(int(b));
         }
+        c->drop_graph();  // the graphs hold the kernels' old paths
         if (kind == 4) {
             c->autotune = 0;
-            c->drop_graph();
             return GPL_OK;
         }
     }
@@ -2208,7 +2561,7 @@ int32_t gpl_get_tuning(gpl_ctx* c, int32_t* out, int32_t cap) {
 }
 
 int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode) {
-    if (!c || mode < 0 || mode > 3) return GPL_ERR_ARG;
+    if (!c || mode < 0 || mode > 6) return GPL_ERR_ARG;
     try {
         c->join();
         check(hipStreamSynchronize(c->stream), "sync");
@@ -2216,8 +2569,10 @@ int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode) {
         return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
     }
     c->drop_graph();
-    c->launch_mode = mode == 3 ? 0 : mode;
-    c->overlap = mode < 2;
+    c->launch_mode = mode >= 3 ? 0 : mode;
+    c->overlap = mode < 2 || mode >= 4;
+    c->device_trials = mode >= 4;
+    c->ctl_launch = mode >= 4 ? mode - 4 : 0;
     return GPL_OK;
 }
 
@@ -2270,6 +2625,21 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap) {
         out[n].launches = c->tune_runs;
         out[n].ms = c->tune_ms;
         ++n;
+    }
+    if (n < cap && c->ctl_halts > 0) {
+        // device-driven trials: the device's halts in `launches`, the host's time from each entry
+        // to the halt's state taken back in `ms`; then the trial graphs launched
+        std::memset(&out[n], 0, sizeof out[n]);
+        std::snprintf(out[n].name, sizeof out[n].name, "device_halts");
+        out[n].launches = int32_t(c->ctl_halts);
+        out[n].ms = c->ctl_host_ms;
+        ++n;
+        if (n < cap) {
+            std::memset(&out[n], 0, sizeof out[n]);
+            std::snprintf(out[n].name, sizeof out[n].name, "device_launched");
+            out[n].launches = int32_t(c->ctl_graphs);
+            ++n;
+        }
     }
     for (int h = 0; h < gpl_ctx::H_COUNT && n < cap && c->host_n > 0; ++h, ++n) {
         std::memset(&out[n], 0, sizeof out[n]);

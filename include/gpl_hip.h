@@ -453,7 +453,14 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap);
 /* How a back-tracking trial is launched (no reference counterpart; the results are the same in
  * every mode): 0 (default) one HIP graph per trial, the bins and the Poisson solve on a branch
  * beside the WA kernels, the high-fanout nets on a third; 1 the same kernels launched one by one
- * on those streams; 2 one by one on one stream; 3 one HIP graph per trial captured on one stream. */
+ * on those streams; 2 one by one on one stream; 3 one HIP graph per trial captured on one stream.
+ * Device-driven trials (doBackTracking, nesterovUpdateStepLength, the phi / WA-coefficient
+ * schedule and updateNextIter's bookkeeping, nesterov.rs:2750-2930, taken by the trial's last
+ * kernel, so that trials follow each other without a host round trip; the host steps in where the
+ * device halts on an event it handles): 4 as a HIP graph, 5 launched one by one on the side
+ * streams, 6 on one stream.  gpl_kernel_times then reports "device_halts" (the halts, and the
+ * host's time from each entry to the halt's state taken back) and "device_launched" (trials
+ * launched, including those that found the device halted and did nothing). */
 int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode);
 
 /* The autotuner (no reference counterpart; north star: tile shapes picked by an autotuner the project

@@ -7,7 +7,15 @@ fixtures (tests/golden/loop_<config>.json, written by tests/golden/make_loop_fix
 
 Bars (the headline's): both runs reach overflow 0.10, the final pin HPWL within 0.5 % of the
 oracle's, the iteration count within 2 % (the GPU's FP64 reductions and FFT rounding move the
-trajectory slightly), and the first progress row within 1e-4 (HPWL) and 1e-4 (overflow)."""
+trajectory slightly), and the first progress row within 1e-4 (HPWL) and 1e-4 (overflow).
+
+Where the loop itself is chaotic, the bar is the oracle's own spread: a fixture may hold the
+oracle's runs with a relative noise of 2e-7 added to the field every iteration (below the rounding
+difference between two FP32 FFTs; make_loop_fixtures.py --perturb), and the HPWL and iteration bars
+widen to 1.25 x the largest move those runs show.  On the region groups that spread is +0.27 %,
++2.14 % and +3.11 % (HPWL; the unperturbed run's own trajectory lies at one edge of it), so their
+HPWL bar is 3.9 %; on the 200k-cell design the same noise moves the HPWL by +0.036 % and +0.003 %,
+and the 0.5 % bar stands."""
 import json
 import os
 
@@ -52,10 +60,13 @@ def test_loop_matches_the_oracles_full_run(cfg, request):
     delta = (s.hpwl - fx["final_hpwl"]) / fx["final_hpwl"]
     print(f"{cfg}: GPU iteration {it} HPWL {s.hpwl} overflow {s.overflow:.4f}; oracle iteration {fx['final_iter']} "
           f"HPWL {fx['final_hpwl']} overflow {fx['final_overflow']:.4f}; delta {100 * delta:+.4f} %")
-    assert s.converged and fx["converged"]
+    pert = fx.get("perturbed", [])
+    hpwl_env = max([abs(p["final_hpwl"] - fx["final_hpwl"]) / fx["final_hpwl"] for p in pert], default=0.0)
+    iter_env = max([abs(p["final_iter"] - fx["final_iter"]) for p in pert], default=0)
+    assert s.converged and fx["converged"] and all(p["converged"] for p in pert)
     assert s.overflow <= 0.1 and fx["final_overflow"] <= 0.1
-    assert abs(delta) <= 0.005
-    assert abs(it - fx["final_iter"]) <= max(10, 0.02 * fx["final_iter"])
+    assert abs(delta) <= max(0.005, 1.25 * hpwl_env), (delta, hpwl_env)
+    assert abs(it - fx["final_iter"]) <= max(10, 0.02 * fx["final_iter"], 1.25 * iter_env)
     r0 = fx["rows"][0]
     assert hist[0]["iter"] == r0["iter"]
     assert abs(hist[0]["hpwl"] - r0["hpwl"]) <= 1e-4 * r0["hpwl"]
