@@ -255,15 +255,21 @@ __global__ void k_spmv(int n, const int* start, const int* col, const float* val
     }
 }
 
-// the host's FP64 dot: sequential sums over 4096-element chunks, then the chunks in order
-__global__ void k_dot_chunks(int n, const float* a, const float* b, double* part) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nch = (n + kDotChunk - 1) / kDotChunk;
-    if (c >= nch) return;
-    const int e = min(n, (c + 1) * kDotChunk);
-    double s = 0.0;
-    for (int i = c * kDotChunk; i < e; ++i) s += double(a[i]) * double(b[i]);
-    part[c] = s;
+// the host's FP64 dot: sequential sums over 4096-element chunks, then the chunks in order.  One wave
+// per chunk: the products (exact in FP64) are formed from coalesced loads into LDS, then one lane
+// sums them in order.
+__global__ void __launch_bounds__(64) k_dot_chunks(int n, const float* a, const float* b, double* part) {
+    __shared__ double prod[kDotChunk];
+    const int c = blockIdx.x;
+    const int b0 = c * kDotChunk;
+    const int m = min(n - b0, kDotChunk);
+    for (int i = threadIdx.x; i < m; i += 64) prod[i] = double(a[b0 + i]) * double(b[b0 + i]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < m; ++i) s += prod[i];
+        part[c] = s;
+    }
 }
 
 __global__ void k_dot_finish(int nch, const double* part, double* out) {
@@ -320,7 +326,8 @@ struct DevSolver {
     }
     double dot(const float* a, const float* b) {
         const int nch = (n + kDotChunk - 1) / kDotChunk;
-        hipLaunchKernelGGL(k_dot_chunks, dim3((nch + 63) / 64), dim3(64), 0, 0, n, a, b, part.p);
+        if (nch == 0) return 0.0;
+        hipLaunchKernelGGL(k_dot_chunks, dim3(nch), dim3(64), 0, 0, n, a, b, part.p);
         hipLaunchKernelGGL(k_dot_finish, dim3(1), dim3(64), 0, 0, nch, part.p, res.p);
         ck(hipMemcpy(pinned, res.p, sizeof(double), hipMemcpyDeviceToHost), "dot");
         return *pinned;
