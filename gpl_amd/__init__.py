@@ -86,6 +86,8 @@ CONFIGS = {
                 opts=dict(bin_cnt=0, density=0.70, initial_place_max_iter=5)),
     "c4s": dict(gen=dict(num_cells=50_000, num_nets=52_000, degree_p=0.5, degree_cap=64, num_io=128, utilization=0.65),
                 opts=dict(bin_cnt=256, density=0.70, initial_place_max_iter=5)),
+    "c5": dict(gen=dict(num_cells=500_000, num_nets=520_000, degree_p=0.5, degree_cap=64, num_io=512, utilization=0.65),
+               opts=dict(bin_cnt=768, density=0.70, initial_place_max_iter=5)),
     "c4r_small": dict(gen=dict(num_cells=1_000, num_nets=18_000, degree_p=0.5, degree_cap=64, num_io=96, utilization=0.65,
                                group_cells=[250, 500, 1_000, 1_500, 2_000, 3_000, 4_000, 5_000]),
                       opts=dict(bin_cnt=0, density=0.70, initial_place_max_iter=5)),

@@ -577,15 +577,17 @@ __device__ __forceinline__ void st2(lds_float2* p, float2 v) {
     *p = t;
 }
 
-// Complex FFTs of N points (a power of two), NB of them side by side (each held in LDS buffer
-// x[b]; the NB transforms share the twiddles and the barriers), by T = max(N/4, 1) threads of a
-// row group (tid < T), Stockham autosort: radix-2 first when log2 N is odd, then radix-4 stages;
-// natural order in and out.  tw[m] = e^{-2 pi i m / N}, m < N (TP: an LDS or a global pointer);
-// dir -1 forward, +1 inverse (unnormalised).  LOGN > 0 fixes N = 2^LOGN at compile time (the
-// stages unroll, the index arithmetic folds); 0 takes N and logN as given.  Every thread of the
-// block calls it (block barriers inside); on return x[b] is the buffer that holds result b.
+// Complex FFTs of N points (N = 2^logN, or 3 * 2^logN with r3), NB of them side by side (each held
+// in LDS buffer x[b]; the NB transforms share the twiddles and the barriers), by T = max(N/4, 1)
+// threads of a row group (tid < T), Stockham autosort: radix-2 first when logN is odd, then radix-4
+// stages, then with r3 one radix-3 stage (its sub-transforms of length 2^logN, twiddles w^k and
+// w^2k); natural order in and out.  tw[m] = e^{-2 pi i m / N}, m < N (TP: an LDS or a global
+// pointer); dir -1 forward, +1 inverse (unnormalised).  LOGN > 0 fixes N = 2^LOGN at compile time
+// (the stages unroll, the index arithmetic folds); 0 takes N, logN and r3 as given.  Every thread
+// of the block calls it (block barriers inside); on return x[b] is the buffer that holds result b.
 template <int LOGN, int NB, typename TP>
-__device__ void fft_stockham_n(lds_float2* (&x)[NB], lds_float2* (&y)[NB], int N_, int logN_, int T_, int tid, TP tw, int dir) {
+__device__ void fft_stockham_n(lds_float2* (&x)[NB], lds_float2* (&y)[NB], int N_, int logN_, int T_, int tid, TP tw, int dir,
+                               int r3 = 0) {
     const int logN = LOGN > 0 ? LOGN : logN_;
     const int N = LOGN > 0 ? (1 << LOGN) : N_;
     const int T = LOGN > 0 ? (LOGN >= 2 ? (1 << (LOGN - 2)) : 1) : T_;
@@ -670,21 +672,50 @@ __device__ void fft_stockham_n(lds_float2* (&x)[NB], lds_float2* (&y)[NB], int N
         swap_all();
         __syncthreads();
     }
+    if (LOGN == 0 && r3) {
+        // the radix-3 stage: X0 = a + b + c, X1,2 = a - (b + c)/2 -/+ i (sqrt(3)/2)(b - c) forward
+        // (the signs swapped inverse), b and c twiddled by w^k, w^2k
+        const int third = N / 3;
+        const float s3 = 0.86602540378443864676f;
+        for (int j = tid; j < third; j += T) {
+            float2 w1 = ld2(tw + j), w2 = ld2(tw + 2 * j);
+            if (dir > 0) {
+                w1 = cconj(w1);
+                w2 = cconj(w2);
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float2 a0 = ld2(x[b] + j);
+                const float2 b0 = cmul(ld2(x[b] + j + third), w1), c0 = cmul(ld2(x[b] + j + 2 * third), w2);
+                const float2 t1 = make_float2(b0.x + c0.x, b0.y + c0.y);
+                const float2 t2 = make_float2(a0.x - 0.5f * t1.x, a0.y - 0.5f * t1.y);
+                const float2 t3 = make_float2(s3 * (b0.x - c0.x), s3 * (b0.y - c0.y));
+                const float2 m = make_float2(t2.x + t3.y, t2.y - t3.x);  // t2 - i t3
+                const float2 p = make_float2(t2.x - t3.y, t2.y + t3.x);  // t2 + i t3
+                st2(y[b] + j, make_float2(a0.x + t1.x, a0.y + t1.y));
+                st2(y[b] + j + third, dir < 0 ? m : p);
+                st2(y[b] + j + 2 * third, dir < 0 ? p : m);
+            }
+        }
+        swap_all();
+        __syncthreads();
+    }
 }
 
 template <int LOGN, typename TP>
-__device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N_, int logN_, int T_, int tid, TP tw, int dir) {
+__device__ lds_float2* fft_stockham(lds_float2* x, lds_float2* y, int N_, int logN_, int T_, int tid, TP tw, int dir, int r3 = 0) {
     lds_float2* xs[1] = {x};
     lds_float2* ys[1] = {y};
-    fft_stockham_n<LOGN, 1>(xs, ys, N_, logN_, T_, tid, tw, dir);
+    fft_stockham_n<LOGN, 1>(xs, ys, N_, logN_, T_, tid, tw, dir, r3);
     return xs[0];
 }
 
 // A row's transform context.  LOGL > 0 fixes the row length L = 2^LOGL (and with it h = L/2, the
-// T = h/4 threads per row, the FFT's stages) at compile time; 0 takes them from the fields.
+// T = h/4 threads per row, the FFT's stages) at compile time; 0 takes them from the fields (h =
+// 2^logh, or 3 * 2^logh with r3).
 template <typename TP, int LOGL>
 struct RowCtx {
-    int L_, h_, logh_, T_, tid;
+    int L_, h_, logh_, T_, r3_, tid;
     TP tw;  // e^{-2 pi i m / h}, m < h
     TP th;  // e^{-2 pi i k / L}, k <= h
     TP dw;  // (cos, -sin)(pi k / 2L), k < L
@@ -694,6 +725,7 @@ struct RowCtx {
     __device__ __forceinline__ int h() const { return LOGL > 0 ? (1 << (LOGL - 1)) : h_; }
     __device__ __forceinline__ int logh() const { return LOGL > 0 ? LOGL - 1 : logh_; }
     __device__ __forceinline__ int T() const { return LOGL > 0 ? (LOGL >= 3 ? (1 << (LOGL - 3)) : 1) : T_; }
+    __device__ __forceinline__ int r3() const { return LOGL > 0 ? 0 : r3_; }
 };
 
 // DCT-II of the real row given by src(j), j < L (Makhoul: v = (x[0], x[2], ..., x[3], x[1]), its
@@ -712,7 +744,7 @@ __device__ void dct2_row(const RowCtx<TP, LOGL>& R, bool active, SRC src, OUT ou
             st2(R.A + m, make_float2(a, b));
         }
     }
-    lds_float2* Z = fft_stockham<(LOGL > 0 ? LOGL - 1 : 0)>(R.A, R.B, h, R.logh(), T, R.tid, R.tw, -1);
+    lds_float2* Z = fft_stockham<(LOGL > 0 ? LOGL - 1 : 0)>(R.A, R.B, h, R.logh(), T, R.tid, R.tw, -1, R.r3());
     if (active) {
         auto post = [&](int k) {
             float2 zk = ld2(Z + (k == h ? 0 : k)), zc = cconj(ld2(Z + (k == 0 ? 0 : h - k)));
@@ -786,7 +818,7 @@ __device__ __forceinline__ void dct3_post(const RowCtx<TP, LOGL>& R, const lds_f
 template <typename TP, int LOGL, typename F, typename OUT>
 __device__ void dct3_row(const RowCtx<TP, LOGL>& R, bool active, F coef, bool sine, OUT out) {
     if (active) dct3_pre(R, R.A, coef, sine);
-    lds_float2* z = fft_stockham<(LOGL > 0 ? LOGL - 1 : 0)>(R.A, R.B, R.h(), R.logh(), R.T(), R.tid, R.tw, +1);
+    lds_float2* z = fft_stockham<(LOGL > 0 ? LOGL - 1 : 0)>(R.A, R.B, R.h(), R.logh(), R.T(), R.tid, R.tw, +1, R.r3());
     if (active) dct3_post(R, z, sine, out);
     __syncthreads();
 }
@@ -802,7 +834,7 @@ __device__ void dct3_row2(const RowCtx<TP, LOGL>& R, lds_float2* A1, lds_float2*
     }
     lds_float2* x[2] = {R.A, A1};
     lds_float2* y[2] = {R.B, B1};
-    fft_stockham_n<(LOGL > 0 ? LOGL - 1 : 0), 2>(x, y, R.h(), R.logh(), R.T(), R.tid, R.tw, +1);
+    fft_stockham_n<(LOGL > 0 ? LOGL - 1 : 0), 2>(x, y, R.h(), R.logh(), R.T(), R.tid, R.tw, +1, R.r3());
     if (active) {
         dct3_post(R, x[0], sine0, out0);
         dct3_post(R, x[1], sine1, out1);
@@ -817,6 +849,7 @@ struct RowArgs {
     float* out1;
     float* out2;
     int L, logh, nrows, T;
+    int r3;              // the half length h = 3 * 2^logh (else 2^logh)
     const float2* tw;
     const float2* th;
     const float2* dw;
@@ -859,6 +892,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     R.h_ = a.L >> 1;
     R.logh_ = a.logh;
     R.T_ = a.T;
+    R.r3_ = a.r3;
     const int T = R.T(), L = R.L(), h = R.h();
     const int rpb = LOGL > 0 ? 256 / T : (int)blockDim.x / T;
     const int sub = threadIdx.x / T;
@@ -910,7 +944,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     // output column (a 4-byte store per element into a different line otherwise).
     const bool group = tr && rpb >= 2 && (rpb & 1) == 0 && nrows % rpb == 0;
     const int lh = R.logh();
-    const bool res_b = (((lh & 1) + (lh >> 1)) & 1) != 0;
+    const bool res_b = (((lh & 1) + (lh >> 1) + R.r3()) & 1) != 0;
     auto stage_of = [&](bool second, int s) -> lds_float* {
         lds_float2* base = (second ? dbase : lbase) + (size_t)s * L;
         return (lds_float*)(res_b ? base : base + h);
@@ -2214,6 +2248,45 @@ void launch_zero_bins(hipStream_t s, const RegionDev& r) {
     hipLaunchKernelGGL(k_zero_bins, dim3(grid_for(M)), dim3(kBlock), 0, s, r.inst, r.filler, M);
 }
 
+// The scatter of the stored density boxes with every overlap added straight to the bins in global
+// memory (int64 atomics: the sums are exact in any order, so the bins are those of the tiled form):
+// one thread per cell, no LDS tables.
+__global__ void __launch_bounds__(kBlock) k_place_scatter_direct(RegionDev r, CellsDev c, PlaceArgs pa) {
+    if (r.ctl && (r.ctl->hot.flags & 1)) return;
+    __builtin_amdgcn_s_setprio(2);
+    const int n = r.cell_end - r.cell_begin;
+    unsigned adds = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int g = r.cell_begin + k;
+        const int4 d = c.dbox[g];
+        const uint8_t fl = c.flags[g];
+        const float sc = c.dscale[g];
+        int x0, x1, y0, y1;
+        bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
+        bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
+        const bool inst = fl & 1u, macro = inst && (fl & 2u);
+        unsigned long long* glob = inst ? r.inst : r.filler;
+        for (int y = y0; y < y1; ++y) {
+            const int ey0 = bin_edge(r.bin_y, r.edges_rounded, r.ly, r.size_y, r.cnt_y, y);
+            const int ey1 = bin_edge(r.bin_y, r.edges_rounded, r.ly, r.size_y, r.cnt_y, y + 1);
+            for (int x = x0; x < x1; ++x) {
+                const int ex0 = bin_edge(r.bin_x, r.edges_rounded, r.lx, r.size_x, r.cnt_x, x);
+                const int ex1 = bin_edge(r.bin_x, r.edges_rounded, r.lx, r.size_x, r.cnt_x, x + 1);
+                float a = overlap_area(ex0, ey0, ex1, ey1, d) * sc;
+                if (macro) a *= r.target_density;
+                const long long v = (long long)a;  // the reference's `as i64`
+                if (v == 0) continue;
+                atomicAdd(glob + ((unsigned)y * (unsigned)r.cnt_x + (unsigned)x), (unsigned long long)v);
+                ++adds;
+            }
+        }
+    }
+    if (pa.global_adds) {
+        for (int o = 32; o > 0; o >>= 1) adds += __shfl_xor(adds, o, 64);
+        if ((threadIdx.x & 63) == 0 && adds) atomicAdd(pa.global_adds, (unsigned long long)adds);
+    }
+}
+
 template <int MODE, int CH>
 static void place_scatter_ch(hipStream_t s, const RegionDev& r, const CellsDev& c, const PlaceArgs& pa, int grid) {
     if (r.narrow) hipLaunchKernelGGL((k_place_scatter<MODE, unsigned, CH>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
@@ -2223,6 +2296,11 @@ static void place_scatter_ch(hipStream_t s, const RegionDev& r, const CellsDev& 
 template <int MODE>
 static void place_scatter_mode(hipStream_t s, const RegionDev& r, const CellsDev& c, const PlaceArgs& pa) {
     const int n = r.cell_end - r.cell_begin;
+    if (MODE == PLACE_STORED && r.chunk == 1) {
+        hipLaunchKernelGGL(k_place_scatter_direct, dim3(round_xcds(std::min((n + kBlock - 1) / kBlock, 8192))), dim3(kBlock), 0, s, r,
+                           c, pa);
+        return;
+    }
     const int ch = r.chunk == 256 ? 256 : kPlaceChunk;
     const int chunks = (n + ch - 1) / ch;
     const int grid = round_xcds(chunks < 4096 ? chunks : 4096);
@@ -2321,7 +2399,8 @@ static int rows_per_block(int L) {
 static void launch_rows(hipStream_t s, int mode, RowArgs a) {
     const int h = a.L >> 1;
     a.T = h >= 4 ? h >> 2 : 1;  // L/8 threads per row (one for the tiny lengths)
-    a.logh = ilog2(h);
+    a.r3 = (h & (h - 1)) != 0 ? 1 : 0;  // h = 3 * 2^k (the grid check admits no other)
+    a.logh = ilog2(a.r3 ? h / 3 : h);
     const int rpb = rows_per_block(a.L);
     const int block = a.T * rpb;
     const int grid = (a.nrows + rpb - 1) / rpb;

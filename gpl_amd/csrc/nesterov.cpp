@@ -1770,6 +1770,9 @@ void make_tables(int L, std::vector<float2>& tw, std::vector<float2>& th, std::v
 }
 
 bool pow2(int v) { return v >= 2 && (v & (v - 1)) == 0; }
+// a bin count the solve transforms: 2^k (2..2048), or 3 * 2^k (24..1536: the row FFT's half
+// length then takes one radix-3 stage)
+bool fft_len_ok(int v) { return v <= 2048 && (pow2(v) || (v % 3 == 0 && pow2(v / 3) && v / 3 >= 8)); }
 
 int32_t fail(gpl_ctx* c, int32_t code, const std::string& msg) {
     if (c) c->err = msg;
@@ -1790,8 +1793,9 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
     }
     for (int b = 0; b < p->num_regions; ++b) {
         const gpl_region& R = p->regions[b];
-        if (!pow2(R.cnt_x) || !pow2(R.cnt_y) || R.cnt_x > 2048 || R.cnt_y > 2048) {
-            fprintf(stderr, "gpl_ctx_create: bin grid %d x %d is not a power of two in [2, 2048]\n", R.cnt_x, R.cnt_y);
+        if (!fft_len_ok(R.cnt_x) || !fft_len_ok(R.cnt_y)) {
+            fprintf(stderr, "gpl_ctx_create: bin grid %d x %d: each side must be 2^k (2..2048) or 3 * 2^k (24..1536)\n", R.cnt_x,
+                    R.cnt_y);
             return GPL_ERR_REFUSED;
         }
     }
@@ -2525,7 +2529,7 @@ int32_t gpl_set_tuning(gpl_ctx* c, int32_t region, int32_t chunk, int32_t table_
         c->autotune = chunk != 0 ? 1 : 0;  // region -1: the tuner on (chunk 1) or off (chunk 0)
         return GPL_OK;
     }
-    if (bad_region(c, region) || (chunk != 256 && chunk != gplk::kPlaceChunk) || (table_bits != 32 && table_bits != 64) ||
+    if (bad_region(c, region) || (chunk != 1 && chunk != 256 && chunk != gplk::kPlaceChunk) || (table_bits != 32 && table_bits != 64) ||
         (stage != 0 && stage != 1))
         return GPL_ERR_ARG;
     try {

@@ -14,8 +14,10 @@
  * twiddles from the C library in double, tabled once per length as Ooura's makewt/makect tables
  * are, ooura.rs:260-382), not Ooura's split-radix butterflies, so results agree
  * with the reference's to rounding, not bit for bit (the reference pins its own bits on macOS
- * only, ooura.rs:3124-3159).  Powers of two use the FFT; any other length falls back to the
- * direct O(n^2) sum with double accumulation.
+ * only, ooura.rs:3124-3159).  Powers of two use the FFT, and so do 3 * 2^k (the half-length FFT
+ * split by 3 in time, each third a power of two -- the reference itself takes powers of two only,
+ * ooura.rs:16-17, so these sizes are pinned by the direct sums and scipy alone); any other length
+ * falls back to the direct O(n^2) sum with double accumulation.
  *
  * Pinned by tests/test_oracle.py: brute-force O(n^2) FP64 sums, scipy.fft, forward-inverse
  * identity, and the single-cosine-mode eigenfunction of the Poisson solve.
@@ -29,6 +31,9 @@
 #define ORC_PI 3.14159265358979323846
 
 static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+/* the lengths the fast path transforms: 2^k, and 3 * 2^k (k >= 1: the half-length FFT then has a
+ * radix-3 step) */
+static int fast_len(int n) { return is_pow2(n) || (n % 6 == 0 && is_pow2(n / 6)); }
 
 /* ---- generic body, instantiated for float (f32) and double (f64) ---------------------------- */
 #define DEFINE_DCT(T, SFX)                                                                          \
@@ -40,11 +45,11 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
         int n;                                                                                      \
         T *fft, *wn, *w4n;                                                                          \
     } tab_t_##SFX;                                                                                  \
-    static tab_t_##SFX tabs_##SFX[32];                                                               \
+    static tab_t_##SFX tabs_##SFX[64];                                                               \
     static const tab_t_##SFX* tab_##SFX(int n) {                                                   \
         int lg = 0;                                                                                 \
         while ((1 << lg) < n) ++lg;                                                                 \
-        tab_t_##SFX* t = &tabs_##SFX[lg];                                                           \
+        tab_t_##SFX* t = &tabs_##SFX[lg + (is_pow2(n) ? 0 : 32)]; /* 3 * 2^k beside the powers */ \
         if (t->n == n) return t;                                                                    \
         T* buf = (T*)malloc(sizeof(T) * (2 * (size_t)n + 2 * ((size_t)n / 2 + 1) + 2 * (size_t)n + 2)); \
         t->fft = buf;                                                                               \
@@ -71,9 +76,15 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
         t->n = n;                                                                                   \
         return t;                                                                                   \
     }                                                                                               \
+    static void fft3_##SFX(int m, T* z, int sign);                                                \
     /* in-place radix-2 complex FFT of m points (re, im interleaved), sign -1 forward, +1 inverse  \
-     * (no scaling); twiddles exp(sign 2 pi i k / len) from the length's table */                   \
+     * (no scaling); twiddles exp(sign 2 pi i k / len) from the length's table; m = 3 * 2^j goes   \
+     * to fft3 */                                                                                   \
     static void fft_##SFX(int m, T* z, int sign) {                                                  \
+        if (!is_pow2(m)) {                                                                          \
+            fft3_##SFX(m, z, sign);                                                                 \
+            return;                                                                                 \
+        }                                                                                           \
         for (int i = 1, j = 0; i < m; ++i) {                                                        \
             int bit = m >> 1;                                                                       \
             for (; j & bit; bit >>= 1) j ^= bit;                                                    \
@@ -109,6 +120,33 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
                 }                                                                                   \
             }                                                                                       \
         }                                                                                           \
+    }                                                                                               \
+    /* m = 3 p points, p a power of two: decimation in time by 3 -- the three p-point FFTs of      \
+     * z[3j + r], then X[k] = Z0[k mod p] + w^k Z1[k mod p] + w^2k Z2[k mod p], w = e^{sign 2 pi i / m} \
+     * (each twiddle the cosine and sine of the double angle, rounded to T) */                      \
+    static void fft3_##SFX(int m, T* z, int sign) {                                                 \
+        const int p = m / 3;                                                                        \
+        T* s = (T*)malloc(sizeof(T) * 2 * (size_t)m);                                               \
+        for (int r = 0; r < 3; ++r)                                                                 \
+            for (int j = 0; j < p; ++j) {                                                           \
+                s[2 * (r * p + j)] = z[2 * (3 * j + r)];                                            \
+                s[2 * (r * p + j) + 1] = z[2 * (3 * j + r) + 1];                                    \
+            }                                                                                       \
+        for (int r = 0; r < 3; ++r) fft_##SFX(p, s + 2 * (size_t)r * p, sign);                      \
+        for (int k = 0; k < m; ++k) {                                                               \
+            const int kk = k % p;                                                                   \
+            T xr = s[2 * kk], xi = s[2 * kk + 1];                                                   \
+            for (int r = 1; r < 3; ++r) {                                                           \
+                const double ang = 2.0 * ORC_PI * (double)((long)r * k % m) / m;                    \
+                const T wr = (T)cos(ang), wi = (T)(sign * sin(ang));                                \
+                const T br = s[2 * (r * p + kk)], bi = s[2 * (r * p + kk) + 1];                     \
+                xr += wr * br - wi * bi;                                                            \
+                xi += wr * bi + wi * br;                                                            \
+            }                                                                                       \
+            z[2 * k] = xr;                                                                          \
+            z[2 * k + 1] = xi;                                                                      \
+        }                                                                                           \
+        free(s);                                                                                    \
     }                                                                                               \
     /* DFT V[k] = sum_m v[m] e^{-2 pi i m k / n}, k = 0..n/2, of a real sequence v (n even),        \
      * through an n/2-point complex FFT; out holds n/2 + 1 complex values */                        \
@@ -155,7 +193,7 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
     /* Ooura ddct isgn = -1 */                                                                      \
     void orc_dct2_##SFX(int n, T* a) {                                                              \
         if (n == 1) return;                                                                         \
-        if (!is_pow2(n)) {                                                                          \
+        if (!fast_len(n)) {                                                                          \
             direct_dct2_##SFX(n, a);                                                                \
             return;                                                                                 \
         }                                                                                           \
@@ -186,7 +224,7 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
      * v = IDFT(V) / 2 gives y[2m] = v[m], y[2m+1] = v[n-1-m]. */                                  \
     void orc_dct3_##SFX(int n, T* a) {                                                              \
         if (n == 1) return;                                                                         \
-        if (!is_pow2(n)) {                                                                          \
+        if (!fast_len(n)) {                                                                          \
             direct_dct3_##SFX(n, a);                                                                \
             return;                                                                                 \
         }                                                                                           \

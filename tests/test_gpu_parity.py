@@ -104,8 +104,11 @@ def _ctx(tiny):
     return g
 
 
-@pytest.mark.parametrize("nx,ny,bx,by", [(64, 64, 1116, 1125), (512, 256, 100, 200), (1024, 1024, 100, 100)])
+@pytest.mark.parametrize("nx,ny,bx,by", [(64, 64, 1116, 1125), (512, 256, 100, 200), (1024, 1024, 100, 100), (768, 768, 100, 100),
+                                         (96, 384, 300, 100), (24, 48, 1000, 700)])
 def test_solve_matches_fp64_within_1e4(nx, ny, bx, by):
+    """Powers of two and 3 * 2^k sides (the rows' half-length FFT with a radix-3 stage) against the
+    oracle's FP64 solve."""
     t = Tiny([dict(box=(0, 0, 10, 10))], [], regions=[dict(lx=0, ly=0, ux=nx * bx, uy=ny * by, cnt_x=nx, cnt_y=ny)])
     R = t.problem.regions[0]
     wx = np.ctypeslib.as_array(R.wx, (nx,)).copy()

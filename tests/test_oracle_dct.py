@@ -13,8 +13,10 @@ def _run(fn, x, dtype):
     return a
 
 
-@pytest.mark.parametrize("n", [2, 4, 8, 16, 64, 256, 6, 12])
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 64, 256, 6, 12, 24, 48, 96, 384, 768, 10])
 def test_1d_transforms_match_brute_force(n):
+    """Powers of two, 3 * 2^k (the radix-3 split of the half-length FFT) and a length neither
+    covers (10: the direct sums themselves)."""
     rng = np.random.default_rng(n)
     x = rng.standard_normal(n)
     for fast, direct in [("orc_dct2_f64", "orc_direct_dct2_f64"), ("orc_dct3_f64", "orc_direct_dct3_f64"),
@@ -26,7 +28,7 @@ def test_1d_transforms_match_brute_force(n):
         assert np.max(np.abs(a - b)) < 2e-6 * n * np.max(np.abs(x)), (f32, n)
 
 
-@pytest.mark.parametrize("n", [4, 32, 128])
+@pytest.mark.parametrize("n", [4, 32, 128, 12, 96, 768])
 def test_1d_transforms_match_scipy(n):
     rng = np.random.default_rng(100 + n)
     x = rng.standard_normal(n)
@@ -58,7 +60,7 @@ def _wavenumbers(nx, ny, bx, by):
     return wx, wy
 
 
-@pytest.mark.parametrize("nx,ny,bx,by", [(16, 16, 3.0, 3.0), (32, 16, 2.0, 3.0), (8, 32, 5.0, 1.25)])
+@pytest.mark.parametrize("nx,ny,bx,by", [(16, 16, 3.0, 3.0), (32, 16, 2.0, 3.0), (8, 32, 5.0, 1.25), (24, 48, 2.0, 1.0), (96, 96, 1.0, 1.0)])
 @pytest.mark.parametrize("u,v", [(1, 1), (3, 2), (0, 5), (4, 0), (7, 7)])
 def test_poisson_single_mode_is_an_eigenfunction(nx, ny, bx, by, u, v):
     """rho = cos(pi u (x+1/2)/nx) cos(pi v (y+1/2)/ny) must give phi = rho / (wx_u^2 + wy_v^2),
