@@ -251,6 +251,12 @@ def place_concurrently(contexts, max_workers=8, launch_mode=2):
         return list(ex.map(lambda g: g.do_nesterov_place(), contexts))
 
 
+def rudy_wire_width(pitches):
+    """Rudy::new's wire width (route_base.rs:143-152) from the routing layers' pitches."""
+    p = np.ascontiguousarray(pitches, np.int32)
+    return int(_ffi.lib().gpl_rudy_wire_width(p.ctypes.data_as(C.POINTER(C.c_int32)), len(p)))
+
+
 class NesterovGPU:
     """The Nesterov placer of one design on the GPU (gpl_ctx_*)."""
 
@@ -357,6 +363,32 @@ class NesterovGPU:
         arr = (KernelTime * 16)()
         n = _ffi.lib().gpl_kernel_times(self._h, arr, 16)
         return {a.name.decode(): (a.launches, a.ms) for a in arr[:n]}
+
+    # ---- routability's congestion measurement ----
+    def rudy(self, grid, cap_red=None):
+        """gpl_rudy: (demand, blockage) per tile, x-major (x * cnt_y + y); grid a dict of the
+        gpl_rudy_grid fields; cap_red optional (capacity, reduction) uint8 pairs per tile."""
+        g = _ffi.RudyGrid(**grid)
+        n = g.cnt_x * g.cnt_y
+        demand, blockage = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        cr = None if cap_red is None else np.ascontiguousarray(cap_red, np.uint8).reshape(-1)
+        _ok(_ffi.lib().gpl_rudy(self._h, C.byref(g), None if cr is None else cr.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                demand.ctypes.data_as(C.POINTER(C.c_float)), blockage.ctypes.data_as(C.POINTER(C.c_float))),
+            "gpl_rudy", self._h)
+        return demand, blockage
+
+    def rudy_tiles(self, grid, min_cong=0.95, coef=2.0, max_ratio=3.0, ignore_edge=0.8, rc_k=(1.0, 1.0, 0.0, 0.0)):
+        """gpl_rudy_tiles on the last rudy(): (inflation ratio, actionable) per tile (y-major) and
+        dict(total_overflow, overflowed, actionable, top (4), rc)."""
+        g = _ffi.RudyGrid(**grid)
+        v = _ffi.RouteVars(min_cong, coef, max_ratio, ignore_edge, (C.c_float * 4)(*rc_k))
+        n = g.cnt_x * g.cnt_y
+        ratio, act = np.zeros(n, np.float32), np.zeros(n, np.uint8)
+        st = _ffi.RudyStats()
+        _ok(_ffi.lib().gpl_rudy_tiles(self._h, C.byref(g), C.byref(v), ratio.ctypes.data_as(C.POINTER(C.c_float)),
+                                      act.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(st)), "gpl_rudy_tiles", self._h)
+        return ratio, act, dict(total_overflow=st.total_overflow, overflowed=st.overflowed_tiles, actionable=st.actionable_tiles,
+                                top=np.array(st.top[:], np.float64), rc=float(st.rc))
 
     # ---- kernel-level entries ----
     def _n(self, region):

@@ -102,6 +102,20 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 24), ("launches", i32), ("ms", f64)]
 
 
+class RudyGrid(C.Structure):
+    _fields_ = [("lx", i32), ("ly", i32), ("ux", i32), ("uy", i32), ("cnt_x", i32), ("cnt_y", i32), ("tile_size", i32),
+                ("wire_width", i32)]
+
+
+class RouteVars(C.Structure):
+    _fields_ = [("min_congestion_for_inflation", f32), ("inflation_ratio_coef", f32), ("max_inflation_ratio", f32),
+                ("ignore_edge_ratio", f32), ("rc_k", f32 * 4)]
+
+
+class RudyStats(C.Structure):
+    _fields_ = [("total_overflow", f64), ("overflowed_tiles", i32), ("actionable_tiles", i32), ("top", f64 * 4), ("rc", f32)]
+
+
 # (name, restype, argtypes) of every exported function declared in include/gpl_hip.h
 class DensityIO(C.Structure):
     """gpl_density_io: device pointers (as integers) of gpl_density_scatter_noalloc's arrays."""
@@ -149,6 +163,9 @@ SIGNATURES = [
     ("gpl_nesterov_step", i32, [C.c_void_p, i32, f32, f32, P(f32), P(f32)]),
     ("gpl_objective_forward", i32, [C.c_void_p, P(f32), f32, f32, P(f64), P(f64), P(f64)]),
     ("gpl_objective_backward", i32, [C.c_void_p, P(f32), f32, f32, P(f32)]),
+    ("gpl_rudy_wire_width", i32, [P(i32), i32]),
+    ("gpl_rudy", i32, [C.c_void_p, P(RudyGrid), P(C.c_uint8), P(f32), P(f32)]),
+    ("gpl_rudy_tiles", i32, [C.c_void_p, P(RudyGrid), P(RouteVars), P(f32), P(C.c_uint8), P(RudyStats)]),
     ("gpl_debug_inject", i32, [C.c_void_p, i32, i32]),
     ("gpl_kernel_times", i32, [C.c_void_p, P(KernelTime), i32]),
     ("gpl_set_profiling", i32, [C.c_void_p, i32]),

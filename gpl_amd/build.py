@@ -44,7 +44,7 @@ def build(verbose=False):
     design_o = os.path.join(BUILD, "design.o")
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "design.cpp"), "-o", design_o], verbose)
     objs.append(design_o)
-    for src in ("kernels.hip", "nesterov.cpp", "noalloc.hip", "ip_gpu.hip"):
+    for src in ("kernels.hip", "nesterov.cpp", "noalloc.hip", "ip_gpu.hip", "route.hip"):
         o = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", o], verbose)
         objs.append(o)

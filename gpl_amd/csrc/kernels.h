@@ -241,6 +241,50 @@ __host__ __device__ inline void ctl_refresh(CtlHead& g) {
     }
 }
 
+// ---- routability's congestion measurement (route.hip) ------------------------------------------
+// The RUDY tile grid (RudyInputs / makeGrid): the die, the tiles per side, the tile size and the
+// wire width (Rudy::new).
+struct RudyGrid {
+    int32_t lx, ly, ux, uy;
+    int32_t cnt_x, cnt_y, tile, wire_width;
+};
+// RouteBaseVars' inflation fields: minCongestionForInflation, inflationRatioCoef,
+// maxInflationRatio, ignoreEdgeRatio, the top-share weights rcK.
+struct RouteVarsDev {
+    float min_congestion, coef, max_ratio, ignore_edge_ratio;
+    float rc_k[4];
+};
+// Device buffers of the measurement (grown on demand, kept for the context): the pairs and their
+// sort; per tile the demand and blockage (x-major), the actionable flags, inflation ratios and
+// congestions (y-major), the congestions sorted; the statistics (total overflow, overflowed,
+// actionable, the four top averages, the weighted congestion).
+struct RudyBufs {
+    unsigned long long* count = nullptr;
+    unsigned long long* offset = nullptr;
+    unsigned long long* keys = nullptr;
+    unsigned long long* keys2 = nullptr;
+    float* vals = nullptr;
+    float* vals2 = nullptr;
+    unsigned char* temp = nullptr;  // the library sorts' and scans' scratch
+    float* demand = nullptr;
+    float* blockage = nullptr;
+    uint8_t* act = nullptr;
+    float* ratio = nullptr;
+    float* cong = nullptr;
+    float* sorted = nullptr;
+    double* stats = nullptr;
+    size_t cap_nets = 0, cap_off = 0, cap_pairs = 0, cap_pairs2 = 0, cap_pairs3 = 0, cap_pairs4 = 0, cap_temp = 0;
+    size_t cap_tiles = 0, cap_tiles2 = 0, cap_act = 0, cap_ratio = 0, cap_cong = 0, cap_sorted = 0, cap_stats = 0;
+    void release();
+};
+// RUDY demand and blockage of the grid from the nets' boxes in n.rec (a WA forward with the net
+// records written), (capacity, reduction) pairs per tile (x-major, device; null: none); the nets'
+// problem indices in net_orig order the float sums.  Waits for the stream (the pair count).
+hipError_t launch_rudy(hipStream_t s, const NetsDev& n, const int32_t* net_orig, const RudyGrid& g, const uint8_t* cap_red_dev, RudyBufs& b);
+// The tiles of the last launch_rudy: actionable (standard instance cells' density centres among
+// the ncells device cells), inflation ratios, the averages into b.stats.
+hipError_t launch_rudy_tiles(hipStream_t s, const CellsDev& c, int ncells, const RudyGrid& g, const RouteVarsDev& v, RudyBufs& b);
+
 constexpr int kPartials = 2048;  // per-block partial slots per quantity
 constexpr int kGradStage = 2048;  // pin terms of a gradient block's cells staged in LDS, at most
 constexpr int kPlaceChunk = 512;  // cells per block iteration of the scatter family

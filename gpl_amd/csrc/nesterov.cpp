@@ -117,6 +117,9 @@ struct gpl_ctx {
     // back (ctl_depth graphs in flight) and the host steps in only when the device halts on an
     // event it handles.  h_ctl is the record's pinned staging copy.
     bool device_trials = false, ctl_failed = false;
+    gplk::RudyBufs rudy{};  // routability's congestion measurement (grown on demand)
+    uint8_t* d_cap_red = nullptr;
+    size_t cap_red_cap = 0;
     gplk::CtlGlobal* d_ctl = nullptr;
     gplk::CtlGlobal* h_ctl = nullptr;
     hipGraph_t cgraph = nullptr;
@@ -249,6 +252,8 @@ struct gpl_ctx {
         if (h_sums) hipHostFree(h_sums);
         if (h_seq) hipHostFree(h_seq);
         if (h_ctl) hipHostFree(h_ctl);
+        rudy.release();
+        if (d_cap_red) hipFree(d_cap_red);
         if (cexec) hipGraphExecDestroy(cexec);
         if (cgraph) hipGraphDestroy(cgraph);
         if (profiling)
@@ -2696,6 +2701,78 @@ int32_t gpl_objective_backward(gpl_ctx* c, const float* xy, float wl_coef, float
         Region& r = c->rg[0];
         gplk::launch_objective_gradient(c->stream, r.dev, c->cells, c->nets, c->obj_xy, wl_coef, lambda, r.partials, c->obj_grad);
         c->download_region_vec(0, c->obj_grad, grad);
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+// ---- routability's congestion measurement ------------------------------------------------------
+int32_t gpl_rudy_wire_width(const int32_t* pitches, int32_t n) {
+    double terms = 0.0;
+    for (int32_t i = 0; i < n; ++i) terms += 1.0 / double(pitches[i]);
+    return terms != 0.0 ? (int32_t)((1.0 / terms) * 1.25) : 100;
+}
+
+static bool bad_grid(const gpl_rudy_grid* g) {
+    return !g || g->cnt_x <= 0 || g->cnt_y <= 0 || g->tile_size <= 0 || (int64_t)g->cnt_x * g->cnt_y > ((int64_t)1 << 26);
+}
+
+static gplk::RudyGrid rudy_grid(const gpl_rudy_grid* g) {
+    return gplk::RudyGrid{g->lx, g->ly, g->ux, g->uy, g->cnt_x, g->cnt_y, g->tile_size, g->wire_width};
+}
+
+int32_t gpl_rudy(gpl_ctx* c, const gpl_rudy_grid* g, const uint8_t* cap_red, float* demand, float* blockage) {
+    if (!c || bad_grid(g)) return GPL_ERR_ARG;
+    try {
+        const size_t tiles = size_t(g->cnt_x) * size_t(g->cnt_y);
+        c->join();
+        // the nets' boxes at the cells' current centres, into the net records
+        c->wa_forward(true);
+        c->join();
+        const uint8_t* cr = nullptr;
+        if (cap_red) {
+            if (c->cap_red_cap < 2 * tiles) {
+                if (c->d_cap_red) check(hipFree(c->d_cap_red), "free");
+                c->d_cap_red = nullptr;
+                check(hipMalloc(&c->d_cap_red, 2 * tiles), "hipMalloc");
+                c->cap_red_cap = 2 * tiles;
+            }
+            check(hipMemcpyAsync(c->d_cap_red, cap_red, 2 * tiles, hipMemcpyHostToDevice, c->stream), "upload");
+            cr = c->d_cap_red;
+        }
+        check(gplk::launch_rudy(c->stream, c->nets, c->d_net_orig, rudy_grid(g), cr, c->rudy), "rudy");
+        if (demand) check(hipMemcpyAsync(demand, c->rudy.demand, sizeof(float) * tiles, hipMemcpyDeviceToHost, c->stream), "copy");
+        if (blockage) check(hipMemcpyAsync(blockage, c->rudy.blockage, sizeof(float) * tiles, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
+int32_t gpl_rudy_tiles(gpl_ctx* c, const gpl_rudy_grid* g, const gpl_route_vars* v, float* inflation_ratio, uint8_t* actionable,
+                       gpl_rudy_stats* stats) {
+    if (!c || bad_grid(g) || !v || !c->rudy.demand || size_t(g->cnt_x) * size_t(g->cnt_y) > c->rudy.cap_tiles) return GPL_ERR_ARG;
+    try {
+        const size_t tiles = size_t(g->cnt_x) * size_t(g->cnt_y);
+        gplk::RouteVarsDev rv{v->min_congestion_for_inflation, v->inflation_ratio_coef, v->max_inflation_ratio, v->ignore_edge_ratio,
+                              {v->rc_k[0], v->rc_k[1], v->rc_k[2], v->rc_k[3]}};
+        c->join();
+        check(gplk::launch_rudy_tiles(c->stream, c->cells, c->ncells, rudy_grid(g), rv, c->rudy), "rudy tiles");
+        double st[8];
+        check(hipMemcpyAsync(st, c->rudy.stats, sizeof st, hipMemcpyDeviceToHost, c->stream), "copy");
+        if (inflation_ratio)
+            check(hipMemcpyAsync(inflation_ratio, c->rudy.ratio, sizeof(float) * tiles, hipMemcpyDeviceToHost, c->stream), "copy");
+        if (actionable) check(hipMemcpyAsync(actionable, c->rudy.act, tiles, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+        if (stats) {
+            stats->total_overflow = st[0];
+            stats->overflowed_tiles = int32_t(st[1]);
+            stats->actionable_tiles = int32_t(st[2]);
+            for (int k = 0; k < 4; ++k) stats->top[k] = st[3 + k];
+            stats->rc = float(st[7]);
+        }
     } catch (const HipError& e) {
         return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
     }

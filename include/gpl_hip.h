@@ -393,6 +393,45 @@ int32_t gpl_objective_forward(gpl_ctx* c, const float* xy, float wl_coef, float 
 int32_t gpl_objective_backward(gpl_ctx* c, const float* xy, float wl_coef, float lambda,
                                float* grad);
 
+/* ---- routability's congestion measurement (route_base.rs) ---------------------------------- */
+/* The RUDY grid (RudyInputs, route_base.rs:48-55, and makeGrid, 158-174): the die, tiles per side,
+ * the tile size, the wire width (gpl_rudy_wire_width). */
+typedef struct gpl_rudy_grid {
+    int32_t lx, ly, ux, uy;
+    int32_t cnt_x, cnt_y, tile_size, wire_width;
+} gpl_rudy_grid;
+/* RouteBaseVars' inflation fields (route_base.rs:12-46; defaults 0.95, 2, 3, 0.8, {1, 1, 0, 0}). */
+typedef struct gpl_route_vars {
+    float min_congestion_for_inflation, inflation_ratio_coef, max_inflation_ratio, ignore_edge_ratio;
+    float rc_k[4];
+} gpl_route_vars;
+/* updateRudyAverage's figures (route_base.rs:493-536). */
+typedef struct gpl_rudy_stats {
+    double total_overflow;
+    int32_t overflowed_tiles, actionable_tiles;
+    double top[4]; /* the mean congestion of the top 0.5, 1, 2 and 5 % of the actionable tiles */
+    float rc;      /* the weighted routing congestion (RouteBase::measure's cur_rc) */
+} gpl_rudy_stats;
+/* Rudy::new (route_base.rs:143-152): 1.25 / sum(1 / pitch), truncated; 100 with no pitch. */
+int32_t gpl_rudy_wire_width(const int32_t* pitches, int32_t n);
+/* Rudy::calculate (route_base.rs:183-253) on the device: every net's terminal box at the cells'
+ * current centres deposits hpwl * wire_width / area, scaled by its share of each tile it strictly
+ * overlaps, times 100, summed per tile in float in the nets' order (the reference's floats, bit for
+ * bit); the blockage of each tile from (capacity, reduction) pairs (cap_red, 2 bytes per tile,
+ * x-major; NULL: none).  demand and blockage (each cnt_x * cnt_y floats, x-major: x * cnt_y + y)
+ * may be NULL; they stay on the device for gpl_rudy_tiles.  Overwrites the net records and the
+ * pins' WA terms (the loop recomputes them every trial). */
+int32_t gpl_rudy(gpl_ctx* c, const gpl_rudy_grid* g, const uint8_t* cap_red, float* demand, float* blockage);
+/* calculateRudyTiles' tile half and updateRudyAverage (route_base.rs:443-536) on the last
+ * gpl_rudy's demand: the tiles (y-major, iy * cnt_x + ix, origin (0, 0)) holding a standard
+ * instance cell's density centre are actionable unless their blockage reaches
+ * ignore_edge_ratio; an actionable tile's inflation ratio is min((congestion / min)^coef,
+ * max) at congestion >= min, else 1; the statistics as the reference computes them (the top
+ * shares over the congestions sorted in descending order).  inflation_ratio and actionable
+ * (cnt_x * cnt_y each) may be NULL. */
+int32_t gpl_rudy_tiles(gpl_ctx* c, const gpl_rudy_grid* g, const gpl_route_vars* v, float* inflation_ratio, uint8_t* actionable,
+                       gpl_rudy_stats* stats);
+
 /* The density map of gpl_density_scatter on arrays the caller keeps in device memory, with no
  * allocation (no reference counterpart: it serves callers that cannot call hipMalloc -- code under
  * HIP graph capture, tight serving loops, harnesses that forbid it).  The scatter of the density

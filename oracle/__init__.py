@@ -85,6 +85,12 @@ def lib():
             ("orc_set_inject_control", None, [C.c_void_p, C.c_int32, C.c_int32]),
             ("orc_region_rows", C.c_int32, [C.c_void_p, f32p, C.c_int32]),
             ("orc_set_field_noise", None, [C.c_void_p, C.c_float, C.c_uint64]),
+            ("orc_rudy_wire_width", C.c_int32, [i32p, C.c_int32]),
+            ("orc_rudy_rect", None, [i32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, i32p]),
+            ("orc_rudy", None, [i32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, i32p, u8p, f32p, f32p]),
+            ("orc_rudy_congestion", C.c_float, [C.c_float, C.c_float]),
+            ("orc_top_averages", None, [C.c_int32, f64p, f64p]),
+            ("orc_rudy_tiles", None, [C.c_int32, C.c_int32, C.c_int32, f32p, f32p, C.c_int32, i32p, f32p, f32p, u8p, f64p]),
         ]
         for name, res, args in sig:
             fn = getattr(L, name)
@@ -371,3 +377,53 @@ def momentum(prev_a):
     c = C.c_float(0.0)
     a = lib().orc_momentum(np.float32(prev_a), C.byref(c))
     return a, c.value
+
+
+# ---- routability's congestion measurement (route.c) --------------------------------------------
+def rudy_wire_width(pitches):
+    p = np.ascontiguousarray(pitches, np.int32)
+    return int(lib().orc_rudy_wire_width(i32(p), len(p)))
+
+
+def rudy(die, cnt_x, cnt_y, tile, wire_width, boxes, cap_red=None):
+    """Rudy::calculate: (demand, blockage), x-major (index x * cnt_y + y), from net boxes (n x 4,
+    xl yl xh yh, in the nets' order) and optional (capacity, reduction) pairs per tile."""
+    n = cnt_x * cnt_y
+    demand, blockage = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    b = np.ascontiguousarray(boxes, np.int32).reshape(-1)
+    cr = None if cap_red is None else np.ascontiguousarray(cap_red, np.uint8).reshape(-1)
+    lib().orc_rudy(i32(np.ascontiguousarray(die, np.int32)), cnt_x, cnt_y, tile, wire_width, len(b) // 4, i32(b),
+                   None if cr is None else cr.ctypes.data_as(C.POINTER(C.c_uint8)), f32(demand), f32(blockage))
+    return demand, blockage
+
+
+def rudy_rect(die, cnt_x, cnt_y, tile, x, y):
+    r = np.zeros(4, np.int32)
+    lib().orc_rudy_rect(i32(np.ascontiguousarray(die, np.int32)), cnt_x, cnt_y, tile, x, y, i32(r))
+    return tuple(int(v) for v in r)
+
+
+def rudy_congestion(demand, blockage):
+    return float(lib().orc_rudy_congestion(C.c_float(demand), C.c_float(blockage)))
+
+
+def top_averages(desc_list):
+    a = np.ascontiguousarray(desc_list, np.float64)
+    out = np.zeros(4, np.float64)
+    lib().orc_top_averages(len(a), f64(a), f64(out))
+    return out
+
+
+def rudy_tiles(cnt_x, cnt_y, tile, demand, blockage, centres, min_cong=0.95, coef=2.0, max_ratio=3.0, ignore_edge=0.8,
+               rc_k=(1.0, 1.0, 0.0, 0.0)):
+    """calculateRudyTiles' tiles and updateRudyAverage: (inflation ratio, actionable) per tile
+    (y-major), and dict(total_overflow, overflowed, actionable, top (4), rc)."""
+    n = cnt_x * cnt_y
+    ratio, act = np.zeros(n, np.float32), np.zeros(n, np.uint8)
+    c = np.ascontiguousarray(centres, np.int32).reshape(-1)
+    v = np.array([min_cong, coef, max_ratio, ignore_edge, *rc_k], np.float32)
+    st = np.zeros(8, np.float64)
+    lib().orc_rudy_tiles(cnt_x, cnt_y, tile, f32(np.ascontiguousarray(demand, np.float32)), f32(np.ascontiguousarray(blockage, np.float32)),
+                         len(c) // 2, i32(c), f32(v), f32(ratio), act.ctypes.data_as(C.POINTER(C.c_uint8)), f64(st))
+    return ratio, act, dict(total_overflow=st[0], overflowed=int(st[1]), actionable=int(st[2]), top=st[3:7].copy(), rc=float(st[7]))
+
