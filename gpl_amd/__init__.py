@@ -390,6 +390,29 @@ class NesterovGPU:
         return ratio, act, dict(total_overflow=st.total_overflow, overflowed=st.overflowed_tiles, actionable=st.actionable_tiles,
                                 top=np.array(st.top[:], np.float64), rc=float(st.rc))
 
+    def route_inflate(self, region, grid, inflated_ratio, sizes):
+        """gpl_route_inflate: (new sizes (n x 2), area added)."""
+        g = _ffi.RudyGrid(**grid)
+        r = np.ascontiguousarray(inflated_ratio, np.float32)
+        s = np.ascontiguousarray(sizes, np.int32).copy()
+        d = C.c_int64(0)
+        _ok(_ffi.lib().gpl_route_inflate(self._h, region, C.byref(g), r.ctypes.data_as(C.POINTER(C.c_float)),
+                                         s.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(d)), "gpl_route_inflate", self._h)
+        return s.reshape(-1, 2), d.value
+
+    def update_density_size(self, region, sizes):
+        s = np.ascontiguousarray(sizes, np.int32)
+        _ok(_ffi.lib().gpl_update_density_size(self._h, region, s.ctypes.data_as(C.POINTER(C.c_int32))), "gpl_update_density_size",
+            self._h)
+
+    def density_boxes(self, region):
+        """(density boxes (n x 4), density scales) of the region, the problem's order."""
+        n = self._n(region)
+        db, sc = np.zeros(4 * max(n, 1), np.int32), np.zeros(max(n, 1), np.float32)
+        _ok(_ffi.lib().gpl_get_density_boxes(self._h, region, db.ctypes.data_as(C.POINTER(C.c_int32)),
+                                             sc.ctypes.data_as(C.POINTER(C.c_float))), "gpl_get_density_boxes", self._h)
+        return db[:4 * n].reshape(-1, 4), sc[:n]
+
     # ---- kernel-level entries ----
     def _n(self, region):
         R = self.regions[region]

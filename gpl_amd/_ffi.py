@@ -166,6 +166,9 @@ SIGNATURES = [
     ("gpl_rudy_wire_width", i32, [P(i32), i32]),
     ("gpl_rudy", i32, [C.c_void_p, P(RudyGrid), P(C.c_uint8), P(f32), P(f32)]),
     ("gpl_rudy_tiles", i32, [C.c_void_p, P(RudyGrid), P(RouteVars), P(f32), P(C.c_uint8), P(RudyStats)]),
+    ("gpl_route_inflate", i32, [C.c_void_p, i32, P(RudyGrid), P(f32), P(i32), P(C.c_int64)]),
+    ("gpl_update_density_size", i32, [C.c_void_p, i32, P(i32)]),
+    ("gpl_get_density_boxes", i32, [C.c_void_p, i32, P(i32), P(f32)]),
     ("gpl_debug_inject", i32, [C.c_void_p, i32, i32]),
     ("gpl_kernel_times", i32, [C.c_void_p, P(KernelTime), i32]),
     ("gpl_set_profiling", i32, [C.c_void_p, i32]),

@@ -285,6 +285,21 @@ hipError_t launch_rudy(hipStream_t s, const NetsDev& n, const int32_t* net_orig,
 // the ncells device cells), inflation ratios, the averages into b.stats.
 hipError_t launch_rudy_tiles(hipStream_t s, const CellsDev& c, int ncells, const RudyGrid& g, const RouteVarsDev& v, RudyBufs& b);
 
+// updateDensitySize's per-region constants (the reference's extended-precision sqrt(2) bin sizes):
+// max_x the largest integer below sqrt(2) size_x, thr_x that threshold rounded to float, size_x
+// sqrt(2) (float) size_x rounded to float; y alike.
+struct DensitySizeDev {
+    int32_t max_x, max_y;
+    float thr_x, thr_y, size_x, size_y;
+};
+// routability's cell inflation of cells [begin, begin + n) (sizes: their cell-box sizes, device
+// order, updated; *delta += the area added) at the tiles' inflated ratios (y-major)
+hipError_t launch_route_inflate(hipStream_t s, int begin, int n, const CellsDev& c, const RudyGrid& g, const float* ratio, int2* sizes,
+                                unsigned long long* delta);
+// updateDensitySize of cells [begin, begin + n) at cell-box sizes (device order): density boxes,
+// density scales, the preconditioner's areas
+hipError_t launch_density_size(hipStream_t s, int begin, int n, const CellsDev& c, const DensitySizeDev& d, const int2* sizes);
+
 constexpr int kPartials = 2048;  // per-block partial slots per quantity
 constexpr int kGradStage = 2048;  // pin terms of a gradient block's cells staged in LDS, at most
 constexpr int kPlaceChunk = 512;  // cells per block iteration of the scatter family

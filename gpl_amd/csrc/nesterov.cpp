@@ -2779,4 +2779,117 @@ int32_t gpl_rudy_tiles(gpl_ctx* c, const gpl_rudy_grid* g, const gpl_route_vars*
     return GPL_OK;
 }
 
+// The cells of a region between the problem's order (host arrays) and the device's.
+static void region_perm(gpl_ctx* c, int32_t b, std::vector<int32_t>& dev_to_prob) {
+    c->ensure_host_map();
+    const Region& r = c->rg[b];
+    dev_to_prob.resize(r.n);
+    for (int k = 0; k < r.n; ++k) dev_to_prob[k] = c->cell_old_of[r.R.cell_begin + k] - r.R.cell_begin;
+}
+
+// updateDensitySize's per-region constants in the reference's extended precision (nesterov.rs:2022-2036).
+static gplk::DensitySizeDev density_size_consts(const gpl_region& R) {
+    const long double s2 = 1.414213562373095048801L;
+    const long double tx = s2 * (long double)R.size_x, ty = s2 * (long double)R.size_y;
+    gplk::DensitySizeDev d{};
+    const long double fx = floorl(tx), fy = floorl(ty);
+    d.max_x = (int32_t)(fx == tx ? fx - 1 : fx);  // dx < tx for an integer dx
+    d.max_y = (int32_t)(fy == ty ? fy - 1 : fy);
+    d.thr_x = (float)tx;
+    d.thr_y = (float)ty;
+    d.size_x = (float)(s2 * (long double)(float)R.size_x);
+    d.size_y = (float)(s2 * (long double)(float)R.size_y);
+    return d;
+}
+
+int32_t gpl_route_inflate(gpl_ctx* c, int32_t region, const gpl_rudy_grid* g, const float* inflated_ratio, int32_t* sizes, int64_t* delta) {
+    if (bad_region(c, region) || bad_grid(g) || !inflated_ratio || !sizes) return GPL_ERR_ARG;
+    void* d_ratio = nullptr;
+    void* d_sizes = nullptr;
+    void* d_delta = nullptr;
+    int32_t rc = GPL_OK;
+    try {
+        Region& r = c->rg[region];
+        const size_t tiles = size_t(g->cnt_x) * size_t(g->cnt_y);
+        std::vector<int32_t> perm;
+        region_perm(c, region, perm);
+        std::vector<int2> sz(std::max(r.n, 1));
+        for (int k = 0; k < r.n; ++k) sz[k] = make_int2(sizes[2 * perm[k]], sizes[2 * perm[k] + 1]);
+        c->join();
+        check(hipMalloc(&d_ratio, sizeof(float) * tiles), "hipMalloc");
+        check(hipMalloc(&d_sizes, sizeof(int2) * sz.size()), "hipMalloc");
+        check(hipMalloc(&d_delta, sizeof(unsigned long long)), "hipMalloc");
+        check(hipMemcpyAsync(d_ratio, inflated_ratio, sizeof(float) * tiles, hipMemcpyHostToDevice, c->stream), "upload");
+        check(hipMemcpyAsync(d_sizes, sz.data(), sizeof(int2) * sz.size(), hipMemcpyHostToDevice, c->stream), "upload");
+        gplk::launch_set_u64(c->stream, (unsigned long long*)d_delta, 0ull);
+        check(gplk::launch_route_inflate(c->stream, r.R.cell_begin, r.n, c->cells, rudy_grid(g), (const float*)d_ratio, (int2*)d_sizes,
+                                         (unsigned long long*)d_delta),
+              "inflate");
+        unsigned long long dl = 0;
+        check(hipMemcpyAsync(sz.data(), d_sizes, sizeof(int2) * sz.size(), hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipMemcpyAsync(&dl, d_delta, sizeof dl, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+        for (int k = 0; k < r.n; ++k) {
+            sizes[2 * perm[k]] = sz[k].x;
+            sizes[2 * perm[k] + 1] = sz[k].y;
+        }
+        if (delta) *delta = (int64_t)dl;
+    } catch (const HipError& e) {
+        rc = fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    for (void* p : {d_ratio, d_sizes, d_delta})
+        if (p) hipFree(p);
+    return rc;
+}
+
+int32_t gpl_update_density_size(gpl_ctx* c, int32_t region, const int32_t* sizes) {
+    if (bad_region(c, region) || !sizes) return GPL_ERR_ARG;
+    void* d_sizes = nullptr;
+    int32_t rc = GPL_OK;
+    try {
+        Region& r = c->rg[region];
+        std::vector<int32_t> perm;
+        region_perm(c, region, perm);
+        std::vector<int2> sz(std::max(r.n, 1));
+        for (int k = 0; k < r.n; ++k) sz[k] = make_int2(sizes[2 * perm[k]], sizes[2 * perm[k] + 1]);
+        c->join();
+        check(hipMalloc(&d_sizes, sizeof(int2) * sz.size()), "hipMalloc");
+        check(hipMemcpyAsync(d_sizes, sz.data(), sizeof(int2) * sz.size(), hipMemcpyHostToDevice, c->stream), "upload");
+        check(gplk::launch_density_size(c->stream, r.R.cell_begin, r.n, c->cells, density_size_consts(r.R), (const int2*)d_sizes), "density size");
+        check(hipStreamSynchronize(c->stream), "sync");
+    } catch (const HipError& e) {
+        rc = fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    if (d_sizes) hipFree(d_sizes);
+    return rc;
+}
+
+int32_t gpl_get_density_boxes(gpl_ctx* c, int32_t region, int32_t* dbox, float* density_scale) {
+    if (bad_region(c, region)) return GPL_ERR_ARG;
+    try {
+        Region& r = c->rg[region];
+        std::vector<int32_t> perm;
+        region_perm(c, region, perm);
+        std::vector<int4> db(std::max(r.n, 1));
+        std::vector<float> sc(std::max(r.n, 1));
+        c->join();
+        check(hipMemcpyAsync(db.data(), c->cells.dbox + r.R.cell_begin, sizeof(int4) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipMemcpyAsync(sc.data(), c->cells.dscale + r.R.cell_begin, sizeof(float) * r.n, hipMemcpyDeviceToHost, c->stream), "copy");
+        check(hipStreamSynchronize(c->stream), "sync");
+        for (int k = 0; k < r.n; ++k) {
+            if (dbox) {
+                int32_t* o = dbox + 4 * size_t(perm[k]);
+                o[0] = db[k].x;
+                o[1] = db[k].y;
+                o[2] = db[k].z;
+                o[3] = db[k].w;
+            }
+            if (density_scale) density_scale[perm[k]] = sc[k];
+        }
+    } catch (const HipError& e) {
+        return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    return GPL_OK;
+}
+
 }  // extern "C"

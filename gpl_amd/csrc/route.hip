@@ -213,6 +213,59 @@ __global__ void k_rudy_stats(unsigned n, const float* cong, const float* sorted,
     out[7] = rc;
 }
 
+// The per-cell half of a routability pass (nesterov.rs:3945-3960 with set_size, 3754-3760): a
+// standard instance cell whose density centre lies in a tile of inflated ratio r > 1 grows to
+// round(d sqrt r) per side, rebuilt from halves around its centre.
+__global__ void k_route_inflate(int begin, int n, CellsDev c, RudyGrid g, const float* ratio, int2* sizes, unsigned long long* delta) {
+    long long add = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int cell = begin + k;
+        const uint8_t f = c.flags[cell];
+        if (!(f & 1u) || (f & 2u)) continue;  // GPL_CELL_INSTANCE, not GPL_CELL_MACRO
+        const int4 d = c.dbox[cell];
+        const int cx = (d.x + d.z) / 2, cy = (d.y + d.w) / 2;
+        const int ix = min(cx / g.tile, g.cnt_x - 1), iy = min(cy / g.tile, g.cnt_y - 1);
+        if (ix < 0 || iy < 0) continue;
+        const float r = ratio[(size_t)iy * g.cnt_x + ix];
+        if (!((double)r > 1.0)) continue;
+        const int2 s = sizes[k];
+        const float root = sqrtf(r);
+        const int nx = (int)roundf((float)s.x * root), ny = (int)roundf((float)s.y * root);
+        const int2 t = make_int2(2 * (nx / 2), 2 * (ny / 2));
+        add += (long long)t.x * t.y - (long long)s.x * s.y;
+        sizes[k] = t;
+    }
+    for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o, 64);
+    if ((threadIdx.x & 63) == 0 && add) atomicAdd(delta, (unsigned long long)add);
+}
+
+// updateDensitySize (nesterov.rs:2022-2036): a side narrower than sqrt(2) bins (max_x: the largest
+// integer below sqrt(2) size_x, the reference's extended-precision threshold) spreads to
+// sqrt(2) bins (size_x, rounded as the reference rounds it) with the scale d / thr; the density box
+// rebuilt from halves around its centre; the preconditioner's area the cell's.
+__global__ void k_density_size(int begin, int n, CellsDev c, DensitySizeDev ds, const int2* sizes) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int cell = begin + k;
+        const int2 s = sizes[k];
+        float sx = 1.0f, sy = 1.0f, dsx = (float)s.x, dsy = (float)s.y;
+        if (s.x <= ds.max_x) {
+            sx = (float)s.x / ds.thr_x;
+            dsx = ds.size_x;
+        }
+        if (s.y <= ds.max_y) {
+            sy = (float)s.y / ds.thr_y;
+            dsy = ds.size_y;
+        }
+        const int ddx = (int)dsx, ddy = (int)dsy;
+        const int4 d = c.dbox[cell];
+        const int cx = (d.x + d.z) / 2, cy = (d.y + d.w) / 2;
+        c.dbox[cell] = make_int4(cx - ddx / 2, cy - ddy / 2, cx + ddx / 2, cy + ddy / 2);
+        c.dscale[cell] = sx * sy;
+        const float2 p = c.pre[cell];
+        c.pre[cell] = make_float2(p.x, (float)s.x * (float)s.y);
+    }
+}
+
 int grid_of(unsigned long long n) { return (int)std::max<unsigned long long>(1, std::min<unsigned long long>((n + kRB - 1) / kRB, 8192)); }
 
 template <typename T>
@@ -313,6 +366,17 @@ hipError_t launch_rudy_tiles(hipStream_t s, const CellsDev& c, int ncells, const
     e = hipcub::DeviceRadixSort::SortKeysDescending(b.temp, tb, b.cong, b.sorted, (int)tiles, 0, 32, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_rudy_stats, dim3(1), dim3(64), 0, s, (unsigned)tiles, b.cong, b.sorted, v, b.stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_route_inflate(hipStream_t s, int begin, int n, const CellsDev& c, const RudyGrid& g, const float* ratio, int2* sizes,
+                                unsigned long long* delta) {
+    if (n > 0) hipLaunchKernelGGL(k_route_inflate, dim3(grid_of(n)), dim3(kRB), 0, s, begin, n, c, g, ratio, sizes, delta);
+    return hipGetLastError();
+}
+
+hipError_t launch_density_size(hipStream_t s, int begin, int n, const CellsDev& c, const DensitySizeDev& d, const int2* sizes) {
+    if (n > 0) hipLaunchKernelGGL(k_density_size, dim3(grid_of(n)), dim3(kRB), 0, s, begin, n, c, d, sizes);
     return hipGetLastError();
 }
 

@@ -432,6 +432,24 @@ int32_t gpl_rudy(gpl_ctx* c, const gpl_rudy_grid* g, const uint8_t* cap_red, flo
 int32_t gpl_rudy_tiles(gpl_ctx* c, const gpl_rudy_grid* g, const gpl_route_vars* v, float* inflation_ratio, uint8_t* actionable,
                        gpl_rudy_stats* stats);
 
+/* A routability pass's cell inflation (RouteBase::routability's per-cell loop, nesterov.rs:3945-3960,
+ * with set_size, 3754-3760): each standard instance cell of the region whose density centre lies in
+ * a tile (y-major, origin (0, 0), the grid's tile size) of inflated ratio r > 1 grows to
+ * round(dx sqrt r) x round(dy sqrt r), rebuilt from halves around its centre (an odd side loses
+ * one).  sizes: the region's cell-box sizes (dx, dy per cell, the problem's order), updated in
+ * place; *delta: the area added. */
+int32_t gpl_route_inflate(gpl_ctx* c, int32_t region, const gpl_rudy_grid* g, const float* inflated_ratio, int32_t* sizes,
+                          int64_t* delta);
+/* updateDensitySize (nesterov.rs:2022-2036) of the region's cells at the given cell-box sizes (dx,
+ * dy per cell, the problem's order): a side narrower than sqrt(2) bins (the reference's
+ * extended-precision comparison) spreads to sqrt(2) bins with its density scaled down to match;
+ * the density box rebuilt from halves around its centre; the preconditioner's cell area.  The
+ * device cell model is updated in place. */
+int32_t gpl_update_density_size(gpl_ctx* c, int32_t region, const int32_t* sizes);
+/* The region's density boxes (4 per cell) and density scales, the problem's order (either may be
+ * NULL). */
+int32_t gpl_get_density_boxes(gpl_ctx* c, int32_t region, int32_t* dbox, float* density_scale);
+
 /* The density map of gpl_density_scatter on arrays the caller keeps in device memory, with no
  * allocation (no reference counterpart: it serves callers that cannot call hipMalloc -- code under
  * HIP graph capture, tight serving loops, harnesses that forbid it).  The scatter of the density

@@ -91,6 +91,9 @@ def lib():
             ("orc_rudy_congestion", C.c_float, [C.c_float, C.c_float]),
             ("orc_top_averages", None, [C.c_int32, f64p, f64p]),
             ("orc_rudy_tiles", None, [C.c_int32, C.c_int32, C.c_int32, f32p, f32p, C.c_int32, i32p, f32p, f32p, u8p, f64p]),
+            ("orc_set_size", None, [i32p, C.c_int32, C.c_int32, i32p]),
+            ("orc_route_inflate", C.c_int64, [C.c_int32, i32p, i32p, u8p, C.c_int32, C.c_int32, C.c_int32, f32p]),
+            ("orc_density_size", None, [C.c_int32, i32p, C.c_double, C.c_double, i32p, f32p]),
         ]
         for name, res, args in sig:
             fn = getattr(L, name)
@@ -426,4 +429,29 @@ def rudy_tiles(cnt_x, cnt_y, tile, demand, blockage, centres, min_cong=0.95, coe
     lib().orc_rudy_tiles(cnt_x, cnt_y, tile, f32(np.ascontiguousarray(demand, np.float32)), f32(np.ascontiguousarray(blockage, np.float32)),
                          len(c) // 2, i32(c), f32(v), f32(ratio), act.ctypes.data_as(C.POINTER(C.c_uint8)), f64(st))
     return ratio, act, dict(total_overflow=st[0], overflowed=int(st[1]), actionable=int(st[2]), top=st[3:7].copy(), rc=float(st[7]))
+
+
+def set_size(box, dx, dy):
+    out = np.zeros(4, np.int32)
+    lib().orc_set_size(i32(np.ascontiguousarray(box, np.int32)), dx, dy, i32(out))
+    return tuple(int(v) for v in out)
+
+
+def route_inflate(boxes, centres, std_cell, cnt_x, cnt_y, tile, ratio):
+    """The per-cell inflation: (new boxes (n x 4), area added)."""
+    b = np.ascontiguousarray(boxes, np.int32).reshape(-1).copy()
+    c = np.ascontiguousarray(centres, np.int32).reshape(-1)
+    s = np.ascontiguousarray(std_cell, np.uint8)
+    d = lib().orc_route_inflate(len(s), i32(b), i32(c), s.ctypes.data_as(C.POINTER(C.c_uint8)), cnt_x, cnt_y, tile,
+                                f32(np.ascontiguousarray(ratio, np.float32)))
+    return b.reshape(-1, 4), int(d)
+
+
+def density_size(sizes, bin_x, bin_y):
+    """updateDensitySize: (density sizes (n x 2), scales)."""
+    s = np.ascontiguousarray(sizes, np.int32).reshape(-1)
+    n = len(s) // 2
+    ds, sc = np.zeros(2 * n, np.int32), np.zeros(n, np.float32)
+    lib().orc_density_size(n, i32(s), bin_x, bin_y, i32(ds), f32(sc))
+    return ds.reshape(-1, 2), sc
 

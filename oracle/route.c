@@ -9,11 +9,14 @@
  * visitNetTiles, 191-253), the tile congestion (getTileCongestion, 256-264), the tile grid's
  * actionable tiles and inflation ratios (calculateRudyTiles, 443-488) and the top 0.5/1/2/5 %
  * averages with the weighted routing congestion (updateRudyAverage / top_averages, 493-536,
- * 661-675).  Single thread, the reference's order and float/double types throughout.
+ * 661-675); and the per-cell half of a routability pass (nesterov.rs:3945-3960: the cells in
+ * inflated tiles grown by the root of the ratio, set_size 3754-3760) with updateDensitySize
+ * (2022-2036).  Single thread, the reference's order and float/double types throughout.
  *
  * Pinned by tests/test_oracle_route.py: the reference's own unit-test vectors (route_base.rs
  * 682-757: the wire width, the stretched last tile, strict overlap, blockage and congestion, the
- * top shares) and a brute-force per-tile restatement on random nets.
+ * top shares; nesterov.rs:4841-4845: set_size), the total-demand identity on random nets, and
+ * hand-computed inflation, threshold, cap and density-size cases.
  */
 #include <math.h>
 #include <stdint.h>
@@ -171,4 +174,65 @@ void orc_rudy_tiles(int32_t cnt_x, int32_t cnt_y, int32_t tile, const float* dem
     for (int s = 0; s < 4; ++s) stats[3 + s] = top[s];
     stats[7] = rc;
     free(cong);
+}
+
+/* set_size (nesterov.rs:3754-3760): the box rebuilt from halves around its centre. */
+void orc_set_size(const int32_t* box, int32_t dx, int32_t dy, int32_t* out) {
+    const int32_t cx = (box[0] + box[2]) / 2, cy = (box[1] + box[3]) / 2;
+    out[0] = cx - dx / 2;
+    out[1] = cy - dy / 2;
+    out[2] = cx + dx / 2;
+    out[3] = cy + dy / 2;
+}
+
+/* RouteBase::routability's per-cell loop (nesterov.rs:3945-3960): each standard cell (std[i]) whose
+ * density centre (centres, 2 per cell) lies in a tile (y-major, origin (0, 0), tile size `tile`,
+ * the last column and row clamping) of inflated ratio r > 1 takes round(d * sqrt r) per side,
+ * set_size around its own box's centre (boxes, 4 per cell, updated); returns the area added. */
+int64_t orc_route_inflate(int32_t n, int32_t* boxes, const int32_t* centres, const uint8_t* std_cell, int32_t cnt_x, int32_t cnt_y, int32_t tile,
+                          const float* ratio) {
+    int64_t delta = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!std_cell[i]) continue;
+        int32_t ix = centres[2 * i] / tile, iy = centres[2 * i + 1] / tile;
+        if (ix > cnt_x - 1) ix = cnt_x - 1;
+        if (iy > cnt_y - 1) iy = cnt_y - 1;
+        if (ix < 0 || iy < 0) continue;
+        const float r = ratio[(size_t)iy * cnt_x + ix];
+        if ((double)r <= 1.0) continue;
+        int32_t* b = boxes + 4 * (size_t)i;
+        const int32_t dx = b[2] - b[0], dy = b[3] - b[1];
+        const int64_t prev = (int64_t)dx * dy;
+        const float root = sqrtf(r);
+        const int32_t nx = (int32_t)roundf((float)dx * root), ny = (int32_t)roundf((float)dy * root);
+        int32_t nb[4];
+        orc_set_size(b, nx, ny, nb);
+        memcpy(b, nb, sizeof nb);
+        delta += (int64_t)(nb[2] - nb[0]) * (nb[3] - nb[1]) - prev;
+    }
+    return delta;
+}
+
+/* updateDensitySize (nesterov.rs:2022-2036) in the reference's extended precision: a side below
+ * sqrt(2) bins spreads to sqrt(2) bins (the size rounded from the float bin size) with the scale
+ * d / thr; out: 2 per cell the density size, scale per cell. */
+void orc_density_size(int32_t n, const int32_t* sizes, double bin_x, double bin_y, int32_t* dsize, float* scale) {
+    const long double s2 = 1.414213562373095048801L;
+    const long double thr_x = s2 * (long double)bin_x, thr_y = s2 * (long double)bin_y;
+    const float size_x = (float)(s2 * (long double)(float)bin_x), size_y = (float)(s2 * (long double)(float)bin_y);
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t dx = sizes[2 * i], dy = sizes[2 * i + 1];
+        float sx = 1.0f, sy = 1.0f, dsx = (float)dx, dsy = (float)dy;
+        if ((long double)dx < thr_x) {
+            sx = (float)dx / (float)thr_x;
+            dsx = size_x;
+        }
+        if ((long double)dy < thr_y) {
+            sy = (float)dy / (float)thr_y;
+            dsy = size_y;
+        }
+        dsize[2 * i] = (int32_t)dsx;
+        dsize[2 * i + 1] = (int32_t)dsy;
+        scale[i] = sx * sy;
+    }
 }

@@ -120,3 +120,44 @@ def test_rudy_at_200k_cells():
     finally:
         d.close()
     check(gpu, orc)
+
+
+def test_inflation_and_density_size_match_the_oracle(c1):
+    """The tiles' inflated ratios (max(r, 1)) of a measured RUDY grid: the standard cells in
+    inflated tiles grown on the GPU as the oracle grows them (sizes and area added, exact), then the
+    density boxes and scales of every cell at the new sizes (updateDensitySize) as the oracle
+    computes them (a box of density size d spans 2 (d / 2), rebuilt from halves)."""
+    cnt = 32
+    a = c1.arrays()
+    R = c1.problem().regions[0]
+    xy = spread_points(c1, 150)
+    grid, die = grid_for(c1, cnt)
+    box = a["cell_box"]
+    sizes = np.stack([box[:, 2] - box[:, 0], box[:, 3] - box[:, 1]], 1).astype(np.int32)
+    g = G.NesterovGPU(c1)
+    try:
+        g.set_coordinates(0, G.GPL_SLOT_CUR_SLP, xy)
+        g.density_scatter(0, G.GPL_SLOT_CUR_SLP)
+        g.rudy(grid)
+        ratio, _, _ = g.rudy_tiles(grid, min_cong=0.3)  # a low threshold: many inflated tiles
+        infl = np.maximum(ratio, np.float32(1.0))
+        assert (infl > 1.0).sum() > 10
+        new, delta = g.route_inflate(0, grid, infl, sizes)
+        db, _ = g.density_boxes(0)
+        g.update_density_size(0, new)
+        db2, sc2 = g.density_boxes(0)
+    finally:
+        g.close()
+    flags = a["cell_flags"]
+    std = ((flags & 1) != 0) & ((flags & 2) == 0)
+    cx, cy = (db[:, 0] + db[:, 2]) // 2, (db[:, 1] + db[:, 3]) // 2
+    boxes = np.stack([cx - sizes[:, 0] // 2, cy - sizes[:, 1] // 2, cx - sizes[:, 0] // 2 + sizes[:, 0],
+                      cy - sizes[:, 1] // 2 + sizes[:, 1]], 1)
+    obox, odelta = O.route_inflate(boxes, np.stack([cx, cy], 1), std.astype(np.uint8), cnt, cnt, grid["tile_size"], infl)
+    osz = np.stack([obox[:, 2] - obox[:, 0], obox[:, 3] - obox[:, 1]], 1)
+    assert np.array_equal(new, osz) and delta == odelta and delta > 0
+    ds, sc = O.density_size(new, R.size_x, R.size_y)
+    assert np.array_equal(np.stack([db2[:, 2] - db2[:, 0], db2[:, 3] - db2[:, 1]], 1), 2 * (ds // 2))
+    assert np.array_equal(sc2, sc)
+    # the centres stay
+    assert np.array_equal((db2[:, 0] + db2[:, 2]) // 2, cx) and np.array_equal((db2[:, 1] + db2[:, 3]) // 2, cy)

@@ -86,3 +86,49 @@ def test_tile_inflation_threshold_power_and_cap():
     # a tile without a centre is not actionable whatever its congestion
     _, act2, st2 = O.rudy_tiles(cnt, cnt, tile, demand, blockage, centres[:1])
     assert act2.tolist() == [1, 0, 0, 0] and st2["actionable"] == 1
+
+
+def test_set_size_rebuilds_from_halves():
+    """nesterov.rs:4841-4845: box (0, 0, 10, 4) set to 7 x 5 becomes (2, 0, 8, 4)."""
+    assert O.set_size((0, 0, 10, 4), 7, 5) == (2, 0, 8, 4)
+
+
+def test_inflation_grows_by_the_root_of_the_ratio():
+    """A standard cell of 10 x 4 in a tile of ratio 2.25 grows by 1.5 per side: 15 x 6, rebuilt
+    from halves: 14 x 6 (area 84, 44 added); a cell in a tile of ratio 1 and a macro stay."""
+    boxes = [(0, 0, 10, 4), (20, 0, 30, 4), (0, 20, 10, 24)]
+    centres = [(5, 2), (25, 2), (5, 22)]
+    std = [1, 1, 0]
+    ratio = np.array([2.25, 1.0, 2.25, 2.25], np.float32)  # 2 x 2 tiles of 16, y-major
+    out, d = O.route_inflate(boxes, centres, std, 2, 2, 16, ratio)
+    assert out.tolist() == [[-2, -1, 12, 5], [20, 0, 30, 4], [0, 20, 10, 24]]
+    assert d == 14 * 6 - 40
+
+
+def test_density_size_threshold_and_scale():
+    """Bins of 10: the threshold is sqrt(2) * 10 = 14.142...; a side of 14 spreads to
+    int(float(sqrt 2 * 10)) = 14 with the scale 14 / 14.142..., a side of 15 keeps its size at scale 1."""
+    ds, sc = O.density_size([(14, 15), (15, 14), (3, 40)], 10.0, 10.0)
+    thr = np.float32(np.sqrt(2.0) * 10.0)
+    assert ds.tolist() == [[14, 15], [15, 14], [14, 40]]
+    assert sc[0] == np.float32(np.float32(14) / thr) and sc[1] == sc[0]
+    assert sc[2] == np.float32(np.float32(3) / thr)
+
+
+def test_density_size_is_what_init_built():
+    """The 2k-cell design's density boxes and scales, as the host's init left them, are
+    updateDensitySize's at the cells' own sizes (each box 2 (d / 2) wide, rebuilt from halves)."""
+    import gpl_amd as G
+    d = G.Design.from_config("c1")
+    d.init_nesterov_place()
+    try:
+        a = d.arrays()
+        R = d.problem().regions[0]
+        box, db = a["cell_box"], a["cell_dbox"]
+        sizes = np.stack([box[:, 2] - box[:, 0], box[:, 3] - box[:, 1]], 1)
+        ds, sc = O.density_size(sizes, R.size_x, R.size_y)
+        assert np.array_equal(np.stack([db[:, 2] - db[:, 0], db[:, 3] - db[:, 1]], 1), 2 * (ds // 2))
+        assert np.array_equal(sc, a["density_scale"])
+        assert (sc < 1.0).any()  # some cells spread
+    finally:
+        d.close()
