@@ -2248,45 +2248,6 @@ void launch_zero_bins(hipStream_t s, const RegionDev& r) {
     hipLaunchKernelGGL(k_zero_bins, dim3(grid_for(M)), dim3(kBlock), 0, s, r.inst, r.filler, M);
 }
 
-// The scatter of the stored density boxes with every overlap added straight to the bins in global
-// memory (int64 atomics: the sums are exact in any order, so the bins are those of the tiled form):
-// one thread per cell, no LDS tables.
-__global__ void __launch_bounds__(kBlock) k_place_scatter_direct(RegionDev r, CellsDev c, PlaceArgs pa) {
-    if (r.ctl && (r.ctl->hot.flags & 1)) return;
-    __builtin_amdgcn_s_setprio(2);
-    const int n = r.cell_end - r.cell_begin;
-    unsigned adds = 0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int g = r.cell_begin + k;
-        const int4 d = c.dbox[g];
-        const uint8_t fl = c.flags[g];
-        const float sc = c.dscale[g];
-        int x0, x1, y0, y1;
-        bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
-        bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
-        const bool inst = fl & 1u, macro = inst && (fl & 2u);
-        unsigned long long* glob = inst ? r.inst : r.filler;
-        for (int y = y0; y < y1; ++y) {
-            const int ey0 = bin_edge(r.bin_y, r.edges_rounded, r.ly, r.size_y, r.cnt_y, y);
-            const int ey1 = bin_edge(r.bin_y, r.edges_rounded, r.ly, r.size_y, r.cnt_y, y + 1);
-            for (int x = x0; x < x1; ++x) {
-                const int ex0 = bin_edge(r.bin_x, r.edges_rounded, r.lx, r.size_x, r.cnt_x, x);
-                const int ex1 = bin_edge(r.bin_x, r.edges_rounded, r.lx, r.size_x, r.cnt_x, x + 1);
-                float a = overlap_area(ex0, ey0, ex1, ey1, d) * sc;
-                if (macro) a *= r.target_density;
-                const long long v = (long long)a;  // the reference's `as i64`
-                if (v == 0) continue;
-                atomicAdd(glob + ((unsigned)y * (unsigned)r.cnt_x + (unsigned)x), (unsigned long long)v);
-                ++adds;
-            }
-        }
-    }
-    if (pa.global_adds) {
-        for (int o = 32; o > 0; o >>= 1) adds += __shfl_xor(adds, o, 64);
-        if ((threadIdx.x & 63) == 0 && adds) atomicAdd(pa.global_adds, (unsigned long long)adds);
-    }
-}
-
 template <int MODE, int CH>
 static void place_scatter_ch(hipStream_t s, const RegionDev& r, const CellsDev& c, const PlaceArgs& pa, int grid) {
     if (r.narrow) hipLaunchKernelGGL((k_place_scatter<MODE, unsigned, CH>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
@@ -2296,11 +2257,6 @@ static void place_scatter_ch(hipStream_t s, const RegionDev& r, const CellsDev& 
 template <int MODE>
 static void place_scatter_mode(hipStream_t s, const RegionDev& r, const CellsDev& c, const PlaceArgs& pa) {
     const int n = r.cell_end - r.cell_begin;
-    if (MODE == PLACE_STORED && r.chunk == 1) {
-        hipLaunchKernelGGL(k_place_scatter_direct, dim3(round_xcds(std::min((n + kBlock - 1) / kBlock, 8192))), dim3(kBlock), 0, s, r,
-                           c, pa);
-        return;
-    }
     const int ch = r.chunk == 256 ? 256 : kPlaceChunk;
     const int chunks = (n + ch - 1) / ch;
     const int grid = round_xcds(chunks < 4096 ? chunks : 4096);

@@ -2534,7 +2534,7 @@ int32_t gpl_set_tuning(gpl_ctx* c, int32_t region, int32_t chunk, int32_t table_
         c->autotune = chunk != 0 ? 1 : 0;  // region -1: the tuner on (chunk 1) or off (chunk 0)
         return GPL_OK;
     }
-    if (bad_region(c, region) || (chunk != 1 && chunk != 256 && chunk != gplk::kPlaceChunk) || (table_bits != 32 && table_bits != 64) ||
+    if (bad_region(c, region) || (chunk != 256 && chunk != gplk::kPlaceChunk) || (table_bits != 32 && table_bits != 64) ||
         (stage != 0 && stage != 1))
         return GPL_ERR_ARG;
     try {
