@@ -87,7 +87,9 @@ CONFIGS = {
     "c4s": dict(gen=dict(num_cells=50_000, num_nets=52_000, degree_p=0.5, degree_cap=64, num_io=128, utilization=0.65),
                 opts=dict(bin_cnt=256, density=0.70, initial_place_max_iter=5)),
     "c5": dict(gen=dict(num_cells=500_000, num_nets=520_000, degree_p=0.5, degree_cap=64, num_io=512, utilization=0.65),
-               opts=dict(bin_cnt=768, density=0.70, initial_place_max_iter=5)),
+               opts=dict(bin_cnt=768, density=0.70, initial_place_max_iter=5, routability=1, route_gcells=1024, route_tracks=8,
+                         routability_target_rc=1.0, routability_max_inflation_ratio=2.5, routability_max_passes=5,
+                         routability_net_weight_max=2.0)),
     "c4r_small": dict(gen=dict(num_cells=1_000, num_nets=18_000, degree_p=0.5, degree_cap=64, num_io=96, utilization=0.65,
                                group_cells=[250, 500, 1_000, 1_500, 2_000, 3_000, 4_000, 5_000]),
                       opts=dict(bin_cnt=0, density=0.70, initial_place_max_iter=5)),
@@ -413,6 +415,15 @@ class NesterovGPU:
                                              sc.ctypes.data_as(C.POINTER(C.c_float))), "gpl_get_density_boxes", self._h)
         return db[:4 * n].reshape(-1, 4), sc[:n]
 
+    def route_stats(self):
+        """gpl_get_route_stats: (totals dict, one dict per routability pass)."""
+        st = _ffi.RouteStats()
+        n = _ffi.lib().gpl_get_route_stats(self._h, C.byref(st), None, 0)
+        rows = (_ffi.RoutePass * max(n, 1))()
+        _ffi.lib().gpl_get_route_stats(self._h, None, rows, n)
+        passes = [{k: getattr(r, k) for k, _ in _ffi.RoutePass._fields_} for r in rows[:n]]
+        return {k: getattr(st, k) for k, _ in _ffi.RouteStats._fields_}, passes
+
     # ---- kernel-level entries ----
     def _n(self, region):
         R = self.regions[region]
@@ -532,7 +543,7 @@ _FLAGS = [
     "-enable_routing_congestion",
 ]
 _MODELLED_FLAGS = {"-skip_initial_place", "-skip_nesterov_place", "-skip_io", "-disable_revert_if_diverge",
-                   "-disable_pin_density_adjust", "-force_center_initial_place"}
+                   "-disable_pin_density_adjust", "-force_center_initial_place", "-routability_driven"}
 
 
 def parse_key_args(args, keys=_KEYS, flags=_FLAGS):
@@ -581,8 +592,28 @@ def opts_from_args(args, base=None):
         "-initial_place_max_fanout": ("initial_place_max_fanout", int), "-random_seed": ("seed", int),
         "-perturb_dist": ("perturb_dist", float), "-pad_left": ("pad_left", int), "-pad_right": ("pad_right", int),
     }
+    route_conv = {
+        "-routability_check_overflow": "routability_end_overflow", "-routability_snapshot_overflow": "routability_snapshot_overflow",
+        "-routability_target_rc_metric": "routability_target_rc", "-routability_max_density": "routability_max_density",
+        "-routability_inflation_ratio_coef": "routability_inflation_ratio_coef",
+        "-routability_max_inflation_ratio": "routability_max_inflation_ratio",
+        "-routability_min_congestion_for_inflation": "routability_min_congestion",
+        "-routability_max_inflation_total": "routability_max_inflation_total",
+        "-routability_net_weight_max": "routability_net_weight_max",
+        "-routability_congested_nets_percentage": "routability_congested_nets_percentage",
+    }
     for key, val in keys.items():
-        if key == "-density":
+        if key in route_conv:
+            setattr(o, route_conv[key], float(val))
+        elif key == "-routability_rc_coefficients":
+            try:
+                k = [float(x) for x in val.split()]
+            except ValueError:
+                k = []
+            if len(k) != 4:
+                raise GplError(GPL_ERR_REFUSED, f"-routability_rc_coefficients {val}: not four numbers")
+            o.routability_rc_k[:] = k
+        elif key == "-density":
             if val == "uniform":
                 o.uniform_density = 1
             else:
@@ -596,6 +627,10 @@ def opts_from_args(args, base=None):
     o.skip_io = int("-skip_io" in flags)
     o.disable_revert_if_diverge = int("-disable_revert_if_diverge" in flags)
     o.disable_pin_density_adjust = int("-disable_pin_density_adjust" in flags)
+    # -routability_driven, unless -skip_io turned it off (run.rs:253-269)
+    o.routability = int(o.routability or ("-routability_driven" in flags and not o.skip_io))
+    if o.routability and "-routability_use_grt" in flags:
+        raise GplError(GPL_ERR_REFUSED, "-routability_use_grt is not modelled (no global router)")
     return o, flags
 
 

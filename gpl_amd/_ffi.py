@@ -38,6 +38,11 @@ class Opts(C.Structure):
         ("disable_pin_density_adjust", i32), ("initial_place_max_iter", i32), ("initial_place_max_fanout", i32),
         ("initial_place_min_diff_length", i32), ("initial_place_max_solver_iter", i32),
         ("initial_place_net_weight_scale", f32), ("skip_initial_place", i32),
+        ("routability", i32), ("routability_end_overflow", f32), ("routability_snapshot_overflow", f32),
+        ("routability_target_rc", f32), ("routability_inflation_ratio_coef", f32), ("routability_max_inflation_ratio", f32),
+        ("routability_max_density", f32), ("routability_min_congestion", f32), ("routability_max_inflation_total", f32),
+        ("routability_net_weight_max", f32), ("routability_congested_nets_percentage", f32), ("routability_rc_k", f32 * 4),
+        ("routability_max_passes", i32), ("route_gcells", i32), ("route_tracks", i32),
     ]
 
 
@@ -66,7 +71,8 @@ class Region(C.Structure):
         ("cell_end", i32), ("num_fillers", i32), ("filler_dx", i32), ("filler_dy", i32), ("std_insts_area", i64),
         ("macro_insts_area", i64), ("non_place_insts_area", i64), ("white_space_area", i64), ("movable_area", i64),
         ("total_filler_area", i64), ("non_place", P(i64)), ("non_place_unscaled", P(i64)), ("bin_x", P(i32)),
-        ("bin_y", P(i32)), ("wx", P(f32)), ("wy", P(f32)), ("group", i32),
+        ("bin_y", P(i32)), ("wx", P(f32)), ("wy", P(f32)), ("group", i32), ("non_place_raw", P(i64)),
+        ("non_place_union", P(i64)),
     ]
 
 
@@ -77,7 +83,7 @@ class Problem(C.Structure):
         ("num_pins", i32), ("pin_cell", P(i32)), ("pin_offset", P(i32)), ("pin_xy", P(i32)), ("pin_net", P(i32)),
         ("num_nets", i32), ("net_pin_start", P(i32)), ("net_pins", P(i32)), ("net_weight", P(f32)),
         ("num_regions", i32), ("regions", P(Region)), ("opts", Opts), ("dbu", i32), ("site_x", i32), ("site_y", i32),
-        ("inst_box", P(i32)), ("num_insts", i32),
+        ("inst_box", P(i32)), ("num_insts", i32), ("die", i32 * 4),
     ]
 
 
@@ -114,6 +120,16 @@ class RouteVars(C.Structure):
 
 class RudyStats(C.Structure):
     _fields_ = [("total_overflow", f64), ("overflowed_tiles", i32), ("actionable_tiles", i32), ("top", f64 * 4), ("rc", f32)]
+
+
+class RoutePass(C.Structure):
+    _fields_ = [("iter", i32), ("rc", f32), ("overflowed_tiles", i32), ("actionable_tiles", i32), ("inflation", i64),
+                ("target_density", f32), ("fillers", i32), ("weighted_nets", i32), ("need", i32), ("revert", i32)]
+
+
+class RouteStats(C.Structure):
+    _fields_ = [("passes", i32), ("route_iters", i32), ("final_rc", f32), ("min_rc", f32), ("total_inflation", i64),
+                ("original_area", i64)]
 
 
 # (name, restype, argtypes) of every exported function declared in include/gpl_hip.h
@@ -169,6 +185,7 @@ SIGNATURES = [
     ("gpl_route_inflate", i32, [C.c_void_p, i32, P(RudyGrid), P(f32), P(i32), P(C.c_int64)]),
     ("gpl_update_density_size", i32, [C.c_void_p, i32, P(i32)]),
     ("gpl_get_density_boxes", i32, [C.c_void_p, i32, P(i32), P(f32)]),
+    ("gpl_get_route_stats", i32, [C.c_void_p, P(RouteStats), P(RoutePass), i32]),
     ("gpl_debug_inject", i32, [C.c_void_p, i32, i32]),
     ("gpl_kernel_times", i32, [C.c_void_p, P(KernelTime), i32]),
     ("gpl_set_profiling", i32, [C.c_void_p, i32]),

@@ -1355,7 +1355,7 @@ int32_t init_nesterov(gpl_design* g, std::string* err) {
                 for (int32_t y = yr.first; y < yr.second; ++y)
                     for (int32_t x = xr.first; x < xr.second; ++x) touching[size_t(y) * R.cnt_x + x].push_back(int32_t(k));
             }
-            std::vector<int64_t> uni(nbins, 0), raw(nbins, 0);
+            std::vector<int64_t> uni(nbins, -1), raw(nbins, 0);
             std::vector<Rect> rects;
             for (size_t b = 0; b < nbins; ++b) {
                 if (touching[b].empty()) continue;
@@ -1383,6 +1383,8 @@ int32_t init_nesterov(gpl_design* g, std::string* err) {
                 int64_t cap = (int64_t)(float(uni[b]) * R.target_density * 1.10f);
                 if (nb.non_place[b] > cap) nb.non_place[b] = cap;
             }
+            nb.non_place_raw = raw;
+            nb.non_place_union = uni;
         }
         // ---- FFT wavenumbers (nesterov.rs:382)
         nb.wx.resize(R.cnt_x);
@@ -1488,6 +1490,8 @@ int32_t init_nesterov(gpl_design* g, std::string* err) {
         gpl_region R = nb.r;
         R.non_place = nb.non_place.data();
         R.non_place_unscaled = nb.non_place_unscaled.data();
+        R.non_place_raw = nb.non_place_raw.data();
+        R.non_place_union = nb.non_place_union.data();
         R.bin_x = nb.bin_x.data();
         R.bin_y = nb.bin_y.data();
         R.wx = nb.wx.data();
@@ -1528,6 +1532,22 @@ void gpl_opts_default(gpl_opts* o) {
     o->initial_place_min_diff_length = 1500;
     o->initial_place_max_solver_iter = 100;
     o->initial_place_net_weight_scale = 800.0f;
+    // RouteVars::default (route_base.rs:29-45), Opts::default (nesterov.rs:77-78)
+    o->routability_end_overflow = 0.3f;
+    o->routability_snapshot_overflow = 0.6f;
+    o->routability_target_rc = 1.01f;
+    o->routability_inflation_ratio_coef = 2.0f;
+    o->routability_max_inflation_ratio = 3.0f;
+    o->routability_max_density = 0.99f;
+    o->routability_min_congestion = 0.95f;
+    o->routability_max_inflation_total = 0.30f;
+    o->routability_net_weight_max = 1.0f;
+    o->routability_congested_nets_percentage = 1.0f;
+    o->routability_rc_k[0] = 1.0f;
+    o->routability_rc_k[1] = 1.0f;
+    o->routability_max_passes = 50;
+    o->route_gcells = 64;
+    o->route_tracks = 8;
 }
 
 int32_t gpl_design_generate(const gpl_gen_config* cfg, const gpl_opts* o, gpl_design** out) {
@@ -1671,6 +1691,10 @@ int32_t gpl_design_problem(const gpl_design* d, gpl_problem* v) {
     if (!d || !v || !d->ni.built) return GPL_ERR_STATE;
     const gpl::NesterovInit& ni = d->ni;
     std::memset(v, 0, sizeof *v);
+    v->die[0] = d->d.die.lx;
+    v->die[1] = d->d.die.ly;
+    v->die[2] = d->d.die.ux;
+    v->die[3] = d->d.die.uy;
     v->num_cells = int32_t(ni.density_scale.size());
     v->cell_box = ni.cell_box.data();
     v->cell_dbox = ni.cell_dbox.data();

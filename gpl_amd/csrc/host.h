@@ -151,6 +151,7 @@ struct Placer {
 struct NbInit {
     gpl_region r{};
     std::vector<int64_t> non_place, non_place_unscaled;
+    std::vector<int64_t> non_place_raw, non_place_union;  // the fixed instances' overlap sums and union areas (-1: none touches)
     std::vector<int32_t> bin_x, bin_y;
     std::vector<float> wx, wy;
 };

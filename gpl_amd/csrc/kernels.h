@@ -300,6 +300,8 @@ hipError_t launch_route_inflate(hipStream_t s, int begin, int n, const CellsDev&
 // density scales, the preconditioner's areas
 hipError_t launch_density_size(hipStream_t s, int begin, int n, const CellsDev& c, const DensitySizeDev& d, const int2* sizes);
 
+
+
 constexpr int kPartials = 2048;  // per-block partial slots per quantity
 constexpr int kGradStage = 2048;  // pin terms of a gradient block's cells staged in LDS, at most
 constexpr int kPlaceChunk = 512;  // cells per block iteration of the scatter family
@@ -383,7 +385,7 @@ struct RegionTable {
     int32_t begin[kMaxRegions + 1];
     int32_t lx[kMaxRegions], ly[kMaxRegions], cnt_x[kMaxRegions], cnt_y[kMaxRegions];
     double size_x[kMaxRegions], size_y[kMaxRegions];
-    int32_t morton_bits;  // bits of the Morton part of a key (the region index above them)
+    int32_t morton_bits;  // bits of the Morton part of a key (a parked-cell bit, then the region index above them)
     int32_t end_bit;      // key bits the radix sort looks at
 };
 struct ResortBufs {
@@ -398,7 +400,7 @@ struct ResortBufs {
 };
 size_t resort_temp_bytes(int n_cells);
 // keys (region, Morton code of the current bin) and the stable sort: perm and inv
-void launch_resort_sort(hipStream_t s, int n_cells, const int2* ctr, const RegionTable& t, ResortBufs& b);
+void launch_resort_sort(hipStream_t s, int n_cells, const int2* ctr, const uint8_t* flags, const RegionTable& t, ResortBufs& b);
 // every per-cell array gathered into `out` (pin CSR rebuilt, the net pins' cell-pin indices
 // `cpi` renumbered), old_of composed
 void launch_resort_cells(hipStream_t s, int n_cells, const CellsDev& in, const CellsDev& out, ResortBufs& b, int32_t* cpi);
@@ -470,5 +472,33 @@ hipError_t set_gradient_node(hipGraphExec_t e, const TrialNode& n, const RegionD
 
 int block_partials_bins(const RegionDev& r);
 int block_partials_cells(const RegionDev& r);
+
+// ---- routability passes (route.hip) ------------------------------------------------------------
+// A cell's flag while routability has taken it out of its region (a removed filler): the re-sort
+// puts such cells after the region's live cells, which the kernels alone cover.
+constexpr uint8_t kCellParked = 8u;
+// The per-cell slots a removed filler keeps (the re-sort's live set): saved to and restored from a
+// region's store, stride cells per slot.
+struct ParkSlots {
+    float2* v[kGatherMax];
+    int32_t count;
+};
+// store[s * stride + idx[i]] = v[s][pos[i]] (save) or the reverse (restore), i < k
+hipError_t launch_park_save(hipStream_t s, int k, const int32_t* pos, const int32_t* idx, const ParkSlots& v, float2* store, int stride);
+hipError_t launch_park_restore(hipStream_t s, int k, const int32_t* pos, const int32_t* idx, const ParkSlots& v, const float2* store,
+                               int stride);
+// flags[pos[i]] = (flags[pos[i]] & ~clear) | set, i < k
+hipError_t launch_flag_list(hipStream_t s, int k, const int32_t* pos, uint8_t* flags, uint8_t set, uint8_t clear);
+// updateBinsNonPlaceArea at target density td from its target-independent halves (nesterov.rs:
+// 1956-2018): per bin non_place = min((raw * td) as i64, (union * td * 1.10) as i64) (the cap where
+// a fixed instance touches the bin, union >= 0; 0 where raw is 0), non_place_unscaled =
+// (union * td) as i64; written as int64 and as the bins pass's float pairs.
+hipError_t launch_non_place(hipStream_t s, int m, const long long* raw, const long long* uni, float td, long long* non_place,
+                            float2* non_place_f);
+// getNetDemandInTiles (route_base.rs:200-218): each net's demand summed in float over the tiles of
+// its box (x outer, y inner) whose inflation ratio (y-major, the last launch_rudy_tiles) is above 1,
+// written at the net's problem index (net_orig) into out.
+hipError_t launch_net_congested_demand(hipStream_t s, const NetsDev& n, const int32_t* net_orig, const RudyGrid& g, const float* ratio,
+                                       float* out);
 
 }  // namespace gplk

@@ -2616,7 +2616,7 @@ __device__ __forceinline__ uint32_t spread_bits(uint32_t v) {  // 11 bits -> eve
     return v;
 }
 
-__global__ void k_resort_keys(int N, const int2* ctr, RegionTable t, uint32_t* keys, int32_t* idx) {
+__global__ void k_resort_keys(int N, const int2* ctr, const uint8_t* flags, RegionTable t, uint32_t* keys, int32_t* idx) {
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < N; g += gridDim.x * blockDim.x) {
         int b = 0;
         while (b + 1 < t.n && g >= t.begin[b + 1]) ++b;
@@ -2632,7 +2632,9 @@ __global__ void k_resort_keys(int N, const int2* ctr, RegionTable t, uint32_t* k
         }
         bx >>= kResortShift;
         by >>= kResortShift;
-        keys[g] = ((uint32_t)b << t.morton_bits) | spread_bits(bx) | (spread_bits(by) << 1);
+        // a cell taken out of its region (kCellParked) after the region's live cells
+        const uint32_t parked = (flags[g] & kCellParked) ? 1u : 0u;
+        keys[g] = ((uint32_t)b << (t.morton_bits + 1)) | (parked << t.morton_bits) | spread_bits(bx) | (spread_bits(by) << 1);
         idx[g] = g;
     }
 }
@@ -2821,8 +2823,8 @@ size_t resort_temp_bytes(int N) {
     return a > b ? a : b;
 }
 
-void launch_resort_sort(hipStream_t s, int N, const int2* ctr, const RegionTable& t, ResortBufs& b) {
-    hipLaunchKernelGGL(k_resort_keys, dim3(grid_for(N)), dim3(kBlock), 0, s, N, ctr, t, b.keys_in, b.idx_in);
+void launch_resort_sort(hipStream_t s, int N, const int2* ctr, const uint8_t* flags, const RegionTable& t, ResortBufs& b) {
+    hipLaunchKernelGGL(k_resort_keys, dim3(grid_for(N)), dim3(kBlock), 0, s, N, ctr, flags, t, b.keys_in, b.idx_in);
     size_t bytes = b.temp_bytes;
     (void)hipcub::DeviceRadixSort::SortPairs(b.temp, bytes, b.keys_in, b.keys_out, b.idx_in, b.perm, N, 0, t.end_bit, s);
     hipLaunchKernelGGL(k_resort_inv, dim3(grid_for(N)), dim3(kBlock), 0, s, N, b.perm, b.inv);

@@ -192,6 +192,54 @@ struct gpl_ctx {
     float diverge_coef_x = 0, diverge_coef_y = 0;
     int64_t diverge_hpwl = 0;
     int32_t diverge_reverts = 0;
+    // Routability-driven placement (opts.routability): RouteBase (route_base.rs:279-326) and the
+    // Nesterov placer's routability state.  Cells routability takes out of a region (fillers) are
+    // parked after the region's live cells by a re-sort (kCellParked), their per-cell slots kept in
+    // the region's store, so that every kernel covers the live cells alone.
+    struct Route {
+        bool on = false;
+        gplk::RudyGrid grid{};
+        gplk::RouteVarsDev vars{};
+        int32_t revert_count = 0;  // RouteBase::revert_count: the measurements so far
+        float min_rc = 1e30f;
+        int32_t min_rc_violated = 0;
+        std::vector<float> min_rc_target_density;
+        std::vector<int64_t> inflated_delta, accumulated, min_rc_inflated;
+        int64_t original_movable_area = 0;
+        std::vector<float> inflation_ratio;  // per tile (y-major): the last measurement's ratios
+        std::vector<float> inflated_ratio;   // per tile: the pass's, trimmed to the budget
+        std::vector<float> net_weight;       // per problem net: RouteBase::net_weights (0: not in the map)
+        std::vector<float> min_rc_net_weight;
+        int32_t weighted = 0;
+        std::vector<int32_t> size;           // (dx, dy) per problem cell: the cell boxes now
+        std::vector<int32_t> min_rc_size;    // per problem cell at the minimum congestion ((0, 0) until kept)
+        bool need = true;                    // is_routability_need
+        int32_t settle_start = -1;           // routability_settle_wait_start_iter
+        // LoopSnapshots' routability half (nesterov.rs:4428-4437)
+        bool saved = false;
+        float coef_x = 0.0f, coef_y = 0.0f, a = 1.0f;
+        int32_t attempts = 0;
+        int32_t route_iters = 0, reverts = 0;
+        float final_rc = 0.0f;
+        int32_t last_overflowed = 0, last_actionable = 0;
+        std::vector<gpl_route_pass> passes;
+        // per region: its fillers in the reference's cell order (taken out from the back), the
+        // fillers taken out since the last minimum (in removal order), the filler area, the
+        // parked cells now on the device, the store, the first filler's problem index
+        std::vector<std::vector<int32_t>> fillers, removed;
+        std::vector<std::vector<uint8_t>> parked;  // per region, per filler: parked on the device
+        std::vector<int64_t> total_filler_area;
+        std::vector<float2*> store;
+        std::vector<int32_t> filler_base, num_fillers, n_alloc;
+        std::vector<long long*> np_raw, np_union;
+        int2* d_sizes = nullptr;     // N: a region's cell sizes in device order
+        int32_t* d_list = nullptr;   // 3 N: index lists of the parking kernels
+        float* d_ratio = nullptr;    // tiles: the inflated ratios
+        float* d_demand = nullptr;   // E: the nets' demand in congested tiles
+        unsigned long long* d_delta = nullptr;  // the inflation's added area
+        std::vector<float> demand;   // per problem net: its demand in the measured congested tiles
+    } rt;
+    std::vector<uint8_t> cell_flags_prob;  // GPL_CELL_* per problem cell, as constructed
     // bookkeeping
     int64_t cur_hpwl = 0;
     int32_t trials = 0, total_trials = 0, last_iter = -1;
@@ -648,7 +696,7 @@ struct gpl_ctx {
     // live region with locked cells (their next slots are fixed by a host-launched kernel) and few
     // enough regions for the control record.
     bool ctl_usable() const {
-        if (!device_trials || ctl_failed || launch_mode != 0 || profiling) return false;
+        if (!device_trials || ctl_failed || launch_mode != 0 || profiling || rt.on) return false;
         if (rg.size() > size_t(gplk::kCtlMaxRegions)) return false;
         for (auto& r : rg)
             if (!r.is_converged && r.has_locked) return false;
@@ -1224,7 +1272,7 @@ struct gpl_ctx {
     void resort() {
         const int32_t N = ncells;
         mark(F_REDUCE);
-        gplk::launch_resort_sort(stream, N, cells.ctr, rtab, rb);
+        gplk::launch_resort_sort(stream, N, cells.ctr, cells.flags, rtab, rb);
         gplk::launch_resort_cells(stream, N, cells, cells_alt, rb, nets.cpi);
         std::swap(cells, cells_alt);
         d_flags = cells.flags;
@@ -1233,7 +1281,7 @@ struct gpl_ctx {
         // slots are rewritten before any read); all of a region's in one launch
         static const int live[] = {S_CUR_SLP, S_CUR_SLP_SUM, S_PREV_SLP, S_PREV_SLP_SUM, S_CUR, S_SNAP, S_SNAP_SLP, S_SNAP_SLP_SUM,
                                    S_SNAP_PREV_SLP_SUM};
-        const int nlive = (diverge_saved || diverge_reverts > 0) ? 9 : 5;
+        const int nlive = (diverge_saved || diverge_reverts > 0 || rt.saved) ? 9 : 5;
         for (size_t b = 0; b < rg.size(); ++b) {
             Region& r = rg[b];
             gplk::GatherSet gs{};
@@ -1390,11 +1438,14 @@ struct gpl_ctx {
         cur_hpwl = hpwl_now();
     }
 
-    // isDiverged (nesterov.rs:3009), routability off: 0 none, 1 resume, 2 stop.
+    // isDiverged (nesterov.rs:3009): 0 none, 1 resume, 2 stop.  The min-HPWL snapshot only once
+    // routability is off (it shares the routability snapshot's buffer); a divergence tries the
+    // routability snapshot first.
     int np_is_diverged(float* cur_a) {
         num_region_diverged = 0;
         for (auto& r : rg) num_region_diverged += check_divergence(r) ? 1 : 0;
-        if (num_region_diverged == 0 && is_min_hpwl && !o.disable_revert_if_diverge) {
+        const bool routability_off = !rt.on || !rt.need;
+        if (num_region_diverged == 0 && is_min_hpwl && !o.disable_revert_if_diverge && routability_off) {
             diverge_coef_x = wl_coef_x;
             diverge_coef_y = wl_coef_y;
             diverge_hpwl = min_hpwl;
@@ -1404,6 +1455,10 @@ struct gpl_ctx {
         if (num_region_diverged == 0) return 0;
         log.report(strfmt("Divergence occured in %d regions.", num_region_diverged));
         bool disable = o.disable_revert_if_diverge != 0;
+        if (!disable && rt.on && try_routability_diverge_recovery(cur_a)) {
+            num_region_diverged = 0;
+            return 1;
+        }
         if (!disable && diverge_saved && diverge_reverts < 3) {
             diverge_reverts += 1;
             log.warn(998, strfmt("Divergence detected, reverting to snapshot with min hpwl and resuming (%d of 3 reverts allowed).", diverge_reverts));
@@ -1567,6 +1622,595 @@ struct gpl_ctx {
     }
 
     // checkConvergence (nesterov.rs:3107).
+    // ---- routability-driven placement ---------------------------------------------------------
+    bool is_std(int32_t g) const { return (cell_flags_prob[g] & GPL_CELL_INSTANCE) && !(cell_flags_prob[g] & GPL_CELL_MACRO); }
+    double dbu_area(double a) const { return a / (double(dbu) * double(dbu)); }
+
+    // The routability state at construction: RouteBase::new + route_base_init (route_base.rs:
+    // 298-339, nesterov.rs:3776-3782), the grid the router would supply, the parking stores.
+    void route_setup(const gpl_problem* p) {
+        rt.on = true;
+        // the die (region 0's box when the problem names none)
+        const gpl_region& R0 = p->regions[0];
+        int32_t die[4] = {p->die[0], p->die[1], p->die[2], p->die[3]};
+        if (die[2] <= die[0] || die[3] <= die[1]) {
+            die[0] = R0.lx;
+            die[1] = R0.ly;
+            die[2] = R0.ux;
+            die[3] = R0.uy;
+        }
+        const int32_t gcells = o.route_gcells > 0 ? o.route_gcells : 64;
+        const int32_t tile = std::max(1, (die[2] - die[0]) / gcells);
+        const int32_t pitch = std::max(1, tile / (o.route_tracks > 0 ? o.route_tracks : 8));
+        const int32_t pitches[2] = {pitch, pitch};
+        rt.grid = gplk::RudyGrid{die[0], die[1], die[2], die[3], gcells, std::max(1, (die[3] - die[1]) / tile), tile,
+                                 gpl_rudy_wire_width(pitches, 2)};
+        rt.vars = gplk::RouteVarsDev{o.routability_min_congestion, o.routability_inflation_ratio_coef, o.routability_max_inflation_ratio,
+                                     0.8f, {o.routability_rc_k[0], o.routability_rc_k[1], o.routability_rc_k[2], o.routability_rc_k[3]}};
+        const size_t nb = rg.size();
+        rt.min_rc_target_density.assign(nb, 0.0f);
+        rt.inflated_delta.assign(nb, 0);
+        rt.accumulated.assign(nb, 0);
+        rt.min_rc_inflated.assign(nb, 0);
+        rt.original_movable_area = 0;
+        for (auto& r : rg) rt.original_movable_area += nesterov_insts_area(r);
+        rt.size.resize(2 * size_t(ncells));
+        for (int32_t g = 0; g < ncells; ++g) {
+            rt.size[2 * g] = p->cell_box[4 * g + 2] - p->cell_box[4 * g];
+            rt.size[2 * g + 1] = p->cell_box[4 * g + 3] - p->cell_box[4 * g + 1];
+        }
+        rt.min_rc_size.assign(2 * size_t(ncells), 0);
+        rt.net_weight.assign(size_t(nnets), 0.0f);
+        rt.min_rc_net_weight.assign(size_t(nnets), 0.0f);
+        rt.fillers.resize(nb);
+        rt.removed.assign(nb, {});
+        rt.parked.resize(nb);
+        rt.total_filler_area.resize(nb);
+        for (size_t b = 0; b < nb; ++b) {
+            const gpl_region& R = p->regions[b];
+            rt.filler_base.push_back(R.cell_end - R.num_fillers);
+            rt.num_fillers.push_back(R.num_fillers);
+            rt.n_alloc.push_back(R.cell_end - R.cell_begin);
+            for (int32_t k = 0; k < R.num_fillers; ++k) rt.fillers[b].push_back(R.cell_end - R.num_fillers + k);
+            rt.parked[b].assign(size_t(R.num_fillers), 0);
+            rt.total_filler_area[b] = R.total_filler_area;
+            rt.store.push_back(dalloc<float2>(size_t(gplk::kGatherMax) * std::max(1, R.num_fillers)));
+            const size_t M = size_t(R.cnt_x) * R.cnt_y;
+            std::vector<int64_t> none(M, 0), neg(M, -1);
+            rt.np_raw.push_back((long long*)upload(R.non_place_raw ? R.non_place_raw : none.data(), M));
+            rt.np_union.push_back((long long*)upload(R.non_place_union ? R.non_place_union : neg.data(), M));
+        }
+        rt.d_sizes = dalloc<int2>(size_t(ncells));
+        rt.d_list = dalloc<int32_t>(3 * size_t(ncells));
+        rt.d_ratio = dalloc<float>(size_t(rt.grid.cnt_x) * rt.grid.cnt_y);
+        rt.d_demand = dalloc<float>(size_t(std::max(nnets, 1)));
+        rt.d_delta = dalloc<unsigned long long>(1);
+    }
+
+    // setTargetDensity (nesterov.rs:3793-3801): the region's and its bins' target density, the
+    // bins' fixed area again at it.
+    void set_target_density(int b, float td) {
+        Region& r = rg[b];
+        r.R.target_density = td;
+        r.dev.target_density = td;
+        check(gplk::launch_non_place(stream, r.R.cnt_x * r.R.cnt_y, rt.np_raw[b], rt.np_union[b], td, const_cast<long long*>(r.dev.non_place),
+                                     const_cast<float2*>(r.dev.non_place_f)),
+              "non place");
+        r.nia = nesterov_insts_area(r);
+        drop_graph();
+    }
+
+    // updateAreas (nesterov.rs:3804-3822) from the cell sizes now.
+    void update_areas(int b) {
+        Region& r = rg[b];
+        int64_t st = 0, mac = 0;
+        for (int32_t g = r.R.cell_begin; g < rt.filler_base[b]; ++g) {
+            if (!(cell_flags_prob[g] & GPL_CELL_INSTANCE)) continue;
+            const int64_t a = int64_t(rt.size[2 * g]) * int64_t(rt.size[2 * g + 1]);
+            if (cell_flags_prob[g] & GPL_CELL_MACRO)
+                mac += a;
+            else
+                st += a;
+        }
+        r.R.std_insts_area = st;
+        r.R.macro_insts_area = mac;
+        r.nia = nesterov_insts_area(r);
+    }
+
+    // the region's live cells' sizes in device order
+    void upload_sizes(int b) {
+        ensure_host_map();
+        Region& r = rg[b];
+        std::vector<int2> sz((size_t)std::max(r.n, 1));
+        for (int k = 0; k < r.n; ++k) {
+            const int32_t g = cell_old_of[r.R.cell_begin + k];
+            sz[k] = make_int2(rt.size[2 * size_t(g)], rt.size[2 * size_t(g) + 1]);
+        }
+        check(hipMemcpyAsync(rt.d_sizes, sz.data(), sizeof(int2) * sz.size(), hipMemcpyHostToDevice, stream), "upload");
+        check(hipStreamSynchronize(stream), "sync");
+    }
+
+    // updateDensitySize of the region's live cells at their sizes now (nesterov.rs:2022-2036).
+    void update_density_size(int b);
+
+    // The device's parked cells made those routability took out: newly taken fillers saved and
+    // parked, restored ones back among the live cells with their slots; one re-sort.
+    void sync_parking() {
+        ensure_host_map();
+        bool change = false;
+        std::vector<std::vector<int32_t>> to_park(rg.size()), to_back(rg.size());
+        for (size_t b = 0; b < rg.size(); ++b) {
+            std::vector<uint8_t> live(size_t(rt.num_fillers[b]), 0);
+            for (int32_t g : rt.fillers[b]) live[size_t(g - rt.filler_base[b])] = 1;
+            for (int32_t k = 0; k < rt.num_fillers[b]; ++k) {
+                if (!live[k] && !rt.parked[b][k]) to_park[b].push_back(rt.filler_base[b] + k);
+                if (live[k] && rt.parked[b][k]) to_back[b].push_back(rt.filler_base[b] + k);
+            }
+            change = change || !to_park[b].empty() || !to_back[b].empty();
+        }
+        if (!change) return;
+        join();
+        auto slots = [&](Region& r) {
+            static const int live[] = {S_CUR_SLP, S_CUR_SLP_SUM, S_PREV_SLP, S_PREV_SLP_SUM, S_CUR, S_SNAP, S_SNAP_SLP, S_SNAP_SLP_SUM,
+                                       S_SNAP_PREV_SLP_SUM};
+            gplk::ParkSlots ps{};
+            ps.count = gplk::kGatherMax;
+            for (int i = 0; i < gplk::kGatherMax; ++i) ps.v[i] = r.v[live[i]];
+            return ps;
+        };
+        // d_list: the cells' device indices, their indices within the region (the slots'), their
+        // store indices
+        auto lists = [&](const std::vector<int32_t>& cells_, int b) {
+            const size_t k = cells_.size();
+            std::vector<int32_t> h(3 * k);
+            for (size_t i = 0; i < k; ++i) {
+                h[i] = cell_new_of[cells_[i]];
+                h[k + i] = h[i] - rg[b].R.cell_begin;
+                h[2 * k + i] = cells_[i] - rt.filler_base[b];
+            }
+            if (!h.empty()) check(hipMemcpyAsync(rt.d_list, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, stream), "upload");
+            check(hipStreamSynchronize(stream), "sync");
+        };
+        for (size_t b = 0; b < rg.size(); ++b) {
+            Region& r = rg[b];
+            const int kp = int(to_park[b].size()), kb = int(to_back[b].size());
+            if (kp) {
+                lists(to_park[b], int(b));
+                check(gplk::launch_park_save(stream, kp, rt.d_list + kp, rt.d_list + 2 * kp, slots(r), rt.store[b], rt.num_fillers[b]), "park");
+                check(gplk::launch_flag_list(stream, kp, rt.d_list, cells.flags, gplk::kCellParked, 0), "flags");
+            }
+            if (kb) {
+                lists(to_back[b], int(b));
+                check(gplk::launch_flag_list(stream, kb, rt.d_list, cells.flags, 0, gplk::kCellParked), "flags");
+            }
+            for (int32_t g : to_park[b]) rt.parked[b][size_t(g - rt.filler_base[b])] = 1;
+            for (int32_t g : to_back[b]) rt.parked[b][size_t(g - rt.filler_base[b])] = 0;
+            int32_t np_ = 0;
+            for (uint8_t q : rt.parked[b]) np_ += q;
+            r.n = rt.n_alloc[b] - np_;
+            r.dev.cell_end = r.R.cell_begin + r.n;
+        }
+        resort();
+        ensure_host_map();
+        for (size_t b = 0; b < rg.size(); ++b) {
+            const int kb = int(to_back[b].size());
+            if (!kb) continue;
+            lists(to_back[b], int(b));
+            check(gplk::launch_park_restore(stream, kb, rt.d_list + kb, rt.d_list + 2 * kb, slots(rg[b]), rt.store[b], rt.num_fillers[b]),
+                  "unpark");
+        }
+        check(hipStreamSynchronize(stream), "sync");
+        last_resort_iter = iter_now;
+        drop_graph();
+    }
+
+    // cutFillerCells (nesterov.rs:3826-3870): whole fillers from the back of the region's cells
+    // for the inflation's area, the target density raised for what they cannot absorb.
+    void cut_filler_cells(int b, int64_t inflation_area) {
+        Region& r = rg[b];
+        if (inflation_area < 0) {
+            log.warn(313, strfmt("Negative area provided to remove fillers: %g. Expected positive value, ignoring.", dbu_area(double(inflation_area))));
+            return;
+        }
+        const int64_t single = int64_t(r.R.filler_dx) * int64_t(r.R.filler_dy);
+        auto& fl = rt.fillers[b];
+        const int64_t max_remove = single > 0 ? std::min<int64_t>(inflation_area / single, int64_t(fl.size())) : 0;
+        int64_t available = single * int64_t(fl.size()), removed = 0;
+        while (!fl.empty() && removed < max_remove) {
+            rt.removed[b].push_back(fl.back());
+            fl.pop_back();
+            available -= single;
+            ++removed;
+        }
+        rt.total_filler_area[b] = available;
+        if (single * int64_t(fl.size()) != rt.total_filler_area[b])
+            log.warn(312, strfmt("Unexpected filler area! The value %g, should be equal to totalFillerArea_ %g.", dbu_area(double(single * int64_t(fl.size()))),
+                                 dbu_area(double(rt.total_filler_area[b]))));
+        const int64_t removed_area = single * removed, remaining = inflation_area - removed_area;
+        if (remaining > single) {
+            const int64_t total = nesterov_insts_area(r) + removed_area + rt.total_filler_area[b] + remaining;
+            set_target_density(b, (float)total / (float)r.R.white_space_area);
+            r.R.movable_area = (int64_t)((float)r.R.white_space_area * r.R.target_density);
+        }
+    }
+
+    // restoreRemovedFillers (nesterov.rs:3873-3891): put back at the end in the order taken.
+    void restore_removed_fillers(int b) {
+        const int64_t single = int64_t(rg[b].R.filler_dx) * int64_t(rg[b].R.filler_dy);
+        for (int32_t g : rt.removed[b]) {
+            rt.fillers[b].push_back(g);
+            rt.total_filler_area[b] += single;
+        }
+        rt.removed[b].clear();
+    }
+
+    // applyNetWeights (nesterov.rs:3922-3937): each net's weight from the map (at least 1), every
+    // other net 1; uploaded in the device's net order.
+    int32_t apply_net_weights() {
+        ensure_net_map();
+        int32_t weighted = 0;
+        std::vector<float> w((size_t)nnets);
+        for (int32_t i = 0; i < nnets; ++i) {
+            const float x = rt.net_weight[size_t(net_orig_of[i])];
+            w[i] = std::max(1.0f, x);
+            if (w[i] > 1.0f) ++weighted;
+        }
+        check(hipMemcpyAsync(const_cast<float*>(nets.weight), w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, stream), "upload");
+        check(hipStreamSynchronize(stream), "sync");
+        rt.weighted = weighted;
+        return weighted;
+    }
+
+    // revertToMinCongestion (nesterov.rs:3896-3918).
+    void revert_to_min_congestion() {
+        if (rt.revert_count < 1) return;
+        log.info(55, strfmt("Reverting inflation and target density to the pass with the minimum observed routing congestion (%.4f), target density %.4f.",
+                            (double)rt.min_rc, (double)(rt.min_rc_target_density.empty() ? 0.0f : rt.min_rc_target_density[0])));
+        for (int32_t g = 0; g < ncells; ++g)
+            if (is_std(g)) {
+                rt.size[2 * size_t(g)] = rt.min_rc_size[2 * size_t(g)];
+                rt.size[2 * size_t(g) + 1] = rt.min_rc_size[2 * size_t(g) + 1];
+            }
+        for (size_t b = 0; b < rg.size(); ++b) {
+            set_target_density(int(b), rt.min_rc_target_density[b]);
+            restore_removed_fillers(int(b));
+        }
+        sync_parking();
+        for (size_t b = 0; b < rg.size(); ++b) {
+            update_density_size(int(b));
+            rt.accumulated[b] = rt.min_rc_inflated[b];
+        }
+        rt.net_weight = rt.min_rc_net_weight;
+        apply_net_weights();
+        std::fill(rt.inflated_delta.begin(), rt.inflated_delta.end(), 0);
+    }
+
+    // The tile under a point (getTile, route_base.rs:428-437): origin (0, 0), the last column and
+    // row clamped; -1 left of or below the grid.
+    int64_t tile_at(int32_t x, int32_t y) const {
+        const int32_t ix = std::min(x / rt.grid.tile, rt.grid.cnt_x - 1), iy = std::min(y / rt.grid.tile, rt.grid.cnt_y - 1);
+        if (ix < 0 || iy < 0) return -1;
+        return int64_t(iy) * rt.grid.cnt_x + ix;
+    }
+
+    // The density centres (d_cx, d_cy) of every live cell, the problem's order.
+    std::vector<int32_t> density_centres() {
+        ensure_host_map();
+        std::vector<int4> db((size_t)ncells);
+        check(hipMemcpyAsync(db.data(), cells.dbox, sizeof(int4) * ncells, hipMemcpyDeviceToHost, stream), "copy");
+        check(hipStreamSynchronize(stream), "sync");
+        std::vector<int32_t> c(2 * size_t(ncells));
+        for (int32_t i = 0; i < ncells; ++i) {
+            const int32_t g = cell_old_of[i];
+            c[2 * size_t(g)] = (db[i].x + db[i].z) / 2;
+            c[2 * size_t(g) + 1] = (db[i].y + db[i].w) / 2;
+        }
+        return c;
+    }
+
+    // scaleInflationToBudget (nesterov.rs:3946-3978): whether the budget is spent.
+    bool scale_inflation_to_budget() {
+        if (rt.original_movable_area <= 0) return false;
+        const double budget = double(rt.original_movable_area) * double(o.routability_max_inflation_total);
+        int64_t total = 0;
+        for (int64_t a : rt.accumulated) total += a;
+        const double spent = double(total);
+        const double allowed = std::min(budget / 6.0, budget - spent);
+        const std::vector<int32_t> ctr = density_centres();
+        double wanted = 0.0;
+        for (size_t b = 0; b < rg.size(); ++b)
+            for (int32_t g = rg[b].R.cell_begin; g < rt.filler_base[b]; ++g) {
+                if (!is_std(g)) continue;
+                const int64_t t = tile_at(ctr[2 * size_t(g)], ctr[2 * size_t(g) + 1]);
+                if (t < 0 || t >= int64_t(rt.inflated_ratio.size())) continue;
+                const float ratio = rt.inflated_ratio[size_t(t)];
+                if ((double)ratio <= 1.0) continue;
+                const double area = double(rt.size[2 * size_t(g)]) * double(rt.size[2 * size_t(g) + 1]);
+                wanted += area * ((double)ratio - 1.0);
+            }
+        if (wanted <= 0.0 || wanted <= allowed) return spent + wanted >= budget;
+        const double scale = std::max(0.0, allowed / wanted);
+        for (float& r : rt.inflated_ratio)
+            if (r > 1.0f) r = (float)(1.0 + scale * ((double)r - 1.0));
+        return spent + allowed >= budget;
+    }
+
+    // RUDY and the tile grid at the current placement (calculateRudyTiles + updateRudyAverage,
+    // route_base.rs:443-536): the weighted congestion; the tiles' inflation ratios kept.
+    float measure_rudy() {
+        join();
+        wa_forward(true);  // the nets' boxes at the cells' centres, into the net records
+        join();
+        check(gplk::launch_rudy(stream, nets, d_net_orig, rt.grid, nullptr, rudy), "rudy");
+        check(gplk::launch_rudy_tiles(stream, cells, ncells, rt.grid, rt.vars, rudy), "rudy tiles");
+        double st[8];
+        const size_t tiles = size_t(rt.grid.cnt_x) * rt.grid.cnt_y;
+        rt.inflation_ratio.resize(tiles);
+        check(hipMemcpyAsync(st, rudy.stats, sizeof st, hipMemcpyDeviceToHost, stream), "copy");
+        check(hipMemcpyAsync(rt.inflation_ratio.data(), rudy.ratio, sizeof(float) * tiles, hipMemcpyDeviceToHost, stream), "copy");
+        check(hipStreamSynchronize(stream), "sync");
+        rt.last_overflowed = int32_t(st[1]);
+        rt.last_actionable = int32_t(st[2]);
+        return float(st[7]);
+    }
+
+    // getNetDemandInTiles over the measured congested tiles (inflation ratio above 1), per problem
+    // net, while the net records hold the measured boxes.
+    void congested_net_demand() {
+        rt.demand.clear();
+        if (o.routability_net_weight_max <= 1.0f || o.routability_congested_nets_percentage <= 0.0f || nnets == 0) return;
+        check(gplk::launch_net_congested_demand(stream, nets, d_net_orig, rt.grid, rudy.ratio, rt.d_demand), "net demand");
+        rt.demand.resize(size_t(nnets));
+        check(hipMemcpyAsync(rt.demand.data(), rt.d_demand, sizeof(float) * rt.demand.size(), hipMemcpyDeviceToHost, stream), "copy");
+        check(hipStreamSynchronize(stream), "sync");
+    }
+
+    // weightCongestedNets (route_base.rs:381-426): the nets depositing the most demand in the
+    // tiles about to be inflated weighted; -1 when none were (the map cleared either way).
+    int32_t weight_congested_nets() {
+        std::fill(rt.net_weight.begin(), rt.net_weight.end(), 0.0f);
+        const float wmax = o.routability_net_weight_max, pct = o.routability_congested_nets_percentage;
+        if (wmax <= 1.0f || pct <= 0.0f || nnets == 0 || rt.demand.size() != size_t(nnets)) return -1;
+        const std::vector<float>& dem = rt.demand;
+        std::vector<std::pair<int32_t, float>> d;
+        for (int32_t e = 0; e < nnets; ++e)
+            if (dem[e] > 0.0f) d.push_back({e, dem[e]});
+        if (d.empty()) return -1;
+        // by demand, largest first (ties in the nets' order: a net tied at the cutoff gets weight 1
+        // whichever way the tie is broken, unless every chosen net ties with the worst)
+        std::stable_sort(d.begin(), d.end(), [](const std::pair<int32_t, float>& a, const std::pair<int32_t, float>& b) { return a.second > b.second; });
+        const size_t n = d.size();
+        const size_t share = (size_t)std::ceil((double)((float)n * pct) / 100.0);
+        const size_t count = std::min(n, std::max<size_t>(share, 1));
+        const float worst = d[0].second, cutoff = d[count - 1].second;
+        for (size_t i = 0; i < count; ++i) {
+            const float w = worst > cutoff ? 1.0f + (wmax - 1.0f) * (d[i].second - cutoff) / (worst - cutoff) : wmax;
+            rt.net_weight[size_t(d[i].first)] = w;
+        }
+        return int32_t(n);
+    }
+
+    // RouteBase::routability (nesterov.rs:3985-4093): (is_routability_need, is_revert_init_needed).
+    std::pair<bool, bool> routability(int32_t revert_count) {
+        rt.revert_count += 1;
+        const float cur_rc = measure_rudy();
+        congested_net_demand();
+        {
+            const int32_t over = rt.last_overflowed, act = rt.last_actionable;
+            const double pct = act > 0 ? double(over) / double(act) * 100.0 : 0.0;
+            log.info(40, strfmt("Routability pass %d: weighted congestion %.4f (target %.4f), %d of %d actionable tiles overflowed (%.1f%%).",
+                                rt.revert_count, (double)cur_rc, (double)o.routability_target_rc, over, act, pct));
+        }
+        gpl_route_pass row{};
+        row.iter = iter_now;
+        row.rc = cur_rc;
+        row.overflowed_tiles = rt.last_overflowed;
+        row.actionable_tiles = rt.last_actionable;
+        auto finish = [&](bool need, bool revert) {
+            int64_t add = 0;
+            for (int64_t v : rt.inflated_delta) add += v;
+            row.inflation = add;
+            row.target_density = rg[0].R.target_density;
+            row.fillers = int32_t(rt.fillers[0].size());
+            row.weighted_nets = rt.weighted;
+            row.need = need;
+            row.revert = revert;
+            rt.passes.push_back(row);
+            return std::make_pair(need, revert);
+        };
+        if (cur_rc < o.routability_target_rc) {
+            log.info(50, strfmt("Weighted routing congestion is lower than target routing congestion(%.4f), end routability optimization.",
+                                (double)o.routability_target_rc));
+            auto res = finish(false, false);
+            std::fill(rt.inflated_delta.begin(), rt.inflated_delta.end(), 0);
+            return res;
+        }
+        if ((rt.min_rc - cur_rc) > rt.min_rc * 0.005f) {
+            log.info(48, strfmt("Congestion improved on previous minimum (%.4g); updating minimum.", (double)rt.min_rc));
+            rt.min_rc = cur_rc;
+            rt.min_rc_violated = 0;
+            for (size_t b = 0; b < rg.size(); ++b) {
+                rt.min_rc_target_density[b] = rg[b].R.target_density;
+                rt.removed[b].clear();
+            }
+            for (int32_t g = 0; g < ncells; ++g)
+                if (is_std(g)) {
+                    rt.min_rc_size[2 * size_t(g)] = rt.size[2 * size_t(g)];
+                    rt.min_rc_size[2 * size_t(g) + 1] = rt.size[2 * size_t(g) + 1];
+                }
+            rt.min_rc_inflated = rt.accumulated;
+            rt.min_rc_net_weight = rt.net_weight;
+        } else {
+            rt.min_rc_violated += 1;
+            log.info(49, strfmt("No improvement on minimum (%.4f); consecutive non-improvement count: %d.", (double)rt.min_rc, rt.min_rc_violated));
+        }
+        rt.inflated_ratio.resize(rt.inflation_ratio.size());
+        for (size_t t = 0; t < rt.inflation_ratio.size(); ++t) rt.inflated_ratio[t] = rt.inflation_ratio[t] > 1.0f ? rt.inflation_ratio[t] : 1.0f;
+        const bool budget_spent = scale_inflation_to_budget();
+        check(hipMemcpyAsync(rt.d_ratio, rt.inflated_ratio.data(), sizeof(float) * rt.inflated_ratio.size(), hipMemcpyHostToDevice, stream),
+              "upload");
+        for (size_t b = 0; b < rg.size(); ++b) {
+            Region& r = rg[b];
+            rt.inflated_delta[b] = 0;
+            {
+                // the standard cells in inflated tiles grown (device), their sizes back in the problem's order
+                upload_sizes(int(b));
+                gplk::launch_set_u64(stream, rt.d_delta, 0ull);
+                check(gplk::launch_route_inflate(stream, r.R.cell_begin, r.n, cells, rt.grid, rt.d_ratio, rt.d_sizes, rt.d_delta), "inflate");
+                std::vector<int2> sz((size_t)std::max(r.n, 1));
+                unsigned long long dl = 0;
+                check(hipMemcpyAsync(sz.data(), rt.d_sizes, sizeof(int2) * sz.size(), hipMemcpyDeviceToHost, stream), "copy");
+                check(hipMemcpyAsync(&dl, rt.d_delta, sizeof dl, hipMemcpyDeviceToHost, stream), "copy");
+                check(hipStreamSynchronize(stream), "sync");
+                for (int k = 0; k < r.n; ++k) {
+                    const int32_t g = cell_old_of[r.R.cell_begin + k];
+                    rt.size[2 * size_t(g)] = sz[k].x;
+                    rt.size[2 * size_t(g) + 1] = sz[k].y;
+                }
+                rt.inflated_delta[b] = (int64_t)dl;
+            }
+            rt.accumulated[b] += rt.inflated_delta[b];
+            const int64_t delta = rt.inflated_delta[b];
+            const float pct = ((float)delta / (float)nesterov_insts_area(r)) * 100.0f;
+            const float delta_um2 = (float)dbu_area(double(delta));
+            cut_filler_cells(int(b), delta);
+            {
+                int64_t total = 0;
+                for (int64_t a : rt.accumulated) total += a;
+                const double spent = rt.original_movable_area > 0 ? 100.0 * double(total) / double(rt.original_movable_area) : 0.0;
+                const std::string group = r.R.group >= 0 ? strfmt(" in group group%d", r.R.group) : std::string();
+                log.info(86, strfmt("Inflation %+.3f um^2 (%+.2f%% of movable), %.2f%% of the %.2f%% budget spent; target density %.4f%s.",
+                                    (double)delta_um2, (double)pct, spent, 100.0 * (double)o.routability_max_inflation_total,
+                                    (double)r.R.target_density, group.c_str()));
+            }
+            if (r.R.target_density > o.routability_max_density) {
+                const std::string group = r.R.group >= 0 ? strfmt(" in group group%d", r.R.group) : std::string();
+                log.info(53, strfmt("Target density %.4f exceeds the maximum allowed %.4f%s.", (double)r.R.target_density,
+                                    (double)o.routability_max_density, group.c_str()));
+                revert_to_min_congestion();
+                return finish(false, true);
+            }
+        }
+        if (revert_count >= o.routability_max_passes) {
+            log.info(91, strfmt("Routability mode reached the maximum allowed reverts %d", revert_count));
+            revert_to_min_congestion();
+            return finish(false, true);
+        }
+        if (rt.min_rc_violated >= 3) {
+            log.info(54, strfmt("No improvement in routing congestion for %d consecutive iterations (limit is %d).", rt.min_rc_violated, 3));
+            revert_to_min_congestion();
+            return finish(false, true);
+        }
+        sync_parking();
+        for (size_t b = 0; b < rg.size(); ++b) {
+            update_areas(int(b));
+            update_density_size(int(b));
+        }
+        const int32_t depositors = weight_congested_nets();
+        const int32_t weighted = apply_net_weights();
+        if (depositors >= 0)
+            log.info(93, strfmt("Weighted %d of %d nets feeding congested tiles, up to %.2fx wirelength weight.", weighted, depositors,
+                                (double)o.routability_net_weight_max));
+        auto res = finish(!budget_spent, true);
+        std::fill(rt.inflated_delta.begin(), rt.inflated_delta.end(), 0);
+        if (budget_spent) {
+            int64_t total = 0;
+            for (int64_t a : rt.accumulated) total += a;
+            log.info(97, strfmt("Total routability inflation %.2f%% of the movable area spent the %.2f%% budget, ending routability.",
+                                100.0 * double(total) / double(rt.original_movable_area), 100.0 * (double)o.routability_max_inflation_total));
+            revert_to_min_congestion();
+        }
+        return res;
+    }
+
+    // isPlacementSettled / getWorstSettleRatio (nesterov.rs:3172-3181).
+    bool is_placement_settled() const {
+        for (auto& r : rg)
+            if (!(r.peak_coordi_distance > 0.0f && r.coordi_distance <= 0.6f * r.peak_coordi_distance)) return false;
+        return true;
+    }
+    float worst_settle_ratio() const {
+        float w = 0.0f;
+        for (auto& r : rg) w = std::max(w, r.peak_coordi_distance <= 0.0f ? 1.0f : r.coordi_distance / r.peak_coordi_distance);
+        return w;
+    }
+
+    // tryRoutabilityDivergeRecovery (nesterov.rs:3062-3090): true when it reverted.
+    bool try_routability_diverge_recovery(float* cur_a) {
+        if (!rt.saved || rt.attempts >= 2) return false;
+        if (!rt.need && (diverge_saved || diverge_reverts > 0)) return false;
+        rt.attempts = rt.need ? rt.attempts + 1 : 2;
+        const bool last = rt.attempts == 2;
+        if (rt.need) revert_to_min_congestion();
+        if (last) rt.need = false;
+        log.warn(112, strfmt("Divergence detected, reverting to the routability snapshot (attempt %d of %d), further inflation %s.", rt.attempts, 2,
+                             last ? "disabled" : "still allowed"));
+        wl_coef_x = rt.coef_x;
+        wl_coef_y = rt.coef_y;
+        wa_forward();
+        for (size_t b = 0; b < rg.size(); ++b) {
+            revert_to_snapshot(int(b));
+            rg[b].min_sum_overflow = 1e30f;
+            rg[b].hpwl_with_min_sum_overflow = 1e30f;
+        }
+        sync_stats();
+        *cur_a = 1.0f;
+        rt.settle_start = -1;
+        min_hpwl = INT64_MAX;
+        is_min_hpwl = false;
+        return true;
+    }
+
+    // The loop's routability step after the divergence test (nesterov.rs:4521-4567): the
+    // routability snapshot, then a routability pass once the overflow reached its end and the
+    // cells settled.
+    void routability_step(int32_t iter, float* cur_a) {
+        if (!rt.saved && o.routability_snapshot_overflow >= average_overflow_unscaled) {
+            rt.coef_x = wl_coef_x;
+            rt.coef_y = wl_coef_y;
+            rt.a = *cur_a;
+            rt.saved = true;
+            for (auto& r : rg) save_snapshot(r);
+            log.info(38, strfmt("Routability snapshot saved at iter = %d", iter + 1));
+            log.report(strfmt("%9d | %8.4f | %13.6e | %8s | %9s | %5s", iter, (double)average_overflow_unscaled, (double)cur_hpwl / (double)dbu, " ", " ", " "));
+        }
+        const bool active = rt.need && average_overflow_unscaled <= o.routability_end_overflow;
+        const bool settled = active && is_placement_settled();
+        if (active && !settled && rt.settle_start == -1) {
+            rt.settle_start = iter;
+            log.info(99, strfmt("Routability end extended at iter = %d: overflow %.4f reached %.4f, but cells are still moving at %.0f%% of peak displacement (waiting for %.0f%%).",
+                                iter + 1, (double)average_overflow_unscaled, (double)o.routability_end_overflow,
+                                (double)(worst_settle_ratio() * 100.0f), (double)(0.6f * 100.0f)));
+        } else if (settled && rt.settle_start != -1) {
+            log.info(103, strfmt("Routability end extension finished at iter = %d after %d extra iterations: overflow %.4f, cells moving at %.0f%% of peak displacement.",
+                                 iter + 1, iter - rt.settle_start, (double)average_overflow_unscaled, (double)(worst_settle_ratio() * 100.0f)));
+            rt.settle_start = -1;
+        }
+        if (!settled) return;
+        rg[0].reprint_header = true;
+        rt.reverts += 1;
+        const std::pair<bool, bool> res = routability(rt.reverts);
+        const bool need = res.first, revert_init = res.second;
+        rt.need = need;
+        min_hpwl = INT64_MAX;
+        is_min_hpwl = false;
+        if (need || revert_init) {
+            *cur_a = rt.a;
+            wl_coef_x = rt.coef_x;
+            wl_coef_y = rt.coef_y;
+            wa_forward();
+            for (size_t b = 0; b < rg.size(); ++b) {
+                revert_to_snapshot(int(b));
+                rg[b].min_sum_overflow = 1e30f;
+                rg[b].hpwl_with_min_sum_overflow = 1e30f;
+            }
+            sync_stats();
+        }
+        if (need && revert_init) log.info(87, "Routability end iteration: increase inflation and revert back to snapshot.");
+        if (!need && revert_init) log.info(89, "Routability finished. Reverting to minimal observed routing congestion, could not reach target.");
+        if (!need && !revert_init) log.info(90, "Routability finished. Target routing congestion achieved succesfully.");
+    }
+
     bool check_convergence(int b, int iter) {
         Region& r = rg[b];
         if (r.is_converged) return true;
@@ -1578,8 +2222,14 @@ struct gpl_ctx {
         log.report("---------------------------------------------------------------");
         if (r.R.group >= 0)
             log.info(1016, strfmt("Region '%s' placement finished at iteration %d", group.c_str(), iter));
-        else
+        else {
             log.info(1001, strfmt("Global placement finished at iteration %d", iter));
+            if (rt.on) log.info(1017, strfmt("Routability mode iteration count: %d", rt.route_iters));
+        }
+        if (rt.on) {
+            rt.final_rc = measure_rudy();
+            log.info(1005, strfmt("Routability final weighted congestion: %.4f", (double)rt.final_rc));
+        }
         double dbu2 = double(dbu) * double(dbu);
         double w = double(r.R.white_space_area);
         log.info(1002, strfmt("Placed Cell Area            %10.4f", double(r.nia) / dbu2));
@@ -1638,6 +2288,10 @@ struct gpl_ctx {
         diverge_saved = false;
         diverge_reverts = 0;
         diverge_hpwl = 0;
+        rt.saved = false;
+        rt.attempts = 0;
+        rt.route_iters = 0;
+        rt.reverts = 0;
         converged = false;
         ls.iter = start_iter;
         ls.active = true;
@@ -1683,6 +2337,11 @@ struct gpl_ctx {
                 last_iter = ls.iter;
                 ++iters_run;
             }
+            // is_routability_gpl_iter: each such iteration extends the limit by one
+            if (rt.on && rt.saved && average_overflow_unscaled > o.routability_end_overflow) {
+                rt.route_iters += 1;
+                max_iter += 1;
+            }
             int act = np_is_diverged(&ls.cur_a);
             if (act == 2) {
                 ls.done = true;
@@ -1692,6 +2351,7 @@ struct gpl_ctx {
                 ++ls.iter;
                 continue;
             }
+            if (rt.on) routability_step(ls.iter, &ls.cur_a);
             size_t conv = 0;
             for (size_t b = 0; b < rg.size(); ++b) conv += check_convergence(int(b), ls.iter) ? 1 : 0;
             if (conv == rg.size()) {
@@ -1722,6 +2382,13 @@ struct gpl_ctx {
         int64_t new_area = 0;
         for (auto& r : rg) new_area += r.nia;
         double dbu2 = double(dbu) * double(dbu);
+        if (rt.on) {
+            log.info(1011, strfmt("Original area (um^2): %.2f", double(ls.original_area) / dbu2));
+            int64_t inflation = 0;
+            for (int64_t a : rt.accumulated) inflation += a;
+            const float pct = (float)(100.0 * double(inflation) / double(ls.original_area));
+            log.info(1012, strfmt("Total routability artificial inflation: %.2f (%+.2f%%)", double(inflation) / dbu2, (double)pct));
+        }
         float diff = (float)(100.0 * double(new_area - ls.original_area) / double(ls.original_area));
         log.info(1014, strfmt("Final placement area: %.2f (%+.2f%%)", double(new_area) / dbu2, diff));
         check(hipStreamSynchronize(stream), "sync");
@@ -2127,12 +2794,22 @@ int32_t gpl_ctx_create(const gpl_problem* p, gpl_ctx** out) {
             int32_t rbits = 0;
             while ((1 << rbits) < c->rtab.n) ++rbits;
             c->rtab.morton_bits = 2 * bits;
-            c->rtab.end_bit = 2 * bits + rbits;
+            c->rtab.end_bit = 2 * bits + 1 + rbits;
             if (c->rtab.end_bit > 32 || bits > 11) c->resort_enabled = false;
         }
         for (int32_t b = 0; b < p->num_regions; ++b) {
             c->region_scratch.emplace_back();
             for (int i = 0; i < gplk::kGatherMax; ++i) c->region_scratch.back().push_back(c->dalloc<float2>(std::max(1, c->rg[b].n)));
+        }
+        c->cell_flags_prob.assign(p->cell_flags, p->cell_flags + N);
+        if (p->opts.routability) {
+            // the cells routability takes out are parked by re-sorts
+            if (!c->resort_enabled) {
+                fprintf(stderr, "gpl_ctx_create: routability needs the device re-sort (at most %d regions)\n", gplk::kMaxRegions);
+                delete c;
+                return GPL_ERR_REFUSED;
+            }
+            c->route_setup(p);
         }
         check(hipStreamSynchronize(c->stream), "sync");
     } catch (const HipError& e) {
@@ -2802,6 +3479,14 @@ static gplk::DensitySizeDev density_size_consts(const gpl_region& R) {
     return d;
 }
 
+void gpl_ctx::update_density_size(int b) {
+    upload_sizes(b);
+    Region& r = rg[b];
+    check(gplk::launch_density_size(stream, r.R.cell_begin, r.n, cells, density_size_consts(r.R), rt.d_sizes), "density size");
+    check(hipStreamSynchronize(stream), "sync");
+    drop_graph();
+}
+
 int32_t gpl_route_inflate(gpl_ctx* c, int32_t region, const gpl_rudy_grid* g, const float* inflated_ratio, int32_t* sizes, int64_t* delta) {
     if (bad_region(c, region) || bad_grid(g) || !inflated_ratio || !sizes) return GPL_ERR_ARG;
     void* d_ratio = nullptr;
@@ -2862,6 +3547,22 @@ int32_t gpl_update_density_size(gpl_ctx* c, int32_t region, const int32_t* sizes
     }
     if (d_sizes) hipFree(d_sizes);
     return rc;
+}
+
+int32_t gpl_get_route_stats(gpl_ctx* c, gpl_route_stats* s, gpl_route_pass* passes, int32_t cap) {
+    if (!c) return GPL_ERR_ARG;
+    if (s) {
+        std::memset(s, 0, sizeof *s);
+        s->passes = int32_t(c->rt.passes.size());
+        s->route_iters = c->rt.route_iters;
+        s->final_rc = c->rt.final_rc;
+        s->min_rc = c->rt.min_rc;
+        for (int64_t a : c->rt.accumulated) s->total_inflation += a;
+        s->original_area = c->rt.original_movable_area;
+    }
+    const int32_t n = int32_t(c->rt.passes.size());
+    for (int32_t i = 0; passes && i < n && i < cap; ++i) passes[i] = c->rt.passes[size_t(i)];
+    return n;
 }
 
 int32_t gpl_get_density_boxes(gpl_ctx* c, int32_t region, int32_t* dbox, float* density_scale) {

@@ -266,6 +266,63 @@ __global__ void k_density_size(int begin, int n, CellsDev c, DensitySizeDev ds, 
     }
 }
 
+__global__ void k_park(int k, const int32_t* pos, const int32_t* idx, ParkSlots v, float2* store, int stride, int restore) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
+        const int p = pos[i];
+        const size_t q = (size_t)idx[i];
+        for (int s = 0; s < v.count; ++s) {
+            if (restore)
+                v.v[s][p] = store[(size_t)s * stride + q];
+            else
+                store[(size_t)s * stride + q] = v.v[s][p];
+        }
+    }
+}
+
+__global__ void k_flag_list(int k, const int32_t* pos, uint8_t* flags, uint8_t set, uint8_t clear) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
+        const int p = pos[i];
+        flags[p] = (uint8_t)((flags[p] & (uint8_t)~clear) | set);
+    }
+}
+
+__global__ void k_non_place(int m, const long long* raw, const long long* uni, float td, long long* np, float2* npf) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < m; b += gridDim.x * blockDim.x) {
+        const long long r = raw[b], u = uni[b];
+        long long v = r != 0 ? (long long)((float)r * td) : 0;
+        long long vu = 0;
+        if (u >= 0) {
+            vu = (long long)((float)u * td);
+            const long long cap = (long long)((float)u * td * 1.10f);
+            if (v > cap) v = cap;
+        }
+        np[b] = v;
+        npf[b] = make_float2((float)v, (float)vu);
+    }
+}
+
+__global__ void k_net_congested_demand(int E, const NetRec* rec, const int32_t* net_orig, RudyGrid g, const float* ratio, float* out) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const int4 b = rec[e].box;
+        int x0, x1, y0, y1;
+        float cong;
+        float d = 0.0f;
+        if (rudy_range(g, b, x0, x1, y0, y1, cong)) {
+            for (int x = x0; x <= x1; ++x)
+                for (int y = y0; y <= y1; ++y) {
+                    if (!(ratio[(size_t)y * g.cnt_x + x] > 1.0f)) continue;
+                    const int4 r = rudy_rect(g, x, y);
+                    if (!(r.z > b.x && r.x < b.z && r.w > b.y && r.y < b.w)) continue;
+                    const long long inter = (long long)(min(b.z, r.z) - max(b.x, r.x)) * (long long)(min(b.w, r.w) - max(b.y, r.y));
+                    const long long tile_area = (long long)(r.z - r.x) * (long long)(r.w - r.y);
+                    const float rt = (float)inter / (float)tile_area;
+                    d += cong * rt * 100.0f;
+                }
+        }
+        out[net_orig[e]] = d;
+    }
+}
+
 int grid_of(unsigned long long n) { return (int)std::max<unsigned long long>(1, std::min<unsigned long long>((n + kRB - 1) / kRB, 8192)); }
 
 template <typename T>
@@ -377,6 +434,35 @@ hipError_t launch_route_inflate(hipStream_t s, int begin, int n, const CellsDev&
 
 hipError_t launch_density_size(hipStream_t s, int begin, int n, const CellsDev& c, const DensitySizeDev& d, const int2* sizes) {
     if (n > 0) hipLaunchKernelGGL(k_density_size, dim3(grid_of(n)), dim3(kRB), 0, s, begin, n, c, d, sizes);
+    return hipGetLastError();
+}
+
+hipError_t launch_park_save(hipStream_t s, int k, const int32_t* pos, const int32_t* idx, const ParkSlots& v, float2* store, int stride) {
+    if (k > 0) hipLaunchKernelGGL(k_park, dim3(grid_of(k)), dim3(kRB), 0, s, k, pos, idx, v, store, stride, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_park_restore(hipStream_t s, int k, const int32_t* pos, const int32_t* idx, const ParkSlots& v, const float2* store,
+                               int stride) {
+    if (k > 0) hipLaunchKernelGGL(k_park, dim3(grid_of(k)), dim3(kRB), 0, s, k, pos, idx, v, const_cast<float2*>(store), stride, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_flag_list(hipStream_t s, int k, const int32_t* pos, uint8_t* flags, uint8_t set, uint8_t clear) {
+    if (k > 0) hipLaunchKernelGGL(k_flag_list, dim3(grid_of(k)), dim3(kRB), 0, s, k, pos, flags, set, clear);
+    return hipGetLastError();
+}
+
+hipError_t launch_non_place(hipStream_t s, int m, const long long* raw, const long long* uni, float td, long long* non_place,
+                            float2* non_place_f) {
+    if (m > 0) hipLaunchKernelGGL(k_non_place, dim3(grid_of(m)), dim3(kRB), 0, s, m, raw, uni, td, non_place, non_place_f);
+    return hipGetLastError();
+}
+
+hipError_t launch_net_congested_demand(hipStream_t s, const NetsDev& n, const int32_t* net_orig, const RudyGrid& g, const float* ratio,
+                                       float* out) {
+    if (n.num_nets > 0)
+        hipLaunchKernelGGL(k_net_congested_demand, dim3(grid_of(n.num_nets)), dim3(kRB), 0, s, n.num_nets, n.rec, net_orig, g, ratio, out);
     return hipGetLastError();
 }
 

@@ -74,6 +74,26 @@ typedef struct gpl_opts {
     int32_t initial_place_max_solver_iter; /* 100 */
     float initial_place_net_weight_scale;  /* 800 */
     int32_t skip_initial_place;
+    /* Routability-driven placement (-routability_driven: Opts.routability, routability_end_overflow
+     * and routability_snapshot_overflow, nesterov.rs:50-54, 77-78; RouteVars, route_base.rs:12-46). */
+    int32_t routability;                      /* 0: off */
+    float routability_end_overflow;           /* -routability_check_overflow, 0.3 */
+    float routability_snapshot_overflow;      /* -routability_snapshot_overflow, 0.6 */
+    float routability_target_rc;              /* -routability_target_rc_metric, 1.01 */
+    float routability_inflation_ratio_coef;   /* 2 */
+    float routability_max_inflation_ratio;    /* 3 */
+    float routability_max_density;            /* 0.99 */
+    float routability_min_congestion;         /* -routability_min_congestion_for_inflation, 0.95 */
+    float routability_max_inflation_total;    /* 0.30 of the instances' area */
+    float routability_net_weight_max;         /* 1: no net reweighting */
+    float routability_congested_nets_percentage; /* 1 */
+    float routability_rc_k[4];                /* -routability_rc_coefficients, 1 1 0 0 */
+    int32_t routability_max_passes;           /* the pass count that ends routability (GPL-0091), 50 */
+    /* The RUDY grid the router supplies (RudyInputs, route_base.rs:48-55), for the synthetic
+     * designs: tiles of (die width / route_gcells) DBU from the die's lower left, route_tracks
+     * horizontal and vertical tracks per tile (two routing layers of pitch tile / route_tracks). */
+    int32_t route_gcells;                     /* tiles along x, 64 */
+    int32_t route_tracks;                     /* 8 */
 } gpl_opts;
 
 void gpl_opts_default(gpl_opts* o);
@@ -209,6 +229,12 @@ typedef struct gpl_region {
     const float* wx;                   /* cnt_x wavenumbers (nesterov.rs:351, x87 extended) */
     const float* wy;                   /* cnt_y wavenumbers (nesterov.rs:359) */
     int32_t group;                     /* -1: top level */
+    /* updateBinsNonPlaceArea's target-independent halves (nesterov.rs:1956-2018), from which a new
+     * target density rebuilds non_place and non_place_unscaled: per bin the fixed instances'
+     * summed overlap areas, and the area of their union (-1 where no fixed instance touches the
+     * bin). */
+    const int64_t* non_place_raw;      /* cnt_x*cnt_y */
+    const int64_t* non_place_union;    /* cnt_x*cnt_y */
 } gpl_region;
 
 typedef struct gpl_problem {
@@ -237,6 +263,7 @@ typedef struct gpl_problem {
     int32_t site_y;
     const int32_t* inst_box;      /* 4 per placer instance: the instance box (for placed_locations) */
     int32_t num_insts;
+    int32_t die[4];               /* the die area (the RUDY grid's, RudyInputs.die) */
 } gpl_problem;
 
 /* Builds the Nesterov placer on the host (nesterov.rs:787 init_nesterov_place: cells, pins,
@@ -449,6 +476,31 @@ int32_t gpl_update_density_size(gpl_ctx* c, int32_t region, const int32_t* sizes
 /* The region's density boxes (4 per cell) and density scales, the problem's order (either may be
  * NULL). */
 int32_t gpl_get_density_boxes(gpl_ctx* c, int32_t region, int32_t* dbox, float* density_scale);
+
+/* Routability-driven placement's record (opts.routability; the loop's routability hooks of
+ * resume_nesterov_place, nesterov.rs:4509-4567, and RouteBase::routability, 3985-4093): one row
+ * per routability pass, and the totals. */
+typedef struct gpl_route_pass {
+    int32_t iter;             /* the iteration after which the pass ran */
+    float rc;                 /* the measured weighted congestion (RouteBase::measure) */
+    int32_t overflowed_tiles, actionable_tiles;
+    int64_t inflation;        /* the area the pass's cell inflation added, over the regions */
+    float target_density;     /* region 0's after the pass */
+    int32_t fillers;          /* region 0's filler count after the pass */
+    int32_t weighted_nets;    /* nets of weight above 1 after the pass */
+    int32_t need, revert;     /* routability's (is_routability_need, is_revert_init_needed) */
+} gpl_route_pass;
+typedef struct gpl_route_stats {
+    int32_t passes;           /* routability passes run */
+    int32_t route_iters;      /* iterations counted as routability iterations (GPL-1017) */
+    float final_rc;           /* the convergence's closing measurement (GPL-1005); 0 before */
+    float min_rc;             /* the lowest congestion a pass kept (1e30 before the first pass) */
+    int64_t total_inflation;  /* RouteBase::total_inflation */
+    int64_t original_area;    /* the instances' area when the loop began (GPL-1011) */
+} gpl_route_stats;
+/* The routability record so far: the totals into *s (may be NULL), up to cap pass rows into
+ * passes; returns the number of passes. */
+int32_t gpl_get_route_stats(gpl_ctx* c, gpl_route_stats* s, gpl_route_pass* passes, int32_t cap);
 
 /* The density map of gpl_density_scatter on arrays the caller keeps in device memory, with no
  * allocation (no reference counterpart: it serves callers that cannot call hipMalloc -- code under

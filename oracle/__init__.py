@@ -94,6 +94,11 @@ def lib():
             ("orc_set_size", None, [i32p, C.c_int32, C.c_int32, i32p]),
             ("orc_route_inflate", C.c_int64, [C.c_int32, i32p, i32p, u8p, C.c_int32, C.c_int32, C.c_int32, f32p]),
             ("orc_density_size", None, [C.c_int32, i32p, C.c_double, C.c_double, i32p, f32p]),
+            ("orc_route_stats", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
+            ("orc_non_place_at", None, [C.c_int32, i64p, i64p, C.c_float, i64p, i64p]),
+            ("orc_cut_fillers_plan", C.c_int32, [C.c_int64, C.c_int64, C.c_int32, C.c_int64, C.c_int64, f32p]),
+            ("orc_budget_trim", C.c_int32, [C.c_double, C.c_double, C.c_double, C.c_int32, f32p]),
+            ("orc_congested_net_weights", C.c_int32, [C.c_int32, f32p, C.c_float, C.c_float, f32p]),
         ]
         for name, res, args in sig:
             fn = getattr(L, name)
@@ -152,6 +157,17 @@ class Oracle:
         s = Stats()
         lib().orc_stats(self._h, C.byref(s))
         return s
+
+    def route_stats(self):
+        """The routability record: (totals dict, one dict per routability pass), as
+        NesterovGPU.route_stats."""
+        from gpl_amd._ffi import RoutePass, RouteStats
+        st = RouteStats()
+        n = lib().orc_route_stats(self._h, C.byref(st), None, 0)
+        rows = (RoutePass * max(n, 1))()
+        lib().orc_route_stats(self._h, None, rows, n)
+        passes = [{k: getattr(r, k) for k, _ in RoutePass._fields_} for r in rows[:n]]
+        return {k: getattr(st, k) for k, _ in RouteStats._fields_}, passes
 
     def history(self):
         from gpl_amd._ffi import IterRow
@@ -454,4 +470,37 @@ def density_size(sizes, bin_x, bin_y):
     ds, sc = np.zeros(2 * n, np.int32), np.zeros(n, np.float32)
     lib().orc_density_size(n, i32(s), bin_x, bin_y, i32(ds), f32(sc))
     return ds.reshape(-1, 2), sc
+
+
+# ---- routability passes' host arithmetic (oracle.c) ----------------------------------------------
+def non_place_at(raw, uni, td):
+    """updateBinsNonPlaceArea at target density td from the raw overlap sums and union areas:
+    (non_place, non_place_unscaled)."""
+    r = np.ascontiguousarray(raw, np.int64)
+    u = np.ascontiguousarray(uni, np.int64)
+    a, b = np.zeros(len(r), np.int64), np.zeros(len(r), np.int64)
+    lib().orc_non_place_at(len(r), i64(r), i64(u), C.c_float(td), i64(a), i64(b))
+    return a, b
+
+
+def cut_fillers_plan(area, single, fillers, insts_area, white_space, td):
+    """cutFillerCells' arithmetic: (fillers taken out, target density afterwards)."""
+    t = np.array([td], np.float32)
+    k = lib().orc_cut_fillers_plan(area, single, fillers, insts_area, white_space, f32(t))
+    return int(k), float(t[0])
+
+
+def budget_trim(budget, spent, wanted, ratios):
+    """scaleInflationToBudget's trim: (budget spent, the trimmed ratios)."""
+    r = np.ascontiguousarray(ratios, np.float32).copy()
+    spent_flag = lib().orc_budget_trim(budget, spent, wanted, len(r), f32(r))
+    return bool(spent_flag), r
+
+
+def congested_net_weights(demand, pct, wmax):
+    """weightCongestedNets' weights: (count of nets with demand or -1, weight per net, 0 = none)."""
+    d = np.ascontiguousarray(demand, np.float32)
+    w = np.zeros(len(d), np.float32)
+    n = lib().orc_congested_net_weights(len(d), f32(d), C.c_float(pct), C.c_float(wmax), f32(w))
+    return int(n), w
 

@@ -193,6 +193,16 @@ enum { S_CUR_SLP, S_CUR_SLP_WL, S_CUR_SLP_DEN, S_CUR_SLP_SUM, S_NEXT_SLP, S_NEXT
 typedef struct {
     gpl_region R;
     int32_t n, begin;
+    int32_t* cells; /* the region's live cells (problem indices) in the reference's order: its
+                     * instances, then its fillers (routability takes them from the back) */
+    int32_t n_alloc, fillers;
+    int64_t total_filler_area;
+    /* the fillers routability took out since the last minimum, in removal order, each with its 18
+     * per-cell values (RemovedFiller, nesterov.rs:552-571) */
+    int32_t nrem;
+    int32_t* rem_cell;
+    fpt* rem_vals;
+    int64_t *non_place_raw, *non_place_union;
     int64_t *non_place, *non_place_unscaled, *inst, *filler;
     int32_t *bin_x, *bin_y;
     float *wx, *wy;
@@ -256,6 +266,25 @@ typedef struct orc_np {
      * GPU's transforms differ by, to measure how far the loop's outcome moves under them */
     float field_noise;
     uint64_t noise_state;
+    /* routability (opts.routability): RouteBase (route_base.rs:279-326) and the placer's state */
+    struct {
+        int32_t on, die[4], cnt_x, cnt_y, tile, wire_width;
+        float vars[8]; /* min congestion, coefficient, max ratio, ignore-edge ratio, rc_k[4] */
+        int32_t revert_count, min_rc_violated;
+        float min_rc;
+        float *min_rc_target_density;
+        int64_t *inflated_delta, *accumulated, *min_rc_inflated;
+        int64_t original_movable_area;
+        float *inflation_ratio, *inflated_ratio, *demand, *blockage; /* per tile */
+        uint8_t* actionable;
+        float *net_weight, *min_rc_net_weight, *net_demand; /* per net; 0: not in the map */
+        int32_t *min_rc_dxy;                                /* per cell */
+        int32_t need, settle_start, saved, attempts, route_iters, reverts, weighted;
+        float coef_x, coef_y, a, final_rc;
+        int32_t last_overflowed, last_actionable;
+        gpl_route_pass* passes;
+        int32_t npasses, cap_passes;
+    } rt;
 } orc_np;
 
 static void* xcalloc(size_t n, size_t s) {
@@ -266,6 +295,10 @@ static void* xcalloc(size_t n, size_t s) {
     }
     return p;
 }
+
+static void route_setup(orc_np* np, const gpl_problem* p);
+static void route_free(orc_np* np);
+static int try_routability_diverge_recovery(orc_np* np, float* cur_a);
 
 orc_np* orc_create(const gpl_problem* p) {
     orc_np* np = (orc_np*)xcalloc(1, sizeof(orc_np));
@@ -338,6 +371,22 @@ orc_np* orc_create(const gpl_problem* p) {
         nb->R = p->regions[b];
         nb->begin = nb->R.cell_begin;
         nb->n = nb->R.cell_end - nb->R.cell_begin;
+        nb->n_alloc = nb->n;
+        nb->cells = (int32_t*)xcalloc((size_t)nb->n, 4);
+        for (int32_t i = 0; i < nb->n; ++i) nb->cells[i] = nb->begin + i;
+        nb->fillers = nb->R.num_fillers;
+        nb->total_filler_area = nb->R.total_filler_area;
+        nb->rem_cell = (int32_t*)xcalloc((size_t)nb->R.num_fillers, 4);
+        nb->rem_vals = (fpt*)xcalloc((size_t)nb->R.num_fillers * S_COUNT, sizeof(fpt));
+        {
+            size_t M0 = (size_t)nb->R.cnt_x * nb->R.cnt_y;
+            nb->non_place_raw = (int64_t*)xcalloc(M0, 8);
+            nb->non_place_union = (int64_t*)xcalloc(M0, 8);
+            for (size_t k = 0; k < M0; ++k) {
+                nb->non_place_raw[k] = nb->R.non_place_raw ? nb->R.non_place_raw[k] : 0;
+                nb->non_place_union[k] = nb->R.non_place_union ? nb->R.non_place_union[k] : -1;
+            }
+        }
         size_t M = (size_t)nb->R.cnt_x * nb->R.cnt_y;
         nb->non_place = (int64_t*)xcalloc(M, 8);
         nb->non_place_unscaled = (int64_t*)xcalloc(M, 8);
@@ -374,6 +423,7 @@ orc_np* orc_create(const gpl_problem* p) {
     np->inject_diverge_at = -1;
     np->inject_diverge_at2 = -1;
     np->iter_now = -1;
+    if (np->o.routability) route_setup(np, p);
     return np;
 }
 
@@ -394,6 +444,11 @@ void orc_destroy(orc_np* np) {
         free(nb->field_y);
         free(nb->phi);
         free(nb->fft_a);
+        free(nb->cells);
+        free(nb->rem_cell);
+        free(nb->rem_vals);
+        free(nb->non_place_raw);
+        free(nb->non_place_union);
         for (int s = 0; s < S_COUNT; ++s) free(nb->v[s]);
     }
     free(np->nbs);
@@ -408,6 +463,7 @@ void orc_destroy(orc_np* np) {
     free(np->nsum);
     free(np->hist);
     free(np->rrows);
+    route_free(np);
     free(np);
 }
 
@@ -488,7 +544,7 @@ static void update_bins_density_area(orc_np* np, orc_nb* nb) {
     int cnt_x = nb->R.cnt_x;
     float td = nb->R.target_density;
     for (int i = 0; i < nb->n; ++i) {
-        int g = nb->begin + i, x0, x1, y0, y1;
+        int g = nb->cells[i], x0, x1, y0, y1;
         didx_x(nb, np->dlx[g], np->dux[g], &x0, &x1);
         didx_y(nb, np->dly[g], np->duy[g], &y0, &y1);
         int inst = np->flags[g] & GPL_CELL_INSTANCE, macro = inst && (np->flags[g] & GPL_CELL_MACRO);
@@ -523,7 +579,7 @@ static void update_bins_density_area(orc_np* np, orc_nb* nb) {
 
 /* nesterov.rs:2268 updateGCellDensityCenterLocation then the bins. */
 static void update_center_location(orc_np* np, orc_nb* nb, const fpt* c) {
-    for (int i = 0; i < nb->n; ++i) set_density_center(np, nb->begin + i, (int)c[i].x, (int)c[i].y);
+    for (int i = 0; i < nb->n; ++i) set_density_center(np, nb->cells[i], (int)c[i].x, (int)c[i].y);
     update_bins_density_area(np, nb);
 }
 
@@ -730,10 +786,19 @@ static void update_gradients(orc_np* np, orc_nb* nb, int slot) {
     if (nb->is_converged) return;
     int base = slot == SLOT_PREV ? S_PREV_SLP : slot == SLOT_CUR ? S_CUR_SLP : S_NEXT_SLP;
     fpt *wl = nb->v[base + 1], *den = nb->v[base + 2], *sum = nb->v[base + 3];
-    for (int i = 0; i < nb->n; ++i) wl[i] = wl_gradient(np, nb->begin + i);
-    for (int i = 0; i < nb->n; ++i) den[i] = density_gradient(np, nb, nb->begin + i);
-    orc_precondition(nb->n, (const float*)wl, (const float*)den, np->cnpins + nb->begin, np->dxy + 2 * (size_t)nb->begin, nb->density_penalty,
-                     (float*)sum, &nb->wl_grad_sum, &nb->density_grad_sum);
+    for (int i = 0; i < nb->n; ++i) wl[i] = wl_gradient(np, nb->cells[i]);
+    for (int i = 0; i < nb->n; ++i) den[i] = density_gradient(np, nb, nb->cells[i]);
+    int32_t* pins = (int32_t*)xcalloc((size_t)nb->n, 4);
+    int32_t* dxy = (int32_t*)xcalloc(2 * (size_t)nb->n, 4);
+    for (int i = 0; i < nb->n; ++i) {
+        pins[i] = np->cnpins[nb->cells[i]];
+        dxy[2 * i] = np->dxy[2 * (size_t)nb->cells[i]];
+        dxy[2 * i + 1] = np->dxy[2 * (size_t)nb->cells[i] + 1];
+    }
+    orc_precondition(nb->n, (const float*)wl, (const float*)den, pins, dxy, nb->density_penalty, (float*)sum, &nb->wl_grad_sum,
+                     &nb->density_grad_sum);
+    free(pins);
+    free(dxy);
 }
 
 static int bad_f(float v) { return isnan(v) || isinf(v); }
@@ -773,7 +838,7 @@ static void update_wirelength_coef(orc_np* np, float overflow) {
 static void init_density1(orc_np* np, int b) {
     orc_nb* nb = &np->nbs[b];
     for (int i = 0; i < nb->n; ++i) {
-        int g = nb->begin + i;
+        int g = nb->cells[i];
         /* updateDensityCoordiLayoutInside (nesterov.rs:2173) */
         float tlx = (float)np->dlx[g] > (float)nb->R.lx ? (float)np->dlx[g] : (float)nb->R.lx;
         float tly = (float)np->dly[g] > (float)nb->R.ly ? (float)np->dly[g] : (float)nb->R.ly;
@@ -800,7 +865,7 @@ static void update_initial_prev_slp(orc_np* np, int b) {
     orc_nb* nb = &np->nbs[b];
     float c = np->initial_prev_coordi_update_coef;
     for (int i = 0; i < nb->n; ++i) {
-        int g = nb->begin + i;
+        int g = nb->cells[i];
         if (np->flags[g] & GPL_CELL_LOCKED) {
             nb->v[S_PREV_SLP][i] = nb->v[S_CUR_SLP][i];
             continue;
@@ -925,7 +990,7 @@ static void nesterov_update_coordinates(orc_np* np, int b, float coeff) {
     uint8_t* locked = (uint8_t*)xcalloc(nb->n, 1);
     int32_t* dd = (int32_t*)xcalloc(2 * (size_t)nb->n, 4);
     for (int k = 0; k < nb->n; ++k) {
-        int g = nb->begin + k;
+        int g = nb->cells[k];
         locked[k] = (np->flags[g] & GPL_CELL_LOCKED) ? 1 : 0;
         dd[2 * k] = np->dux[g] - np->dlx[g];
         dd[2 * k + 1] = np->duy[g] - np->dly[g];
@@ -1026,7 +1091,7 @@ static void nb_update_next_iter(orc_np* np, int b, int iter) {
     if (nb->is_converged) return;
     for (int q = 0; q < 4; ++q) swap_slots(nb, S_PREV_SLP + q, S_CUR_SLP + q);
     for (int k = 0; k < nb->n; ++k) {
-        int g = nb->begin + k;
+        int g = nb->cells[k];
         if ((np->flags[g] & GPL_CELL_INSTANCE) && (np->flags[g] & GPL_CELL_LOCKED)) {
             for (int q = 0; q < 4; ++q) nb->v[S_NEXT_SLP + q][k] = nb->v[S_CUR_SLP + q][k];
             nb->v[S_NEXT][k] = nb->v[S_CUR][k];
@@ -1145,11 +1210,13 @@ static void revert_to_diverge_snapshot(orc_np* np) {
     }
 }
 
-/* nesterov.rs:3009 isDiverged (routability off): 0 none, 1 resume, 2 stop. */
+/* nesterov.rs:3009 isDiverged: 0 none, 1 resume, 2 stop.  The min-HPWL snapshot only once
+ * routability is off; a divergence tries the routability snapshot first. */
 static int np_is_diverged(orc_np* np, float* cur_a) {
     np->num_region_diverged = 0;
     for (int b = 0; b < np->nregions; ++b) np->num_region_diverged += check_divergence(np, &np->nbs[b]);
-    if (np->num_region_diverged == 0 && np->is_min_hpwl && !np->o.disable_revert_if_diverge) {
+    const int routability_off = !np->rt.on || !np->rt.need;
+    if (np->num_region_diverged == 0 && np->is_min_hpwl && !np->o.disable_revert_if_diverge && routability_off) {
         np->diverge_coef_x = np->wl_coef_x;
         np->diverge_coef_y = np->wl_coef_y;
         np->diverge_hpwl = np->min_hpwl;
@@ -1157,6 +1224,10 @@ static int np_is_diverged(orc_np* np, float* cur_a) {
         np->diverge_saved = 1;
     }
     if (np->num_region_diverged == 0) return 0;
+    if (!np->o.disable_revert_if_diverge && np->rt.on && try_routability_diverge_recovery(np, cur_a)) {
+        np->num_region_diverged = 0;
+        return 1;
+    }
     switch (orc_diverge_fallback(np->diverge_saved, np->diverge_reverts, np->o.disable_revert_if_diverge)) {
     case 0:
         np->diverge_reverts += 1;
@@ -1175,13 +1246,585 @@ static int np_is_diverged(orc_np* np, float* cur_a) {
     }
 }
 
+/* ---- routability-driven placement (opts.routability) ------------------------------------------
+ * RouteBase::routability (nesterov.rs:3985-4093) with its helpers (cutFillerCells 3826-3870,
+ * restoreRemovedFillers 3873-3891, revertToMinCongestion 3896-3918, applyNetWeights 3922-3937,
+ * scaleInflationToBudget 3946-3978, setTargetDensity 3793-3801, updateAreas 3804-3822), RouteBase's
+ * measurement (route_base.rs:443-536 through route.c) and net weighting (381-426), and the loop's
+ * routability hooks (nesterov.rs:4509-4567, 3009-3090, 3132-3140).  The reference's cell order
+ * throughout: a removed filler leaves from the back of its region's cells with its per-cell values
+ * and comes back at the end in removal order. */
+int32_t orc_rudy_wire_width(const int32_t* pitches, int32_t n);
+void orc_rudy(const int32_t die[4], int32_t cnt_x, int32_t cnt_y, int32_t tile, int32_t wire_width, int32_t nnets, const int32_t* boxes,
+              const uint8_t* cap_red, float* demand, float* blockage);
+void orc_rudy_tiles(int32_t cnt_x, int32_t cnt_y, int32_t tile, const float* demand, const float* blockage, int32_t ncentres,
+                    const int32_t* centres, const float* vars, float* inflation_ratio, uint8_t* actionable, double* stats);
+void orc_set_size(const int32_t* box, int32_t dx, int32_t dy, int32_t* out);
+void orc_rudy_rect(const int32_t die[4], int32_t cnt_x, int32_t cnt_y, int32_t tile, int32_t x, int32_t y, int32_t* r);
+
+static int is_std(const orc_np* np, int g) { return (np->flags[g] & GPL_CELL_INSTANCE) && !(np->flags[g] & GPL_CELL_MACRO); }
+
+static void route_setup(orc_np* np, const gpl_problem* p) {
+    const orc_nb* nb0 = &np->nbs[0];
+    int32_t* die = np->rt.die;
+    memcpy(die, p->die, sizeof(int32_t) * 4);
+    if (die[2] <= die[0] || die[3] <= die[1]) { /* region 0's box when the problem names no die */
+        die[0] = nb0->R.lx;
+        die[1] = nb0->R.ly;
+        die[2] = nb0->R.ux;
+        die[3] = nb0->R.uy;
+    }
+    const int32_t gcells = np->o.route_gcells > 0 ? np->o.route_gcells : 64;
+    int32_t tile = (die[2] - die[0]) / gcells;
+    if (tile < 1) tile = 1;
+    int32_t pitch = tile / (np->o.route_tracks > 0 ? np->o.route_tracks : 8);
+    if (pitch < 1) pitch = 1;
+    const int32_t pitches[2] = {pitch, pitch};
+    np->rt.on = 1;
+    np->rt.cnt_x = gcells;
+    np->rt.cnt_y = (die[3] - die[1]) / tile > 0 ? (die[3] - die[1]) / tile : 1;
+    np->rt.tile = tile;
+    np->rt.wire_width = orc_rudy_wire_width(pitches, 2);
+    const float v[8] = {np->o.routability_min_congestion, np->o.routability_inflation_ratio_coef, np->o.routability_max_inflation_ratio, 0.8f,
+                        np->o.routability_rc_k[0], np->o.routability_rc_k[1], np->o.routability_rc_k[2], np->o.routability_rc_k[3]};
+    memcpy(np->rt.vars, v, sizeof v);
+    np->rt.min_rc = 1e30f;
+    int R = np->nregions;
+    np->rt.min_rc_target_density = (float*)xcalloc(R, 4);
+    np->rt.inflated_delta = (int64_t*)xcalloc(R, 8);
+    np->rt.accumulated = (int64_t*)xcalloc(R, 8);
+    np->rt.min_rc_inflated = (int64_t*)xcalloc(R, 8);
+    for (int b = 0; b < R; ++b) np->rt.original_movable_area += nesterov_insts_area(&np->nbs[b]);
+    size_t T = (size_t)np->rt.cnt_x * np->rt.cnt_y;
+    np->rt.inflation_ratio = (float*)xcalloc(T, 4);
+    np->rt.inflated_ratio = (float*)xcalloc(T, 4);
+    np->rt.demand = (float*)xcalloc(T, 4);
+    np->rt.blockage = (float*)xcalloc(T, 4);
+    np->rt.actionable = (uint8_t*)xcalloc(T, 1);
+    np->rt.net_weight = (float*)xcalloc(np->nnets, 4);
+    np->rt.min_rc_net_weight = (float*)xcalloc(np->nnets, 4);
+    np->rt.net_demand = (float*)xcalloc(np->nnets, 4);
+    np->rt.min_rc_dxy = (int32_t*)xcalloc(2 * (size_t)np->ncells, 4);
+    np->rt.need = 1;
+    np->rt.settle_start = -1;
+    np->rt.a = 1.0f;
+}
+
+static void route_free(orc_np* np) {
+    void* f[] = {np->rt.min_rc_target_density, np->rt.inflated_delta, np->rt.accumulated, np->rt.min_rc_inflated, np->rt.inflation_ratio,
+                 np->rt.inflated_ratio, np->rt.demand, np->rt.blockage, np->rt.actionable, np->rt.net_weight, np->rt.min_rc_net_weight,
+                 np->rt.net_demand, np->rt.min_rc_dxy, np->rt.passes};
+    for (size_t i = 0; i < sizeof f / sizeof f[0]; ++i) free(f[i]);
+}
+
+/* updateBinsNonPlaceArea's last steps (nesterov.rs:1990-2017) at target density td from the
+ * target-independent halves: a bin's non_place is its fixed instances' summed overlap (raw) times
+ * td, capped at its union times td times 1.10 where a fixed instance touches it (uni >= 0), and
+ * non_place_unscaled its union times td -- float products truncated to int64 as the reference's
+ * `as i64`.  Pinned by the construction's arrays at the initial target density (every bin of the
+ * 1M-cell design's macro map) and hand-computed bins (tests/test_oracle_route.py). */
+void orc_non_place_at(int32_t m, const int64_t* raw, const int64_t* uni, float td, int64_t* non_place, int64_t* non_place_unscaled) {
+    for (int32_t k = 0; k < m; ++k) {
+        const int64_t r = raw[k], u = uni[k];
+        int64_t v = r != 0 ? (int64_t)((float)r * td) : 0, vu = 0;
+        if (u >= 0) {
+            vu = (int64_t)((float)u * td);
+            const int64_t cap = (int64_t)((float)u * td * 1.10f);
+            if (v > cap) v = cap;
+        }
+        non_place[k] = v;
+        non_place_unscaled[k] = vu;
+    }
+}
+
+/* setTargetDensity: the region's and its bins' target density, the fixed area again at it. */
+static void set_target_density(orc_nb* nb, float td) {
+    nb->R.target_density = td;
+    orc_non_place_at(nb->R.cnt_x * nb->R.cnt_y, nb->non_place_raw, nb->non_place_union, td, nb->non_place, nb->non_place_unscaled);
+}
+
+/* updateAreas: the standard and macro instance areas from the cells' sizes now. */
+static void update_areas(orc_np* np, orc_nb* nb) {
+    int64_t st = 0, mac = 0;
+    for (int i = 0; i < nb->n; ++i) {
+        const int g = nb->cells[i];
+        if (!(np->flags[g] & GPL_CELL_INSTANCE)) continue;
+        const int64_t a = (int64_t)(np->ux[g] - np->lx[g]) * (int64_t)(np->uy[g] - np->ly[g]);
+        if (np->flags[g] & GPL_CELL_MACRO)
+            mac += a;
+        else
+            st += a;
+    }
+    nb->R.std_insts_area = st;
+    nb->R.macro_insts_area = mac;
+}
+
+/* updateDensitySize (nesterov.rs:2022-2036) of the region's cells, the density box rebuilt around
+ * its centre (set_density_size, 171-177). */
+static void update_density_size_nb(orc_np* np, orc_nb* nb) {
+    const long double s2 = 1.414213562373095048801L;
+    const long double thr_x = s2 * (long double)nb->R.size_x, thr_y = s2 * (long double)nb->R.size_y;
+    const float size_x = (float)(s2 * (long double)(float)nb->R.size_x), size_y = (float)(s2 * (long double)(float)nb->R.size_y);
+    for (int i = 0; i < nb->n; ++i) {
+        const int g = nb->cells[i];
+        const int32_t dx = np->ux[g] - np->lx[g], dy = np->uy[g] - np->ly[g];
+        float sx = 1.0f, sy = 1.0f, dsx = (float)dx, dsy = (float)dy;
+        if ((long double)(uint64_t)dx < thr_x) {
+            sx = (float)dx / (float)thr_x;
+            dsx = size_x;
+        }
+        if ((long double)(uint64_t)dy < thr_y) {
+            sy = (float)dy / (float)thr_y;
+            dsy = size_y;
+        }
+        const int32_t ddx = (int32_t)dsx, ddy = (int32_t)dsy, cx = d_cx(np, g), cy = d_cy(np, g);
+        np->dlx[g] = cx - ddx / 2;
+        np->dly[g] = cy - ddy / 2;
+        np->dux[g] = cx + ddx / 2;
+        np->duy[g] = cy + ddy / 2;
+        np->dscale[g] = sx * sy;
+    }
+}
+
+/* cutFillerCells: whole fillers from the back of the region's cells for the inflation's area
+ * (each with its values, swapAndPop of the last), the target density raised for the rest. */
+/* cutFillerCells' arithmetic (nesterov.rs:3834-3866): how many whole fillers of area `single` an
+ * inflation of `area` takes out of `fillers`, and the target density when more than a filler's
+ * area is left over -- (instance area + removed + fillers left + remainder) / white space, in
+ * float; *td is left as it is otherwise.  Returns the count.  Pinned by hand-computed cases
+ * (tests/test_oracle_route.py). */
+int32_t orc_cut_fillers_plan(int64_t area, int64_t single, int32_t fillers, int64_t insts_area, int64_t white_space, float* td) {
+    if (area < 0) return 0;
+    int64_t max_remove = single > 0 ? area / single : 0;
+    if (max_remove > fillers) max_remove = fillers;
+    const int64_t left = single * ((int64_t)fillers - max_remove), removed_area = single * max_remove, remaining = area - removed_area;
+    if (remaining > single) *td = (float)(insts_area + removed_area + left + remaining) / (float)white_space;
+    return (int32_t)max_remove;
+}
+
+static void cut_filler_cells(orc_np* np, orc_nb* nb, int64_t area) {
+    if (area < 0) return; /* GPL-0313 */
+    const int64_t single = (int64_t)nb->R.filler_dx * nb->R.filler_dy;
+    float td = nb->R.target_density;
+    const int64_t max_remove = orc_cut_fillers_plan(area, single, nb->fillers, nesterov_insts_area(nb), nb->R.white_space_area, &td);
+    int64_t available = single * nb->fillers, removed = 0;
+    for (int i = nb->n - 1; i >= 0 && removed < max_remove; --i) {
+        const int g = nb->cells[i];
+        if (np->flags[g] & GPL_CELL_INSTANCE) continue;
+        /* swap_pop(i): the last cell moves to i (i is the last here: the back holds the fillers) */
+        const int last = nb->n - 1;
+        nb->rem_cell[nb->nrem] = g;
+        for (int s = 0; s < S_COUNT; ++s) nb->rem_vals[(size_t)nb->nrem * S_COUNT + s] = nb->v[s][i];
+        nb->nrem++;
+        nb->cells[i] = nb->cells[last];
+        for (int s = 0; s < S_COUNT; ++s) nb->v[s][i] = nb->v[s][last];
+        nb->n = last;
+        nb->fillers--;
+        available -= single;
+        removed++;
+    }
+    nb->total_filler_area = available;
+    if (area - single * removed > single) {
+        set_target_density(nb, td);
+        nb->R.movable_area = (int64_t)((float)nb->R.white_space_area * nb->R.target_density);
+    }
+}
+
+/* restoreRemovedFillers: each filler taken out since the last minimum back at the end, in the
+ * order taken, with its values. */
+static void restore_removed_fillers(orc_nb* nb) {
+    const int64_t single = (int64_t)nb->R.filler_dx * nb->R.filler_dy;
+    for (int k = 0; k < nb->nrem; ++k) {
+        nb->cells[nb->n] = nb->rem_cell[k];
+        for (int s = 0; s < S_COUNT; ++s) nb->v[s][nb->n] = nb->rem_vals[(size_t)k * S_COUNT + s];
+        nb->n++;
+        nb->fillers++;
+        nb->total_filler_area += single;
+    }
+    nb->nrem = 0;
+}
+
+/* applyNetWeights: each net's weight from the map, at least 1; the count above 1. */
+static int apply_net_weights(orc_np* np) {
+    int w = 0;
+    for (int e = 0; e < np->nnets; ++e) {
+        np->nw[e] = np->rt.net_weight[e] > 1.0f ? np->rt.net_weight[e] : 1.0f;
+        if (np->nw[e] > 1.0f) ++w;
+    }
+    np->rt.weighted = w;
+    return w;
+}
+
+/* revertToMinCongestion. */
+static void revert_to_min_congestion(orc_np* np) {
+    if (np->rt.revert_count < 1) return;
+    for (int g = 0; g < np->ncells; ++g) {
+        if (!is_std(np, g)) continue;
+        int32_t box[4] = {np->lx[g], np->ly[g], np->ux[g], np->uy[g]}, nbx[4];
+        orc_set_size(box, np->rt.min_rc_dxy[2 * (size_t)g], np->rt.min_rc_dxy[2 * (size_t)g + 1], nbx);
+        np->lx[g] = nbx[0];
+        np->ly[g] = nbx[1];
+        np->ux[g] = nbx[2];
+        np->uy[g] = nbx[3];
+        np->dxy[2 * (size_t)g] = nbx[2] - nbx[0];
+        np->dxy[2 * (size_t)g + 1] = nbx[3] - nbx[1];
+    }
+    for (int b = 0; b < np->nregions; ++b) {
+        orc_nb* nb = &np->nbs[b];
+        set_target_density(nb, np->rt.min_rc_target_density[b]);
+        restore_removed_fillers(nb);
+        update_density_size_nb(np, nb);
+        np->rt.accumulated[b] = np->rt.min_rc_inflated[b];
+    }
+    memcpy(np->rt.net_weight, np->rt.min_rc_net_weight, sizeof(float) * np->nnets);
+    apply_net_weights(np);
+    memset(np->rt.inflated_delta, 0, sizeof(int64_t) * np->nregions);
+}
+
+/* getTile: origin (0, 0), the last column and row clamped; -1 left of or below the grid. */
+static int64_t tile_at(const orc_np* np, int32_t x, int32_t y) {
+    int32_t ix = x / np->rt.tile, iy = y / np->rt.tile;
+    if (ix > np->rt.cnt_x - 1) ix = np->rt.cnt_x - 1;
+    if (iy > np->rt.cnt_y - 1) iy = np->rt.cnt_y - 1;
+    if (ix < 0 || iy < 0) return -1;
+    return (int64_t)iy * np->rt.cnt_x + ix;
+}
+
+/* RouteBase::measure's RUDY half (calculateRudyTiles + updateRudyAverage) at the placement now,
+ * with each net's demand in the congested tiles (getNetDemandInTiles over the tiles of inflation
+ * ratio above 1, route_base.rs:200-218, 381-400). */
+static float measure_rudy(orc_np* np) {
+    for (int e = 0; e < np->nnets; ++e) update_box(np, e);
+    orc_rudy(np->rt.die, np->rt.cnt_x, np->rt.cnt_y, np->rt.tile, np->rt.wire_width, np->nnets, np->nbox, NULL, np->rt.demand, np->rt.blockage);
+    int32_t nc = 0;
+    for (int b = 0; b < np->nregions; ++b)
+        for (int i = 0; i < np->nbs[b].n; ++i) nc += is_std(np, np->nbs[b].cells[i]);
+    int32_t* centres = (int32_t*)xcalloc(2 * (size_t)(nc ? nc : 1), 4);
+    nc = 0;
+    for (int b = 0; b < np->nregions; ++b)
+        for (int i = 0; i < np->nbs[b].n; ++i) {
+            const int g = np->nbs[b].cells[i];
+            if (!is_std(np, g)) continue;
+            centres[2 * nc] = d_cx(np, g);
+            centres[2 * nc + 1] = d_cy(np, g);
+            nc++;
+        }
+    double st[8];
+    orc_rudy_tiles(np->rt.cnt_x, np->rt.cnt_y, np->rt.tile, np->rt.demand, np->rt.blockage, nc, centres, np->rt.vars, np->rt.inflation_ratio,
+                   np->rt.actionable, st);
+    free(centres);
+    np->rt.last_overflowed = (int32_t)st[1];
+    np->rt.last_actionable = (int32_t)st[2];
+    /* each net's demand in the congested tiles, in the visit's order (x outer, y inner) */
+    const int32_t cx = np->rt.cnt_x, cy = np->rt.cnt_y, tile = np->rt.tile;
+    for (int e = 0; e < np->nnets; ++e) {
+        const int32_t* bx = np->nbox + 4 * (size_t)e;
+        const int32_t xl = bx[0], yl = bx[1], xh = bx[2], yh = bx[3];
+        float d = 0.0f;
+        if (!(xl > xh || yl > yh) && (int64_t)(xh - xl) * (int64_t)(yh - yl) != 0) {
+            const int64_t net_area = (int64_t)(xh - xl) * (int64_t)(yh - yl);
+            const float cong = ((float)((xh - xl) + (yh - yl)) * (float)np->rt.wire_width) / (float)net_area;
+            int32_t x0 = (xl - np->rt.die[0]) / tile, x1 = (xh - np->rt.die[0]) / tile, y0 = (yl - np->rt.die[1]) / tile, y1 = (yh - np->rt.die[1]) / tile;
+            if (x0 < 0) x0 = 0;
+            if (y0 < 0) y0 = 0;
+            if (x1 > cx - 1) x1 = cx - 1;
+            if (y1 > cy - 1) y1 = cy - 1;
+            for (int32_t x = x0; x <= x1; ++x)
+                for (int32_t y = y0; y <= y1; ++y) {
+                    int32_t r[4];
+                    orc_rudy_rect(np->rt.die, cx, cy, tile, x, y, r);
+                    if (!(r[2] > xl && r[0] < xh && r[3] > yl && r[1] < yh)) continue;
+                    if (!(np->rt.inflation_ratio[(size_t)y * cx + x] > 1.0f)) continue;
+                    const int64_t inter = (int64_t)((xh < r[2] ? xh : r[2]) - (xl > r[0] ? xl : r[0])) *
+                                          (int64_t)((yh < r[3] ? yh : r[3]) - (yl > r[1] ? yl : r[1]));
+                    const int64_t tile_area = (int64_t)(r[2] - r[0]) * (int64_t)(r[3] - r[1]);
+                    d += cong * ((float)inter / (float)tile_area) * 100.0f;
+                }
+        }
+        np->rt.net_demand[e] = d;
+    }
+    return (float)st[7];
+}
+
+/* scaleInflationToBudget's trim (nesterov.rs:3950-3977): the pass may add a sixth of the budget or
+ * what is left of it; when the cells ask for more (`wanted`, the sum of area x (ratio - 1)) every
+ * inflated tile's excess over 1 is scaled by allowed / wanted (in double).  Returns whether the
+ * budget is then spent.  Pinned by hand-computed cases (tests/test_oracle_route.py). */
+int32_t orc_budget_trim(double budget, double spent, double wanted, int32_t ntiles, float* inflated_ratio) {
+    const double allowed = budget / 6.0 < budget - spent ? budget / 6.0 : budget - spent;
+    if (wanted <= 0.0 || wanted <= allowed) return spent + wanted >= budget;
+    const double scale = allowed / wanted > 0.0 ? allowed / wanted : 0.0;
+    for (int32_t t = 0; t < ntiles; ++t)
+        if (inflated_ratio[t] > 1.0f) inflated_ratio[t] = (float)(1.0 + scale * ((double)inflated_ratio[t] - 1.0));
+    return spent + allowed >= budget;
+}
+
+/* scaleInflationToBudget: the area this pass asks for against a sixth of the budget (or what is
+ * left); over it every inflated tile's excess scaled down; whether the budget is spent. */
+static int scale_inflation_to_budget(orc_np* np) {
+    if (np->rt.original_movable_area <= 0) return 0;
+    const double budget = (double)np->rt.original_movable_area * (double)np->o.routability_max_inflation_total;
+    int64_t tot = 0;
+    for (int b = 0; b < np->nregions; ++b) tot += np->rt.accumulated[b];
+    const double spent = (double)tot;
+    double wanted = 0.0;
+    for (int b = 0; b < np->nregions; ++b)
+        for (int i = 0; i < np->nbs[b].n; ++i) {
+            const int g = np->nbs[b].cells[i];
+            if (!is_std(np, g)) continue;
+            const int64_t t = tile_at(np, d_cx(np, g), d_cy(np, g));
+            if (t < 0) continue;
+            const float ratio = np->rt.inflated_ratio[t];
+            if ((double)ratio <= 1.0) continue;
+            const double area = (double)(np->ux[g] - np->lx[g]) * (double)(np->uy[g] - np->ly[g]);
+            wanted += area * ((double)ratio - 1.0);
+        }
+    return orc_budget_trim(budget, spent, wanted, np->rt.cnt_x * np->rt.cnt_y, np->rt.inflated_ratio);
+}
+
+/* weightCongestedNets' ranking (route_base.rs:381-426): -1 when no net was weighted. */
+typedef struct {
+    int32_t net;
+    float d;
+} net_dem;
+static int by_demand(const void* a, const void* b) {
+    const net_dem *x = (const net_dem*)a, *y = (const net_dem*)b;
+    if (x->d != y->d) return x->d > y->d ? -1 : 1;
+    return x->net < y->net ? -1 : (x->net > y->net ? 1 : 0);
+}
+/* weightCongestedNets' weights (route_base.rs:400-424) from each net's demand in the congested
+ * tiles (nnets of them; 0: none): the nets with some, ranked by demand (largest first, ties in the
+ * nets' order), the top `pct` % (at least one, ceil((float)n * pct) / 100) weighted from wmax at
+ * the worst down to 1 at the cutoff (wmax throughout when they all tie); 0 for every other net.
+ * Returns the count of nets with demand, -1 when none or when weighting is off.  Pinned by
+ * hand-computed cases (tests/test_oracle_route.py). */
+int32_t orc_congested_net_weights(int32_t nnets, const float* demand, float pct, float wmax, float* weight) {
+    memset(weight, 0, sizeof(float) * (size_t)(nnets > 0 ? nnets : 0));
+    if (wmax <= 1.0f || pct <= 0.0f || nnets == 0) return -1;
+    net_dem* d = (net_dem*)xcalloc(nnets, sizeof(net_dem));
+    size_t n = 0;
+    for (int e = 0; e < nnets; ++e)
+        if (demand[e] > 0.0f) {
+            d[n].net = e;
+            d[n].d = demand[e];
+            n++;
+        }
+    if (n == 0) {
+        free(d);
+        return -1;
+    }
+    qsort(d, n, sizeof(net_dem), by_demand);
+    const size_t share = (size_t)ceil((double)((float)n * pct) / 100.0);
+    const size_t count = n < (share > 1 ? share : 1) ? n : (share > 1 ? share : 1);
+    const float worst = d[0].d, cutoff = d[count - 1].d;
+    for (size_t i = 0; i < count; ++i)
+        weight[d[i].net] = worst > cutoff ? 1.0f + (wmax - 1.0f) * (d[i].d - cutoff) / (worst - cutoff) : wmax;
+    free(d);
+    return (int)n;
+}
+
+static int weight_congested_nets(orc_np* np) {
+    return orc_congested_net_weights(np->nnets, np->rt.net_demand, np->o.routability_congested_nets_percentage,
+                                     np->o.routability_net_weight_max, np->rt.net_weight);
+}
+
+static void push_pass(orc_np* np, gpl_route_pass* row, int need, int revert) {
+    int64_t add = 0;
+    for (int b = 0; b < np->nregions; ++b) add += np->rt.inflated_delta[b];
+    row->inflation = add;
+    row->target_density = np->nbs[0].R.target_density;
+    row->fillers = np->nbs[0].fillers;
+    row->weighted_nets = np->rt.weighted;
+    row->need = need;
+    row->revert = revert;
+    if (np->rt.npasses == np->rt.cap_passes) {
+        np->rt.cap_passes = np->rt.cap_passes ? 2 * np->rt.cap_passes : 16;
+        np->rt.passes = (gpl_route_pass*)realloc(np->rt.passes, sizeof(gpl_route_pass) * np->rt.cap_passes);
+    }
+    np->rt.passes[np->rt.npasses++] = *row;
+}
+
+/* RouteBase::routability: *need (is_routability_need), *revert (is_revert_init_needed). */
+static void routability(orc_np* np, int revert_count, int* need, int* revert) {
+    np->rt.revert_count += 1;
+    const float cur_rc = measure_rudy(np);
+    gpl_route_pass row;
+    memset(&row, 0, sizeof row);
+    row.iter = np->iter_now;
+    row.rc = cur_rc;
+    row.overflowed_tiles = np->rt.last_overflowed;
+    row.actionable_tiles = np->rt.last_actionable;
+    if (cur_rc < np->o.routability_target_rc) { /* GPL-0050 */
+        push_pass(np, &row, 0, 0);
+        memset(np->rt.inflated_delta, 0, sizeof(int64_t) * np->nregions);
+        *need = 0;
+        *revert = 0;
+        return;
+    }
+    if ((np->rt.min_rc - cur_rc) > np->rt.min_rc * 0.005f) { /* GPL-0048 */
+        np->rt.min_rc = cur_rc;
+        np->rt.min_rc_violated = 0;
+        for (int b = 0; b < np->nregions; ++b) {
+            np->rt.min_rc_target_density[b] = np->nbs[b].R.target_density;
+            np->nbs[b].nrem = 0;
+        }
+        for (int g = 0; g < np->ncells; ++g)
+            if (is_std(np, g)) {
+                np->rt.min_rc_dxy[2 * (size_t)g] = np->ux[g] - np->lx[g];
+                np->rt.min_rc_dxy[2 * (size_t)g + 1] = np->uy[g] - np->ly[g];
+            }
+        memcpy(np->rt.min_rc_inflated, np->rt.accumulated, sizeof(int64_t) * np->nregions);
+        memcpy(np->rt.min_rc_net_weight, np->rt.net_weight, sizeof(float) * np->nnets);
+    } else {
+        np->rt.min_rc_violated += 1; /* GPL-0049 */
+    }
+    const size_t T = (size_t)np->rt.cnt_x * np->rt.cnt_y;
+    for (size_t t = 0; t < T; ++t) np->rt.inflated_ratio[t] = np->rt.inflation_ratio[t] > 1.0f ? np->rt.inflation_ratio[t] : 1.0f;
+    const int budget_spent = scale_inflation_to_budget(np);
+    for (int b = 0; b < np->nregions; ++b) {
+        orc_nb* nb = &np->nbs[b];
+        np->rt.inflated_delta[b] = 0;
+        for (int i = 0; i < nb->n; ++i) {
+            const int g = nb->cells[i];
+            if (!is_std(np, g)) continue;
+            const int64_t t = tile_at(np, d_cx(np, g), d_cy(np, g));
+            if (t < 0) continue;
+            const float ratio = np->rt.inflated_ratio[t];
+            if ((double)ratio <= 1.0) continue;
+            const int64_t prev = (int64_t)(np->ux[g] - np->lx[g]) * (int64_t)(np->uy[g] - np->ly[g]);
+            const float root = sqrtf(ratio);
+            const int32_t nx = (int32_t)roundf((float)(np->ux[g] - np->lx[g]) * root), ny = (int32_t)roundf((float)(np->uy[g] - np->ly[g]) * root);
+            int32_t box[4] = {np->lx[g], np->ly[g], np->ux[g], np->uy[g]}, nbx[4];
+            orc_set_size(box, nx, ny, nbx);
+            np->lx[g] = nbx[0];
+            np->ly[g] = nbx[1];
+            np->ux[g] = nbx[2];
+            np->uy[g] = nbx[3];
+            np->dxy[2 * (size_t)g] = nbx[2] - nbx[0];
+            np->dxy[2 * (size_t)g + 1] = nbx[3] - nbx[1];
+            np->rt.inflated_delta[b] += (int64_t)(nbx[2] - nbx[0]) * (int64_t)(nbx[3] - nbx[1]) - prev;
+        }
+        np->rt.accumulated[b] += np->rt.inflated_delta[b];
+        cut_filler_cells(np, nb, np->rt.inflated_delta[b]);
+        if (nb->R.target_density > np->o.routability_max_density) { /* GPL-0053 */
+            revert_to_min_congestion(np);
+            push_pass(np, &row, 0, 1);
+            *need = 0;
+            *revert = 1;
+            return;
+        }
+    }
+    if (revert_count >= np->o.routability_max_passes || np->rt.min_rc_violated >= 3) { /* GPL-0091, GPL-0054 */
+        revert_to_min_congestion(np);
+        push_pass(np, &row, 0, 1);
+        *need = 0;
+        *revert = 1;
+        return;
+    }
+    for (int b = 0; b < np->nregions; ++b) {
+        update_areas(np, &np->nbs[b]);
+        update_density_size_nb(np, &np->nbs[b]);
+    }
+    weight_congested_nets(np);
+    apply_net_weights(np);
+    push_pass(np, &row, !budget_spent, 1);
+    memset(np->rt.inflated_delta, 0, sizeof(int64_t) * np->nregions);
+    if (budget_spent) revert_to_min_congestion(np); /* GPL-0097 */
+    *need = !budget_spent;
+    *revert = 1;
+}
+
+static int is_placement_settled(const orc_np* np) {
+    for (int b = 0; b < np->nregions; ++b) {
+        const orc_nb* nb = &np->nbs[b];
+        if (!(nb->peak_coordi_distance > 0.0f && nb->coordi_distance <= 0.6f * nb->peak_coordi_distance)) return 0;
+    }
+    return 1;
+}
+
+/* tryRoutabilityDivergeRecovery: 1 when it reverted to the routability snapshot. */
+static int try_routability_diverge_recovery(orc_np* np, float* cur_a) {
+    if (!np->rt.saved || np->rt.attempts >= 2) return 0;
+    if (!np->rt.need && (np->diverge_saved || np->diverge_reverts > 0)) return 0;
+    np->rt.attempts = np->rt.need ? np->rt.attempts + 1 : 2;
+    const int last = np->rt.attempts == 2;
+    if (np->rt.need) revert_to_min_congestion(np);
+    if (last) np->rt.need = 0;
+    np->wl_coef_x = np->rt.coef_x;
+    np->wl_coef_y = np->rt.coef_y;
+    update_wirelength_force_wa(np);
+    for (int b = 0; b < np->nregions; ++b) {
+        revert_to_snapshot(np, &np->nbs[b]);
+        np->nbs[b].min_sum_overflow = 1e30f;
+        np->nbs[b].hpwl_with_min_sum_overflow = 1e30f;
+    }
+    *cur_a = 1.0f;
+    np->rt.settle_start = -1;
+    np->min_hpwl = INT64_MAX;
+    np->is_min_hpwl = 0;
+    return 1;
+}
+
+/* The loop's routability step after the divergence test: the routability snapshot, then a pass
+ * once the overflow reached its end and the cells settled. */
+static void routability_step(orc_np* np, int iter, float* cur_a) {
+    if (!np->rt.saved && np->o.routability_snapshot_overflow >= np->average_overflow_unscaled) {
+        np->rt.coef_x = np->wl_coef_x;
+        np->rt.coef_y = np->wl_coef_y;
+        np->rt.a = *cur_a;
+        np->rt.saved = 1;
+        for (int b = 0; b < np->nregions; ++b) save_snapshot(&np->nbs[b]);
+    }
+    const int active = np->rt.need && np->average_overflow_unscaled <= np->o.routability_end_overflow;
+    const int settled = active && is_placement_settled(np);
+    if (active && !settled && np->rt.settle_start == -1)
+        np->rt.settle_start = iter;
+    else if (settled && np->rt.settle_start != -1)
+        np->rt.settle_start = -1;
+    if (!settled) return;
+    np->rt.reverts += 1;
+    int need = 0, revert = 0;
+    routability(np, np->rt.reverts, &need, &revert);
+    np->rt.need = need;
+    np->min_hpwl = INT64_MAX;
+    np->is_min_hpwl = 0;
+    if (need || revert) {
+        *cur_a = np->rt.a;
+        np->wl_coef_x = np->rt.coef_x;
+        np->wl_coef_y = np->rt.coef_y;
+        update_wirelength_force_wa(np);
+        for (int b = 0; b < np->nregions; ++b) {
+            revert_to_snapshot(np, &np->nbs[b]);
+            np->nbs[b].min_sum_overflow = 1e30f;
+            np->nbs[b].hpwl_with_min_sum_overflow = 1e30f;
+        }
+    }
+}
+
+/* The routability record: totals (passes, routability iterations, final and minimum congestion,
+ * inflation, original area) into s, the pass rows into rows (up to cap); returns the pass count. */
+int32_t orc_route_stats(orc_np* np, gpl_route_stats* s, gpl_route_pass* rows, int32_t cap) {
+    if (s) {
+        memset(s, 0, sizeof *s);
+        s->passes = np->rt.npasses;
+        s->route_iters = np->rt.route_iters;
+        s->final_rc = np->rt.final_rc;
+        s->min_rc = np->rt.min_rc;
+        for (int b = 0; b < np->nregions && np->rt.accumulated; ++b) s->total_inflation += np->rt.accumulated[b];
+        s->original_area = np->rt.original_movable_area;
+    }
+    for (int32_t i = 0; rows && i < np->rt.npasses && i < cap; ++i) rows[i] = np->rt.passes[i];
+    return np->rt.npasses;
+}
+
 /* nesterov.rs:3107 checkConvergence (report lines aside). */
 static int check_convergence(orc_np* np, int b) {
     orc_nb* nb = &np->nbs[b];
     if (nb->is_converged) return 1;
     if (nb->sum_overflow_unscaled > np->o.overflow) return 0;
+    if (np->rt.on) np->rt.final_rc = measure_rudy(np); /* GPL-1005 */
     for (int k = 0; k < nb->n; ++k) {
-        int g = nb->begin + k;
+        int g = nb->cells[k];
         if (np->flags[g] & GPL_CELL_INSTANCE) np->flags[g] |= GPL_CELL_LOCKED;
     }
     nb->is_converged = 1;
@@ -1252,6 +1895,10 @@ int32_t orc_run(orc_np* np, int32_t start_iter, int32_t iter_budget, double time
     np->diverge_saved = 0;
     np->diverge_reverts = 0;
     np->diverge_hpwl = 0;
+    np->rt.saved = 0;
+    np->rt.attempts = 0;
+    np->rt.route_iters = 0;
+    np->rt.reverts = 0;
     float cur_a = 1.0f;
     int iter = start_iter, ran = 0;
     np->converged = 0;
@@ -1269,12 +1916,18 @@ int32_t orc_run(orc_np* np, int32_t start_iter, int32_t iter_budget, double time
         push_history(np, iter);
         ran++;
         np->iter_done = ran;
+        /* is_routability_gpl_iter: each such iteration extends the limit by one */
+        if (np->rt.on && np->rt.saved && np->average_overflow_unscaled > np->o.routability_end_overflow) {
+            np->rt.route_iters += 1;
+            np->max_iter += 1;
+        }
         int act = np_is_diverged(np, &cur_a);
         if (act == 2) break;
         if (act == 1) {
             iter++;
             continue;
         }
+        if (np->rt.on) routability_step(np, iter, &cur_a);
         int conv = 0;
         for (int b = 0; b < np->nregions; ++b) conv += check_convergence(np, b);
         if (conv == np->nregions) {
@@ -1410,7 +2063,7 @@ void orc_scatter(orc_np* np, int32_t b, const float* xy, int64_t* inst, int64_t*
 /* Sets the pins of a region's cells from density centres (ints), without touching the bins. */
 void orc_place_pins(orc_np* np, int32_t b, const float* xy) {
     orc_nb* nb = &np->nbs[b];
-    for (int i = 0; i < nb->n; ++i) set_density_center(np, nb->begin + i, (int)xy[2 * i], (int)xy[2 * i + 1]);
+    for (int i = 0; i < nb->n; ++i) set_density_center(np, nb->cells[i], (int)xy[2 * i], (int)xy[2 * i + 1]);
 }
 
 /* WA forward at the current pins: faithful float net sums and boxes, FP64 net sums, HPWL. */
@@ -1452,7 +2105,7 @@ void orc_wa_forward(orc_np* np, float coef_x, float coef_y, float* sums, int32_t
 void orc_wa_backward(orc_np* np, int32_t b, float* wl, const double* sums_f64, double* wl_f64) {
     orc_nb* nb = &np->nbs[b];
     for (int i = 0; i < nb->n; ++i) {
-        int g = nb->begin + i;
+        int g = nb->cells[i];
         fpt v = wl_gradient(np, g);
         if (wl) {
             wl[2 * i] = v.x;
@@ -1487,7 +2140,7 @@ void orc_gather(orc_np* np, int32_t b, const float* field_x, const float* field_
     if (field_x) memcpy(nb->field_x, field_x, 4 * M);
     if (field_y) memcpy(nb->field_y, field_y, 4 * M);
     for (int i = 0; i < nb->n; ++i) {
-        int g = nb->begin + i;
+        int g = nb->cells[i];
         if (den) {
             fpt f = density_gradient(np, nb, g);
             den[2 * i] = f.x;

@@ -1,5 +1,6 @@
 """Writes the end-to-end loop fixtures: the oracle's full run to overflow 0.10 on the 200k-cell
-(512 x 512 bins) and 1M-cell (1024 x 1024 bins) synthetic placements, seed 1.
+(512 x 512 bins) and 1M-cell (1024 x 1024 bins) synthetic placements, seed 1, and the other
+workloads (c4r, c4s; c5 with its routability passes).
 
 The oracle is the single-thread C restatement of the reference's loop (oracle/); a 1M-cell run
 takes about eleven minutes on one core, too long for a test, so its outcome is kept here as data:
@@ -98,6 +99,11 @@ def run(name, seed=1):
               for r in hist[:10]],
         every_50th=[dict(iter=r["iter"], overflow=float(r["overflow"]), hpwl=int(r["hpwl"])) for r in hist[::50]],
     )
+    if d.problem().opts.routability:
+        # the routability-driven loop's record: totals and one row per pass
+        tot, passes = o.route_stats()
+        out["route"] = dict(stats={k: (float(v) if isinstance(v, float) else int(v)) for k, v in tot.items()},
+                            passes=[{k: (float(v) if isinstance(v, float) else int(v)) for k, v in p.items()} for p in passes])
     o.close()
     d.close()
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"loop_{name}.json")

@@ -136,12 +136,33 @@ def test_keys_match_by_prefix(kats):
 
 
 def test_unmodelled_modes_are_refused():
-    for a in (["-routability_driven"], ["-timing_driven"], ["-incremental"], ["-place_ios"]):
+    for a in (["-timing_driven"], ["-incremental"], ["-place_ios"], ["-routability_driven", "-routability_use_grt"],
+              ["-routability_driven", "-routability_rc_coefficients", "1 1 0"]):
         with pytest.raises(G.GplError) as e:
             G.opts_from_args(a)
         assert e.value.code == G.GPL_ERR_REFUSED
     o, _ = G.opts_from_args(["-bin", "128", "-den", "0.8", "-overflow", "0.2", "-random_seed", "7"])
     assert (o.bin_cnt, round(o.density, 6), round(o.overflow, 6), o.seed) == (128, 0.8, 0.2, 7)
+    assert o.routability == 0
+
+
+def test_routability_options_as_run_reads_them():
+    """run.rs:251-269: -routability_driven turns the mode on (not with -skip_io); its keys by
+    prefix; the defaults of RouteVars (route_base.rs:29-45) and Opts (nesterov.rs:77-78)."""
+    o, _ = G.opts_from_args(["-routability_driven"])
+    assert o.routability == 1
+    assert [round(v, 6) for v in (o.routability_end_overflow, o.routability_snapshot_overflow, o.routability_target_rc,
+                                  o.routability_inflation_ratio_coef, o.routability_max_inflation_ratio,
+                                  o.routability_max_density, o.routability_min_congestion, o.routability_max_inflation_total,
+                                  o.routability_net_weight_max, o.routability_congested_nets_percentage)] == \
+        [0.3, 0.6, 1.01, 2.0, 3.0, 0.99, 0.95, 0.3, 1.0, 1.0]
+    assert list(o.routability_rc_k) == [1.0, 1.0, 0.0, 0.0] and o.routability_max_passes == 50
+    o, _ = G.opts_from_args(["-routability_driven", "-routability_check", "0.25", "-routability_target_rc", "1.2",
+                             "-routability_rc_coefficients", "1 2 3 4"])
+    assert (round(o.routability_end_overflow, 6), round(o.routability_target_rc, 6)) == (0.25, 1.2)
+    assert list(o.routability_rc_k) == [1.0, 2.0, 3.0, 4.0]
+    o, _ = G.opts_from_args(["-routability_driven", "-skip_io"])
+    assert o.routability == 0
 
 
 def test_generator_is_deterministic_and_sized():

@@ -44,5 +44,5 @@ def test_job_statuses():
     assert r["status"] == "refused"
     r = run_job({"steps": [{"cmd": "read_synthetic", "args": ["c1"]}, {"cmd": "global_placement", "args": ["-bogus", "1"]}]})
     assert r["status"] == "error" and r["code"] == "STA-0562"
-    r = run_job({"steps": [{"cmd": "read_synthetic", "args": ["c1"]}, {"cmd": "global_placement", "args": ["-routability_driven"]}]})
+    r = run_job({"steps": [{"cmd": "read_synthetic", "args": ["c1"]}, {"cmd": "global_placement", "args": ["-timing_driven"]}]})
     assert r["status"] == "refused" and "not modelled" in r["reason"]

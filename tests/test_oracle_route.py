@@ -132,3 +132,76 @@ def test_density_size_is_what_init_built():
         assert (sc < 1.0).any()  # some cells spread
     finally:
         d.close()
+
+
+# ---- the routability pass's host arithmetic (RouteBase::routability and its helpers) -------------
+def test_non_place_at_rebuilds_the_construction():
+    """updateBinsNonPlaceArea (nesterov.rs:1956-2018) from its target-independent halves at the
+    initial target density gives the construction's fixed-area map bit for bit (a 20k-cell design
+    with 5 % of its core in macros: the bivariate-normal overlaps and the unions; the 1.10 cap is
+    pinned by the hand-computed bins)."""
+    import gpl_amd as G
+    d = G.Design.from_config("c2", gen_over=dict(num_cells=20_000, num_nets=21_000, num_hf_nets=0, macro_area_frac=0.05,
+                                                  macro_rows_min=5, macro_rows_max=20), bin_cnt=128)
+    d.init_nesterov_place()
+    try:
+        R = d.problem().regions[0]
+        m = R.cnt_x * R.cnt_y
+        raw = np.ctypeslib.as_array(R.non_place_raw, (m,)).copy()
+        uni = np.ctypeslib.as_array(R.non_place_union, (m,)).copy()
+        np_, npu = O.non_place_at(raw, uni, R.target_density)
+        assert np.array_equal(np_, np.ctypeslib.as_array(R.non_place, (m,)))
+        assert np.array_equal(npu, np.ctypeslib.as_array(R.non_place_unscaled, (m,)))
+        assert (uni >= 0).sum() > 100 and (np_ > 0).sum() > 100
+        assert (raw != np.maximum(uni, 0)).any()  # the macros' bivariate-normal overlaps differ from the union
+    finally:
+        d.close()
+
+
+def test_non_place_at_hand_computed():
+    td = np.float32(0.8)
+    raw, uni = [1000, 0, 500, 1000], [900, 400, -1, 1000]
+    a, b = O.non_place_at(raw, uni, float(td))
+    f = lambda v: int(np.float32(v) * td)  # noqa: E731
+    cap = int(np.float32(np.float32(900) * td) * np.float32(1.10))
+    assert a.tolist() == [min(f(1000), cap), 0, f(500), f(1000)]
+    assert b.tolist() == [f(900), f(400), 0, f(1000)]
+
+
+def test_cut_fillers_plan_hand_computed():
+    """cutFillerCells (nesterov.rs:3826-3866): whole fillers for the area, at most the fillers there
+    are; the target density raised only when more than a filler's area is left over."""
+    assert O.cut_fillers_plan(350, 100, 10, 50_000, 100_000, 0.7) == (3, pytest.approx(0.7))
+    assert O.cut_fillers_plan(1100, 100, 10, 50_000, 100_000, 0.7) == (10, pytest.approx(0.7))  # 100 left: not more
+    k, td = O.cut_fillers_plan(1250, 100, 10, 50_000, 100_000, 0.7)
+    assert k == 10 and td == np.float32(np.float32(50_000 + 1000 + 0 + 250) / np.float32(100_000))
+    assert O.cut_fillers_plan(-5, 100, 10, 50_000, 100_000, 0.7) == (0, pytest.approx(0.7))
+    assert O.cut_fillers_plan(500, 100, 0, 50_000, 100_000, 0.7)[0] == 0
+
+
+def test_budget_trim_hand_computed():
+    """scaleInflationToBudget (nesterov.rs:3946-3978): a pass may add min(budget / 6, what is left);
+    over it every excess above 1 is scaled by allowed / wanted."""
+    spent, r = O.budget_trim(600.0, 0.0, 50.0, [3.0, 1.0, 0.8])
+    assert not spent and r.tolist() == [3.0, 1.0, pytest.approx(0.8)]
+    spent, r = O.budget_trim(600.0, 0.0, 200.0, [3.0, 1.0, 1.5])
+    assert not spent and r.tolist() == [2.0, 1.0, 1.25]
+    spent, r = O.budget_trim(600.0, 550.0, 100.0, [3.0])
+    assert spent and r.tolist() == [2.0]
+    spent, _ = O.budget_trim(600.0, 590.0, 5.0, [1.2])
+    assert not spent
+    spent, _ = O.budget_trim(600.0, 590.0, 10.0, [1.2])
+    assert spent
+
+
+def test_congested_net_weights_hand_computed():
+    """weightCongestedNets (route_base.rs:381-426): the top share of the depositing nets, weighted
+    linearly from the maximum at the worst to 1 at the cutoff."""
+    n, w = O.congested_net_weights([0, 5, 3, 1, 5, 2], 50.0, 2.0)
+    assert n == 5 and w.tolist() == [0.0, 2.0, 1.0, 0.0, 2.0, 0.0]  # ceil(5 * 0.5) = 3 nets: 5, 5, 3
+    n, w = O.congested_net_weights([0, 5, 3, 1, 5, 2], 1.0, 1.5)
+    assert n == 5 and w.tolist() == [0.0, 1.5, 0.0, 0.0, 0.0, 0.0]  # one net, worst == cutoff: the maximum
+    n, w = O.congested_net_weights([0, 4, 2, 1], 100.0, 3.0)
+    assert w.tolist() == [0.0, 3.0, pytest.approx(1.0 + 2.0 * 1.0 / 3.0), 1.0]
+    assert O.congested_net_weights([1, 2], 10.0, 1.0)[0] == -1
+    assert O.congested_net_weights([0, 0], 10.0, 2.0)[0] == -1
