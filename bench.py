@@ -22,7 +22,9 @@ Besides the throughput the line carries:
                 twelve minutes, too long to repeat here; the problems' fingerprints must match),
                 and a fresh GPU-vs-oracle run to convergence on the 2k-cell placement;
   batched       16 seeds of the 50k-cell design placed one after another and from concurrent
-                contexts (each its own placement and control), with the results compared.
+                contexts (each its own placement and control), with the results compared;
+  routability   the routability-driven loop on the 500k-cell design (768 x 768 bins, RUDY on
+                1024 x 1024 tiles), np_init to convergence, against the oracle's full run.
 """
 import argparse
 import json
@@ -288,6 +290,45 @@ def batched(cfg, k, emit=True):
     return line
 
 
+def routability_leg(cfg="c5"):
+    """The routability-driven loop on the 500k-cell design (768 x 768 bins, RUDY on 1024 x 1024
+    tiles of 8 + 8 tracks, at most 5 passes): the whole placement timed from np_init to convergence
+    on the GPU, against the oracle's full run of the same problem (tests/golden/loop_c5.json)."""
+    import gpl_amd as G
+
+    t = time.perf_counter()
+    d = G.Design.from_config(cfg)
+    d.initial_place(device=True)
+    d.init_nesterov_place()
+    setup = time.perf_counter() - t
+    g = G.NesterovGPU(d)
+    t = time.perf_counter()
+    it = g.do_nesterov_place()
+    el = time.perf_counter() - t
+    s = g.stats()
+    tot, passes = g.route_stats()
+    g.close()
+    out = {"config": {"workload": "500k-cell synthetic placement: 500,000 cells, 520,000 nets, 768x768 bins, RUDY on "
+                                  "1024x1024 tiles (8 H + 8 V tracks), target congestion 1.0, at most 5 inflation passes",
+                      "name": cfg},
+           "iterations": it, "seconds": round(el, 3), "iters_per_s": round(it / el, 1) if el > 0 else None,
+           "converged": bool(s.converged), "final_overflow": round(s.overflow, 4), "final_hpwl": s.hpwl,
+           "passes": tot["passes"], "routability_iterations": tot["route_iters"], "final_congestion": round(tot["final_rc"], 4),
+           "min_congestion": round(tot["min_rc"], 4),
+           "inflation_pct": round(100.0 * tot["total_inflation"] / tot["original_area"], 2) if tot["original_area"] else None,
+           "pass_congestion": [round(p["rc"], 4) for p in passes], "setup_s": round(setup, 2)}
+    fx = load_fixture(cfg)
+    if fx and fx.get("route"):
+        out["cpu_run"] = {"iterations": fx["final_iter"], "seconds": round(fx["cpu_seconds_total"], 1), "final_hpwl": fx["final_hpwl"],
+                          "passes": fx["route"]["stats"]["passes"], "same_problem": d.problem_sha256() == fx["problem_sha256"],
+                          "hpwl_delta": round((s.hpwl - fx["final_hpwl"]) / fx["final_hpwl"], 6),
+                          "speedup": round(fx["cpu_seconds_total"] / el, 1) if el > 0 else None,
+                          "source": f"tests/golden/loop_{cfg}.json (the oracle's full run, one core)"}
+    d.close()
+    print(f"[bench] {cfg}: {it} iterations, {tot['passes']} routability passes in {el:.3f}s", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_info():
     model = platform.processor() or ""
     try:
@@ -311,6 +352,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the HPWL check")
     ap.add_argument("--hpwl-config", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--no-batched", action="store_true", help="skip the 16-seed batched comparison")
+    ap.add_argument("--no-routability", action="store_true", help="skip the routability-driven loop on the 500k-cell design")
     ap.add_argument("--launch-mode", type=int, default=0, choices=(0, 1, 2, 3, 4, 5, 6),
                     help="0: each trial one HIP graph (branches on side streams); 1: kernels launched one by "
                          "one on the side streams; 2: one by one on one stream; 3: one graph captured on one stream; "
@@ -402,11 +444,11 @@ def main():
                           "iters_per_s": round(steps / (t1 - t0), 2), "families": fam}), flush=True)
         d.close()
         return
-    # on to iteration 50, then to convergence: the steady-state window
-    t50, it50 = None, None
-    if not done and it < 50:
+    # the steady-state window: iteration 50 (or the timed window's start, when later) to convergence
+    t50, it50 = (t0, it0) if it0 >= 50 else (None, None)
+    if t50 is None and not done and it < 50:
         it, done = g.loop_step(50 - it)
-    if not done:
+    if t50 is None and not done:
         t50, it50 = time.perf_counter(), it
     while not done:
         it, done = g.loop_step(1 << 20)
@@ -514,6 +556,7 @@ def main():
     batched_line = None
     if not args.no_batched:
         batched_line = batched("c4s", 16, emit=False)
+    route_line = None if args.no_routability or args.config != "c3" else routability_leg("c5")
 
     out = {
         "metric": METRIC,
@@ -550,6 +593,7 @@ def main():
                           "note": "iteration 50 to convergence of the timed run"}
                          if t50 is not None and t_end > t50 and final_iter > it50 else None),
         "batched": batched_line,
+        "routability": route_line,
     }
     print(json.dumps(out), flush=True)
     d.close()

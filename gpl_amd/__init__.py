@@ -340,6 +340,9 @@ class NesterovGPU:
     def set_profiling(self, on):
         _ok(_ffi.lib().gpl_set_profiling(self._h, 1 if on else 0), "gpl_set_profiling")
 
+    def set_resort_policy(self, min_gap, growth):
+        _ok(_ffi.lib().gpl_set_resort_policy(self._h, min_gap, growth), "gpl_set_resort_policy", self._h)
+
     def set_launch_mode(self, mode):
         """0: one HIP graph per back-tracking trial (default); 1: kernels one by one on the side
         streams; 2: one by one on one stream; 3: one graph captured on one stream; 4-6:

@@ -198,6 +198,7 @@ SIGNATURES = [
     ("gpl_util_round_down_pow2", u32, [u32]),
     ("gpl_util_struct_sizes", i32, [P(i64), i32]),
     ("gpl_set_launch_mode", i32, [C.c_void_p, i32]),
+    ("gpl_set_resort_policy", i32, [C.c_void_p, i32, f32]),
     ("gpl_set_tuning", i32, [C.c_void_p, i32, i32, i32, i32]),
     ("gpl_get_tuning", i32, [C.c_void_p, P(i32), i32]),
     ("gpl_density_scatter_noalloc", i32, [P(DensityIO)]),

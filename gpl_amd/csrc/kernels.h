@@ -203,6 +203,10 @@ struct alignas(64) CtlHead {
     // length from iters_run >= step_nan_from, a re-sort at every iteration >= resort_from, the WA sum
     // read as zero in iteration zero_wl_at, a divergence in iterations diverge_at / diverge_at2
     int32_t nan_from, step_nan_from, resort_from, zero_wl_at, diverge_at, diverge_at2;
+    // the re-sort policy: a re-sort when the scatter's global adds per trial grew by resort_growth
+    // over their level after the last one, at least resort_gap iterations after it
+    int32_t resort_gap;
+    float resort_growth;
     CtlRegion reg[kCtlMaxRegions];
 };
 

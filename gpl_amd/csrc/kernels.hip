@@ -1863,7 +1863,7 @@ __device__ void ctl_trial(CtlHead& g, const Sums* sums, CtlGlobal* G, float* nst
         const double rate = double(ga - g.last_global_adds) / double(trials > 1 ? trials : 1);
         double base = g.adds_base;
         if (iter == g.last_resort_iter + 1 || base == 0.0) base = rate;
-        if (rate > 1.5 * base && iter - g.last_resort_iter >= 50) return ctl_halt(g, CTL_RESORT, 2);
+        if (rate > (double)g.resort_growth * base && iter - g.last_resort_iter >= g.resort_gap) return ctl_halt(g, CTL_RESORT, 2);
         g.last_global_adds = ga;
         g.adds_base = base;
     }

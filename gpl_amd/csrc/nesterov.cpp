@@ -173,6 +173,8 @@ struct gpl_ctx {
     uint64_t last_global_adds = 0;
     double adds_base = 0.0;  // global adds per trial in the first iteration after the last sort
     int32_t last_resort_iter = -1000, resorts = 0;
+    int32_t resort_gap = 50;       // the re-sort policy (gpl_set_resort_policy)
+    float resort_growth = 1.5f;
     int32_t site_x = 0, dbu = 1000;
     gpl_opts o{};
     // NesterovPlace
@@ -829,6 +831,8 @@ struct gpl_ctx {
         g.step_nan_from = inject_kind == 2 && after ? inject_after : -1;
         g.resort_from = inject_kind == 3 && after ? inject_after : -1;
         g.zero_wl_at = inject_zero_wl_at;
+        g.resort_gap = resort_gap;
+        g.resort_growth = resort_growth;
         g.diverge_at = inject_diverge_at;
         g.diverge_at2 = inject_diverge_at2;
         for (size_t b = 0; b < rg.size(); ++b) {
@@ -1339,7 +1343,7 @@ struct gpl_ctx {
         const double rate = double(ga - last_global_adds) / double(std::max(trials, 1));
         last_global_adds = ga;
         if (iter == last_resort_iter + 1 || adds_base == 0.0) adds_base = rate;
-        if (rate > 1.5 * adds_base && iter - last_resort_iter >= 50) {
+        if (rate > (double)resort_growth * adds_base && iter - last_resort_iter >= resort_gap) {
             autotune_before_resort();
             resort();
             last_resort_iter = iter;
@@ -3259,6 +3263,13 @@ int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode) {
     c->overlap = mode < 2 || mode >= 4;
     c->device_trials = mode >= 4;
     c->ctl_launch = mode >= 4 ? mode - 4 : 0;
+    return GPL_OK;
+}
+
+int32_t gpl_set_resort_policy(gpl_ctx* c, int32_t min_gap, float growth) {
+    if (!c || min_gap < 1 || !(growth >= 1.0f)) return GPL_ERR_ARG;
+    c->resort_gap = min_gap;
+    c->resort_growth = growth;
     return GPL_OK;
 }
 

@@ -586,6 +586,11 @@ int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode);
 int32_t gpl_set_tuning(gpl_ctx* c, int32_t region, int32_t chunk, int32_t table_bits, int32_t stage);
 int32_t gpl_get_tuning(gpl_ctx* c, int32_t* out, int32_t cap);
 int32_t gpl_set_profiling(gpl_ctx* c, int32_t on);
+/* The device re-sort policy (no reference counterpart: the cells' storage order does not change
+ * the placement): a re-sort when the scatter's global atomic adds per trial have grown by
+ * `growth` (>= 1) over their level in the first iteration after the last re-sort, at least
+ * `min_gap` (>= 1) iterations after it.  Defaults 50 and 1.5. */
+int32_t gpl_set_resort_policy(gpl_ctx* c, int32_t min_gap, float growth);
 
 #ifdef __cplusplus
 }

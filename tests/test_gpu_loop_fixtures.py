@@ -2,8 +2,10 @@
 fixtures (tests/golden/loop_<config>.json, written by tests/golden/make_loop_fixtures.py: the
 1M-cell oracle run takes about twelve minutes on one core): the 200k-cell (512 x 512 bins) and
 1M-cell (1024 x 1024 bins) workloads, the eight region groups of 5k-150k cells (485k, grids
-32^2..512^2) with 20k top-level cells, and seed 1 of the 50k-cell design (256 x 256 bins); and the
-16 seeds of that design placed concurrently against one after another.
+32^2..512^2) with 20k top-level cells, seed 1 of the 50k-cell design (256 x 256 bins), and the
+routability-driven 500k-cell placement (768 x 768 bins, RUDY on 1024 x 1024 tiles, at most five
+inflation passes: the passes' decisions, iterations, congestion and added area too); and the 16
+seeds of the 50k-cell design placed concurrently against one after another.
 
 Bars (the headline's): both runs reach overflow 0.10, the final pin HPWL within 0.5 % of the
 oracle's, the iteration count within 2 % (the GPU's FP64 reductions and FFT rounding move the
@@ -29,7 +31,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-@pytest.mark.parametrize("cfg", ["c2", "c3", "c4r", "c4s"])
+@pytest.mark.parametrize("cfg", ["c2", "c3", "c4r", "c4s", "c5"])
 def test_loop_matches_the_oracles_full_run(cfg, request):
     with open(os.path.join(HERE, "golden", f"loop_{cfg}.json")) as f:
         fx = json.load(f)
@@ -46,6 +48,7 @@ def test_loop_matches_the_oracles_full_run(cfg, request):
         g = G.NesterovGPU(d)
         it = g.do_nesterov_place()
         s, hist = g.stats(), g.history()
+        route = g.route_stats() if cfg == "c5" else None
         g.close()
         if cfg == "c4r":
             # nine regions in one trial graph against the kernels launched one by one on one stream
@@ -67,6 +70,18 @@ def test_loop_matches_the_oracles_full_run(cfg, request):
     assert s.overflow <= 0.1 and fx["final_overflow"] <= 0.1
     assert abs(delta) <= max(0.005, 1.25 * hpwl_env), (delta, hpwl_env)
     assert abs(it - fx["final_iter"]) <= max(10, 0.02 * fx["final_iter"], 1.25 * iter_env)
+    if route is not None:
+        # the routability passes: the same count and decisions, each at the oracle's iteration
+        # within 2, its congestion within 3 % and its added area within 3 %
+        tot, passes = route
+        want = fx["route"]["passes"]
+        print("passes", [(p["iter"], round(p["rc"], 4)) for p in passes], "oracle", [(p["iter"], round(p["rc"], 4)) for p in want])
+        assert tot["passes"] == fx["route"]["stats"]["passes"] == len(want)
+        for a, b in zip(passes, want):
+            assert (a["need"], a["revert"]) == (b["need"], b["revert"])
+            assert abs(a["iter"] - b["iter"]) <= 2
+            assert abs(a["rc"] - b["rc"]) <= 0.03 * b["rc"]
+            assert abs(a["inflation"] - b["inflation"]) <= 0.03 * max(b["inflation"], 1) + 1
     r0 = fx["rows"][0]
     assert hist[0]["iter"] == r0["iter"]
     assert abs(hist[0]["hpwl"] - r0["hpwl"]) <= 1e-4 * r0["hpwl"]
