@@ -46,6 +46,7 @@ WORKLOADS = {
     "c3": "1M-cell synthetic placement: 1,000,000 cells, 1,050,000 nets (+50 of 1k-5k pins), 5% fixed macros, 1024x1024 bins",
     "c4s": "50k-cell synthetic placement: 50,000 cells, 52,000 nets, 256x256 bins",
     "c4r": "8 region groups (5k-150k cells, 485k) + 20k top-level cells, per-region bin grids",
+    "c5": "500k-cell routability-driven placement: 500,000 cells, 520,000 nets, 768x768 bins, RUDY on 1024x1024 tiles",
 }
 
 

@@ -277,8 +277,13 @@ struct RudyBufs {
     float* cong = nullptr;
     float* sorted = nullptr;
     double* stats = nullptr;
+    float* over = nullptr;          // the overflowed tiles' congestions in the tiles' order
+    int* nover = nullptr;           // their count
+    int* col_start = nullptr;       // per column (cnt_x + 1): its congested tiles in col_y
+    int* col_y = nullptr;           // the congested tiles' rows, column by column, ascending
     size_t cap_nets = 0, cap_off = 0, cap_pairs = 0, cap_pairs2 = 0, cap_pairs3 = 0, cap_pairs4 = 0, cap_temp = 0;
     size_t cap_tiles = 0, cap_tiles2 = 0, cap_act = 0, cap_ratio = 0, cap_cong = 0, cap_sorted = 0, cap_stats = 0;
+    size_t cap_over = 0, cap_nover = 0, cap_col = 0, cap_coly = 0;
     void release();
 };
 // RUDY demand and blockage of the grid from the nets' boxes in n.rec (a WA forward with the net
@@ -501,8 +506,9 @@ hipError_t launch_non_place(hipStream_t s, int m, const long long* raw, const lo
                             float2* non_place_f);
 // getNetDemandInTiles (route_base.rs:200-218): each net's demand summed in float over the tiles of
 // its box (x outer, y inner) whose inflation ratio (y-major, the last launch_rudy_tiles) is above 1,
-// written at the net's problem index (net_orig) into out.
+// written at the net's problem index (net_orig) into out.  The congested tiles are first listed per
+// column (rows ascending), so a net visits only those of its box, in the same order.
 hipError_t launch_net_congested_demand(hipStream_t s, const NetsDev& n, const int32_t* net_orig, const RudyGrid& g, const float* ratio,
-                                       float* out);
+                                       float* out, RudyBufs& b);
 
 }  // namespace gplk

@@ -1963,7 +1963,7 @@ struct gpl_ctx {
     void congested_net_demand() {
         rt.demand.clear();
         if (o.routability_net_weight_max <= 1.0f || o.routability_congested_nets_percentage <= 0.0f || nnets == 0) return;
-        check(gplk::launch_net_congested_demand(stream, nets, d_net_orig, rt.grid, rudy.ratio, rt.d_demand), "net demand");
+        check(gplk::launch_net_congested_demand(stream, nets, d_net_orig, rt.grid, rudy.ratio, rt.d_demand, rudy), "net demand");
         rt.demand.resize(size_t(nnets));
         check(hipMemcpyAsync(rt.demand.data(), rt.d_demand, sizeof(float) * rt.demand.size(), hipMemcpyDeviceToHost, stream), "copy");
         check(hipStreamSynchronize(stream), "sync");

@@ -174,33 +174,75 @@ __global__ void k_rudy_ratio(RudyGrid g, RouteVarsDev v, const float* demand, co
     }
 }
 
-// updateRudyAverage's sums, one lane, the reference's orders: the total overflow over the tiles
-// (y-major), the shares over the congestions sorted in descending order.
-__global__ void k_rudy_stats(unsigned n, const float* cong, const float* sorted, RouteVarsDev v, double* out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double total = 0.0;
-    unsigned over = 0, m = 0;
-    for (unsigned t = 0; t < n; ++t) {
-        const float c = cong[t];
-        if (c < 0.0f) continue;
-        const double r = (double)c;
-        total += (-1.0 + r) > 0.0 ? (-1.0 + r) : 0.0;
-        ++m;
-        if (c > 1.0f) ++over;
+// updateRudyAverage's sums in the reference's orders, by one block: the counts from the congestions
+// sorted in descending order (the uncounted tiles hold -1 and sort last); the total overflow over the
+// overflowed tiles in the tiles' order (a term max(0, c - 1) of any other tile is +0, which leaves a
+// non-negative double sum as it is); the shares over the sorted congestions.  The block stages 256
+// values at a time in LDS, where its first lane adds them in order.
+constexpr int kStatBlock = 256;
+__device__ unsigned count_desc_above(const float* sorted, unsigned n, float t, bool or_equal) {
+    // the first index whose value is not above t (not at or above t with or_equal), by bisection
+    unsigned lo = 0, hi = n;
+    while (lo < hi) {
+        const unsigned mid = (lo + hi) >> 1;
+        const float v = sorted[mid];
+        if (or_equal ? v >= t : v > t)
+            lo = mid + 1;
+        else
+            hi = mid;
     }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kStatBlock) k_rudy_stats(unsigned n, const float* sorted, const float* over_list, const int* nover,
+                                                           RouteVarsDev v, double* out) {
+    __shared__ float s_v[kStatBlock];
+    __shared__ unsigned s_m, s_over;
+    if (threadIdx.x == 0) {
+        s_m = count_desc_above(sorted, n, 0.0f, true);
+        s_over = (unsigned)*nover;
+    }
+    __syncthreads();
+    const unsigned m = s_m, over = s_over;
+    // the total overflow over the overflowed tiles, in the tiles' order
+    double total = 0.0;
+    for (unsigned b0 = 0; b0 < over; b0 += kStatBlock) {
+        const unsigned i = b0 + threadIdx.x;
+        s_v[threadIdx.x] = i < over ? over_list[i] : 0.0f;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned e = min(over - b0, (unsigned)kStatBlock);
+            for (unsigned j = 0; j < e; ++j) {
+                const double r = (double)s_v[j];
+                total += (-1.0 + r) > 0.0 ? (-1.0 + r) : 0.0;
+            }
+        }
+        __syncthreads();
+    }
+    double top[4] = {0.0, 0.0, 0.0, 0.0}, rc = 0.0;
+    const double shares[4] = {0.005, 0.01, 0.02, 0.05};
+    const double nd = (double)m;
+    double sums[4] = {0.0, 0.0, 0.0, 0.0};
+    // the elements i < 0.05 m of the sorted list (the widest share: ceil(0.05 m) of them), staged
+    // and added in order
+    const unsigned lim = m > 0 ? min(m, (unsigned)ceil(shares[3] * nd)) : 0u;
+    for (unsigned b0 = 0; b0 < lim; b0 += kStatBlock) {
+        const unsigned i = b0 + threadIdx.x;
+        s_v[threadIdx.x] = i < lim ? sorted[i] : 0.0f;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned e = min(lim - b0, (unsigned)kStatBlock);
+            for (unsigned j = 0; j < e; ++j)
+                for (int q = 0; q < 4; ++q)
+                    if ((double)(b0 + j) < shares[q] * nd) sums[q] += (double)s_v[j];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     out[0] = total;
     out[1] = (double)over;
     out[2] = (double)m;
-    double top[4] = {0.0, 0.0, 0.0, 0.0}, rc = 0.0;
     if (m > 0) {
-        const double shares[4] = {0.005, 0.01, 0.02, 0.05};
-        const double nd = (double)m;
-        double sums[4] = {0.0, 0.0, 0.0, 0.0};
-        for (unsigned i = 0; i < m; ++i) {
-            if (!((double)i < shares[3] * nd)) break;
-            for (int s = 0; s < 4; ++s)
-                if ((double)i < shares[s] * nd) sums[s] += (double)sorted[i];
-        }
         top[0] = sums[0] / ceil(0.005 * nd);
         top[1] = sums[1] / ceil(0.010 * nd);
         top[2] = sums[2] / ceil(0.020 * nd);
@@ -301,16 +343,45 @@ __global__ void k_non_place(int m, const long long* raw, const long long* uni, f
     }
 }
 
-__global__ void k_net_congested_demand(int E, const NetRec* rec, const int32_t* net_orig, RudyGrid g, const float* ratio, float* out) {
+// The congested tiles (inflation ratio above 1) listed per column, rows ascending: the counts,
+// then (after a scan into col_start) the rows.
+__global__ void k_col_count(RudyGrid g, const float* ratio, int* cnt) {
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < g.cnt_x; x += gridDim.x * blockDim.x) {
+        int k = 0;
+        for (int y = 0; y < g.cnt_y; ++y) k += ratio[(size_t)y * g.cnt_x + x] > 1.0f ? 1 : 0;
+        cnt[x] = k;
+    }
+}
+__global__ void k_col_fill(RudyGrid g, const float* ratio, const int* start, int* col_y) {
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < g.cnt_x; x += gridDim.x * blockDim.x) {
+        int at = start[x];
+        for (int y = 0; y < g.cnt_y; ++y)
+            if (ratio[(size_t)y * g.cnt_x + x] > 1.0f) col_y[at++] = y;
+    }
+}
+
+// Each net's demand over the congested tiles of its box, in visitNetTiles' order (x outer, y inner):
+// per column the first listed row at or above the box's by bisection, then the rows up to its top.
+__global__ void k_net_congested_demand(int E, const NetRec* rec, const int32_t* net_orig, RudyGrid g, const int* col_start, const int* col_y,
+                                       float* out) {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
         const int4 b = rec[e].box;
         int x0, x1, y0, y1;
         float cong;
         float d = 0.0f;
         if (rudy_range(g, b, x0, x1, y0, y1, cong)) {
-            for (int x = x0; x <= x1; ++x)
-                for (int y = y0; y <= y1; ++y) {
-                    if (!(ratio[(size_t)y * g.cnt_x + x] > 1.0f)) continue;
+            for (int x = x0; x <= x1; ++x) {
+                int lo = col_start[x], hi = col_start[x + 1];
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (col_y[mid] < y0)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                for (int q = lo; q < col_start[x + 1]; ++q) {
+                    const int y = col_y[q];
+                    if (y > y1) break;
                     const int4 r = rudy_rect(g, x, y);
                     if (!(r.z > b.x && r.x < b.z && r.w > b.y && r.y < b.w)) continue;
                     const long long inter = (long long)(min(b.z, r.z) - max(b.x, r.x)) * (long long)(min(b.w, r.w) - max(b.y, r.y));
@@ -318,6 +389,7 @@ __global__ void k_net_congested_demand(int E, const NetRec* rec, const int32_t* 
                     const float rt = (float)inter / (float)tile_area;
                     d += cong * rt * 100.0f;
                 }
+            }
         }
         out[net_orig[e]] = d;
     }
@@ -354,6 +426,10 @@ void RudyBufs::release() {
     if (cong) (void)hipFree(cong);
     if (sorted) (void)hipFree(sorted);
     if (stats) (void)hipFree(stats);
+    if (over) (void)hipFree(over);
+    if (nover) (void)hipFree(nover);
+    if (col_start) (void)hipFree(col_start);
+    if (col_y) (void)hipFree(col_y);
     *this = RudyBufs{};
 }
 
@@ -422,7 +498,19 @@ hipError_t launch_rudy_tiles(hipStream_t s, const CellsDev& c, int ncells, const
     if (!b.temp) return hipErrorOutOfMemory;
     e = hipcub::DeviceRadixSort::SortKeysDescending(b.temp, tb, b.cong, b.sorted, (int)tiles, 0, 32, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_rudy_stats, dim3(1), dim3(64), 0, s, (unsigned)tiles, b.cong, b.sorted, v, b.stats);
+    // the overflowed tiles' congestions in the tiles' order (the total overflow's terms)
+    grow(b.over, b.cap_over, tiles);
+    grow(b.nover, b.cap_nover, 1);
+    if (!b.over || !b.nover) return hipErrorOutOfMemory;
+    auto overflowed = [] __device__(float c) { return c > 1.0f; };
+    tb = 0;
+    e = hipcub::DeviceSelect::If(nullptr, tb, b.cong, b.over, b.nover, (int)tiles, overflowed, s);
+    if (e != hipSuccess) return e;
+    grow(b.temp, b.cap_temp, tb);
+    if (!b.temp) return hipErrorOutOfMemory;
+    e = hipcub::DeviceSelect::If(b.temp, tb, b.cong, b.over, b.nover, (int)tiles, overflowed, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rudy_stats, dim3(1), dim3(kStatBlock), 0, s, (unsigned)tiles, b.sorted, b.over, b.nover, v, b.stats);
     return hipGetLastError();
 }
 
@@ -460,9 +548,26 @@ hipError_t launch_non_place(hipStream_t s, int m, const long long* raw, const lo
 }
 
 hipError_t launch_net_congested_demand(hipStream_t s, const NetsDev& n, const int32_t* net_orig, const RudyGrid& g, const float* ratio,
-                                       float* out) {
-    if (n.num_nets > 0)
-        hipLaunchKernelGGL(k_net_congested_demand, dim3(grid_of(n.num_nets)), dim3(kRB), 0, s, n.num_nets, n.rec, net_orig, g, ratio, out);
+                                       float* out, RudyBufs& b) {
+    if (n.num_nets <= 0) return hipSuccess;
+    const size_t tiles = (size_t)g.cnt_x * (size_t)g.cnt_y;
+    grow(b.col_start, b.cap_col, (size_t)g.cnt_x + 1);
+    grow(b.col_y, b.cap_coly, tiles);
+    if (!b.col_start || !b.col_y) return hipErrorOutOfMemory;
+    hipLaunchKernelGGL(k_col_count, dim3(grid_of(g.cnt_x)), dim3(kRB), 0, s, g, ratio, b.col_start);
+    // the count at cnt_x is the scan's extra element (the total lands there)
+    hipError_t e = hipMemsetAsync(b.col_start + g.cnt_x, 0, sizeof(int), s);
+    if (e != hipSuccess) return e;
+    size_t tb = 0;
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, b.col_start, b.col_start, g.cnt_x + 1, s);
+    if (e != hipSuccess) return e;
+    grow(b.temp, b.cap_temp, tb);
+    if (!b.temp) return hipErrorOutOfMemory;
+    e = hipcub::DeviceScan::ExclusiveSum(b.temp, tb, b.col_start, b.col_start, g.cnt_x + 1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_col_fill, dim3(grid_of(g.cnt_x)), dim3(kRB), 0, s, g, ratio, b.col_start, b.col_y);
+    hipLaunchKernelGGL(k_net_congested_demand, dim3(grid_of(n.num_nets)), dim3(kRB), 0, s, n.num_nets, n.rec, net_orig, g, b.col_start, b.col_y,
+                       out);
     return hipGetLastError();
 }
 
