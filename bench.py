@@ -160,7 +160,7 @@ def summarize(paths):
     from collections import defaultdict
 
     def short(name):
-        name = name.split("(")[0]
+        name = name.replace("(anonymous namespace)::", "").split("(")[0]
         return name.replace("void ", "").strip()
 
     for path in paths:
