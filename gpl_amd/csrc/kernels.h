@@ -342,6 +342,10 @@ void launch_copy_f64(hipStream_t s, const double* a, double* b, int n);
 // the Poisson solve of r.density into r.field (and r.phi when want_phi); nothing while *skip (when
 // given) is set.
 void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi, const int* skip = nullptr);
+// launch_bin_density (clearing, partials into `partials`) then launch_solve without phi, the bins
+// pass fused into the solve's first row pass where the grid allows: the same densities, partials
+// and field bit for bit.
+void launch_bins_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, double* partials, const int* skip = nullptr);
 // WA forward over every net, the HPWL, and each pin's weighted WA gradient into n.pgrad (at its
 // cell-pin index); the net records too when write_rec.  The high-fanout nets go to s_big (which
 // may be s; the caller orders s_big after the zeroed HPWL counter and joins it).

@@ -499,33 +499,96 @@ __device__ void finish_sums(const double* partials, int nb_bins, int nb_cells, S
     }
 }
 
-// updateBinsGCellDensityArea's per-bin part (nesterov.rs:2252-2262): density, the overflow
-// terms (summed here in FP64), and the accumulators cleared for the next scatter.
-__global__ void k_bin_density(RegionDev r, double* partials, int clear) {
+// LDS pointers typed as such, so that every access compiles to ds_read / ds_write (a generic
+// pointer into LDS makes flat accesses, which wait like global loads).
+#define GPL_LDS __attribute__((address_space(3)))
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef GPL_LDS vf2 lds_float2;
+typedef GPL_LDS float lds_float;
+__device__ __forceinline__ float2 ld2(const lds_float2* p) {
+    const vf2 v = *p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float2 ld2(const float2* p) { return *p; }
+__device__ __forceinline__ void st2(lds_float2* p, float2 v) {
+    vf2 t;
+    t.x = v.x;
+    t.y = v.y;
+    *p = t;
+}
+
+// The bins pass's layout, shared with the solve's first row pass (which runs it fused): a block of
+// kBlock threads takes rows_of_block consecutive bin rows, row_threads threads per row (the row
+// transform's L/8), each thread the elements x = tid + i * row_threads; its FP64 overflow terms are
+// summed in that order, then over the block in a fixed tree, one partial per block.  The same
+// layout in both kernels gives the same partials bit for bit.
+__host__ __device__ constexpr int row_threads(int L) { return (L >> 1) >= 4 ? (L >> 1) >> 2 : 1; }
+__host__ __device__ constexpr int rows_of_block(int L) { return kBlock / row_threads(L) > 0 ? kBlock / row_threads(L) : 1; }
+
+// updateBinsGCellDensityArea's per-bin part (nesterov.rs:2252-2262) over one thread's elements of
+// bin row y: density (to dens, and with ROW to the LDS row), the overflow terms added to o and ou,
+// the accumulators cleared when asked.
+template <bool ROW>
+__device__ __forceinline__ void bins_row_terms(unsigned long long* inst_a, unsigned long long* fill_a, const float2* npf_a, const int32_t* bin_x,
+                                               const int32_t* bin_y, float td, float* dens, lds_float* rowv, int cnt_x, int y, int tid,
+                                               int T, bool clear, double& o, double& ou) {
+    const size_t base = (size_t)y * (size_t)cnt_x;
+    const int ey = bin_y[y + 1] - bin_y[y];
+    // eight elements at a time, every load issued before the first store (the stores could alias
+    // the next loads, so a fused loop would wait out each load in turn)
+    constexpr int kPer = 8;
+    for (int x0 = tid; x0 < cnt_x; x0 += kPer * T) {
+        long long iv[kPer], fv[kPer];
+        float2 nv[kPer];
+        int bw[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int x = x0 + i * T;
+            if (x < cnt_x) {
+                const size_t k = base + (size_t)x;
+                iv[i] = (long long)inst_a[k];
+                fv[i] = (long long)fill_a[k];
+                nv[i] = npf_a[k];  // the two int64 constants, converted once on the host
+                bw[i] = bin_x[x + 1] - bin_x[x];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int x = x0 + i * T;
+            if (x >= cnt_x) break;
+            const size_t k = base + (size_t)x;
+            const long long area = (long long)bw[i] * (long long)ey;
+            const float scaled = (float)area * td;
+            const float d = ((float)iv[i] + (float)fv[i] + nv[i].x) / scaled;
+            dens[k] = d;
+            if (ROW) rowv[x] = d;
+            float t = rmax((float)iv[i] + nv[i].x - scaled, 0.0f);
+            float tu = rmax((float)iv[i] + nv[i].y - scaled, 0.0f);
+            if (!(t >= 0.0f)) t = 0.0f;
+            if (!(tu >= 0.0f)) tu = 0.0f;
+            o += (double)t;
+            ou += (double)tu;
+            if (clear) {
+                inst_a[k] = 0ull;
+                fill_a[k] = 0ull;
+            }
+        }
+    }
+}
+
+// The bins pass alone: density, the overflow partials, the accumulators cleared for the next
+// scatter (unless `clear` is 0).
+__global__ void __launch_bounds__(kBlock) k_bin_density(RegionDev r, double* partials, int clear) {
     if (r.ctl && (r.ctl->hot.flags & 1)) return;
     __builtin_amdgcn_s_setprio(2);  // on the trial's critical chain (as the scatter)
     __shared__ double sh[kBlock / 64];
-    const size_t M = (size_t)r.cnt_x * r.cnt_y;
+    const int T = row_threads(r.cnt_x), rpb = rows_of_block(r.cnt_x);
+    const int sub = (int)threadIdx.x / T, tid = (int)threadIdx.x - sub * T;
+    const int y = (int)blockIdx.x * rpb + sub;
     double o = 0.0, ou = 0.0;
-    const float td = r.target_density;
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < M; k += (size_t)gridDim.x * blockDim.x) {
-        int x = (int)(k % r.cnt_x), y = (int)(k / r.cnt_x);
-        long long inst = (long long)r.inst[k], fill = (long long)r.filler[k];
-        const float2 npf = r.non_place_f[k];  // the two int64 constants, converted once on the host
-        long long area = (long long)(r.bin_x[x + 1] - r.bin_x[x]) * (long long)(r.bin_y[y + 1] - r.bin_y[y]);
-        float scaled = (float)area * td;
-        r.density[k] = ((float)inst + (float)fill + npf.x) / scaled;
-        float t = rmax((float)inst + npf.x - scaled, 0.0f);
-        float tu = rmax((float)inst + npf.y - scaled, 0.0f);
-        if (!(t >= 0.0f)) t = 0.0f;
-        if (!(tu >= 0.0f)) tu = 0.0f;
-        o += (double)t;
-        ou += (double)tu;
-        if (clear) {
-            r.inst[k] = 0ull;
-            r.filler[k] = 0ull;
-        }
-    }
+    if (sub < rpb && y < r.cnt_y)
+        bins_row_terms<false>(r.inst, r.filler, r.non_place_f, r.bin_x, r.bin_y, r.target_density, r.density, nullptr, r.cnt_x, y, tid, T,
+                       clear != 0, o, ou);
     double so = block_sum(o, sh);
     double sou = block_sum(ou, sh);
     if (threadIdx.x == 0) {
@@ -550,7 +613,8 @@ enum RowMode {
     ROW_POISSON = 2,
     ROW_INV_FIELDS = 3,
     ROW_INV_PHI = 4,
-    ROW_SPECTRUM_POISSON = 6
+    ROW_SPECTRUM_POISSON = 6,
+    ROW_BINS_DCT2 = 7
 };
 
 // The modes that run two inverse transforms side by side (a second pair of FFT buffers per row).
@@ -559,23 +623,6 @@ __host__ __device__ constexpr bool row_mode_dual(int m) { return m == ROW_POISSO
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
-// LDS pointers typed as such, so that every access compiles to ds_read / ds_write (a generic
-// pointer into LDS makes flat accesses, which wait like global loads).
-#define GPL_LDS __attribute__((address_space(3)))
-typedef float vf2 __attribute__((ext_vector_type(2)));
-typedef GPL_LDS vf2 lds_float2;
-typedef GPL_LDS float lds_float;
-__device__ __forceinline__ float2 ld2(const lds_float2* p) {
-    const vf2 v = *p;
-    return make_float2(v.x, v.y);
-}
-__device__ __forceinline__ float2 ld2(const float2* p) { return *p; }
-__device__ __forceinline__ void st2(lds_float2* p, float2 v) {
-    vf2 t;
-    t.x = v.x;
-    t.y = v.y;
-    *p = t;
-}
 
 // Complex FFTs of N points (N = 2^logN, or 3 * 2^logN with r3), NB of them side by side (each held
 // in LDS buffer x[b]; the NB transforms share the twiddles and the barriers), by T = max(N/4, 1)
@@ -862,6 +909,15 @@ struct RowArgs {
     int interleave;      // ROW_SPECTRUM_POISSON: the two fields as pairs (Ex', Ey') in out0, [k][row][2];
                          // ROW_INV_FIELDS: in0 holds such pairs, [row][v][2]
     float2* pairs;       // ROW_INV_FIELDS: the two outputs as (out0, out1) pairs, [row][j] (out0/out1 unused)
+    // ROW_BINS_DCT2: the bins pass's inputs and outputs (k_bin_density's), the rows being bin rows
+    unsigned long long* inst;
+    unsigned long long* filler;
+    const float2* npf;
+    const int32_t* bin_x;
+    const int32_t* bin_y;
+    float target;
+    float* density;
+    double* partials;
 };
 
 // Row transforms, L/8 threads per row, 256 / (L/8) rows per block (rows of one XCD contiguous,
@@ -874,6 +930,9 @@ struct RowArgs {
 //  ROW_INV_FIELDS  out0 = DST-III(in0), out1 = DCT-III(in1)
 //  ROW_INV_PHI     out0 = DCT-III(in0)
 //  ROW_SPECTRUM_POISSON  ROW_DCT2_SCALE kept in LDS, then ROW_POISSON on it
+//  ROW_BINS_DCT2   the bins pass of the row (k_bin_density's per-bin part and partials, in its
+//                  layout: L a power of two, so that a block is kBlock threads) into LDS, then
+//                  ROW_DCT2 of it
 // With transpose_out the outputs go out transposed, so that the next pass reads rows.
 // TL: the twiddle tables (and the element wavenumbers) are first copied to LDS (the launcher
 // picks it when they fit beside the rows' buffers); else they are read from global memory.
@@ -966,7 +1025,24 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
         }
         __syncthreads();
     };
-    if (MODE == ROW_DCT2 || MODE == ROW_DCT2_SCALE) {
+    if (MODE == ROW_BINS_DCT2) {
+        // the row's densities into the FFT buffer that dct2_row fills last (B: its first loop
+        // writes A, and the transform's first barrier comes before any write into B)
+        __shared__ double sh[kBlock / 64];
+        lds_float* rowv = (lds_float*)R.B;
+        double o = 0.0, ou = 0.0;
+        if (active)
+            bins_row_terms<true>(a.inst, a.filler, a.npf, a.bin_x, a.bin_y, a.target, a.density, rowv, L, row, R.tid, T, true, o, ou);
+        const double so = block_sum(o, sh);
+        const double sou = block_sum(ou, sh);
+        if (threadIdx.x == 0) {
+            a.partials[lb] = so;
+            a.partials[kPartials + lb] = sou;
+        }
+        __syncthreads();
+        dct2_row(R, active, [&](int j) { return rowv[j]; }, [&](int k, float v) { emit(a.out0, st0, k, v); });
+        flush(a.out0, false);
+    } else if (MODE == ROW_DCT2 || MODE == ROW_DCT2_SCALE) {
         const double scale = a.scale;
         const float* src = a.in0 + off;
         dct2_row(R, active, [&](int j) { return src[j]; }, [&](int k, float v) {
@@ -2225,7 +2301,10 @@ static int grid_for(size_t n) {
     return round_xcds((int)b);
 }
 
-int block_partials_bins(const RegionDev& r) { return grid_for((size_t)r.cnt_x * r.cnt_y); }
+int block_partials_bins(const RegionDev& r) {
+    const int rpb = rows_of_block(r.cnt_x);
+    return (r.cnt_y + rpb - 1) / rpb;
+}
 // The grid of the per-cell kernels that leave FP64 partials (the gradient, the distances): no more
 // blocks than the device holds at once at the gradient's occupancy, so that no second, part-filled
 // round of blocks follows the first.  Fixed per device (and so is the partials' summation order).
@@ -2380,6 +2459,7 @@ static void launch_rows(hipStream_t s, int mode, RowArgs a) {
     case ROW_POISSON: GPL_ROWS(ROW_POISSON); break;
     case ROW_INV_FIELDS: GPL_ROWS(ROW_INV_FIELDS); break;
     case ROW_SPECTRUM_POISSON: GPL_ROWS(ROW_SPECTRUM_POISSON); break;
+    case ROW_BINS_DCT2: GPL_ROWS(ROW_BINS_DCT2); break;
     default: GPL_ROWS(ROW_INV_PHI); break;
     }
 #undef GPL_ROWS
@@ -2395,7 +2475,10 @@ static bool three_pass(const RegionDev& r) { return r.cnt_x <= 2048 && r.cnt_y <
 //    Poisson divide (425-440) and the inverse transforms along y -> Ex'[y][u] (cos), Ey'[y][u]
 //    (sin), phi'[y][u];
 // Q3 rows along x: the field pairs (sin along x, cos along x) (and phi = cos along x).
-static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi, const int* skip) {
+// bins_partials: Q1 runs the bins pass fused (ROW_BINS_DCT2, its partials there) instead of
+// reading r.density.
+static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi, const int* skip,
+                             double* bins_partials = nullptr) {
     const int nx = r.cnt_x, ny = r.cnt_y;
     RowArgs a{};
     a.in0 = r.density;
@@ -2407,7 +2490,17 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs&
     a.dw = sb.dw_x;
     a.transpose_out = 1;
     a.skip = skip;
-    launch_rows(s, ROW_DCT2, a);
+    if (bins_partials) {
+        a.inst = r.inst;
+        a.filler = r.filler;
+        a.npf = r.non_place_f;
+        a.bin_x = r.bin_x;
+        a.bin_y = r.bin_y;
+        a.target = r.target_density;
+        a.density = r.density;
+        a.partials = bins_partials;
+    }
+    launch_rows(s, bins_partials ? ROW_BINS_DCT2 : ROW_DCT2, a);
     a = RowArgs{};
     a.in0 = sb.work1;
     a.out0 = sb.work0;  // (Ex', Ey') pairs [y][u][2] (work0 holds 2 cnt_x cnt_y floats)
@@ -2454,6 +2547,21 @@ static void solve_three_pass(hipStream_t s, const RegionDev& r, const SolveBufs&
 // P2: rows along y, DCT-II + halving + scale -> A[u][v]; P3: rows along v, the Poisson divide and
 // the inverse transforms along y -> Ex'[u][y] (cos), Ey'[u][y] (sin), phi'[u][y]; transpose ->
 // [y][u]; P4: rows along u -> the field pairs (sin along x, cos along x), phi = cos along x.
+// The fused form needs the bins pass's layout in the row pass: L a power of two (so that the row
+// blocks are kBlock threads).
+static bool bins_fusable(const RegionDev& r) {
+    return three_pass(r) && (r.cnt_x & (r.cnt_x - 1)) == 0 && row_threads(r.cnt_x) * rows_of_block(r.cnt_x) == kBlock;
+}
+
+void launch_bins_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, double* partials, const int* skip) {
+    if (!bins_fusable(r)) {
+        launch_bin_density(s, r, partials);
+        launch_solve(s, r, sb, false, skip);
+        return;
+    }
+    solve_three_pass(s, r, sb, false, skip, partials);
+}
+
 void launch_solve(hipStream_t s, const RegionDev& r, const SolveBufs& sb, bool want_phi, const int* skip) {
     if (three_pass(r)) {
         solve_three_pass(s, r, sb, want_phi, skip);

@@ -629,7 +629,10 @@ struct gpl_ctx {
             mark(F_SCATTER);
             gplk::launch_density_scatter(stream, r.dev, cells, d_global_adds);
             mark_end();
-            bins_and_solve(r, int(b), false);  // finished with the trial's gradient sums
+            // the overflow sums finished with the trial's gradient sums; while profiling the bins
+            // pass and the solve run apart (one family each), else fused (the same bits)
+            if (profiling) bins_and_solve(r, int(b), false);
+            else gplk::launch_bins_solve(stream, r.dev, r.sb, r.partials);
         }
         if (ovl) {
             std::swap(stream, aux);
@@ -738,8 +741,7 @@ struct gpl_ctx {
         for (size_t b = 0; b < R; ++b) {
             Region& r = rg[b];
             gplk::launch_density_scatter(s1, ctl_dv[b], cells, d_global_adds);
-            gplk::launch_bin_density(s1, ctl_dv[b], r.partials);
-            gplk::launch_solve(s1, ctl_dv[b], r.sb, false, &d_ctl->h.reg[b].hot.flags);
+            gplk::launch_bins_solve(s1, ctl_dv[b], r.sb, r.partials, &d_ctl->h.reg[b].hot.flags);
         }
         if (side) check(hipEventRecord(ev_join, aux), "event");
         const bool big = side && nets.num_small < nnets;

@@ -286,15 +286,19 @@ def test_loop_is_deterministic(c1):
 @pytest.mark.parametrize("cfg", ["c1", "c2"])
 def test_launch_modes_give_the_same_placement(cfg):
     """A trial as one HIP graph (replayed every trial, with the loop's eager work -- re-sorts,
-    snapshots, statistics -- between replays), as kernels on the side streams, and as kernels on
-    one stream: the same iterations, rows and placement."""
+    snapshots, statistics -- between replays), as kernels on the side streams, as kernels on one
+    stream, and profiled (the bins pass apart from the solve's first row pass, where it runs fused
+    otherwise): the same iterations, rows and placement."""
     d = G.Design.from_config(cfg)
     d.initial_place()
     d.init_nesterov_place()
     out = []
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, "profiled"):
         g = G.NesterovGPU(d)
-        g.set_launch_mode(mode)
+        if mode == "profiled":
+            g.set_profiling(True)
+        else:
+            g.set_launch_mode(mode)
         it = g.do_nesterov_place()
         out.append((it, g.history(), g.placed_locations(), g.stats().hpwl))
         g.close()
