@@ -253,6 +253,25 @@ def place_concurrently(contexts, max_workers=8, launch_mode=2):
         return list(ex.map(lambda g: g.do_nesterov_place(), contexts))
 
 
+def place_batched(contexts):
+    """do_nesterov_place on each of several independent contexts in lockstep (gpl_place_batch):
+    each iteration's first trial of all of them as one launch per kernel; returns the iteration
+    counts in order.  Each context's result is its own do_nesterov_place's."""
+    if not contexts:
+        return []
+    n = len(contexts)
+    hs = (C.c_void_p * n)(*[g._h for g in contexts])
+    its = (C.c_int32 * n)()
+    rc = _ffi.lib().gpl_place_batch(hs, n, its)
+    if rc != GPL_OK:
+        for g in contexts:
+            msg = _ffi.lib().gpl_last_error(g._h)
+            if msg:
+                raise GplError(rc, f"gpl_place_batch: {msg.decode(errors='replace')}")
+        raise GplError(rc, "gpl_place_batch")
+    return list(its)
+
+
 def rudy_wire_width(pitches):
     """Rudy::new's wire width (route_base.rs:143-152) from the routing layers' pitches."""
     p = np.ascontiguousarray(pitches, np.int32)

@@ -159,6 +159,7 @@ SIGNATURES = [
     ("gpl_init_nesterov_loop", i32, [C.c_void_p]),
     ("gpl_run_nesterov_loop", i32, [C.c_void_p, i32, P(i32)]),
     ("gpl_do_nesterov_place", i32, [C.c_void_p, P(i32)]),
+    ("gpl_place_batch", i32, [P(C.c_void_p), i32, P(i32)]),
     ("gpl_continue_nesterov_place", i32, [C.c_void_p, f32, i32, i32, P(i32)]),
     ("gpl_get_stats", i32, [C.c_void_p, P(Stats)]),
     ("gpl_loop_begin", i32, [C.c_void_p, i32]),

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace gplk {
 
@@ -477,6 +478,32 @@ hipError_t set_update_node(hipGraphExec_t e, const TrialNode& n, const RegionDev
                            unsigned long long* zero);
 hipError_t set_scatter_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c,
                             unsigned long long* global_adds);
+// ---- one trial of several placements of one shape as one launch per kernel ------------------
+// While a recorder is set on the calling thread, the trial's launchers (the update, the density
+// scatter, the solve's row passes with the bins pass fused, the WA kernels, the gradient, the
+// finishing reduction) record their kernel variants, grids and arguments instead of launching; a
+// launch with no grouped form (any other launcher, or a variant without one) marks the record
+// failed and launches nothing.  launch_grouped runs records of the same sequence as one launch
+// per kernel, whose blocks take the records' arguments in turn (record j: blocks j * grid ..
+// (j + 1) * grid - 1), after one copy of the arguments to the device.
+constexpr int kRecArgBytes = 768;
+struct LaunchRec {
+    int kind, grid, block;
+    unsigned lds;
+    int bytes;
+    alignas(16) unsigned char args[kRecArgBytes];
+};
+struct TrialRecorder {
+    std::vector<LaunchRec> recs;
+    bool failed = false;
+};
+void set_recorder(TrialRecorder* r);
+bool same_sequence(const TrialRecorder& a, const TrialRecorder& b);
+size_t grouped_arg_bytes(const TrialRecorder& r, int n);  // the staging bytes for n records like r
+// recs[0 .. n): records of one sequence; host_stage (pinned) and dev_stage hold
+// grouped_arg_bytes(*recs[0], n) bytes; the host stage must not change until the copy is done.
+hipError_t launch_grouped(hipStream_t s, const TrialRecorder* const* recs, int n, unsigned char* host_stage, unsigned char* dev_stage);
+
 hipError_t set_wa_node(hipGraphExec_t e, const TrialNode& n, const NetsDev& nd, const CellsDev& c, float cx, float cy,
                        unsigned long long* hpwl);
 hipError_t set_gradient_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c, const NetsDev& nd,

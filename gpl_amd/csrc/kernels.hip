@@ -10,6 +10,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -188,7 +189,7 @@ __device__ __forceinline__ void nesterov_point(float2 a, float2 sm, float2 cu, f
 // (updateGCellDensityCenterLocation, nesterov.rs:2139-2153) and the cell's centre, where its pins
 // sit at centre + offset (updatePinLocation, 2268-2276) for the WA kernels.  The first thread
 // also clears the trial's HPWL counter.
-__global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c, PlaceArgs pa) {
+__device__ __forceinline__ void place_update_body(RegionDev r, CellsDev c, PlaceArgs pa, const unsigned bid_, const unsigned nblk_) {
     int n = r.cell_end - r.cell_begin;
     if (r.ctl) {
         // a device-driven trial: the step, the momentum and the rotated slots from the control
@@ -206,8 +207,8 @@ __global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c
         if (h.flags & 1) n = 0;
     }
     const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
-    if (pa.zero && blockIdx.x == 0 && threadIdx.x == 0) *pa.zero = 0ull;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    if (pa.zero && bid_ == 0 && threadIdx.x == 0) *pa.zero = 0ull;
+    for (int k = bid_ * blockDim.x + threadIdx.x; k < n; k += nblk_ * blockDim.x) {
         const int g = r.cell_begin + k;
         int4 d = c.dbox[g];
         const uint8_t f = c.flags[g];
@@ -226,6 +227,10 @@ __global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c
     }
 }
 
+__global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c, PlaceArgs pa) {
+    place_update_body(r, c, pa, blockIdx.x, gridDim.x);
+}
+
 // V: the LDS tables' sum type.  32 bits when a chunk's sum in one bin cannot reach 2^32 (every
 // overlap, times its scale and target, is at most the bin's area, so kChunk * the largest bin area
 // bounds it; the host checks that), which halves the tables and doubles the blocks per CU; else 64.
@@ -239,7 +244,7 @@ __global__ void __launch_bounds__(kBlock) k_place_update(RegionDev r, CellsDev c
 // would otherwise send the whole chunk to the hash table.  Otherwise the hash table.
 // CH: the cells a block takes at a time (a tile shape the autotuner picks per region: 256 or 512).
 template <int MODE, typename V, int CH>
-__global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev c, PlaceArgs pa) {
+__device__ __forceinline__ void place_scatter_body(RegionDev r, CellsDev c, PlaceArgs pa, const unsigned bid_, const unsigned nblk_) {
     if (r.ctl && (r.ctl->hot.flags & 1)) return;
     constexpr int kChunk = CH;
     constexpr int kCellsPerThread = CH / kBlock;
@@ -261,10 +266,10 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
     const float lx = (float)r.lx, ly = (float)r.ly, ux = (float)r.ux, uy = (float)r.uy;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (threadIdx.x == 0) s_adds = 0;
-    if (pa.zero && blockIdx.x == 0 && threadIdx.x == 0) *pa.zero = 0ull;
+    if (pa.zero && bid_ == 0 && threadIdx.x == 0) *pa.zero = 0ull;
     const int chunks = (n + kChunk - 1) / kChunk;
-    const int per = (chunks + gridDim.x - 1) / gridDim.x;
-    const int L = xcd_block(blockIdx.x, gridDim.x);
+    const int per = (chunks + nblk_ - 1) / nblk_;
+    const int L = xcd_block(bid_, nblk_);
     const int ch_end = min((L + 1) * per, chunks);
     for (int ch = L * per; ch < ch_end; ++ch) {
         const int base = ch * kChunk;
@@ -471,6 +476,11 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev 
         __syncthreads();
     }
     if (threadIdx.x == 0 && s_adds && pa.global_adds) atomicAdd(pa.global_adds, (unsigned long long)s_adds);
+}
+
+template <int MODE, typename V, int CH>
+__global__ void __launch_bounds__(kBlock) k_place_scatter(RegionDev r, CellsDev c, PlaceArgs pa) {
+    place_scatter_body<MODE, V, CH>(r, c, pa, blockIdx.x, gridDim.x);
 }
 
 // The finishing reduction of a kernel's FP64 block partials into *out, in a fixed order (one
@@ -940,7 +950,7 @@ struct RowArgs {
 // side: the second transform's two buffers, L float2 per row], [ROW_SPECTRUM_POISSON: the spectrum, L floats],
 // then [TL: tw h, th h + 1, dw L float2, the element wavenumbers L floats].
 template <int MODE, bool TL, int LOGL>
-__global__ void __launch_bounds__(256) k_rows(RowArgs a) {
+__device__ __forceinline__ void rows_body(RowArgs a, const unsigned bid_, const unsigned nblk_) {
     if (a.skip && (*a.skip & 1)) return;
     __builtin_amdgcn_s_setprio(2);  // on the trial's critical chain (as the scatter)
     extern __shared__ float2 lds2[];
@@ -987,7 +997,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
         if constexpr (TL) return wcol_l[v];
         else return a.wcol[v];
     };
-    const int lb = gridDim.x % kXcds == 0 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int lb = nblk_ % kXcds == 0 ? xcd_block(bid_, nblk_) : (int)bid_;
     const int row = lb * rpb + sub;
     const bool active = row < a.nrows;
     const size_t off = (size_t)(active ? row : 0) * L;
@@ -1149,6 +1159,11 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
         dct3_row(R, active, [&](int v) { return X[v]; }, false, [&](int j, float y) { emit(a.out0, st0, j, y); });
         flush(a.out0, false);
     }
+}
+
+template <int MODE, bool TL, int LOGL>
+__global__ void __launch_bounds__(256) k_rows(RowArgs a) {
+    rows_body<MODE, TL, LOGL>(a, blockIdx.x, gridDim.x);
 }
 
 // Tiled transpose: in [R][C] -> out [C][R].
@@ -1353,8 +1368,8 @@ __device__ __forceinline__ void pin_grad_kept(const NetsDev& n, int q, int2 p, f
 constexpr int kBigBlock = 1024;
 constexpr int kBigRegs = 8;
 template <bool SUMS>
-__global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
-                                                      int write_rec) {
+__device__ __forceinline__ void wa_big_body(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
+                                                      int write_rec, const unsigned bid_, const unsigned nblk_) {
     if (n.cg) {
         const int halt = n.cg->halt;
         cx = n.cg->wl_coef_x;
@@ -1365,7 +1380,7 @@ __global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, flo
     __shared__ double shd[8][kBigBlock / 64];
     __shared__ float shr[8];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int e = n.num_small + blockIdx.x;
+    const int e = n.num_small + bid_;
     const int s = n.start[e], t = n.start[e + 1];
     const int rest = s + kBigRegs * kBigBlock;  // pins from here on are loaded again in each pass
     int2 pos[kBigRegs];
@@ -1463,6 +1478,12 @@ __global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, flo
     if (threadIdx.x == 0 && ux >= lx) atomicAdd(hpwl, (unsigned long long)((long long)ux - lx + (long long)uy - ly));
 }
 
+template <bool SUMS>
+__global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
+                                                      int write_rec) {
+    wa_big_body<SUMS>(n, c, cx, cy, hpwl, write_rec, blockIdx.x, gridDim.x);
+}
+
 // k_wa_tiles: the small nets (below num_small) in tiles of consecutive nets (the nets whose first
 // pin lies in one range of kTileSpan pins: at most kTilePins pins; XCD x takes a contiguous
 // eighth of the tiles).  A block first stages its
@@ -1476,8 +1497,8 @@ __global__ void __launch_bounds__(kBigBlock) k_wa_big(NetsDev n, CellsDev c, flo
 constexpr int kKeepPins = 8;  // pins whose four exponentials are kept in registers
 constexpr int kTileStage = kTilePins / kBlock;
 template <bool SUMS>
-__global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
-                                                     int write_rec) {
+__device__ __forceinline__ void wa_tiles_body(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
+                                                     int write_rec, const unsigned bid_, const unsigned nblk_) {
     if (n.cg) {
         const int halt = n.cg->halt;
         cx = n.cg->wl_coef_x;
@@ -1488,8 +1509,8 @@ __global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, floa
     __shared__ int scq[SUMS ? kTilePins : 1];
     __shared__ unsigned long long shh[kBlock / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int x = blockIdx.x % kXcds;
-    const int t = x * n.tile_blocks + (int)(blockIdx.x / kXcds);
+    const int x = bid_ % kXcds;
+    const int t = x * n.tile_blocks + (int)(bid_ / kXcds);
     unsigned long long h = 0;
     if (t < n.num_tiles) {
         const int e0 = n.tile[t], e1 = n.tile[t + 1];
@@ -1565,6 +1586,12 @@ __global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, floa
     }
 }
 
+template <bool SUMS>
+__global__ void __launch_bounds__(kBlock) k_wa_tiles(NetsDev n, CellsDev c, float cx, float cy, unsigned long long* hpwl,
+                                                     int write_rec) {
+    wa_tiles_body<SUMS>(n, c, cx, cy, hpwl, write_rec, blockIdx.x, gridDim.x);
+}
+
 constexpr int kGradPins = 6;
 // updateGradients (nesterov.rs:2455-2489) per cell: the WA gradient over the cell's pins in
 // their order (each pin's weighted term from the WA kernel); the density gradient over the bins its box
@@ -1572,9 +1599,9 @@ constexpr int kGradPins = 6;
 // and, for a trial, the squared coordinate and gradient distances (nesterovUpdateStepLength,
 // 2755-2756) to a_slp/a_sum.
 // (six waves per SIMD: 76 registers, no spill)
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float cy, float penalty, float2* wl_out, float2* den_out,
+__device__ __forceinline__ void gradient_body(RegionDev r, CellsDev c, NetsDev nt, float cx, float cy, float penalty, float2* wl_out, float2* den_out,
                            float2* sum_out, const float2* a_slp, const float2* b_slp, const float2* a_sum, double* partials,
-                           int nan_inject) {
+                           int nan_inject, const unsigned bid_, const unsigned nblk_) {
     int n = r.cell_end - r.cell_begin;
     if (r.ctl) {
         // a device-driven trial: the penalty and the rotated slots from the control record's line
@@ -1594,7 +1621,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
     __shared__ float2 spg[kGradStage];
     double s_wl = 0.0, s_den = 0.0, s_c = 0.0, s_g = 0.0;
     // the block's cells in groups of kBlock consecutive cells (the trip count is the block's own)
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    for (int base = bid_ * blockDim.x; base < n; base += nblk_ * blockDim.x) {
         // the group's pin terms are one contiguous range (cells in order, each cell's pins in
         // order): read coalesced into LDS when it fits
         const int qb = c.pin_start[r.cell_begin + base];
@@ -1709,11 +1736,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
     double t3 = block_sum(s_g, sh);
     if (threadIdx.x == 0) {
         if (nan_inject) t0 = __longlong_as_double(0x7ff8000000000000ll);
-        partials[2 * kPartials + blockIdx.x] = t0;
-        partials[3 * kPartials + blockIdx.x] = t1;
-        partials[4 * kPartials + blockIdx.x] = t2;
-        partials[5 * kPartials + blockIdx.x] = t3;
+        partials[2 * kPartials + bid_] = t0;
+        partials[3 * kPartials + bid_] = t1;
+        partials[4 * kPartials + bid_] = t2;
+        partials[5 * kPartials + bid_] = t3;
     }
+}
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_gradient(RegionDev r, CellsDev c, NetsDev nt, float cx, float cy, float penalty, float2* wl_out, float2* den_out,
+                           float2* sum_out, const float2* a_slp, const float2* b_slp, const float2* a_sum, double* partials,
+                           int nan_inject) {
+    gradient_body(r, c, nt, cx, cy, penalty, wl_out, den_out, sum_out, a_slp, b_slp, a_sum, partials, nan_inject, blockIdx.x, gridDim.x);
 }
 
 // Squared distances between coordinate sets (getDistance's sums, nesterov.rs:319) as partials 4
@@ -2301,6 +2334,82 @@ static int grid_for(size_t n) {
     return round_xcds((int)b);
 }
 
+// ---- the trial recorder (set_recorder): the grouped kernels' argument records --------------------
+static thread_local TrialRecorder* t_rec = nullptr;
+void set_recorder(TrialRecorder* r) { t_rec = r; }
+
+// record kinds: the kernel and its template variant
+enum RecKind {
+    RK_UPDATE = 1,
+    RK_WA_TILES = 2,
+    RK_WA_BIG = 3,
+    RK_GRADIENT = 4,
+    RK_FINISH = 5,
+    RK_SCATTER = 16,  // + 1 for 32-bit LDS sums, + 2 for 256-cell chunks
+    RK_ROWS = 64,     // + 4 mode + (0, 1, 2: L of 256, 512, 1024; 3: any L), the tables in LDS
+};
+
+struct UpdateArgsG {
+    RegionDev r;
+    CellsDev c;
+    PlaceArgs pa;
+};
+struct WaArgsG {
+    NetsDev n;
+    CellsDev c;
+    float cx, cy;
+    unsigned long long* hpwl;
+    int write_rec;
+};
+struct GradArgsG {
+    RegionDev r;
+    CellsDev c;
+    NetsDev nt;
+    float cx, cy, penalty;
+    float2* wl_out;
+    float2* den_out;
+    float2* sum_out;
+    const float2* a_slp;
+    const float2* b_slp;
+    const float2* a_sum;
+    double* partials;
+    int nan_inject;
+};
+struct FinishArgsG {
+    const double* partials;
+    int nb_bins, nb_cells;
+    Sums* out;
+    const unsigned long long* hpwl;
+    Sums* host_out;
+    const Sums* dev_tail;
+    Sums* host_tail;
+    unsigned long long* dev_seq;
+    unsigned long long* host_seq;
+    const int* skip;
+};
+
+// true when recording (the launch is recorded, not made)
+template <typename A>
+static bool rec_launch(int kind, int grid, int block, unsigned lds, const A& a) {
+    static_assert(sizeof(A) <= (size_t)kRecArgBytes, "a grouped kernel's arguments fit a record");
+    if (!t_rec) return false;
+    LaunchRec q;
+    q.kind = kind;
+    q.grid = grid;
+    q.block = block;
+    q.lds = lds;
+    q.bytes = (int)sizeof(A);
+    std::memcpy(q.args, &a, sizeof(A));
+    t_rec->recs.push_back(q);
+    return true;
+}
+// true when recording: a launch with no grouped form, so the record fails (nothing is launched)
+static bool rec_refuse() {
+    if (!t_rec) return false;
+    t_rec->failed = true;
+    return true;
+}
+
 int block_partials_bins(const RegionDev& r) {
     const int rpb = rows_of_block(r.cnt_x);
     return (r.cnt_y + rpb - 1) / rpb;
@@ -2323,12 +2432,18 @@ static int cells_grid(size_t n) {
 int block_partials_cells(const RegionDev& r) { return cells_grid((size_t)(r.cell_end - r.cell_begin)); }
 
 void launch_zero_bins(hipStream_t s, const RegionDev& r) {
+    if (rec_refuse()) return;
     size_t M = (size_t)r.cnt_x * r.cnt_y;
     hipLaunchKernelGGL(k_zero_bins, dim3(grid_for(M)), dim3(kBlock), 0, s, r.inst, r.filler, M);
 }
 
 template <int MODE, int CH>
 static void place_scatter_ch(hipStream_t s, const RegionDev& r, const CellsDev& c, const PlaceArgs& pa, int grid) {
+    if (t_rec) {
+        if (MODE != PLACE_STORED) rec_refuse();
+        else rec_launch(RK_SCATTER + (r.narrow ? 1 : 0) + (CH == 256 ? 2 : 0), grid, kBlock, 0, UpdateArgsG{r, c, pa});
+        return;
+    }
     if (r.narrow) hipLaunchKernelGGL((k_place_scatter<MODE, unsigned, CH>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
     else hipLaunchKernelGGL((k_place_scatter<MODE, unsigned long long, CH>), dim3(grid), dim3(kBlock), 0, s, r, c, pa);
 }
@@ -2393,6 +2508,7 @@ void launch_update(hipStream_t s, const RegionDev& r, const CellsDev& c, const f
         return;
     }
     const PlaceArgs pa = update_args(cur_slp, cur_sum, cur, next, next_slp, step, coeff, zero);
+    if (rec_launch(RK_UPDATE, update_grid(r), kBlock, 0, UpdateArgsG{r, c, pa})) return;
     hipLaunchKernelGGL(k_place_update, dim3(update_grid(r)), dim3(kBlock), 0, s, r, c, pa);
 }
 
@@ -2403,6 +2519,7 @@ void launch_density_scatter(hipStream_t s, const RegionDev& r, const CellsDev& c
 }
 
 void launch_bin_density(hipStream_t s, const RegionDev& r, double* partials, bool clear) {
+    if (rec_refuse()) return;
     hipLaunchKernelGGL(k_bin_density, dim3(block_partials_bins(r)), dim3(kBlock), 0, s, r, partials, clear ? 1 : 0);
 }
 
@@ -2421,6 +2538,7 @@ static int ilog2(int L) {
 }
 
 static void transpose(hipStream_t s, const float* in, float* out, int R, int C) {
+    if (rec_refuse()) return;
     dim3 g((C + 31) / 32, (R + 31) / 32), b(32, 8);
     hipLaunchKernelGGL(k_transpose, g, b, 0, s, in, out, R, C);
 }
@@ -2445,6 +2563,15 @@ static void launch_rows(hipStream_t s, int mode, RowArgs a) {
     const size_t tab_lds = sizeof(float2) * (2 * (size_t)a.L + 1) + (wav ? 4 * (size_t)a.L : 0);
     const bool tl = rows_lds + tab_lds <= 64 * 1024;
     const size_t lds = rows_lds + (tl ? tab_lds : 0);
+    if (t_rec) {
+        // the variant launch_rows takes below: L of 1024, 512, 256, or any (its fields) with the
+        // tables in LDS
+        const bool grouped = tl && (mode == ROW_BINS_DCT2 || mode == ROW_SPECTRUM_POISSON || mode == ROW_INV_FIELDS);
+        const int v = a.L == 256 ? 0 : a.L == 512 ? 1 : a.L == 1024 ? 2 : 3;
+        if (!grouped) rec_refuse();
+        else rec_launch(RK_ROWS + 4 * mode + v, grid, block, (unsigned)lds, a);
+        return;
+    }
 #define GPL_ROWS(M)                                                                                        \
     do {                                                                                                   \
         if (tl && a.L == 1024) hipLaunchKernelGGL((k_rows<M, true, 10>), dim3(grid), dim3(block), lds, s, a); \
@@ -2641,6 +2768,16 @@ template <bool SUMS>
 static void wa_launch(hipStream_t s, hipStream_t s_big, const NetsDev& n, const CellsDev& c, float coef_x, float coef_y,
                       unsigned long long* hpwl, int write_rec) {
     const int nbig = n.num_nets - n.num_small;
+    if (t_rec) {
+        if (!SUMS) {
+            rec_refuse();
+            return;
+        }
+        const WaArgsG a{n, c, coef_x, coef_y, hpwl, write_rec};
+        if (nbig > 0) rec_launch(RK_WA_BIG, nbig, kBigBlock, 0, a);
+        if (n.num_small > 0) rec_launch(RK_WA_TILES, kXcds * n.tile_blocks, kBlock, 0, a);
+        return;
+    }
     if (nbig > 0) hipLaunchKernelGGL(k_wa_big<SUMS>, dim3(nbig), dim3(kBigBlock), 0, s_big, n, c, coef_x, coef_y, hpwl, write_rec);
     if (n.num_small > 0)
         hipLaunchKernelGGL(k_wa_tiles<SUMS>, dim3(kXcds * n.tile_blocks), dim3(kBlock), 0, s, n, c, coef_x, coef_y, hpwl, write_rec);
@@ -2658,6 +2795,9 @@ void launch_hpwl(hipStream_t s, const NetsDev& n, const CellsDev& c, unsigned lo
 void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const NetsDev& n, float coef_x, float coef_y,
                      float penalty, float2* wl, float2* den, float2* sum, const float2* a_slp, const float2* b_slp,
                      const float2* a_sum, double* partials, int nan_inject) {
+    if (rec_launch(RK_GRADIENT, block_partials_cells(r), kBlock, 0,
+                   GradArgsG{r, c, n, coef_x, coef_y, penalty, wl, den, sum, a_slp, b_slp, a_sum, partials, nan_inject}))
+        return;
     hipLaunchKernelGGL(k_gradient, dim3(block_partials_cells(r)), dim3(kBlock), 0, s, r, c, n, coef_x, coef_y, penalty, wl, den, sum,
                        a_slp, b_slp, a_sum, partials, nan_inject);
 }
@@ -2665,6 +2805,13 @@ void launch_gradient(hipStream_t s, const RegionDev& r, const CellsDev& c, const
 void launch_finish(hipStream_t s, const double* partials, int nparts_bins, int nparts_cells, Sums* out, const unsigned long long* hpwl,
                    Sums* host_out, const Sums* dev_tail, Sums* host_tail, unsigned long long* dev_seq, unsigned long long* host_seq,
                    const int* skip, const CtlArgs& ctl) {
+    if (t_rec) {
+        if (ctl.g) rec_refuse();
+        else
+            rec_launch(RK_FINISH, 1, kFinishBlock, 0,
+                       FinishArgsG{partials, nparts_bins, nparts_cells, out, hpwl, host_out, dev_tail, host_tail, dev_seq, host_seq, skip});
+        return;
+    }
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishBlock), 0, s, partials, nparts_bins, nparts_cells, out, hpwl, host_out, dev_tail,
                        host_tail, dev_seq, host_seq, skip, ctl);
 }
@@ -2970,7 +3117,129 @@ void launch_remap_pcell(hipStream_t s, int P, int32_t* pcell, const int32_t* inv
 }
 
 void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v) {
+    if (rec_refuse()) return;
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, p, v);
+}
+
+// ---- the grouped kernels: record g of a launch takes blocks g * per .. (g + 1) * per - 1 ----------
+__global__ void __launch_bounds__(kBlock) k_group_update(const UpdateArgsG* t, unsigned per) {
+    const unsigned g = blockIdx.x / per;
+    place_update_body(t[g].r, t[g].c, t[g].pa, blockIdx.x - g * per, per);
+}
+
+template <typename V, int CH>
+__global__ void __launch_bounds__(kBlock) k_group_scatter(const UpdateArgsG* t, unsigned per) {
+    const unsigned g = blockIdx.x / per;
+    place_scatter_body<PLACE_STORED, V, CH>(t[g].r, t[g].c, t[g].pa, blockIdx.x - g * per, per);
+}
+
+template <int MODE, int LOGL>
+__global__ void __launch_bounds__(256) k_group_rows(const RowArgs* t, unsigned per) {
+    const unsigned g = blockIdx.x / per;
+    rows_body<MODE, true, LOGL>(t[g], blockIdx.x - g * per, per);
+}
+
+__global__ void __launch_bounds__(kBlock) k_group_wa_tiles(const WaArgsG* t, unsigned per) {
+    const unsigned g = blockIdx.x / per;
+    wa_tiles_body<true>(t[g].n, t[g].c, t[g].cx, t[g].cy, t[g].hpwl, t[g].write_rec, blockIdx.x - g * per, per);
+}
+
+__global__ void __launch_bounds__(kBigBlock) k_group_wa_big(const WaArgsG* t, unsigned per) {
+    const unsigned g = blockIdx.x / per;
+    wa_big_body<true>(t[g].n, t[g].c, t[g].cx, t[g].cy, t[g].hpwl, t[g].write_rec, blockIdx.x - g * per, per);
+}
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_group_gradient(const GradArgsG* t, unsigned per) {
+    const unsigned g = blockIdx.x / per;
+    const GradArgsG& a = t[g];
+    gradient_body(a.r, a.c, a.nt, a.cx, a.cy, a.penalty, a.wl_out, a.den_out, a.sum_out, a.a_slp, a.b_slp, a.a_sum, a.partials, a.nan_inject,
+                  blockIdx.x - g * per, per);
+}
+
+__global__ void __launch_bounds__(kFinishBlock) k_group_finish(const FinishArgsG* t) {
+    const FinishArgsG& a = t[blockIdx.x];
+    if (a.skip && (*a.skip & 1)) return;
+    finish_body(a.partials, a.nb_bins, a.nb_cells, a.out, a.hpwl, a.host_out, a.dev_tail, a.host_tail, a.dev_seq, a.host_seq, nullptr);
+}
+
+bool same_sequence(const TrialRecorder& a, const TrialRecorder& b) {
+    if (a.failed || b.failed || a.recs.size() != b.recs.size()) return false;
+    for (size_t i = 0; i < a.recs.size(); ++i) {
+        const LaunchRec &p = a.recs[i], &q = b.recs[i];
+        if (p.kind != q.kind || p.grid != q.grid || p.block != q.block || p.lds != q.lds || p.bytes != q.bytes) return false;
+    }
+    return true;
+}
+
+static size_t round16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+size_t grouped_arg_bytes(const TrialRecorder& r, int n) {
+    size_t b = 0;
+    for (const LaunchRec& q : r.recs) b += round16((size_t)q.bytes * (size_t)n);
+    return b;
+}
+
+hipError_t launch_grouped(hipStream_t s, const TrialRecorder* const* recs, int n, unsigned char* host_stage, unsigned char* dev_stage) {
+    if (n <= 0) return hipSuccess;
+    const std::vector<LaunchRec>& r0 = recs[0]->recs;
+    for (int j = 0; j < n; ++j)
+        if (!same_sequence(*recs[0], *recs[j])) return hipErrorInvalidValue;
+    std::vector<size_t> off(r0.size());
+    size_t at = 0;
+    for (size_t i = 0; i < r0.size(); ++i) {
+        off[i] = at;
+        const size_t b = (size_t)r0[i].bytes;
+        for (int j = 0; j < n; ++j) std::memcpy(host_stage + at + (size_t)j * b, recs[j]->recs[i].args, b);
+        at += round16(b * (size_t)n);
+    }
+    hipError_t e = hipMemcpyAsync(dev_stage, host_stage, at, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    for (size_t i = 0; i < r0.size(); ++i) {
+        const LaunchRec& q = r0[i];
+        const unsigned per = (unsigned)q.grid;
+        const dim3 grid(per * (unsigned)n), block(q.block);
+        const void* p = dev_stage + off[i];
+        const int k = q.kind;
+        if (k == RK_UPDATE) {
+            hipLaunchKernelGGL(k_group_update, grid, block, 0, s, (const UpdateArgsG*)p, per);
+        } else if (k >= RK_SCATTER && k < RK_SCATTER + 4) {
+            const UpdateArgsG* t = (const UpdateArgsG*)p;
+            switch (k - RK_SCATTER) {
+            case 0: hipLaunchKernelGGL((k_group_scatter<unsigned long long, kPlaceChunk>), grid, block, 0, s, t, per); break;
+            case 1: hipLaunchKernelGGL((k_group_scatter<unsigned, kPlaceChunk>), grid, block, 0, s, t, per); break;
+            case 2: hipLaunchKernelGGL((k_group_scatter<unsigned long long, 256>), grid, block, 0, s, t, per); break;
+            default: hipLaunchKernelGGL((k_group_scatter<unsigned, 256>), grid, block, 0, s, t, per); break;
+            }
+        } else if (k >= RK_ROWS) {
+            const RowArgs* t = (const RowArgs*)p;
+            const int mode = (k - RK_ROWS) / 4, v = (k - RK_ROWS) % 4;
+#define GPL_GROUP_ROWS(M)                                                                                        \
+    do {                                                                                                         \
+        if (v == 2) hipLaunchKernelGGL((k_group_rows<M, 10>), grid, block, q.lds, s, t, per);                    \
+        else if (v == 1) hipLaunchKernelGGL((k_group_rows<M, 9>), grid, block, q.lds, s, t, per);                \
+        else if (v == 0) hipLaunchKernelGGL((k_group_rows<M, 8>), grid, block, q.lds, s, t, per);                \
+        else hipLaunchKernelGGL((k_group_rows<M, 0>), grid, block, q.lds, s, t, per);                            \
+    } while (0)
+            if (mode == ROW_BINS_DCT2) GPL_GROUP_ROWS(ROW_BINS_DCT2);
+            else if (mode == ROW_SPECTRUM_POISSON) GPL_GROUP_ROWS(ROW_SPECTRUM_POISSON);
+            else if (mode == ROW_INV_FIELDS) GPL_GROUP_ROWS(ROW_INV_FIELDS);
+            else return hipErrorInvalidValue;
+#undef GPL_GROUP_ROWS
+        } else if (k == RK_WA_TILES) {
+            hipLaunchKernelGGL(k_group_wa_tiles, grid, block, 0, s, (const WaArgsG*)p, per);
+        } else if (k == RK_WA_BIG) {
+            hipLaunchKernelGGL(k_group_wa_big, grid, block, 0, s, (const WaArgsG*)p, per);
+        } else if (k == RK_GRADIENT) {
+            hipLaunchKernelGGL(k_group_gradient, grid, block, 0, s, (const GradArgsG*)p, per);
+        } else if (k == RK_FINISH) {
+            hipLaunchKernelGGL(k_group_finish, dim3((unsigned)n), block, 0, s, (const FinishArgsG*)p);
+        } else {
+            return hipErrorInvalidValue;
+        }
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // ---- the trial graph's argument-dependent kernel nodes ---------------------------------------

@@ -1114,14 +1114,15 @@ struct gpl_ctx {
     }
 
     // `resume` > 0: the iteration's first resume - 1 trials ran on the device and trial `resume`
-    // did too, up to its gradient sums (in d_sums): the host takes it from there.
-    void do_back_tracking(float coeff, int resume = 0) {
+    // did too, up to its gradient sums (in d_sums, and with stats_on_host in h_sums already): the
+    // host takes it from there.
+    void do_back_tracking(float coeff, int resume = 0, bool stats_on_host = false) {
         trials = resume > 0 ? resume - 1 : 0;
         for (int nbt = trials; nbt < 10; ++nbt) {
             ++trials;
             num_region_diverged = 0;
             if (resume > 0 && nbt == resume - 1) {
-                sync_stats();
+                if (!stats_on_host) sync_stats();
                 gradients_read(0, int(rg.size()), SLOT_NEXT, true);
             } else if (!trial_by_graph(coeff)) {
                 update_coordinates(coeff);
@@ -1606,7 +1607,8 @@ struct gpl_ctx {
 
     // Just before a shape's first re-sort the storage order is at its most decayed, where the
     // scatter's variants differ most: they are measured there once and that pick is kept (and
-    // cached for the shape, so later contexts start with it).
+    // cached for the shape, so later contexts start with it, and contexts that reach the point
+    // after another of the same shape take its pick).
     void autotune_before_resort() {
         if (!autotune) return;
         bool need = false;
@@ -1617,6 +1619,18 @@ struct gpl_ctx {
         for (size_t b = 0; b < rg.size(); ++b) {
             Region& r = rg[b];
             if (r.is_converged || tune[b].decayed) continue;
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                auto it = g_tune_cache.find(TuneKey{r.n, r.R.cnt_x, r.R.cnt_y, net_pin_count});
+                if (it != g_tune_cache.end() && it->second.decayed) {
+                    const Tune before = tune[b];
+                    tune[b] = it->second;
+                    tune[b].source = 2;
+                    apply_tune(int(b));
+                    if (tune[b].chunk != before.chunk || tune[b].narrow != before.narrow || tune[b].stage != before.stage) drop_graph();
+                    continue;
+                }
+            }
             tune_scatter(int(b), 2);
             tune[b].decayed = true;
             std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -2310,14 +2324,8 @@ struct gpl_ctx {
     void loop_step(int32_t n) {
         const int32_t stop_at = n >= INT32_MAX - ls.iter ? INT32_MAX : ls.iter + n;
         while (ls.active && !ls.done && ls.iter < stop_at) {
-            if (ls.iter >= max_iter) {
-                ls.done = true;
-                break;
-            }
-            float prev_a = ls.cur_a;
-            ls.cur_a = (float)((1.0 + std::sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
-            float coeff = (float)(((double)prev_a - 1.0) / (double)ls.cur_a);
-            iter_now = ls.iter;
+            float coeff = 0.0f;
+            if (!iter_begin(coeff)) break;
             int resume = 0;
             bool booked = false;  // the iteration's bookkeeping (up to the history row) is done
             if (ctl_usable()) {
@@ -2330,45 +2338,84 @@ struct gpl_ctx {
                     coeff = h_ctl->h.coeff;
                 }
             }
-            if (!booked) {
-                do_back_tracking(coeff, resume);
-                total_trials += trials;
-                for (auto& r : rg) adjust_phi(r);
-                if (num_region_diverged > 0) {
-                    ls.done = true;
-                    break;
-                }
-                np_update_next_iter(ls.iter);
-                push_history(ls.iter);
-                last_iter = ls.iter;
-                ++iters_run;
-            }
-            // is_routability_gpl_iter: each such iteration extends the limit by one
-            if (rt.on && rt.saved && average_overflow_unscaled > o.routability_end_overflow) {
-                rt.route_iters += 1;
-                max_iter += 1;
-            }
-            int act = np_is_diverged(&ls.cur_a);
-            if (act == 2) {
-                ls.done = true;
-                break;
-            }
-            if (act == 1) {
-                ++ls.iter;
-                continue;
-            }
-            if (rt.on) routability_step(ls.iter, &ls.cur_a);
-            size_t conv = 0;
-            for (size_t b = 0; b < rg.size(); ++b) conv += check_convergence(int(b), ls.iter) ? 1 : 0;
-            if (conv == rg.size()) {
-                converged = true;
-                t_converge = t_init + std::chrono::duration<double>(Clock::now() - ls.t0).count();
-                ls.done = true;
-                break;
-            }
-            maybe_resort(ls.iter);
-            ++ls.iter;
+            if (!iter_rest(coeff, resume, booked, false)) break;
         }
+    }
+
+    // An iteration's start: the iteration limit, the momentum (a_k) and its coefficient; false
+    // when the loop ends here.
+    bool iter_begin(float& coeff) {
+        if (ls.iter >= max_iter) {
+            ls.done = true;
+            return false;
+        }
+        float prev_a = ls.cur_a;
+        ls.cur_a = (float)((1.0 + std::sqrt(4.0 * (double)prev_a * (double)prev_a + 1.0)) * 0.5);
+        coeff = (float)(((double)prev_a - 1.0) / (double)ls.cur_a);
+        iter_now = ls.iter;
+        return true;
+    }
+
+    // The rest of the iteration: the back-tracking (from trial `resume` when the first trials ran
+    // elsewhere; stats_on_host: that trial's statistics are already in the pinned copy), the
+    // next-iteration bookkeeping unless `booked`, divergence, routability, convergence, the
+    // re-sort; false when the loop ends here.
+    bool iter_rest(float coeff, int resume, bool booked, bool stats_on_host) {
+        if (!booked) {
+            do_back_tracking(coeff, resume, stats_on_host);
+            total_trials += trials;
+            for (auto& r : rg) adjust_phi(r);
+            if (num_region_diverged > 0) {
+                ls.done = true;
+                return false;
+            }
+            np_update_next_iter(ls.iter);
+            push_history(ls.iter);
+            last_iter = ls.iter;
+            ++iters_run;
+        }
+        // is_routability_gpl_iter: each such iteration extends the limit by one
+        if (rt.on && rt.saved && average_overflow_unscaled > o.routability_end_overflow) {
+            rt.route_iters += 1;
+            max_iter += 1;
+        }
+        int act = np_is_diverged(&ls.cur_a);
+        if (act == 2) {
+            ls.done = true;
+            return false;
+        }
+        if (act == 1) {
+            ++ls.iter;
+            return true;
+        }
+        if (rt.on) routability_step(ls.iter, &ls.cur_a);
+        size_t conv = 0;
+        for (size_t b = 0; b < rg.size(); ++b) conv += check_convergence(int(b), ls.iter) ? 1 : 0;
+        if (conv == rg.size()) {
+            converged = true;
+            t_converge = t_init + std::chrono::duration<double>(Clock::now() - ls.t0).count();
+            ls.done = true;
+            return false;
+        }
+        maybe_resort(ls.iter);
+        ++ls.iter;
+        return true;
+    }
+
+    // One trial's kernels (the trial graph's sequence: the update, the scatter, the bins and the
+    // solve, the WA kernels, the gradients with the statistics and the completion signal to the
+    // host) into the thread's recorder, for a grouped launch with other placements'.
+    void record_trial(float coeff) {
+        finish_to_host = true;
+        signal_region = -1;
+        for (size_t b = 0; b < rg.size(); ++b)
+            if (!rg[b].is_converged) signal_region = int(b);
+        update_coordinates(coeff);
+        field();
+        wa_forward(false, true);
+        join();
+        launch_gradients(0, int(rg.size()), SLOT_NEXT, true);
+        finish_to_host = false;
     }
 
     int32_t loop_end(int32_t* iter_out) {
@@ -2900,6 +2947,164 @@ int32_t gpl_do_nesterov_place(gpl_ctx* c, int32_t* iter_out) {
     int32_t rc = gpl_init_nesterov_loop(c);
     if (rc < 0) return rc;
     return gpl_run_nesterov_loop(c, 0, iter_out);
+}
+
+// Independent placements in lockstep: each iteration's first trial of every context as one
+// launch per kernel (the contexts' records grouped by sequence), the rest of each iteration by its
+// own context; the contexts share one stream meanwhile.  Each context's result is its own
+// gpl_do_nesterov_place's.
+int32_t gpl_place_batch(gpl_ctx* const* cs, int32_t n, int32_t* iters) {
+    if (!cs || n <= 0) return GPL_ERR_ARG;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!cs[i]) return GPL_ERR_ARG;
+        for (int32_t j = 0; j < i; ++j)
+            if (cs[j] == cs[i]) return GPL_ERR_ARG;
+    }
+    bool plain = true;  // every context can take part (else each runs on its own)
+    for (int32_t i = 0; i < n; ++i) plain = plain && !cs[i]->rt.on && !cs[i]->profiling;
+    if (!plain) {
+        int32_t rc = GPL_OK;
+        for (int32_t i = 0; i < n; ++i) {
+            int32_t it = 0;
+            const int32_t r = gpl_do_nesterov_place(cs[i], &it);
+            if (iters) iters[i] = it;
+            if (r < 0 && rc == GPL_OK) rc = r;
+        }
+        return rc;
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t rc = gpl_init_nesterov_loop(cs[i]);
+        if (rc < 0) return rc;
+    }
+    struct Saved {
+        hipStream_t stream, aux, aux2;
+        int launch_mode;
+        bool overlap, device_trials;
+    };
+    std::vector<Saved> saved(n);
+    hipStream_t bs = nullptr;
+    unsigned char* hstage = nullptr;
+    unsigned char* dstage = nullptr;
+    size_t stage_cap = 0;
+    int32_t rc = GPL_OK;
+    gpl_ctx* failing = cs[0];
+    try {
+        check(hipStreamCreate(&bs), "stream");
+        for (int32_t i = 0; i < n; ++i) {
+            gpl_ctx* c = cs[i];
+            c->join();
+            check(hipStreamSynchronize(c->stream), "sync");
+            c->drop_graph();
+            saved[i] = Saved{c->stream, c->aux, c->aux2, c->launch_mode, c->overlap, c->device_trials};
+            c->stream = c->aux = c->aux2 = bs;
+            c->launch_mode = 2;
+            c->overlap = false;
+            c->device_trials = false;
+            c->loop_begin(0);
+        }
+        std::vector<gplk::TrialRecorder> rec(n);
+        std::vector<float> coeff(n, 0.0f);
+        std::vector<int32_t> live, grouped;
+        std::vector<const gplk::TrialRecorder*> grp;
+        std::vector<char> taken(n);
+        for (;;) {
+            live.clear();
+            for (int32_t i = 0; i < n; ++i) {
+                gpl_ctx* c = cs[i];
+                if (c->ls.active && !c->ls.done && c->iter_begin(coeff[i])) live.push_back(i);
+            }
+            if (live.empty()) break;
+            // the first trials recorded, then launched in groups of one sequence
+            grouped.clear();
+            for (int32_t i : live) {
+                failing = cs[i];
+                rec[i].recs.clear();
+                rec[i].failed = false;
+                gplk::set_recorder(&rec[i]);
+                try {
+                    cs[i]->record_trial(coeff[i]);
+                } catch (...) {
+                    gplk::set_recorder(nullptr);
+                    throw;
+                }
+                gplk::set_recorder(nullptr);
+                taken[i] = 0;
+            }
+            // groups of one sequence, each with its own part of the argument staging
+            std::vector<std::vector<int32_t>> groups;
+            std::vector<size_t> goff;
+            size_t need = 0;
+            for (size_t a = 0; a < live.size(); ++a) {
+                const int32_t i = live[a];
+                if (taken[i] || rec[i].failed) continue;
+                std::vector<int32_t> members;
+                for (size_t b2 = a; b2 < live.size(); ++b2) {
+                    const int32_t j = live[b2];
+                    if (!taken[j] && gplk::same_sequence(rec[i], rec[j])) {
+                        members.push_back(j);
+                        taken[j] = 1;
+                    }
+                }
+                goff.push_back(need);
+                need += gplk::grouped_arg_bytes(rec[i], int(members.size()));
+                groups.push_back(std::move(members));
+            }
+            if (need > stage_cap) {
+                check(hipStreamSynchronize(bs), "sync");
+                if (hstage) hipHostFree(hstage);
+                if (dstage) hipFree(dstage);
+                hstage = nullptr;
+                dstage = nullptr;
+                stage_cap = 0;
+                check(hipHostMalloc((void**)&hstage, need, hipHostMallocDefault), "hipHostMalloc");
+                check(hipMalloc((void**)&dstage, need), "hipMalloc");
+                stage_cap = need;
+            }
+            for (size_t g = 0; g < groups.size(); ++g) {
+                grp.clear();
+                for (int32_t j : groups[g]) grp.push_back(&rec[j]);
+                failing = cs[groups[g][0]];
+                check(gplk::launch_grouped(bs, grp.data(), int(grp.size()), hstage + goff[g], dstage + goff[g]), "grouped trial");
+                for (int32_t j : groups[g]) grouped.push_back(j);
+            }
+            for (int32_t i : grouped) cs[i]->wait_trial_signal();
+            // the rest of each iteration by its own context (a record that failed: the whole
+            // iteration there)
+            for (int32_t i : live) {
+                failing = cs[i];
+                const bool done_here = std::find(grouped.begin(), grouped.end(), i) != grouped.end();
+                cs[i]->iter_rest(coeff[i], done_here ? 1 : 0, false, done_here);
+            }
+        }
+    } catch (const HipError& e) {
+        gplk::set_recorder(nullptr);
+        rc = fail(failing, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+    }
+    if (bs) hipStreamSynchronize(bs);
+    for (int32_t i = 0; i < n; ++i) {
+        gpl_ctx* c = cs[i];
+        if (!saved[i].stream) continue;
+        c->stream = saved[i].stream;
+        c->aux = saved[i].aux;
+        c->aux2 = saved[i].aux2;
+        c->launch_mode = saved[i].launch_mode;
+        c->overlap = saved[i].overlap;
+        c->device_trials = saved[i].device_trials;
+        if (rc == GPL_OK) {
+            int32_t it = 0;
+            try {
+                const int32_t r = c->loop_end(&it);
+                if (r < 0 && rc == GPL_OK) rc = r;
+            } catch (const HipError& e) {
+                rc = fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
+            }
+            if (iters) iters[i] = it;
+        }
+    }
+    if (hstage) hipHostFree(hstage);
+    if (dstage) hipFree(dstage);
+    if (bs) hipStreamDestroy(bs);
+    return rc;
 }
 
 int32_t gpl_continue_nesterov_place(gpl_ctx* c, float overflow, int32_t max_iter, int32_t start_iter, int32_t* iter_out) {

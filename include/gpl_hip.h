@@ -314,6 +314,16 @@ int32_t gpl_init_nesterov_loop(gpl_ctx* c);
 int32_t gpl_run_nesterov_loop(gpl_ctx* c, int32_t start_iter, int32_t* iter_out);
 /* nesterov.rs:4115: init, then the loop from 0. */
 int32_t gpl_do_nesterov_place(gpl_ctx* c, int32_t* iter_out);
+/* gpl_do_nesterov_place on each of n independent contexts (no reference counterpart: the crate
+ * places one design at a time; for callers with many placements, e.g. seeds of one design).  The
+ * contexts run in lockstep: each iteration's first back-tracking trial of every context is one
+ * launch per kernel over all of them (contexts whose trials have the same kernels and grids, i.e.
+ * the same shape, are grouped; the kernels' blocks take each context's arguments in turn), and
+ * each context takes the rest of its iteration (its decisions, further trials, re-sorts) itself.
+ * Each context's placement, rows and statistics are its own gpl_do_nesterov_place's, bit for bit.
+ * iters (n entries, may be null) gets the final iterations.  Contexts with routability or
+ * profiling on run one after another instead. */
+int32_t gpl_place_batch(gpl_ctx* const* ctxs, int32_t n, int32_t* iters);
 /* nesterov.rs:4243: the loop again with a new target overflow and iteration limit. */
 int32_t gpl_continue_nesterov_place(gpl_ctx* c, float overflow, int32_t max_iter, int32_t start_iter,
                                     int32_t* iter_out);
