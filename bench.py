@@ -21,8 +21,9 @@ Besides the throughput the line carries:
                 oracle's full run on the same problem (tests/golden/loop_c3.json: one core, about
                 twelve minutes, too long to repeat here; the problems' fingerprints must match),
                 and a fresh GPU-vs-oracle run to convergence on the 2k-cell placement;
-  batched       16 seeds of the 50k-cell design placed one after another and from concurrent
-                contexts (each its own placement and control), with the results compared;
+  batched       16 seeds of the 50k-cell design placed one after another, then as one lockstep
+                batch (each iteration's first trial of all 16 as one launch per kernel, each
+                context its own control) and from concurrent contexts, with the results compared;
   routability   the routability-driven loop on the 500k-cell design (768 x 768 bins, RUDY on
                 1024 x 1024 tiles), np_init to convergence, against the oracle's full run.
 """
@@ -247,8 +248,9 @@ def load_fixture(cfg):
 
 def batched(cfg, k, emit=True):
     """K independent placements (seeds 1..K: their own perturbations) of one design: run one
-    after another, then at once from K host threads (one context each); one JSON line when
-    `emit`, and the record returned."""
+    after another, then as one lockstep batch (gpl_place_batch: each iteration's first trial of
+    all K as one launch per kernel), and for comparison at once from host threads (one context
+    each); one JSON line when `emit`, and the record returned."""
     import gpl_amd as G
 
     def log(*a):
@@ -273,18 +275,26 @@ def batched(cfg, k, emit=True):
     t_seq = time.perf_counter() - t0
     b = contexts()
     t0 = time.perf_counter()
-    it_par = G.place_concurrently(b)
+    it_bat = G.place_batched(b)
+    t_bat = time.perf_counter() - t0
+    same = it_seq == it_bat and all(x.stats().hpwl == y.stats().hpwl for x, y in zip(a, b))
+    for g in b:
+        g.close()
+    c = contexts()
+    t0 = time.perf_counter()
+    it_par = G.place_concurrently(c)
     t_par = time.perf_counter() - t0
-    same = it_seq == it_par and all(x.stats().hpwl == y.stats().hpwl for x, y in zip(a, b))
-    for g in a + b:
+    same_par = it_seq == it_par and all(x.stats().hpwl == y.stats().hpwl for x, y in zip(a, c))
+    for g in a + c:
         g.close()
     d.close()
-    log(f"[bench] sequential {t_seq:.3f}s, concurrent {t_par:.3f}s, same results: {same}")
+    log(f"[bench] sequential {t_seq:.3f}s, batch {t_bat:.3f}s, concurrent contexts {t_par:.3f}s, same results: {same} {same_par}")
     line = {
-        "metric": f"placements/s, {k} seeds of one design, each to overflow 0.10, concurrent contexts",
-        "value": round(k / t_par, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
-        "sequential_value": round(k / t_seq, 3), "speedup_vs_sequential": round(t_seq / t_par, 2),
-        "iterations": it_par, "same_results_as_sequential": bool(same), "dtype": "f32", "data": "synthetic",
+        "metric": f"placements/s, {k} seeds of one design, each to overflow 0.10, one lockstep batch",
+        "value": round(k / t_bat, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
+        "sequential_value": round(k / t_seq, 3), "speedup_vs_sequential": round(t_seq / t_bat, 2),
+        "concurrent_contexts_value": round(k / t_par, 3), "concurrent_speedup_vs_sequential": round(t_seq / t_par, 2),
+        "iterations": it_bat, "same_results_as_sequential": bool(same and same_par), "dtype": "f32", "data": "synthetic",
         "config": {"workload": WORKLOADS.get(cfg, cfg), "name": cfg, "seeds": k}}
     if emit:
         print(json.dumps(line), flush=True)
@@ -366,7 +376,7 @@ def main():
                          "calibration under rocprofv3 --pmc)")
     ap.add_argument("--seeds", type=int, default=0,
                     help="the batched form instead of the headline: this many seeds of --config (c4s: the "
-                         "50k-cell design) placed one after another, then all at once from concurrent contexts")
+                         "50k-cell design) placed one after another, then as one lockstep batch and from concurrent contexts")
     ap.add_argument("--summarize", nargs="+", metavar="CSV",
                     help="print a text summary of rocprofv3 CSV output (kernel_stats.csv or counter_collection.csv) and exit")
     ap.add_argument("--timeline", metavar="KERNEL_TRACE_CSV",

@@ -502,7 +502,10 @@ bool same_sequence(const TrialRecorder& a, const TrialRecorder& b);
 size_t grouped_arg_bytes(const TrialRecorder& r, int n);  // the staging bytes for n records like r
 // recs[0 .. n): records of one sequence; host_stage (pinned) and dev_stage hold
 // grouped_arg_bytes(*recs[0], n) bytes; the host stage must not change until the copy is done.
-hipError_t launch_grouped(hipStream_t s, const TrialRecorder* const* recs, int n, unsigned char* host_stage, unsigned char* dev_stage);
+// With a side stream (and two events), the WA kernels run there beside the scatter, bins and
+// solve, from the update on, and the gradient waits for them (as in the trial graph).
+hipError_t launch_grouped(hipStream_t s, const TrialRecorder* const* recs, int n, unsigned char* host_stage, unsigned char* dev_stage,
+                          hipStream_t side = nullptr, hipEvent_t ev_fork = nullptr, hipEvent_t ev_join = nullptr);
 
 hipError_t set_wa_node(hipGraphExec_t e, const TrialNode& n, const NetsDev& nd, const CellsDev& c, float cx, float cy,
                        unsigned long long* hpwl);

@@ -3162,6 +3162,10 @@ __global__ void __launch_bounds__(kFinishBlock) k_group_finish(const FinishArgsG
     finish_body(a.partials, a.nb_bins, a.nb_cells, a.out, a.hpwl, a.host_out, a.dev_tail, a.host_tail, a.dev_seq, a.host_seq, nullptr);
 }
 
+__global__ void k_stage_copy(const uint4* src, uint4* dst, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 bool same_sequence(const TrialRecorder& a, const TrialRecorder& b) {
     if (a.failed || b.failed || a.recs.size() != b.recs.size()) return false;
     for (size_t i = 0; i < a.recs.size(); ++i) {
@@ -3179,7 +3183,10 @@ size_t grouped_arg_bytes(const TrialRecorder& r, int n) {
     return b;
 }
 
-hipError_t launch_grouped(hipStream_t s, const TrialRecorder* const* recs, int n, unsigned char* host_stage, unsigned char* dev_stage) {
+hipError_t launch_grouped(hipStream_t s0, const TrialRecorder* const* recs, int n, unsigned char* host_stage, unsigned char* dev_stage,
+                          hipStream_t side, hipEvent_t ev_fork, hipEvent_t ev_join) {
+    const bool two = side && ev_fork && ev_join;
+    bool forked = false, waited = false, joined = false;
     if (n <= 0) return hipSuccess;
     const std::vector<LaunchRec>& r0 = recs[0]->recs;
     for (int j = 0; j < n; ++j)
@@ -3192,7 +3199,15 @@ hipError_t launch_grouped(hipStream_t s, const TrialRecorder* const* recs, int n
         for (int j = 0; j < n; ++j) std::memcpy(host_stage + at + (size_t)j * b, recs[j]->recs[i].args, b);
         at += round16(b * (size_t)n);
     }
-    hipError_t e = hipMemcpyAsync(dev_stage, host_stage, at, hipMemcpyHostToDevice, s);
+    // the arguments to the device by a small kernel reading the pinned staging (in the stream's
+    // kernel order: no copy engine between the trials' kernels)
+    void* hdev = nullptr;
+    hipError_t e = hipHostGetDevicePointer(&hdev, host_stage, 0);
+    if (e != hipSuccess) return e;
+    const int n16 = (int)(at / 16);
+    hipLaunchKernelGGL(k_stage_copy, dim3((unsigned)std::min(64, (n16 + 255) / 256)), dim3(256), 0, s0, (const uint4*)hdev, (uint4*)dev_stage,
+                       n16);
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     for (size_t i = 0; i < r0.size(); ++i) {
         const LaunchRec& q = r0[i];
@@ -3200,6 +3215,22 @@ hipError_t launch_grouped(hipStream_t s, const TrialRecorder* const* recs, int n
         const dim3 grid(per * (unsigned)n), block(q.block);
         const void* p = dev_stage + off[i];
         const int k = q.kind;
+        const bool wa = k == RK_WA_TILES || k == RK_WA_BIG;
+        hipStream_t s = s0;
+        if (two && wa && forked) {
+            if (!waited) {
+                e = hipStreamWaitEvent(side, ev_fork, 0);
+                if (e != hipSuccess) return e;
+                waited = true;
+            }
+            s = side;
+        }
+        if (two && waited && !joined && !wa) {
+            e = hipEventRecord(ev_join, side);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s0, ev_join, 0);
+            if (e != hipSuccess) return e;
+            joined = true;
+        }
         if (k == RK_UPDATE) {
             hipLaunchKernelGGL(k_group_update, grid, block, 0, s, (const UpdateArgsG*)p, per);
         } else if (k >= RK_SCATTER && k < RK_SCATTER + 4) {
@@ -3237,6 +3268,16 @@ hipError_t launch_grouped(hipStream_t s, const TrialRecorder* const* recs, int n
             return hipErrorInvalidValue;
         }
         e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if (two && k == RK_UPDATE && !forked) {
+            e = hipEventRecord(ev_fork, s0);
+            if (e != hipSuccess) return e;
+            forked = true;
+        }
+    }
+    if (two && waited && !joined) {
+        e = hipEventRecord(ev_join, side);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s0, ev_join, 0);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

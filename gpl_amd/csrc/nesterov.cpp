@@ -2982,14 +2982,24 @@ int32_t gpl_place_batch(gpl_ctx* const* cs, int32_t n, int32_t* iters) {
         bool overlap, device_trials;
     };
     std::vector<Saved> saved(n);
-    hipStream_t bs = nullptr;
-    unsigned char* hstage = nullptr;
-    unsigned char* dstage = nullptr;
-    size_t stage_cap = 0;
+    hipStream_t bs = nullptr, side = nullptr;  // the contexts' stream; the grouped WA kernels'
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // The contexts' round: the live ones and those whose first trial ran grouped; the grouped
+    // launches' arguments (pinned staging) and their device copy.
+    struct Round {
+        std::vector<int32_t> members, live, grouped;
+        unsigned char* hstage = nullptr;
+        unsigned char* dstage = nullptr;
+        size_t cap = 0;
+        bool pending = false;  // a launched trial not yet finished
+    } L;
     int32_t rc = GPL_OK;
     gpl_ctx* failing = cs[0];
     try {
         check(hipStreamCreate(&bs), "stream");
+        check(hipStreamCreate(&side), "stream");
+        check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "event");
+        check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "event");
         for (int32_t i = 0; i < n; ++i) {
             gpl_ctx* c = cs[i];
             c->join();
@@ -3002,21 +3012,24 @@ int32_t gpl_place_batch(gpl_ctx* const* cs, int32_t n, int32_t* iters) {
             c->device_trials = false;
             c->loop_begin(0);
         }
+        // (two rounds in turn -- one on the GPU while the host finishes the other -- measured
+        // slower: each round's launches cost the host as much as the whole batch's)
+        for (int32_t i = 0; i < n; ++i) L.members.push_back(i);
         std::vector<gplk::TrialRecorder> rec(n);
         std::vector<float> coeff(n, 0.0f);
-        std::vector<int32_t> live, grouped;
         std::vector<const gplk::TrialRecorder*> grp;
         std::vector<char> taken(n);
-        for (;;) {
-            live.clear();
-            for (int32_t i = 0; i < n; ++i) {
+        // an iteration's start for the live contexts: the first trials recorded, then launched in
+        // groups of one sequence, each group with its part of the staging
+        auto launch_round = [&]() {
+            L.live.clear();
+            L.grouped.clear();
+            for (int32_t i : L.members) {
                 gpl_ctx* c = cs[i];
-                if (c->ls.active && !c->ls.done && c->iter_begin(coeff[i])) live.push_back(i);
+                if (c->ls.active && !c->ls.done && c->iter_begin(coeff[i])) L.live.push_back(i);
             }
-            if (live.empty()) break;
-            // the first trials recorded, then launched in groups of one sequence
-            grouped.clear();
-            for (int32_t i : live) {
+            L.pending = !L.live.empty();
+            for (int32_t i : L.live) {
                 failing = cs[i];
                 rec[i].recs.clear();
                 rec[i].failed = false;
@@ -3030,16 +3043,15 @@ int32_t gpl_place_batch(gpl_ctx* const* cs, int32_t n, int32_t* iters) {
                 gplk::set_recorder(nullptr);
                 taken[i] = 0;
             }
-            // groups of one sequence, each with its own part of the argument staging
             std::vector<std::vector<int32_t>> groups;
             std::vector<size_t> goff;
             size_t need = 0;
-            for (size_t a = 0; a < live.size(); ++a) {
-                const int32_t i = live[a];
+            for (size_t a = 0; a < L.live.size(); ++a) {
+                const int32_t i = L.live[a];
                 if (taken[i] || rec[i].failed) continue;
                 std::vector<int32_t> members;
-                for (size_t b2 = a; b2 < live.size(); ++b2) {
-                    const int32_t j = live[b2];
+                for (size_t b2 = a; b2 < L.live.size(); ++b2) {
+                    const int32_t j = L.live[b2];
                     if (!taken[j] && gplk::same_sequence(rec[i], rec[j])) {
                         members.push_back(j);
                         taken[j] = 1;
@@ -3049,33 +3061,38 @@ int32_t gpl_place_batch(gpl_ctx* const* cs, int32_t n, int32_t* iters) {
                 need += gplk::grouped_arg_bytes(rec[i], int(members.size()));
                 groups.push_back(std::move(members));
             }
-            if (need > stage_cap) {
+            if (need > L.cap) {
                 check(hipStreamSynchronize(bs), "sync");
-                if (hstage) hipHostFree(hstage);
-                if (dstage) hipFree(dstage);
-                hstage = nullptr;
-                dstage = nullptr;
-                stage_cap = 0;
-                check(hipHostMalloc((void**)&hstage, need, hipHostMallocDefault), "hipHostMalloc");
-                check(hipMalloc((void**)&dstage, need), "hipMalloc");
-                stage_cap = need;
+                if (L.hstage) hipHostFree(L.hstage);
+                if (L.dstage) hipFree(L.dstage);
+                L.hstage = nullptr;
+                L.dstage = nullptr;
+                L.cap = 0;
+                check(hipHostMalloc((void**)&L.hstage, need, hipHostMallocDefault), "hipHostMalloc");
+                check(hipMalloc((void**)&L.dstage, need), "hipMalloc");
+                L.cap = need;
             }
             for (size_t g = 0; g < groups.size(); ++g) {
                 grp.clear();
                 for (int32_t j : groups[g]) grp.push_back(&rec[j]);
                 failing = cs[groups[g][0]];
-                check(gplk::launch_grouped(bs, grp.data(), int(grp.size()), hstage + goff[g], dstage + goff[g]), "grouped trial");
-                for (int32_t j : groups[g]) grouped.push_back(j);
+                check(gplk::launch_grouped(bs, grp.data(), int(grp.size()), L.hstage + goff[g], L.dstage + goff[g], side, ev_fork, ev_join),
+                      "grouped trial");
+                for (int32_t j : groups[g]) L.grouped.push_back(j);
             }
-            for (int32_t i : grouped) cs[i]->wait_trial_signal();
-            // the rest of each iteration by its own context (a record that failed: the whole
-            // iteration there)
-            for (int32_t i : live) {
+        };
+        // the rest of each iteration by its own context (a context whose record failed: the whole
+        // iteration there)
+        auto finish_round = [&]() {
+            for (int32_t i : L.grouped) cs[i]->wait_trial_signal();
+            for (int32_t i : L.live) {
                 failing = cs[i];
-                const bool done_here = std::find(grouped.begin(), grouped.end(), i) != grouped.end();
+                const bool done_here = std::find(L.grouped.begin(), L.grouped.end(), i) != L.grouped.end();
                 cs[i]->iter_rest(coeff[i], done_here ? 1 : 0, false, done_here);
             }
-        }
+            L.pending = false;
+        };
+        for (launch_round(); L.pending; launch_round()) finish_round();
     } catch (const HipError& e) {
         gplk::set_recorder(nullptr);
         rc = fail(failing, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
@@ -3101,8 +3118,12 @@ int32_t gpl_place_batch(gpl_ctx* const* cs, int32_t n, int32_t* iters) {
             if (iters) iters[i] = it;
         }
     }
-    if (hstage) hipHostFree(hstage);
-    if (dstage) hipFree(dstage);
+    if (side) hipStreamSynchronize(side);
+    if (L.hstage) hipHostFree(L.hstage);
+    if (L.dstage) hipFree(L.dstage);
+    if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_join) hipEventDestroy(ev_join);
+    if (side) hipStreamDestroy(side);
     if (bs) hipStreamDestroy(bs);
     return rc;
 }

@@ -71,6 +71,10 @@ def test_seeds_of_one_design_in_one_batch():
     _check([("c1", (1, 2, 3, 4, 5, 6))])
 
 
+def test_nine_seeds_in_one_batch():
+    _check([("c1", tuple(range(1, 10)))])
+
+
 def test_seeds_of_the_50k_cell_design_in_one_batch():
     _check([("c4s", (1, 2, 3, 4))])
 
