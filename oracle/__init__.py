@@ -85,6 +85,7 @@ def lib():
             ("orc_set_inject_control", None, [C.c_void_p, C.c_int32, C.c_int32]),
             ("orc_region_rows", C.c_int32, [C.c_void_p, f32p, C.c_int32]),
             ("orc_set_field_noise", None, [C.c_void_p, C.c_float, C.c_uint64]),
+            ("orc_set_wide_sums", None, [C.c_int32]),
             ("orc_rudy_wire_width", C.c_int32, [i32p, C.c_int32]),
             ("orc_rudy_rect", None, [i32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, i32p]),
             ("orc_rudy", None, [i32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, i32p, u8p, f32p, f32p]),
@@ -286,6 +287,12 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+
+def set_wide_sums(on):
+    """The oracle's global float sums (step-length distances, |wl| and |den|, overflow) in double
+    (on) or as the reference's float sums (off, the default); process-wide."""
+    lib().orc_set_wide_sums(1 if on else 0)
 
 
 def solve_f64(nx, ny, wx, wy, density_ym):

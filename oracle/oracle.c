@@ -58,16 +58,28 @@ float orc_fast_exp(float x) {
     return e;
 }
 
+/* Wide sums (a test switch, off by default): the loop's global float sums -- the step length's
+ * distances, the gradients' |wl| and |den|, the bins' overflow -- accumulated in double instead of
+ * the reference's float, the precision of the GPU path's FP64 reductions.  It separates what the
+ * float sums' rounding contributes to a difference from what the arithmetic does. */
+static int g_wide_sums = 0;
+void orc_set_wide_sums(int32_t on) { g_wide_sums = on ? 1 : 0; }
+
 /* nesterov.rs:319 getDistance: a float sum of squared differences in point order, then
  * sqrt(sum / (2 n)) in double.  a, b: n points, x and y interleaved. */
 float orc_distance(const float* a, const float* b, int32_t n) {
     float sum = 0.0f;
+    double wide = 0.0;
     for (int32_t i = 0; i < n; ++i) {
-        sum += (a[2 * i] - b[2 * i]) * (a[2 * i] - b[2 * i]);
-        sum += (a[2 * i + 1] - b[2 * i + 1]) * (a[2 * i + 1] - b[2 * i + 1]);
+        const float dx = (a[2 * i] - b[2 * i]) * (a[2 * i] - b[2 * i]);
+        const float dy = (a[2 * i + 1] - b[2 * i + 1]) * (a[2 * i + 1] - b[2 * i + 1]);
+        sum += dx;
+        sum += dy;
+        wide += (double)dx;
+        wide += (double)dy;
     }
     if (n == 0) return 0.0f;
-    return (float)sqrt((double)sum / (2.0 * (double)n));
+    return (float)sqrt((g_wide_sums ? wide : (double)sum) / (2.0 * (double)n));
 }
 
 /* nesterov.rs:96 derived_reference_hpwl. */
@@ -562,6 +574,7 @@ static void update_bins_density_area(orc_np* np, orc_nb* nb) {
     }
     nb->overflow = 0;
     nb->overflow_unscaled = 0;
+    double wo = 0.0, wou = 0.0;
     for (int y = 0; y < nb->R.cnt_y; ++y)
         for (int x = 0; x < cnt_x; ++x) {
             size_t k = (size_t)y * cnt_x + x;
@@ -571,10 +584,16 @@ static void update_bins_density_area(orc_np* np, orc_nb* nb) {
             float o = (float)nb->inst[k] + (float)nb->non_place[k] - scaled;
             o = o > 0.0f ? o : 0.0f;
             nb->overflow = (int64_t)((float)nb->overflow + o);
+            wo += (double)o;
             float ou = (float)nb->inst[k] + (float)nb->non_place_unscaled[k] - scaled;
             ou = ou > 0.0f ? ou : 0.0f;
             nb->overflow_unscaled = (int64_t)((float)nb->overflow_unscaled + ou);
+            wou += (double)ou;
         }
+    if (g_wide_sums) {
+        nb->overflow = (int64_t)wo;
+        nb->overflow_unscaled = (int64_t)wou;
+    }
 }
 
 /* nesterov.rs:2268 updateGCellDensityCenterLocation then the bins. */
@@ -771,14 +790,17 @@ void orc_precondition(int32_t n, const float* wl, const float* den, const int32_
         sum[2 * i + 1] = sy / pre;
     }
     float a = 0.0f, d = 0.0f;
+    double wa = 0.0, wd = 0.0;
     for (int32_t i = 0; i < n; ++i) {
         a += fabsf(wl[2 * i]);
         a += fabsf(wl[2 * i + 1]);
         d += fabsf(den[2 * i]);
         d += fabsf(den[2 * i + 1]);
+        wa += (double)fabsf(wl[2 * i]) + (double)fabsf(wl[2 * i + 1]);
+        wd += (double)fabsf(den[2 * i]) + (double)fabsf(den[2 * i + 1]);
     }
-    *wl_abs = a;
-    *den_abs = d;
+    *wl_abs = g_wide_sums ? (float)wa : a;
+    *den_abs = g_wide_sums ? (float)wd : d;
 }
 
 /* nesterov.rs:2455 updateGradients into a slot. */

@@ -9,11 +9,17 @@ generated problem (the GPU test checks that it places the same problem).
 
     python tests/golden/make_loop_fixtures.py c2 c3
     python tests/golden/make_loop_fixtures.py --perturb 3 c4r   # adds the sensitivity runs
+    python tests/golden/make_loop_fixtures.py --wide-rows c2 c3 c4r c4s c5   # adds the wide-sum first rows
 
 --perturb N adds N more oracle runs of the same problem with every field value scaled by
 (1 + 2e-7 u), u uniform in [-1, 1) (the oracle's sensitivity probe: the size by which the GPU's
 transforms differ from the oracle's), recording how far the loop's outcome moves under
 rounding-level perturbations.
+
+--wide-rows adds each fixture's first rows from the oracle with its global float sums in double
+(the step length's distances, |wl| and |den|, the overflow), the precision of the GPU path's FP64
+reductions: the GPU's first rows equal these, while the reference's float sums move the plain
+rows by up to ~1e-4 at 500k-1M cells.
 """
 import json
 import os
@@ -43,24 +49,66 @@ def cpu_model():
     return platform.processor() or platform.machine()
 
 
-def perturbed(name, n, eps=2e-7, seed=1):
+def one_perturbed(name, eps, k, seed=1):
+    """One sensitivity run: the oracle's loop with noise seed k (a process of its own)."""
     import gpl_amd as G
     import oracle as O
     d = G.Design.from_config(name, seed=seed)
     d.initial_place()
     d.init_nesterov_place()
-    out = []
-    for k in range(1, n + 1):
-        o = O.Oracle(d.problem())
-        o.set_field_noise(eps, k)
-        o.init()
-        rc, it = o.run()
-        s = o.stats()
-        out.append(dict(field_noise=eps, noise_seed=k, status=int(rc), converged=bool(s.converged), final_iter=int(it),
-                        final_hpwl=int(s.hpwl), final_overflow=float(s.overflow)))
-        print(f"{name} perturbed {k}: iter {it} hpwl {s.hpwl}", flush=True)
-        o.close()
+    o = O.Oracle(d.problem())
+    o.set_field_noise(eps, k)
+    o.init()
+    rc, it = o.run()
+    s = o.stats()
+    out = dict(field_noise=eps, noise_seed=k, status=int(rc), converged=bool(s.converged), final_iter=int(it),
+               final_hpwl=int(s.hpwl), final_overflow=float(s.overflow))
+    if d.problem().opts.routability:
+        _, passes = o.route_stats()
+        out["route_passes"] = [{k2: (float(v) if isinstance(v, float) else int(v)) for k2, v in q.items()} for q in passes]
+    print(f"{name} perturbed {k}: iter {it} hpwl {s.hpwl}", flush=True)
+    o.close()
     d.close()
+    return out
+
+
+def one_wide_rows(name, seed=1, iters=3):
+    """The oracle's first rows with its global float sums in double (the GPU path's FP64
+    reductions): what the GPU's first rows must equal, the reference's float sums aside."""
+    import gpl_amd as G
+    import oracle as O
+    O.set_wide_sums(True)
+    d = G.Design.from_config(name, seed=seed)
+    d.initial_place()
+    d.init_nesterov_place()
+    o = O.Oracle(d.problem())
+    o.init()
+    o.run(iter_budget=iters)
+    rows = [dict(iter=r["iter"], trials=r["trials"], overflow=float(r["overflow"]), hpwl=int(r["hpwl"]),
+                 density_penalty=float(r["density_penalty"]), step_length=float(r["step_length"])) for r in o.history()[:iters]]
+    o.close()
+    d.close()
+    print(f"{name} wide rows: {[(r['iter'], r['hpwl']) for r in rows]}", flush=True)
+    return name, rows
+
+
+def wide_rows(names):
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=max(1, min(len(names), os.cpu_count() or 1))) as ex:
+        for name, rows in ex.map(one_wide_rows, names):
+            path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"loop_{name}.json")
+            with open(path) as f:
+                fx = json.load(f)
+            fx["rows_wide"] = rows
+            with open(path, "w") as f:
+                json.dump(fx, f, indent=1)
+
+
+def perturbed(name, n, eps=2e-7, seed=1):
+    """n sensitivity runs, one process each (at most the host's cores at once)."""
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=max(1, min(n, os.cpu_count() or 1))) as ex:
+        out = list(ex.map(one_perturbed, [name] * n, [eps] * n, range(1, n + 1), [seed] * n))
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"loop_{name}.json")
     with open(path) as f:
         fx = json.load(f)
@@ -114,7 +162,9 @@ def run(name, seed=1):
 
 if __name__ == "__main__":
     args = sys.argv[1:]
-    if args[:1] == ["--perturb"]:
+    if args[:1] == ["--wide-rows"]:
+        wide_rows(args[1:])
+    elif args[:1] == ["--perturb"]:
         k = int(args[1])
         for n in args[2:]:
             perturbed(n, k)

@@ -72,19 +72,36 @@ def test_loop_matches_the_oracles_full_run(cfg, request):
     assert abs(it - fx["final_iter"]) <= max(10, 0.02 * fx["final_iter"], 1.25 * iter_env)
     if route is not None:
         # the routability passes: the same count and decisions, each at the oracle's iteration
-        # within 2, its congestion within 3 % and its added area within 3 %
+        # within 2, its congestion within 3 % and its added area within 3 % -- or, where the
+        # oracle's own sensitivity runs (rounding-level noise in the field) move a pass further,
+        # within 1.25 times that spread
         tot, passes = route
         want = fx["route"]["passes"]
-        print("passes", [(p["iter"], round(p["rc"], 4)) for p in passes], "oracle", [(p["iter"], round(p["rc"], 4)) for p in want])
+        print("passes", [(p["iter"], round(p["rc"], 4), p["inflation"]) for p in passes],
+              "oracle", [(p["iter"], round(p["rc"], 4), p["inflation"]) for p in want])
         assert tot["passes"] == fx["route"]["stats"]["passes"] == len(want)
-        for a, b in zip(passes, want):
+        for q, (a, b) in enumerate(zip(passes, want)):
+            pq = [p["route_passes"][q] for p in pert if len(p.get("route_passes", [])) > q]
+            env_it = max([abs(p["iter"] - b["iter"]) for p in pq], default=0)
+            env_rc = max([abs(p["rc"] - b["rc"]) / b["rc"] for p in pq], default=0.0)
+            env_inf = max([abs(p["inflation"] - b["inflation"]) / max(b["inflation"], 1) for p in pq], default=0.0)
             assert (a["need"], a["revert"]) == (b["need"], b["revert"])
-            assert abs(a["iter"] - b["iter"]) <= 2
-            assert abs(a["rc"] - b["rc"]) <= 0.03 * b["rc"]
-            assert abs(a["inflation"] - b["inflation"]) <= 0.03 * max(b["inflation"], 1) + 1
-    r0 = fx["rows"][0]
+            assert abs(a["iter"] - b["iter"]) <= max(2, 1.25 * env_it)
+            assert abs(a["rc"] - b["rc"]) <= max(0.03, 1.25 * env_rc) * b["rc"]
+            assert abs(a["inflation"] - b["inflation"]) <= max(0.03, 1.25 * env_inf) * max(b["inflation"], 1) + 1
+    # the first rows against the oracle with its global float sums in double (the GPU's FP64
+    # reductions: the same arithmetic, so the rows agree to the transforms' rounding), and against
+    # the plain oracle within 1e-4 or the gap its float sums open (up to ~1e-4 at 500k-1M cells)
+    r0, rw = fx["rows"][0], fx.get("rows_wide", [])
     assert hist[0]["iter"] == r0["iter"]
-    assert abs(hist[0]["hpwl"] - r0["hpwl"]) <= 1e-4 * r0["hpwl"]
+    print("first rows", [(r["iter"], r["hpwl"], r["overflow"]) for r in hist[:3]], "wide oracle",
+          [(r["iter"], r["hpwl"], r["overflow"]) for r in rw])
+    for k, (a, b) in enumerate(zip(hist, rw)):
+        assert a["iter"] == b["iter"]
+        assert abs(a["hpwl"] - b["hpwl"]) <= (2e-6 if k == 0 else 2e-5) * b["hpwl"]
+        assert abs(a["overflow"] - b["overflow"]) <= 1e-5
+    gap = abs(rw[0]["hpwl"] - r0["hpwl"]) / r0["hpwl"] if rw else 0.0
+    assert abs(hist[0]["hpwl"] - r0["hpwl"]) <= max(1e-4, 1.25 * gap) * r0["hpwl"]
     assert abs(hist[0]["overflow"] - r0["overflow"]) <= 1e-4
 
 

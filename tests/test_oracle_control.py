@@ -132,6 +132,26 @@ def test_precondition_abs_sums_are_float_sums_in_cell_order():
     assert wa == 2.0 ** 24 + 4
 
 
+def test_wide_sums_switch_sums_in_double():
+    # the same lost-ones case with the switch on: every +1 counts in either order; the distances
+    # likewise (2^24 + four 1s of squared difference)
+    wl = [[2.0 ** 24, 0]] + [[1.0, 0]] * 4
+    a = np.zeros((5, 2))
+    b = np.array([[2.0 ** 12, 0]] + [[1.0, 0]] * 4)
+    try:
+        O.set_wide_sums(True)
+        _, wa, _ = O.precondition(wl, np.zeros((5, 2)), [1] * 5, [[1, 1]] * 5, 0.0)
+        assert wa == 2.0 ** 24 + 4
+        st, cd = O.init_step(a, b, a, a)
+        assert cd == f32(np.sqrt((2.0 ** 24 + 4) / 10.0))
+    finally:
+        O.set_wide_sums(False)
+    _, wa, _ = O.precondition(wl, np.zeros((5, 2)), [1] * 5, [[1, 1]] * 5, 0.0)
+    assert wa == 2.0 ** 24
+    st, cd = O.init_step(a, b, a, a)
+    assert cd == f32(np.sqrt(2.0 ** 24 / 10.0))
+
+
 # ---- initDensity2's first penalty and step (nesterov.rs:2600-2617) -----------------------------
 
 def test_initial_penalty_and_step():
