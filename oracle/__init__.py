@@ -86,6 +86,7 @@ def lib():
             ("orc_region_rows", C.c_int32, [C.c_void_p, f32p, C.c_int32]),
             ("orc_set_field_noise", None, [C.c_void_p, C.c_float, C.c_uint64]),
             ("orc_set_wide_sums", None, [C.c_int32]),
+            ("orc_set_threads", None, [C.c_int32]),
             ("orc_rudy_wire_width", C.c_int32, [i32p, C.c_int32]),
             ("orc_rudy_rect", None, [i32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, i32p]),
             ("orc_rudy", None, [i32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, i32p, u8p, f32p, f32p]),
@@ -287,6 +288,13 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+
+def set_threads(n):
+    """Host threads for the loop's independent per-cell, per-net, per-row and per-bin work
+    (process-wide; 1, the default, is the reference's single thread).  Any count gives the same
+    bits: the sums whose order the reference fixes stay sequential."""
+    lib().orc_set_threads(int(n))
 
 
 def set_wide_sums(on):

@@ -30,6 +30,8 @@
 
 #define ORC_PI 3.14159265358979323846
 
+int orc_threads(void); /* oracle.c: the host threads the 2-D transforms may use */
+
 static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 /* the lengths the fast path transforms: 2^k, and 3 * 2^k (k >= 1: the half-length FFT then has a
  * radix-3 step) */
@@ -46,11 +48,19 @@ static int fast_len(int n) { return is_pow2(n) || (n % 6 == 0 && is_pow2(n / 6))
         T *fft, *wn, *w4n;                                                                          \
     } tab_t_##SFX;                                                                                  \
     static tab_t_##SFX tabs_##SFX[64];                                                               \
+    static void tab_fill_##SFX(tab_t_##SFX* t, int n);                                             \
     static const tab_t_##SFX* tab_##SFX(int n) {                                                   \
         int lg = 0;                                                                                 \
         while ((1 << lg) < n) ++lg;                                                                 \
         tab_t_##SFX* t = &tabs_##SFX[lg + (is_pow2(n) ? 0 : 32)]; /* 3 * 2^k beside the powers */ \
-        if (t->n == n) return t;                                                                    \
+        /* (made once under a lock: the 2-D transforms' threads may ask for one at once; a made   \
+         * table is seen through the acquire load of its length, stored last) */                  \
+        if (__atomic_load_n(&t->n, __ATOMIC_ACQUIRE) == n) return t;                                \
+        _Pragma("omp critical(orc_dct_tabs)")                                                       \
+        if (t->n != n) tab_fill_##SFX(t, n);                                                        \
+        return t;                                                                                   \
+    }                                                                                               \
+    static void tab_fill_##SFX(tab_t_##SFX* t, int n) {                                             \
         T* buf = (T*)malloc(sizeof(T) * (2 * (size_t)n + 2 * ((size_t)n / 2 + 1) + 2 * (size_t)n + 2)); \
         t->fft = buf;                                                                               \
         t->wn = buf + 2 * (size_t)n;                                                                \
@@ -73,8 +83,7 @@ static int fast_len(int n) { return is_pow2(n) || (n % 6 == 0 && is_pow2(n / 6))
             t->w4n[2 * k] = (T)cos(ang);                                                            \
             t->w4n[2 * k + 1] = (T)sin(ang);                                                        \
         }                                                                                           \
-        t->n = n;                                                                                   \
-        return t;                                                                                   \
+        __atomic_store_n(&t->n, n, __ATOMIC_RELEASE);                                               \
     }                                                                                               \
     static void fft3_##SFX(int m, T* z, int sign);                                                \
     /* in-place radix-2 complex FFT of m points (re, im interleaved), sign -1 forward, +1 inverse  \
@@ -279,20 +288,27 @@ static int fast_len(int n) { return is_pow2(n) || (n % 6 == 0 && is_pow2(n / 6))
         for (int k = 1; k < n; k += 2) a[k] = -a[k];                                                \
     }                                                                                               \
     /* 2-D: rows (length n2) with `row`, then columns (length n1) with `col` */                     \
+    /* (rows, and column blocks, are independent: with several host threads each takes its own) */ \
     static void xt2d_##SFX(int n1, int n2, T* a, void (*row)(int, T*), void (*col)(int, T*)) {      \
+        const int nt = orc_threads();                                                               \
+        _Pragma("omp parallel for num_threads(nt) if (nt > 1) schedule(static)")                    \
         for (int i = 0; i < n1; ++i) row(n2, a + (size_t)i * n2);                                   \
         /* columns in blocks of 16 gathered together (runs of 16 consecutive values per row) */     \
         enum { B = 16 };                                                                            \
-        T* t = (T*)malloc(sizeof(T) * (size_t)n1 * B);                                              \
-        for (int j0 = 0; j0 < n2; j0 += B) {                                                        \
-            const int nb = n2 - j0 < B ? n2 - j0 : B;                                               \
-            for (int i = 0; i < n1; ++i)                                                            \
-                for (int q = 0; q < nb; ++q) t[(size_t)q * n1 + i] = a[(size_t)i * n2 + j0 + q];    \
-            for (int q = 0; q < nb; ++q) col(n1, t + (size_t)q * n1);                               \
-            for (int i = 0; i < n1; ++i)                                                            \
-                for (int q = 0; q < nb; ++q) a[(size_t)i * n2 + j0 + q] = t[(size_t)q * n1 + i];    \
+        _Pragma("omp parallel num_threads(nt) if (nt > 1)")                                         \
+        {                                                                                           \
+            T* t = (T*)malloc(sizeof(T) * (size_t)n1 * B);                                          \
+            _Pragma("omp for schedule(static)")                                                     \
+            for (int j0 = 0; j0 < n2; j0 += B) {                                                    \
+                const int nb = n2 - j0 < B ? n2 - j0 : B;                                           \
+                for (int i = 0; i < n1; ++i)                                                        \
+                    for (int q = 0; q < nb; ++q) t[(size_t)q * n1 + i] = a[(size_t)i * n2 + j0 + q]; \
+                for (int q = 0; q < nb; ++q) col(n1, t + (size_t)q * n1);                           \
+                for (int i = 0; i < n1; ++i)                                                        \
+                    for (int q = 0; q < nb; ++q) a[(size_t)i * n2 + j0 + q] = t[(size_t)q * n1 + i]; \
+            }                                                                                       \
+            free(t);                                                                                \
         }                                                                                           \
-        free(t);                                                                                    \
     }                                                                                               \
     void orc_ddct2d_fwd_##SFX(int n1, int n2, T* a) { xt2d_##SFX(n1, n2, a, orc_dct2_##SFX, orc_dct2_##SFX); } \
     void orc_ddct2d_inv_##SFX(int n1, int n2, T* a) { xt2d_##SFX(n1, n2, a, orc_dct3_##SFX, orc_dct3_##SFX); } \

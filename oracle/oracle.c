@@ -58,6 +58,13 @@ float orc_fast_exp(float x) {
     return e;
 }
 
+/* Host threads for the loop's independent per-cell, per-net, per-row and per-bin work (1 by
+ * default: the reference is single-threaded).  Every sum whose order the reference fixes stays
+ * sequential, and the bins' integer sums are exact in any order, so any count gives the same bits. */
+static int g_threads = 1;
+void orc_set_threads(int32_t n) { g_threads = n > 1 ? n : 1; }
+int orc_threads(void) { return g_threads; }
+
 /* Wide sums (a test switch, off by default): the loop's global float sums -- the step length's
  * distances, the gradients' |wl| and |den|, the bins' overflow -- accumulated in double instead of
  * the reference's float, the precision of the GPU path's FP64 reductions.  It separates what the
@@ -555,6 +562,9 @@ static void update_bins_density_area(orc_np* np, orc_nb* nb) {
     memset(nb->filler, 0, 8 * M);
     int cnt_x = nb->R.cnt_x;
     float td = nb->R.target_density;
+    /* the integer areas: with several threads the adds are atomic (exact in any order) */
+    const int par = g_threads > 1;
+#pragma omp parallel for num_threads(g_threads) if (par) schedule(static)
     for (int i = 0; i < nb->n; ++i) {
         int g = nb->cells[i], x0, x1, y0, y1;
         didx_x(nb, np->dlx[g], np->dux[g], &x0, &x1);
@@ -564,11 +574,13 @@ static void update_bins_density_area(orc_np* np, orc_nb* nb) {
             for (int x = x0; x < x1; ++x) {
                 float a = overlap_density_area(nb->bin_x[x], nb->bin_y[y], nb->bin_x[x + 1], nb->bin_y[y + 1], np->dlx[g], np->dly[g], np->dux[g], np->duy[g]) * np->dscale[g];
                 size_t k = (size_t)y * cnt_x + x;
-                if (inst) {
-                    if (macro) a *= td;
-                    nb->inst[k] += (int64_t)a;
+                if (inst && macro) a *= td;
+                int64_t* dst = inst ? &nb->inst[k] : &nb->filler[k];
+                if (par) {
+#pragma omp atomic
+                    *dst += (int64_t)a;
                 } else {
-                    nb->filler[k] += (int64_t)a;
+                    *dst += (int64_t)a;
                 }
             }
     }
@@ -598,6 +610,7 @@ static void update_bins_density_area(orc_np* np, orc_nb* nb) {
 
 /* nesterov.rs:2268 updateGCellDensityCenterLocation then the bins. */
 static void update_center_location(orc_np* np, orc_nb* nb, const fpt* c) {
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
     for (int i = 0; i < nb->n; ++i) set_density_center(np, nb->cells[i], (int)c[i].x, (int)c[i].y);
     update_bins_density_area(np, nb);
 }
@@ -615,9 +628,12 @@ static float noise_factor(orc_np* np) {
 static void update_density_field_bin(orc_np* np, orc_nb* nb) {
     int nx = nb->R.cnt_x, ny = nb->R.cnt_y;
     float *a = nb->fft_a, *ph = nb->fft_b, *fx = nb->fft_c, *fy = nb->fft_d;
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) a[(size_t)x * ny + y] = nb->density[(size_t)y * nx + x];
     orc_fft_solve_f32(nx, ny, nb->wx, nb->wy, a, ph, fx, fy);
+    /* (the noise probe draws its factors in order: one thread) */
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1 && !(np && np->field_noise != 0.0f)) schedule(static)
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
             size_t s = (size_t)x * ny + y, d = (size_t)y * nx + x;
@@ -656,6 +672,7 @@ static void update_box(orc_np* np, int n) {
 /* nesterov.rs:2297 getHpwl. */
 int64_t orc_np_hpwl(orc_np* np) {
     int64_t total = 0;
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static) reduction(+ : total)
     for (int n = 0; n < np->nnets; ++n) {
         update_box(np, n);
         int32_t* b = &np->nbox[4 * n];
@@ -670,6 +687,8 @@ static void update_wirelength_force_wa(orc_np* np) {
     float cx = np->wl_coef_x, cy = np->wl_coef_y;
     memset(np->has, 0, np->npins);
     memset(np->pexp, 0, sizeof(float) * 4 * (size_t)np->npins);
+    /* nets are independent (a pin is on one net) */
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(dynamic, 256)
     for (int n = 0; n < np->nnets; ++n) {
         float* s = &np->nsum[8 * n];
         for (int q = 0; q < 8; ++q) s[q] = 0.0f;
@@ -808,8 +827,11 @@ static void update_gradients(orc_np* np, orc_nb* nb, int slot) {
     if (nb->is_converged) return;
     int base = slot == SLOT_PREV ? S_PREV_SLP : slot == SLOT_CUR ? S_CUR_SLP : S_NEXT_SLP;
     fpt *wl = nb->v[base + 1], *den = nb->v[base + 2], *sum = nb->v[base + 3];
-    for (int i = 0; i < nb->n; ++i) wl[i] = wl_gradient(np, nb->cells[i]);
-    for (int i = 0; i < nb->n; ++i) den[i] = density_gradient(np, nb, nb->cells[i]);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
+    for (int i = 0; i < nb->n; ++i) {
+        wl[i] = wl_gradient(np, nb->cells[i]);
+        den[i] = density_gradient(np, nb, nb->cells[i]);
+    }
     int32_t* pins = (int32_t*)xcalloc((size_t)nb->n, 4);
     int32_t* dxy = (int32_t*)xcalloc(2 * (size_t)nb->n, 4);
     for (int i = 0; i < nb->n; ++i) {
