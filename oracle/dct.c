@@ -288,27 +288,36 @@ static int fast_len(int n) { return is_pow2(n) || (n % 6 == 0 && is_pow2(n / 6))
         for (int k = 1; k < n; k += 2) a[k] = -a[k];                                                \
     }                                                                                               \
     /* 2-D: rows (length n2) with `row`, then columns (length n1) with `col` */                     \
-    /* (rows, and column blocks, are independent: with several host threads each takes its own) */ \
+    /* columns j0 .. j0 + 15 gathered together (runs of 16 consecutive values per row) into t */  \
+    enum { B_##SFX = 16 };                                                                          \
+    static void colblock_##SFX(int n1, int n2, T* a, T* t, int j0, void (*col)(int, T*)) {         \
+        const int nb = n2 - j0 < B_##SFX ? n2 - j0 : B_##SFX;                                       \
+        for (int i = 0; i < n1; ++i)                                                                \
+            for (int q = 0; q < nb; ++q) t[(size_t)q * n1 + i] = a[(size_t)i * n2 + j0 + q];        \
+        for (int q = 0; q < nb; ++q) col(n1, t + (size_t)q * n1);                                   \
+        for (int i = 0; i < n1; ++i)                                                                \
+            for (int q = 0; q < nb; ++q) a[(size_t)i * n2 + j0 + q] = t[(size_t)q * n1 + i];        \
+    }                                                                                               \
+    /* 2-D: rows (length n2) with `row`, then columns (length n1) with `col`; rows, and column      \
+     * blocks, are independent: with several host threads each takes its own */                    \
     static void xt2d_##SFX(int n1, int n2, T* a, void (*row)(int, T*), void (*col)(int, T*)) {      \
         const int nt = orc_threads();                                                               \
-        _Pragma("omp parallel for num_threads(nt) if (nt > 1) schedule(static)")                    \
-        for (int i = 0; i < n1; ++i) row(n2, a + (size_t)i * n2);                                   \
-        /* columns in blocks of 16 gathered together (runs of 16 consecutive values per row) */     \
-        enum { B = 16 };                                                                            \
-        _Pragma("omp parallel num_threads(nt) if (nt > 1)")                                         \
-        {                                                                                           \
-            T* t = (T*)malloc(sizeof(T) * (size_t)n1 * B);                                          \
-            _Pragma("omp for schedule(static)")                                                     \
-            for (int j0 = 0; j0 < n2; j0 += B) {                                                    \
-                const int nb = n2 - j0 < B ? n2 - j0 : B;                                           \
-                for (int i = 0; i < n1; ++i)                                                        \
-                    for (int q = 0; q < nb; ++q) t[(size_t)q * n1 + i] = a[(size_t)i * n2 + j0 + q]; \
-                for (int q = 0; q < nb; ++q) col(n1, t + (size_t)q * n1);                           \
-                for (int i = 0; i < n1; ++i)                                                        \
-                    for (int q = 0; q < nb; ++q) a[(size_t)i * n2 + j0 + q] = t[(size_t)q * n1 + i]; \
+        if (nt > 1) {                                                                               \
+            _Pragma("omp parallel for num_threads(nt) schedule(static)")                            \
+            for (int i = 0; i < n1; ++i) row(n2, a + (size_t)i * n2);                               \
+            _Pragma("omp parallel num_threads(nt)")                                                 \
+            {                                                                                       \
+                T* t = (T*)malloc(sizeof(T) * (size_t)n1 * B_##SFX);                                \
+                _Pragma("omp for schedule(static)")                                                 \
+                for (int j0 = 0; j0 < n2; j0 += B_##SFX) colblock_##SFX(n1, n2, a, t, j0, col);     \
+                free(t);                                                                            \
             }                                                                                       \
-            free(t);                                                                                \
+            return;                                                                                 \
         }                                                                                           \
+        for (int i = 0; i < n1; ++i) row(n2, a + (size_t)i * n2);                                   \
+        T* t = (T*)malloc(sizeof(T) * (size_t)n1 * B_##SFX);                                        \
+        for (int j0 = 0; j0 < n2; j0 += B_##SFX) colblock_##SFX(n1, n2, a, t, j0, col);             \
+        free(t);                                                                                    \
     }                                                                                               \
     void orc_ddct2d_fwd_##SFX(int n1, int n2, T* a) { xt2d_##SFX(n1, n2, a, orc_dct2_##SFX, orc_dct2_##SFX); } \
     void orc_ddct2d_inv_##SFX(int n1, int n2, T* a) { xt2d_##SFX(n1, n2, a, orc_dct3_##SFX, orc_dct3_##SFX); } \

@@ -75,18 +75,21 @@ void orc_set_wide_sums(int32_t on) { g_wide_sums = on ? 1 : 0; }
 /* nesterov.rs:319 getDistance: a float sum of squared differences in point order, then
  * sqrt(sum / (2 n)) in double.  a, b: n points, x and y interleaved. */
 float orc_distance(const float* a, const float* b, int32_t n) {
-    float sum = 0.0f;
-    double wide = 0.0;
-    for (int32_t i = 0; i < n; ++i) {
-        const float dx = (a[2 * i] - b[2 * i]) * (a[2 * i] - b[2 * i]);
-        const float dy = (a[2 * i + 1] - b[2 * i + 1]) * (a[2 * i + 1] - b[2 * i + 1]);
-        sum += dx;
-        sum += dy;
-        wide += (double)dx;
-        wide += (double)dy;
-    }
     if (n == 0) return 0.0f;
-    return (float)sqrt((g_wide_sums ? wide : (double)sum) / (2.0 * (double)n));
+    if (g_wide_sums) {
+        double wide = 0.0;
+        for (int32_t i = 0; i < n; ++i) {
+            wide += (double)((a[2 * i] - b[2 * i]) * (a[2 * i] - b[2 * i]));
+            wide += (double)((a[2 * i + 1] - b[2 * i + 1]) * (a[2 * i + 1] - b[2 * i + 1]));
+        }
+        return (float)sqrt(wide / (2.0 * (double)n));
+    }
+    float sum = 0.0f;
+    for (int32_t i = 0; i < n; ++i) {
+        sum += (a[2 * i] - b[2 * i]) * (a[2 * i] - b[2 * i]);
+        sum += (a[2 * i + 1] - b[2 * i + 1]) * (a[2 * i + 1] - b[2 * i + 1]);
+    }
+    return (float)sqrt((double)sum / (2.0 * (double)n));
 }
 
 /* nesterov.rs:96 derived_reference_hpwl. */
@@ -562,27 +565,42 @@ static void update_bins_density_area(orc_np* np, orc_nb* nb) {
     memset(nb->filler, 0, 8 * M);
     int cnt_x = nb->R.cnt_x;
     float td = nb->R.target_density;
-    /* the integer areas: with several threads the adds are atomic (exact in any order) */
-    const int par = g_threads > 1;
-#pragma omp parallel for num_threads(g_threads) if (par) schedule(static)
-    for (int i = 0; i < nb->n; ++i) {
-        int g = nb->cells[i], x0, x1, y0, y1;
-        didx_x(nb, np->dlx[g], np->dux[g], &x0, &x1);
-        didx_y(nb, np->dly[g], np->duy[g], &y0, &y1);
-        int inst = np->flags[g] & GPL_CELL_INSTANCE, macro = inst && (np->flags[g] & GPL_CELL_MACRO);
-        for (int y = y0; y < y1; ++y)
-            for (int x = x0; x < x1; ++x) {
-                float a = overlap_density_area(nb->bin_x[x], nb->bin_y[y], nb->bin_x[x + 1], nb->bin_y[y + 1], np->dlx[g], np->dly[g], np->dux[g], np->duy[g]) * np->dscale[g];
-                size_t k = (size_t)y * cnt_x + x;
-                if (inst && macro) a *= td;
-                int64_t* dst = inst ? &nb->inst[k] : &nb->filler[k];
-                if (par) {
+    if (g_threads > 1) {
+        /* the same integer areas from several threads, the adds atomic (exact in any order) */
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int i = 0; i < nb->n; ++i) {
+            int g = nb->cells[i], x0, x1, y0, y1;
+            didx_x(nb, np->dlx[g], np->dux[g], &x0, &x1);
+            didx_y(nb, np->dly[g], np->duy[g], &y0, &y1);
+            int inst = np->flags[g] & GPL_CELL_INSTANCE, macro = inst && (np->flags[g] & GPL_CELL_MACRO);
+            for (int y = y0; y < y1; ++y)
+                for (int x = x0; x < x1; ++x) {
+                    float a = overlap_density_area(nb->bin_x[x], nb->bin_y[y], nb->bin_x[x + 1], nb->bin_y[y + 1], np->dlx[g], np->dly[g], np->dux[g], np->duy[g]) * np->dscale[g];
+                    size_t k = (size_t)y * cnt_x + x;
+                    if (inst && macro) a *= td;
+                    int64_t* dst = inst ? &nb->inst[k] : &nb->filler[k];
 #pragma omp atomic
                     *dst += (int64_t)a;
-                } else {
-                    *dst += (int64_t)a;
                 }
-            }
+        }
+    } else {
+        for (int i = 0; i < nb->n; ++i) {
+            int g = nb->cells[i], x0, x1, y0, y1;
+            didx_x(nb, np->dlx[g], np->dux[g], &x0, &x1);
+            didx_y(nb, np->dly[g], np->duy[g], &y0, &y1);
+            int inst = np->flags[g] & GPL_CELL_INSTANCE, macro = inst && (np->flags[g] & GPL_CELL_MACRO);
+            for (int y = y0; y < y1; ++y)
+                for (int x = x0; x < x1; ++x) {
+                    float a = overlap_density_area(nb->bin_x[x], nb->bin_y[y], nb->bin_x[x + 1], nb->bin_y[y + 1], np->dlx[g], np->dly[g], np->dux[g], np->duy[g]) * np->dscale[g];
+                    size_t k = (size_t)y * cnt_x + x;
+                    if (inst) {
+                        if (macro) a *= td;
+                        nb->inst[k] += (int64_t)a;
+                    } else {
+                        nb->filler[k] += (int64_t)a;
+                    }
+                }
+        }
     }
     nb->overflow = 0;
     nb->overflow_unscaled = 0;
@@ -596,11 +614,13 @@ static void update_bins_density_area(orc_np* np, orc_nb* nb) {
             float o = (float)nb->inst[k] + (float)nb->non_place[k] - scaled;
             o = o > 0.0f ? o : 0.0f;
             nb->overflow = (int64_t)((float)nb->overflow + o);
-            wo += (double)o;
             float ou = (float)nb->inst[k] + (float)nb->non_place_unscaled[k] - scaled;
             ou = ou > 0.0f ? ou : 0.0f;
             nb->overflow_unscaled = (int64_t)((float)nb->overflow_unscaled + ou);
-            wou += (double)ou;
+            if (g_wide_sums) {
+                wo += (double)o;
+                wou += (double)ou;
+            }
         }
     if (g_wide_sums) {
         nb->overflow = (int64_t)wo;
@@ -610,8 +630,12 @@ static void update_bins_density_area(orc_np* np, orc_nb* nb) {
 
 /* nesterov.rs:2268 updateGCellDensityCenterLocation then the bins. */
 static void update_center_location(orc_np* np, orc_nb* nb, const fpt* c) {
-#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
-    for (int i = 0; i < nb->n; ++i) set_density_center(np, nb->cells[i], (int)c[i].x, (int)c[i].y);
+    if (g_threads > 1) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int i = 0; i < nb->n; ++i) set_density_center(np, nb->cells[i], (int)c[i].x, (int)c[i].y);
+    } else {
+        for (int i = 0; i < nb->n; ++i) set_density_center(np, nb->cells[i], (int)c[i].x, (int)c[i].y);
+    }
     update_bins_density_area(np, nb);
 }
 
@@ -625,26 +649,42 @@ static float noise_factor(orc_np* np) {
     return 1.0f + np->field_noise * ((float)(x & 0xffffu) / 32768.0f - 1.0f);
 }
 
-static void update_density_field_bin(orc_np* np, orc_nb* nb) {
-    int nx = nb->R.cnt_x, ny = nb->R.cnt_y;
-    float *a = nb->fft_a, *ph = nb->fft_b, *fx = nb->fft_c, *fy = nb->fft_d;
-#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
-    for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) a[(size_t)x * ny + y] = nb->density[(size_t)y * nx + x];
-    orc_fft_solve_f32(nx, ny, nb->wx, nb->wy, a, ph, fx, fy);
-    /* (the noise probe draws its factors in order: one thread) */
-#pragma omp parallel for num_threads(g_threads) if (g_threads > 1 && !(np && np->field_noise != 0.0f)) schedule(static)
-    for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) {
-            size_t s = (size_t)x * ny + y, d = (size_t)y * nx + x;
-            nb->field_x[d] = fx[s];
-            nb->field_y[d] = fy[s];
-            nb->phi[d] = ph[s];
-            if (np && np->field_noise != 0.0f) {
-                nb->field_x[d] *= noise_factor(np);
-                nb->field_y[d] *= noise_factor(np);
-            }
+static void density_to_xmajor(orc_nb* nb, float* a, int y) {
+    const int nx = nb->R.cnt_x, ny = nb->R.cnt_y;
+    for (int x = 0; x < nx; ++x) a[(size_t)x * ny + y] = nb->density[(size_t)y * nx + x];
+}
+static void fields_from_xmajor(orc_np* np, orc_nb* nb, const float* ph, const float* fx, const float* fy, int y) {
+    const int nx = nb->R.cnt_x, ny = nb->R.cnt_y;
+    for (int x = 0; x < nx; ++x) {
+        size_t s = (size_t)x * ny + y, d = (size_t)y * nx + x;
+        nb->field_x[d] = fx[s];
+        nb->field_y[d] = fy[s];
+        nb->phi[d] = ph[s];
+        if (np && np->field_noise != 0.0f) {
+            nb->field_x[d] *= noise_factor(np);
+            nb->field_y[d] *= noise_factor(np);
         }
+    }
+}
+
+static void update_density_field_bin(orc_np* np, orc_nb* nb) {
+    int ny = nb->R.cnt_y;
+    float *a = nb->fft_a, *ph = nb->fft_b, *fx = nb->fft_c, *fy = nb->fft_d;
+    /* (the noise probe draws its factors in order: one thread) */
+    const int par = g_threads > 1 && !(np && np->field_noise != 0.0f);
+    if (par) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int y = 0; y < ny; ++y) density_to_xmajor(nb, a, y);
+    } else {
+        for (int y = 0; y < ny; ++y) density_to_xmajor(nb, a, y);
+    }
+    orc_fft_solve_f32(nb->R.cnt_x, ny, nb->wx, nb->wy, a, ph, fx, fy);
+    if (par) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int y = 0; y < ny; ++y) fields_from_xmajor(np, nb, ph, fx, fy, y);
+    } else {
+        for (int y = 0; y < ny; ++y) fields_from_xmajor(np, nb, ph, fx, fy, y);
+    }
 }
 
 /* Turns the sensitivity probe on (eps > 0, stream seeded with `seed`) or off (eps = 0). */
@@ -670,63 +710,74 @@ static void update_box(orc_np* np, int n) {
 }
 
 /* nesterov.rs:2297 getHpwl. */
+static inline int64_t net_hpwl(orc_np* np, int n) {
+    update_box(np, n);
+    const int32_t* b = &np->nbox[4 * n];
+    return b[2] < b[0] ? 0 : ((int64_t)b[2] - b[0]) + ((int64_t)b[3] - b[1]);
+}
+
 int64_t orc_np_hpwl(orc_np* np) {
     int64_t total = 0;
-#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static) reduction(+ : total)
-    for (int n = 0; n < np->nnets; ++n) {
-        update_box(np, n);
-        int32_t* b = &np->nbox[4 * n];
-        if (b[2] < b[0]) continue;
-        total += ((int64_t)b[2] - b[0]) + ((int64_t)b[3] - b[1]);
+    if (g_threads > 1) {
+#pragma omp parallel for num_threads(g_threads) schedule(static) reduction(+ : total)
+        for (int n = 0; n < np->nnets; ++n) total += net_hpwl(np, n);
+    } else {
+        for (int n = 0; n < np->nnets; ++n) total += net_hpwl(np, n);
     }
     return total;
 }
 
-/* nesterov.rs:2325 updateWireLengthForceWA. */
+/* nesterov.rs:2325 updateWireLengthForceWA: one net (its box, its pins' exponentials and sums). */
+static inline void wa_net(orc_np* np, int n, float cx, float cy) {
+    float* s = &np->nsum[8 * n];
+    for (int q = 0; q < 8; ++q) s[q] = 0.0f;
+    update_box(np, n);
+    int nlx = np->nbox[4 * n], nly = np->nbox[4 * n + 1], nux = np->nbox[4 * n + 2], nuy = np->nbox[4 * n + 3];
+    for (int k = np->net_start[n]; k < np->net_start[n + 1]; ++k) {
+        int p = np->net_pins[k], px = np->pcx[p], py = np->pcy[p];
+        float emin_x = (float)(nlx - px) * cx, emax_x = (float)(px - nux) * cx;
+        float emin_y = (float)(nly - py) * cy, emax_y = (float)(py - nuy) * cy;
+        if (emin_x > -300.0f) {
+            float v = orc_fast_exp(emin_x);
+            np->has[p] |= 1;
+            np->pexp[4 * p] = v;
+            s[0] += v;
+            s[1] += (float)px * v;
+        }
+        if (emax_x > -300.0f) {
+            float v = orc_fast_exp(emax_x);
+            np->has[p] |= 2;
+            np->pexp[4 * p + 1] = v;
+            s[2] += v;
+            s[3] += (float)px * v;
+        }
+        if (emin_y > -300.0f) {
+            float v = orc_fast_exp(emin_y);
+            np->has[p] |= 4;
+            np->pexp[4 * p + 2] = v;
+            s[4] += v;
+            s[5] += (float)py * v;
+        }
+        if (emax_y > -300.0f) {
+            float v = orc_fast_exp(emax_y);
+            np->has[p] |= 8;
+            np->pexp[4 * p + 3] = v;
+            s[6] += v;
+            s[7] += (float)py * v;
+        }
+    }
+}
+
+/* nesterov.rs:2325 updateWireLengthForceWA (nets are independent: a pin is on one net). */
 static void update_wirelength_force_wa(orc_np* np) {
     float cx = np->wl_coef_x, cy = np->wl_coef_y;
     memset(np->has, 0, np->npins);
     memset(np->pexp, 0, sizeof(float) * 4 * (size_t)np->npins);
-    /* nets are independent (a pin is on one net) */
-#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(dynamic, 256)
-    for (int n = 0; n < np->nnets; ++n) {
-        float* s = &np->nsum[8 * n];
-        for (int q = 0; q < 8; ++q) s[q] = 0.0f;
-        update_box(np, n);
-        int nlx = np->nbox[4 * n], nly = np->nbox[4 * n + 1], nux = np->nbox[4 * n + 2], nuy = np->nbox[4 * n + 3];
-        for (int k = np->net_start[n]; k < np->net_start[n + 1]; ++k) {
-            int p = np->net_pins[k], px = np->pcx[p], py = np->pcy[p];
-            float emin_x = (float)(nlx - px) * cx, emax_x = (float)(px - nux) * cx;
-            float emin_y = (float)(nly - py) * cy, emax_y = (float)(py - nuy) * cy;
-            if (emin_x > -300.0f) {
-                float v = orc_fast_exp(emin_x);
-                np->has[p] |= 1;
-                np->pexp[4 * p] = v;
-                s[0] += v;
-                s[1] += (float)px * v;
-            }
-            if (emax_x > -300.0f) {
-                float v = orc_fast_exp(emax_x);
-                np->has[p] |= 2;
-                np->pexp[4 * p + 1] = v;
-                s[2] += v;
-                s[3] += (float)px * v;
-            }
-            if (emin_y > -300.0f) {
-                float v = orc_fast_exp(emin_y);
-                np->has[p] |= 4;
-                np->pexp[4 * p + 2] = v;
-                s[4] += v;
-                s[5] += (float)py * v;
-            }
-            if (emax_y > -300.0f) {
-                float v = orc_fast_exp(emax_y);
-                np->has[p] |= 8;
-                np->pexp[4 * p + 3] = v;
-                s[6] += v;
-                s[7] += (float)py * v;
-            }
-        }
+    if (g_threads > 1) {
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 256)
+        for (int n = 0; n < np->nnets; ++n) wa_net(np, n, cx, cy);
+    } else {
+        for (int n = 0; n < np->nnets; ++n) wa_net(np, n, cx, cy);
     }
 }
 
@@ -808,18 +859,25 @@ void orc_precondition(int32_t n, const float* wl, const float* den, const int32_
         sum[2 * i] = sx / pre;
         sum[2 * i + 1] = sy / pre;
     }
+    if (g_wide_sums) {
+        double wa = 0.0, wd = 0.0;
+        for (int32_t i = 0; i < n; ++i) {
+            wa += (double)fabsf(wl[2 * i]) + (double)fabsf(wl[2 * i + 1]);
+            wd += (double)fabsf(den[2 * i]) + (double)fabsf(den[2 * i + 1]);
+        }
+        *wl_abs = (float)wa;
+        *den_abs = (float)wd;
+        return;
+    }
     float a = 0.0f, d = 0.0f;
-    double wa = 0.0, wd = 0.0;
     for (int32_t i = 0; i < n; ++i) {
         a += fabsf(wl[2 * i]);
         a += fabsf(wl[2 * i + 1]);
         d += fabsf(den[2 * i]);
         d += fabsf(den[2 * i + 1]);
-        wa += (double)fabsf(wl[2 * i]) + (double)fabsf(wl[2 * i + 1]);
-        wd += (double)fabsf(den[2 * i]) + (double)fabsf(den[2 * i + 1]);
     }
-    *wl_abs = g_wide_sums ? (float)wa : a;
-    *den_abs = g_wide_sums ? (float)wd : d;
+    *wl_abs = a;
+    *den_abs = d;
 }
 
 /* nesterov.rs:2455 updateGradients into a slot. */
@@ -827,10 +885,15 @@ static void update_gradients(orc_np* np, orc_nb* nb, int slot) {
     if (nb->is_converged) return;
     int base = slot == SLOT_PREV ? S_PREV_SLP : slot == SLOT_CUR ? S_CUR_SLP : S_NEXT_SLP;
     fpt *wl = nb->v[base + 1], *den = nb->v[base + 2], *sum = nb->v[base + 3];
-#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
-    for (int i = 0; i < nb->n; ++i) {
-        wl[i] = wl_gradient(np, nb->cells[i]);
-        den[i] = density_gradient(np, nb, nb->cells[i]);
+    if (g_threads > 1) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int i = 0; i < nb->n; ++i) {
+            wl[i] = wl_gradient(np, nb->cells[i]);
+            den[i] = density_gradient(np, nb, nb->cells[i]);
+        }
+    } else {
+        for (int i = 0; i < nb->n; ++i) wl[i] = wl_gradient(np, nb->cells[i]);
+        for (int i = 0; i < nb->n; ++i) den[i] = density_gradient(np, nb, nb->cells[i]);
     }
     int32_t* pins = (int32_t*)xcalloc((size_t)nb->n, 4);
     int32_t* dxy = (int32_t*)xcalloc(2 * (size_t)nb->n, 4);
