@@ -321,8 +321,9 @@ int32_t gpl_do_nesterov_place(gpl_ctx* c, int32_t* iter_out);
  * the same shape, are grouped; the kernels' blocks take each context's arguments in turn), and
  * each context takes the rest of its iteration (its decisions, further trials, re-sorts) itself.
  * Each context's placement, rows and statistics are its own gpl_do_nesterov_place's, bit for bit.
- * iters (n entries, may be null) gets the final iterations.  Contexts with routability or
- * profiling on run one after another instead. */
+ * iters (n entries, may be null) gets the final iterations.  Returns 0 or a negative API / HIP
+ * error; a context's own outcome (a divergence's GPL code included) is in its gpl_get_stats.
+ * Contexts with routability or profiling on run one after another instead. */
 int32_t gpl_place_batch(gpl_ctx* const* ctxs, int32_t n, int32_t* iters);
 /* nesterov.rs:4243: the loop again with a new target overflow and iteration limit. */
 int32_t gpl_continue_nesterov_place(gpl_ctx* c, float overflow, int32_t max_iter, int32_t start_iter,

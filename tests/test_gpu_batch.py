@@ -17,7 +17,9 @@ STAT_KEYS = ("iter", "converged", "status", "total_trials", "hpwl", "overflow", 
 
 def _result(g, it):
     s = g.stats()
-    return dict(it=it, hist=g.history(), log=g.log(), stats={k: getattr(s, k) for k in STAT_KEYS}, loc=g.placed_locations())
+    # (a NaN sum, as a diverged run leaves it, compares as the string "nan")
+    stats = {k: (v if v == v else "nan") for k, v in ((k, getattr(s, k)) for k in STAT_KEYS)}
+    return dict(it=it, hist=g.history(), log=g.log(), stats=stats, loc=g.placed_locations())
 
 
 def _contexts(cfg, seeds):
@@ -38,23 +40,31 @@ def _same(a, b):
     assert np.array_equal(a["loc"], b["loc"])
 
 
-def _check(specs, profiling=False):
-    """specs: (config, seeds) pairs; the contexts placed one by one, then as one batch."""
+def _check(specs, profiling=False, inject=None):
+    """specs: (config, seeds) pairs; the contexts placed one by one, then as one batch.  inject:
+    {context index: [(debug kind, after iteration)]} forced on that context in both."""
+    inject = inject or {}
     designs, alone, batch = [], [], []
     try:
+        k = 0
         for cfg, seeds in specs:
             d, cs = _contexts(cfg, seeds)
             designs.append(d)
             for g in cs:
-                it = g.do_nesterov_place()
+                for kind, at in inject.get(k, []):
+                    g.debug_inject(kind, at)
+                it = g.run_nesterov_loop(check=False)[1] if g.init_nesterov_loop() == 0 else None
                 alone.append(_result(g, it))
                 g.close()
+                k += 1
         for (cfg, seeds), d in zip(specs, designs):
             for seed in seeds:
                 d.init_nesterov_place(G.default_opts(**dict(G.CONFIGS[cfg]["opts"], seed=seed)))
                 g = G.NesterovGPU(d)
                 if profiling:
                     g.set_profiling(True)
+                for kind, at in inject.get(len(batch), []):
+                    g.debug_inject(kind, at)
                 batch.append(g)
         its = G.place_batched(batch)
         assert len(its) == len(alone)
@@ -85,3 +95,10 @@ def test_designs_of_two_shapes_in_one_batch():
 
 def test_a_batch_that_cannot_be_grouped_runs_one_by_one():
     _check([("c1", (1, 2))], profiling=True)
+
+
+def test_contexts_that_diverge_or_resort_every_iteration_keep_their_own_outcome():
+    """In one batch: a context whose gradient sum turns NaN at iteration 5 (GPL-0306: it leaves the
+    batch there), one whose step length does at 30 (GPL-0305), one re-sorting its cells every
+    iteration (its arrays swapped between the grouped launches), and two plain ones."""
+    _check([("c1", (1, 2, 3, 4, 5))], inject={1: [(1, 5)], 2: [(3, 0)], 3: [(2, 30)]})
