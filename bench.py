@@ -247,7 +247,7 @@ def load_fixture(cfg):
         return None
 
 
-def batched(cfg, k, emit=True):
+def batched(cfg, k, emit=True, batch_only=False):
     """K independent placements (seeds 1..K: their own perturbations) of one design: run one
     after another, then as one lockstep batch (gpl_place_batch: each iteration's first trial of
     all K as one launch per kernel), and for comparison at once from host threads (one context
@@ -271,6 +271,15 @@ def batched(cfg, k, emit=True):
     a = contexts()
     p = d.problem()
     log(f"[bench] {cfg} x {k} seeds: cells {p.num_cells} nets {p.num_nets}; setup {time.perf_counter() - t:.1f}s")
+    if batch_only:
+        t0 = time.perf_counter()
+        it_bat = G.place_batched(a)
+        t_bat = time.perf_counter() - t0
+        for g in a:
+            g.close()
+        d.close()
+        print(json.dumps({"batch_only": True, "config": cfg, "seeds": k, "seconds": round(t_bat, 4), "iterations": it_bat}), flush=True)
+        return None
     t0 = time.perf_counter()
     it_seq = [g.do_nesterov_place() for g in a]
     t_seq = time.perf_counter() - t0
@@ -372,6 +381,11 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="np_init + W + K iterations of the profiled (serial) path only, the run rocprofv3 wraps; "
                          "no convergence, no CPU leg")
+    ap.add_argument("--loop-only", action="store_true",
+                    help="np_init + W + K iterations of the timed (graph) path only, the run rocprofv3 wraps for the "
+                         "production kernels; no convergence, no CPU leg")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="with --seeds: the lockstep batch alone (no sequential or concurrent runs), for rocprofv3")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a plain device copy of this many MiB three times and exit (the HBM counter "
                          "calibration under rocprofv3 --pmc)")
@@ -402,7 +416,7 @@ def main():
         return
 
     if args.seeds:
-        batched(args.config, args.seeds)
+        batched(args.config, args.seeds, batch_only=args.batch_only)
         return
     if args.config != "c3":
         args.no_batched = True
@@ -447,12 +461,12 @@ def main():
     t1 = time.perf_counter()
     steps = it - it0 if not done else it - it0 + 1
     log(f"[bench] timed {steps} iterations in {t1 - t0:.3f}s")
-    if args.profile_only:
+    if args.profile_only or args.loop_only:
         kt = g.kernel_times()
         g.close()
         fam = {k: {"launches": v[0], "avg_us": round(1000.0 * v[1] / max(v[0], 1), 2)} for k, v in kt.items()
                if k in FAMILY_KERNELS}
-        print(json.dumps({"profile_only": True, "config": args.config, "steps": steps,
+        print(json.dumps({"profile_only" if args.profile_only else "loop_only": True, "config": args.config, "steps": steps,
                           "iters_per_s": round(steps / (t1 - t0), 2), "families": fam}), flush=True)
         d.close()
         return
