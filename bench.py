@@ -374,10 +374,11 @@ def main():
     ap.add_argument("--hpwl-config", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--no-batched", action="store_true", help="skip the 16-seed batched comparison")
     ap.add_argument("--no-routability", action="store_true", help="skip the routability-driven loop on the 500k-cell design")
-    ap.add_argument("--launch-mode", type=int, default=0, choices=(0, 1, 2, 3, 4, 5, 6),
+    ap.add_argument("--launch-mode", type=int, default=0, choices=(0, 1, 2, 3, 4, 5, 6, 7),
                     help="0: each trial one HIP graph (branches on side streams); 1: kernels launched one by "
                          "one on the side streams; 2: one by one on one stream; 3: one graph captured on one stream; "
-                         "4: device-driven trials (the graph's control kernel takes the loop's decisions)")
+                         "4: device-driven trials (the graph's control kernel takes the loop's decisions); "
+                         "7: grouped trials (one launch per kernel variant and step over the regions)")
     ap.add_argument("--profile-only", action="store_true",
                     help="np_init + W + K iterations of the profiled (serial) path only, the run rocprofv3 wraps; "
                          "no convergence, no CPU leg")

@@ -366,7 +366,8 @@ class NesterovGPU:
         """0: one HIP graph per back-tracking trial (default); 1: kernels one by one on the side
         streams; 2: one by one on one stream; 3: one graph captured on one stream; 4-6:
         device-driven trials (the loop's decisions taken by the trial's last kernel) as a graph,
-        one by one on the side streams, on one stream.  The results do not depend on it."""
+        one by one on the side streams, on one stream; 7: grouped trials (one launch per kernel
+        variant and step over the regions).  The results do not depend on it."""
         _ok(_ffi.lib().gpl_set_launch_mode(self._h, mode), "gpl_set_launch_mode", self._h)
 
     def set_tuning(self, region, chunk, table_bits, stage):

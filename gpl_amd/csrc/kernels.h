@@ -478,14 +478,17 @@ hipError_t set_update_node(hipGraphExec_t e, const TrialNode& n, const RegionDev
                            unsigned long long* zero);
 hipError_t set_scatter_node(hipGraphExec_t e, const TrialNode& n, const RegionDev& r, const CellsDev& c,
                             unsigned long long* global_adds);
-// ---- one trial of several placements of one shape as one launch per kernel ------------------
+// ---- trials recorded and launched grouped: one launch per kernel variant ----------------------
 // While a recorder is set on the calling thread, the trial's launchers (the update, the density
 // scatter, the solve's row passes with the bins pass fused, the WA kernels, the gradient, the
 // finishing reduction) record their kernel variants, grids and arguments instead of launching; a
 // launch with no grouped form (any other launcher, or a variant without one) marks the record
-// failed and launches nothing.  launch_grouped runs records of the same sequence as one launch
-// per kernel, whose blocks take the records' arguments in turn (record j: blocks j * grid ..
-// (j + 1) * grid - 1), after one copy of the arguments to the device.
+// failed and launches nothing.  launch_grouped runs the records of one or more recorders (one
+// trial of each of several placements, each with one or more regions) as one launch per kernel
+// variant and step: every region's chain (update, scatter, row passes, gradient, finishing
+// reduction) in its order, the WA kernels after every update, the finishing reductions in each
+// recorder's order; a grouped launch's blocks take each record's arguments in turn (record g:
+// its own grid's worth of blocks), after one copy of the arguments to the device.
 constexpr int kRecArgBytes = 768;
 struct LaunchRec {
     int kind, grid, block;
@@ -498,12 +501,11 @@ struct TrialRecorder {
     bool failed = false;
 };
 void set_recorder(TrialRecorder* r);
-bool same_sequence(const TrialRecorder& a, const TrialRecorder& b);
-size_t grouped_arg_bytes(const TrialRecorder& r, int n);  // the staging bytes for n records like r
-// recs[0 .. n): records of one sequence; host_stage (pinned) and dev_stage hold
-// grouped_arg_bytes(*recs[0], n) bytes; the host stage must not change until the copy is done.
+size_t grouped_arg_bytes(const TrialRecorder* const* recs, int n);  // the staging bytes launch_grouped needs
+// recs[0 .. n): recorders none of which failed; host_stage (pinned) and dev_stage hold
+// grouped_arg_bytes(recs, n) bytes; the host stage must not change until the copy is done.
 // With a side stream (and two events), the WA kernels run there beside the scatter, bins and
-// solve, from the update on, and the gradient waits for them (as in the trial graph).
+// solve, from the updates on, and the gradients wait for them (as in the trial graph).
 hipError_t launch_grouped(hipStream_t s, const TrialRecorder* const* recs, int n, unsigned char* host_stage, unsigned char* dev_stage,
                           hipStream_t side = nullptr, hipEvent_t ev_fork = nullptr, hipEvent_t ev_join = nullptr);
 

@@ -3121,39 +3121,59 @@ void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v) 
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, p, v);
 }
 
-// ---- the grouped kernels: record g of a launch takes blocks g * per .. (g + 1) * per - 1 ----------
-__global__ void __launch_bounds__(kBlock) k_group_update(const UpdateArgsG* t, unsigned per) {
-    const unsigned g = blockIdx.x / per;
-    place_update_body(t[g].r, t[g].c, t[g].pa, blockIdx.x - g * per, per);
+// ---- the grouped kernels: record g of a launch takes the blocks boff[g] .. boff[g+1]-1, as block
+// blockIdx.x - boff[g] of boff[g+1] - boff[g] (with per > 0: blocks g per .. (g+1) per - 1) ---------
+constexpr int kGroupMax = 63;  // records per grouped launch (one wave's ballot finds a block's record)
+struct GroupPos {
+    unsigned g, bid, nb;
+};
+__device__ __forceinline__ GroupPos group_pos(unsigned per, const int* boff, int n) {
+    if (per) {
+        const unsigned g = blockIdx.x / per;
+        return GroupPos{g, blockIdx.x - g * per, per};
+    }
+    const int lane = threadIdx.x & 63;
+    const int v = lane <= n ? boff[lane] : INT_MAX;
+    const int g = (int)__popcll(__ballot(v <= (int)blockIdx.x)) - 1;
+    const int lo = __shfl(v, g, 64), hi = __shfl(v, g + 1, 64);
+    return GroupPos{(unsigned)g, blockIdx.x - (unsigned)lo, (unsigned)(hi - lo)};
+}
+
+__global__ void __launch_bounds__(kBlock) k_group_update(const UpdateArgsG* t, unsigned per, const int* boff, int n) {
+    const GroupPos q = group_pos(per, boff, n);
+    place_update_body(t[q.g].r, t[q.g].c, t[q.g].pa, q.bid, q.nb);
 }
 
 template <typename V, int CH>
-__global__ void __launch_bounds__(kBlock) k_group_scatter(const UpdateArgsG* t, unsigned per) {
-    const unsigned g = blockIdx.x / per;
-    place_scatter_body<PLACE_STORED, V, CH>(t[g].r, t[g].c, t[g].pa, blockIdx.x - g * per, per);
+__global__ void __launch_bounds__(kBlock) k_group_scatter(const UpdateArgsG* t, unsigned per, const int* boff, int n) {
+    const GroupPos q = group_pos(per, boff, n);
+    place_scatter_body<PLACE_STORED, V, CH>(t[q.g].r, t[q.g].c, t[q.g].pa, q.bid, q.nb);
 }
 
 template <int MODE, int LOGL>
-__global__ void __launch_bounds__(256) k_group_rows(const RowArgs* t, unsigned per) {
-    const unsigned g = blockIdx.x / per;
-    rows_body<MODE, true, LOGL>(t[g], blockIdx.x - g * per, per);
+__global__ void __launch_bounds__(256) k_group_rows(const RowArgs* t, unsigned per, const int* boff, int n) {
+    const GroupPos q = group_pos(per, boff, n);
+    rows_body<MODE, true, LOGL>(t[q.g], q.bid, q.nb);
 }
 
-__global__ void __launch_bounds__(kBlock) k_group_wa_tiles(const WaArgsG* t, unsigned per) {
-    const unsigned g = blockIdx.x / per;
-    wa_tiles_body<true>(t[g].n, t[g].c, t[g].cx, t[g].cy, t[g].hpwl, t[g].write_rec, blockIdx.x - g * per, per);
+__global__ void __launch_bounds__(kBlock) k_group_wa_tiles(const WaArgsG* t, unsigned per, const int* boff, int n) {
+    const GroupPos q = group_pos(per, boff, n);
+    const WaArgsG& a = t[q.g];
+    wa_tiles_body<true>(a.n, a.c, a.cx, a.cy, a.hpwl, a.write_rec, q.bid, q.nb);
 }
 
-__global__ void __launch_bounds__(kBigBlock) k_group_wa_big(const WaArgsG* t, unsigned per) {
-    const unsigned g = blockIdx.x / per;
-    wa_big_body<true>(t[g].n, t[g].c, t[g].cx, t[g].cy, t[g].hpwl, t[g].write_rec, blockIdx.x - g * per, per);
+__global__ void __launch_bounds__(kBigBlock) k_group_wa_big(const WaArgsG* t, unsigned per, const int* boff, int n) {
+    const GroupPos q = group_pos(per, boff, n);
+    const WaArgsG& a = t[q.g];
+    wa_big_body<true>(a.n, a.c, a.cx, a.cy, a.hpwl, a.write_rec, q.bid, q.nb);
 }
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_group_gradient(const GradArgsG* t, unsigned per) {
-    const unsigned g = blockIdx.x / per;
-    const GradArgsG& a = t[g];
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
+k_group_gradient(const GradArgsG* t, unsigned per, const int* boff, int n) {
+    const GroupPos q = group_pos(per, boff, n);
+    const GradArgsG& a = t[q.g];
     gradient_body(a.r, a.c, a.nt, a.cx, a.cy, a.penalty, a.wl_out, a.den_out, a.sum_out, a.a_slp, a.b_slp, a.a_sum, a.partials, a.nan_inject,
-                  blockIdx.x - g * per, per);
+                  q.bid, q.nb);
 }
 
 __global__ void __launch_bounds__(kFinishBlock) k_group_finish(const FinishArgsG* t) {
@@ -3166,38 +3186,116 @@ __global__ void k_stage_copy(const uint4* src, uint4* dst, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
-bool same_sequence(const TrialRecorder& a, const TrialRecorder& b) {
-    if (a.failed || b.failed || a.recs.size() != b.recs.size()) return false;
-    for (size_t i = 0; i < a.recs.size(); ++i) {
-        const LaunchRec &p = a.recs[i], &q = b.recs[i];
-        if (p.kind != q.kind || p.grid != q.grid || p.block != q.block || p.lds != q.lds || p.bytes != q.bytes) return false;
-    }
-    return true;
-}
-
 static size_t round16(size_t b) { return (b + 15) & ~(size_t)15; }
 
-size_t grouped_arg_bytes(const TrialRecorder& r, int n) {
+// The order a trial's kernels run in when grouped: every region's chain (update, scatter, the
+// solve's three row passes, the gradient, the finishing reduction) keeps its order, the WA kernels
+// come after every update; the finishing reductions last.
+static int phase_of(int k) {
+    if (k == RK_UPDATE) return 0;
+    if (k >= RK_SCATTER && k < RK_SCATTER + 4) return 1;
+    if (k >= RK_ROWS) {
+        const int mode = (k - RK_ROWS) / 4;
+        return mode == ROW_BINS_DCT2 ? 2 : mode == ROW_SPECTRUM_POISSON ? 3 : 4;
+    }
+    if (k == RK_WA_BIG) return 5;
+    if (k == RK_WA_TILES) return 6;
+    if (k == RK_GRADIENT) return 7;
+    return 8;
+}
+
+// The grouped launches of a set of records: per phase, the records of one kernel variant (kind,
+// block, LDS) together, at most kGroupMax each; the finishing reductions grouped by their place
+// among their recorder's finishes (so that each recorder's last, the one that signals the host,
+// runs after its others).  Each group's arguments, then its block offsets, in the staging.
+struct PlanGroup {
+    int kind, block, phase;
+    unsigned lds;
+    unsigned per;  // every member's grid when they are equal, else 0 (the offsets table)
+    int blocks;
+    std::vector<const LaunchRec*> mem;
+    size_t args_off, boff_off;
+};
+
+static std::vector<PlanGroup> plan_grouped(const TrialRecorder* const* recs, int n, size_t& bytes) {
+    std::vector<PlanGroup> gs;
+    std::vector<int> open;  // the group each (phase, kind, block, lds) or finish ordinal is filling
+    for (int j = 0; j < n; ++j) {
+        int fin = 0;
+        for (const LaunchRec& q : recs[j]->recs) {
+            const int ph = phase_of(q.kind);
+            const int ord = ph == 8 ? fin++ : 0;
+            int at = -1;
+            for (int o : open) {
+                const PlanGroup& g = gs[(size_t)o];
+                if (g.phase == ph && g.kind == q.kind && g.block == q.block && g.lds == q.lds && (int)g.mem.size() < kGroupMax &&
+                    (ph != 8 || g.args_off == (size_t)ord)) {
+                    at = o;
+                    break;
+                }
+            }
+            if (at < 0) {
+                PlanGroup g{};
+                g.kind = q.kind;
+                g.block = q.block;
+                g.phase = ph;
+                g.lds = q.lds;
+                g.args_off = (size_t)ord;  // (the finishes' ordinal until the layout below)
+                gs.push_back(g);
+                at = (int)gs.size() - 1;
+                open.push_back(at);
+            }
+            gs[(size_t)at].mem.push_back(&q);
+        }
+    }
+    std::stable_sort(gs.begin(), gs.end(), [](const PlanGroup& a, const PlanGroup& b) {
+        if (a.phase != b.phase) return a.phase < b.phase;
+        return a.phase == 8 && a.args_off < b.args_off;
+    });
+    size_t at = 0;
+    for (PlanGroup& g : gs) {
+        bool uni = true;
+        int tot = 0;
+        for (const LaunchRec* q : g.mem) {
+            uni = uni && q->grid == g.mem[0]->grid;
+            tot += q->grid;
+        }
+        g.per = uni ? (unsigned)g.mem[0]->grid : 0u;
+        g.blocks = tot;
+        g.args_off = at;
+        at += round16((size_t)g.mem[0]->bytes * g.mem.size());
+        g.boff_off = at;
+        at += round16(sizeof(int) * (g.mem.size() + 1));
+    }
+    bytes = at;
+    return gs;
+}
+
+size_t grouped_arg_bytes(const TrialRecorder* const* recs, int n) {
     size_t b = 0;
-    for (const LaunchRec& q : r.recs) b += round16((size_t)q.bytes * (size_t)n);
+    plan_grouped(recs, n, b);
     return b;
 }
 
 hipError_t launch_grouped(hipStream_t s0, const TrialRecorder* const* recs, int n, unsigned char* host_stage, unsigned char* dev_stage,
                           hipStream_t side, hipEvent_t ev_fork, hipEvent_t ev_join) {
-    const bool two = side && ev_fork && ev_join;
-    bool forked = false, waited = false, joined = false;
     if (n <= 0) return hipSuccess;
-    const std::vector<LaunchRec>& r0 = recs[0]->recs;
     for (int j = 0; j < n; ++j)
-        if (!same_sequence(*recs[0], *recs[j])) return hipErrorInvalidValue;
-    std::vector<size_t> off(r0.size());
+        if (recs[j]->failed) return hipErrorInvalidValue;
     size_t at = 0;
-    for (size_t i = 0; i < r0.size(); ++i) {
-        off[i] = at;
-        const size_t b = (size_t)r0[i].bytes;
-        for (int j = 0; j < n; ++j) std::memcpy(host_stage + at + (size_t)j * b, recs[j]->recs[i].args, b);
-        at += round16(b * (size_t)n);
+    const std::vector<PlanGroup> gs = plan_grouped(recs, n, at);
+    if (gs.empty()) return hipSuccess;
+    for (const PlanGroup& g : gs) {
+        const size_t b = (size_t)g.mem[0]->bytes;
+        int* boff = reinterpret_cast<int*>(host_stage + g.boff_off);
+        int o = 0;
+        for (size_t m = 0; m < g.mem.size(); ++m) {
+            if ((size_t)g.mem[m]->bytes != b) return hipErrorInvalidValue;
+            std::memcpy(host_stage + g.args_off + m * b, g.mem[m]->args, b);
+            boff[m] = o;
+            o += g.mem[m]->grid;
+        }
+        boff[g.mem.size()] = o;
     }
     // the arguments to the device by a small kernel reading the pinned staging (in the stream's
     // kernel order: no copy engine between the trials' kernels)
@@ -3209,15 +3307,19 @@ hipError_t launch_grouped(hipStream_t s0, const TrialRecorder* const* recs, int 
                        n16);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    for (size_t i = 0; i < r0.size(); ++i) {
-        const LaunchRec& q = r0[i];
-        const unsigned per = (unsigned)q.grid;
-        const dim3 grid(per * (unsigned)n), block(q.block);
-        const void* p = dev_stage + off[i];
-        const int k = q.kind;
+    // with a side stream the WA kernels run there from the updates on, joined before the gradients
+    const bool two = side && ev_fork && ev_join;
+    bool forked = false, waited = false, joined = false;
+    for (const PlanGroup& g : gs) {
+        const int k = g.kind;
         const bool wa = k == RK_WA_TILES || k == RK_WA_BIG;
+        if (two && !forked && g.phase > 0) {
+            e = hipEventRecord(ev_fork, s0);
+            if (e != hipSuccess) return e;
+            forked = true;
+        }
         hipStream_t s = s0;
-        if (two && wa && forked) {
+        if (two && wa) {
             if (!waited) {
                 e = hipStreamWaitEvent(side, ev_fork, 0);
                 if (e != hipSuccess) return e;
@@ -3225,31 +3327,37 @@ hipError_t launch_grouped(hipStream_t s0, const TrialRecorder* const* recs, int 
             }
             s = side;
         }
-        if (two && waited && !joined && !wa) {
+        if (two && waited && !joined && !wa && g.phase > 6) {
             e = hipEventRecord(ev_join, side);
             if (e == hipSuccess) e = hipStreamWaitEvent(s0, ev_join, 0);
             if (e != hipSuccess) return e;
             joined = true;
         }
+        const dim3 grid((unsigned)g.blocks), block((unsigned)g.block);
+        const void* p = dev_stage + g.args_off;
+        const int* boff = reinterpret_cast<const int*>(dev_stage + g.boff_off);
+        const int nm = (int)g.mem.size();
+        const unsigned per = g.per;
         if (k == RK_UPDATE) {
-            hipLaunchKernelGGL(k_group_update, grid, block, 0, s, (const UpdateArgsG*)p, per);
+            hipLaunchKernelGGL(k_group_update, grid, block, 0, s, (const UpdateArgsG*)p, per, boff, nm);
         } else if (k >= RK_SCATTER && k < RK_SCATTER + 4) {
             const UpdateArgsG* t = (const UpdateArgsG*)p;
             switch (k - RK_SCATTER) {
-            case 0: hipLaunchKernelGGL((k_group_scatter<unsigned long long, kPlaceChunk>), grid, block, 0, s, t, per); break;
-            case 1: hipLaunchKernelGGL((k_group_scatter<unsigned, kPlaceChunk>), grid, block, 0, s, t, per); break;
-            case 2: hipLaunchKernelGGL((k_group_scatter<unsigned long long, 256>), grid, block, 0, s, t, per); break;
-            default: hipLaunchKernelGGL((k_group_scatter<unsigned, 256>), grid, block, 0, s, t, per); break;
+            case 0: hipLaunchKernelGGL((k_group_scatter<unsigned long long, kPlaceChunk>), grid, block, 0, s, t, per, boff, nm); break;
+            case 1: hipLaunchKernelGGL((k_group_scatter<unsigned, kPlaceChunk>), grid, block, 0, s, t, per, boff, nm); break;
+            case 2: hipLaunchKernelGGL((k_group_scatter<unsigned long long, 256>), grid, block, 0, s, t, per, boff, nm); break;
+            default: hipLaunchKernelGGL((k_group_scatter<unsigned, 256>), grid, block, 0, s, t, per, boff, nm); break;
             }
         } else if (k >= RK_ROWS) {
             const RowArgs* t = (const RowArgs*)p;
             const int mode = (k - RK_ROWS) / 4, v = (k - RK_ROWS) % 4;
+            const unsigned lds = g.lds;
 #define GPL_GROUP_ROWS(M)                                                                                        \
     do {                                                                                                         \
-        if (v == 2) hipLaunchKernelGGL((k_group_rows<M, 10>), grid, block, q.lds, s, t, per);                    \
-        else if (v == 1) hipLaunchKernelGGL((k_group_rows<M, 9>), grid, block, q.lds, s, t, per);                \
-        else if (v == 0) hipLaunchKernelGGL((k_group_rows<M, 8>), grid, block, q.lds, s, t, per);                \
-        else hipLaunchKernelGGL((k_group_rows<M, 0>), grid, block, q.lds, s, t, per);                            \
+        if (v == 2) hipLaunchKernelGGL((k_group_rows<M, 10>), grid, block, lds, s, t, per, boff, nm);            \
+        else if (v == 1) hipLaunchKernelGGL((k_group_rows<M, 9>), grid, block, lds, s, t, per, boff, nm);        \
+        else if (v == 0) hipLaunchKernelGGL((k_group_rows<M, 8>), grid, block, lds, s, t, per, boff, nm);        \
+        else hipLaunchKernelGGL((k_group_rows<M, 0>), grid, block, lds, s, t, per, boff, nm);                    \
     } while (0)
             if (mode == ROW_BINS_DCT2) GPL_GROUP_ROWS(ROW_BINS_DCT2);
             else if (mode == ROW_SPECTRUM_POISSON) GPL_GROUP_ROWS(ROW_SPECTRUM_POISSON);
@@ -3257,23 +3365,18 @@ hipError_t launch_grouped(hipStream_t s0, const TrialRecorder* const* recs, int 
             else return hipErrorInvalidValue;
 #undef GPL_GROUP_ROWS
         } else if (k == RK_WA_TILES) {
-            hipLaunchKernelGGL(k_group_wa_tiles, grid, block, 0, s, (const WaArgsG*)p, per);
+            hipLaunchKernelGGL(k_group_wa_tiles, grid, block, 0, s, (const WaArgsG*)p, per, boff, nm);
         } else if (k == RK_WA_BIG) {
-            hipLaunchKernelGGL(k_group_wa_big, grid, block, 0, s, (const WaArgsG*)p, per);
+            hipLaunchKernelGGL(k_group_wa_big, grid, block, 0, s, (const WaArgsG*)p, per, boff, nm);
         } else if (k == RK_GRADIENT) {
-            hipLaunchKernelGGL(k_group_gradient, grid, block, 0, s, (const GradArgsG*)p, per);
+            hipLaunchKernelGGL(k_group_gradient, grid, block, 0, s, (const GradArgsG*)p, per, boff, nm);
         } else if (k == RK_FINISH) {
-            hipLaunchKernelGGL(k_group_finish, dim3((unsigned)n), block, 0, s, (const FinishArgsG*)p);
+            hipLaunchKernelGGL(k_group_finish, dim3((unsigned)nm), block, 0, s, (const FinishArgsG*)p);
         } else {
             return hipErrorInvalidValue;
         }
         e = hipGetLastError();
         if (e != hipSuccess) return e;
-        if (two && k == RK_UPDATE && !forked) {
-            e = hipEventRecord(ev_fork, s0);
-            if (e != hipSuccess) return e;
-            forked = true;
-        }
     }
     if (two && waited && !joined) {
         e = hipEventRecord(ev_join, side);

@@ -112,6 +112,14 @@ struct gpl_ctx {
     hipGraphExec_t texec = nullptr;
     uint64_t tgraph_key = ~0ull;
     std::vector<gplk::TrialNode> tnodes;
+    // Grouped trials (launch mode 7): the trial's kernels recorded, then launched grouped over the
+    // regions (one launch per kernel variant and step), the WA kernels on the side stream; the
+    // arguments' pinned staging and its device copy.
+    bool grouped_trials = false;
+    gplk::TrialRecorder grec;
+    unsigned char* gst_h = nullptr;
+    unsigned char* gst_d = nullptr;
+    size_t gst_cap = 0;
     // Device-driven trials (launch mode 4): the trial graph ends in k_control, which takes the
     // loop's decisions on the device from the control record d_ctl, so that trials run back to
     // back (ctl_depth graphs in flight) and the host steps in only when the device halts on an
@@ -298,7 +306,10 @@ struct gpl_ctx {
     }
     ~gpl_ctx() {
         if (stream) hipStreamSynchronize(stream);
+        if (aux) hipStreamSynchronize(aux);
         for (void* p : allocs) hipFree(p);
+        if (gst_h) hipHostFree(gst_h);
+        if (gst_d) hipFree(gst_d);
         if (h_sums) hipHostFree(h_sums);
         if (h_seq) hipHostFree(h_seq);
         if (h_ctl) hipHostFree(h_ctl);
@@ -1113,6 +1124,40 @@ struct gpl_ctx {
         return true;
     }
 
+    // one trial recorded and launched grouped (launch mode 7); false when off or when a kernel of
+    // the trial has no grouped form (the caller then launches it as usual)
+    bool trial_grouped(float coeff) {
+        if (!grouped_trials || profiling) return false;
+        grec.recs.clear();
+        grec.failed = false;
+        gplk::set_recorder(&grec);
+        try {
+            record_trial(coeff);
+        } catch (...) {
+            gplk::set_recorder(nullptr);
+            throw;
+        }
+        gplk::set_recorder(nullptr);
+        if (grec.failed) return false;
+        const gplk::TrialRecorder* one = &grec;
+        const size_t need = gplk::grouped_arg_bytes(&one, 1);
+        if (need > gst_cap) {
+            check(hipStreamSynchronize(stream), "sync");
+            if (gst_h) hipHostFree(gst_h);
+            if (gst_d) hipFree(gst_d);
+            gst_h = nullptr;
+            gst_d = nullptr;
+            gst_cap = 0;
+            check(hipHostMalloc((void**)&gst_h, need, hipHostMallocDefault), "hipHostMalloc");
+            check(hipMalloc((void**)&gst_d, need), "hipMalloc");
+            gst_cap = need;
+        }
+        check(gplk::launch_grouped(stream, &one, 1, gst_h, gst_d, aux, ev_fork, ev_join), "grouped trial");
+        wait_trial_signal();
+        gradients_read(0, int(rg.size()), SLOT_NEXT, true);
+        return true;
+    }
+
     // `resume` > 0: the iteration's first resume - 1 trials ran on the device and trial `resume`
     // did too, up to its gradient sums (in d_sums, and with stats_on_host in h_sums already): the
     // host takes it from there.
@@ -1124,7 +1169,7 @@ struct gpl_ctx {
             if (resume > 0 && nbt == resume - 1) {
                 if (!stats_on_host) sync_stats();
                 gradients_read(0, int(rg.size()), SLOT_NEXT, true);
-            } else if (!trial_by_graph(coeff)) {
+            } else if (!trial_by_graph(coeff) && !trial_grouped(coeff)) {
                 update_coordinates(coeff);
                 field();
                 wa_forward(false, true);
@@ -3018,9 +3063,8 @@ int32_t gpl_place_batch(gpl_ctx* const* cs, int32_t n, int32_t* iters) {
         std::vector<gplk::TrialRecorder> rec(n);
         std::vector<float> coeff(n, 0.0f);
         std::vector<const gplk::TrialRecorder*> grp;
-        std::vector<char> taken(n);
-        // an iteration's start for the live contexts: the first trials recorded, then launched in
-        // groups of one sequence, each group with its part of the staging
+        // an iteration's start for the live contexts: the first trials recorded, then launched
+        // grouped (one launch per kernel variant and step over every context and region)
         auto launch_round = [&]() {
             L.live.clear();
             L.grouped.clear();
@@ -3041,26 +3085,15 @@ int32_t gpl_place_batch(gpl_ctx* const* cs, int32_t n, int32_t* iters) {
                     throw;
                 }
                 gplk::set_recorder(nullptr);
-                taken[i] = 0;
             }
-            std::vector<std::vector<int32_t>> groups;
-            std::vector<size_t> goff;
-            size_t need = 0;
-            for (size_t a = 0; a < L.live.size(); ++a) {
-                const int32_t i = L.live[a];
-                if (taken[i] || rec[i].failed) continue;
-                std::vector<int32_t> members;
-                for (size_t b2 = a; b2 < L.live.size(); ++b2) {
-                    const int32_t j = L.live[b2];
-                    if (!taken[j] && gplk::same_sequence(rec[i], rec[j])) {
-                        members.push_back(j);
-                        taken[j] = 1;
-                    }
+            grp.clear();
+            for (int32_t i : L.live)
+                if (!rec[i].failed) {
+                    grp.push_back(&rec[i]);
+                    L.grouped.push_back(i);
                 }
-                goff.push_back(need);
-                need += gplk::grouped_arg_bytes(rec[i], int(members.size()));
-                groups.push_back(std::move(members));
-            }
+            if (grp.empty()) return;
+            const size_t need = gplk::grouped_arg_bytes(grp.data(), int(grp.size()));
             if (need > L.cap) {
                 check(hipStreamSynchronize(bs), "sync");
                 if (L.hstage) hipHostFree(L.hstage);
@@ -3072,14 +3105,8 @@ int32_t gpl_place_batch(gpl_ctx* const* cs, int32_t n, int32_t* iters) {
                 check(hipMalloc((void**)&L.dstage, need), "hipMalloc");
                 L.cap = need;
             }
-            for (size_t g = 0; g < groups.size(); ++g) {
-                grp.clear();
-                for (int32_t j : groups[g]) grp.push_back(&rec[j]);
-                failing = cs[groups[g][0]];
-                check(gplk::launch_grouped(bs, grp.data(), int(grp.size()), L.hstage + goff[g], L.dstage + goff[g], side, ev_fork, ev_join),
-                      "grouped trial");
-                for (int32_t j : groups[g]) L.grouped.push_back(j);
-            }
+            failing = cs[L.grouped[0]];
+            check(gplk::launch_grouped(bs, grp.data(), int(grp.size()), L.hstage, L.dstage, side, ev_fork, ev_join), "grouped trial");
         };
         // the rest of each iteration by its own context (a context whose record failed: the whole
         // iteration there)
@@ -3479,7 +3506,7 @@ int32_t gpl_get_tuning(gpl_ctx* c, int32_t* out, int32_t cap) {
 }
 
 int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode) {
-    if (!c || mode < 0 || mode > 6) return GPL_ERR_ARG;
+    if (!c || mode < 0 || mode > 7) return GPL_ERR_ARG;
     try {
         c->join();
         check(hipStreamSynchronize(c->stream), "sync");
@@ -3487,6 +3514,8 @@ int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode) {
         return fail(c, GPL_ERR_HIP, strfmt("%s: %s", e.what, hipGetErrorString(e.e)));
     }
     c->drop_graph();
+    c->grouped_trials = mode == 7;
+    if (mode == 7) mode = 2;  // the trials grouped; every other launch one by one on one stream
     c->launch_mode = mode >= 3 ? 0 : mode;
     c->overlap = mode < 2 || mode >= 4;
     c->device_trials = mode >= 4;

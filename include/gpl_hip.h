@@ -580,7 +580,10 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap);
  * device halts on an event it handles): 4 as a HIP graph, 5 launched one by one on the side
  * streams, 6 on one stream.  gpl_kernel_times then reports "device_halts" (the halts, and the
  * host's time from each entry to the halt's state taken back) and "device_launched" (trials
- * launched, including those that found the device halted and did nothing). */
+ * launched, including those that found the device halted and did nothing).  7: grouped trials --
+ * the trial's kernels recorded and launched as one launch per kernel variant and step over the
+ * context's regions (every region's update at once, then every scatter, each row pass, the WA
+ * kernels beside on the second stream, every gradient), the finishing reductions in order. */
 int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode);
 
 /* The autotuner (no reference counterpart; north star: tile shapes picked by an autotuner the project

@@ -283,17 +283,18 @@ def test_loop_is_deterministic(c1):
     assert np.array_equal(runs[0][1], runs[1][1])
 
 
-@pytest.mark.parametrize("cfg", ["c1", "c2"])
+@pytest.mark.parametrize("cfg", ["c1", "c2", "c4r_small"])
 def test_launch_modes_give_the_same_placement(cfg):
     """A trial as one HIP graph (replayed every trial, with the loop's eager work -- re-sorts,
     snapshots, statistics -- between replays), as kernels on the side streams, as kernels on one
-    stream, and profiled (the bins pass apart from the solve's first row pass, where it runs fused
-    otherwise): the same iterations, rows and placement."""
+    stream, recorded and launched grouped over the regions (mode 7; the region groups: nine regions
+    of six grid sizes), and profiled (the bins pass apart from the solve's first row pass, where it
+    runs fused otherwise): the same iterations, rows and placement."""
     d = G.Design.from_config(cfg)
     d.initial_place()
     d.init_nesterov_place()
     out = []
-    for mode in (0, 1, 2, "profiled"):
+    for mode in (0, 1, 2, 7, "profiled"):
         g = G.NesterovGPU(d)
         if mode == "profiled":
             g.set_profiling(True)
