@@ -26,7 +26,10 @@ Besides the throughput the line carries:
                 batch (each iteration's first trial of all 16 as one launch per kernel, each
                 context its own control) and from concurrent contexts, with the results compared;
   routability   the routability-driven loop on the 500k-cell design (768 x 768 bins, RUDY on
-                1024 x 1024 tiles), np_init to convergence, against the oracle's full run.
+                1024 x 1024 tiles), np_init to convergence, against the oracle's full run;
+  region_groups the region-group placement (nine regions, grids of 64 to 512 bins a side), np_init
+                to convergence with its trials grouped over the regions and with one graph per
+                trial, the results compared.
 """
 import argparse
 import json
@@ -311,6 +314,39 @@ def batched(cfg, k, emit=True, batch_only=False):
     return line
 
 
+def region_groups_leg(cfg="c4r"):
+    """The region-group workload (nine regions of 5k-209k cells on grids of 64 to 512 bins a side,
+    one placement): np_init to convergence with its trials grouped over the regions (the default
+    for region groups: one launch per kernel variant and step) and with one HIP graph per trial."""
+    import gpl_amd as G
+
+    t = time.perf_counter()
+    d = G.Design.from_config(cfg)
+    d.initial_place(device=True)
+    d.init_nesterov_place()
+    setup = time.perf_counter() - t
+    p = d.problem()
+    out = {"config": {"workload": WORKLOADS.get(cfg, cfg), "name": cfg, "regions": p.num_regions,
+                      "grids": [int(p.regions[b].cnt_x) for b in range(p.num_regions)]}, "setup_s": round(setup, 2)}
+    runs = {}
+    for name, mode in (("grouped", 7), ("graph", 0)):
+        g = G.NesterovGPU(d)
+        g.set_launch_mode(mode)
+        g.init_nesterov_loop()
+        t0 = time.perf_counter()
+        it = g.run_nesterov_loop()
+        el = time.perf_counter() - t0
+        runs[name] = (it, el, g.stats().hpwl, g.history())
+        g.close()
+    d.close()
+    it, el, hpwl, hist = runs["grouped"]
+    out.update({"iterations": it, "seconds": round(el, 4), "us_per_iteration": round(1e6 * el / max(it, 1), 1),
+                "final_hpwl": hpwl, "graph_seconds": round(runs["graph"][1], 4),
+                "speedup_vs_graph": round(runs["graph"][1] / el, 2),
+                "same_results_as_graph": runs["graph"][0] == it and runs["graph"][2] == hpwl and runs["graph"][3] == hist})
+    return out
+
+
 def routability_leg(cfg="c5"):
     """The routability-driven loop on the 500k-cell design (768 x 768 bins, RUDY on 1024 x 1024
     tiles of 8 + 8 tracks, at most 5 passes): the whole placement timed from np_init to convergence
@@ -374,6 +410,7 @@ def main():
     ap.add_argument("--hpwl-config", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--no-batched", action="store_true", help="skip the 16-seed batched comparison")
     ap.add_argument("--no-routability", action="store_true", help="skip the routability-driven loop on the 500k-cell design")
+    ap.add_argument("--no-region-groups", action="store_true", help="skip the region-group placement (nine regions)")
     ap.add_argument("--launch-mode", type=int, default=0, choices=(0, 1, 2, 3, 4, 5, 6, 7),
                     help="0: each trial one HIP graph (branches on side streams); 1: kernels launched one by "
                          "one on the side streams; 2: one by one on one stream; 3: one graph captured on one stream; "
@@ -602,6 +639,7 @@ def main():
     if not args.no_batched:
         batched_line = batched("c4s", 16, emit=False)
     route_line = None if args.no_routability or args.config != "c3" else routability_leg("c5")
+    groups_line = None if args.no_region_groups or args.config != "c3" else region_groups_leg("c4r")
 
     out = {
         "metric": METRIC,
@@ -641,6 +679,7 @@ def main():
                          if t50 is not None and t_end > t50 and final_iter > it50 else None),
         "batched": batched_line,
         "routability": route_line,
+        "region_groups": groups_line,
     }
     print(json.dumps(out), flush=True)
     d.close()

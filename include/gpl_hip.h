@@ -571,8 +571,9 @@ typedef struct gpl_kernel_time {
 int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap);
 
 /* How a back-tracking trial is launched (no reference counterpart; the results are the same in
- * every mode): 0 (default) one HIP graph per trial, the bins and the Poisson solve on a branch
- * beside the WA kernels, the high-fanout nets on a third; 1 the same kernels launched one by one
+ * every mode): 0 (the default for one region) one HIP graph per trial, the bins and the Poisson
+ * solve on a branch beside the WA kernels, the high-fanout nets on a third; 1 the same kernels
+ * launched one by one
  * on those streams; 2 one by one on one stream; 3 one HIP graph per trial captured on one stream.
  * Device-driven trials (doBackTracking, nesterovUpdateStepLength, the phi / WA-coefficient
  * schedule and updateNextIter's bookkeeping, nesterov.rs:2750-2930, taken by the trial's last
@@ -583,7 +584,8 @@ int32_t gpl_kernel_times(gpl_ctx* c, gpl_kernel_time* out, int32_t cap);
  * launched, including those that found the device halted and did nothing).  7: grouped trials --
  * the trial's kernels recorded and launched as one launch per kernel variant and step over the
  * context's regions (every region's update at once, then every scatter, each row pass, the WA
- * kernels beside on the second stream, every gradient), the finishing reductions in order. */
+ * kernels beside on the second stream, every gradient), the finishing reductions in order; the
+ * default for a context of more than one region (mode 0 for one). */
 int32_t gpl_set_launch_mode(gpl_ctx* c, int32_t mode);
 
 /* The autotuner (no reference counterpart; north star: tile shapes picked by an autotuner the project

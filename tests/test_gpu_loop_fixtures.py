@@ -51,12 +51,14 @@ def test_loop_matches_the_oracles_full_run(cfg, request):
         route = g.route_stats() if cfg == "c5" else None
         g.close()
         if cfg == "c4r":
-            # nine regions in one trial graph against the kernels launched one by one on one stream
-            g = G.NesterovGPU(d)
-            g.set_launch_mode(2)
-            assert g.do_nesterov_place() == it
-            assert g.history() == hist
-            g.close()
+            # the nine regions' trials grouped (the default for region groups) against one trial
+            # graph per trial and against the kernels launched one by one on one stream
+            for mode in (0, 2):
+                g = G.NesterovGPU(d)
+                g.set_launch_mode(mode)
+                assert g.do_nesterov_place() == it
+                assert g.history() == hist
+                g.close()
     finally:
         if own:
             d.close()
