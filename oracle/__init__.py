@@ -49,6 +49,15 @@ def lib():
             ("orc_direct_dct2_f64", None, [C.c_int, f64p]),
             ("orc_direct_dct3_f64", None, [C.c_int, f64p]),
             ("orc_direct_dst3_f64", None, [C.c_int, f64p]),
+            ("orc_set_batch_isa", None, [C.c_int]),
+            ("orc_ddct2d_fwd_f32", None, [C.c_int, C.c_int, f32p]),
+            ("orc_ddct2d_inv_f32", None, [C.c_int, C.c_int, f32p]),
+            ("orc_ddsct2d_inv_f32", None, [C.c_int, C.c_int, f32p]),
+            ("orc_ddcst2d_inv_f32", None, [C.c_int, C.c_int, f32p]),
+            ("orc_ddct2d_fwd_f64", None, [C.c_int, C.c_int, f64p]),
+            ("orc_ddct2d_inv_f64", None, [C.c_int, C.c_int, f64p]),
+            ("orc_ddsct2d_inv_f64", None, [C.c_int, C.c_int, f64p]),
+            ("orc_ddcst2d_inv_f64", None, [C.c_int, C.c_int, f64p]),
             ("orc_fft_solve_f32", None, [C.c_int32, C.c_int32, f32p, f32p, f32p, f32p, f32p, f32p]),
             ("orc_fft_solve_f64", None, [C.c_int32, C.c_int32, f32p, f32p, f32p, f64p, f64p, f64p]),
             ("orc_create", C.c_void_p, [C.c_void_p]),

@@ -155,6 +155,18 @@ def run(name, seed=1):
     o.close()
     d.close()
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"loop_{name}.json")
+    if os.path.exists(path):
+        # a run that reproduces the fixture's results (a re-timing) keeps its sensitivity runs and
+        # wide-sum rows; a run with other results drops them (they belong to the old trajectory)
+        with open(path) as f:
+            old = json.load(f)
+        same = all(old.get(k) == out.get(k) for k in ("problem_sha256", "final_iter", "final_hpwl", "final_overflow",
+                                                        "total_trials", "rows", "every_50th", "route"))
+        print(f"{name}: {'the same results as' if same else 'results differ from'} the previous fixture", flush=True)
+        if same:
+            for k in ("perturbed", "rows_wide"):
+                if k in old:
+                    out[k] = old[k]
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     print(f"{name}: iter {it} hpwl {s.hpwl} overflow {s.overflow:.4f} cpu {t_run:.1f}s -> {path}", flush=True)

@@ -99,3 +99,45 @@ def test_solve_is_linear_and_zero_mean_blind():
     pc = O.solve_f64(nx, ny, wx, wy, (a + 5.0).astype(np.float32))
     for q in range(3):
         assert np.allclose(pc[q], pa[q], atol=1e-5)
+
+
+_2D = {"orc_ddct2d_fwd": ("dct2", "dct2"), "orc_ddct2d_inv": ("dct3", "dct3"), "orc_ddsct2d_inv": ("dct3", "dst3"),
+       "orc_ddcst2d_inv": ("dst3", "dct3")}
+
+
+def _by_lines(fn2d, a, sfx):
+    """The 2-D transform as the one-sequence transforms over every row, then every column."""
+    rk, ck = _2D[fn2d]
+    dt = np.float32 if sfx == "f32" else np.float64
+    out = np.array(a, dt, copy=True)
+    for i in range(out.shape[0]):
+        out[i] = _run(f"orc_{rk}_{sfx}", out[i], dt)
+    for j in range(out.shape[1]):
+        out[:, j] = _run(f"orc_{ck}_{sfx}", out[:, j], dt)
+    return out
+
+
+@pytest.mark.parametrize("n1,n2", [(1, 8), (2, 4), (8, 8), (12, 16), (16, 12), (24, 48), (10, 16), (16, 10),
+                                   (64, 32), (256, 512), (768, 96)])
+@pytest.mark.parametrize("sfx", ["f32", "f64"])
+@pytest.mark.parametrize("isa", [-1, 0])
+def test_2d_transforms_are_the_1d_transforms_bit_for_bit(n1, n2, sfx, isa):
+    """The 2-D transforms run eight rows (columns) at once, one in each lane of a vector: every
+    lane must give the one-sequence transform's bits, with partial groups (12 and 10 rows or
+    columns), the radix-3 lengths, lengths without a fast transform (10) and several threads,
+    in the AVX2 build of the batched code (where the host has AVX2) and the baseline one."""
+    dt = np.float32 if sfx == "f32" else np.float64
+    x = np.random.default_rng(n1 * 1000 + n2).standard_normal((n1, n2)).astype(dt)
+    for fn in _2D:
+        want = _by_lines(fn, x, sfx)
+        for threads in (1, 3):
+            O.set_threads(threads)
+            O.lib().orc_set_batch_isa(isa)
+            try:
+                got = np.array(x, copy=True)
+                getattr(O.lib(), f"{fn}_{sfx}")(n1, n2, O.f32(got) if sfx == "f32" else O.f64(got))
+            finally:
+                O.set_threads(1)
+                O.lib().orc_set_batch_isa(-1)
+            assert np.array_equal(got.view(np.uint32 if sfx == "f32" else np.uint64),
+                                  want.view(np.uint32 if sfx == "f32" else np.uint64)), (fn, threads)
