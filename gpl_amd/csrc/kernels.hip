@@ -1627,13 +1627,14 @@ __device__ __forceinline__ void gradient_body(RegionDev r, CellsDev c, NetsDev n
         const int qb = c.pin_start[r.cell_begin + base];
         const int qe = c.pin_start[r.cell_begin + min(base + kBlock, n)];
         const bool staged = qe - qb <= min(r.grad_stage, kGradStage);
-        __syncthreads();
-        if (staged)
-            for (int i = threadIdx.x; i < qe - qb; i += kBlock) spg[i] = nt.pgrad[qb + i];
-        __syncthreads();
+        // The cell's own reads and its 3 x 3 bin window of the field are issued first, so that
+        // they are in flight with the staging of the group's pin terms (a lane past the region's
+        // end reads its last cell and stores nothing); indices are clamped into the arrays, with
+        // no guards, so that the compiler does not wait after each read; cells with more pins or
+        // a wider span finish in the loops below.
         const int k = base + (int)threadIdx.x;
-        if (k >= n) continue;
-        const int g = r.cell_begin + k;
+        const bool live = k < n;
+        const int g = r.cell_begin + min(k, n - 1);
         const int4 d = c.dbox[g];
         const int q0 = c.pin_start[g], q1 = c.pin_start[g + 1];
         const float scale = c.dscale[g];
@@ -1641,20 +1642,6 @@ __device__ __forceinline__ void gradient_body(RegionDev r, CellsDev c, NetsDev n
         int x0, x1, y0, y1;
         bin_span(d.x, d.z, r.lx, r.size_x, r.cnt_x, x0, x1);
         bin_span(d.y, d.w, r.ly, r.size_y, r.cnt_y, y0, y1);
-        // Every read of the cell's first kGradPins pin terms and of a 3 x 3 bin window is issued
-        // before any is used (indices clamped into the arrays, no guards, so that the compiler
-        // does not wait after each); cells with more pins or a wider span finish in the loops.
-        float2 t[kGradPins];
-        // a pinless cell (q0 == q1) clamps to a term inside the staged range (or the array), so
-        // every read stays in bounds; its values are not used
-        const int qlast = staged ? max(q1 - 1, qb) : max(q1 - 1, 0);
-        if (staged) {
-#pragma unroll
-            for (int j = 0; j < kGradPins; ++j) t[j] = spg[min(q0 + j, qlast) - qb];
-        } else {
-#pragma unroll
-            for (int j = 0; j < kGradPins; ++j) t[j] = nt.pgrad[min(q0 + j, qlast)];
-        }
         int ex[4], ey[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -1669,6 +1656,32 @@ __device__ __forceinline__ void gradient_body(RegionDev r, CellsDev c, NetsDev n
                 const size_t kb = (size_t)min(y0 + j, r.cnt_y - 1) * r.cnt_x + min(x0 + i, r.cnt_x - 1);
                 wf[3 * i + j] = r.field[kb];
             }
+        }
+        __syncthreads();
+        if (staged) {
+            // four reads in flight per thread before their stores; indices past the range clamp
+            // to its last term (read and stored again, the same value), so there are no guards
+            const int cnt = qe - qb;
+            for (int i0 = (int)threadIdx.x; i0 < cnt; i0 += 4 * kBlock) {
+                float2 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = nt.pgrad[qb + min(i0 + u * kBlock, cnt - 1)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) spg[min(i0 + u * kBlock, cnt - 1)] = v[u];
+            }
+        }
+        __syncthreads();
+        if (!live) continue;
+        float2 t[kGradPins];
+        // a pinless cell (q0 == q1) clamps to a term inside the staged range (or the array), so
+        // every read stays in bounds; its values are not used
+        const int qlast = staged ? max(q1 - 1, qb) : max(q1 - 1, 0);
+        if (staged) {
+#pragma unroll
+            for (int j = 0; j < kGradPins; ++j) t[j] = spg[min(q0 + j, qlast) - qb];
+        } else {
+#pragma unroll
+            for (int j = 0; j < kGradPins; ++j) t[j] = nt.pgrad[min(q0 + j, qlast)];
         }
         // the WA gradient: the pins' terms in the cell's pin order
         float wx = 0.0f, wy = 0.0f;
