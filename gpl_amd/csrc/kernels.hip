@@ -2040,12 +2040,31 @@ __device__ __forceinline__ void finish_body(const double* partials, int nb_bins,
             tail_p1 = dev_tail->pad[1];
         }
     }
+    // each thread's two partials of every quantity (slots tid and tid + kFinishBlock) read at
+    // once, then added in that order: the sums of a strided loop, without a wait after each
+    // read.  Only the written slots [0, cnt) of a quantity are read (indices clamped to cnt - 1,
+    // none for a quantity with no partials): a caller's buffer may hold only the quantities it
+    // counts (the bins' two, with no cells).
+    static_assert(kPartials <= 2 * kFinishBlock, "two partial slots per thread and quantity");
+    double a0[6], a1[6];
+    const int i0 = (int)threadIdx.x, i1 = (int)threadIdx.x + kFinishBlock;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int cnt = q < 2 ? nb_bins : nb_cells;
+        a0[q] = 0.0;
+        a1[q] = 0.0;
+        if (cnt > 0) {
+            a0[q] = partials[q * kPartials + min(i0, cnt - 1)];
+            a1[q] = partials[q * kPartials + min(i1, cnt - 1)];
+        }
+    }
     double v[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const int cnt = q < 2 ? nb_bins : nb_cells;
         double s = 0.0;
-        for (int i = threadIdx.x; i < cnt; i += kFinishBlock) s += partials[q * kPartials + i];
+        if (i0 < cnt) s += a0[q];
+        if (i1 < cnt) s += a1[q];
         v[q] = wave_sum(s);
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
